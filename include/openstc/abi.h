@@ -1,0 +1,177 @@
+/*
+ * openstc/abi.h -- C ABI of the MI355X-native RHS + SSP-RK3 library
+ * (libostc_hip.so).  Plain pointers and sizes only; every function returns
+ * an int status (OSTC_OK = 0, negative on error) and never aborts; the text
+ * of the last error is available from ostc_last_error().
+ *
+ * The reference (OpenSTC-MixingLayer) has no plugin or FFI layer: its
+ * operator surface is a set of C++ templates called from RK3
+ * (src/RK.cuh:20-228).  Each entry point below names the reference routine
+ * it stands in for.
+ *
+ * Layout contract (the reference's DZone/Field, src/Field.h:12-90,
+ * src/gxl_lib/Array.cuh:162-230): per-variable planes, i fastest, ngg ghost
+ * layers on every side,
+ *     idx(i,j,k,l) = (k+g)*(mx+2g)*(my+2g) + (j+g)*(mx+2g) + (i+g) + l*sz,
+ *     sz = (mx+2g)(my+2g)(mz+2g),
+ * with g = ngg for cv [n_var], bv [6: rho,u,v,w,p,T], mul, mach, jac and the
+ * 9 metric planes, and g = 0 for dq [n_var] and dt_local.
+ *
+ * Threading: one host thread per context; all device work is queued on the
+ * stream given at creation.  Host synchronisation happens only where a
+ * function returns device data to the host (dt, residuals, downloads).
+ */
+#ifndef OPENSTC_ABI_H
+#define OPENSTC_ABI_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OSTC_OK 0
+#define OSTC_ERR_ARG (-1)      /* bad argument, size or index */
+#define OSTC_ERR_HIP (-2)      /* HIP runtime error */
+#define OSTC_ERR_STATE (-3)    /* call not valid in the current state */
+#define OSTC_ERR_NUMERIC (-4)  /* NaN / non-positive density or pressure detected */
+#define OSTC_ERR_UNSUPPORTED (-5)
+
+/* boundary labels of the reference (src/BoundCond.cuh:1488-1678) */
+#define OSTC_BC_FARFIELD 4
+#define OSTC_BC_OUTFLOW 6
+#define OSTC_BC_PERIODIC 8
+
+/* time-step modes of ostc_step */
+#define OSTC_MODE_PLAIN 0    /* per-block launches shaped like the reference */
+#define OSTC_MODE_BATCHED 1  /* fused RHS+update over a block work-list, graph-captured */
+
+/* fields for ostc_get_field / ostc_set_field */
+#define OSTC_FIELD_CV 0
+#define OSTC_FIELD_BV 1
+#define OSTC_FIELD_MUL 2
+#define OSTC_FIELD_MACH 3
+#define OSTC_FIELD_DQ 4
+#define OSTC_FIELD_DT_LOCAL 5
+#define OSTC_FIELD_JAC 6
+#define OSTC_FIELD_METRIC 7
+#define OSTC_FIELD_QN 8
+
+typedef struct ostc_ctx_s* ostc_ctx;
+
+/* Scheme and physics keys of setup.txt (src/Parameter.cpp:187-437,
+ * src/DParameter.cu:7-283).  Air (calorically perfect, gamma 1.4,
+ * Sutherland viscosity) is the mixture model on this path. */
+typedef struct {
+  int n_var;            /* 5 */
+  int inviscid_scheme;  /* 51 WENO-Z5 cp, 71 WENO-Z7 cp, 50 WENO5-JS cp (no reference counterpart) */
+  int viscous_order;    /* 0 inviscid, 2 second-order central */
+  int fixed_time_step;  /* 1: use dt below, 0: CFL-based global dt */
+  double dt;
+  double cfl;
+  double prandtl;       /* prandtl_number */
+  double rho_ref;       /* rho_inf of the reference state (src/BoundCond.cpp:300-321) */
+  double v_ref;         /* v_inf of the reference state */
+} ostc_setup;
+
+typedef struct {
+  int mx, my, mz;       /* interior node counts (mz > 1: three-dimensional) */
+} ostc_block_desc;
+
+/* Device view of one block, the DZone equivalent (src/Field.h:12-90). */
+typedef struct {
+  int mx, my, mz, ngg;
+  long long sz;         /* padded plane size */
+  double* cv;           /* n_var planes */
+  double* bv;           /* 6 planes */
+  double* mul;
+  double* mach;
+  double* jac;
+  double* metric;       /* 9 planes, plane 3(r-1)+(c-1) = m(r,c) */
+  double* dq;           /* n_var planes, no ghosts */
+  double* dt_local;     /* no ghosts */
+} ostc_zone;
+
+/* Driver/Field construction: src/Driver.cu:12-113, src/Field.cu:469-647 */
+int ostc_create(const ostc_setup* s, void* hip_stream, ostc_ctx* out);
+int ostc_destroy(ostc_ctx ctx);
+int ostc_add_block(ostc_ctx ctx, const ostc_block_desc* d, int* block_id);
+/* Boundary face (Boundary::register_boundary, src/Mesh.cpp:14-32):
+ * axis 0/1/2, side -1 (index 0) or +1 (index m-1), type OSTC_BC_*. */
+int ostc_add_boundary(ostc_ctx ctx, int block, int axis, int side, int type);
+/* 1-to-1 conforming inner face with aligned index directions
+ * (InnerFace::register_boundary, src/Mesh.cpp:38-176).  Register each side
+ * of an interface, as the reference's inner N.txt files do. */
+int ostc_add_inner_face(ostc_ctx ctx, int block, int axis, int side, int target_block);
+/* Allocates device memory and builds the work-lists; blocks and faces are
+ * fixed afterwards. */
+int ostc_finalize(ostc_ctx ctx);
+int ostc_ngg(ostc_ctx ctx, int* ngg);
+int ostc_zone_view(ostc_ctx ctx, int block, ostc_zone* out);
+
+/* Geometry: host arrays on the padded domain.  metric component c (0..8,
+ * c = 3(r-1)+(c'-1)) of point p is read at
+ * metric[p*elem_stride + c*comp_stride + offset] (in doubles), so both SoA
+ * planes (1, sz, 0) and the reference's 80-byte gxl::Matrix AoS (10, 1, 1;
+ * src/gxl_lib/Matrix.hpp:53-72) can be passed. */
+int ostc_set_geometry(ostc_ctx ctx, int block, const double* jac, const double* metric,
+                      long long elem_stride, long long comp_stride, long long offset);
+/* Host <-> device copies of a field in the layout above (synchronous). */
+int ostc_set_field(ostc_ctx ctx, int block, int field, const double* host);
+int ostc_get_field(ostc_ctx ctx, int block, int field, double* host);
+
+/* Driver::initialize_computation (src/Driver.cu:115-199) followed by the
+ * RK3 prologue compute_cv_from_bv (src/RK.cuh:38-49): boundary conditions,
+ * inner-face exchange, properties, cv from bv over the padded domain. */
+int ostc_initialize(ostc_ctx ctx);
+
+/* n_steps SSP-RK3 steps (src/RK.cuh:85-226).  last_dt may be NULL. */
+int ostc_step(ostc_ctx ctx, int n_steps, int mode, double* last_dt);
+
+/* Stage-level operators, shaped like the reference's per-block launches. */
+int ostc_stage_rhs(ostc_ctx ctx, int block);                 /* dq = 0; inviscid; viscous (src/RK.cuh:113-139) */
+int ostc_compute_dt(ostc_ctx ctx, double* dt_out);            /* local_time_step + global_time_step */
+int ostc_stage_update(ostc_ctx ctx, int block, int rk, double dt); /* update_cv_and_bv_rk (src/RK.cuh:230-284) */
+int ostc_apply_bc(ostc_ctx ctx, int block);                   /* DBoundCond::apply_boundary_conditions */
+int ostc_exchange(ostc_ctx ctx);                              /* data_communication, inner faces */
+int ostc_update_properties(ostc_ctx ctx, int block);          /* update_physical_properties<Air> */
+int ostc_copy_qn(ostc_ctx ctx, int block);                    /* qn <- cv (src/RK.cuh:99-102) */
+/* Batched fused RHS of one block written to dq (no update), for checking
+ * the fused kernel's residual against the per-direction kernels. */
+int ostc_batched_rhs(ostc_ctx ctx, int block);
+
+/* Residual norms sqrt(sum(dv^2)/N) of (rho, |V|, p, T) against the last
+ * ostc_store_last_step (src/TimeAdvanceFunc.cu:6-37, src/Residual.cuh:75-122). */
+int ostc_store_last_step(ostc_ctx ctx);
+int ostc_residual(ostc_ctx ctx, double res[4]);
+
+/* Tile shape of the fused kernel: tile index into the compiled table and
+ * resident workgroups per CU; ostc_autotune times every table entry on the
+ * current blocks and keeps the fastest. */
+int ostc_tile_count(int* n);
+int ostc_tile_shape(int index, int* tx, int* ty, int* tz);
+int ostc_set_tile(ostc_ctx ctx, int index, int waves_per_cu);
+int ostc_autotune(ostc_ctx ctx, int reps, int* best_index, double* best_ms);
+
+/* Device timing of the last ostc_step call: total milliseconds measured
+ * with HIP events on the context's stream, and the launch count. */
+int ostc_last_step_timing(ostc_ctx ctx, double* ms, long long* launches);
+/* Durations (ms) of the three fused RHS+update kernels of the last batched
+ * step, from HIP events recorded around them inside the step graph. */
+int ostc_fused_kernel_ms(ostc_ctx ctx, double ms[3]);
+int ostc_sync(ostc_ctx ctx);
+const char* ostc_last_error(ostc_ctx ctx);
+
+/* Setup helpers (host code, src/Mesh.cpp:241-300 and src/Initialize.cu:259-325):
+ * a uniform Cartesian grid's jac and metric on the padded domain (SoA), and
+ * the air mixing-layer initial state bv on the padded domain plus a seeded
+ * velocity perturbation amplitude*U(-1,1) from SplitMix64(seed, node, comp). */
+int ostc_util_uniform_geometry(int mx, int my, int mz, int ngg, double x0, double y0, double z0,
+                               double dx, double dy, double dz, double* jac, double* metric);
+int ostc_util_mixing_layer_bv(int mx, int my, int mz, int ngg, double y0, double dy,
+                              const double upper[6], const double lower[6], double delta_omega,
+                              int gi0, int gj0, int gk0, int period_i, int period_j, int period_k,
+                              uint64_t seed, double amplitude, double* bv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,294 @@
+"""Host-side mirror of OpenSTC's RHS + SSP-RK3 operator surface over the C ABI
+of libostc_hip.so (include/openstc/abi.h).
+
+The reference drives these operators from ``RK3<mix,turb>(Driver&)``
+(src/RK.cuh:20-228); this module exposes them under the same names for
+Python callers (tests, benchmarks, the adjoint wrapper):
+
+* ``Solver.rk3(n_steps, mode)``                        -- src/RK.cuh:85-226
+* ``Solver.compute_rhs(b)``  (inviscid + viscous dq)    -- src/SchemeSelector.cuh:24-67
+* ``Solver.global_time_step()``                        -- src/TimeAdvanceFunc.cu:39-64
+* ``Solver.update_cv_and_bv_rk(b, rk, dt)``             -- src/RK.cuh:230-284
+* ``Solver.apply_boundary_conditions(b)``               -- src/BoundCond.cuh:1488-1678
+* ``Solver.data_communication()``                      -- src/DataCommunication.cuh:29-58
+* ``Solver.update_physical_properties(b)``              -- src/FieldOperation.cuh:107-140
+* ``Solver.compute_residual()``                        -- src/Residual.cuh:75-187
+
+Where the reference aborts the run (MPI_Abort after cudaGetLastError), these
+raise ``OstcError`` carrying the library's message.  There is no CPU
+fallback: without the built library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import importlib
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libostc_hip.so")
+
+cases = importlib.import_module(__name__ + ".cases")
+
+MODE_PLAIN, MODE_BATCHED = 0, 1
+FIELD_CV, FIELD_BV, FIELD_MUL, FIELD_MACH, FIELD_DQ, FIELD_DT_LOCAL, FIELD_JAC, FIELD_METRIC, FIELD_QN = range(9)
+
+
+class OstcError(RuntimeError):
+    pass
+
+
+class _Setup(C.Structure):
+    _fields_ = [("n_var", C.c_int), ("inviscid_scheme", C.c_int), ("viscous_order", C.c_int),
+                ("fixed_time_step", C.c_int), ("dt", C.c_double), ("cfl", C.c_double),
+                ("prandtl", C.c_double), ("rho_ref", C.c_double), ("v_ref", C.c_double)]
+
+
+class _BlockDesc(C.Structure):
+    _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int)]
+
+
+class Zone(C.Structure):
+    """Device view of a block (the DZone equivalent)."""
+    _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int), ("ngg", C.c_int), ("sz", C.c_longlong),
+                ("cv", C.c_void_p), ("bv", C.c_void_p), ("mul", C.c_void_p), ("mach", C.c_void_p),
+                ("jac", C.c_void_p), ("metric", C.c_void_p), ("dq", C.c_void_p), ("dt_local", C.c_void_p)]
+
+
+_lib = None
+
+# every exported symbol with its (restype, argtypes)
+_I, _D, _P, _LL = C.c_int, C.c_double, C.c_void_p, C.c_longlong
+_DP = C.POINTER(C.c_double)
+SIGNATURES = {
+    "ostc_create": (_I, [C.POINTER(_Setup), _P, C.POINTER(_P)]),
+    "ostc_destroy": (_I, [_P]),
+    "ostc_add_block": (_I, [_P, C.POINTER(_BlockDesc), C.POINTER(_I)]),
+    "ostc_add_boundary": (_I, [_P, _I, _I, _I, _I]),
+    "ostc_add_inner_face": (_I, [_P, _I, _I, _I, _I]),
+    "ostc_finalize": (_I, [_P]),
+    "ostc_ngg": (_I, [_P, C.POINTER(_I)]),
+    "ostc_zone_view": (_I, [_P, _I, C.POINTER(Zone)]),
+    "ostc_set_geometry": (_I, [_P, _I, _DP, _DP, _LL, _LL, _LL]),
+    "ostc_set_field": (_I, [_P, _I, _I, _DP]),
+    "ostc_get_field": (_I, [_P, _I, _I, _DP]),
+    "ostc_initialize": (_I, [_P]),
+    "ostc_step": (_I, [_P, _I, _I, _DP]),
+    "ostc_stage_rhs": (_I, [_P, _I]),
+    "ostc_compute_dt": (_I, [_P, _DP]),
+    "ostc_stage_update": (_I, [_P, _I, _I, _D]),
+    "ostc_apply_bc": (_I, [_P, _I]),
+    "ostc_exchange": (_I, [_P]),
+    "ostc_update_properties": (_I, [_P, _I]),
+    "ostc_copy_qn": (_I, [_P, _I]),
+    "ostc_batched_rhs": (_I, [_P, _I]),
+    "ostc_store_last_step": (_I, [_P]),
+    "ostc_residual": (_I, [_P, _DP]),
+    "ostc_tile_count": (_I, [C.POINTER(_I)]),
+    "ostc_tile_shape": (_I, [_I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "ostc_set_tile": (_I, [_P, _I, _I]),
+    "ostc_autotune": (_I, [_P, _I, C.POINTER(_I), _DP]),
+    "ostc_last_step_timing": (_I, [_P, _DP, C.POINTER(_LL)]),
+    "ostc_fused_kernel_ms": (_I, [_P, _DP]),
+    "ostc_sync": (_I, [_P]),
+    "ostc_last_error": (C.c_char_p, [_P]),
+    "ostc_util_uniform_geometry": (_I, [_I, _I, _I, _I, _D, _D, _D, _D, _D, _D, _DP, _DP]),
+    "ostc_util_mixing_layer_bv": (_I, [_I, _I, _I, _I, _D, _D, _DP, _DP, _D, _I, _I, _I, _I, _I, _I,
+                                       C.c_uint64, _D, _DP]),
+}
+
+
+def lib():
+    """Loads libostc_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OstcError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_DP)
+
+
+def padded_shape(blk: dict, g: int):
+    return (blk["mz"] + 2 * g, blk["my"] + 2 * g, blk["mx"] + 2 * g)
+
+
+class Solver:
+    """A set of blocks on one GPU with the reference's layout and operators."""
+
+    def __init__(self, case: dict, stream: int | None = None):
+        L = lib()
+        self.case = case
+        s = _Setup(case["n_var"], case["inviscid_scheme"], case["viscous_order"],
+                   int(case.get("fixed_time_step", 0)), float(case.get("dt", 0.0)), case["cfl"],
+                   case["prandtl"], case["rho_ref"], case["v_ref"])
+        ctx = C.c_void_p()
+        r = L.ostc_create(C.byref(s), stream, C.byref(ctx))
+        if r != 0:
+            raise OstcError(f"ostc_create failed ({r})")
+        self.ctx = ctx
+        g = C.c_int()
+        L.ostc_ngg(self.ctx, C.byref(g))
+        self.ngg = g.value
+        for b in case["blocks"]:
+            self._chk(L.ostc_add_block(self.ctx, C.byref(_BlockDesc(b["mx"], b["my"], b["mz"])), None))
+        for f in case["boundaries"]:
+            self._chk(L.ostc_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"]))
+        for f in case["inner_faces"]:
+            self._chk(L.ostc_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"]))
+        self._chk(L.ostc_finalize(self.ctx))
+
+    def _chk(self, r):
+        if r != 0:
+            raise OstcError(f"error {r}: {lib().ostc_last_error(self.ctx).decode()}")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().ostc_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- geometry and state ---------------------------------------------
+    def shape(self, b: int, field: int):
+        blk = self.case["blocks"][b]
+        if field in (FIELD_DQ, FIELD_DT_LOCAL):
+            per = (blk["mz"], blk["my"], blk["mx"])
+        else:
+            per = padded_shape(blk, self.ngg)
+        planes = {FIELD_CV: 5, FIELD_BV: 6, FIELD_MUL: 1, FIELD_MACH: 1, FIELD_DQ: 5, FIELD_DT_LOCAL: 1,
+                  FIELD_JAC: 1, FIELD_METRIC: 9, FIELD_QN: 5}[field]
+        return (planes, *per)
+
+    def get(self, b: int, field: int) -> np.ndarray:
+        out = np.empty(self.shape(b, field))
+        self._chk(lib().ostc_get_field(self.ctx, b, field, _dp(out)))
+        return out
+
+    def set(self, b: int, field: int, arr: np.ndarray):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.size != int(np.prod(self.shape(b, field))):
+            raise OstcError("size mismatch")
+        self._chk(lib().ostc_set_field(self.ctx, b, field, _dp(a)))
+
+    def set_geometry(self, b: int, jac: np.ndarray, metric: np.ndarray, elem_stride=None, comp_stride=None,
+                     offset=0):
+        jac = np.ascontiguousarray(jac, dtype=np.float64)
+        metric = np.ascontiguousarray(metric, dtype=np.float64)
+        if elem_stride is None:
+            elem_stride, comp_stride = 1, jac.size
+        self._chk(lib().ostc_set_geometry(self.ctx, b, _dp(jac), _dp(metric), elem_stride, comp_stride, offset))
+
+    def setup_mixing_layer(self):
+        """Uniform grid geometry and the mixing-layer initial state of the case."""
+        L, c, g = lib(), self.case, self.ngg
+        up = np.array(c["streams"]["upper"], dtype=np.float64)
+        lo = np.array(c["streams"]["lower"], dtype=np.float64)
+        per = c["periods"]
+        for b, blk in enumerate(c["blocks"]):
+            shp = padded_shape(blk, g)
+            jac = np.empty(shp)
+            met = np.empty((9, *shp))
+            self._chk(L.ostc_util_uniform_geometry(blk["mx"], blk["my"], blk["mz"], g, blk["x0"], blk["y0"],
+                                                   blk["z0"], c["dx"], c["dy"], c["dz"], _dp(jac), _dp(met)))
+            self.set_geometry(b, jac, met)
+            bv = np.empty((6, *shp))
+            self._chk(L.ostc_util_mixing_layer_bv(blk["mx"], blk["my"], blk["mz"], g, blk["y0"], c["dy"], _dp(up),
+                                                  _dp(lo), c["delta_omega"], blk["gi0"], blk["gj0"], blk["gk0"],
+                                                  per[0], per[1], per[2], c["seed"], c["amplitude"], _dp(bv)))
+            self.set(b, FIELD_BV, bv)
+
+    def initialize(self):
+        self._chk(lib().ostc_initialize(self.ctx))
+
+    def zone(self, b: int) -> Zone:
+        z = Zone()
+        self._chk(lib().ostc_zone_view(self.ctx, b, C.byref(z)))
+        return z
+
+    # -- the reference's operators ---------------------------------------
+    def rk3(self, n_steps: int = 1, mode: int = MODE_BATCHED, want_dt: bool = True) -> float:
+        dt = C.c_double(0.0)
+        self._chk(lib().ostc_step(self.ctx, n_steps, mode, C.byref(dt) if want_dt else None))
+        return dt.value
+
+    def compute_rhs(self, b: int):
+        self._chk(lib().ostc_stage_rhs(self.ctx, b))
+
+    def compute_rhs_batched(self, b: int):
+        self._chk(lib().ostc_batched_rhs(self.ctx, b))
+
+    def global_time_step(self) -> float:
+        dt = C.c_double()
+        self._chk(lib().ostc_compute_dt(self.ctx, C.byref(dt)))
+        return dt.value
+
+    def update_cv_and_bv_rk(self, b: int, rk: int, dt: float):
+        self._chk(lib().ostc_stage_update(self.ctx, b, rk, dt))
+
+    def copy_qn(self, b: int):
+        self._chk(lib().ostc_copy_qn(self.ctx, b))
+
+    def apply_boundary_conditions(self, b: int):
+        self._chk(lib().ostc_apply_bc(self.ctx, b))
+
+    def data_communication(self):
+        self._chk(lib().ostc_exchange(self.ctx))
+
+    def update_physical_properties(self, b: int):
+        self._chk(lib().ostc_update_properties(self.ctx, b))
+
+    def store_last_step(self):
+        self._chk(lib().ostc_store_last_step(self.ctx))
+
+    def compute_residual(self) -> np.ndarray:
+        r = np.zeros(4)
+        self._chk(lib().ostc_residual(self.ctx, _dp(r)))
+        return r
+
+    # -- tuning and timing -------------------------------------------------
+    def set_tile(self, index: int, waves_per_cu: int = 0):
+        self._chk(lib().ostc_set_tile(self.ctx, index, waves_per_cu))
+
+    def autotune(self, reps: int = 3):
+        bi, bm = C.c_int(), C.c_double()
+        self._chk(lib().ostc_autotune(self.ctx, reps, C.byref(bi), C.byref(bm)))
+        return bi.value, bm.value
+
+    def last_step_timing(self):
+        ms, n = C.c_double(), C.c_longlong()
+        self._chk(lib().ostc_last_step_timing(self.ctx, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def fused_kernel_ms(self):
+        a = np.zeros(3)
+        self._chk(lib().ostc_fused_kernel_ms(self.ctx, _dp(a)))
+        return a
+
+    def sync(self):
+        self._chk(lib().ostc_sync(self.ctx))
+
+
+def tile_shapes():
+    L = lib()
+    n = C.c_int()
+    L.ostc_tile_count(C.byref(n))
+    out = []
+    for i in range(n.value):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        L.ostc_tile_shape(i, C.byref(a), C.byref(b), C.byref(c))
+        out.append((a.value, b.value, c.value))
+    return out

@@ -1,0 +1,66 @@
+"""Builds libostc_hip.so for gfx950 with hipcc (in-tree, no JIT cache).
+
+Usage: python openstc-mixinglayer_amd/build.py [--jobs N]
+Kernels are built with -ffp-contract=off: every product and sum is rounded
+separately, in the order the expressions are written, which is the order
+the reference's routines write them.  The residual dq is a small difference
+of split fluxes carrying a large +-lambda*Q/2 term, so contraction changes
+the result at about 1e-12 relative.  OSTC_FP_CONTRACT=fast builds with
+contraction for comparison.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "lib", "libostc_hip.so")
+SOURCES = ["ostc_api.hip", "ostc_kernels_plain.hip", "ostc_kernels_fused.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+
+def flags():
+    f = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
+         # packed FP32 instructions stay off (build flags)
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-I", os.path.join(ROOT, "include"), "-I", CSRC]
+    f.append("-ffp-contract=" + ("fast" if os.environ.get("OSTC_FP_CONTRACT") == "fast" else "off"))
+    return f
+
+
+def compile_one(src: str) -> str:
+    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC, *flags(), "-c", os.path.join(CSRC, src), "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"hipcc failed on {src}")
+    return out
+
+
+def build(jobs: int = 3) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("link failed")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=3)
+    a = ap.parse_args()
+    print(build(a.jobs))

@@ -1,0 +1,1038 @@
+// ostc_api.hip -- host runtime behind the C ABI (include/openstc/abi.h):
+// block and face registry, device memory, the ghost-layer work-lists, the
+// per-block (reference-shaped) step and the batched, graph-captured step.
+#include "openstc/abi.h"
+#include "ostc_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace ostc;
+
+namespace {
+
+struct FaceDesc { int axis, side, type; };
+struct InnerDesc { int axis, side, target; };
+struct Box { int lo[3], hi[3]; };  // inclusive bounds
+
+bool box_ok(const Box& b) { return b.lo[0] <= b.hi[0] && b.lo[1] <= b.hi[1] && b.lo[2] <= b.hi[2]; }
+bool box_hit(const Box& a, const Box& b) {
+  for (int d = 0; d < 3; ++d)
+    if (a.hi[d] < b.lo[d] || b.hi[d] < a.lo[d]) return false;
+  return true;
+}
+// The parts of `b` outside the interior [0, m-1]^3.
+std::vector<Box> minus_interior(Box r, const int m[3]) {
+  std::vector<Box> out;
+  for (int a = 0; a < 3 && box_ok(r); ++a) {
+    if (r.lo[a] < 0) {
+      Box p = r;
+      p.hi[a] = std::min(r.hi[a], -1);
+      if (box_ok(p)) out.push_back(p);
+      r.lo[a] = std::max(r.lo[a], 0);
+    }
+    if (r.hi[a] > m[a] - 1) {
+      Box p = r;
+      p.lo[a] = std::max(r.lo[a], m[a]);
+      if (box_ok(p)) out.push_back(p);
+      r.hi[a] = std::min(r.hi[a], m[a] - 1);
+    }
+  }
+  return out;
+}
+
+struct HBlock {
+  int m[3];
+  std::vector<FaceDesc> faces;
+  std::vector<InnerDesc> inner;
+  DevBlock d{};
+};
+
+struct Group { int lo = 0, hi = 0; long long total = 0; };
+
+struct Access { int block; Box box; };
+
+int bc_priority(int type) {  // src/BoundCond.cuh:1488-1678 (farfield, outflow, ..., periodic)
+  switch (type) {
+    case OSTC_BC_FARFIELD: return 0;
+    case OSTC_BC_OUTFLOW: return 1;
+    case OSTC_BC_PERIODIC: return 7;
+    default: return 99;
+  }
+}
+
+}  // namespace
+
+struct ostc_ctx_s {
+  ostc_setup s{};
+  Params P{};
+  int ngg = 3;
+  hipStream_t st = nullptr;
+  bool own_stream = false;
+  int cus = 256;
+  std::vector<HBlock> blocks;
+  bool finalized = false;
+  DevBlock* d_blocks = nullptr;
+  std::vector<DevOp> ops;
+  DevOp* d_ops = nullptr;
+  std::vector<Group> bc_rounds, comm_levels;
+  Group props, rim;
+  std::vector<std::vector<int>> plain_bc;
+  std::vector<int> plain_comm;
+  int tile_index = 0, n_wg = 0, xcd_remap = 0, n_tiles = 0;
+  int* d_tile_block = nullptr;
+  double* d_scratch = nullptr;
+  int n_first = 1024;
+  double* d_dt = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_valid = false;
+  bool graph_events = false;
+  hipEvent_t ev_f[6] = {};
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  long long graph_launches = 0;
+  double last_ms = 0;
+  long long last_launches = 0;
+  std::vector<void*> allocs;
+  std::string err;
+};
+
+namespace {
+
+int fail(ostc_ctx c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail((c), OSTC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int dalloc(ostc_ctx c, double** p, size_t n) {
+  void* v = nullptr;
+  HIPCHK(c, hipMalloc(&v, n * sizeof(double)));
+  HIPCHK(c, hipMemset(v, 0, n * sizeof(double)));
+  c->allocs.push_back(v);
+  *p = static_cast<double*>(v);
+  return OSTC_OK;
+}
+
+DevOp make_op(int type, int block, const Box& b) {
+  DevOp op{};
+  op.type = type;
+  op.block = block;
+  op.target = block;
+  for (int d = 0; d < 3; ++d) {
+    op.lo[d] = b.lo[d];
+    op.n[d] = b.hi[d] - b.lo[d] + 1;
+  }
+  op.count = (long long)op.n[0] * op.n[1] * op.n[2];
+  return op;
+}
+
+Box full_tangential(const HBlock& B, int axis, int lo, int hi, int ext) {
+  Box b;
+  for (int d = 0; d < 3; ++d) {
+    b.lo[d] = -ext;
+    b.hi[d] = B.m[d] - 1 + ext;
+  }
+  b.lo[axis] = lo;
+  b.hi[axis] = hi;
+  return b;
+}
+
+// Work-lists for the ghost layers.  Boundary ops of one block keep the
+// reference's sequential order (one launch "round" per position in the
+// block's priority-sorted list; blocks are independent).  Inner-face ops keep
+// the reference's order (blocks, then faces) only between ops whose cell
+// sets conflict: each op gets level 1 + the highest level of an earlier
+// conflicting op, and one launch runs one level.
+int build_ops(ostc_ctx c) {
+  const int G = c->ngg;
+  const int nb = (int)c->blocks.size();
+  std::vector<std::vector<DevOp>> bc_of_block(nb);
+  std::vector<DevOp> rim_ops, props_ops;
+  for (int b = 0; b < nb; ++b) {
+    HBlock& B = c->blocks[b];
+    std::vector<FaceDesc> f = B.faces;
+    std::stable_sort(f.begin(), f.end(),
+                     [](const FaceDesc& x, const FaceDesc& y) { return bc_priority(x.type) < bc_priority(y.type); });
+    for (const FaceDesc& fd : f) {
+      const int a = fd.axis, m = B.m[a];
+      const int idx = fd.side < 0 ? 0 : m - 1;
+      DevOp op = make_op(fd.type == OSTC_BC_PERIODIC ? OP_BC_PERIODIC : OP_BC_OUTFLOW, b,
+                         full_tangential(B, a, idx, idx, G));
+      op.axis = a;
+      op.side = fd.side;
+      bc_of_block[b].push_back(op);
+      // ghost cells this op reads before the same stage can have written them
+      Box rd = fd.type == OSTC_BC_PERIODIC
+                   ? (fd.side < 0 ? full_tangential(B, a, m - 1 - G, m - 2, G) : full_tangential(B, a, 1, G, G))
+                   : full_tangential(B, a, idx, idx, G);
+      for (const Box& rb : minus_interior(rd, B.m)) rim_ops.push_back(make_op(OP_COPY_BV, b, rb));
+    }
+    Box pad;
+    for (int d = 0; d < 3; ++d) { pad.lo[d] = -G; pad.hi[d] = B.m[d] - 1 + G; }
+    for (const Box& sb : minus_interior(pad, B.m)) props_ops.push_back(make_op(OP_PROPS, b, sb));
+  }
+  // inner faces, reference order
+  struct CommOp { DevOp op; std::vector<Access> w, r; int level; };
+  std::vector<CommOp> comm;
+  for (int b = 0; b < nb; ++b) {
+    HBlock& A = c->blocks[b];
+    for (const InnerDesc& f : A.inner) {
+      HBlock& T = c->blocks[f.target];
+      const int a = f.axis;
+      for (int d = 0; d < 3; ++d)
+        if (d != a && A.m[d] != T.m[d])
+          return fail(c, OSTC_ERR_ARG, "inner face: tangential extents of the two blocks differ");
+      Box box, shA, shT, layT;
+      for (int d = 0; d < 3; ++d) {
+        box.lo[d] = shA.lo[d] = shT.lo[d] = layT.lo[d] = 0;
+        box.hi[d] = shA.hi[d] = shT.hi[d] = layT.hi[d] = A.m[d] - 1;
+      }
+      const int mA = A.m[a], mT = T.m[a];
+      if (f.side > 0) {
+        box.lo[a] = mA - 1; box.hi[a] = mA - 1 + G;
+        shA.lo[a] = shA.hi[a] = mA - 1;
+        shT.lo[a] = shT.hi[a] = 0;
+        layT.lo[a] = 0; layT.hi[a] = G;
+      } else {
+        box.lo[a] = -G; box.hi[a] = 0;
+        shA.lo[a] = shA.hi[a] = 0;
+        shT.lo[a] = shT.hi[a] = mT - 1;
+        layT.lo[a] = mT - 1 - G; layT.hi[a] = mT - 1;
+      }
+      CommOp co;
+      co.op = make_op(OP_COMM, b, box);
+      co.op.target = f.target;
+      co.op.axis = a;
+      co.op.side = f.side;
+      co.op.shared = f.side > 0 ? mA - 1 : 0;
+      co.op.tstart = f.side > 0 ? 0 : mT - 1 - G;
+      co.w = {{b, box}, {f.target, shT}};
+      co.r = {{f.target, layT}, {b, shA}};
+      co.level = 0;
+      for (auto& e : comm) {
+        auto meet = [](const std::vector<Access>& x, const std::vector<Access>& y) {
+          for (auto& p : x)
+            for (auto& q : y)
+              if (p.block == q.block && box_hit(p.box, q.box)) return true;
+          return false;
+        };
+        if (meet(e.w, co.w) || meet(e.w, co.r) || meet(e.r, co.w)) co.level = std::max(co.level, e.level + 1);
+      }
+      comm.push_back(co);
+      props_ops.push_back(make_op(OP_PROPS, b, shA));  // shared layers after averaging
+    }
+  }
+  // assemble: [bc rounds][comm levels][props][rim]
+  c->ops.clear();
+  c->bc_rounds.clear();
+  c->comm_levels.clear();
+  c->plain_bc.assign(nb, {});
+  size_t rounds = 0;
+  for (auto& v : bc_of_block) rounds = std::max(rounds, v.size());
+  auto close_group = [&](Group& g, int lo) {
+    g.lo = lo;
+    g.hi = (int)c->ops.size();
+    long long acc = 0;
+    for (int i = g.lo; i < g.hi; ++i) { c->ops[i].begin = acc; acc += c->ops[i].count; }
+    g.total = acc;
+  };
+  for (size_t r = 0; r < rounds; ++r) {
+    const int lo = (int)c->ops.size();
+    for (int b = 0; b < nb; ++b)
+      if (r < bc_of_block[b].size()) {
+        c->plain_bc[b].push_back((int)c->ops.size());
+        c->ops.push_back(bc_of_block[b][r]);
+      }
+    Group g;
+    close_group(g, lo);
+    c->bc_rounds.push_back(g);
+  }
+  int max_level = -1;
+  for (auto& e : comm) max_level = std::max(max_level, e.level);
+  std::vector<int> pos(comm.size());
+  for (int L = 0; L <= max_level; ++L) {
+    const int lo = (int)c->ops.size();
+    for (size_t q = 0; q < comm.size(); ++q)
+      if (comm[q].level == L) {
+        pos[q] = (int)c->ops.size();
+        c->ops.push_back(comm[q].op);
+      }
+    Group g;
+    close_group(g, lo);
+    c->comm_levels.push_back(g);
+  }
+  c->plain_comm = pos;
+  {
+    const int lo = (int)c->ops.size();
+    for (auto& o : props_ops) c->ops.push_back(o);
+    close_group(c->props, lo);
+  }
+  {
+    const int lo = (int)c->ops.size();
+    for (auto& o : rim_ops) c->ops.push_back(o);
+    close_group(c->rim, lo);
+  }
+  if (!c->ops.empty()) {
+    void* v = nullptr;
+    HIPCHK(c, hipMalloc(&v, c->ops.size() * sizeof(DevOp)));
+    c->allocs.push_back(v);
+    c->d_ops = static_cast<DevOp*>(v);
+    HIPCHK(c, hipMemcpy(c->d_ops, c->ops.data(), c->ops.size() * sizeof(DevOp), hipMemcpyHostToDevice));
+  }
+  return OSTC_OK;
+}
+
+void invalidate_graph(ostc_ctx c) {
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_valid = false;
+}
+
+int upload_blocks(ostc_ctx c) {
+  std::vector<DevBlock> h;
+  for (auto& B : c->blocks) h.push_back(B.d);
+  HIPCHK(c, hipMemcpy(c->d_blocks, h.data(), h.size() * sizeof(DevBlock), hipMemcpyHostToDevice));
+  return OSTC_OK;
+}
+
+int set_tiles(ostc_ctx c, int index, int n_wg) {
+  const TileShape t = fused_shape(c->P.scheme, index);
+  const int lds = fused_lds_bytes(c->P.scheme, index);
+  if (lds < 0 || lds > 160 * 1024) return fail(c, OSTC_ERR_ARG, "tile shape does not fit in LDS");
+  std::vector<int> tb;
+  int base = 0;
+  for (size_t b = 0; b < c->blocks.size(); ++b) {
+    DevBlock& d = c->blocks[b].d;
+    d.ntx = (d.mx + t.tx - 1) / t.tx;
+    d.nty = (d.my + t.ty - 1) / t.ty;
+    d.ntz = (d.mz + t.tz - 1) / t.tz;
+    d.tile_base = base;
+    const int n = d.ntx * d.nty * d.ntz;
+    for (int i = 0; i < n; ++i) tb.push_back((int)b);
+    base += n;
+  }
+  c->n_tiles = base;
+  if (c->d_tile_block) hipFree(c->d_tile_block);
+  void* v = nullptr;
+  HIPCHK(c, hipMalloc(&v, tb.size() * sizeof(int)));
+  c->d_tile_block = static_cast<int*>(v);
+  HIPCHK(c, hipMemcpy(c->d_tile_block, tb.data(), tb.size() * sizeof(int), hipMemcpyHostToDevice));
+  c->tile_index = index;
+  c->n_wg = n_wg > 0 ? std::min(n_wg, c->n_tiles) : c->n_tiles;
+  invalidate_graph(c);
+  return upload_blocks(c);
+}
+
+FusedArgs fused_args(ostc_ctx c, int in, int out, int rk) {
+  FusedArgs a{};
+  a.blocks = c->d_blocks;
+  a.tile_block = c->d_tile_block;
+  a.n_tiles = c->n_tiles;
+  a.in = in;
+  a.out = out;
+  const RkCoef k = rk_coef(rk);
+  a.ra = k.a;
+  a.rb = k.b;
+  a.rc = k.c;
+  a.dt = c->d_dt;
+  a.P = c->P;
+  a.xcd_remap = c->xcd_remap;
+  a.target_block = -1;
+  return a;
+}
+
+void launch_group(ostc_ctx c, const Group& g, int slot, int src, long long* n) {
+  if (g.hi > g.lo) ops_launch(c->d_blocks, c->d_ops, g.lo, g.hi, g.total, slot, src, c->st, n);
+}
+
+void launch_op(ostc_ctx c, int idx, int slot, long long* n) {
+  ops_launch(c->d_blocks, c->d_ops, idx, idx + 1, c->ops[idx].count, slot, slot, c->st, n);
+}
+
+// ---------------------------------------------------- per-block path pieces
+int plain_bc(ostc_ctx c, int b, long long* n) {
+  for (int idx : c->plain_bc[b]) launch_op(c, idx, 0, n);
+  return OSTC_OK;
+}
+
+int plain_exchange(ostc_ctx c, long long* n) {
+  for (int idx : c->plain_comm) launch_op(c, idx, 0, n);
+  return OSTC_OK;
+}
+
+int plain_dt(ostc_ctx c, double* dt, long long* n) {
+  double v = 1e6;
+  for (auto& B : c->blocks) {
+    plain_local_dt(B.d, c->P, c->st, n);
+    min_reduce(B.d.dtl, B.d.n0, c->d_scratch, c->n_first, c->st, n);
+    double h = 0;
+    HIPCHK(c, hipMemcpyAsync(&h, c->d_scratch, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    v = std::min(v, h);
+  }
+  *dt = v;
+  return OSTC_OK;
+}
+
+// One SSP-RK3 step shaped like src/RK.cuh:85-226.
+int plain_step(ostc_ctx c, double* dt_io, long long* n) {
+  for (auto& B : c->blocks) {
+    HIPCHK(c, hipMemcpyAsync(B.d.cv[1], B.d.cv[0], 5 * B.d.sz * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    ++*n;
+  }
+  double dt = *dt_io;
+  for (int rk = 0; rk < 3; ++rk) {
+    for (auto& B : c->blocks) {
+      HIPCHK(c, hipMemsetAsync(B.d.dq, 0, 5 * B.d.n0 * sizeof(double), c->st));
+      ++*n;
+    }
+    if (!c->s.fixed_time_step && rk == 0) {
+      int r = plain_dt(c, &dt, n);
+      if (r) return r;
+    }
+    for (size_t b = 0; b < c->blocks.size(); ++b) {
+      const DevBlock& B = c->blocks[b].d;
+      plain_rhs(B, c->P, c->st, n);
+      plain_update(B, dt, rk, c->st, n);
+      plain_bc(c, (int)b, n);
+    }
+    plain_exchange(c, n);
+    for (auto& B : c->blocks) padded_pass(B.d, 0, OP_PROPS, c->st, n);
+  }
+  *dt_io = dt;
+  return OSTC_OK;
+}
+
+// ------------------------------------------------------- batched step
+int batched_record(ostc_ctx c, long long* n, bool events) {
+  if (!c->s.fixed_time_step) {
+    FusedArgs a = fused_args(c, 0, 1, 0);
+    const TileShape t = fused_shape(c->P.scheme, c->tile_index);
+    dt_batched(a, t.tx, t.ty, t.tz, std::min(c->n_tiles, c->cus * 4), c->d_dt, c->st, n);
+  }
+  for (int rk = 0; rk < 3; ++rk) {
+    const int in = rk, out = (rk + 1) % kRing;
+    FusedArgs a = fused_args(c, in, out, rk);
+    if (events) HIPCHK(c, hipEventRecord(c->ev_f[2 * rk], c->st));
+    if (fused_launch(c->tile_index, 0, a, c->n_wg, c->st) < 0)
+      return fail(c, OSTC_ERR_UNSUPPORTED, "fused kernel: scheme/tile not available");
+    ++*n;
+    if (events) HIPCHK(c, hipEventRecord(c->ev_f[2 * rk + 1], c->st));
+    launch_group(c, c->rim, out, in, n);
+    for (auto& g : c->bc_rounds) launch_group(c, g, out, out, n);
+    for (auto& g : c->comm_levels) launch_group(c, g, out, out, n);
+    launch_group(c, c->props, out, out, n);
+  }
+  return OSTC_OK;
+}
+
+int build_graph(ostc_ctx c) {
+  invalidate_graph(c);
+  long long n = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const bool ev = attempt == 0;
+    n = 0;
+    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    int r = batched_record(c, &n, ev);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->st, &g);
+    if (r == OSTC_OK && e == hipSuccess) {
+      hipGraphExec_t ex = nullptr;
+      e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      if (e == hipSuccess) {
+        c->graph = g;
+        c->gexec = ex;
+        c->graph_valid = true;
+        c->graph_events = ev;
+        c->graph_launches = n;
+        return OSTC_OK;
+      }
+    }
+    if (g) hipGraphDestroy(g);
+    (void)hipGetLastError();
+    if (!ev) return fail(c, OSTC_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+  }
+  return fail(c, OSTC_ERR_HIP, "graph capture failed");
+}
+
+int check_ready(ostc_ctx c) {
+  if (!c) return OSTC_ERR_ARG;
+  if (!c->finalized) return fail(c, OSTC_ERR_STATE, "ostc_finalize has not been called");
+  return OSTC_OK;
+}
+
+int check_block(ostc_ctx c, int b) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (b < 0 || b >= (int)c->blocks.size()) return fail(c, OSTC_ERR_ARG, "block index out of range");
+  return OSTC_OK;
+}
+
+}  // namespace
+
+// =================================================================== ABI
+extern "C" {
+
+int ostc_create(const ostc_setup* s, void* hip_stream, ostc_ctx* out) {
+  if (!s || !out) return OSTC_ERR_ARG;
+  *out = nullptr;
+  if (s->n_var != 5) return OSTC_ERR_UNSUPPORTED;
+  if (s->inviscid_scheme != 51 && s->inviscid_scheme != 50 && s->inviscid_scheme != 71) return OSTC_ERR_UNSUPPORTED;
+  if (s->viscous_order != 0 && s->viscous_order != 2) return OSTC_ERR_UNSUPPORTED;
+  if (!(s->cfl > 0) || !(s->prandtl > 0)) return OSTC_ERR_ARG;
+  auto* c = new ostc_ctx_s;
+  c->s = *s;
+  c->ngg = s->inviscid_scheme == 71 ? 4 : 3;
+  c->P.n_var = 5;
+  c->P.scheme = s->inviscid_scheme;
+  c->P.visc = s->viscous_order == 2;
+  c->P.cfl = s->cfl;
+  c->P.prandtl = s->prandtl;
+  c->P.weno_eps_scale = s->rho_ref * s->v_ref * s->rho_ref * s->v_ref;  // src/DParameter.cu:26-29
+  c->P.v_ref = s->v_ref;
+  if (hip_stream) {
+    c->st = static_cast<hipStream_t>(hip_stream);
+  } else {
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return OSTC_ERR_HIP;
+    }
+    c->own_stream = true;
+  }
+  int dev = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev);
+  *out = c;
+  return OSTC_OK;
+}
+
+int ostc_destroy(ostc_ctx c) {
+  if (!c) return OSTC_ERR_ARG;
+  if (c->st) hipStreamSynchronize(c->st);
+  invalidate_graph(c);
+  for (auto& e : c->ev_f)
+    if (e) hipEventDestroy(e);
+  if (c->ev_a) hipEventDestroy(c->ev_a);
+  if (c->ev_b) hipEventDestroy(c->ev_b);
+  for (void* p : c->allocs) hipFree(p);
+  if (c->d_tile_block) hipFree(c->d_tile_block);
+  if (c->own_stream) hipStreamDestroy(c->st);
+  delete c;
+  return OSTC_OK;
+}
+
+int ostc_add_block(ostc_ctx c, const ostc_block_desc* d, int* id) {
+  if (!c || !d) return OSTC_ERR_ARG;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "blocks are fixed after ostc_finalize");
+  if (d->mx < c->ngg + 2 || d->my < c->ngg + 2 || d->mz < c->ngg + 2)
+    return fail(c, OSTC_ERR_UNSUPPORTED, "each extent must be at least ngg+2 (three-dimensional blocks only)");
+  HBlock B;
+  B.m[0] = d->mx;
+  B.m[1] = d->my;
+  B.m[2] = d->mz;
+  c->blocks.push_back(B);
+  if (id) *id = (int)c->blocks.size() - 1;
+  return OSTC_OK;
+}
+
+int ostc_add_boundary(ostc_ctx c, int block, int axis, int side, int type) {
+  if (!c) return OSTC_ERR_ARG;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_finalize");
+  if (block < 0 || block >= (int)c->blocks.size() || axis < 0 || axis > 2 || (side != -1 && side != 1))
+    return fail(c, OSTC_ERR_ARG, "bad boundary face");
+  if (type != OSTC_BC_OUTFLOW && type != OSTC_BC_PERIODIC)
+    return fail(c, OSTC_ERR_UNSUPPORTED, "boundary type not implemented (outflow 6 and periodic 8 are)");
+  c->blocks[block].faces.push_back({axis, side, type});
+  return OSTC_OK;
+}
+
+int ostc_add_inner_face(ostc_ctx c, int block, int axis, int side, int target) {
+  if (!c) return OSTC_ERR_ARG;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_finalize");
+  const int nb = (int)c->blocks.size();
+  if (block < 0 || block >= nb || target < 0 || target >= nb || axis < 0 || axis > 2 || (side != -1 && side != 1))
+    return fail(c, OSTC_ERR_ARG, "bad inner face");
+  c->blocks[block].inner.push_back({axis, side, target});
+  return OSTC_OK;
+}
+
+int ostc_finalize(ostc_ctx c) {
+  if (!c) return OSTC_ERR_ARG;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "already finalized");
+  if (c->blocks.empty()) return fail(c, OSTC_ERR_STATE, "no blocks");
+  const int G = c->ngg;
+  for (auto& B : c->blocks) {
+    DevBlock& d = B.d;
+    d.mx = B.m[0];
+    d.my = B.m[1];
+    d.mz = B.m[2];
+    d.g = G;
+    d.sx = d.mx + 2 * G;
+    d.sy = d.my + 2 * G;
+    d.sz = (long long)d.sx * d.sy * (d.mz + 2 * G);
+    d.n0 = (long long)d.mx * d.my * d.mz;
+    int r = 0;
+    for (int q = 0; q < kRing && !r; ++q) {
+      r = dalloc(c, &d.cv[q], 5 * d.sz);
+      if (!r) r = dalloc(c, &d.bv[q], 6 * d.sz);
+      if (!r) r = dalloc(c, &d.mul[q], d.sz);
+    }
+    double *jac = nullptr, *met = nullptr;
+    if (!r) r = dalloc(c, &d.mach, d.sz);
+    if (!r) r = dalloc(c, &jac, d.sz);
+    if (!r) r = dalloc(c, &met, 9 * d.sz);
+    if (!r) r = dalloc(c, &d.dq, 5 * d.n0);
+    if (!r) r = dalloc(c, &d.dtl, d.n0);
+    if (!r) r = dalloc(c, &d.bv_last, 4 * d.n0);
+    if (r) return r;
+    d.jac = jac;
+    d.met = met;
+  }
+  {
+    void* v = nullptr;
+    HIPCHK(c, hipMalloc(&v, c->blocks.size() * sizeof(DevBlock)));
+    c->allocs.push_back(v);
+    c->d_blocks = static_cast<DevBlock*>(v);
+  }
+  int r = dalloc(c, &c->d_scratch, 1 + 4 * (size_t)c->n_first);
+  if (!r) r = dalloc(c, &c->d_dt, 1);
+  if (!r) r = build_ops(c);
+  if (r) return r;
+  for (auto& e : c->ev_f) HIPCHK(c, hipEventCreate(&e));
+  HIPCHK(c, hipEventCreate(&c->ev_a));
+  HIPCHK(c, hipEventCreate(&c->ev_b));
+  if (c->s.fixed_time_step) HIPCHK(c, hipMemcpy(c->d_dt, &c->s.dt, sizeof(double), hipMemcpyHostToDevice));
+  c->finalized = true;
+  return set_tiles(c, 0, 0);
+}
+
+int ostc_ngg(ostc_ctx c, int* ngg) {
+  if (!c || !ngg) return OSTC_ERR_ARG;
+  *ngg = c->ngg;
+  return OSTC_OK;
+}
+
+int ostc_zone_view(ostc_ctx c, int b, ostc_zone* z) {
+  int r = check_block(c, b);
+  if (r) return r;
+  if (!z) return OSTC_ERR_ARG;
+  const DevBlock& d = c->blocks[b].d;
+  z->mx = d.mx; z->my = d.my; z->mz = d.mz; z->ngg = d.g; z->sz = d.sz;
+  z->cv = d.cv[0]; z->bv = d.bv[0]; z->mul = d.mul[0]; z->mach = d.mach;
+  z->jac = const_cast<double*>(d.jac); z->metric = const_cast<double*>(d.met);
+  z->dq = d.dq; z->dt_local = d.dtl;
+  return OSTC_OK;
+}
+
+int ostc_set_geometry(ostc_ctx c, int b, const double* jac, const double* metric, long long es, long long cs,
+                      long long off) {
+  int r = check_block(c, b);
+  if (r) return r;
+  if (!jac || !metric) return fail(c, OSTC_ERR_ARG, "null geometry");
+  const DevBlock& d = c->blocks[b].d;
+  std::vector<double> m(9 * (size_t)d.sz);
+  for (long long p = 0; p < d.sz; ++p) {
+    if (!(jac[p] > 0)) return fail(c, OSTC_ERR_NUMERIC, "non-positive Jacobian");
+    for (int q = 0; q < 9; ++q) m[q * d.sz + p] = metric[p * es + q * cs + off];
+  }
+  HIPCHK(c, hipMemcpy(const_cast<double*>(d.jac), jac, d.sz * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(const_cast<double*>(d.met), m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice));
+  return OSTC_OK;
+}
+
+static int field_ptr(ostc_ctx c, int b, int f, int slot, double** p, size_t* n) {
+  const DevBlock& d = c->blocks[b].d;
+  switch (f) {
+    case OSTC_FIELD_CV: *p = d.cv[slot]; *n = 5 * d.sz; break;
+    case OSTC_FIELD_BV: *p = d.bv[slot]; *n = 6 * d.sz; break;
+    case OSTC_FIELD_MUL: *p = d.mul[slot]; *n = d.sz; break;
+    case OSTC_FIELD_MACH: *p = d.mach; *n = d.sz; break;
+    case OSTC_FIELD_DQ: *p = d.dq; *n = 5 * d.n0; break;
+    case OSTC_FIELD_DT_LOCAL: *p = d.dtl; *n = d.n0; break;
+    case OSTC_FIELD_JAC: *p = const_cast<double*>(d.jac); *n = d.sz; break;
+    case OSTC_FIELD_METRIC: *p = const_cast<double*>(d.met); *n = 9 * d.sz; break;
+    case OSTC_FIELD_QN: *p = d.cv[1]; *n = 5 * d.sz; break;
+    default: return fail(c, OSTC_ERR_ARG, "unknown field");
+  }
+  return OSTC_OK;
+}
+
+int ostc_set_field(ostc_ctx c, int b, int f, const double* host) {
+  int r = check_block(c, b);
+  if (r) return r;
+  if (!host) return OSTC_ERR_ARG;
+  const bool ring = f == OSTC_FIELD_CV || f == OSTC_FIELD_BV || f == OSTC_FIELD_MUL;
+  for (int slot = 0; slot < (ring ? kRing : 1); ++slot) {
+    double* p;
+    size_t n;
+    r = field_ptr(c, b, f, slot, &p, &n);
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(p, host, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_get_field(ostc_ctx c, int b, int f, double* host) {
+  int r = check_block(c, b);
+  if (r) return r;
+  if (!host) return OSTC_ERR_ARG;
+  double* p;
+  size_t n;
+  r = field_ptr(c, b, f, 0, &p, &n);
+  if (r) return r;
+  HIPCHK(c, hipMemcpyAsync(host, p, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_initialize(ostc_ctx c) {
+  int r = check_ready(c);
+  if (r) return r;
+  long long n = 0;
+  for (size_t b = 0; b < c->blocks.size(); ++b) plain_bc(c, (int)b, &n);
+  plain_exchange(c, &n);
+  for (auto& B : c->blocks) padded_pass(B.d, 0, OP_PROPS, c->st, &n);
+  for (auto& B : c->blocks) padded_pass(B.d, 0, OP_CV_FROM_BV, c->st, &n);
+  // every ring slot starts identical, so ghost cells that no operator
+  // rewrites hold the same value whichever slot a stage reads
+  for (auto& B : c->blocks)
+    for (int q = 1; q < kRing; ++q) {
+      HIPCHK(c, hipMemcpyAsync(B.d.cv[q], B.d.cv[0], 5 * B.d.sz * 8, hipMemcpyDeviceToDevice, c->st));
+      HIPCHK(c, hipMemcpyAsync(B.d.bv[q], B.d.bv[0], 6 * B.d.sz * 8, hipMemcpyDeviceToDevice, c->st));
+      HIPCHK(c, hipMemcpyAsync(B.d.mul[q], B.d.mul[0], B.d.sz * 8, hipMemcpyDeviceToDevice, c->st));
+    }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_step(ostc_ctx c, int n_steps, int mode, double* last_dt) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (n_steps < 0) return fail(c, OSTC_ERR_ARG, "negative step count");
+  long long n = 0;
+  HIPCHK(c, hipEventRecord(c->ev_a, c->st));
+  double dt = c->s.fixed_time_step ? c->s.dt : 1e6;
+  if (mode == OSTC_MODE_PLAIN) {
+    for (int s = 0; s < n_steps; ++s) {
+      r = plain_step(c, &dt, &n);
+      if (r) return r;
+    }
+  } else if (mode == OSTC_MODE_BATCHED) {
+    if (!c->graph_valid) {
+      r = build_graph(c);
+      if (r) return r;
+    }
+    for (int s = 0; s < n_steps; ++s) HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+    n = c->graph_launches * n_steps;
+    if (last_dt && !c->s.fixed_time_step)
+      HIPCHK(c, hipMemcpyAsync(&dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  } else {
+    return fail(c, OSTC_ERR_ARG, "unknown mode");
+  }
+  HIPCHK(c, hipEventRecord(c->ev_b, c->st));
+  HIPCHK(c, hipGetLastError());
+  c->last_launches = n;
+  if (last_dt) {
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    *last_dt = dt;
+  }
+  return OSTC_OK;
+}
+
+int ostc_last_step_timing(ostc_ctx c, double* ms, long long* launches) {
+  int r = check_ready(c);
+  if (r) return r;
+  HIPCHK(c, hipEventSynchronize(c->ev_b));
+  float f = 0;
+  HIPCHK(c, hipEventElapsedTime(&f, c->ev_a, c->ev_b));
+  if (ms) *ms = f;
+  if (launches) *launches = c->last_launches;
+  return OSTC_OK;
+}
+
+int ostc_fused_kernel_ms(ostc_ctx c, double ms[3]) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (!c->graph_valid || !c->graph_events) return fail(c, OSTC_ERR_STATE, "no timed batched step");
+  HIPCHK(c, hipEventSynchronize(c->ev_f[5]));
+  for (int q = 0; q < 3; ++q) {
+    float f = 0;
+    HIPCHK(c, hipEventElapsedTime(&f, c->ev_f[2 * q], c->ev_f[2 * q + 1]));
+    ms[q] = f;
+  }
+  return OSTC_OK;
+}
+
+int ostc_stage_rhs(ostc_ctx c, int b) {
+  int r = check_block(c, b);
+  if (r) return r;
+  long long n = 0;
+  const DevBlock& d = c->blocks[b].d;
+  HIPCHK(c, hipMemsetAsync(d.dq, 0, 5 * d.n0 * sizeof(double), c->st));
+  plain_rhs(d, c->P, c->st, &n);
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_batched_rhs(ostc_ctx c, int b) {
+  int r = check_block(c, b);
+  if (r) return r;
+  FusedArgs a = fused_args(c, 0, 1, 0);
+  a.target_block = b;
+  if (fused_launch(c->tile_index, 1, a, c->n_wg, c->st) < 0)
+    return fail(c, OSTC_ERR_UNSUPPORTED, "fused kernel: scheme/tile not available");
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_compute_dt(ostc_ctx c, double* dt_out) {
+  int r = check_ready(c);
+  if (r) return r;
+  long long n = 0;
+  double dt = 0;
+  r = plain_dt(c, &dt, &n);
+  if (r) return r;
+  if (!(dt > 0) || std::isnan(dt)) return fail(c, OSTC_ERR_NUMERIC, "non-positive or NaN time step");
+  HIPCHK(c, hipMemcpyAsync(c->d_dt, &dt, sizeof(double), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (dt_out) *dt_out = dt;
+  return OSTC_OK;
+}
+
+int ostc_stage_update(ostc_ctx c, int b, int rk, double dt) {
+  int r = check_block(c, b);
+  if (r) return r;
+  if (rk < 0 || rk > 2) return fail(c, OSTC_ERR_ARG, "rk must be 0, 1 or 2");
+  long long n = 0;
+  plain_update(c->blocks[b].d, dt, rk, c->st, &n);
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_apply_bc(ostc_ctx c, int b) {
+  int r = check_block(c, b);
+  if (r) return r;
+  long long n = 0;
+  plain_bc(c, b, &n);
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_exchange(ostc_ctx c) {
+  int r = check_ready(c);
+  if (r) return r;
+  long long n = 0;
+  plain_exchange(c, &n);
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_update_properties(ostc_ctx c, int b) {
+  int r = check_block(c, b);
+  if (r) return r;
+  long long n = 0;
+  padded_pass(c->blocks[b].d, 0, OP_PROPS, c->st, &n);
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_copy_qn(ostc_ctx c, int b) {
+  int r = check_block(c, b);
+  if (r) return r;
+  const DevBlock& d = c->blocks[b].d;
+  HIPCHK(c, hipMemcpyAsync(d.cv[1], d.cv[0], 5 * d.sz * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  return OSTC_OK;
+}
+
+int ostc_store_last_step(ostc_ctx c) {
+  int r = check_ready(c);
+  if (r) return r;
+  for (auto& B : c->blocks) store_last(B.d, c->st);
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_residual(ostc_ctx c, double res[4]) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (!res) return OSTC_ERR_ARG;
+  double acc[4] = {0, 0, 0, 0};
+  long long n_total = 0;
+  const int nblk = 256;
+  std::vector<double> h(4 * nblk);
+  for (auto& B : c->blocks) {
+    residual_partial(B.d, c->d_scratch + 1, nblk, c->st);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_scratch + 1, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int q = 0; q < nblk; ++q)
+      for (int l = 0; l < 4; ++l) acc[l] += h[4 * q + l];
+    n_total += B.d.n0;
+  }
+  for (int l = 0; l < 4; ++l) res[l] = std::sqrt(acc[l] / (double)n_total);
+  if (std::isnan(res[0] + res[1] + res[2] + res[3])) return fail(c, OSTC_ERR_NUMERIC, "NaN in the residual");
+  return OSTC_OK;
+}
+
+int ostc_tile_count(int* n) {
+  if (!n) return OSTC_ERR_ARG;
+  *n = fused_shape_count(51);
+  return OSTC_OK;
+}
+
+int ostc_tile_shape(int index, int* tx, int* ty, int* tz) {
+  if (index < 0 || index >= fused_shape_count(51)) return OSTC_ERR_ARG;
+  const TileShape t = fused_shape(51, index);
+  if (tx) *tx = t.tx;
+  if (ty) *ty = t.ty;
+  if (tz) *tz = t.tz;
+  return OSTC_OK;
+}
+
+int ostc_set_tile(ostc_ctx c, int index, int waves_per_cu) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (index < 0 || index >= fused_shape_count(c->P.scheme)) return fail(c, OSTC_ERR_ARG, "tile index out of range");
+  // waves_per_cu <= 0: one workgroup per tile; otherwise a persistent grid of
+  // waves_per_cu workgroups per compute unit, with XCD-contiguous chunks.
+  c->xcd_remap = waves_per_cu > 0 ? 1 : 0;
+  return set_tiles(c, index, waves_per_cu > 0 ? waves_per_cu * c->cus : 0);
+}
+
+int ostc_autotune(ostc_ctx c, int reps, int* best_index, double* best_ms) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (reps < 1) reps = 3;
+  double best = 1e30;
+  int bi = c->tile_index, bw = 0;
+  const int waves[] = {0, 1, 2, 4};
+  for (int idx = 0; idx < fused_shape_count(c->P.scheme); ++idx) {
+    const int lds = fused_lds_bytes(c->P.scheme, idx);
+    if (lds < 0 || lds > 160 * 1024) continue;
+    for (int w : waves) {
+      r = ostc_set_tile(c, idx, w);
+      if (r) return r;
+      FusedArgs a = fused_args(c, 0, 1, 0);
+      if (fused_launch(idx, 0, a, c->n_wg, c->st) < 0) continue;  // warm-up
+      HIPCHK(c, hipEventRecord(c->ev_a, c->st));
+      for (int q = 0; q < reps; ++q) fused_launch(idx, 0, a, c->n_wg, c->st);
+      HIPCHK(c, hipEventRecord(c->ev_b, c->st));
+      HIPCHK(c, hipEventSynchronize(c->ev_b));
+      HIPCHK(c, hipGetLastError());
+      float f = 0;
+      HIPCHK(c, hipEventElapsedTime(&f, c->ev_a, c->ev_b));
+      const double ms = f / reps;
+      if (ms < best) { best = ms; bi = idx; bw = w; }
+    }
+  }
+  r = ostc_set_tile(c, bi, bw);
+  if (r) return r;
+  if (best_index) *best_index = bi;
+  if (best_ms) *best_ms = best;
+  return OSTC_OK;
+}
+
+int ostc_sync(ostc_ctx c) {
+  if (!c) return OSTC_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+const char* ostc_last_error(ostc_ctx c) { return c ? c->err.c_str() : "null context"; }
+
+// ----------------------------------------------------------- setup helpers
+// Block::compute_jac_metric (src/Mesh.cpp:241-300) on an analytic uniform
+// grid x = x0 + i dx (ghost nodes included), SoA output on the padded domain.
+int ostc_util_uniform_geometry(int mx, int my, int mz, int g, double x0, double y0, double z0, double dx, double dy,
+                               double dz, double* jac, double* metric) {
+  if (mx < 2 || my < 2 || mz < 2 || g < 1 || !jac || !metric) return OSTC_ERR_ARG;
+  const long long sx = mx + 2 * g, sy = my + 2 * g, sz = sx * sy * (mz + 2 * g);
+  for (int k = -g; k < mz + g; ++k)
+    for (int j = -g; j < my + g; ++j)
+      for (int i = -g; i < mx + g; ++i) {
+        auto X = [&](int a) { return x0 + a * dx; };
+        auto Y = [&](int a) { return y0 + a * dy; };
+        auto Z = [&](int a) { return z0 + a * dz; };
+        const double dxd1 = (X(i + 1) - X(i - 1)) * 0.5, dyd1 = (Y(j) - Y(j)) * 0.5, dzd1 = (Z(k) - Z(k)) * 0.5;
+        const double dxd2 = (X(i) - X(i)) * 0.5, dyd2 = (Y(j + 1) - Y(j - 1)) * 0.5, dzd2 = (Z(k) - Z(k)) * 0.5;
+        const double dxd3 = (X(i) - X(i)) * 0.5, dyd3 = (Y(j) - Y(j)) * 0.5, dzd3 = (Z(k + 1) - Z(k - 1)) * 0.5;
+        const double J = dxd1 * dyd2 * dzd3 + dxd2 * dyd3 * dzd1 + dxd3 * dyd1 * dzd2 - dxd1 * dyd3 * dzd2 -
+                         dxd2 * dyd1 * dzd3 - dxd3 * dyd2 * dzd1;
+        if (!(J > 0)) return OSTC_ERR_NUMERIC;
+        const long long p = ((long long)(k + g) * sy + (j + g)) * sx + (i + g);
+        jac[p] = J;
+        metric[p + 0 * sz] = (dyd2 * dzd3 - dyd3 * dzd2) / J;
+        metric[p + 1 * sz] = (dxd3 * dzd2 - dxd2 * dzd3) / J;
+        metric[p + 2 * sz] = (dxd2 * dyd3 - dxd3 * dyd2) / J;
+        metric[p + 3 * sz] = (dyd3 * dzd1 - dyd1 * dzd3) / J;
+        metric[p + 4 * sz] = (dxd1 * dzd3 - dxd3 * dzd1) / J;
+        metric[p + 5 * sz] = (dxd3 * dyd1 - dxd1 * dyd3) / J;
+        metric[p + 6 * sz] = (dyd1 * dzd2 - dyd2 * dzd1) / J;
+        metric[p + 7 * sz] = (dxd2 * dzd1 - dxd1 * dzd2) / J;
+        metric[p + 8 * sz] = (dxd1 * dyd2 - dxd2 * dyd1) / J;
+      }
+  return OSTC_OK;
+}
+
+static uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  uint64_t z = x;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// initialize_mixing_layer_with_info for air (src/Initialize.cu:259-325) on the
+// padded domain, y = y0 + j dy, plus the seeded velocity perturbation.
+int ostc_util_mixing_layer_bv(int mx, int my, int mz, int g, double y0, double dy, const double upper[6],
+                              const double lower[6], double delta_omega, int gi0, int gj0, int gk0, int period_i,
+                              int period_j, int period_k, uint64_t seed, double amplitude, double* bv) {
+  if (mx < 2 || my < 2 || mz < 2 || g < 1 || !upper || !lower || !bv) return OSTC_ERR_ARG;
+  const long long sx = mx + 2 * g, sy = my + 2 * g, sz = sx * sy * (mz + 2 * g);
+  auto wrap = [](long long v, int per) -> long long {
+    if (per <= 0) return v;
+    long long r = v % per;
+    return r < 0 ? r + per : r;
+  };
+  for (int k = -g; k < mz + g; ++k)
+    for (int j = -g; j < my + g; ++j)
+      for (int i = -g; i < mx + g; ++i) {
+        const long long p = ((long long)(k + g) * sy + (j + g)) * sx + (i + g);
+        const double y = y0 + j * dy;
+        bv[p + sz] = 0.5 * (upper[1] + lower[1]) + 0.5 * (upper[1] - lower[1]) * std::tanh(2 * y / delta_omega);
+        const double* var = y >= 0 ? upper : lower;
+        bv[p] = var[0];
+        bv[p + 2 * sz] = var[2];
+        bv[p + 3 * sz] = var[3];
+        bv[p + 4 * sz] = var[4];
+        bv[p + 5 * sz] = var[5];
+        if (amplitude != 0) {
+          const long long gi = wrap((long long)gi0 + i, period_i), gj = wrap((long long)gj0 + j, period_j),
+                          gk = wrap((long long)gk0 + k, period_k);
+          const uint64_t node = (uint64_t)((gk * 1000003LL + gj) * 1000033LL + gi);
+          for (int comp = 0; comp < 3; ++comp) {
+            const uint64_t h = splitmix64(seed ^ splitmix64(node * 4 + (uint64_t)comp));
+            const double r = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+            bv[p + (1 + comp) * sz] += amplitude * (2.0 * r - 1.0);
+          }
+        }
+      }
+  return OSTC_OK;
+}
+
+}  // extern "C"

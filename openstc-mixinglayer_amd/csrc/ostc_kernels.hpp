@@ -1,0 +1,41 @@
+// ostc_kernels.hpp -- host-side launcher declarations for the HIP kernels.
+#pragma once
+#include <algorithm>
+#include "ostc_ops.hpp"
+
+namespace ostc {
+
+// per-block (reference-shaped) path, ostc_kernels_plain.hip
+void plain_rhs(const DevBlock& B, const Params& P, hipStream_t st, long long* n);
+void plain_update(const DevBlock& B, double dt, int rk, hipStream_t st, long long* n);
+void plain_local_dt(const DevBlock& B, const Params& P, hipStream_t st, long long* n);
+void min_reduce(const double* in, long long len, double* scratch, int n_first, hipStream_t st, long long* n);
+void padded_pass(const DevBlock& B, int slot, int what, hipStream_t st, long long* n);
+void ops_launch(const DevBlock* d_blocks, const DevOp* d_ops, int op_lo, int op_hi, long long total, int slot,
+                int src, hipStream_t st, long long* n);
+void store_last(const DevBlock& B, hipStream_t st);
+void residual_partial(const DevBlock& B, double* out4, int nblk, hipStream_t st);
+
+// fused multi-block path, ostc_kernels_fused.hip
+struct FusedArgs {
+  const DevBlock* blocks;
+  const int* tile_block;     // block of each tile
+  int n_tiles;
+  int in, out;               // ring slots read / written
+  double ra, rb, rc;         // SSP-RK3 coefficients of this stage
+  const double* dt;          // device dt (bit pattern of a positive double)
+  Params P;
+  int xcd_remap;             // 1: contiguous tile chunks per XCD
+  int target_block;          // >= 0: only this block's tiles (dq mode)
+};
+
+struct TileShape { int tx, ty, tz; };
+int fused_shape_count(int scheme);
+TileShape fused_shape(int scheme, int index);
+// Launches the fused kernel of table entry `index`; mode 0 = RK update,
+// mode 1 = write dq only.  Returns the dynamic LDS bytes used, or -1.
+int fused_launch(int index, int mode, const FusedArgs& a, int n_wg, hipStream_t st);
+int fused_lds_bytes(int scheme, int index);
+void dt_batched(const FusedArgs& a, int tx, int ty, int tz, int n_wg, double* dt_bits, hipStream_t st, long long* n);
+
+}  // namespace ostc

@@ -1,0 +1,365 @@
+// ostc_kernels_plain.hip -- per-block kernels launched the way the reference
+// launches them (src/RK.cuh:85-226): 64-thread pencils per direction for the
+// inviscid and viscous fluxes with dq read-modify-write in HBM, an (8,8,4)
+// interior grid for the update and local time step, a two-pass MIN with a
+// host read-back for dt, and one launch per boundary or inner face.  This is
+// the drop-in path; the fused batched kernel lives in ostc_kernels_fused.hip.
+#include "ostc_kernels.hpp"
+
+namespace ostc {
+
+// compute_convective_term_weno_{x,y,z} (src/WENO.cu:8-461): one 64-thread
+// block makes 64 face fluxes and updates 63 cells along DIR.  Launch grid
+// (segments along DIR, other axis a, other axis b).
+template <int DIR, int SCHEME>
+__global__ __launch_bounds__(64) void k_weno_pencil(DevBlock B, Params P, int slot) {
+  constexpr int R = Stencil<SCHEME>::R, W = Stencil<SCHEME>::W;
+  constexpr int NP = 63 + 2 * R;
+  __shared__ double sFp[5][NP], sFm[5][NP], sFc[5][64];
+  const int m = DIR == 0 ? B.mx : (DIR == 1 ? B.my : B.mz);
+  const int c0 = 63 * blockIdx.x;
+  const int tid = threadIdx.x;
+  int base[3];
+  if (DIR == 0) { base[1] = blockIdx.y; base[2] = blockIdx.z; }
+  if (DIR == 1) { base[0] = blockIdx.y; base[2] = blockIdx.z; }
+  if (DIR == 2) { base[0] = blockIdx.y; base[1] = blockIdx.z; }
+  const double* cv = B.cv[slot];
+  const double* bv = B.bv[slot];
+  const long long s = B.sz;
+  for (int pt = tid; pt < NP; pt += 64) {
+    const int pos = c0 - R + pt;
+    if (pos > m - 1 + R) continue;
+    int ijk[3] = {base[0], base[1], base[2]};
+    ijk[DIR] = pos;
+    const long long p = pidx(B, ijk[0], ijk[1], ijk[2]);
+    double q[5];
+    for (int l = 0; l < 5; ++l) q[l] = cv[p + l * s];
+    const double c = sqrt(kGamma * kRAir * bv[p + 5 * s]);
+    double fp[5], fm[5];
+    lf_split5(q, bv[p + 4 * s], c, B.met[p + (3 * DIR) * s], B.met[p + (3 * DIR + 1) * s],
+              B.met[p + (3 * DIR + 2) * s], B.jac[p], fp, fm);
+    for (int l = 0; l < 5; ++l) { sFp[l][pt] = fp[l]; sFm[l][pt] = fm[l]; }
+  }
+  __syncthreads();
+  const int L = c0 - 1 + tid;  // face L+1/2
+  if (L <= m - 1) {
+    int ijk[3] = {base[0], base[1], base[2]};
+    ijk[DIR] = L;
+    const long long p1 = pidx(B, ijk[0], ijk[1], ijk[2]);
+    ijk[DIR] = L + 1;
+    const long long p2 = pidx(B, ijk[0], ijk[1], ijk[2]);
+    const double j1 = B.jac[p1], j2 = B.jac[p2];
+    double eps3[3];
+    weno_eps(P, B.met[p1 + 3 * DIR * s] * j1, B.met[p1 + (3 * DIR + 1) * s] * j1, B.met[p1 + (3 * DIR + 2) * s] * j1,
+             B.met[p2 + 3 * DIR * s] * j2, B.met[p2 + (3 * DIR + 1) * s] * j2, B.met[p2 + (3 * DIR + 2) * s] * j2,
+             eps3);
+    const int sL = tid + R - 1;  // LDS index of cell L
+    for (int l = 0; l < 5; ++l) {
+      double vp[W], vm[W];
+      for (int r = 0; r < W; ++r) {
+        vp[r] = sFp[l][sL - (W / 2) + r];
+        vm[r] = sFm[l][sL - (W / 2) + 1 + r];
+      }
+      const double e = l == 0 ? eps3[0] : (l == 4 ? eps3[2] : eps3[1]);
+      sFc[l][tid] = weno_face<SCHEME>(vp, vm, e);
+    }
+  }
+  __syncthreads();
+  if (tid > 0 && L <= m - 1) {
+    int ijk[3] = {base[0], base[1], base[2]};
+    ijk[DIR] = L;
+    const long long p0 = pidx0(B, ijk[0], ijk[1], ijk[2]);
+    for (int l = 0; l < 5; ++l) B.dq[p0 + l * B.n0] -= sFc[l][tid] - sFc[l][tid - 1];
+  }
+}
+
+// Gradients in computational space at face (i,j,k)+1/2 along DIR from a bv
+// accessor, src/ViscousScheme.cuh:41-56 (fv), :338-353 (gv), :636-651 (hv).
+template <int DIR, class PV>
+__device__ __forceinline__ void face_gradients(PV pv, int i, int j, int k, double d[4][3]) {
+  const int var[4] = {1, 2, 3, 5};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int l = var[q];
+    if (DIR == 0) {
+      d[q][0] = pv(i + 1, j, k, l) - pv(i, j, k, l);
+      d[q][1] = 0.25 * (pv(i, j + 1, k, l) - pv(i, j - 1, k, l) + pv(i + 1, j + 1, k, l) - pv(i + 1, j - 1, k, l));
+      d[q][2] = 0.25 * (pv(i, j, k + 1, l) - pv(i, j, k - 1, l) + pv(i + 1, j, k + 1, l) - pv(i + 1, j, k - 1, l));
+    } else if (DIR == 1) {
+      d[q][0] = 0.25 * (pv(i + 1, j, k, l) - pv(i - 1, j, k, l) + pv(i + 1, j + 1, k, l) - pv(i - 1, j + 1, k, l));
+      d[q][1] = pv(i, j + 1, k, l) - pv(i, j, k, l);
+      d[q][2] = 0.25 * (pv(i, j, k + 1, l) - pv(i, j, k - 1, l) + pv(i, j + 1, k + 1, l) - pv(i, j + 1, k - 1, l));
+    } else {
+      d[q][0] = 0.25 * (pv(i + 1, j, k, l) - pv(i - 1, j, k, l) + pv(i + 1, j, k + 1, l) - pv(i - 1, j, k + 1, l));
+      d[q][1] = 0.25 * (pv(i, j + 1, k, l) - pv(i, j - 1, k, l) + pv(i, j + 1, k + 1, l) - pv(i, j - 1, k + 1, l));
+      d[q][2] = pv(i, j, k + 1, l) - pv(i, j, k, l);
+    }
+  }
+}
+
+// viscous_flux_{fv,gv,hv} (src/SchemeSelector.cuh:69-154).
+template <int DIR>
+__global__ __launch_bounds__(64) void k_visc_pencil(DevBlock B, Params P, int slot) {
+  __shared__ double sFv[4][64];
+  const int m = DIR == 0 ? B.mx : (DIR == 1 ? B.my : B.mz);
+  const int tid = threadIdx.x;
+  const int L = 63 * blockIdx.x - 1 + tid;
+  int ijk[3];
+  if (DIR == 0) { ijk[1] = blockIdx.y; ijk[2] = blockIdx.z; }
+  if (DIR == 1) { ijk[0] = blockIdx.y; ijk[2] = blockIdx.z; }
+  if (DIR == 2) { ijk[0] = blockIdx.y; ijk[1] = blockIdx.z; }
+  ijk[DIR] = L;
+  const double* bv = B.bv[slot];
+  const double* mu = B.mul[slot];
+  const long long s = B.sz;
+  if (L <= m - 1) {
+    auto pv = [&](int a, int b, int c, int l) { return bv[pidx(B, a, b, c) + l * s]; };
+    double d[4][3];
+    face_gradients<DIR>(pv, ijk[0], ijk[1], ijk[2], d);
+    int o[3] = {0, 0, 0};
+    o[DIR] = 1;
+    const long long pA = pidx(B, ijk[0], ijk[1], ijk[2]);
+    const long long pB = pidx(B, ijk[0] + o[0], ijk[1] + o[1], ijk[2] + o[2]);
+    double mA[9], mB[9];
+    for (int c = 0; c < 9; ++c) { mA[c] = B.met[pA + c * s]; mB[c] = B.met[pB + c * s]; }
+    const double um = 0.5 * (bv[pA + s] + bv[pB + s]);
+    const double vm = 0.5 * (bv[pA + 2 * s] + bv[pB + 2 * s]);
+    const double wm = 0.5 * (bv[pA + 3 * s] + bv[pB + 3 * s]);
+    double fv[4];
+    viscous_face_flux<DIR>(d, mA, mB, B.jac[pA], B.jac[pB], mu[pA], mu[pB], um, vm, wm, P.prandtl, fv);
+    for (int l = 0; l < 4; ++l) sFv[l][tid] = fv[l];
+  }
+  __syncthreads();
+  if (tid > 0 && L <= m - 1) {
+    const long long p0 = pidx0(B, ijk[0], ijk[1], ijk[2]);
+    for (int l = 0; l < 4; ++l) B.dq[p0 + (l + 1) * B.n0] += sFv[l][tid] - sFv[l][tid - 1];
+  }
+}
+
+// update_cv_and_bv_rk<Air> (src/RK.cuh:230-284), interior cells, tpb (8,8,4).
+__global__ __launch_bounds__(256) void k_update(DevBlock B, double dt, double ra, double rb, double rc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z * blockDim.z + threadIdx.z;
+  if (i >= B.mx || j >= B.my || k >= B.mz) return;
+  double* cv = B.cv[0];
+  double* bv = B.bv[0];
+  const double* qn = B.cv[1];
+  const long long p = pidx(B, i, j, k), p0 = pidx0(B, i, j, k), s = B.sz;
+  const double dt_div_jac = dt / B.jac[p];
+  double c[5];
+  for (int l = 0; l < 5; ++l) {
+    c[l] = ra * qn[p + l * s] + rb * cv[p + l * s] + rc * dt_div_jac * B.dq[p0 + l * B.n0];
+    cv[p + l * s] = c[l];
+  }
+  bv[p] = c[0];
+  const double density_inv = 1.0 / c[0];
+  const double u = c[1] * density_inv, v = c[2] * density_inv, w = c[3] * density_inv;
+  bv[p + s] = u;
+  bv[p + 2 * s] = v;
+  bv[p + 3 * s] = w;
+  const double V2 = u * u + v * v + w * w;
+  const double pr = (kGamma - 1) * (c[4] - 0.5 * c[0] * V2);
+  bv[p + 4 * s] = pr;
+  bv[p + 5 * s] = pr * kMwAir * density_inv / kRu;
+}
+
+// local_time_step<Air, Laminar>, src/TimeAdvanceFunc.cuh:28-83.
+__device__ __forceinline__ double local_dt_point(const DevBlock& B, const Params& P, int slot, long long p) {
+  const long long s = B.sz;
+  const double* m = B.met;
+  const double gxi = sqrt(m[p] * m[p] + m[p + s] * m[p + s] + m[p + 2 * s] * m[p + 2 * s]);
+  const double geta = sqrt(m[p + 3 * s] * m[p + 3 * s] + m[p + 4 * s] * m[p + 4 * s] + m[p + 5 * s] * m[p + 5 * s]);
+  const double gzeta = sqrt(m[p + 6 * s] * m[p + 6 * s] + m[p + 7 * s] * m[p + 7 * s] + m[p + 8 * s] * m[p + 8 * s]);
+  const double* bv = B.bv[slot];
+  const double u = bv[p + s], v = bv[p + 2 * s], w = bv[p + 3 * s];
+  const double U = u * m[p] + v * m[p + s] + w * m[p + 2 * s];
+  const double V = u * m[p + 3 * s] + v * m[p + 4 * s] + w * m[p + 5 * s];
+  const double Wc = u * m[p + 6 * s] + v * m[p + 7 * s] + w * m[p + 8 * s];
+  const double a = sqrt(kGamma * kRu * bv[p + 5 * s] / kMwAir);
+  const double sr_inv = (fabs(U) + a * gxi) + (fabs(V) + a * geta) + (fabs(Wc) + a * gzeta);
+  const double coeff_1 = (kGamma > 4.0 / 3.0 ? kGamma : 4.0 / 3.0) / bv[p];
+  const double coeff_2 = B.mul[slot][p] / P.prandtl;
+  double sr_visc = gxi * gxi + geta * geta;
+  sr_visc += gzeta * gzeta;
+  sr_visc *= coeff_1 * coeff_2;
+  return P.cfl / (sr_inv + sr_visc);
+}
+
+__global__ __launch_bounds__(256) void k_local_dt(DevBlock B, Params P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z * blockDim.z + threadIdx.z;
+  if (i >= B.mx || j >= B.my || k >= B.mz) return;
+  B.dtl[pidx0(B, i, j, k)] = local_dt_point(B, P, 0, pidx(B, i, j, k));
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// min_of_arr (src/TimeAdvanceFunc.cu:70-89): grid-stride MIN starting from
+// 1e6, reduced across the 64-lane wavefront with shuffles, then across waves.
+__global__ __launch_bounds__(256) void k_min_pass(const double* in, long long n, double* out) {
+  __shared__ double part[4];
+  double v = 1e6;
+  for (long long t = blockIdx.x * 256ll + threadIdx.x; t < n; t += 256ll * gridDim.x) v = fmin(v, in[t]);
+  v = wave_min(v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = fmin(fmin(part[0], part[1]), fmin(part[2], part[3]));
+}
+
+// Padded-domain kernels, tpb (8,8,4): compute_cv_from_bv and
+// update_physical_properties (src/FieldOperation.cuh:40-76, :107-140).
+__global__ __launch_bounds__(256) void k_padded(DevBlock B, int slot, int what) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x - B.g;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y - B.g;
+  const int k = blockIdx.z * blockDim.z + threadIdx.z - B.g;
+  if (i >= B.mx + B.g || j >= B.my + B.g || k >= B.mz + B.g) return;
+  DevOp op{};
+  op.type = what;
+  op.block = 0;
+  op.lo[0] = i; op.lo[1] = j; op.lo[2] = k;
+  op.n[0] = op.n[1] = op.n[2] = 1;
+  op_point(&B, op, 0, slot, slot);
+}
+
+// One launch of a range of independent ops.  Thread -> op by binary search
+// over the ops' running point offsets.
+__global__ __launch_bounds__(256) void k_ops(const DevBlock* blocks, const DevOp* ops, int op_lo, int op_hi,
+                                             long long total, int slot, int src) {
+  const long long gid = blockIdx.x * 256ll + threadIdx.x;
+  if (gid >= total) return;
+  const long long base = ops[op_lo].begin;
+  int lo = op_lo, hi = op_hi - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ops[mid].begin - base <= gid) lo = mid; else hi = mid - 1;
+  }
+  const DevOp op = ops[lo];
+  op_point(blocks, op, gid - (op.begin - base), slot, src);
+}
+
+// store_last_step and compute_square_of_dbv + reduction_of_dv_squared<4>
+// (src/TimeAdvanceFunc.cu:6-37, src/Residual.cuh:12-69).
+__global__ __launch_bounds__(256) void k_store_last(DevBlock B) {
+  const long long t = blockIdx.x * 256ll + threadIdx.x;
+  if (t >= B.n0) return;
+  const int i = (int)(t % B.mx), j = (int)((t / B.mx) % B.my), k = (int)(t / ((long long)B.mx * B.my));
+  const long long p = pidx(B, i, j, k), s = B.sz;
+  const double* bv = B.bv[0];
+  const double u = bv[p + s], v = bv[p + 2 * s], w = bv[p + 3 * s];
+  B.bv_last[t] = bv[p];
+  B.bv_last[t + B.n0] = sqrt(u * u + v * v + w * w);
+  B.bv_last[t + 2 * B.n0] = bv[p + 4 * s];
+  B.bv_last[t + 3 * B.n0] = bv[p + 5 * s];
+}
+
+__global__ __launch_bounds__(256) void k_residual(DevBlock B, double* out4) {
+  __shared__ double part[4][4];
+  double acc[4] = {0, 0, 0, 0};
+  const long long s = B.sz;
+  const double* bv = B.bv[0];
+  for (long long t = blockIdx.x * 256ll + threadIdx.x; t < B.n0; t += 256ll * gridDim.x) {
+    const int i = (int)(t % B.mx), j = (int)((t / B.mx) % B.my), k = (int)(t / ((long long)B.mx * B.my));
+    const long long p = pidx(B, i, j, k);
+    const double u = bv[p + s], v = bv[p + 2 * s], w = bv[p + 3 * s];
+    const double vel = sqrt(u * u + v * v + w * w);
+    const double d0 = bv[p] - B.bv_last[t], d1 = vel - B.bv_last[t + B.n0];
+    const double d2 = bv[p + 4 * s] - B.bv_last[t + 2 * B.n0], d3 = bv[p + 5 * s] - B.bv_last[t + 3 * B.n0];
+    acc[0] += d0 * d0; acc[1] += d1 * d1; acc[2] += d2 * d2; acc[3] += d3 * d3;
+  }
+  for (int l = 0; l < 4; ++l) {
+    const double v = wave_sum(acc[l]);
+    if ((threadIdx.x & 63) == 0) part[l][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int l = threadIdx.x;
+    out4[blockIdx.x * 4 + l] = part[l][0] + part[l][1] + part[l][2] + part[l][3];
+  }
+}
+
+// ------------------------------------------------------------- launchers
+static dim3 pencil_grid(const DevBlock& B, int dir) {
+  const int m = dir == 0 ? B.mx : (dir == 1 ? B.my : B.mz);
+  const unsigned seg = (m - 1) / 63 + 1;
+  if (dir == 0) return dim3(seg, B.my, B.mz);
+  if (dir == 1) return dim3(seg, B.mx, B.mz);
+  return dim3(seg, B.mx, B.my);
+}
+
+template <int SCHEME>
+static void launch_weno(const DevBlock& B, const Params& P, hipStream_t st, long long* n) {
+  hipLaunchKernelGGL((k_weno_pencil<0, SCHEME>), pencil_grid(B, 0), dim3(64), 0, st, B, P, 0);
+  hipLaunchKernelGGL((k_weno_pencil<1, SCHEME>), pencil_grid(B, 1), dim3(64), 0, st, B, P, 0);
+  hipLaunchKernelGGL((k_weno_pencil<2, SCHEME>), pencil_grid(B, 2), dim3(64), 0, st, B, P, 0);
+  *n += 3;
+}
+
+void plain_rhs(const DevBlock& B, const Params& P, hipStream_t st, long long* n) {
+  if (P.scheme == 71) launch_weno<71>(B, P, st, n);
+  else if (P.scheme == 50) launch_weno<50>(B, P, st, n);
+  else launch_weno<51>(B, P, st, n);
+  if (P.visc) {
+    hipLaunchKernelGGL(k_visc_pencil<0>, pencil_grid(B, 0), dim3(64), 0, st, B, P, 0);
+    hipLaunchKernelGGL(k_visc_pencil<1>, pencil_grid(B, 1), dim3(64), 0, st, B, P, 0);
+    hipLaunchKernelGGL(k_visc_pencil<2>, pencil_grid(B, 2), dim3(64), 0, st, B, P, 0);
+    *n += 3;
+  }
+}
+
+static dim3 interior_grid(const DevBlock& B) {
+  return dim3((B.mx - 1) / 8 + 1, (B.my - 1) / 8 + 1, (B.mz - 1) / 4 + 1);
+}
+
+void plain_update(const DevBlock& B, double dt, int rk, hipStream_t st, long long* n) {
+  const RkCoef c = rk_coef(rk);
+  hipLaunchKernelGGL(k_update, interior_grid(B), dim3(8, 8, 4), 0, st, B, dt, c.a, c.b, c.c);
+  *n += 1;
+}
+
+void plain_local_dt(const DevBlock& B, const Params& P, hipStream_t st, long long* n) {
+  hipLaunchKernelGGL(k_local_dt, interior_grid(B), dim3(8, 8, 4), 0, st, B, P);
+  *n += 1;
+}
+
+void min_reduce(const double* in, long long len, double* scratch, int n_first, hipStream_t st, long long* n) {
+  const int nb = (int)std::min<long long>(n_first, (len + 255) / 256);
+  hipLaunchKernelGGL(k_min_pass, dim3(nb), dim3(256), 0, st, in, len, scratch + 1);
+  hipLaunchKernelGGL(k_min_pass, dim3(1), dim3(256), 0, st, scratch + 1, (long long)nb, scratch);
+  *n += 2;
+}
+
+void padded_pass(const DevBlock& B, int slot, int what, hipStream_t st, long long* n) {
+  const dim3 g((B.mx + 2 * B.g - 1) / 8 + 1, (B.my + 2 * B.g - 1) / 8 + 1, (B.mz + 2 * B.g - 1) / 4 + 1);
+  hipLaunchKernelGGL(k_padded, g, dim3(8, 8, 4), 0, st, B, slot, what);
+  *n += 1;
+}
+
+void ops_launch(const DevBlock* d_blocks, const DevOp* d_ops, int op_lo, int op_hi, long long total, int slot,
+                int src, hipStream_t st, long long* n) {
+  if (total <= 0) return;
+  const unsigned nb = (unsigned)((total + 255) / 256);
+  hipLaunchKernelGGL(k_ops, dim3(nb), dim3(256), 0, st, d_blocks, d_ops, op_lo, op_hi, total, slot, src);
+  *n += 1;
+}
+
+void store_last(const DevBlock& B, hipStream_t st) {
+  hipLaunchKernelGGL(k_store_last, dim3((unsigned)((B.n0 + 255) / 256)), dim3(256), 0, st, B);
+}
+
+void residual_partial(const DevBlock& B, double* out4, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(k_residual, dim3(nblk), dim3(256), 0, st, B, out4);
+}
+
+}  // namespace ostc

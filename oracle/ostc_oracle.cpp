@@ -1,0 +1,995 @@
+// ostc_oracle.cpp -- CPU oracle for OpenSTC's per-stage RHS + SSP-RK3 path (air).
+//
+// TEST INFRASTRUCTURE ONLY: a host-C++ restatement, OpenMP over cells, of the
+// reference routines listed beside each function.  Expression order follows
+// the reference operation by operation; it is built with -ffp-contract=off so
+// every product and sum is rounded separately, as written.  Only tests/,
+// __graft_entry__.smoke() and the cpu_baseline leg of bench.py use it; the
+// HIP library never links or calls it.
+//
+// Parity pins (see tests/test_oracle_*.py): WENO-Z5/Z7/JS closed forms
+// (polynomial exactness, constant-data weights, mirror symmetry, 50-digit
+// mpmath evaluation), LF split identities, free-stream preservation,
+// telescoping conservation on a periodic box, RK3 amplification polynomial,
+// Sutherland and dt closed forms, viscous linear-profile exactness, and an
+// independent numpy restatement of the whole RHS on tiny grids.
+#include "ostc_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// src/Constants.h:17-34
+constexpr double kRu = 8314.462618;
+constexpr double kGammaAir = 1.4;
+constexpr double kMwAir = 28.9647;
+constexpr double kRAir = kRu / kMwAir;
+
+// src/RK.cuh:10-14
+constexpr double kRkA[3] = {1.0, 0.75, 1.0 / 3.0};
+constexpr double kRkB[3] = {0.0, 0.25, 2.0 / 3.0};
+constexpr double kRkC[3] = {1.0, 0.25, 2.0 / 3.0};
+
+struct Face {       // physical boundary face, src/Mesh.cpp:14-32
+  int axis, side, type;
+};
+
+struct InnerFace {  // aligned 1-to-1 inner face, src/Mesh.cpp:38-176
+  int axis, side, target;
+};
+
+struct Block {
+  int mx, my, mz, g;
+  long long sx, sy, sz;  // padded strides for g ghosts
+  std::vector<double> cv, bv, qn, dq, mul, mach, jac, metric, dtl, x, y, z, bv_last;
+  std::vector<Face> faces;
+  std::vector<InnerFace> inner;
+
+  long long id(int i, int j, int k) const { return (k + g) * sx * sy + (j + g) * sx + (i + g); }
+  long long id0(int i, int j, int k) const { return (long long)k * mx * my + (long long)j * mx + i; }
+  // grid arrays carry g+1 ghost layers (src/Mesh.cpp:687)
+  long long idg(int i, int j, int k) const {
+    const long long gx = mx + 2 * (g + 1), gy = my + 2 * (g + 1);
+    return (k + g + 1) * gx * gy + (j + g + 1) * gx + (i + g + 1);
+  }
+  long long n0() const { return (long long)mx * my * mz; }
+};
+
+struct Ctx {
+  orc_setup s;
+  int ngg;
+  std::vector<Block> blocks;
+  double weno_eps_scale;
+  double dt_cur = 1e6;
+};
+
+inline double sqr(double a) { return a * a; }
+
+// src/Transport.cu:6-9
+inline double sutherland(double t) {
+  return 1.7894e-5 * std::pow(t / 288.16, 1.5) * (288.16 + 110) / (t + 110);
+}
+
+// ---------------------------------------------------------------- WENO kernels
+// WENO-Z5, src/WENO.cu:1582-1617
+double weno5z(const double* vp, const double* vm, double eps) {
+  constexpr double one6th{1.0 / 6};
+  double v0{one6th * (2 * vp[2] + 5 * vp[3] - vp[4])};
+  double v1{one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3])};
+  double v2{one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2])};
+  constexpr double c1312{13.0 / 12};
+  double b0 = c1312 * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
+              0.25 * (3 * vp[2] - 4 * vp[3] + vp[4]) * (3 * vp[2] - 4 * vp[3] + vp[4]);
+  double b1 = c1312 * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
+              0.25 * (vp[1] - vp[3]) * (vp[1] - vp[3]);
+  double b2 = c1312 * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
+              0.25 * (vp[0] - 4 * vp[1] + 3 * vp[2]) * (vp[0] - 4 * vp[1] + 3 * vp[2]);
+  double tau{(b0 - b2) * (b0 - b2)};
+  double a0{0.3 + 0.3 * tau / ((eps + b0) * (eps + b0))};
+  double a1{0.6 + 0.6 * tau / ((eps + b1) * (eps + b1))};
+  double a2{0.1 + 0.1 * tau / ((eps + b2) * (eps + b2))};
+  const double fp{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
+
+  v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
+  v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
+  v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
+  b0 = c1312 * (vm[2] + vm[4] - 2 * vm[3]) * (vm[2] + vm[4] - 2 * vm[3]) +
+       0.25 * (3 * vm[2] - 4 * vm[3] + vm[4]) * (3 * vm[2] - 4 * vm[3] + vm[4]);
+  b1 = c1312 * (vm[1] + vm[3] - 2 * vm[2]) * (vm[1] + vm[3] - 2 * vm[2]) +
+       0.25 * (vm[1] - vm[3]) * (vm[1] - vm[3]);
+  b2 = c1312 * (vm[0] + vm[2] - 2 * vm[1]) * (vm[0] + vm[2] - 2 * vm[1]) +
+       0.25 * (vm[0] - 4 * vm[1] + 3 * vm[2]) * (vm[0] - 4 * vm[1] + 3 * vm[2]);
+  tau = (b0 - b2) * (b0 - b2);
+  a0 = 0.1 + 0.1 * tau / ((eps + b0) * (eps + b0));
+  a1 = 0.6 + 0.6 * tau / ((eps + b1) * (eps + b1));
+  a2 = 0.3 + 0.3 * tau / ((eps + b2) * (eps + b2));
+  const double fm{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
+  return fp + fm;
+}
+
+// WENO5-JS (Jiang & Shu 1996, J. Comput. Phys. 126:202): the same candidates
+// and smoothness indicators as WENO-Z5 above, weights alpha_k = d_k/(eps+beta_k)^2.
+// Not present in the reference; selected by inviscid_scheme = 50.
+double weno5js(const double* vp, const double* vm, double eps) {
+  constexpr double one6th{1.0 / 6};
+  double v0{one6th * (2 * vp[2] + 5 * vp[3] - vp[4])};
+  double v1{one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3])};
+  double v2{one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2])};
+  constexpr double c1312{13.0 / 12};
+  double b0 = c1312 * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
+              0.25 * (3 * vp[2] - 4 * vp[3] + vp[4]) * (3 * vp[2] - 4 * vp[3] + vp[4]);
+  double b1 = c1312 * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
+              0.25 * (vp[1] - vp[3]) * (vp[1] - vp[3]);
+  double b2 = c1312 * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
+              0.25 * (vp[0] - 4 * vp[1] + 3 * vp[2]) * (vp[0] - 4 * vp[1] + 3 * vp[2]);
+  double a0{0.3 / ((eps + b0) * (eps + b0))};
+  double a1{0.6 / ((eps + b1) * (eps + b1))};
+  double a2{0.1 / ((eps + b2) * (eps + b2))};
+  const double fp{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
+
+  v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
+  v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
+  v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
+  b0 = c1312 * (vm[2] + vm[4] - 2 * vm[3]) * (vm[2] + vm[4] - 2 * vm[3]) +
+       0.25 * (3 * vm[2] - 4 * vm[3] + vm[4]) * (3 * vm[2] - 4 * vm[3] + vm[4]);
+  b1 = c1312 * (vm[1] + vm[3] - 2 * vm[2]) * (vm[1] + vm[3] - 2 * vm[2]) +
+       0.25 * (vm[1] - vm[3]) * (vm[1] - vm[3]);
+  b2 = c1312 * (vm[0] + vm[2] - 2 * vm[1]) * (vm[0] + vm[2] - 2 * vm[1]) +
+       0.25 * (vm[0] - 4 * vm[1] + 3 * vm[2]) * (vm[0] - 4 * vm[1] + 3 * vm[2]);
+  a0 = 0.1 / ((eps + b0) * (eps + b0));
+  a1 = 0.6 / ((eps + b1) * (eps + b1));
+  a2 = 0.3 / ((eps + b2) * (eps + b2));
+  const double fm{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
+  return fp + fm;
+}
+
+// WENO-Z7, src/WENO.cu:1619-1693
+double weno7z(const double* vp, const double* vm, double eps) {
+  constexpr double one6th{1.0 / 6};
+  constexpr double d12{13.0 / 12.0}, d13{1043.0 / 960}, d14{1.0 / 12};
+  double s1{one6th * (-2 * vp[0] + 9 * vp[1] - 18 * vp[2] + 11 * vp[3])};
+  double s2{-vp[0] + 4 * vp[1] - 5 * vp[2] + 2 * vp[3]};
+  double s3{-vp[0] + 3 * vp[1] - 3 * vp[2] + vp[3]};
+  double b0{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
+  s1 = one6th * (vp[1] - 6 * vp[2] + 3 * vp[3] + 2 * vp[4]);
+  s2 = vp[2] - 2 * vp[3] + vp[4];
+  s3 = -vp[1] + 3 * vp[2] - 3 * vp[3] + vp[4];
+  double b1{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
+  s1 = one6th * (-2 * vp[2] - 3 * vp[3] + 6 * vp[4] - vp[5]);
+  s3 = -vp[2] + 3 * vp[3] - 3 * vp[4] + vp[5];
+  double b2{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
+  s1 = one6th * (-11 * vp[3] + 18 * vp[4] - 9 * vp[5] + 2 * vp[6]);
+  s2 = 2 * vp[3] - 5 * vp[4] + 4 * vp[5] - vp[6];
+  s3 = -vp[3] + 3 * vp[4] - 3 * vp[5] + vp[6];
+  double b3{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
+  double tau{(b0 - b3) * (b0 - b3)};
+  constexpr double c0{1.0 / 35}, c1{12.0 / 35}, c2{18.0 / 35}, c3{4.0 / 35};
+  double a0{c0 + c0 * tau / ((eps + b0) * (eps + b0))};
+  double a1{c1 + c1 * tau / ((eps + b1) * (eps + b1))};
+  double a2{c2 + c2 * tau / ((eps + b2) * (eps + b2))};
+  double a3{c3 + c3 * tau / ((eps + b3) * (eps + b3))};
+  constexpr double one12th{1.0 / 12};
+  double v0{one12th * (-3 * vp[0] + 13 * vp[1] - 23 * vp[2] + 25 * vp[3])};
+  double v1{one12th * (vp[1] - 5 * vp[2] + 13 * vp[3] + 3 * vp[4])};
+  double v2{one12th * (-vp[2] + 7 * vp[3] + 7 * vp[4] - vp[5])};
+  double v3{one12th * (3 * vp[3] + 13 * vp[4] - 5 * vp[5] + vp[6])};
+  const double fp{(a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3)};
+
+  s1 = one6th * (-2 * vm[6] + 9 * vm[5] - 18 * vm[4] + 11 * vm[3]);
+  s2 = -vm[6] + 4 * vm[5] - 5 * vm[4] + 2 * vm[3];
+  s3 = -vm[6] + 3 * vm[5] - 3 * vm[4] + vm[3];
+  b0 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = one6th * (vm[5] - 6 * vm[4] + 3 * vm[3] + 2 * vm[2]);
+  s2 = vm[4] - 2 * vm[3] + vm[2];
+  s3 = -vm[5] + 3 * vm[4] - 3 * vm[3] + vm[2];
+  b1 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = one6th * (-2 * vm[4] - 3 * vm[3] + 6 * vm[2] - vm[1]);
+  s3 = -vm[4] + 3 * vm[3] - 3 * vm[2] + vm[1];
+  b2 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = one6th * (-11 * vm[3] + 18 * vm[2] - 9 * vm[1] + 2 * vm[0]);
+  s2 = 2 * vm[3] - 5 * vm[2] + 4 * vm[1] - vm[0];
+  s3 = -vm[3] + 3 * vm[2] - 3 * vm[1] + vm[0];
+  b3 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  tau = (b0 - b3) * (b0 - b3);
+  a0 = c0 + c0 * tau / ((eps + b0) * (eps + b0));
+  a1 = c1 + c1 * tau / ((eps + b1) * (eps + b1));
+  a2 = c2 + c2 * tau / ((eps + b2) * (eps + b2));
+  a3 = c3 + c3 * tau / ((eps + b3) * (eps + b3));
+  v0 = one12th * (-3 * vm[6] + 13 * vm[5] - 23 * vm[4] + 25 * vm[3]);
+  v1 = one12th * (vm[5] - 5 * vm[4] + 13 * vm[3] + 3 * vm[2]);
+  v2 = one12th * (-vm[4] + 7 * vm[3] + 7 * vm[2] - vm[1]);
+  v3 = one12th * (3 * vm[3] + 13 * vm[2] - 5 * vm[1] + vm[0]);
+  const double fm{(a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3)};
+  return fp + fm;
+}
+
+// Point-local Lax-Friedrichs splitting, src/WENO.cu:697-720.
+// q = [cv(0..nv-1), p, c]; k3 = metric row of the sweep direction.
+void lf_split(const double* q, int nv, const double* m, double jac, double* fp, double* fm) {
+  const double Uk{(q[1] * m[0] + q[2] * m[1] + q[3] * m[2]) / q[0]};
+  const double pk{q[nv]};
+  const double cGradK = q[nv + 1] * std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  const double lambda0 = std::abs(Uk) + cGradK;
+  fp[0] = 0.5 * jac * (Uk * q[0] + lambda0 * q[0]);
+  fp[1] = 0.5 * jac * (Uk * q[1] + pk * m[0] + lambda0 * q[1]);
+  fp[2] = 0.5 * jac * (Uk * q[2] + pk * m[1] + lambda0 * q[2]);
+  fp[3] = 0.5 * jac * (Uk * q[3] + pk * m[2] + lambda0 * q[3]);
+  fp[4] = 0.5 * jac * (Uk * (q[4] + pk) + lambda0 * q[4]);
+  fm[0] = 0.5 * jac * (Uk * q[0] - lambda0 * q[0]);
+  fm[1] = 0.5 * jac * (Uk * q[1] + pk * m[0] - lambda0 * q[1]);
+  fm[2] = 0.5 * jac * (Uk * q[2] + pk * m[1] - lambda0 * q[2]);
+  fm[3] = 0.5 * jac * (Uk * q[3] + pk * m[2] - lambda0 * q[3]);
+  fm[4] = 0.5 * jac * (Uk * (q[4] + pk) - lambda0 * q[4]);
+  for (int l = 5; l < nv; ++l) {
+    fp[l] = 0.5 * jac * (Uk * q[l] + lambda0 * q[l]);
+    fm[l] = 0.5 * jac * (Uk * q[l] - lambda0 * q[l]);
+  }
+}
+
+// ------------------------------------------------------------------ operators
+// compute_cv_from_bv / compute_cv_from_bv_1_point / compute_total_energy<Air>,
+// src/FieldOperation.cuh:17-105
+inline void cv_from_bv_point(Block& b, long long p) {
+  const long long s = b.sz;
+  const double rho = b.bv[p];
+  b.cv[p] = rho;
+  b.cv[p + s] = rho * b.bv[p + s];
+  b.cv[p + 2 * s] = rho * b.bv[p + 2 * s];
+  b.cv[p + 3 * s] = rho * b.bv[p + 3 * s];
+  const double u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+  const double V2 = u * u + v * v + w * w;
+  double te = 0.5 * V2;
+  te *= rho;
+  te += b.bv[p + 4 * s] / (kGammaAir - 1);
+  b.cv[p + 4 * s] = te;
+}
+
+void cv_from_bv(Block& b) {
+  const int g = b.g;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = -g; k < b.mz + g; ++k)
+    for (int j = -g; j < b.my + g; ++j)
+      for (int i = -g; i < b.mx + g; ++i) cv_from_bv_point(b, b.id(i, j, k));
+}
+
+// update_physical_properties<Air>, src/FieldOperation.cuh:107-140
+void properties(Block& b) {
+  const int g = b.g;
+  const long long s = b.sz;
+  constexpr double c_temp{kGammaAir * kRu / kMwAir};
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = -g; k < b.mz + g; ++k)
+    for (int j = -g; j < b.my + g; ++j)
+      for (int i = -g; i < b.mx + g; ++i) {
+        const long long p = b.id(i, j, k);
+        const double t = b.bv[p + 5 * s];
+        const double u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+        const double V = std::sqrt(u * u + v * v + w * w);
+        b.mul[p] = sutherland(t);
+        b.mach[p] = V / std::sqrt(c_temp * t);
+      }
+}
+
+// local_time_step<Air, Laminar>, src/TimeAdvanceFunc.cuh:28-83
+void local_dt(const Ctx& c, Block& b) {
+  const long long s = b.sz, ms = b.sz;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = b.id(i, j, k);
+        const double* M = &b.metric[0];
+        auto m = [&](int r, int cc) { return M[p + (3 * (r - 1) + (cc - 1)) * ms]; };
+        const double gxi = std::sqrt(m(1, 1) * m(1, 1) + m(1, 2) * m(1, 2) + m(1, 3) * m(1, 3));
+        const double geta = std::sqrt(m(2, 1) * m(2, 1) + m(2, 2) * m(2, 2) + m(2, 3) * m(2, 3));
+        const double gzeta = std::sqrt(m(3, 1) * m(3, 1) + m(3, 2) * m(3, 2) + m(3, 3) * m(3, 3));
+        const double u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+        const double U = u * m(1, 1) + v * m(1, 2) + w * m(1, 3);
+        const double V = u * m(2, 1) + v * m(2, 2) + w * m(2, 3);
+        const double W = u * m(3, 1) + v * m(3, 2) + w * m(3, 3);
+        const double a = std::sqrt(kGammaAir * kRu * b.bv[p + 5 * s] / kMwAir);
+        const double r0 = std::abs(U) + a * gxi;
+        const double r1 = std::abs(V) + a * geta;
+        const double r2 = std::abs(W) + a * gzeta;
+        const double sr_inv = r0 + r1 + r2;
+        const double coeff_1 = std::max(kGammaAir, 4.0 / 3.0) / b.bv[p];
+        const double coeff_2 = b.mul[p] / c.s.prandtl;
+        double sr_visc = gxi * gxi + geta * geta;
+        sr_visc += gzeta * gzeta;
+        sr_visc *= coeff_1 * coeff_2;
+        b.dtl[b.id0(i, j, k)] = c.s.cfl / (sr_inv + sr_visc);
+      }
+}
+
+// compute_convective_term_weno_{x,y,z} + compute_weno_flux_cp,
+// src/WENO.cu:8-461 (pencil kernels), :1455-1536 (cp flux), :463-491 (launcher).
+// dq(i) -= fc(i+1/2) - fc(i-1/2), directions x, y, z in that order.
+void inviscid(const Ctx& c, Block& b) {
+  const int nv = c.s.n_var, g = b.g;
+  const int scheme = c.s.inviscid_scheme;
+  const long long s = b.sz;
+  const int ext[3] = {b.mx, b.my, b.mz};
+  for (int dir = 0; dir < 3; ++dir) {
+    const int m = ext[dir];
+    const int oa = (dir == 0) ? 1 : 0, ob = (dir == 2) ? 1 : 2;  // the two other axes
+    const int na = ext[oa], nb = ext[ob];
+#pragma omp parallel
+    {
+      const int np = m + 2 * g;
+      std::vector<double> Fp(np * nv), Fm(np * nv), kj(np * 3), fc((m + 1) * nv);
+      std::vector<double> q(nv + 2);
+#pragma omp for collapse(2) schedule(static)
+      for (int bb = 0; bb < nb; ++bb)
+        for (int aa = 0; aa < na; ++aa) {
+          int idx[3];
+          idx[oa] = aa;
+          idx[ob] = bb;
+          // split fluxes at every point of the pencil, ghosts included
+          for (int t = -g; t < m + g; ++t) {
+            idx[dir] = t;
+            const long long p = b.id(idx[0], idx[1], idx[2]);
+            for (int l = 0; l < nv; ++l) q[l] = b.cv[p + l * s];
+            q[nv] = b.bv[p + 4 * s];
+            q[nv + 1] = std::sqrt(kGammaAir * kRAir * b.bv[p + 5 * s]);
+            double mrow[3];
+            for (int cc = 0; cc < 3; ++cc) mrow[cc] = b.metric[p + (3 * dir + cc) * s];
+            const double jac = b.jac[p];
+            const int ps = t + g;
+            lf_split(q.data(), nv, mrow, jac, &Fp[ps * nv], &Fm[ps * nv]);
+            for (int cc = 0; cc < 3; ++cc) kj[ps * 3 + cc] = mrow[cc];
+          }
+          // faces t+1/2, t = -1 .. m-1
+          for (int t = -1; t < m; ++t) {
+            const int is = t + g;
+            idx[dir] = t;
+            const long long p1 = b.id(idx[0], idx[1], idx[2]);
+            idx[dir] = t + 1;
+            const long long p2 = b.id(idx[0], idx[1], idx[2]);
+            const double jac1{b.jac[p1]}, jac2{b.jac[p2]};
+            const double* m1 = &kj[is * 3];
+            const double* m2 = &kj[(is + 1) * 3];
+            constexpr double eps{1e-20};
+            const double eps_ref = eps * c.weno_eps_scale * 0.25 *
+                                   ((m1[0] * jac1 + m2[0] * jac2) * (m1[0] * jac1 + m2[0] * jac2) +
+                                    (m1[1] * jac1 + m2[1] * jac2) * (m1[1] * jac1 + m2[1] * jac2) +
+                                    (m1[2] * jac1 + m2[2] * jac2) * (m1[2] * jac1 + m2[2] * jac2));
+            double eps_scaled[3];
+            eps_scaled[0] = eps_ref;
+            eps_scaled[1] = eps_ref * c.s.v_ref * c.s.v_ref;
+            eps_scaled[2] = eps_scaled[1] * c.s.v_ref * c.s.v_ref;
+            for (int l = 0; l < nv; ++l) {
+              double eh{eps_scaled[0]};
+              if (l == 1 || l == 2 || l == 3) eh = eps_scaled[1];
+              else if (l == 4) eh = eps_scaled[2];
+              double f;
+              if (scheme == 71) {
+                double vp[7], vm[7];
+                for (int r = 0; r < 7; ++r) {
+                  vp[r] = Fp[(is - 3 + r) * nv + l];
+                  vm[r] = Fm[(is - 2 + r) * nv + l];
+                }
+                f = weno7z(vp, vm, eh);
+              } else {
+                double vp[5], vm[5];
+                for (int r = 0; r < 5; ++r) {
+                  vp[r] = Fp[(is - 2 + r) * nv + l];
+                  vm[r] = Fm[(is - 1 + r) * nv + l];
+                }
+                f = (scheme == 50) ? weno5js(vp, vm, eh) : weno5z(vp, vm, eh);
+              }
+              fc[(t + 1) * nv + l] = f;
+            }
+          }
+          for (int t = 0; t < m; ++t) {
+            idx[dir] = t;
+            const long long p0 = b.id0(idx[0], idx[1], idx[2]);
+            const long long n0 = b.n0();
+            for (int l = 0; l < nv; ++l)
+              b.dq[p0 + l * n0] -= fc[(t + 1) * nv + l] - fc[t * nv + l];
+          }
+        }
+    }
+  }
+}
+
+// compute_{fv,gv,hv}_2nd_order<Air, Laminar>, src/ViscousScheme.cuh:23-318,
+// :320-616, :618-909; launched per direction by viscous_flux_{fv,gv,hv},
+// src/SchemeSelector.cuh:47-154.  dq(i) += fv(i+1/2) - fv(i-1/2).
+void viscous_face(const Ctx& c, const Block& b, int dir, int i, int j, int k, double* fv) {
+  const long long s = b.sz;
+  int o[3] = {0, 0, 0};
+  o[dir] = 1;
+  const long long p = b.id(i, j, k), p1 = b.id(i + o[0], j + o[1], k + o[2]);
+  auto M = [&](long long pt, int r, int cc) { return b.metric[pt + (3 * (r - 1) + (cc - 1)) * s]; };
+  const double xi_x = 0.5 * (M(p, 1, 1) + M(p1, 1, 1));
+  const double xi_y = 0.5 * (M(p, 1, 2) + M(p1, 1, 2));
+  const double xi_z = 0.5 * (M(p, 1, 3) + M(p1, 1, 3));
+  const double eta_x = 0.5 * (M(p, 2, 1) + M(p1, 2, 1));
+  const double eta_y = 0.5 * (M(p, 2, 2) + M(p1, 2, 2));
+  const double eta_z = 0.5 * (M(p, 2, 3) + M(p1, 2, 3));
+  const double zeta_x = 0.5 * (M(p, 3, 1) + M(p1, 3, 1));
+  const double zeta_y = 0.5 * (M(p, 3, 2) + M(p1, 3, 2));
+  const double zeta_z = 0.5 * (M(p, 3, 3) + M(p1, 3, 3));
+  auto pv = [&](int ii, int jj, int kk, int l) { return b.bv[b.id(ii, jj, kk) + l * s]; };
+  // derivative along the face normal: 2-point; tangential: 4-point average
+  double d[4][3];  // [u,v,w,T][xi,eta,zeta]
+  const int var[4] = {1, 2, 3, 5};
+  for (int q = 0; q < 4; ++q) {
+    const int l = var[q];
+    if (dir == 0) {
+      d[q][0] = pv(i + 1, j, k, l) - pv(i, j, k, l);
+      d[q][1] = 0.25 * (pv(i, j + 1, k, l) - pv(i, j - 1, k, l) + pv(i + 1, j + 1, k, l) - pv(i + 1, j - 1, k, l));
+      d[q][2] = 0.25 * (pv(i, j, k + 1, l) - pv(i, j, k - 1, l) + pv(i + 1, j, k + 1, l) - pv(i + 1, j, k - 1, l));
+    } else if (dir == 1) {
+      d[q][0] = 0.25 * (pv(i + 1, j, k, l) - pv(i - 1, j, k, l) + pv(i + 1, j + 1, k, l) - pv(i - 1, j + 1, k, l));
+      d[q][1] = pv(i, j + 1, k, l) - pv(i, j, k, l);
+      d[q][2] = 0.25 * (pv(i, j, k + 1, l) - pv(i, j, k - 1, l) + pv(i, j + 1, k + 1, l) - pv(i, j + 1, k - 1, l));
+    } else {
+      d[q][0] = 0.25 * (pv(i + 1, j, k, l) - pv(i - 1, j, k, l) + pv(i + 1, j, k + 1, l) - pv(i - 1, j, k + 1, l));
+      d[q][1] = 0.25 * (pv(i, j + 1, k, l) - pv(i, j - 1, k, l) + pv(i, j + 1, k + 1, l) - pv(i, j - 1, k + 1, l));
+      d[q][2] = pv(i, j, k + 1, l) - pv(i, j, k, l);
+    }
+  }
+  const double u_x = d[0][0] * xi_x + d[0][1] * eta_x + d[0][2] * zeta_x;
+  const double u_y = d[0][0] * xi_y + d[0][1] * eta_y + d[0][2] * zeta_y;
+  const double u_z = d[0][0] * xi_z + d[0][1] * eta_z + d[0][2] * zeta_z;
+  const double v_x = d[1][0] * xi_x + d[1][1] * eta_x + d[1][2] * zeta_x;
+  const double v_y = d[1][0] * xi_y + d[1][1] * eta_y + d[1][2] * zeta_y;
+  const double v_z = d[1][0] * xi_z + d[1][1] * eta_z + d[1][2] * zeta_z;
+  const double w_x = d[2][0] * xi_x + d[2][1] * eta_x + d[2][2] * zeta_x;
+  const double w_y = d[2][0] * xi_y + d[2][1] * eta_y + d[2][2] * zeta_y;
+  const double w_z = d[2][0] * xi_z + d[2][1] * eta_z + d[2][2] * zeta_z;
+  const double mul = 0.5 * (b.mul[p] + b.mul[p1]);
+  const double mut{0};
+  const double viscosity = mul + mut;
+  const double tau_xx = viscosity * (4 * u_x - 2 * v_y - 2 * w_z) / 3.0;
+  const double tau_yy = viscosity * (4 * v_y - 2 * u_x - 2 * w_z) / 3.0;
+  const double tau_zz = viscosity * (4 * w_z - 2 * u_x - 2 * v_y) / 3.0;
+  const double tau_xy = viscosity * (u_y + v_x);
+  const double tau_xz = viscosity * (u_z + w_x);
+  const double tau_yz = viscosity * (v_z + w_y);
+  const int r = dir + 1;
+  const double kx = 0.5 * (M(p, r, 1) * b.jac[p] + M(p1, r, 1) * b.jac[p1]);
+  const double ky = 0.5 * (M(p, r, 2) * b.jac[p] + M(p1, r, 2) * b.jac[p1]);
+  const double kz = 0.5 * (M(p, r, 3) * b.jac[p] + M(p1, r, 3) * b.jac[p1]);
+  fv[0] = 0;
+  fv[1] = kx * tau_xx + ky * tau_xy + kz * tau_xz;
+  fv[2] = kx * tau_xy + ky * tau_yy + kz * tau_yz;
+  fv[3] = kx * tau_xz + ky * tau_yz + kz * tau_zz;
+  const double um = 0.5 * (pv(i, j, k, 1) + pv(i + o[0], j + o[1], k + o[2], 1));
+  const double vm = 0.5 * (pv(i, j, k, 2) + pv(i + o[0], j + o[1], k + o[2], 2));
+  const double wm = 0.5 * (pv(i, j, k, 3) + pv(i + o[0], j + o[1], k + o[2], 3));
+  const double t_x = d[3][0] * xi_x + d[3][1] * eta_x + d[3][2] * zeta_x;
+  const double t_y = d[3][0] * xi_y + d[3][1] * eta_y + d[3][2] * zeta_y;
+  const double t_z = d[3][0] * xi_z + d[3][1] * eta_z + d[3][2] * zeta_z;
+  constexpr double cp{kGammaAir * kRu / kMwAir / (kGammaAir - 1)};
+  const double prt = 0.9;  // turbulent_prandtl_number default; multiplies mut = 0
+  const double conductivity = (mul / c.s.prandtl + mut / prt) * cp;
+  fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + conductivity * (kx * t_x + ky * t_y + kz * t_z);
+}
+
+void viscous(const Ctx& c, Block& b) {
+  if (c.s.viscous_order == 0) return;
+  const int nv = c.s.n_var;
+  const long long n0 = b.n0();
+  const int ext[3] = {b.mx, b.my, b.mz};
+  for (int dir = 0; dir < 3; ++dir) {
+    const int m = ext[dir];
+    const int oa = (dir == 0) ? 1 : 0, ob = (dir == 2) ? 1 : 2;
+    const int na = ext[oa], nb = ext[ob];
+#pragma omp parallel
+    {
+      std::vector<double> fv((m + 1) * 5);
+#pragma omp for collapse(2) schedule(static)
+      for (int bb = 0; bb < nb; ++bb)
+        for (int aa = 0; aa < na; ++aa) {
+          int idx[3];
+          idx[oa] = aa;
+          idx[ob] = bb;
+          for (int t = -1; t < m; ++t) {
+            idx[dir] = t;
+            viscous_face(c, b, dir, idx[0], idx[1], idx[2], &fv[(t + 1) * 5]);
+          }
+          for (int t = 0; t < m; ++t) {
+            idx[dir] = t;
+            const long long p0 = b.id0(idx[0], idx[1], idx[2]);
+            for (int l = 0; l < nv; ++l) b.dq[p0 + l * n0] += fv[(t + 1) * 5 + l] - fv[t * 5 + l];
+          }
+        }
+    }
+  }
+}
+
+// update_cv_and_bv_rk<Air>, src/RK.cuh:230-284
+void update(const Ctx& c, Block& b, double dt, int rk) {
+  const int nv = c.s.n_var;
+  const long long s = b.sz, n0 = b.n0();
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = b.id(i, j, k), p0 = b.id0(i, j, k);
+        const double dt_div_jac = dt / b.jac[p];
+        for (int l = 0; l < nv; ++l)
+          b.cv[p + l * s] = kRkA[rk] * b.qn[p + l * s] + kRkB[rk] * b.cv[p + l * s] +
+                            kRkC[rk] * dt_div_jac * b.dq[p0 + l * n0];
+        b.bv[p] = b.cv[p];
+        const double density_inv = 1.0 / b.cv[p];
+        b.bv[p + s] = b.cv[p + s] * density_inv;
+        b.bv[p + 2 * s] = b.cv[p + 2 * s] * density_inv;
+        b.bv[p + 3 * s] = b.cv[p + 3 * s] * density_inv;
+        const double V2 = b.bv[p + s] * b.bv[p + s] + b.bv[p + 2 * s] * b.bv[p + 2 * s] +
+                          b.bv[p + 3 * s] * b.bv[p + 3 * s];
+        b.bv[p + 4 * s] = (kGammaAir - 1) * (b.cv[p + 4 * s] - 0.5 * b.bv[p] * V2);
+        b.bv[p + 5 * s] = b.bv[p + 4 * s] * kMwAir * density_inv / kRu;
+      }
+}
+
+// apply_outflow / apply_periodic, src/BoundCond.cuh:205-233, :1443-1486.
+// Ranges: the face plus ngg on the tangential directions (src/Mesh.cpp:14-32).
+void apply_face(Block& b, const Face& f) {
+  const int g = b.g;
+  const long long s = b.sz;
+  const int ext[3] = {b.mx, b.my, b.mz};
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = -g;
+    hi[a] = ext[a] - 1 + g;
+  }
+  lo[f.axis] = hi[f.axis] = (f.side < 0) ? 0 : ext[f.axis] - 1;
+  int dir[3] = {0, 0, 0};
+  dir[f.axis] = f.side;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = lo[2]; k <= hi[2]; ++k)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int i = lo[0]; i <= hi[0]; ++i) {
+        int other[3] = {i, j, k};
+        if (f.type == ORC_BC_PERIODIC) other[f.axis] = (f.side < 0) ? ext[f.axis] - 1 : 0;
+        for (int gg = 1; gg <= g; ++gg) {
+          const long long pg = b.id(i + gg * dir[0], j + gg * dir[1], k + gg * dir[2]);
+          long long ps;
+          if (f.type == ORC_BC_PERIODIC)
+            ps = b.id(other[0] + gg * dir[0], other[1] + gg * dir[1], other[2] + gg * dir[2]);
+          else
+            ps = b.id(i, j, k);
+          for (int l = 0; l < 6; ++l) b.bv[pg + l * s] = b.bv[ps + l * s];
+          cv_from_bv_point(b, pg);
+        }
+      }
+}
+
+void apply_bc(Block& b) {
+  // priority: farfield(4), outflow(6), ..., periodic(8) (src/BoundCond.cuh:1488-1678)
+  const int order[] = {ORC_BC_FARFIELD, ORC_BC_OUTFLOW, ORC_BC_PERIODIC};
+  for (int t : order)
+    for (const Face& f : b.faces)
+      if (f.type == t) apply_face(b, f);
+}
+
+// inner_communication, src/DataCommunication.cuh:60-105, for an aligned face.
+void communicate_face(Ctx& c, Block& b, const InnerFace& f) {
+  Block& t = c.blocks[f.target];
+  const int g = b.g;
+  const long long s = b.sz, ts = t.sz;
+  const int ext[3] = {b.mx, b.my, b.mz};
+  const int text[3] = {t.mx, t.my, t.mz};
+  // layers n = 0..g along the face normal; n indexes from range_start
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = 0;
+    hi[a] = ext[a] - 1;
+  }
+  const int a = f.axis;
+  if (f.side > 0) { lo[a] = ext[a] - 1; hi[a] = ext[a] - 1 + g; }
+  else { lo[a] = -g; hi[a] = 0; }
+  const int shared = (f.side > 0) ? lo[a] : hi[a];
+  // target index along the axis: side>0 -> target low face starting at 0;
+  // side<0 -> target high face starting at m_t-1-g
+  const int tstart = (f.side > 0) ? 0 : text[a] - 1 - g;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = lo[2]; k <= hi[2]; ++k)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int i = lo[0]; i <= hi[0]; ++i) {
+        int idx[3] = {i, j, k};
+        int tid[3] = {i, j, k};
+        tid[a] = tstart + (idx[a] - lo[a]);
+        const long long p = b.id(i, j, k), pt = t.id(tid[0], tid[1], tid[2]);
+        if (idx[a] == shared) {
+          for (int l = 0; l < 6; ++l) {
+            const double ave{0.5 * (t.bv[pt + l * ts] + b.bv[p + l * s])};
+            b.bv[p + l * s] = ave;
+            t.bv[pt + l * ts] = ave;
+          }
+        } else {
+          for (int l = 0; l < 6; ++l) b.bv[p + l * s] = t.bv[pt + l * ts];
+        }
+        cv_from_bv_point(b, p);
+      }
+}
+
+void communicate(Ctx& c) {
+  for (auto& b : c.blocks)
+    for (const auto& f : b.inner) communicate_face(c, b, f);
+}
+
+// global_time_step, src/TimeAdvanceFunc.cu:39-64 (min_of_arr starts from 1e6)
+double global_dt(Ctx& c) {
+  double dt{1e6};
+  for (auto& b : c.blocks) {
+    double m{1e6};
+    for (double v : b.dtl) m = std::min(m, v);
+    dt = std::min(dt, m);
+  }
+  return dt;
+}
+
+void alloc_block(Block& b, int nv) {
+  const int g = b.g;
+  b.sx = b.mx + 2 * g;
+  b.sy = b.my + 2 * g;
+  b.sz = b.sx * b.sy * (b.mz + 2 * g);
+  b.cv.assign(b.sz * nv, 0.0);
+  b.qn.assign(b.sz * nv, 0.0);
+  b.bv.assign(b.sz * 6, 0.0);
+  b.mul.assign(b.sz, 0.0);
+  b.mach.assign(b.sz, 0.0);
+  b.jac.assign(b.sz, 0.0);
+  b.metric.assign(b.sz * 9, 0.0);
+  b.dq.assign(b.n0() * nv, 0.0);
+  b.dtl.assign(b.n0(), 0.0);
+  b.bv_last.assign(b.n0() * 4, 0.0);
+  const long long gs = (long long)(b.mx + 2 * (g + 1)) * (b.my + 2 * (g + 1)) * (b.mz + 2 * (g + 1));
+  b.x.assign(gs, 0.0);
+  b.y.assign(gs, 0.0);
+  b.z.assign(gs, 0.0);
+}
+
+uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  uint64_t z = x;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+void* orc_create(const orc_setup* s) {
+  auto* c = new Ctx;
+  c->s = *s;
+  c->ngg = (s->inviscid_scheme == 71 || s->inviscid_scheme == 72) ? 4 : 3;  // src/Parameter.cpp:213-238
+  // src/DParameter.cu:26-29
+  c->weno_eps_scale = s->rho_ref * s->v_ref * s->rho_ref * s->v_ref;
+#ifdef _OPENMP
+  if (s->n_threads > 0) omp_set_num_threads(s->n_threads);
+#endif
+  return c;
+}
+
+void orc_destroy(void* ctx) { delete static_cast<Ctx*>(ctx); }
+
+int orc_ngg(void* ctx) { return static_cast<Ctx*>(ctx)->ngg; }
+
+int orc_add_block(void* ctx, int mx, int my, int mz) {
+  auto* c = static_cast<Ctx*>(ctx);
+  if (mx < 2 || my < 2 || mz < 2) return -1;
+  Block b;
+  b.mx = mx; b.my = my; b.mz = mz; b.g = c->ngg;
+  alloc_block(b, c->s.n_var);
+  c->blocks.push_back(std::move(b));
+  return (int)c->blocks.size() - 1;
+}
+
+int orc_add_boundary(void* ctx, int block, int axis, int side, int type) {
+  auto* c = static_cast<Ctx*>(ctx);
+  if (block < 0 || block >= (int)c->blocks.size()) return -1;
+  c->blocks[block].faces.push_back(Face{axis, side, type});
+  return 0;
+}
+
+int orc_add_inner_face(void* ctx, int block, int axis, int side, int target) {
+  auto* c = static_cast<Ctx*>(ctx);
+  if (block < 0 || block >= (int)c->blocks.size() || target < 0 || target >= (int)c->blocks.size()) return -1;
+  c->blocks[block].inner.push_back(InnerFace{axis, side, target});
+  return 0;
+}
+
+double* orc_array(void* ctx, int block, int which, long long* n_out) {
+  auto* c = static_cast<Ctx*>(ctx);
+  Block& b = c->blocks[block];
+  std::vector<double>* v = nullptr;
+  switch (which) {
+    case ORC_CV: v = &b.cv; break;
+    case ORC_BV: v = &b.bv; break;
+    case ORC_QN: v = &b.qn; break;
+    case ORC_DQ: v = &b.dq; break;
+    case ORC_MUL: v = &b.mul; break;
+    case ORC_MACH: v = &b.mach; break;
+    case ORC_JAC: v = &b.jac; break;
+    case ORC_METRIC: v = &b.metric; break;
+    case ORC_DT_LOCAL: v = &b.dtl; break;
+    case ORC_X: v = &b.x; break;
+    case ORC_Y: v = &b.y; break;
+    case ORC_Z: v = &b.z; break;
+    case ORC_BV_LAST: v = &b.bv_last; break;
+    default: return nullptr;
+  }
+  if (n_out) *n_out = (long long)v->size();
+  return v->data();
+}
+
+// Analytic uniform Cartesian grid over all ngg+1 ghost layers.  The reference
+// mirrors interior nodes into the ghost layers (src/Mesh.cpp:893-1031); on a
+// uniform grid that gives the same coordinates up to rounding.
+int orc_uniform_grid(void* ctx, int block, double x0, double y0, double z0, double dx, double dy, double dz) {
+  auto* c = static_cast<Ctx*>(ctx);
+  Block& b = c->blocks[block];
+  const int G = b.g + 1;
+  for (int k = -G; k < b.mz + G; ++k)
+    for (int j = -G; j < b.my + G; ++j)
+      for (int i = -G; i < b.mx + G; ++i) {
+        const long long p = b.idg(i, j, k);
+        b.x[p] = x0 + i * dx;
+        b.y[p] = y0 + j * dy;
+        b.z[p] = z0 + k * dz;
+      }
+  return 0;
+}
+
+// Block::compute_jac_metric, src/Mesh.cpp:241-300 (the interior formula; the
+// ghost-layer loops at :300-502 use the same expressions).
+int orc_compute_metric(void* ctx, int block) {
+  auto* c = static_cast<Ctx*>(ctx);
+  Block& b = c->blocks[block];
+  const int g = b.g;
+  const long long s = b.sz;
+  for (int k = -g; k < b.mz + g; ++k)
+    for (int j = -g; j < b.my + g; ++j)
+      for (int i = -g; i < b.mx + g; ++i) {
+        auto X = [&](int ii, int jj, int kk) { return b.x[b.idg(ii, jj, kk)]; };
+        auto Y = [&](int ii, int jj, int kk) { return b.y[b.idg(ii, jj, kk)]; };
+        auto Z = [&](int ii, int jj, int kk) { return b.z[b.idg(ii, jj, kk)]; };
+        const double dxd1 = (X(i + 1, j, k) - X(i - 1, j, k)) * 0.5, dyd1 = (Y(i + 1, j, k) - Y(i - 1, j, k)) * 0.5,
+                     dzd1 = (Z(i + 1, j, k) - Z(i - 1, j, k)) * 0.5;
+        const double dxd2 = (X(i, j + 1, k) - X(i, j - 1, k)) * 0.5, dyd2 = (Y(i, j + 1, k) - Y(i, j - 1, k)) * 0.5,
+                     dzd2 = (Z(i, j + 1, k) - Z(i, j - 1, k)) * 0.5;
+        const double dxd3 = (X(i, j, k + 1) - X(i, j, k - 1)) * 0.5, dyd3 = (Y(i, j, k + 1) - Y(i, j, k - 1)) * 0.5,
+                     dzd3 = (Z(i, j, k + 1) - Z(i, j, k - 1)) * 0.5;
+        const double jac = dxd1 * dyd2 * dzd3 + dxd2 * dyd3 * dzd1 + dxd3 * dyd1 * dzd2 - dxd1 * dyd3 * dzd2 -
+                           dxd2 * dyd1 * dzd3 - dxd3 * dyd2 * dzd1;
+        if (jac <= 0) return -2;
+        const long long p = b.id(i, j, k);
+        b.jac[p] = jac;
+        double* M = b.metric.data();
+        M[p + 0 * s] = (dyd2 * dzd3 - dyd3 * dzd2) / jac;
+        M[p + 1 * s] = (dxd3 * dzd2 - dxd2 * dzd3) / jac;
+        M[p + 2 * s] = (dxd2 * dyd3 - dxd3 * dyd2) / jac;
+        M[p + 3 * s] = (dyd3 * dzd1 - dyd1 * dzd3) / jac;
+        M[p + 4 * s] = (dxd1 * dzd3 - dxd3 * dzd1) / jac;
+        M[p + 5 * s] = (dxd3 * dyd1 - dxd1 * dyd3) / jac;
+        M[p + 6 * s] = (dyd1 * dzd2 - dyd2 * dzd1) / jac;
+        M[p + 7 * s] = (dxd2 * dzd1 - dxd1 * dzd2) / jac;
+        M[p + 8 * s] = (dxd1 * dyd2 - dxd2 * dyd1) / jac;
+      }
+  return 0;
+}
+
+double orc_sutherland(double t) { return sutherland(t); }
+
+// get_mixing_layer_info for air (n_spec = 0), src/MixingLayer.cu:9-190.
+// Only the branches the configurations use: velocity_ratio <= 0 with the upper
+// Mach number given (:104-107 for upper_faster, :108-110 otherwise).
+void orc_mixing_layer_streams(double ma_c, double ma_upper, double t_upper, double t_lower, double p_upper,
+                              double p_lower, int upper_faster, orc_ml_streams* out) {
+  const double c_upper = std::sqrt(kGammaAir * kRu / kMwAir * t_upper);
+  const double c_lower = std::sqrt(kGammaAir * kRu / kMwAir * t_lower);
+  const double rho_upper = p_upper * kMwAir / (kRu * t_upper);
+  const double rho_lower = p_lower * kMwAir / (kRu * t_lower);
+  double ma_lower;
+  if (upper_faster) ma_lower = (ma_upper - ma_c) * c_upper / c_lower - ma_c;
+  else ma_lower = (ma_c + ma_upper) * c_upper / c_lower + ma_c;
+  const double u_upper{ma_upper * c_upper}, u_lower{ma_lower * c_lower};
+  out->upper[0] = rho_upper; out->upper[1] = u_upper; out->upper[2] = 0; out->upper[3] = 0;
+  out->upper[4] = p_upper; out->upper[5] = t_upper;
+  out->lower[0] = rho_lower; out->lower[1] = u_lower; out->lower[2] = 0; out->lower[3] = 0;
+  out->lower[4] = p_lower; out->lower[5] = t_lower;
+}
+
+// initialize_mixing_layer_with_info, src/Initialize.cu:259-325 (air), on the
+// padded domain, plus a seeded velocity perturbation that the reference does
+// not have: amplitude * U(-1,1) from SplitMix64 of (seed, global node, comp).
+// Global node indices wrap with the given periods (<=0: no wrap).
+int orc_mixing_layer_ic(void* ctx, int block, const orc_ml_streams* st, double delta_omega, int gi0, int gj0, int gk0,
+                        int period_i, int period_j, int period_k, uint64_t seed, double amplitude) {
+  auto* c = static_cast<Ctx*>(ctx);
+  Block& b = c->blocks[block];
+  const int g = b.g;
+  const long long s = b.sz;
+  auto wrap = [](long long v, int per) -> long long {
+    if (per <= 0) return v;
+    long long r = v % per;
+    return r < 0 ? r + per : r;
+  };
+  for (int k = -g; k < b.mz + g; ++k)
+    for (int j = -g; j < b.my + g; ++j)
+      for (int i = -g; i < b.mx + g; ++i) {
+        const long long p = b.id(i, j, k);
+        const double y = b.y[b.idg(i, j, k)];
+        const double u_upper = st->upper[1], u_lower = st->lower[1];
+        b.bv[p + s] = 0.5 * (u_upper + u_lower) + 0.5 * (u_upper - u_lower) * std::tanh(2 * y / delta_omega);
+        const double* var = (y >= 0) ? st->upper : st->lower;
+        b.bv[p] = var[0];
+        b.bv[p + 2 * s] = var[2];
+        b.bv[p + 3 * s] = var[3];
+        b.bv[p + 4 * s] = var[4];
+        b.bv[p + 5 * s] = var[5];
+        if (amplitude != 0) {
+          const long long gi = wrap((long long)gi0 + i, period_i), gj = wrap((long long)gj0 + j, period_j),
+                          gk = wrap((long long)gk0 + k, period_k);
+          const uint64_t node = (uint64_t)((gk * 1000003LL + gj) * 1000033LL + gi);
+          for (int comp = 0; comp < 3; ++comp) {
+            const uint64_t h = splitmix64(seed ^ splitmix64(node * 4 + (uint64_t)comp));
+            const double r = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+            b.bv[p + (1 + comp) * s] += amplitude * (2.0 * r - 1.0);
+          }
+        }
+      }
+  return 0;
+}
+
+int orc_cv_from_bv(void* ctx, int block) { cv_from_bv(static_cast<Ctx*>(ctx)->blocks[block]); return 0; }
+
+int orc_copy_qn(void* ctx, int block) {
+  Block& b = static_cast<Ctx*>(ctx)->blocks[block];
+  b.qn = b.cv;
+  return 0;
+}
+
+int orc_zero_dq(void* ctx, int block) {
+  Block& b = static_cast<Ctx*>(ctx)->blocks[block];
+  std::fill(b.dq.begin(), b.dq.end(), 0.0);
+  return 0;
+}
+
+int orc_inviscid(void* ctx, int block) {
+  auto* c = static_cast<Ctx*>(ctx);
+  inviscid(*c, c->blocks[block]);
+  return 0;
+}
+
+int orc_viscous(void* ctx, int block) {
+  auto* c = static_cast<Ctx*>(ctx);
+  viscous(*c, c->blocks[block]);
+  return 0;
+}
+
+int orc_local_dt(void* ctx, int block) {
+  auto* c = static_cast<Ctx*>(ctx);
+  local_dt(*c, c->blocks[block]);
+  return 0;
+}
+
+double orc_global_dt(void* ctx) { return global_dt(*static_cast<Ctx*>(ctx)); }
+
+int orc_update(void* ctx, int block, double dt, int rk) {
+  auto* c = static_cast<Ctx*>(ctx);
+  update(*c, c->blocks[block], dt, rk);
+  return 0;
+}
+
+int orc_apply_bc(void* ctx, int block) {
+  apply_bc(static_cast<Ctx*>(ctx)->blocks[block]);
+  return 0;
+}
+
+int orc_communicate(void* ctx) {
+  communicate(*static_cast<Ctx*>(ctx));
+  return 0;
+}
+
+int orc_properties(void* ctx, int block) {
+  properties(static_cast<Ctx*>(ctx)->blocks[block]);
+  return 0;
+}
+
+// Driver::initialize_computation (src/Driver.cu:115-199) then the RK3
+// prologue compute_cv_from_bv (src/RK.cuh:38-49).
+int orc_initialize(void* ctx) {
+  auto* c = static_cast<Ctx*>(ctx);
+  for (auto& b : c->blocks) apply_bc(b);
+  communicate(*c);
+  for (auto& b : c->blocks) properties(b);
+  for (auto& b : c->blocks) cv_from_bv(b);
+  return 0;
+}
+
+// One SSP-RK3 step, src/RK.cuh:85-226 (no source term for air/laminar).
+int orc_step(void* ctx, int n_steps, double* last_dt) {
+  auto* c = static_cast<Ctx*>(ctx);
+  double dt = c->s.fixed_time_step ? c->s.dt : 1e6;
+  for (int st = 0; st < n_steps; ++st) {
+    for (auto& b : c->blocks) b.qn = b.cv;
+    for (int rk = 0; rk < 3; ++rk) {
+      for (auto& b : c->blocks) std::fill(b.dq.begin(), b.dq.end(), 0.0);
+      if (!c->s.fixed_time_step && rk == 0) {
+        for (auto& b : c->blocks) local_dt(*c, b);
+        dt = global_dt(*c);
+      }
+      for (auto& b : c->blocks) {
+        inviscid(*c, b);
+        viscous(*c, b);
+        update(*c, b, dt, rk);
+        apply_bc(b);
+      }
+      communicate(*c);
+      for (auto& b : c->blocks) properties(b);
+    }
+  }
+  c->dt_cur = dt;
+  if (last_dt) *last_dt = dt;
+  return 0;
+}
+
+// store_last_step, src/TimeAdvanceFunc.cu:6-19
+int orc_store_last_step(void* ctx) {
+  auto* c = static_cast<Ctx*>(ctx);
+  for (auto& b : c->blocks) {
+    const long long s = b.sz, n0 = b.n0();
+    for (int k = 0; k < b.mz; ++k)
+      for (int j = 0; j < b.my; ++j)
+        for (int i = 0; i < b.mx; ++i) {
+          const long long p = b.id(i, j, k), q = b.id0(i, j, k);
+          const double u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+          b.bv_last[q] = b.bv[p];
+          b.bv_last[q + n0] = std::sqrt(u * u + v * v + w * w);
+          b.bv_last[q + 2 * n0] = b.bv[p + 4 * s];
+          b.bv_last[q + 3 * n0] = b.bv[p + 5 * s];
+        }
+  }
+  return 0;
+}
+
+// compute_square_of_dbv + reduction + sqrt(sum/N), src/TimeAdvanceFunc.cu:21-37,
+// src/Residual.cuh:75-122 (unscaled: res_scale division is left to the caller).
+int orc_residual(void* ctx, double res[4]) {
+  auto* c = static_cast<Ctx*>(ctx);
+  double acc[4] = {0, 0, 0, 0};
+  long long n_total = 0;
+  for (auto& b : c->blocks) {
+    const long long s = b.sz, n0 = b.n0();
+    n_total += n0;
+    for (int k = 0; k < b.mz; ++k)
+      for (int j = 0; j < b.my; ++j)
+        for (int i = 0; i < b.mx; ++i) {
+          const long long p = b.id(i, j, k), q = b.id0(i, j, k);
+          const double u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+          const double vel = std::sqrt(u * u + v * v + w * w);
+          acc[0] += (b.bv[p] - b.bv_last[q]) * (b.bv[p] - b.bv_last[q]);
+          acc[1] += (vel - b.bv_last[q + n0]) * (vel - b.bv_last[q + n0]);
+          acc[2] += (b.bv[p + 4 * s] - b.bv_last[q + 2 * n0]) * (b.bv[p + 4 * s] - b.bv_last[q + 2 * n0]);
+          acc[3] += (b.bv[p + 5 * s] - b.bv_last[q + 3 * n0]) * (b.bv[p + 5 * s] - b.bv_last[q + 3 * n0]);
+        }
+  }
+  for (int l = 0; l < 4; ++l) res[l] = std::sqrt(acc[l] / (double)n_total);
+  return 0;
+}
+
+double orc_weno5z(const double* vp, const double* vm, double eps) { return weno5z(vp, vm, eps); }
+double orc_weno7z(const double* vp, const double* vm, double eps) { return weno7z(vp, vm, eps); }
+double orc_weno5js(const double* vp, const double* vm, double eps) { return weno5js(vp, vm, eps); }
+void orc_lf_split(const double* q, int n_var, const double* k3, double jac, double* fp, double* fm) {
+  lf_split(q, n_var, k3, jac, fp, fm);
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+/*
+ * ostc_oracle.h -- C interface of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a host-C++ (OpenMP over cells)
+ * restatement of OpenSTC's per-stage RHS + SSP-RK3 path for calorically
+ * perfect air.  It exists to check the HIP library; nothing in the product
+ * links, loads or calls it.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may use it.
+ *
+ * Arrays follow the reference's ColMajor SoA ghost-padded layout
+ * (src/gxl_lib/Array.cuh:171-177, allocation :209-230):
+ *   idx(i,j,k,l) = (k+g)*(mx+2g)*(my+2g) + (j+g)*(mx+2g) + (i+g) + l*sz,
+ *   sz = (mx+2g)(my+2g)(mz+2g),  g = ngg for cv/bv/qn/mul/mach/jac/metric,
+ *   g = 0 for dq and dt_local (src/Field.cu:505-550),
+ *   g = ngg+1 for the grid coordinates x/y/z (src/Mesh.cpp:687).
+ * The metric is stored as 9 planes, plane 3*(r-1)+(c-1) holding the
+ * reference's 1-based m(r,c) (src/gxl_lib/Matrix.hpp:53-72; rows xi, eta,
+ * zeta, columns x, y, z).
+ */
+#ifndef OSTC_ORACLE_H
+#define OSTC_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int    n_var;            /* 5 for air */
+  int    inviscid_scheme;  /* 51 WENO-Z5 cp, 71 WENO-Z7 cp, 50 WENO5-JS cp (not in the reference) */
+  int    viscous_order;    /* 0 inviscid, 2 second-order central */
+  int    fixed_time_step;  /* 1: use dt below; 0: CFL-based global dt */
+  double dt;
+  double cfl;
+  double prandtl;          /* Pr */
+  double rho_ref;          /* rho_inf of the reference state */
+  double v_ref;            /* v_inf of the reference state */
+  int    n_threads;        /* OpenMP threads, <=0: library default */
+} orc_setup;
+
+/* Mixing-layer stream states, as produced by get_mixing_layer_info for air
+ * (src/MixingLayer.cu:9-190): [rho,u,v,w,p,T] of the upper and lower streams. */
+typedef struct {
+  double upper[6];
+  double lower[6];
+} orc_ml_streams;
+
+enum { ORC_CV = 0, ORC_BV, ORC_QN, ORC_DQ, ORC_MUL, ORC_MACH, ORC_JAC, ORC_METRIC,
+       ORC_DT_LOCAL, ORC_X, ORC_Y, ORC_Z, ORC_BV_LAST };
+
+enum { ORC_BC_FARFIELD = 4, ORC_BC_OUTFLOW = 6, ORC_BC_PERIODIC = 8 };
+
+void*   orc_create(const orc_setup* s);
+void    orc_destroy(void* ctx);
+int     orc_add_block(void* ctx, int mx, int my, int mz);
+int     orc_ngg(void* ctx);
+/* axis 0/1/2, side -1 (index 0) or +1 (index m-1); type = ORC_BC_*.  Faces
+ * are applied in the reference's priority order by type, then in the order
+ * they were added (src/BoundCond.cuh:1488-1678). */
+int     orc_add_boundary(void* ctx, int block, int axis, int side, int type);
+/* 1-to-1 conforming face between block (axis, side) and target block on the
+ * opposite side of the same axis, same index orientation, full face.  Each
+ * side of an interface is registered separately, as in the reference's
+ * inner N.txt files (src/Mesh.cpp:38-176). */
+int     orc_add_inner_face(void* ctx, int block, int axis, int side, int target_block);
+double* orc_array(void* ctx, int block, int which, long long* n_out);
+
+/* setup */
+int     orc_uniform_grid(void* ctx, int block, double x0, double y0, double z0,
+                         double dx, double dy, double dz);
+int     orc_compute_metric(void* ctx, int block);
+void    orc_mixing_layer_streams(double ma_c, double ma_upper, double t_upper, double t_lower,
+                                 double p_upper, double p_lower, int upper_faster,
+                                 orc_ml_streams* out);
+double  orc_sutherland(double t);
+int     orc_mixing_layer_ic(void* ctx, int block, const orc_ml_streams* st, double delta_omega,
+                            int gi0, int gj0, int gk0, int period_i, int period_j, int period_k,
+                            uint64_t seed, double amplitude);
+
+/* RK3 prologue/epilogue and whole steps (src/Driver.cu:115-199, src/RK.cuh:20-228) */
+int     orc_initialize(void* ctx);
+int     orc_step(void* ctx, int n_steps, double* last_dt);
+
+/* stage-level operators */
+int     orc_cv_from_bv(void* ctx, int block);
+int     orc_copy_qn(void* ctx, int block);
+int     orc_zero_dq(void* ctx, int block);
+int     orc_inviscid(void* ctx, int block);
+int     orc_viscous(void* ctx, int block);
+int     orc_local_dt(void* ctx, int block);
+double  orc_global_dt(void* ctx);
+int     orc_update(void* ctx, int block, double dt, int rk);
+int     orc_apply_bc(void* ctx, int block);
+int     orc_communicate(void* ctx);
+int     orc_properties(void* ctx, int block);
+int     orc_store_last_step(void* ctx);
+int     orc_residual(void* ctx, double res[4]);
+
+/* point kernels, exposed for closed-form tests */
+double  orc_weno5z(const double* vp, const double* vm, double eps);
+double  orc_weno7z(const double* vp, const double* vm, double eps);
+double  orc_weno5js(const double* vp, const double* vm, double eps);
+void    orc_lf_split(const double* q /* n_var cv, p, c */, int n_var, const double* k3, double jac,
+                     double* fp, double* fm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
