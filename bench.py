@@ -1,0 +1,151 @@
+"""Benchmark: FP64 RHS + SSP-RK3 Mcell-updates/s on one MI355X.
+
+Workload: the batched multi-block air mixing layer -- 24 blocks of uneven
+sizes (4 x 3 x 2 tensor-product cuts drawn from seed 42), about 12 M cells,
+WENO-Z5 component-wise + Lax-Friedrichs splitting, second-order viscous
+fluxes, SSP-RK3 at CFL 0.5, in-GPU inner-face exchange.  The BATCHED path
+(fused RHS+update kernel over the block work-list, graph-captured steps) is
+the measured value; the PLAIN path (per-block launches shaped like the
+reference) is timed on the same state beside it.
+
+One update = one full 3-stage SSP-RK3 step of one interior cell.  Inputs are
+resident in HBM before the timed region; the timed region is exactly
+--steps steps bracketed by device synchronisation.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+pkg = importlib.import_module("openstc-mixinglayer_amd")
+
+METRIC = "FP64 RHS+RK3 Mcell-updates/s vs plain launch; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# Contract-minimal HBM bytes per interior cell per RK stage for air:
+# read cv[5] + qn[5] + metric[9] + jac, write cv[5] + bv[6] + mu + Mach.
+BYTES_PER_CELL_STAGE = 8 * (5 + 5 + 10) + 8 * (5 + 6 + 2)
+
+
+def cpu_baseline(case, steps: int):
+    """The CPU oracle (host C++, OpenMP over cells) on a bounded sample of the
+    same workload: `steps` RK3 steps of the full 24-block case."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    threads = min(os.cpu_count() or 1, 16)
+    o = O.Oracle(case, n_threads=threads)
+    o.setup_mixing_layer()
+    o.initialize()
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    cells = case["n_cells"]
+    return {"value": cells * steps / dt / 1e6, "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} RK3 steps of the full {len(case['blocks'])}-block case ({cells} cells), "
+                      f"{dt:.1f} s on {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
+              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        sys.exit(2)
+
+    case = pkg.cases.batched_case(seed=42)
+    cells = case["n_cells"]
+    print(f"[bench] {len(case['blocks'])} blocks, {cells} interior cells", file=sys.stderr, flush=True)
+    s = pkg.Solver(case)
+    s.setup_mixing_layer()
+    s.initialize()
+    tune = None
+    if not a.no_tune:
+        tune = s.autotune(3)
+        print(f"[bench] autotune: tile entry {tune[0]} ({pkg.tile_shapes()[tune[0]]}), {tune[1]:.3f} ms/stage",
+              file=sys.stderr, flush=True)
+
+    # BATCHED: warm-up, then exactly K steps.  The timed region launches the
+    # batched kernels directly with HIP events around every fused RHS+update
+    # launch (the kernel durations behind the roofline figure); the same
+    # steps replayed as a captured graph are timed afterwards as value_graph.
+    s.rk3(a.warmup, pkg.MODE_BATCHED)
+    s.sync()
+    t0 = time.perf_counter()
+    s.rk3(a.steps, pkg.MODE_BATCHED_TIMED, want_dt=False)
+    s.sync()
+    wall = time.perf_counter() - t0
+    dev_ms, launches = s.last_step_timing()
+    avg_kernel_ms, max_kernel_ms, n_kernel = s.fused_kernel_ms()
+    value = cells * a.steps / wall / 1e6
+    print(f"[bench] batched: {wall * 1e3 / a.steps:.3f} ms/step wall, {dev_ms / a.steps:.3f} ms/step device, "
+          f"fused kernel avg {avg_kernel_ms:.3f} ms max {max_kernel_ms:.3f} ms over {n_kernel} launches, "
+          f"{launches // max(a.steps, 1)} launches/step", file=sys.stderr, flush=True)
+    s.sync()
+    t0 = time.perf_counter()
+    s.rk3(a.steps, pkg.MODE_BATCHED, want_dt=False)
+    s.sync()
+    wall_g = time.perf_counter() - t0
+    value_graph = cells * a.steps / wall_g / 1e6
+    print(f"[bench] batched (graph replay): {wall_g * 1e3 / a.steps:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # PLAIN (reference-shaped per-block launches, host dt read-back each step)
+    s.rk3(2, pkg.MODE_PLAIN)
+    s.sync()
+    t0 = time.perf_counter()
+    s.rk3(a.plain_steps, pkg.MODE_PLAIN, want_dt=False)
+    s.sync()
+    wall_p = time.perf_counter() - t0
+    _, launches_p = s.last_step_timing()
+    plain_value = cells * a.plain_steps / wall_p / 1e6
+    print(f"[bench] plain: {wall_p * 1e3 / a.plain_steps:.3f} ms/step, "
+          f"{launches_p // max(a.plain_steps, 1)} launches/step", file=sys.stderr, flush=True)
+
+    avg_kernel_s = avg_kernel_ms / 1e3
+    bytes_per_launch = BYTES_PER_CELL_STAGE * cells
+    achieved = bytes_per_launch / avg_kernel_s / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_fused (RHS+update, one RK stage over all blocks)",
+                "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_kernel_s * 1e3}
+    s.close()
+
+    cpu = None
+    if not a.no_cpu:
+        print(f"[bench] cpu baseline: {a.cpu_steps} oracle steps", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(case, a.cpu_steps)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "Mcell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batched multi-block air temporal mixing layer, 24 uneven blocks (seed 42), "
+                               "WENO-Z5 cp + LF, 2nd-order viscous, SSP-RK3 CFL 0.5",
+                   "cells": cells, "blocks": len(case["blocks"]), "mode": "BATCHED",
+                   "tile": list(pkg.tile_shapes()[tune[0]]) if tune else None},
+        "value_graph": value_graph, "plain_value": plain_value, "speedup_vs_plain": value / plain_value,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -43,6 +43,8 @@ extern "C" {
 /* time-step modes of ostc_step */
 #define OSTC_MODE_PLAIN 0    /* per-block launches shaped like the reference */
 #define OSTC_MODE_BATCHED 1  /* fused RHS+update over a block work-list, graph-captured */
+#define OSTC_MODE_BATCHED_TIMED 2  /* the batched kernels launched directly, with HIP events
+                                      around every fused launch (ostc_fused_kernel_ms) */
 
 /* fields for ostc_get_field / ostc_set_field */
 #define OSTC_FIELD_CV 0
@@ -154,9 +156,10 @@ int ostc_autotune(ostc_ctx ctx, int reps, int* best_index, double* best_ms);
 /* Device timing of the last ostc_step call: total milliseconds measured
  * with HIP events on the context's stream, and the launch count. */
 int ostc_last_step_timing(ostc_ctx ctx, double* ms, long long* launches);
-/* Durations (ms) of the three fused RHS+update kernels of the last batched
- * step, from HIP events recorded around them inside the step graph. */
-int ostc_fused_kernel_ms(ostc_ctx ctx, double ms[3]);
+/* Average and maximum duration (ms) of the fused RHS+update kernel over the
+ * launches of the last OSTC_MODE_BATCHED_TIMED ostc_step call, from HIP
+ * events recorded around each launch on the context's stream. */
+int ostc_fused_kernel_ms(ostc_ctx ctx, double* avg_ms, double* max_ms, long long* count);
 int ostc_sync(ostc_ctx ctx);
 const char* ostc_last_error(ostc_ctx ctx);
 

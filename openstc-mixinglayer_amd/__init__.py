@@ -31,7 +31,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libostc_hip.so")
 
 cases = importlib.import_module(__name__ + ".cases")
 
-MODE_PLAIN, MODE_BATCHED = 0, 1
+MODE_PLAIN, MODE_BATCHED, MODE_BATCHED_TIMED = 0, 1, 2
 FIELD_CV, FIELD_BV, FIELD_MUL, FIELD_MACH, FIELD_DQ, FIELD_DT_LOCAL, FIELD_JAC, FIELD_METRIC, FIELD_QN = range(9)
 
 
@@ -90,7 +90,7 @@ SIGNATURES = {
     "ostc_set_tile": (_I, [_P, _I, _I]),
     "ostc_autotune": (_I, [_P, _I, C.POINTER(_I), _DP]),
     "ostc_last_step_timing": (_I, [_P, _DP, C.POINTER(_LL)]),
-    "ostc_fused_kernel_ms": (_I, [_P, _DP]),
+    "ostc_fused_kernel_ms": (_I, [_P, _DP, _DP, C.POINTER(_LL)]),
     "ostc_sync": (_I, [_P]),
     "ostc_last_error": (C.c_char_p, [_P]),
     "ostc_util_uniform_geometry": (_I, [_I, _I, _I, _I, _D, _D, _D, _D, _D, _D, _DP, _DP]),
@@ -274,9 +274,10 @@ class Solver:
         return ms.value, n.value
 
     def fused_kernel_ms(self):
-        a = np.zeros(3)
-        self._chk(lib().ostc_fused_kernel_ms(self.ctx, _dp(a)))
-        return a
+        """(average ms, max ms, launches) of the fused kernel in the last timed step call."""
+        avg, mx, n = C.c_double(), C.c_double(), C.c_longlong()
+        self._chk(lib().ostc_fused_kernel_ms(self.ctx, C.byref(avg), C.byref(mx), C.byref(n)))
+        return avg.value, mx.value, n.value
 
     def sync(self):
         self._chk(lib().ostc_sync(self.ctx))

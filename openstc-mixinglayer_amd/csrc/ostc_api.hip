@@ -91,8 +91,8 @@ struct ostc_ctx_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool graph_valid = false;
-  bool graph_events = false;
-  hipEvent_t ev_f[6] = {};
+  std::vector<hipEvent_t> evpool;  // pairs around fused launches (timed mode)
+  long long timed_launches = 0;
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   long long graph_launches = 0;
   double last_ms = 0;
@@ -417,7 +417,10 @@ int plain_step(ostc_ctx c, double* dt_io, long long* n) {
 }
 
 // ------------------------------------------------------- batched step
-int batched_record(ostc_ctx c, long long* n, bool events) {
+// One batched step.  ev_base >= 0: record event pairs evpool[ev_base + 2 rk],
+// evpool[ev_base + 2 rk + 1] around the fused kernel of stage rk.
+int batched_record(ostc_ctx c, long long* n, int ev_base) {
+  const bool events = ev_base >= 0;
   if (!c->s.fixed_time_step) {
     FusedArgs a = fused_args(c, 0, 1, 0);
     const TileShape t = fused_shape(c->P.scheme, c->tile_index);
@@ -426,11 +429,11 @@ int batched_record(ostc_ctx c, long long* n, bool events) {
   for (int rk = 0; rk < 3; ++rk) {
     const int in = rk, out = (rk + 1) % kRing;
     FusedArgs a = fused_args(c, in, out, rk);
-    if (events) HIPCHK(c, hipEventRecord(c->ev_f[2 * rk], c->st));
+    if (events) HIPCHK(c, hipEventRecord(c->evpool[ev_base + 2 * rk], c->st));
     if (fused_launch(c->tile_index, 0, a, c->n_wg, c->st) < 0)
       return fail(c, OSTC_ERR_UNSUPPORTED, "fused kernel: scheme/tile not available");
     ++*n;
-    if (events) HIPCHK(c, hipEventRecord(c->ev_f[2 * rk + 1], c->st));
+    if (events) HIPCHK(c, hipEventRecord(c->evpool[ev_base + 2 * rk + 1], c->st));
     launch_group(c, c->rim, out, in, n);
     for (auto& g : c->bc_rounds) launch_group(c, g, out, out, n);
     for (auto& g : c->comm_levels) launch_group(c, g, out, out, n);
@@ -442,30 +445,25 @@ int batched_record(ostc_ctx c, long long* n, bool events) {
 int build_graph(ostc_ctx c) {
   invalidate_graph(c);
   long long n = 0;
-  for (int attempt = 0; attempt < 2; ++attempt) {
-    const bool ev = attempt == 0;
-    n = 0;
-    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-    int r = batched_record(c, &n, ev);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(c->st, &g);
-    if (r == OSTC_OK && e == hipSuccess) {
-      hipGraphExec_t ex = nullptr;
-      e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-      if (e == hipSuccess) {
-        c->graph = g;
-        c->gexec = ex;
-        c->graph_valid = true;
-        c->graph_events = ev;
-        c->graph_launches = n;
-        return OSTC_OK;
-      }
+  HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  int r = batched_record(c, &n, -1);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->st, &g);
+  if (r == OSTC_OK && e == hipSuccess) {
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e == hipSuccess) {
+      c->graph = g;
+      c->gexec = ex;
+      c->graph_valid = true;
+      c->graph_launches = n;
+      return OSTC_OK;
     }
-    if (g) hipGraphDestroy(g);
-    (void)hipGetLastError();
-    if (!ev) return fail(c, OSTC_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(e));
   }
-  return fail(c, OSTC_ERR_HIP, "graph capture failed");
+  if (g) hipGraphDestroy(g);
+  (void)hipGetLastError();
+  if (r) return r;
+  return fail(c, OSTC_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(e));
 }
 
 int check_ready(ostc_ctx c) {
@@ -523,8 +521,7 @@ int ostc_destroy(ostc_ctx c) {
   if (!c) return OSTC_ERR_ARG;
   if (c->st) hipStreamSynchronize(c->st);
   invalidate_graph(c);
-  for (auto& e : c->ev_f)
-    if (e) hipEventDestroy(e);
+  for (auto& e : c->evpool) hipEventDestroy(e);
   if (c->ev_a) hipEventDestroy(c->ev_a);
   if (c->ev_b) hipEventDestroy(c->ev_b);
   for (void* p : c->allocs) hipFree(p);
@@ -611,7 +608,6 @@ int ostc_finalize(ostc_ctx c) {
   if (!r) r = dalloc(c, &c->d_dt, 1);
   if (!r) r = build_ops(c);
   if (r) return r;
-  for (auto& e : c->ev_f) HIPCHK(c, hipEventCreate(&e));
   HIPCHK(c, hipEventCreate(&c->ev_a));
   HIPCHK(c, hipEventCreate(&c->ev_b));
   if (c->s.fixed_time_step) HIPCHK(c, hipMemcpy(c->d_dt, &c->s.dt, sizeof(double), hipMemcpyHostToDevice));
@@ -741,6 +737,19 @@ int ostc_step(ostc_ctx c, int n_steps, int mode, double* last_dt) {
     n = c->graph_launches * n_steps;
     if (last_dt && !c->s.fixed_time_step)
       HIPCHK(c, hipMemcpyAsync(&dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  } else if (mode == OSTC_MODE_BATCHED_TIMED) {
+    while ((long long)c->evpool.size() < 6ll * n_steps) {
+      hipEvent_t e;
+      HIPCHK(c, hipEventCreate(&e));
+      c->evpool.push_back(e);
+    }
+    for (int s = 0; s < n_steps; ++s) {
+      r = batched_record(c, &n, 6 * s);
+      if (r) return r;
+    }
+    c->timed_launches = 3ll * n_steps;
+    if (last_dt && !c->s.fixed_time_step)
+      HIPCHK(c, hipMemcpyAsync(&dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->st));
   } else {
     return fail(c, OSTC_ERR_ARG, "unknown mode");
   }
@@ -765,16 +774,21 @@ int ostc_last_step_timing(ostc_ctx c, double* ms, long long* launches) {
   return OSTC_OK;
 }
 
-int ostc_fused_kernel_ms(ostc_ctx c, double ms[3]) {
+int ostc_fused_kernel_ms(ostc_ctx c, double* avg_ms, double* max_ms, long long* count) {
   int r = check_ready(c);
   if (r) return r;
-  if (!c->graph_valid || !c->graph_events) return fail(c, OSTC_ERR_STATE, "no timed batched step");
-  HIPCHK(c, hipEventSynchronize(c->ev_f[5]));
-  for (int q = 0; q < 3; ++q) {
+  if (c->timed_launches <= 0) return fail(c, OSTC_ERR_STATE, "no OSTC_MODE_BATCHED_TIMED step has run");
+  HIPCHK(c, hipEventSynchronize(c->evpool[2 * c->timed_launches - 1]));
+  double sum = 0, mx = 0;
+  for (long long q = 0; q < c->timed_launches; ++q) {
     float f = 0;
-    HIPCHK(c, hipEventElapsedTime(&f, c->ev_f[2 * q], c->ev_f[2 * q + 1]));
-    ms[q] = f;
+    HIPCHK(c, hipEventElapsedTime(&f, c->evpool[2 * q], c->evpool[2 * q + 1]));
+    sum += f;
+    mx = std::max(mx, (double)f);
   }
+  if (avg_ms) *avg_ms = sum / (double)c->timed_launches;
+  if (max_ms) *max_ms = mx;
+  if (count) *count = c->timed_launches;
   return OSTC_OK;
 }
 

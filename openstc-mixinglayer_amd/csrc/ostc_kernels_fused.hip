@@ -298,7 +298,9 @@ __global__ __launch_bounds__(TX * TY * TZ) void k_fused(FusedArgs A) {
   for (int tile = first; tile < last; tile += step) {
     const int b = A.tile_block[tile];
     if (A.target_block >= 0 && b != A.target_block) continue;
-    const DevBlock B = A.blocks[b];
+    // a reference into global memory: the ring slots are picked by runtime
+    // indices, which would push a private copy of the descriptor to scratch
+    const DevBlock& B = A.blocks[b];
     const int local = tile - B.tile_base;
     t.i0 = (local % B.ntx) * TX;
     t.j0 = ((local / B.ntx) % B.nty) * TY;
@@ -384,7 +386,7 @@ __global__ __launch_bounds__(TX * TY * TZ) void k_dt_tiles(FusedArgs A, unsigned
   const int ti = tid % TX, tj = (tid / TX) % TY, tk = tid / (TX * TY);
   double v = 1e6;
   for (int tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
-    const DevBlock B = A.blocks[A.tile_block[tile]];
+    const DevBlock& B = A.blocks[A.tile_block[tile]];
     const int local = tile - B.tile_base;
     const int i = (local % B.ntx) * TX + ti, j = ((local / B.ntx) % B.nty) * TY + tj,
               k = (local / (B.ntx * B.nty)) * TZ + tk;
