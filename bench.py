@@ -77,11 +77,12 @@ def main():
     s = pkg.Solver(case)
     s.setup_mixing_layer()
     s.initialize()
-    tune = None
     if not a.no_tune:
-        tune = s.autotune(3)
-        print(f"[bench] autotune: tile entry {tune[0]} ({pkg.tile_shapes()[tune[0]]}), {tune[1]:.3f} ms/stage",
-              file=sys.stderr, flush=True)
+        best = s.autotune(3)
+        print(f"[bench] autotune: best {best[1]:.3f} ms/stage", file=sys.stderr, flush=True)
+    tile_index, tile_waves, tile_split = s.tile_config()
+    print(f"[bench] fused kernel config: tile {pkg.tile_shapes()[tile_index]}, "
+          f"{tile_waves} workgroups/CU (0 = one per tile), split={tile_split}", file=sys.stderr, flush=True)
 
     # BATCHED: warm-up, then exactly K steps.  The timed region launches the
     # batched kernels directly with HIP events around every fused RHS+update
@@ -140,7 +141,8 @@ def main():
         "config": {"workload": "batched multi-block air temporal mixing layer, 24 uneven blocks (seed 42), "
                                "WENO-Z5 cp + LF, 2nd-order viscous, SSP-RK3 CFL 0.5",
                    "cells": cells, "blocks": len(case["blocks"]), "mode": "BATCHED",
-                   "tile": list(pkg.tile_shapes()[tune[0]]) if tune else None},
+                   "tile": list(pkg.tile_shapes()[tile_index]), "workgroups_per_cu": tile_waves,
+                   "split_stage": bool(tile_split)},
         "value_graph": value_graph, "plain_value": plain_value, "speedup_vs_plain": value / plain_value,
         "roofline": roofline, "cpu_baseline": cpu,
     }

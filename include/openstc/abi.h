@@ -145,12 +145,15 @@ int ostc_batched_rhs(ostc_ctx ctx, int block);
 int ostc_store_last_step(ostc_ctx ctx);
 int ostc_residual(ostc_ctx ctx, double res[4]);
 
-/* Tile shape of the fused kernel: tile index into the compiled table and
- * resident workgroups per CU; ostc_autotune times every table entry on the
+/* Launch configuration of the fused kernel: tile index into the compiled
+ * table, workgroups per CU of a persistent grid (0: one workgroup per tile),
+ * and split = 1 to run each stage as an inviscid kernel followed by a
+ * viscous + update kernel.  ostc_autotune times every combination on the
  * current blocks and keeps the fastest. */
 int ostc_tile_count(int* n);
 int ostc_tile_shape(int index, int* tx, int* ty, int* tz);
-int ostc_set_tile(ostc_ctx ctx, int index, int waves_per_cu);
+int ostc_set_tile(ostc_ctx ctx, int index, int waves_per_cu, int split);
+int ostc_tile_config(ostc_ctx ctx, int* index, int* waves_per_cu, int* split);
 int ostc_autotune(ostc_ctx ctx, int reps, int* best_index, double* best_ms);
 
 /* Device timing of the last ostc_step call: total milliseconds measured

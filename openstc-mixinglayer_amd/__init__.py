@@ -87,7 +87,8 @@ SIGNATURES = {
     "ostc_residual": (_I, [_P, _DP]),
     "ostc_tile_count": (_I, [C.POINTER(_I)]),
     "ostc_tile_shape": (_I, [_I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
-    "ostc_set_tile": (_I, [_P, _I, _I]),
+    "ostc_set_tile": (_I, [_P, _I, _I, _I]),
+    "ostc_tile_config": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "ostc_autotune": (_I, [_P, _I, C.POINTER(_I), _DP]),
     "ostc_last_step_timing": (_I, [_P, _DP, C.POINTER(_LL)]),
     "ostc_fused_kernel_ms": (_I, [_P, _DP, _DP, C.POINTER(_LL)]),
@@ -260,8 +261,13 @@ class Solver:
         return r
 
     # -- tuning and timing -------------------------------------------------
-    def set_tile(self, index: int, waves_per_cu: int = 0):
-        self._chk(lib().ostc_set_tile(self.ctx, index, waves_per_cu))
+    def set_tile(self, index: int, waves_per_cu: int = 0, split: int = 0):
+        self._chk(lib().ostc_set_tile(self.ctx, index, waves_per_cu, split))
+
+    def tile_config(self):
+        i, w, s = C.c_int(), C.c_int(), C.c_int()
+        self._chk(lib().ostc_tile_config(self.ctx, C.byref(i), C.byref(w), C.byref(s)))
+        return i.value, w.value, s.value
 
     def autotune(self, reps: int = 3):
         bi, bm = C.c_int(), C.c_double()

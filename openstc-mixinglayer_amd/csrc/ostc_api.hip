@@ -83,7 +83,7 @@ struct ostc_ctx_s {
   Group props, rim;
   std::vector<std::vector<int>> plain_bc;
   std::vector<int> plain_comm;
-  int tile_index = 0, n_wg = 0, xcd_remap = 0, n_tiles = 0;
+  int tile_index = 0, n_wg = 0, xcd_remap = 0, n_tiles = 0, waves = 0, split = 0;
   int* d_tile_block = nullptr;
   double* d_scratch = nullptr;
   int n_first = 1024;
@@ -430,9 +430,9 @@ int batched_record(ostc_ctx c, long long* n, int ev_base) {
     const int in = rk, out = (rk + 1) % kRing;
     FusedArgs a = fused_args(c, in, out, rk);
     if (events) HIPCHK(c, hipEventRecord(c->evpool[ev_base + 2 * rk], c->st));
-    if (fused_launch(c->tile_index, 0, a, c->n_wg, c->st) < 0)
+    if (fused_launch(c->tile_index, c->split ? 2 : 0, a, c->n_wg, c->st) < 0)
       return fail(c, OSTC_ERR_UNSUPPORTED, "fused kernel: scheme/tile not available");
-    ++*n;
+    *n += c->split ? 2 : 1;
     if (events) HIPCHK(c, hipEventRecord(c->evpool[ev_base + 2 * rk + 1], c->st));
     launch_group(c, c->rim, out, in, n);
     for (auto& g : c->bc_rounds) launch_group(c, g, out, out, n);
@@ -917,14 +917,25 @@ int ostc_tile_shape(int index, int* tx, int* ty, int* tz) {
   return OSTC_OK;
 }
 
-int ostc_set_tile(ostc_ctx c, int index, int waves_per_cu) {
+int ostc_set_tile(ostc_ctx c, int index, int waves_per_cu, int split) {
   int r = check_ready(c);
   if (r) return r;
   if (index < 0 || index >= fused_shape_count(c->P.scheme)) return fail(c, OSTC_ERR_ARG, "tile index out of range");
   // waves_per_cu <= 0: one workgroup per tile; otherwise a persistent grid of
   // waves_per_cu workgroups per compute unit, with XCD-contiguous chunks.
   c->xcd_remap = waves_per_cu > 0 ? 1 : 0;
+  c->waves = waves_per_cu > 0 ? waves_per_cu : 0;
+  c->split = split ? 1 : 0;
   return set_tiles(c, index, waves_per_cu > 0 ? waves_per_cu * c->cus : 0);
+}
+
+int ostc_tile_config(ostc_ctx c, int* index, int* waves_per_cu, int* split) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (index) *index = c->tile_index;
+  if (waves_per_cu) *waves_per_cu = c->waves;
+  if (split) *split = c->split;
+  return OSTC_OK;
 }
 
 int ostc_autotune(ostc_ctx c, int reps, int* best_index, double* best_ms) {
@@ -932,28 +943,30 @@ int ostc_autotune(ostc_ctx c, int reps, int* best_index, double* best_ms) {
   if (r) return r;
   if (reps < 1) reps = 3;
   double best = 1e30;
-  int bi = c->tile_index, bw = 0;
+  int bi = c->tile_index, bw = 0, bs = 0;
   const int waves[] = {0, 1, 2, 4};
   for (int idx = 0; idx < fused_shape_count(c->P.scheme); ++idx) {
     const int lds = fused_lds_bytes(c->P.scheme, idx);
     if (lds < 0 || lds > 160 * 1024) continue;
-    for (int w : waves) {
-      r = ostc_set_tile(c, idx, w);
-      if (r) return r;
-      FusedArgs a = fused_args(c, 0, 1, 0);
-      if (fused_launch(idx, 0, a, c->n_wg, c->st) < 0) continue;  // warm-up
-      HIPCHK(c, hipEventRecord(c->ev_a, c->st));
-      for (int q = 0; q < reps; ++q) fused_launch(idx, 0, a, c->n_wg, c->st);
-      HIPCHK(c, hipEventRecord(c->ev_b, c->st));
-      HIPCHK(c, hipEventSynchronize(c->ev_b));
-      HIPCHK(c, hipGetLastError());
-      float f = 0;
-      HIPCHK(c, hipEventElapsedTime(&f, c->ev_a, c->ev_b));
-      const double ms = f / reps;
-      if (ms < best) { best = ms; bi = idx; bw = w; }
-    }
+    for (int sp = 0; sp < (c->P.visc ? 2 : 1); ++sp)
+      for (int w : waves) {
+        r = ostc_set_tile(c, idx, w, sp);
+        if (r) return r;
+        FusedArgs a = fused_args(c, 0, 1, 0);
+        const int mode = sp ? 2 : 0;
+        if (fused_launch(idx, mode, a, c->n_wg, c->st) < 0) continue;  // warm-up
+        HIPCHK(c, hipEventRecord(c->ev_a, c->st));
+        for (int q = 0; q < reps; ++q) fused_launch(idx, mode, a, c->n_wg, c->st);
+        HIPCHK(c, hipEventRecord(c->ev_b, c->st));
+        HIPCHK(c, hipEventSynchronize(c->ev_b));
+        HIPCHK(c, hipGetLastError());
+        float f = 0;
+        HIPCHK(c, hipEventElapsedTime(&f, c->ev_a, c->ev_b));
+        const double ms = f / reps;
+        if (ms < best) { best = ms; bi = idx; bw = w; bs = sp; }
+      }
   }
-  r = ostc_set_tile(c, bi, bw);
+  r = ostc_set_tile(c, bi, bw, bs);
   if (r) return r;
   if (best_index) *best_index = bi;
   if (best_ms) *best_ms = best;

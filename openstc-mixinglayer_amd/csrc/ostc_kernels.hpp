@@ -29,11 +29,13 @@ struct FusedArgs {
   int target_block;          // >= 0: only this block's tiles (dq mode)
 };
 
-struct TileShape { int tx, ty, tz; };
+struct TileShape { int tx, ty, tz, waves_per_eu; };  // waves_per_eu: occupancy hint given to the compiler
 int fused_shape_count(int scheme);
 TileShape fused_shape(int scheme, int index);
-// Launches the fused kernel of table entry `index`; mode 0 = RK update,
-// mode 1 = write dq only.  Returns the dynamic LDS bytes used, or -1.
+// Launches the fused kernel of table entry `index`; mode 0 = RK update in
+// one kernel, mode 1 = write dq only, mode 2 = RK update split into an
+// inviscid kernel and a viscous+update kernel.  Returns the dynamic LDS bytes
+// used, or -1.
 int fused_launch(int index, int mode, const FusedArgs& a, int n_wg, hipStream_t st);
 int fused_lds_bytes(int scheme, int index);
 void dt_batched(const FusedArgs& a, int tx, int ty, int tz, int n_wg, double* dt_bits, hipStream_t st, long long* n);

@@ -44,8 +44,10 @@ struct FusedLds {
   static constexpr int NPTS = mx3(pts(TX), pts(TY), pts(TZ));
   static constexpr int NB = (TX + 2) * (TY + 2) * (TZ + 2);
   static constexpr int NFMAX = mx3((TX + 1) * (NT / TX), (TY + 1) * (NT / TY), (TZ + 1) * (NT / TZ));
-  static constexpr int WORDS = (10 * NPTS > 5 * NB + 4 * NFMAX) ? 10 * NPTS : 5 * NB + 4 * NFMAX;
+  static constexpr int VWORDS = 5 * NB + 10 * NT + 4 * NFMAX;  // viscous phase
+  static constexpr int WORDS = (10 * NPTS > VWORDS) ? 10 * NPTS : VWORDS;
   static constexpr int BYTES = WORDS * 8;
+  static constexpr int VBYTES = VWORDS * 8;
 };
 
 struct TileCtx {
@@ -169,13 +171,15 @@ __device__ __forceinline__ void viscous_dir(double* smem, const DevBlock& B, con
   const int origin = D == 0 ? t.i0 : (D == 1 ? t.j0 : t.k0);
   const long long s = B.sz;
   const double* sV = smem;          // [5][NB]: u, v, w, T, mu on the tile box
-  double* sFv = smem + 5 * NB;      // [4][NF]
+  const double* sM = smem + 5 * NB; // [10][NT]: metric (9) and Jacobian of the tile cells
+  double* sFv = smem + 5 * NB + 10 * NT;  // [4][NF]
   auto pv = [&](int i, int j, int k, int l) {
     const int q = l == 5 ? 3 : l - 1;
     return sV[q * NB + ((k - t.k0 + 1) * BY + (j - t.j0 + 1)) * BX + (i - t.i0 + 1)];
   };
-  double fres[FPT][4];
-#pragma unroll
+  // one face at a time, each flux written straight to LDS (sFv does not
+  // alias the staged box, so no barrier is needed before the writes)
+#pragma unroll 1
   for (int u = 0; u < FPT; ++u) {
     const int f = tid + u * NT;
     if (f < NF) {
@@ -214,25 +218,42 @@ __device__ __forceinline__ void viscous_dir(double* smem, const DevBlock& B, con
         }
         const long long pA = pidx(B, i, j, k);
         const long long pB = pA + (D == 0 ? 1 : (D == 1 ? (long long)B.sx : (long long)B.sx * B.sy));
-        double mA[9], mB[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) { mA[c] = B.met[pA + c * s]; mB[c] = B.met[pB + c * s]; }
         const int iB = i + (D == 0), jB = j + (D == 1), kB = k + (D == 2);
+        // metric and Jacobian of the two face cells: from the tile's LDS copy,
+        // or from global memory for the cell beyond either end of a column
+        double mA[9], mB[9], jacA, jacB;
+        const int cA = ((k - t.k0) * TY + (j - t.j0)) * TX + (i - t.i0);
+        const int cB = ((kB - t.k0) * TY + (jB - t.j0)) * TX + (iB - t.i0);
+        const bool inA = (D == 0 ? i >= t.i0 : (D == 1 ? j >= t.j0 : k >= t.k0));
+        const bool inB = (D == 0 ? iB < t.i0 + TX : (D == 1 ? jB < t.j0 + TY : kB < t.k0 + TZ));
+        if (inA) {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) mA[c] = sM[c * NT + cA];
+          jacA = sM[9 * NT + cA];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) mA[c] = B.met[pA + c * s];
+          jacA = B.jac[pA];
+        }
+        if (inB) {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) mB[c] = sM[c * NT + cB];
+          jacB = sM[9 * NT + cB];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) mB[c] = B.met[pB + c * s];
+          jacB = B.jac[pB];
+        }
         const double um = 0.5 * (pv(i, j, k, 1) + pv(iB, jB, kB, 1));
         const double vm = 0.5 * (pv(i, j, k, 2) + pv(iB, jB, kB, 2));
         const double wm = 0.5 * (pv(i, j, k, 3) + pv(iB, jB, kB, 3));
         const double muA = sV[4 * NB + ((k - t.k0 + 1) * BY + (j - t.j0 + 1)) * BX + (i - t.i0 + 1)];
         const double muB = sV[4 * NB + ((kB - t.k0 + 1) * BY + (jB - t.j0 + 1)) * BX + (iB - t.i0 + 1)];
-        viscous_face_flux<D>(d, mA, mB, B.jac[pA], B.jac[pB], muA, muB, um, vm, wm, P.prandtl, fres[u]);
+        double fv[4];
+        viscous_face_flux<D>(d, mA, mB, jacA, jacB, muA, muB, um, vm, wm, P.prandtl, fv);
+#pragma unroll
+        for (int l = 0; l < 4; ++l) sFv[l * NF + f] = fv[l];
       }
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < FPT; ++u) {
-    const int f = tid + u * NT;
-    if (f < NF) {
-#pragma unroll
-      for (int l = 0; l < 4; ++l) sFv[l * NF + f] = fres[u][l];
     }
   }
   __syncthreads();
@@ -264,14 +285,30 @@ __device__ __forceinline__ void viscous_phase(double* smem, const DevBlock& B, c
       smem[4 * NB + p] = mu[q];
     }
   }
+  {
+    // this thread's cell: metric and Jacobian, read once per stage
+    const int i = t.i0 + t.ti, j = t.j0 + t.tj, k = t.k0 + t.tk;
+    double* sM = smem + 5 * NB;
+    const int c = threadIdx.x;
+    if (i <= B.mx && j <= B.my && k <= B.mz) {
+      const long long q = pidx(B, i, j, k);
+#pragma unroll
+      for (int m = 0; m < 9; ++m) sM[m * NT + c] = B.met[q + m * s];
+      sM[9 * NT + c] = B.jac[q];
+    }
+  }
   __syncthreads();
   viscous_dir<TX, TY, TZ, 0>(smem, B, P, t, dq);
   viscous_dir<TX, TY, TZ, 1>(smem, B, P, t, dq);
   viscous_dir<TX, TY, TZ, 2>(smem, B, P, t, dq);
 }
 
-template <int TX, int TY, int TZ, int SCHEME, int VISC, int MODE>
-__global__ __launch_bounds__(TX * TY * TZ) void k_fused(FusedArgs A) {
+// MODE 0: inviscid + viscous + RK update; MODE 1: write dq (inviscid, and
+// viscous when VISC) without updating; MODE 2: viscous + RK update starting
+// from the inviscid dq that a MODE 1 / VISC 0 launch stored (a stage split in
+// two kernels of lower register and LDS pressure).
+template <int TX, int TY, int TZ, int WPE, int SCHEME, int VISC, int MODE>
+__global__ __launch_bounds__(TX * TY * TZ) __attribute__((amdgpu_waves_per_eu(WPE))) void k_fused(FusedArgs A) {
   constexpr int NT = TX * TY * TZ;
   extern __shared__ double smem[];
   const int tid = threadIdx.x;
@@ -308,9 +345,16 @@ __global__ __launch_bounds__(TX * TY * TZ) void k_fused(FusedArgs A) {
     const int i = t.i0 + t.ti, j = t.j0 + t.tj, k = t.k0 + t.tk;
     t.valid = i < B.mx && j < B.my && k < B.mz;
     double dq[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    inviscid_phase<TX, TY, TZ, SCHEME, 0>(smem, B, A.P, A.in, t, dq);
-    inviscid_phase<TX, TY, TZ, SCHEME, 1>(smem, B, A.P, A.in, t, dq);
-    inviscid_phase<TX, TY, TZ, SCHEME, 2>(smem, B, A.P, A.in, t, dq);
+    if (MODE != 2) {
+      inviscid_phase<TX, TY, TZ, SCHEME, 0>(smem, B, A.P, A.in, t, dq);
+      inviscid_phase<TX, TY, TZ, SCHEME, 1>(smem, B, A.P, A.in, t, dq);
+      inviscid_phase<TX, TY, TZ, SCHEME, 2>(smem, B, A.P, A.in, t, dq);
+    } else if (t.valid) {
+      // split stage: the inviscid part of dq was written by the MODE 1 launch
+      const long long p0 = pidx0(B, i, j, k);
+#pragma unroll
+      for (int l = 0; l < 5; ++l) dq[l] = B.dq[p0 + l * B.n0];
+    }
     if (VISC) viscous_phase<TX, TY, TZ>(smem, B, A.P, A.in, t, dq);
     if (!t.valid) continue;
     const long long p = pidx(B, i, j, k), s = B.sz;
@@ -404,10 +448,10 @@ __global__ __launch_bounds__(TX * TY * TZ) void k_dt_tiles(FusedArgs A, unsigned
 }
 
 // ---------------------------------------------------------- shape table
-#define OSTC_SHAPES(X) X(16, 4, 4) X(8, 8, 4) X(8, 4, 4) X(32, 4, 2) X(16, 8, 2) X(8, 8, 8)
+#define OSTC_SHAPES(X) X(16, 4, 4, 1) X(8, 8, 4, 1) X(8, 4, 4, 1) X(32, 4, 2, 1) X(16, 8, 2, 1) X(8, 8, 8, 1)
 
 static const TileShape kShapes[] = {
-#define X(a, b, c) {a, b, c},
+#define X(a, b, c, w) {a, b, c, w},
     OSTC_SHAPES(X)
 #undef X
 };
@@ -416,44 +460,53 @@ static constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 int fused_shape_count(int) { return kNumShapes; }
 TileShape fused_shape(int, int index) { return kShapes[index]; }
 
-template <int TX, int TY, int TZ, int SCHEME>
+template <int TX, int TY, int TZ, int WPE, int SCHEME>
 static int launch_shape(int mode, int visc, const FusedArgs& a, int n_wg, hipStream_t st) {
   constexpr int bytes = FusedLds<TX, TY, TZ, SCHEME>::BYTES;
+  constexpr int vbytes = FusedLds<TX, TY, TZ, SCHEME>::VBYTES;
   if (bytes > 160 * 1024) return -1;
   static bool attr_set = false;  // dynamic LDS above 64 KiB has to be enabled per kernel
   if (!attr_set) {
-    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<TX, TY, TZ, SCHEME, 1, 0>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<TX, TY, TZ, WPE, SCHEME, 1, 2>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, vbytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<TX, TY, TZ, WPE, SCHEME, 1, 0>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<TX, TY, TZ, SCHEME, 0, 0>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<TX, TY, TZ, WPE, SCHEME, 0, 0>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<TX, TY, TZ, SCHEME, 1, 1>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<TX, TY, TZ, WPE, SCHEME, 1, 1>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<TX, TY, TZ, SCHEME, 0, 1>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<TX, TY, TZ, WPE, SCHEME, 0, 1>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     attr_set = true;
   }
   const dim3 grid(n_wg), block(TX * TY * TZ);
-  if (mode == 0 && visc) hipLaunchKernelGGL((k_fused<TX, TY, TZ, SCHEME, 1, 0>), grid, block, bytes, st, a);
-  else if (mode == 0) hipLaunchKernelGGL((k_fused<TX, TY, TZ, SCHEME, 0, 0>), grid, block, bytes, st, a);
-  else if (visc) hipLaunchKernelGGL((k_fused<TX, TY, TZ, SCHEME, 1, 1>), grid, block, bytes, st, a);
-  else hipLaunchKernelGGL((k_fused<TX, TY, TZ, SCHEME, 0, 1>), grid, block, bytes, st, a);
+  if (mode == 2 && visc) {  // split stage
+    hipLaunchKernelGGL((k_fused<TX, TY, TZ, WPE, SCHEME, 0, 1>), grid, block, bytes, st, a);
+    hipLaunchKernelGGL((k_fused<TX, TY, TZ, WPE, SCHEME, 1, 2>), grid, block, vbytes, st, a);
+    return bytes;
+  }
+  if (mode == 2) mode = 0;
+  if (mode == 0 && visc) hipLaunchKernelGGL((k_fused<TX, TY, TZ, WPE, SCHEME, 1, 0>), grid, block, bytes, st, a);
+  else if (mode == 0) hipLaunchKernelGGL((k_fused<TX, TY, TZ, WPE, SCHEME, 0, 0>), grid, block, bytes, st, a);
+  else if (visc) hipLaunchKernelGGL((k_fused<TX, TY, TZ, WPE, SCHEME, 1, 1>), grid, block, bytes, st, a);
+  else hipLaunchKernelGGL((k_fused<TX, TY, TZ, WPE, SCHEME, 0, 1>), grid, block, bytes, st, a);
   return bytes;
 }
 
-template <int TX, int TY, int TZ>
+template <int TX, int TY, int TZ, int WPE>
 static int launch_scheme(int mode, const FusedArgs& a, int n_wg, hipStream_t st) {
   switch (a.P.scheme) {
-    case 51: return launch_shape<TX, TY, TZ, 51>(mode, a.P.visc, a, n_wg, st);
-    case 50: return launch_shape<TX, TY, TZ, 50>(mode, a.P.visc, a, n_wg, st);
-    case 71: return launch_shape<TX, TY, TZ, 71>(mode, a.P.visc, a, n_wg, st);
+    case 51: return launch_shape<TX, TY, TZ, WPE, 51>(mode, a.P.visc, a, n_wg, st);
+    case 50: return launch_shape<TX, TY, TZ, WPE, 50>(mode, a.P.visc, a, n_wg, st);
+    case 71: return launch_shape<TX, TY, TZ, WPE, 71>(mode, a.P.visc, a, n_wg, st);
     default: return -1;
   }
 }
 
 int fused_launch(int index, int mode, const FusedArgs& a, int n_wg, hipStream_t st) {
   int i = 0;
-#define X(a_, b_, c_) \
-  if (index == i) return launch_scheme<a_, b_, c_>(mode, a, n_wg, st); \
+#define X(a_, b_, c_, w_) \
+  if (index == i) return launch_scheme<a_, b_, c_, w_>(mode, a, n_wg, st); \
   ++i;
   OSTC_SHAPES(X)
 #undef X
@@ -462,7 +515,7 @@ int fused_launch(int index, int mode, const FusedArgs& a, int n_wg, hipStream_t 
 
 int fused_lds_bytes(int scheme, int index) {
   int i = 0;
-#define X(a_, b_, c_)                                                        \
+#define X(a_, b_, c_, w_)                                                        \
   if (index == i) {                                                          \
     if (scheme == 71) return FusedLds<a_, b_, c_, 71>::BYTES;                \
     return FusedLds<a_, b_, c_, 51>::BYTES;                                  \
@@ -479,7 +532,7 @@ void dt_batched(const FusedArgs& a, int tx, int ty, int tz, int n_wg, double* dt
   hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, bits);
   int i = 0;
   bool done = false;
-#define X(a_, b_, c_)                                                                              \
+#define X(a_, b_, c_, w_)                                                                              \
   if (!done && tx == a_ && ty == b_ && tz == c_) {                                                 \
     hipLaunchKernelGGL((k_dt_tiles<a_, b_, c_>), dim3(n_wg), dim3(a_ * b_ * c_), 0, st, a, bits); \
     done = true;                                                                                   \

@@ -121,3 +121,22 @@ def test_persistent_grid_matches_one_wg_per_tile():
         np.testing.assert_array_equal(s1.get(b, pkg.FIELD_BV), s2.get(b, pkg.FIELD_BV))
     s1.close()
     s2.close()
+
+
+@pytest.mark.parametrize("split", [0, 1], ids=["one-kernel", "split-stage"])
+def test_batched_launch_configs_match_oracle(split):
+    case = MULTI
+    n = 5
+    print(f"[gpu] batched steps, split={split}, persistent grid", flush=True)
+    o = oracle_for(case)
+    s = _solver(case)
+    s.initialize()
+    s.set_tile(1, 2, split)
+    dt_o = o.step(n)
+    dt_g = s.rk3(n, pkg.MODE_BATCHED)
+    assert abs(dt_g - dt_o) <= 1e-12 * dt_o
+    g = s.ngg
+    for b in range(len(case["blocks"])):
+        e = rel_err(interior(s.get(b, pkg.FIELD_BV), g), interior(o.array(b, O.BV), g))
+        assert np.all(e <= BV_TOL), (b, e)
+    s.close()
