@@ -182,9 +182,19 @@ int build_ops(ostc_ctx c) {
     for (int d = 0; d < 3; ++d) { pad.lo[d] = -G; pad.hi[d] = B.m[d] - 1 + G; }
     for (const Box& sb : minus_interior(pad, B.m)) props_ops.push_back(make_op(OP_PROPS, b, sb));
   }
-  // inner faces, reference order
-  struct CommOp { DevOp op; std::vector<Access> w, r; int level; };
-  std::vector<CommOp> comm;
+  // Inner faces.  The per-block path launches one op per face in the
+  // reference's order (blocks, then faces).  The batched path splits every
+  // face of a two-sided interface into
+  //  * a bulk part, the face without a frame of ngg+1 cells along its
+  //    tangential rims: no other face's op reads or writes these cells, so
+  //    all bulk parts run in one launch; the side with side > 0 averages the
+  //    shared node layer and recomputes cv in both blocks, the other side
+  //    copies its ghost layers only;
+  //  * the frame, whose cells neighbouring faces also touch: frame ops keep
+  //    the reference's order wherever their cell sets conflict (level =
+  //    1 + the highest level of an earlier conflicting frame op).
+  struct FaceGeom { int A, T, a, side; Box box, shA, shT, layT; bool paired; };
+  std::vector<FaceGeom> geo;
   for (int b = 0; b < nb; ++b) {
     HBlock& A = c->blocks[b];
     for (const InnerDesc& f : A.inner) {
@@ -193,44 +203,92 @@ int build_ops(ostc_ctx c) {
       for (int d = 0; d < 3; ++d)
         if (d != a && A.m[d] != T.m[d])
           return fail(c, OSTC_ERR_ARG, "inner face: tangential extents of the two blocks differ");
-      Box box, shA, shT, layT;
+      FaceGeom F;
+      F.A = b; F.T = f.target; F.a = a; F.side = f.side;
       for (int d = 0; d < 3; ++d) {
-        box.lo[d] = shA.lo[d] = shT.lo[d] = layT.lo[d] = 0;
-        box.hi[d] = shA.hi[d] = shT.hi[d] = layT.hi[d] = A.m[d] - 1;
+        F.box.lo[d] = F.shA.lo[d] = F.shT.lo[d] = F.layT.lo[d] = 0;
+        F.box.hi[d] = F.shA.hi[d] = F.shT.hi[d] = F.layT.hi[d] = A.m[d] - 1;
       }
       const int mA = A.m[a], mT = T.m[a];
       if (f.side > 0) {
-        box.lo[a] = mA - 1; box.hi[a] = mA - 1 + G;
-        shA.lo[a] = shA.hi[a] = mA - 1;
-        shT.lo[a] = shT.hi[a] = 0;
-        layT.lo[a] = 0; layT.hi[a] = G;
+        F.box.lo[a] = mA - 1; F.box.hi[a] = mA - 1 + G;
+        F.shA.lo[a] = F.shA.hi[a] = mA - 1;
+        F.shT.lo[a] = F.shT.hi[a] = 0;
+        F.layT.lo[a] = 0; F.layT.hi[a] = G;
       } else {
-        box.lo[a] = -G; box.hi[a] = 0;
-        shA.lo[a] = shA.hi[a] = 0;
-        shT.lo[a] = shT.hi[a] = mT - 1;
-        layT.lo[a] = mT - 1 - G; layT.hi[a] = mT - 1;
+        F.box.lo[a] = -G; F.box.hi[a] = 0;
+        F.shA.lo[a] = F.shA.hi[a] = 0;
+        F.shT.lo[a] = F.shT.hi[a] = mT - 1;
+        F.layT.lo[a] = mT - 1 - G; F.layT.hi[a] = mT - 1;
       }
+      F.paired = false;
+      for (const InnerDesc& g : T.inner)
+        if (g.target == b && g.axis == a && g.side == -f.side) F.paired = true;
+      geo.push_back(F);
+      props_ops.push_back(make_op(OP_PROPS, b, F.shA));  // shared layers after averaging
+    }
+  }
+  auto comm_op = [&](const FaceGeom& F, const Box& bx) {
+    DevOp op = make_op(OP_COMM, F.A, bx);
+    op.target = F.T;
+    op.axis = F.a;
+    op.side = F.side;
+    op.shared = F.side > 0 ? c->blocks[F.A].m[F.a] - 1 : 0;
+    op.tstart = F.side > 0 ? 0 : c->blocks[F.T].m[F.a] - 1 - G;
+    // tstart refers to lo[axis] of the full face box
+    op.tstart += bx.lo[F.a] - F.box.lo[F.a];
+    return op;
+  };
+  auto restrict_tangential = [](Box b, const Box& to, int a) {
+    for (int d = 0; d < 3; ++d)
+      if (d != a) { b.lo[d] = to.lo[d]; b.hi[d] = to.hi[d]; }
+    return b;
+  };
+  std::vector<DevOp> plain_ops, bulk_ops;
+  struct CommOp { DevOp op; std::vector<Access> w, r; int level; };
+  std::vector<CommOp> edge;
+  for (const FaceGeom& F : geo) {
+    plain_ops.push_back(comm_op(F, F.box));
+    const HBlock& A = c->blocks[F.A];
+    const int W = G + 1;
+    Box core = F.box;
+    for (int d = 0; d < 3; ++d)
+      if (d != F.a) { core.lo[d] = W; core.hi[d] = A.m[d] - 1 - W; }
+    std::vector<Box> frame;
+    if (F.paired && box_ok(core)) {
+      Box cb = core;
+      if (F.side < 0) cb.hi[F.a] = -1;  // ghost layers only; the partner averages
+      DevOp op = comm_op(F, cb);
+      if (F.side > 0) op.flags = 1;     // recompute the target block's cv on the shared layer
+      bulk_ops.push_back(op);
+      // frame = box minus core, as slabs along the first tangential axis
+      // and then the second
+      const int t1 = F.a == 0 ? 1 : 0, t2 = F.a == 2 ? 1 : 2;
+      Box r = F.box;
+      Box p = r; p.hi[t1] = W - 1; frame.push_back(p);
+      p = r; p.lo[t1] = A.m[t1] - W; frame.push_back(p);
+      r.lo[t1] = W; r.hi[t1] = A.m[t1] - 1 - W;
+      p = r; p.hi[t2] = W - 1; frame.push_back(p);
+      p = r; p.lo[t2] = A.m[t2] - W; frame.push_back(p);
+    } else {
+      frame.push_back(F.box);
+    }
+    for (const Box& eb : frame) {
       CommOp co;
-      co.op = make_op(OP_COMM, b, box);
-      co.op.target = f.target;
-      co.op.axis = a;
-      co.op.side = f.side;
-      co.op.shared = f.side > 0 ? mA - 1 : 0;
-      co.op.tstart = f.side > 0 ? 0 : mT - 1 - G;
-      co.w = {{b, box}, {f.target, shT}};
-      co.r = {{f.target, layT}, {b, shA}};
+      co.op = comm_op(F, eb);
+      co.w = {{F.A, eb}, {F.T, restrict_tangential(F.shT, eb, F.a)}};
+      co.r = {{F.T, restrict_tangential(F.layT, eb, F.a)}, {F.A, restrict_tangential(F.shA, eb, F.a)}};
       co.level = 0;
-      for (auto& e : comm) {
+      for (auto& e : edge) {
         auto meet = [](const std::vector<Access>& x, const std::vector<Access>& y) {
-          for (auto& p : x)
-            for (auto& q : y)
-              if (p.block == q.block && box_hit(p.box, q.box)) return true;
+          for (auto& u : x)
+            for (auto& v : y)
+              if (u.block == v.block && box_hit(u.box, v.box)) return true;
           return false;
         };
         if (meet(e.w, co.w) || meet(e.w, co.r) || meet(e.r, co.w)) co.level = std::max(co.level, e.level + 1);
       }
-      comm.push_back(co);
-      props_ops.push_back(make_op(OP_PROPS, b, shA));  // shared layers after averaging
+      edge.push_back(co);
     }
   }
   // assemble: [bc rounds][comm levels][props][rim]
@@ -258,21 +316,30 @@ int build_ops(ostc_ctx c) {
     close_group(g, lo);
     c->bc_rounds.push_back(g);
   }
-  int max_level = -1;
-  for (auto& e : comm) max_level = std::max(max_level, e.level);
-  std::vector<int> pos(comm.size());
-  for (int L = 0; L <= max_level; ++L) {
+  if (!bulk_ops.empty()) {
     const int lo = (int)c->ops.size();
-    for (size_t q = 0; q < comm.size(); ++q)
-      if (comm[q].level == L) {
-        pos[q] = (int)c->ops.size();
-        c->ops.push_back(comm[q].op);
-      }
+    for (auto& o : bulk_ops) c->ops.push_back(o);
     Group g;
     close_group(g, lo);
     c->comm_levels.push_back(g);
   }
-  c->plain_comm = pos;
+  int max_level = -1;
+  for (auto& e : edge) max_level = std::max(max_level, e.level);
+  for (int L = 0; L <= max_level; ++L) {
+    const int lo = (int)c->ops.size();
+    for (size_t q = 0; q < edge.size(); ++q)
+      if (edge[q].level == L) c->ops.push_back(edge[q].op);
+    Group g;
+    close_group(g, lo);
+    c->comm_levels.push_back(g);
+  }
+  c->plain_comm.clear();
+  for (auto& o : plain_ops) {
+    // one launch per face: each op is a group of its own (begin = 0)
+    c->plain_comm.push_back((int)c->ops.size());
+    o.begin = 0;
+    c->ops.push_back(o);
+  }
   {
     const int lo = (int)c->ops.size();
     for (auto& o : props_ops) c->ops.push_back(o);
@@ -355,11 +422,11 @@ FusedArgs fused_args(ostc_ctx c, int in, int out, int rk) {
 }
 
 void launch_group(ostc_ctx c, const Group& g, int slot, int src, long long* n) {
-  if (g.hi > g.lo) ops_launch(c->d_blocks, c->d_ops, g.lo, g.hi, g.total, slot, src, c->st, n);
+  if (g.hi > g.lo) ops_launch(c->d_blocks, (int)c->blocks.size(), c->d_ops, g.lo, g.hi, g.total, slot, src, c->st, n);
 }
 
 void launch_op(ostc_ctx c, int idx, int slot, long long* n) {
-  ops_launch(c->d_blocks, c->d_ops, idx, idx + 1, c->ops[idx].count, slot, slot, c->st, n);
+  ops_launch(c->d_blocks, (int)c->blocks.size(), c->d_ops, idx, idx + 1, c->ops[idx].count, slot, slot, c->st, n);
 }
 
 // ---------------------------------------------------- per-block path pieces

@@ -235,18 +235,39 @@ __global__ __launch_bounds__(256) void k_padded(DevBlock B, int slot, int what) 
 
 // One launch of a range of independent ops.  Thread -> op by binary search
 // over the ops' running point offsets.
-__global__ __launch_bounds__(256) void k_ops(const DevBlock* blocks, const DevOp* ops, int op_lo, int op_hi,
-                                             long long total, int slot, int src) {
+// The launch's op offsets and the block descriptors are first staged in LDS
+// (when they fit), so a thread finds its op and its arrays without a chain of
+// dependent global loads.
+constexpr int kOpsInLds = 256;
+constexpr int kBlocksInLds = 64;
+
+__global__ __launch_bounds__(256) void k_ops(const DevBlock* blocks, int n_blocks, const DevOp* ops, int op_lo,
+                                             int op_hi, long long total, int slot, int src) {
+  __shared__ long long s_begin[kOpsInLds];
+  __shared__ DevBlock s_blocks[kBlocksInLds];
+  const int n_ops = op_hi - op_lo;
+  const bool ops_lds = n_ops <= kOpsInLds, blk_lds = n_blocks <= kBlocksInLds;
+  const long long base = ops[op_lo].begin;
+  if (ops_lds)
+    for (int t = threadIdx.x; t < n_ops; t += 256) s_begin[t] = ops[op_lo + t].begin - base;
+  if (blk_lds) {
+    static_assert(sizeof(DevBlock) % 8 == 0, "DevBlock is copied as 8-byte words");
+    const int words = n_blocks * (int)(sizeof(DevBlock) / 8);
+    const long long* src_w = reinterpret_cast<const long long*>(blocks);
+    long long* dst_w = reinterpret_cast<long long*>(s_blocks);
+    for (int w = threadIdx.x; w < words; w += 256) dst_w[w] = src_w[w];
+  }
+  __syncthreads();
   const long long gid = blockIdx.x * 256ll + threadIdx.x;
   if (gid >= total) return;
-  const long long base = ops[op_lo].begin;
-  int lo = op_lo, hi = op_hi - 1;
+  int lo = 0, hi = n_ops - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (ops[mid].begin - base <= gid) lo = mid; else hi = mid - 1;
+    const long long b = ops_lds ? s_begin[mid] : ops[op_lo + mid].begin - base;
+    if (b <= gid) lo = mid; else hi = mid - 1;
   }
-  const DevOp op = ops[lo];
-  op_point(blocks, op, gid - (op.begin - base), slot, src);
+  const DevOp op = ops[op_lo + lo];
+  op_point(blk_lds ? s_blocks : blocks, op, gid - (op.begin - base), slot, src);
 }
 
 // store_last_step and compute_square_of_dbv + reduction_of_dv_squared<4>
@@ -346,11 +367,11 @@ void padded_pass(const DevBlock& B, int slot, int what, hipStream_t st, long lon
   *n += 1;
 }
 
-void ops_launch(const DevBlock* d_blocks, const DevOp* d_ops, int op_lo, int op_hi, long long total, int slot,
+void ops_launch(const DevBlock* d_blocks, int n_blocks, const DevOp* d_ops, int op_lo, int op_hi, long long total, int slot,
                 int src, hipStream_t st, long long* n) {
   if (total <= 0) return;
   const unsigned nb = (unsigned)((total + 255) / 256);
-  hipLaunchKernelGGL(k_ops, dim3(nb), dim3(256), 0, st, d_blocks, d_ops, op_lo, op_hi, total, slot, src);
+  hipLaunchKernelGGL(k_ops, dim3(nb), dim3(256), 0, st, d_blocks, n_blocks, d_ops, op_lo, op_hi, total, slot, src);
   *n += 1;
 }
 

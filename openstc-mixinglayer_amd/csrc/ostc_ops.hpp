@@ -24,6 +24,7 @@ struct DevOp {
   int n[3];            // box extents
   int shared;          // OP_COMM: index along axis of the shared node layer
   int tstart;          // OP_COMM: target index matching lo[axis]
+  int flags;           // OP_COMM bit 0: also recompute the target's cv on the shared layer
   long long begin;     // first point of this op in its launch range
   long long count;     // n[0]*n[1]*n[2]
 };
@@ -83,6 +84,7 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
           bv[p + l * s] = ave;
           tbv[pt + l * ts] = ave;
         }
+        if (op.flags & 1) cv_point(T, pt);
       } else {
         for (int l = 0; l < 6; ++l) bv[p + l * s] = tbv[pt + l * ts];
       }
