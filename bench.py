@@ -56,6 +56,51 @@ def cpu_baseline(case, steps: int):
                       f"{dt:.1f} s on {threads} OpenMP threads"}
 
 
+def adjoint_leg(n: int, steps: int, every: int, fd_modes=(0, 1), h=1e-6):
+    """Differentiation workload: n^3 single-block mixing layer, 32 Fourier-mode
+    amplitudes -> momentum thickness after `steps` fixed-dt SSP-RK3 steps.
+    Forward cost = the same steps through the graph-captured batched path;
+    adjoint cost = the reverse sweep, recomputation from the checkpoints
+    included.  The gradient is checked against central differences of the
+    forward run (step h) for the amplitudes that move theta at first order."""
+    case = pkg.cases.fourier_case(n, n_steps=steps, checkpoint_every=every)
+    s = pkg.Solver(case)
+    s.setup_fourier_case()
+    s.rk3(2, pkg.MODE_BATCHED, want_dt=False)  # builds the graph
+    s.setup_fourier_case()
+    s.sync()
+    s.rk3(steps, pkg.MODE_BATCHED, want_dt=False)
+    s.sync()
+    fwd_ms, _ = s.last_step_timing()
+    s.setup_fourier_case()
+    theta = s.adjoint_forward(steps, every, case["theta"])
+    grad = s.adjoint_backward(case["modes"])
+    fwd_ckpt_ms, bwd_ms = s.adjoint_timing()
+    a = np.array(case["modes"]["amp"])
+    fd_rel = {}
+    for m in fd_modes:
+        th = []
+        for sign in (1, -1):
+            amp = a.copy()
+            amp[m] += sign * h
+            s.setup_mixing_layer()
+            s.perturb_fourier(0, case["modes"], amp)
+            s.initialize()
+            s.rk3(steps, pkg.MODE_BATCHED, want_dt=False)
+            th.append(s.momentum_thickness(0, case["theta"]))
+        fd = (th[0] - th[1]) / (2 * h)
+        fd_rel[str(m)] = abs(fd - grad[m]) / abs(grad[m])
+    s.close()
+    print(f"[bench] adjoint {n}^3 x {steps} steps: forward {fwd_ms:.2f} ms, forward with checkpoints "
+          f"{fwd_ckpt_ms:.2f} ms, reverse {bwd_ms:.2f} ms, ratio {bwd_ms / fwd_ms:.2f}; fd rel {fd_rel}",
+          file=sys.stderr, flush=True)
+    return {"ratio": bwd_ms / fwd_ms, "forward_ms": fwd_ms, "forward_with_checkpoints_ms": fwd_ckpt_ms,
+            "reverse_ms": bwd_ms, "theta": theta, "n_params": len(a), "fd_step": h, "fd_rel": fd_rel,
+            "workload": f"{n}^3 single-block air mixing layer, WENO-Z5 cp + LF, viscous, {steps} fixed-dt "
+                        f"SSP-RK3 steps, checkpoints every {every} steps, 32 Fourier-mode amplitudes -> "
+                        f"momentum thickness"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +110,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-n", type=int, default=128)
+    ap.add_argument("--adjoint-steps", type=int, default=50)
+    ap.add_argument("--adjoint-every", type=int, default=10)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -129,6 +178,10 @@ def main():
                 "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_kernel_s * 1e3}
     s.close()
 
+    adjoint = None
+    if not a.no_adjoint:
+        adjoint = adjoint_leg(a.adjoint_n, a.adjoint_steps, a.adjoint_every)
+
     cpu = None
     if not a.no_cpu:
         print(f"[bench] cpu baseline: {a.cpu_steps} oracle steps", file=sys.stderr, flush=True)
@@ -144,7 +197,7 @@ def main():
                    "tile": list(pkg.tile_shapes()[tile_index]), "workgroups_per_cu": tile_waves,
                    "split_stage": bool(tile_split)},
         "value_graph": value_graph, "plain_value": plain_value, "speedup_vs_plain": value / plain_value,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "adjoint": adjoint,
     }
     print(json.dumps(out))
 

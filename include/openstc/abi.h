@@ -177,6 +177,75 @@ int ostc_util_mixing_layer_bv(int mx, int my, int mz, int ngg, double y0, double
                               int gi0, int gj0, int gk0, int period_i, int period_j, int period_k,
                               uint64_t seed, double amplitude, double* bv);
 
+/* ------------------------------------------------------------ reverse mode
+ * The reference has no adjoint; these entry points differentiate its RHS
+ * (src/WENO.cu:697-720, :1582-1617; src/ViscousScheme.cuh:23-909) and its
+ * SSP-RK3 stage (src/RK.cuh:85-284) for callers that need gradients.
+ * Supported: WENO-Z5 (51) and WENO5-JS (50); the RK3 reverse sweep needs a
+ * single block with outflow/periodic faces and fixed_time_step = 1. */
+
+/* Library-owned cotangent fields of each block (allocated on first use):
+ * dq_bar [n_var planes, no ghosts] and cv_bar [n_var planes, padded]. */
+#define OSTC_FIELD_DQ_BAR 9
+#define OSTC_FIELD_CV_BAR 10
+
+/* cv_bar = (d dq / d cv)^T dq_bar for block `block` at its current state
+ * (cv, bv, mul as left by ostc_initialize or a step), over the padded domain
+ * (ghost cells included; the caller folds ghost cotangents back through its
+ * own ghost-cell rules).  dq_bar and cv_bar are device pointers in the
+ * layout above; NULL selects OSTC_FIELD_DQ_BAR / OSTC_FIELD_CV_BAR.
+ * cv_bar is overwritten. */
+int ostc_rhs_vjp(ostc_ctx ctx, int block, const double* dq_bar, double* cv_bar);
+
+/* Fourier-mode velocity perturbation of the differentiation workload:
+ * bv[1 + comp[m]] += amp[m] * du * exp(-((y - yc)/delta)^2)
+ *                    * (trig[m] ? cos : sin)(2 pi (kx[m] i dx / lx + kz[m] k dz / lz)),
+ * y = y0 + j dy, over the padded domain of the block's slot-0 bv (before
+ * ostc_initialize). */
+#define OSTC_MAX_MODES 64
+typedef struct {
+  int n_modes;
+  int comp[OSTC_MAX_MODES];  /* 0 u, 1 v, 2 w */
+  int kx[OSTC_MAX_MODES];
+  int kz[OSTC_MAX_MODES];
+  int trig[OSTC_MAX_MODES];  /* 0 sin, 1 cos */
+  double amp[OSTC_MAX_MODES];
+  double y0;                 /* y of node j = 0 */
+  double dx, dy, dz;         /* uniform node spacing */
+  double lx, lz;             /* periods of the phase */
+  double delta;              /* envelope width */
+  double yc;                 /* envelope centre */
+  double du;                 /* velocity scale */
+} ostc_fourier;
+int ostc_perturb_fourier(ostc_ctx ctx, int block, const ostc_fourier* f);
+
+/* Momentum thickness of a temporal mixing layer,
+ * theta = sum_j w_j <rho>_j (u1 - U_j)(U_j - u2) / (rho0 du^2),
+ * U_j = <rho u>_j / <rho>_j, averages over nodes i < ni, k < nk of row j,
+ * w_j = dy (dy/2 on the first and last row). */
+typedef struct {
+  double u1, u2;             /* upper and lower stream velocities */
+  double rho0, du;           /* normalising density and velocity difference */
+  double dy;
+  int ni, nk;
+} ostc_theta;
+int ostc_momentum_thickness(ostc_ctx ctx, int block, const ostc_theta* t, double* theta);
+
+/* Forward half of the differentiated run: n_steps batched SSP-RK3 steps from
+ * the current (initialised) state with the fixed dt, keeping the state every
+ * checkpoint_every steps (and the stage states of the last segment) on the
+ * device, then theta of the final state.  Replaces ostc_step for the run. */
+int ostc_adjoint_forward(ostc_ctx ctx, int n_steps, int checkpoint_every, const ostc_theta* t,
+                         double* theta);
+/* Reverse sweep of the last ostc_adjoint_forward: segments are recomputed
+ * from their checkpoints, stages are reversed with ostc_rhs_vjp's kernels,
+ * and grad[m] = d theta / d amp[m] for the perturbation f that produced the
+ * initial state (f may be NULL: then only OSTC_FIELD_CV_BAR, d theta / d cv
+ * of the initial state after its boundary copies are folded back, is kept). */
+int ostc_adjoint_backward(ostc_ctx ctx, const ostc_fourier* f, double* grad);
+/* Milliseconds of the last forward and backward calls (HIP events). */
+int ostc_adjoint_timing(ostc_ctx ctx, double* forward_ms, double* backward_ms);
+
 #ifdef __cplusplus
 }
 #endif

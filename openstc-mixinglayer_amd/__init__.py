@@ -32,7 +32,9 @@ LIB_PATH = os.path.join(HERE, "lib", "libostc_hip.so")
 cases = importlib.import_module(__name__ + ".cases")
 
 MODE_PLAIN, MODE_BATCHED, MODE_BATCHED_TIMED = 0, 1, 2
-FIELD_CV, FIELD_BV, FIELD_MUL, FIELD_MACH, FIELD_DQ, FIELD_DT_LOCAL, FIELD_JAC, FIELD_METRIC, FIELD_QN = range(9)
+(FIELD_CV, FIELD_BV, FIELD_MUL, FIELD_MACH, FIELD_DQ, FIELD_DT_LOCAL, FIELD_JAC, FIELD_METRIC, FIELD_QN,
+ FIELD_DQ_BAR, FIELD_CV_BAR) = range(11)
+MAX_MODES = 64
 
 
 class OstcError(RuntimeError):
@@ -54,6 +56,39 @@ class Zone(C.Structure):
     _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int), ("ngg", C.c_int), ("sz", C.c_longlong),
                 ("cv", C.c_void_p), ("bv", C.c_void_p), ("mul", C.c_void_p), ("mach", C.c_void_p),
                 ("jac", C.c_void_p), ("metric", C.c_void_p), ("dq", C.c_void_p), ("dt_local", C.c_void_p)]
+
+
+class Fourier(C.Structure):
+    """ostc_fourier: Fourier-mode velocity perturbation (differentiation workload)."""
+    _fields_ = [("n_modes", C.c_int), ("comp", C.c_int * MAX_MODES), ("kx", C.c_int * MAX_MODES),
+                ("kz", C.c_int * MAX_MODES), ("trig", C.c_int * MAX_MODES), ("amp", C.c_double * MAX_MODES),
+                ("y0", C.c_double), ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("lx", C.c_double), ("lz", C.c_double), ("delta", C.c_double), ("yc", C.c_double),
+                ("du", C.c_double)]
+
+
+class Theta(C.Structure):
+    """ostc_theta: momentum-thickness objective."""
+    _fields_ = [("u1", C.c_double), ("u2", C.c_double), ("rho0", C.c_double), ("du", C.c_double),
+                ("dy", C.c_double), ("ni", C.c_int), ("nk", C.c_int)]
+
+
+def fourier_struct(modes: dict, amp=None) -> Fourier:
+    """ostc_fourier from a case's ``modes`` dict (cases.fourier_modes); ``amp``
+    overrides the amplitudes."""
+    f = Fourier()
+    a = modes["amp"] if amp is None else amp
+    f.n_modes = len(a)
+    for m in range(len(a)):
+        f.comp[m], f.kx[m], f.kz[m], f.trig[m] = (int(modes[k][m]) for k in ("comp", "kx", "kz", "trig"))
+        f.amp[m] = float(a[m])
+    for k in ("y0", "dx", "dy", "dz", "lx", "lz", "delta", "yc", "du"):
+        setattr(f, k, float(modes[k]))
+    return f
+
+
+def theta_struct(t: dict) -> Theta:
+    return Theta(t["u1"], t["u2"], t["rho0"], t["du"], t["dy"], t["ni"], t["nk"])
 
 
 _lib = None
@@ -97,6 +132,12 @@ SIGNATURES = {
     "ostc_util_uniform_geometry": (_I, [_I, _I, _I, _I, _D, _D, _D, _D, _D, _D, _DP, _DP]),
     "ostc_util_mixing_layer_bv": (_I, [_I, _I, _I, _I, _D, _D, _DP, _DP, _D, _I, _I, _I, _I, _I, _I,
                                        C.c_uint64, _D, _DP]),
+    "ostc_rhs_vjp": (_I, [_P, _I, _P, _P]),
+    "ostc_perturb_fourier": (_I, [_P, _I, C.POINTER(Fourier)]),
+    "ostc_momentum_thickness": (_I, [_P, _I, C.POINTER(Theta), _DP]),
+    "ostc_adjoint_forward": (_I, [_P, _I, _I, C.POINTER(Theta), _DP]),
+    "ostc_adjoint_backward": (_I, [_P, C.POINTER(Fourier), _DP]),
+    "ostc_adjoint_timing": (_I, [_P, _DP, _DP]),
 }
 
 
@@ -166,12 +207,12 @@ class Solver:
     # -- geometry and state ---------------------------------------------
     def shape(self, b: int, field: int):
         blk = self.case["blocks"][b]
-        if field in (FIELD_DQ, FIELD_DT_LOCAL):
+        if field in (FIELD_DQ, FIELD_DT_LOCAL, FIELD_DQ_BAR):
             per = (blk["mz"], blk["my"], blk["mx"])
         else:
             per = padded_shape(blk, self.ngg)
         planes = {FIELD_CV: 5, FIELD_BV: 6, FIELD_MUL: 1, FIELD_MACH: 1, FIELD_DQ: 5, FIELD_DT_LOCAL: 1,
-                  FIELD_JAC: 1, FIELD_METRIC: 9, FIELD_QN: 5}[field]
+                  FIELD_JAC: 1, FIELD_METRIC: 9, FIELD_QN: 5, FIELD_DQ_BAR: 5, FIELD_CV_BAR: 5}[field]
         return (planes, *per)
 
     def get(self, b: int, field: int) -> np.ndarray:
@@ -287,6 +328,48 @@ class Solver:
 
     def sync(self):
         self._chk(lib().ostc_sync(self.ctx))
+
+    # -- reverse mode ------------------------------------------------------
+    def rhs_vjp(self, b: int, dq_bar: np.ndarray) -> np.ndarray:
+        """(d dq / d cv)^T dq_bar at block b's current state, on the padded domain."""
+        self.set(b, FIELD_DQ_BAR, dq_bar)
+        self._chk(lib().ostc_rhs_vjp(self.ctx, b, None, None))
+        return self.get(b, FIELD_CV_BAR)
+
+    def perturb_fourier(self, b: int, modes: dict, amp=None):
+        self._chk(lib().ostc_perturb_fourier(self.ctx, b, C.byref(fourier_struct(modes, amp))))
+
+    def momentum_thickness(self, b: int, theta: dict) -> float:
+        v = C.c_double()
+        self._chk(lib().ostc_momentum_thickness(self.ctx, b, C.byref(theta_struct(theta)), C.byref(v)))
+        return v.value
+
+    def adjoint_forward(self, n_steps: int, checkpoint_every: int, theta: dict) -> float:
+        v = C.c_double()
+        self._chk(lib().ostc_adjoint_forward(self.ctx, n_steps, checkpoint_every, C.byref(theta_struct(theta)),
+                                             C.byref(v)))
+        return v.value
+
+    def adjoint_backward(self, modes: dict | None = None, amp=None) -> np.ndarray | None:
+        if modes is None:
+            self._chk(lib().ostc_adjoint_backward(self.ctx, None, None))
+            return None
+        f = fourier_struct(modes, amp)
+        g = np.zeros(f.n_modes)
+        self._chk(lib().ostc_adjoint_backward(self.ctx, C.byref(f), _dp(g)))
+        return g
+
+    def adjoint_timing(self):
+        fwd, bwd = C.c_double(), C.c_double()
+        self._chk(lib().ostc_adjoint_timing(self.ctx, C.byref(fwd), C.byref(bwd)))
+        return fwd.value, bwd.value
+
+    def setup_fourier_case(self):
+        """Geometry, the unperturbed mixing layer and the Fourier perturbation
+        of a differentiation case (cases.fourier_case), then initialisation."""
+        self.setup_mixing_layer()
+        self.perturb_fourier(0, self.case["modes"])
+        self.initialize()
 
 
 def tile_shapes():

@@ -144,3 +144,60 @@ def small_multiblock_case(seed=7, **kw):
     ys = [int(v) for v in rng.integers(8, 16, size=2)]
     zs = [int(v) for v in rng.integers(6, 12, size=2)]
     return mixing_layer_case(xs, ys, zs, **kw)
+
+
+def fourier_modes(case, n_modes=32, seed=3, scale=1e-2, centre=0.25):
+    """The seeded Fourier modes of the differentiation workload.
+
+    Mode m = 3 q + c perturbs velocity component c (u, v, w) by
+    a_m * du * exp(-((y - yc)/delta_omega)^2) * (sin | cos)(2 pi (kx i dx / lx + kz k dz / lz)),
+    kx = q % 4, kz = q // 4, du = scale * (u_upper - u_lower),
+    yc = centre * delta_omega; lx, lz are the periodic box lengths, so every
+    mode is periodic with the box.  Modes with kx = kz = 0 use cos: they
+    change the mean profile, and the off-centre envelope makes that move the
+    momentum thickness at first order; the others alternate sin and cos.
+    The amplitudes a_m ~ U(-1, 1) are the parameters."""
+    blk = case["blocks"][0]
+    m = np.arange(n_modes)
+    q, comp = m // 3, m % 3
+    kx, kz = q % 4, q // 4
+    trig = np.where((kx == 0) & (kz == 0), 1, (q + comp) % 2)
+    rng = np.random.default_rng(seed)
+    st = case["streams"]
+    return dict(comp=comp, kx=kx, kz=kz, trig=trig, amp=rng.uniform(-1.0, 1.0, n_modes),
+                y0=blk["y0"], dx=case["dx"], dy=case["dy"], dz=case["dz"],
+                lx=(blk["mx"] - 1) * case["dx"], lz=(blk["mz"] - 1) * case["dz"],
+                delta=case["delta_omega"], yc=centre * case["delta_omega"],
+                du=scale * (st["upper"][1] - st["lower"][1]))
+
+
+def fixed_dt(case, safety=0.9):
+    """CFL time step of the unperturbed layer (largest |u|, uniform c and mu),
+    times ``safety``: the spectral radii of local_time_step,
+    src/TimeAdvanceFunc.cuh:28-83, on a uniform grid."""
+    st = case["streams"]
+    rho, t = st["upper"][0], st["upper"][5]
+    c = math.sqrt(GAMMA_AIR * R_U / MW_AIR * t)
+    u = max(abs(st["upper"][1]), abs(st["lower"][1]))
+    dx, dy, dz = case["dx"], case["dy"], case["dz"]
+    sr_inv = (u + c) / dx + c / dy + c / dz
+    sr_visc = (1 / dx ** 2 + 1 / dy ** 2 + 1 / dz ** 2) * max(GAMMA_AIR, 4 / 3) / rho * sutherland(t) / case["prandtl"]
+    return safety * case["cfl"] / (sr_inv + sr_visc)
+
+
+def fourier_case(n=128, n_steps=50, checkpoint_every=10, seed=3, n_modes=32, scheme=51, **kw):
+    """Differentiation workload: the n^3 single-block temporal mixing layer
+    (air, WENO-Z5 + LF, viscous) with the seeded random perturbation replaced
+    by ``n_modes`` Fourier modes whose amplitudes are the parameters, a fixed
+    time step, and the momentum thickness of the final state as objective."""
+    case = single_block_case(n, scheme=scheme, perturbation=0.0, **kw)
+    case["fixed_time_step"] = 1
+    case["dt"] = fixed_dt(case)
+    case["modes"] = fourier_modes(case, n_modes=n_modes, seed=seed)
+    st = case["streams"]
+    blk = case["blocks"][0]
+    case["theta"] = dict(u1=st["upper"][1], u2=st["lower"][1], rho0=st["upper"][0],
+                         du=st["upper"][1] - st["lower"][1], dy=case["dy"], ni=blk["mx"] - 1, nk=blk["mz"] - 1)
+    case["n_steps"] = n_steps
+    case["checkpoint_every"] = checkpoint_every
+    return case

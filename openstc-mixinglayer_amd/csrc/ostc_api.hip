@@ -99,6 +99,20 @@ struct ostc_ctx_s {
   long long last_launches = 0;
   std::vector<void*> allocs;
   std::string err;
+  // reverse mode
+  std::vector<double*> dq_bar, cv_bar;   // per block, allocated on first use
+  double* d_fs = nullptr;                // viscous face cotangents
+  long long fs_stride = 0;
+  double* d_lam = nullptr;               // cv cotangent of the reverse sweep (padded)
+  double* d_lam_qn = nullptr;            // cotangent of the step's initial state, through qn
+  double* d_w = nullptr;                 // dq cotangent of one stage
+  double* d_obj = nullptr;               // theta row sums, their cotangents, theta
+  double* d_part = nullptr;              // gradient partials
+  std::vector<double*> ckpt, stage;      // saved states, 12 planes each (cv, bv, mul)
+  int adj_steps = 0, adj_every = 0;
+  bool adj_ready = false;
+  ostc_theta adj_theta{};
+  hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr, ev_b0 = nullptr, ev_b1 = nullptr;
 };
 
 namespace {
@@ -115,10 +129,13 @@ int fail(ostc_ctx c, int code, const std::string& msg) {
       return fail((c), OSTC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Device allocation, zeroed on the context's stream: the stream is
+// non-blocking, so a memset on the null stream would not be ordered before
+// the work queued on it next.
 int dalloc(ostc_ctx c, double** p, size_t n) {
   void* v = nullptr;
   HIPCHK(c, hipMalloc(&v, n * sizeof(double)));
-  HIPCHK(c, hipMemset(v, 0, n * sizeof(double)));
+  HIPCHK(c, hipMemsetAsync(v, 0, n * sizeof(double), c->st));
   c->allocs.push_back(v);
   *p = static_cast<double*>(v);
   return OSTC_OK;
@@ -379,6 +396,8 @@ int set_tiles(ostc_ctx c, int index, int n_wg) {
   const TileShape t = fused_shape(c->P.scheme, index);
   const int lds = fused_lds_bytes(c->P.scheme, index);
   if (lds < 0 || lds > 160 * 1024) return fail(c, OSTC_ERR_ARG, "tile shape does not fit in LDS");
+  // the tile list and block table are replaced below; queued kernels read them
+  HIPCHK(c, hipStreamSynchronize(c->st));
   std::vector<int> tb;
   int base = 0;
   for (size_t b = 0; b < c->blocks.size(); ++b) {
@@ -486,25 +505,34 @@ int plain_step(ostc_ctx c, double* dt_io, long long* n) {
 // ------------------------------------------------------- batched step
 // One batched step.  ev_base >= 0: record event pairs evpool[ev_base + 2 rk],
 // evpool[ev_base + 2 rk + 1] around the fused kernel of stage rk.
-int batched_record(ostc_ctx c, long long* n, int ev_base) {
+// Stage rk of a batched step: ring slot rk -> slot (rk+1) % 3, then the
+// ghost layers of the new slot.  ev_base >= 0: events evpool[ev_base + 2 rk],
+// evpool[ev_base + 2 rk + 1] around the fused kernel.
+int batched_stage(ostc_ctx c, int rk, long long* n, int ev_base) {
   const bool events = ev_base >= 0;
+  const int in = rk, out = (rk + 1) % kRing;
+  FusedArgs a = fused_args(c, in, out, rk);
+  if (events) HIPCHK(c, hipEventRecord(c->evpool[ev_base + 2 * rk], c->st));
+  if (fused_launch(c->tile_index, c->split ? 2 : 0, a, c->n_wg, c->st) < 0)
+    return fail(c, OSTC_ERR_UNSUPPORTED, "fused kernel: scheme/tile not available");
+  *n += c->split ? 2 : 1;
+  if (events) HIPCHK(c, hipEventRecord(c->evpool[ev_base + 2 * rk + 1], c->st));
+  launch_group(c, c->rim, out, in, n);
+  for (auto& g : c->bc_rounds) launch_group(c, g, out, out, n);
+  for (auto& g : c->comm_levels) launch_group(c, g, out, out, n);
+  launch_group(c, c->props, out, out, n);
+  return OSTC_OK;
+}
+
+int batched_record(ostc_ctx c, long long* n, int ev_base) {
   if (!c->s.fixed_time_step) {
     FusedArgs a = fused_args(c, 0, 1, 0);
     const TileShape t = fused_shape(c->P.scheme, c->tile_index);
     dt_batched(a, t.tx, t.ty, t.tz, std::min(c->n_tiles, c->cus * 4), c->d_dt, c->st, n);
   }
   for (int rk = 0; rk < 3; ++rk) {
-    const int in = rk, out = (rk + 1) % kRing;
-    FusedArgs a = fused_args(c, in, out, rk);
-    if (events) HIPCHK(c, hipEventRecord(c->evpool[ev_base + 2 * rk], c->st));
-    if (fused_launch(c->tile_index, c->split ? 2 : 0, a, c->n_wg, c->st) < 0)
-      return fail(c, OSTC_ERR_UNSUPPORTED, "fused kernel: scheme/tile not available");
-    *n += c->split ? 2 : 1;
-    if (events) HIPCHK(c, hipEventRecord(c->evpool[ev_base + 2 * rk + 1], c->st));
-    launch_group(c, c->rim, out, in, n);
-    for (auto& g : c->bc_rounds) launch_group(c, g, out, out, n);
-    for (auto& g : c->comm_levels) launch_group(c, g, out, out, n);
-    launch_group(c, c->props, out, out, n);
+    const int r = batched_stage(c, rk, n, ev_base);
+    if (r) return r;
   }
   return OSTC_OK;
 }
@@ -591,6 +619,8 @@ int ostc_destroy(ostc_ctx c) {
   for (auto& e : c->evpool) hipEventDestroy(e);
   if (c->ev_a) hipEventDestroy(c->ev_a);
   if (c->ev_b) hipEventDestroy(c->ev_b);
+  for (hipEvent_t e : {c->ev_f0, c->ev_f1, c->ev_b0, c->ev_b1})
+    if (e) hipEventDestroy(e);
   for (void* p : c->allocs) hipFree(p);
   if (c->d_tile_block) hipFree(c->d_tile_block);
   if (c->own_stream) hipStreamDestroy(c->st);
@@ -677,6 +707,9 @@ int ostc_finalize(ostc_ctx c) {
   if (r) return r;
   HIPCHK(c, hipEventCreate(&c->ev_a));
   HIPCHK(c, hipEventCreate(&c->ev_b));
+  for (hipEvent_t* e : {&c->ev_f0, &c->ev_f1, &c->ev_b0, &c->ev_b1}) HIPCHK(c, hipEventCreate(e));
+  c->dq_bar.assign(c->blocks.size(), nullptr);
+  c->cv_bar.assign(c->blocks.size(), nullptr);
   if (c->s.fixed_time_step) HIPCHK(c, hipMemcpy(c->d_dt, &c->s.dt, sizeof(double), hipMemcpyHostToDevice));
   c->finalized = true;
   return set_tiles(c, 0, 0);
@@ -706,6 +739,7 @@ int ostc_set_geometry(ostc_ctx c, int b, const double* jac, const double* metric
   if (r) return r;
   if (!jac || !metric) return fail(c, OSTC_ERR_ARG, "null geometry");
   const DevBlock& d = c->blocks[b].d;
+  HIPCHK(c, hipStreamSynchronize(c->st));  // kernels in flight may read the old geometry
   std::vector<double> m(9 * (size_t)d.sz);
   for (long long p = 0; p < d.sz; ++p) {
     if (!(jac[p] > 0)) return fail(c, OSTC_ERR_NUMERIC, "non-positive Jacobian");
@@ -716,8 +750,21 @@ int ostc_set_geometry(ostc_ctx c, int b, const double* jac, const double* metric
   return OSTC_OK;
 }
 
+// The cotangent fields of block b (allocated on first use).
+static int adj_fields(ostc_ctx c, int b) {
+  const DevBlock& d = c->blocks[b].d;
+  int r = OSTC_OK;
+  if (!c->dq_bar[b]) r = dalloc(c, &c->dq_bar[b], 5 * d.n0);
+  if (!r && !c->cv_bar[b]) r = dalloc(c, &c->cv_bar[b], 5 * d.sz);
+  return r;
+}
+
 static int field_ptr(ostc_ctx c, int b, int f, int slot, double** p, size_t* n) {
   const DevBlock& d = c->blocks[b].d;
+  if (f == OSTC_FIELD_DQ_BAR || f == OSTC_FIELD_CV_BAR) {
+    const int r = adj_fields(c, b);
+    if (r) return r;
+  }
   switch (f) {
     case OSTC_FIELD_CV: *p = d.cv[slot]; *n = 5 * d.sz; break;
     case OSTC_FIELD_BV: *p = d.bv[slot]; *n = 6 * d.sz; break;
@@ -728,6 +775,8 @@ static int field_ptr(ostc_ctx c, int b, int f, int slot, double** p, size_t* n) 
     case OSTC_FIELD_JAC: *p = const_cast<double*>(d.jac); *n = d.sz; break;
     case OSTC_FIELD_METRIC: *p = const_cast<double*>(d.met); *n = 9 * d.sz; break;
     case OSTC_FIELD_QN: *p = d.cv[1]; *n = 5 * d.sz; break;
+    case OSTC_FIELD_DQ_BAR: *p = c->dq_bar[b]; *n = 5 * d.n0; break;
+    case OSTC_FIELD_CV_BAR: *p = c->cv_bar[b]; *n = 5 * d.sz; break;
     default: return fail(c, OSTC_ERR_ARG, "unknown field");
   }
   return OSTC_OK;
@@ -1048,6 +1097,274 @@ int ostc_sync(ostc_ctx c) {
 }
 
 const char* ostc_last_error(ostc_ctx c) { return c ? c->err.c_str() : "null context"; }
+
+// ------------------------------------------------------------ reverse mode
+}  // extern "C"
+
+namespace {
+
+FourierModes to_modes(const ostc_fourier& f) {
+  FourierModes F{};
+  F.n = f.n_modes;
+  for (int m = 0; m < f.n_modes; ++m) {
+    F.comp[m] = f.comp[m];
+    F.kx[m] = f.kx[m];
+    F.kz[m] = f.kz[m];
+    F.trig[m] = f.trig[m];
+    F.amp[m] = f.amp[m];
+  }
+  F.y0 = f.y0; F.dx = f.dx; F.dy = f.dy; F.dz = f.dz;
+  F.lx = f.lx; F.lz = f.lz; F.delta = f.delta; F.yc = f.yc; F.du = f.du;
+  return F;
+}
+
+int check_modes(ostc_ctx c, const ostc_fourier* f) {
+  if (!f || f->n_modes < 1 || f->n_modes > OSTC_MAX_MODES) return fail(c, OSTC_ERR_ARG, "n_modes must be 1..64");
+  for (int m = 0; m < f->n_modes; ++m)
+    if (f->comp[m] < 0 || f->comp[m] > 2) return fail(c, OSTC_ERR_ARG, "mode component must be 0, 1 or 2");
+  if (!(f->lx > 0) || !(f->lz > 0) || !(f->delta > 0)) return fail(c, OSTC_ERR_ARG, "lx, lz, delta must be positive");
+  return OSTC_OK;
+}
+
+ThetaParams to_theta(const ostc_theta& t) { return ThetaParams{t.u1, t.u2, t.rho0, t.du, t.dy, t.ni, t.nk}; }
+
+int check_theta(ostc_ctx c, const DevBlock& B, const ostc_theta* t) {
+  if (!t || t->ni < 1 || t->ni > B.mx || t->nk < 1 || t->nk > B.mz || !(t->rho0 > 0) || t->du == 0)
+    return fail(c, OSTC_ERR_ARG, "bad momentum-thickness parameters");
+  return OSTC_OK;
+}
+
+// Buffers shared by every VJP of the context (face cotangents sized for the
+// largest block) and the objective's row sums.
+int adj_common(ostc_ctx c) {
+  long long need = 0;
+  int my = 0;
+  for (auto& B : c->blocks) {
+    need = std::max(need, vjp_face_stride(B.d.mx, B.d.my, B.d.mz));
+    my = std::max(my, B.d.my);
+  }
+  if (need > c->fs_stride) {
+    int r = dalloc(c, &c->d_fs, 48 * (size_t)need);
+    if (r) return r;
+    c->fs_stride = need;
+  }
+  if (!c->d_obj) {
+    int r = dalloc(c, &c->d_obj, 4 * (size_t)my + 1);
+    if (r) return r;
+  }
+  return OSTC_OK;
+}
+
+VjpArgs vjp_args(ostc_ctx c, const DevBlock& B, const double* cv, const double* bv, const double* mul, const double* w,
+                 double* lam) {
+  VjpArgs a{};
+  a.mx = B.mx; a.my = B.my; a.mz = B.mz; a.g = B.g; a.sx = B.sx; a.sy = B.sy; a.sz = B.sz; a.n0 = B.n0;
+  a.cv = cv; a.bv = bv; a.mul = mul; a.jac = B.jac; a.met = B.met;
+  a.w = w; a.lam = lam; a.fs = c->d_fs; a.fs_stride = c->fs_stride; a.P = c->P;
+  return a;
+}
+
+// A saved state: planes cv (5), bv (6), mul (1) of one ring slot.
+int save_state(ostc_ctx c, const DevBlock& B, int slot, double* to) {
+  const size_t s = (size_t)B.sz * sizeof(double);
+  HIPCHK(c, hipMemcpyAsync(to, B.cv[slot], 5 * s, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(to + 5 * B.sz, B.bv[slot], 6 * s, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(to + 11 * B.sz, B.mul[slot], s, hipMemcpyDeviceToDevice, c->st));
+  return OSTC_OK;
+}
+
+int restore_state(ostc_ctx c, const DevBlock& B, const double* from) {
+  const size_t s = (size_t)B.sz * sizeof(double);
+  HIPCHK(c, hipMemcpyAsync(B.cv[0], from, 5 * s, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(B.bv[0], from + 5 * B.sz, 6 * s, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(B.mul[0], from + 11 * B.sz, s, hipMemcpyDeviceToDevice, c->st));
+  return OSTC_OK;
+}
+
+// Transposed boundary copies of block b, in the reverse of their order.
+void bc_transpose(ostc_ctx c, int b, double* lam) {
+  const DevBlock& B = c->blocks[b].d;
+  for (auto it = c->plain_bc[b].rbegin(); it != c->plain_bc[b].rend(); ++it) adj_bc_op(B, c->ops[*it], lam, c->st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ostc_rhs_vjp(ostc_ctx c, int b, const double* dq_bar, double* cv_bar) {
+  int r = check_block(c, b);
+  if (r) return r;
+  if (c->P.scheme != 51 && c->P.scheme != 50)
+    return fail(c, OSTC_ERR_UNSUPPORTED, "reverse mode covers inviscid_scheme 51 and 50");
+  r = adj_common(c);
+  if (!r && (!dq_bar || !cv_bar)) r = adj_fields(c, b);
+  if (r) return r;
+  const DevBlock& B = c->blocks[b].d;
+  const double* w = dq_bar ? dq_bar : c->dq_bar[b];
+  double* lam = cv_bar ? cv_bar : c->cv_bar[b];
+  HIPCHK(c, hipMemsetAsync(lam, 0, 5 * (size_t)B.sz * sizeof(double), c->st));
+  long long n = 0;
+  rhs_vjp(vjp_args(c, B, B.cv[0], B.bv[0], B.mul[0], w, lam), c->st, &n);
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_perturb_fourier(ostc_ctx c, int b, const ostc_fourier* f) {
+  int r = check_block(c, b);
+  if (!r) r = check_modes(c, f);
+  if (r) return r;
+  const DevBlock& B = c->blocks[b].d;
+  fourier_add(B, to_modes(*f), B.bv[0], c->st);
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_momentum_thickness(ostc_ctx c, int b, const ostc_theta* t, double* theta) {
+  int r = check_block(c, b);
+  if (r) return r;
+  const DevBlock& B = c->blocks[b].d;
+  r = check_theta(c, B, t);
+  if (!r) r = adj_common(c);
+  if (r) return r;
+  if (!theta) return fail(c, OSTC_ERR_ARG, "null theta");
+  double* obj = c->d_obj;
+  theta_eval(B, to_theta(*t), B.cv[0], obj, obj + 2 * B.my, obj + 4 * B.my, c->st);
+  HIPCHK(c, hipMemcpyAsync(theta, obj + 4 * B.my, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_adjoint_forward(ostc_ctx c, int n_steps, int every, const ostc_theta* t, double* theta) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (c->blocks.size() != 1 || !c->blocks[0].inner.empty())
+    return fail(c, OSTC_ERR_UNSUPPORTED, "the RK3 reverse sweep covers a single block without inner faces");
+  if (!c->s.fixed_time_step) return fail(c, OSTC_ERR_UNSUPPORTED, "the RK3 reverse sweep needs fixed_time_step = 1");
+  if (c->P.scheme != 51 && c->P.scheme != 50)
+    return fail(c, OSTC_ERR_UNSUPPORTED, "reverse mode covers inviscid_scheme 51 and 50");
+  if (n_steps < 1 || every < 1) return fail(c, OSTC_ERR_ARG, "n_steps and checkpoint_every must be positive");
+  if (!theta) return fail(c, OSTC_ERR_ARG, "null theta");
+  const DevBlock& B = c->blocks[0].d;
+  r = check_theta(c, B, t);
+  if (!r) r = adj_common(c);
+  if (r) return r;
+  const size_t plane = 12 * (size_t)B.sz;
+  const int n_seg = (n_steps + every - 1) / every;
+  const int seg_len = std::min(every, n_steps);
+  if (!c->d_lam) {
+    r = dalloc(c, &c->d_lam, 5 * (size_t)B.sz);
+    if (!r) r = dalloc(c, &c->d_lam_qn, 5 * (size_t)B.sz);
+    if (!r) r = dalloc(c, &c->d_w, 5 * (size_t)B.n0);
+    if (!r) r = dalloc(c, &c->d_part, (size_t)fourier_grad_parts() * kMaxModes + kMaxModes);
+    if (r) return r;
+  }
+  while ((int)c->ckpt.size() < n_seg) {
+    double* p = nullptr;
+    r = dalloc(c, &p, plane);
+    if (r) return r;
+    c->ckpt.push_back(p);
+  }
+  while ((int)c->stage.size() < 3 * seg_len) {
+    double* p = nullptr;
+    r = dalloc(c, &p, plane);
+    if (r) return r;
+    c->stage.push_back(p);
+  }
+  c->adj_ready = false;
+  HIPCHK(c, hipEventRecord(c->ev_f0, c->st));
+  const int last0 = (n_seg - 1) * every;
+  long long n = 0;
+  for (int s = 0; s < n_steps; ++s) {
+    if (s % every == 0) save_state(c, B, 0, c->ckpt[s / every]);
+    for (int rk = 0; rk < 3; ++rk) {
+      if (s >= last0) save_state(c, B, rk, c->stage[3 * (s - last0) + rk]);
+      r = batched_stage(c, rk, &n, -1);
+      if (r) return r;
+    }
+  }
+  double* obj = c->d_obj;
+  theta_eval(B, to_theta(*t), B.cv[0], obj, obj + 2 * B.my, obj + 4 * B.my, c->st);
+  HIPCHK(c, hipEventRecord(c->ev_f1, c->st));
+  HIPCHK(c, hipMemcpyAsync(theta, obj + 4 * B.my, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipGetLastError());
+  c->adj_steps = n_steps;
+  c->adj_every = every;
+  c->adj_theta = *t;
+  c->adj_ready = true;
+  return OSTC_OK;
+}
+
+int ostc_adjoint_backward(ostc_ctx c, const ostc_fourier* f, double* grad) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (!c->adj_ready) return fail(c, OSTC_ERR_STATE, "ostc_adjoint_forward has not run");
+  if (f) {
+    r = check_modes(c, f);
+    if (r) return r;
+    if (!grad) return fail(c, OSTC_ERR_ARG, "null grad");
+  }
+  r = adj_fields(c, 0);
+  if (r) return r;
+  const DevBlock& B = c->blocks[0].d;
+  const int n_steps = c->adj_steps, every = c->adj_every;
+  const int n_seg = (n_steps + every - 1) / every;
+  const double dt = c->s.dt;
+  HIPCHK(c, hipEventRecord(c->ev_b0, c->st));
+  theta_seed(B, to_theta(c->adj_theta), c->d_obj + 2 * B.my, c->d_lam, c->st);
+  HIPCHK(c, hipMemsetAsync(c->d_lam_qn, 0, 5 * (size_t)B.sz * sizeof(double), c->st));
+  long long n = 0;
+  for (int seg = n_seg - 1; seg >= 0; --seg) {
+    const int s0 = seg * every, s1 = std::min(n_steps, s0 + every);
+    if (seg != n_seg - 1) {
+      // recompute the segment from its checkpoint, keeping every stage's input
+      r = restore_state(c, B, c->ckpt[seg]);
+      if (r) return r;
+      for (int s = s0; s < s1; ++s)
+        for (int rk = 0; rk < 3; ++rk) {
+          save_state(c, B, rk, c->stage[3 * (s - s0) + rk]);
+          r = batched_stage(c, rk, &n, -1);
+          if (r) return r;
+        }
+    }
+    for (int s = s1 - 1; s >= s0; --s)
+      for (int rk = 2; rk >= 0; --rk) {
+        const double* S = c->stage[3 * (s - s0) + rk];
+        bc_transpose(c, 0, c->d_lam);
+        const RkCoef k = rk_coef(rk);
+        adj_stage_pre(B, c->d_lam, c->d_lam_qn, c->d_w, k.a, k.b, k.c * dt, c->st);
+        rhs_vjp(vjp_args(c, B, S, S + 5 * B.sz, S + 11 * B.sz, c->d_w, c->d_lam), c->st, &n);
+        if (rk == 0) adj_stage_qn(B, c->d_lam, c->d_lam_qn, c->st);
+      }
+  }
+  // the initial state: cv_from_bv after the boundary copies of ostc_initialize
+  bc_transpose(c, 0, c->d_lam);
+  HIPCHK(c, hipMemcpyAsync(c->cv_bar[0], c->d_lam, 5 * (size_t)B.sz * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  double* d_grad = c->d_part + (size_t)fourier_grad_parts() * kMaxModes;
+  if (f) fourier_grad(B, to_modes(*f), c->d_lam, c->ckpt[0], c->d_part, d_grad, c->st);
+  HIPCHK(c, hipEventRecord(c->ev_b1, c->st));
+  if (f) HIPCHK(c, hipMemcpyAsync(grad, d_grad, f->n_modes * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_adjoint_timing(ostc_ctx c, double* forward_ms, double* backward_ms) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (!c->adj_ready) return fail(c, OSTC_ERR_STATE, "ostc_adjoint_forward has not run");
+  float f = 0, b = 0;
+  HIPCHK(c, hipEventElapsedTime(&f, c->ev_f0, c->ev_f1));
+  if (backward_ms) {
+    if (hipEventElapsedTime(&b, c->ev_b0, c->ev_b1) != hipSuccess) {
+      (void)hipGetLastError();
+      b = -1;  // no backward since the last forward
+    }
+    *backward_ms = b;
+  }
+  if (forward_ms) *forward_ms = f;
+  return OSTC_OK;
+}
 
 // ----------------------------------------------------------- setup helpers
 // Block::compute_jac_metric (src/Mesh.cpp:241-300) on an analytic uniform

@@ -40,4 +40,45 @@ int fused_launch(int index, int mode, const FusedArgs& a, int n_wg, hipStream_t 
 int fused_lds_bytes(int scheme, int index);
 void dt_batched(const FusedArgs& a, int tx, int ty, int tz, int n_wg, double* dt_bits, hipStream_t st, long long* n);
 
+// reverse mode, ostc_kernels_adjoint.hip
+struct VjpArgs {
+  int mx, my, mz, g, sx, sy;
+  long long sz, n0;
+  const double* cv;      // state the RHS was evaluated at (5 planes, padded)
+  const double* bv;      // its primitives (6 planes, padded)
+  const double* mul;     // its viscosity (padded)
+  const double* jac;
+  const double* met;
+  const double* w;       // dq cotangent (5 planes, interior)
+  double* lam;           // cv cotangent (5 planes, padded), accumulated
+  double* fs;            // viscous face cotangents: 3 directions x 16 planes
+  long long fs_stride;   // plane stride of fs, >= vjp_face_stride()
+  Params P;
+};
+long long vjp_face_stride(int mx, int my, int mz);
+void rhs_vjp(const VjpArgs& a, hipStream_t st, long long* n);
+void adj_stage_pre(const DevBlock& B, double* lam, double* lam_qn, double* w, double ra, double rb, double cdt,
+                   hipStream_t st);
+void adj_stage_qn(const DevBlock& B, double* lam, double* lam_qn, hipStream_t st);
+void adj_bc_op(const DevBlock& B, const DevOp& op, double* lam, hipStream_t st);
+
+constexpr int kMaxModes = 64;
+struct FourierModes {        // see ostc_fourier in abi.h
+  int n;
+  int comp[kMaxModes], kx[kMaxModes], kz[kMaxModes], trig[kMaxModes];
+  double amp[kMaxModes];
+  double y0, dx, dy, dz, lx, lz, delta, yc, du;
+};
+struct ThetaParams {         // see ostc_theta in abi.h
+  double u1, u2, rho0, du, dy;
+  int ni, nk;
+};
+void fourier_add(const DevBlock& B, const FourierModes& F, double* bv, hipStream_t st);
+int fourier_grad_parts();
+void fourier_grad(const DevBlock& B, const FourierModes& F, const double* lam, const double* cv, double* part,
+                  double* grad, hipStream_t st);
+void theta_eval(const DevBlock& B, const ThetaParams& T, const double* cv, double* rows, double* coef, double* theta,
+                hipStream_t st);
+void theta_seed(const DevBlock& B, const ThetaParams& T, const double* coef, double* lam, hipStream_t st);
+
 }  // namespace ostc

@@ -14,6 +14,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libostc_oracle.so")
+TANGENT_LIB_PATH = os.path.join(HERE, "_build", "libostc_oracle_tangent.so")
 
 CV, BV, QN, DQ, MUL, MACH, JAC, METRIC, DT_LOCAL, X, Y, Z, BV_LAST = range(13)
 
@@ -29,21 +30,30 @@ class _Streams(C.Structure):
     _fields_ = [("upper", C.c_double * 6), ("lower", C.c_double * 6)]
 
 
+class _Fourier(C.Structure):
+    _fields_ = [("n_modes", C.c_int), ("comp", C.c_void_p), ("kx", C.c_void_p), ("kz", C.c_void_p),
+                ("trig", C.c_void_p), ("amp", C.c_void_p), ("amp_dot", C.c_void_p),
+                ("x0", C.c_double), ("z0", C.c_double), ("lx", C.c_double), ("lz", C.c_double),
+                ("delta", C.c_double), ("yc", C.c_double), ("du", C.c_double)]
+
+
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(LIB_PATH):
+    if force or not os.path.exists(LIB_PATH) or not os.path.exists(TANGENT_LIB_PATH):
         subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB_PATH
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(tangent: bool = False):
+    """The double oracle, or (tangent=True) the same source built with
+    {value, tangent} dual numbers as state values."""
+    if tangent not in _libs:
+        path = TANGENT_LIB_PATH if tangent else LIB_PATH
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         vp, i, d = C.c_void_p, C.c_int, C.c_double
         dp = C.POINTER(C.c_double)
         L.orc_create.restype = vp
@@ -77,8 +87,12 @@ def lib():
             getattr(L, name).restype = d
             getattr(L, name).argtypes = [dp, dp, d]
         L.orc_lf_split.argtypes = [dp, i, dp, d, dp, dp]
-        _lib = L
-    return _lib
+        L.orc_fourier_perturbation.argtypes = [vp, i, C.POINTER(_Fourier)]
+        L.orc_momentum_thickness.restype = d
+        L.orc_momentum_thickness.argtypes = [vp, i, d, d, d, d, d, i, i, dp]
+        L.orc_scalar_width.restype = i
+        _libs[tangent] = L
+    return _libs[tangent]
 
 
 def _dp(a: np.ndarray):
@@ -88,8 +102,9 @@ def _dp(a: np.ndarray):
 class Oracle:
     """A set of blocks on the host with the reference's layout."""
 
-    def __init__(self, case: dict, n_threads: int = 0):
-        L = lib()
+    def __init__(self, case: dict, n_threads: int = 0, tangent: bool = False):
+        self.tangent = tangent
+        self.L = L = lib(tangent)
         s = _Setup(case["n_var"], case["inviscid_scheme"], case["viscous_order"],
                    int(case.get("fixed_time_step", 0)), float(case.get("dt", 0.0)), case["cfl"],
                    case["prandtl"], case["rho_ref"], case["v_ref"], n_threads)
@@ -109,7 +124,7 @@ class Oracle:
 
     def __del__(self):
         try:
-            lib().orc_destroy(self.ctx)
+            self.L.orc_destroy(self.ctx)
         except Exception:
             pass
 
@@ -124,17 +139,21 @@ class Oracle:
         return (blk["mz"] + 2 * g, blk["my"] + 2 * g, blk["mx"] + 2 * g)
 
     def array(self, b: int, which: int) -> np.ndarray:
-        """Zero-copy view, shape (n_planes, k, j, i)."""
+        """Zero-copy view, shape (n_planes, k, j, i); in the tangent build the
+        state arrays carry a last axis of 2 (value, tangent)."""
         n = C.c_longlong()
-        p = lib().orc_array(self.ctx, b, which, C.byref(n))
+        p = self.L.orc_array(self.ctx, b, which, C.byref(n))
         shp = self.shape(b, which)
         per = shp[0] * shp[1] * shp[2]
+        w = 2 if self.tangent and which not in (JAC, METRIC, X, Y, Z) else 1
         flat = np.ctypeslib.as_array(p, shape=(n.value,))
-        return flat.reshape(n.value // per, *shp)
+        if w == 1:
+            return flat.reshape(n.value // per, *shp)
+        return flat.reshape(n.value // (per * 2), *shp, 2)
 
     # -- setup --------------------------------------------------------
     def setup_mixing_layer(self):
-        L, c = lib(), self.case
+        L, c = self.L, self.case
         st = _Streams()
         for i in range(6):
             st.upper[i] = c["streams"]["upper"][i]
@@ -148,34 +167,54 @@ class Oracle:
             L.orc_mixing_layer_ic(self.ctx, b, C.byref(st), c["delta_omega"], blk["gi0"], blk["gj0"],
                                   blk["gk0"], per[0], per[1], per[2], c["seed"], c["amplitude"])
 
+    def perturb_fourier(self, b: int, modes: dict, amp=None, amp_dot=None):
+        """Fourier-mode velocity perturbation (cases.fourier_modes); in the
+        tangent build amp_dot seeds the amplitude tangents."""
+        a = np.ascontiguousarray(modes["amp"] if amp is None else amp, dtype=np.float64)
+        n = len(a)
+        ints = [np.ascontiguousarray(modes[k][:n], dtype=np.int32) for k in ("comp", "kx", "kz", "trig")]
+        ad = None if amp_dot is None else np.ascontiguousarray(amp_dot, dtype=np.float64)
+        blk = self.case["blocks"][b]
+        f = _Fourier(n, *(x.ctypes.data for x in ints), a.ctypes.data, None if ad is None else ad.ctypes.data,
+                     blk["x0"], blk["z0"], modes["lx"], modes["lz"], modes["delta"], modes["yc"], modes["du"])
+        if self.L.orc_fourier_perturbation(self.ctx, b, C.byref(f)) != 0:
+            raise ValueError("bad Fourier modes")
+
+    def momentum_thickness(self, b: int, t: dict):
+        """(theta, d theta) -- the tangent is 0 in the double build."""
+        dth = C.c_double()
+        th = self.L.orc_momentum_thickness(self.ctx, b, t["u1"], t["u2"], t["rho0"], t["du"], t["dy"], t["ni"],
+                                           t["nk"], C.byref(dth))
+        return th, dth.value
+
     def initialize(self):
-        lib().orc_initialize(self.ctx)
+        self.L.orc_initialize(self.ctx)
 
     def step(self, n: int = 1) -> float:
         dt = C.c_double()
-        lib().orc_step(self.ctx, n, C.byref(dt))
+        self.L.orc_step(self.ctx, n, C.byref(dt))
         return dt.value
 
     # -- stage-level operators ----------------------------------------
     def rhs(self, b: int):
         """dq = 0; inviscid; viscous (src/RK.cuh:113-139)."""
-        L = lib()
+        L = self.L
         L.orc_zero_dq(self.ctx, b)
         L.orc_inviscid(self.ctx, b)
         L.orc_viscous(self.ctx, b)
 
     def local_dt(self, b: int):
-        lib().orc_local_dt(self.ctx, b)
+        self.L.orc_local_dt(self.ctx, b)
 
     def global_dt(self) -> float:
-        return lib().orc_global_dt(self.ctx)
+        return self.L.orc_global_dt(self.ctx)
 
     def update(self, b: int, dt: float, rk: int):
-        lib().orc_update(self.ctx, b, dt, rk)
+        self.L.orc_update(self.ctx, b, dt, rk)
 
     def residual(self):
         r = np.zeros(4)
-        lib().orc_residual(self.ctx, _dp(r))
+        self.L.orc_residual(self.ctx, _dp(r))
         return r
 
 

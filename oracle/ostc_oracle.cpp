@@ -24,8 +24,30 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+#ifdef ORC_DUAL
+#include "dual.h"
+#endif
 
 namespace {
+
+// State values are `real`: double, or a {value, tangent} dual number in the
+// tangent-linear build (-DORC_DUAL).  Geometry (grid, metric, Jacobian) and
+// the time step stay double in both.
+#ifdef ORC_DUAL
+using real = orc::Dual;
+using orc::tangent_of;
+using orc::value_of;
+constexpr int kWidth = 2;
+#else
+using real = double;
+inline double value_of(double a) { return a; }
+inline double tangent_of(double) { return 0.0; }
+constexpr int kWidth = 1;
+#endif
+using std::abs;
+using std::pow;
+using std::sqrt;
+using std::tanh;
 
 // src/Constants.h:17-34
 constexpr double kRu = 8314.462618;
@@ -49,7 +71,8 @@ struct InnerFace {  // aligned 1-to-1 inner face, src/Mesh.cpp:38-176
 struct Block {
   int mx, my, mz, g;
   long long sx, sy, sz;  // padded strides for g ghosts
-  std::vector<double> cv, bv, qn, dq, mul, mach, jac, metric, dtl, x, y, z, bv_last;
+  std::vector<real> cv, bv, qn, dq, mul, mach, dtl, bv_last;
+  std::vector<double> jac, metric, x, y, z;
   std::vector<Face> faces;
   std::vector<InnerFace> inner;
 
@@ -74,29 +97,29 @@ struct Ctx {
 inline double sqr(double a) { return a * a; }
 
 // src/Transport.cu:6-9
-inline double sutherland(double t) {
-  return 1.7894e-5 * std::pow(t / 288.16, 1.5) * (288.16 + 110) / (t + 110);
+inline real sutherland(const real& t) {
+  return 1.7894e-5 * pow(t / 288.16, 1.5) * (288.16 + 110) / (t + 110);
 }
 
 // ---------------------------------------------------------------- WENO kernels
 // WENO-Z5, src/WENO.cu:1582-1617
-double weno5z(const double* vp, const double* vm, double eps) {
+real weno5z(const real* vp, const real* vm, double eps) {
   constexpr double one6th{1.0 / 6};
-  double v0{one6th * (2 * vp[2] + 5 * vp[3] - vp[4])};
-  double v1{one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3])};
-  double v2{one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2])};
+  real v0{one6th * (2 * vp[2] + 5 * vp[3] - vp[4])};
+  real v1{one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3])};
+  real v2{one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2])};
   constexpr double c1312{13.0 / 12};
-  double b0 = c1312 * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
+  real b0 = c1312 * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
               0.25 * (3 * vp[2] - 4 * vp[3] + vp[4]) * (3 * vp[2] - 4 * vp[3] + vp[4]);
-  double b1 = c1312 * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
+  real b1 = c1312 * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
               0.25 * (vp[1] - vp[3]) * (vp[1] - vp[3]);
-  double b2 = c1312 * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
+  real b2 = c1312 * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
               0.25 * (vp[0] - 4 * vp[1] + 3 * vp[2]) * (vp[0] - 4 * vp[1] + 3 * vp[2]);
-  double tau{(b0 - b2) * (b0 - b2)};
-  double a0{0.3 + 0.3 * tau / ((eps + b0) * (eps + b0))};
-  double a1{0.6 + 0.6 * tau / ((eps + b1) * (eps + b1))};
-  double a2{0.1 + 0.1 * tau / ((eps + b2) * (eps + b2))};
-  const double fp{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
+  real tau{(b0 - b2) * (b0 - b2)};
+  real a0{0.3 + 0.3 * tau / ((eps + b0) * (eps + b0))};
+  real a1{0.6 + 0.6 * tau / ((eps + b1) * (eps + b1))};
+  real a2{0.1 + 0.1 * tau / ((eps + b2) * (eps + b2))};
+  const real fp{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
 
   v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
   v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
@@ -111,29 +134,29 @@ double weno5z(const double* vp, const double* vm, double eps) {
   a0 = 0.1 + 0.1 * tau / ((eps + b0) * (eps + b0));
   a1 = 0.6 + 0.6 * tau / ((eps + b1) * (eps + b1));
   a2 = 0.3 + 0.3 * tau / ((eps + b2) * (eps + b2));
-  const double fm{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
+  const real fm{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
   return fp + fm;
 }
 
 // WENO5-JS (Jiang & Shu 1996, J. Comput. Phys. 126:202): the same candidates
 // and smoothness indicators as WENO-Z5 above, weights alpha_k = d_k/(eps+beta_k)^2.
 // Not present in the reference; selected by inviscid_scheme = 50.
-double weno5js(const double* vp, const double* vm, double eps) {
+real weno5js(const real* vp, const real* vm, double eps) {
   constexpr double one6th{1.0 / 6};
-  double v0{one6th * (2 * vp[2] + 5 * vp[3] - vp[4])};
-  double v1{one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3])};
-  double v2{one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2])};
+  real v0{one6th * (2 * vp[2] + 5 * vp[3] - vp[4])};
+  real v1{one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3])};
+  real v2{one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2])};
   constexpr double c1312{13.0 / 12};
-  double b0 = c1312 * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
+  real b0 = c1312 * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
               0.25 * (3 * vp[2] - 4 * vp[3] + vp[4]) * (3 * vp[2] - 4 * vp[3] + vp[4]);
-  double b1 = c1312 * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
+  real b1 = c1312 * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
               0.25 * (vp[1] - vp[3]) * (vp[1] - vp[3]);
-  double b2 = c1312 * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
+  real b2 = c1312 * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
               0.25 * (vp[0] - 4 * vp[1] + 3 * vp[2]) * (vp[0] - 4 * vp[1] + 3 * vp[2]);
-  double a0{0.3 / ((eps + b0) * (eps + b0))};
-  double a1{0.6 / ((eps + b1) * (eps + b1))};
-  double a2{0.1 / ((eps + b2) * (eps + b2))};
-  const double fp{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
+  real a0{0.3 / ((eps + b0) * (eps + b0))};
+  real a1{0.6 / ((eps + b1) * (eps + b1))};
+  real a2{0.1 / ((eps + b2) * (eps + b2))};
+  const real fp{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
 
   v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
   v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
@@ -147,41 +170,41 @@ double weno5js(const double* vp, const double* vm, double eps) {
   a0 = 0.1 / ((eps + b0) * (eps + b0));
   a1 = 0.6 / ((eps + b1) * (eps + b1));
   a2 = 0.3 / ((eps + b2) * (eps + b2));
-  const double fm{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
+  const real fm{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
   return fp + fm;
 }
 
 // WENO-Z7, src/WENO.cu:1619-1693
-double weno7z(const double* vp, const double* vm, double eps) {
+real weno7z(const real* vp, const real* vm, double eps) {
   constexpr double one6th{1.0 / 6};
   constexpr double d12{13.0 / 12.0}, d13{1043.0 / 960}, d14{1.0 / 12};
-  double s1{one6th * (-2 * vp[0] + 9 * vp[1] - 18 * vp[2] + 11 * vp[3])};
-  double s2{-vp[0] + 4 * vp[1] - 5 * vp[2] + 2 * vp[3]};
-  double s3{-vp[0] + 3 * vp[1] - 3 * vp[2] + vp[3]};
-  double b0{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
+  real s1{one6th * (-2 * vp[0] + 9 * vp[1] - 18 * vp[2] + 11 * vp[3])};
+  real s2{-vp[0] + 4 * vp[1] - 5 * vp[2] + 2 * vp[3]};
+  real s3{-vp[0] + 3 * vp[1] - 3 * vp[2] + vp[3]};
+  real b0{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
   s1 = one6th * (vp[1] - 6 * vp[2] + 3 * vp[3] + 2 * vp[4]);
   s2 = vp[2] - 2 * vp[3] + vp[4];
   s3 = -vp[1] + 3 * vp[2] - 3 * vp[3] + vp[4];
-  double b1{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
+  real b1{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
   s1 = one6th * (-2 * vp[2] - 3 * vp[3] + 6 * vp[4] - vp[5]);
   s3 = -vp[2] + 3 * vp[3] - 3 * vp[4] + vp[5];
-  double b2{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
+  real b2{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
   s1 = one6th * (-11 * vp[3] + 18 * vp[4] - 9 * vp[5] + 2 * vp[6]);
   s2 = 2 * vp[3] - 5 * vp[4] + 4 * vp[5] - vp[6];
   s3 = -vp[3] + 3 * vp[4] - 3 * vp[5] + vp[6];
-  double b3{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
-  double tau{(b0 - b3) * (b0 - b3)};
+  real b3{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
+  real tau{(b0 - b3) * (b0 - b3)};
   constexpr double c0{1.0 / 35}, c1{12.0 / 35}, c2{18.0 / 35}, c3{4.0 / 35};
-  double a0{c0 + c0 * tau / ((eps + b0) * (eps + b0))};
-  double a1{c1 + c1 * tau / ((eps + b1) * (eps + b1))};
-  double a2{c2 + c2 * tau / ((eps + b2) * (eps + b2))};
-  double a3{c3 + c3 * tau / ((eps + b3) * (eps + b3))};
+  real a0{c0 + c0 * tau / ((eps + b0) * (eps + b0))};
+  real a1{c1 + c1 * tau / ((eps + b1) * (eps + b1))};
+  real a2{c2 + c2 * tau / ((eps + b2) * (eps + b2))};
+  real a3{c3 + c3 * tau / ((eps + b3) * (eps + b3))};
   constexpr double one12th{1.0 / 12};
-  double v0{one12th * (-3 * vp[0] + 13 * vp[1] - 23 * vp[2] + 25 * vp[3])};
-  double v1{one12th * (vp[1] - 5 * vp[2] + 13 * vp[3] + 3 * vp[4])};
-  double v2{one12th * (-vp[2] + 7 * vp[3] + 7 * vp[4] - vp[5])};
-  double v3{one12th * (3 * vp[3] + 13 * vp[4] - 5 * vp[5] + vp[6])};
-  const double fp{(a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3)};
+  real v0{one12th * (-3 * vp[0] + 13 * vp[1] - 23 * vp[2] + 25 * vp[3])};
+  real v1{one12th * (vp[1] - 5 * vp[2] + 13 * vp[3] + 3 * vp[4])};
+  real v2{one12th * (-vp[2] + 7 * vp[3] + 7 * vp[4] - vp[5])};
+  real v3{one12th * (3 * vp[3] + 13 * vp[4] - 5 * vp[5] + vp[6])};
+  const real fp{(a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3)};
 
   s1 = one6th * (-2 * vm[6] + 9 * vm[5] - 18 * vm[4] + 11 * vm[3]);
   s2 = -vm[6] + 4 * vm[5] - 5 * vm[4] + 2 * vm[3];
@@ -207,17 +230,17 @@ double weno7z(const double* vp, const double* vm, double eps) {
   v1 = one12th * (vm[5] - 5 * vm[4] + 13 * vm[3] + 3 * vm[2]);
   v2 = one12th * (-vm[4] + 7 * vm[3] + 7 * vm[2] - vm[1]);
   v3 = one12th * (3 * vm[3] + 13 * vm[2] - 5 * vm[1] + vm[0]);
-  const double fm{(a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3)};
+  const real fm{(a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3)};
   return fp + fm;
 }
 
 // Point-local Lax-Friedrichs splitting, src/WENO.cu:697-720.
 // q = [cv(0..nv-1), p, c]; k3 = metric row of the sweep direction.
-void lf_split(const double* q, int nv, const double* m, double jac, double* fp, double* fm) {
-  const double Uk{(q[1] * m[0] + q[2] * m[1] + q[3] * m[2]) / q[0]};
-  const double pk{q[nv]};
-  const double cGradK = q[nv + 1] * std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
-  const double lambda0 = std::abs(Uk) + cGradK;
+void lf_split(const real* q, int nv, const double* m, double jac, real* fp, real* fm) {
+  const real Uk{(q[1] * m[0] + q[2] * m[1] + q[3] * m[2]) / q[0]};
+  const real pk{q[nv]};
+  const real cGradK = q[nv + 1] * std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  const real lambda0 = abs(Uk) + cGradK;
   fp[0] = 0.5 * jac * (Uk * q[0] + lambda0 * q[0]);
   fp[1] = 0.5 * jac * (Uk * q[1] + pk * m[0] + lambda0 * q[1]);
   fp[2] = 0.5 * jac * (Uk * q[2] + pk * m[1] + lambda0 * q[2]);
@@ -239,14 +262,14 @@ void lf_split(const double* q, int nv, const double* m, double jac, double* fp, 
 // src/FieldOperation.cuh:17-105
 inline void cv_from_bv_point(Block& b, long long p) {
   const long long s = b.sz;
-  const double rho = b.bv[p];
+  const real rho = b.bv[p];
   b.cv[p] = rho;
   b.cv[p + s] = rho * b.bv[p + s];
   b.cv[p + 2 * s] = rho * b.bv[p + 2 * s];
   b.cv[p + 3 * s] = rho * b.bv[p + 3 * s];
-  const double u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
-  const double V2 = u * u + v * v + w * w;
-  double te = 0.5 * V2;
+  const real u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+  const real V2 = u * u + v * v + w * w;
+  real te = 0.5 * V2;
   te *= rho;
   te += b.bv[p + 4 * s] / (kGammaAir - 1);
   b.cv[p + 4 * s] = te;
@@ -270,11 +293,11 @@ void properties(Block& b) {
     for (int j = -g; j < b.my + g; ++j)
       for (int i = -g; i < b.mx + g; ++i) {
         const long long p = b.id(i, j, k);
-        const double t = b.bv[p + 5 * s];
-        const double u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
-        const double V = std::sqrt(u * u + v * v + w * w);
+        const real t = b.bv[p + 5 * s];
+        const real u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+        const real V = sqrt(u * u + v * v + w * w);
         b.mul[p] = sutherland(t);
-        b.mach[p] = V / std::sqrt(c_temp * t);
+        b.mach[p] = V / sqrt(c_temp * t);
       }
 }
 
@@ -291,18 +314,18 @@ void local_dt(const Ctx& c, Block& b) {
         const double gxi = std::sqrt(m(1, 1) * m(1, 1) + m(1, 2) * m(1, 2) + m(1, 3) * m(1, 3));
         const double geta = std::sqrt(m(2, 1) * m(2, 1) + m(2, 2) * m(2, 2) + m(2, 3) * m(2, 3));
         const double gzeta = std::sqrt(m(3, 1) * m(3, 1) + m(3, 2) * m(3, 2) + m(3, 3) * m(3, 3));
-        const double u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
-        const double U = u * m(1, 1) + v * m(1, 2) + w * m(1, 3);
-        const double V = u * m(2, 1) + v * m(2, 2) + w * m(2, 3);
-        const double W = u * m(3, 1) + v * m(3, 2) + w * m(3, 3);
-        const double a = std::sqrt(kGammaAir * kRu * b.bv[p + 5 * s] / kMwAir);
-        const double r0 = std::abs(U) + a * gxi;
-        const double r1 = std::abs(V) + a * geta;
-        const double r2 = std::abs(W) + a * gzeta;
-        const double sr_inv = r0 + r1 + r2;
-        const double coeff_1 = std::max(kGammaAir, 4.0 / 3.0) / b.bv[p];
-        const double coeff_2 = b.mul[p] / c.s.prandtl;
-        double sr_visc = gxi * gxi + geta * geta;
+        const real u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+        const real U = u * m(1, 1) + v * m(1, 2) + w * m(1, 3);
+        const real V = u * m(2, 1) + v * m(2, 2) + w * m(2, 3);
+        const real W = u * m(3, 1) + v * m(3, 2) + w * m(3, 3);
+        const real a = sqrt(kGammaAir * kRu * b.bv[p + 5 * s] / kMwAir);
+        const real r0 = abs(U) + a * gxi;
+        const real r1 = abs(V) + a * geta;
+        const real r2 = abs(W) + a * gzeta;
+        const real sr_inv = r0 + r1 + r2;
+        const real coeff_1 = std::max(kGammaAir, 4.0 / 3.0) / b.bv[p];
+        const real coeff_2 = b.mul[p] / c.s.prandtl;
+        real sr_visc = gxi * gxi + geta * geta;
         sr_visc += gzeta * gzeta;
         sr_visc *= coeff_1 * coeff_2;
         b.dtl[b.id0(i, j, k)] = c.s.cfl / (sr_inv + sr_visc);
@@ -324,8 +347,9 @@ void inviscid(const Ctx& c, Block& b) {
 #pragma omp parallel
     {
       const int np = m + 2 * g;
-      std::vector<double> Fp(np * nv), Fm(np * nv), kj(np * 3), fc((m + 1) * nv);
-      std::vector<double> q(nv + 2);
+      std::vector<real> Fp(np * nv), Fm(np * nv), fc((m + 1) * nv);
+      std::vector<double> kj(np * 3);
+      std::vector<real> q(nv + 2);
 #pragma omp for collapse(2) schedule(static)
       for (int bb = 0; bb < nb; ++bb)
         for (int aa = 0; aa < na; ++aa) {
@@ -338,7 +362,7 @@ void inviscid(const Ctx& c, Block& b) {
             const long long p = b.id(idx[0], idx[1], idx[2]);
             for (int l = 0; l < nv; ++l) q[l] = b.cv[p + l * s];
             q[nv] = b.bv[p + 4 * s];
-            q[nv + 1] = std::sqrt(kGammaAir * kRAir * b.bv[p + 5 * s]);
+            q[nv + 1] = sqrt(kGammaAir * kRAir * b.bv[p + 5 * s]);
             double mrow[3];
             for (int cc = 0; cc < 3; ++cc) mrow[cc] = b.metric[p + (3 * dir + cc) * s];
             const double jac = b.jac[p];
@@ -369,16 +393,16 @@ void inviscid(const Ctx& c, Block& b) {
               double eh{eps_scaled[0]};
               if (l == 1 || l == 2 || l == 3) eh = eps_scaled[1];
               else if (l == 4) eh = eps_scaled[2];
-              double f;
+              real f;
               if (scheme == 71) {
-                double vp[7], vm[7];
+                real vp[7], vm[7];
                 for (int r = 0; r < 7; ++r) {
                   vp[r] = Fp[(is - 3 + r) * nv + l];
                   vm[r] = Fm[(is - 2 + r) * nv + l];
                 }
                 f = weno7z(vp, vm, eh);
               } else {
-                double vp[5], vm[5];
+                real vp[5], vm[5];
                 for (int r = 0; r < 5; ++r) {
                   vp[r] = Fp[(is - 2 + r) * nv + l];
                   vm[r] = Fm[(is - 1 + r) * nv + l];
@@ -403,7 +427,7 @@ void inviscid(const Ctx& c, Block& b) {
 // compute_{fv,gv,hv}_2nd_order<Air, Laminar>, src/ViscousScheme.cuh:23-318,
 // :320-616, :618-909; launched per direction by viscous_flux_{fv,gv,hv},
 // src/SchemeSelector.cuh:47-154.  dq(i) += fv(i+1/2) - fv(i-1/2).
-void viscous_face(const Ctx& c, const Block& b, int dir, int i, int j, int k, double* fv) {
+void viscous_face(const Ctx& c, const Block& b, int dir, int i, int j, int k, real* fv) {
   const long long s = b.sz;
   int o[3] = {0, 0, 0};
   o[dir] = 1;
@@ -420,7 +444,7 @@ void viscous_face(const Ctx& c, const Block& b, int dir, int i, int j, int k, do
   const double zeta_z = 0.5 * (M(p, 3, 3) + M(p1, 3, 3));
   auto pv = [&](int ii, int jj, int kk, int l) { return b.bv[b.id(ii, jj, kk) + l * s]; };
   // derivative along the face normal: 2-point; tangential: 4-point average
-  double d[4][3];  // [u,v,w,T][xi,eta,zeta]
+  real d[4][3];  // [u,v,w,T][xi,eta,zeta]
   const int var[4] = {1, 2, 3, 5};
   for (int q = 0; q < 4; ++q) {
     const int l = var[q];
@@ -438,24 +462,24 @@ void viscous_face(const Ctx& c, const Block& b, int dir, int i, int j, int k, do
       d[q][2] = pv(i, j, k + 1, l) - pv(i, j, k, l);
     }
   }
-  const double u_x = d[0][0] * xi_x + d[0][1] * eta_x + d[0][2] * zeta_x;
-  const double u_y = d[0][0] * xi_y + d[0][1] * eta_y + d[0][2] * zeta_y;
-  const double u_z = d[0][0] * xi_z + d[0][1] * eta_z + d[0][2] * zeta_z;
-  const double v_x = d[1][0] * xi_x + d[1][1] * eta_x + d[1][2] * zeta_x;
-  const double v_y = d[1][0] * xi_y + d[1][1] * eta_y + d[1][2] * zeta_y;
-  const double v_z = d[1][0] * xi_z + d[1][1] * eta_z + d[1][2] * zeta_z;
-  const double w_x = d[2][0] * xi_x + d[2][1] * eta_x + d[2][2] * zeta_x;
-  const double w_y = d[2][0] * xi_y + d[2][1] * eta_y + d[2][2] * zeta_y;
-  const double w_z = d[2][0] * xi_z + d[2][1] * eta_z + d[2][2] * zeta_z;
-  const double mul = 0.5 * (b.mul[p] + b.mul[p1]);
+  const real u_x = d[0][0] * xi_x + d[0][1] * eta_x + d[0][2] * zeta_x;
+  const real u_y = d[0][0] * xi_y + d[0][1] * eta_y + d[0][2] * zeta_y;
+  const real u_z = d[0][0] * xi_z + d[0][1] * eta_z + d[0][2] * zeta_z;
+  const real v_x = d[1][0] * xi_x + d[1][1] * eta_x + d[1][2] * zeta_x;
+  const real v_y = d[1][0] * xi_y + d[1][1] * eta_y + d[1][2] * zeta_y;
+  const real v_z = d[1][0] * xi_z + d[1][1] * eta_z + d[1][2] * zeta_z;
+  const real w_x = d[2][0] * xi_x + d[2][1] * eta_x + d[2][2] * zeta_x;
+  const real w_y = d[2][0] * xi_y + d[2][1] * eta_y + d[2][2] * zeta_y;
+  const real w_z = d[2][0] * xi_z + d[2][1] * eta_z + d[2][2] * zeta_z;
+  const real mul = 0.5 * (b.mul[p] + b.mul[p1]);
   const double mut{0};
-  const double viscosity = mul + mut;
-  const double tau_xx = viscosity * (4 * u_x - 2 * v_y - 2 * w_z) / 3.0;
-  const double tau_yy = viscosity * (4 * v_y - 2 * u_x - 2 * w_z) / 3.0;
-  const double tau_zz = viscosity * (4 * w_z - 2 * u_x - 2 * v_y) / 3.0;
-  const double tau_xy = viscosity * (u_y + v_x);
-  const double tau_xz = viscosity * (u_z + w_x);
-  const double tau_yz = viscosity * (v_z + w_y);
+  const real viscosity = mul + mut;
+  const real tau_xx = viscosity * (4 * u_x - 2 * v_y - 2 * w_z) / 3.0;
+  const real tau_yy = viscosity * (4 * v_y - 2 * u_x - 2 * w_z) / 3.0;
+  const real tau_zz = viscosity * (4 * w_z - 2 * u_x - 2 * v_y) / 3.0;
+  const real tau_xy = viscosity * (u_y + v_x);
+  const real tau_xz = viscosity * (u_z + w_x);
+  const real tau_yz = viscosity * (v_z + w_y);
   const int r = dir + 1;
   const double kx = 0.5 * (M(p, r, 1) * b.jac[p] + M(p1, r, 1) * b.jac[p1]);
   const double ky = 0.5 * (M(p, r, 2) * b.jac[p] + M(p1, r, 2) * b.jac[p1]);
@@ -464,15 +488,15 @@ void viscous_face(const Ctx& c, const Block& b, int dir, int i, int j, int k, do
   fv[1] = kx * tau_xx + ky * tau_xy + kz * tau_xz;
   fv[2] = kx * tau_xy + ky * tau_yy + kz * tau_yz;
   fv[3] = kx * tau_xz + ky * tau_yz + kz * tau_zz;
-  const double um = 0.5 * (pv(i, j, k, 1) + pv(i + o[0], j + o[1], k + o[2], 1));
-  const double vm = 0.5 * (pv(i, j, k, 2) + pv(i + o[0], j + o[1], k + o[2], 2));
-  const double wm = 0.5 * (pv(i, j, k, 3) + pv(i + o[0], j + o[1], k + o[2], 3));
-  const double t_x = d[3][0] * xi_x + d[3][1] * eta_x + d[3][2] * zeta_x;
-  const double t_y = d[3][0] * xi_y + d[3][1] * eta_y + d[3][2] * zeta_y;
-  const double t_z = d[3][0] * xi_z + d[3][1] * eta_z + d[3][2] * zeta_z;
+  const real um = 0.5 * (pv(i, j, k, 1) + pv(i + o[0], j + o[1], k + o[2], 1));
+  const real vm = 0.5 * (pv(i, j, k, 2) + pv(i + o[0], j + o[1], k + o[2], 2));
+  const real wm = 0.5 * (pv(i, j, k, 3) + pv(i + o[0], j + o[1], k + o[2], 3));
+  const real t_x = d[3][0] * xi_x + d[3][1] * eta_x + d[3][2] * zeta_x;
+  const real t_y = d[3][0] * xi_y + d[3][1] * eta_y + d[3][2] * zeta_y;
+  const real t_z = d[3][0] * xi_z + d[3][1] * eta_z + d[3][2] * zeta_z;
   constexpr double cp{kGammaAir * kRu / kMwAir / (kGammaAir - 1)};
   const double prt = 0.9;  // turbulent_prandtl_number default; multiplies mut = 0
-  const double conductivity = (mul / c.s.prandtl + mut / prt) * cp;
+  const real conductivity = (mul / c.s.prandtl + mut / prt) * cp;
   fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + conductivity * (kx * t_x + ky * t_y + kz * t_z);
 }
 
@@ -487,7 +511,7 @@ void viscous(const Ctx& c, Block& b) {
     const int na = ext[oa], nb = ext[ob];
 #pragma omp parallel
     {
-      std::vector<double> fv((m + 1) * 5);
+      std::vector<real> fv((m + 1) * 5);
 #pragma omp for collapse(2) schedule(static)
       for (int bb = 0; bb < nb; ++bb)
         for (int aa = 0; aa < na; ++aa) {
@@ -522,11 +546,11 @@ void update(const Ctx& c, Block& b, double dt, int rk) {
           b.cv[p + l * s] = kRkA[rk] * b.qn[p + l * s] + kRkB[rk] * b.cv[p + l * s] +
                             kRkC[rk] * dt_div_jac * b.dq[p0 + l * n0];
         b.bv[p] = b.cv[p];
-        const double density_inv = 1.0 / b.cv[p];
+        const real density_inv = 1.0 / b.cv[p];
         b.bv[p + s] = b.cv[p + s] * density_inv;
         b.bv[p + 2 * s] = b.cv[p + 2 * s] * density_inv;
         b.bv[p + 3 * s] = b.cv[p + 3 * s] * density_inv;
-        const double V2 = b.bv[p + s] * b.bv[p + s] + b.bv[p + 2 * s] * b.bv[p + 2 * s] +
+        const real V2 = b.bv[p + s] * b.bv[p + s] + b.bv[p + 2 * s] * b.bv[p + 2 * s] +
                           b.bv[p + 3 * s] * b.bv[p + 3 * s];
         b.bv[p + 4 * s] = (kGammaAir - 1) * (b.cv[p + 4 * s] - 0.5 * b.bv[p] * V2);
         b.bv[p + 5 * s] = b.bv[p + 4 * s] * kMwAir * density_inv / kRu;
@@ -604,7 +628,7 @@ void communicate_face(Ctx& c, Block& b, const InnerFace& f) {
         const long long p = b.id(i, j, k), pt = t.id(tid[0], tid[1], tid[2]);
         if (idx[a] == shared) {
           for (int l = 0; l < 6; ++l) {
-            const double ave{0.5 * (t.bv[pt + l * ts] + b.bv[p + l * s])};
+            const real ave{0.5 * (t.bv[pt + l * ts] + b.bv[p + l * s])};
             b.bv[p + l * s] = ave;
             t.bv[pt + l * ts] = ave;
           }
@@ -624,8 +648,8 @@ void communicate(Ctx& c) {
 double global_dt(Ctx& c) {
   double dt{1e6};
   for (auto& b : c.blocks) {
-    double m{1e6};
-    for (double v : b.dtl) m = std::min(m, v);
+    double m{1e6};  // the tangent build treats dt as a constant
+    for (const real& v : b.dtl) m = std::min(m, value_of(v));
     dt = std::min(dt, m);
   }
   return dt;
@@ -708,7 +732,8 @@ int orc_add_inner_face(void* ctx, int block, int axis, int side, int target) {
 double* orc_array(void* ctx, int block, int which, long long* n_out) {
   auto* c = static_cast<Ctx*>(ctx);
   Block& b = c->blocks[block];
-  std::vector<double>* v = nullptr;
+  std::vector<real>* v = nullptr;
+  std::vector<double>* g = nullptr;
   switch (which) {
     case ORC_CV: v = &b.cv; break;
     case ORC_BV: v = &b.bv; break;
@@ -716,17 +741,22 @@ double* orc_array(void* ctx, int block, int which, long long* n_out) {
     case ORC_DQ: v = &b.dq; break;
     case ORC_MUL: v = &b.mul; break;
     case ORC_MACH: v = &b.mach; break;
-    case ORC_JAC: v = &b.jac; break;
-    case ORC_METRIC: v = &b.metric; break;
+    case ORC_JAC: g = &b.jac; break;
+    case ORC_METRIC: g = &b.metric; break;
     case ORC_DT_LOCAL: v = &b.dtl; break;
-    case ORC_X: v = &b.x; break;
-    case ORC_Y: v = &b.y; break;
-    case ORC_Z: v = &b.z; break;
+    case ORC_X: g = &b.x; break;
+    case ORC_Y: g = &b.y; break;
+    case ORC_Z: g = &b.z; break;
     case ORC_BV_LAST: v = &b.bv_last; break;
     default: return nullptr;
   }
-  if (n_out) *n_out = (long long)v->size();
-  return v->data();
+  if (g) {
+    if (n_out) *n_out = (long long)g->size();
+    return g->data();
+  }
+  // state arrays: kWidth doubles per value ({value, tangent} in the tangent build)
+  if (n_out) *n_out = (long long)v->size() * kWidth;
+  return reinterpret_cast<double*>(v->data());
 }
 
 // Analytic uniform Cartesian grid over all ngg+1 ghost layers.  The reference
@@ -785,7 +815,7 @@ int orc_compute_metric(void* ctx, int block) {
   return 0;
 }
 
-double orc_sutherland(double t) { return sutherland(t); }
+double orc_sutherland(double t) { return value_of(sutherland(t)); }
 
 // get_mixing_layer_info for air (n_spec = 0), src/MixingLayer.cu:9-190.
 // Only the branches the configurations use: velocity_ratio <= 0 with the upper
@@ -846,6 +876,67 @@ int orc_mixing_layer_ic(void* ctx, int block, const orc_ml_streams* st, double d
         }
       }
   return 0;
+}
+
+// Fourier-mode velocity perturbation of the differentiation workload (no
+// counterpart in the reference): bv[1 + comp_m] += a_m * phi_m over the padded
+// domain, phi_m = du * exp(-((y - yc)/delta)^2) * trig_m(2 pi (kx_m (x - x0)/lx +
+// kz_m (z - z0)/lz)), trig = sin (0) or cos (1).  In the tangent build the
+// amplitudes carry the tangents amp_dot.
+int orc_fourier_perturbation(void* ctx, int block, const orc_fourier* f) {
+  auto* c = static_cast<Ctx*>(ctx);
+  Block& b = c->blocks[block];
+  const int g = b.g;
+  const long long s = b.sz;
+  const double two_pi = 6.283185307179586;
+  for (int m = 0; m < f->n_modes; ++m)
+    if (f->comp[m] < 0 || f->comp[m] > 2) return -1;
+  for (int k = -g; k < b.mz + g; ++k)
+    for (int j = -g; j < b.my + g; ++j)
+      for (int i = -g; i < b.mx + g; ++i) {
+        const long long p = b.id(i, j, k), q = b.idg(i, j, k);
+        const double x = b.x[q], y = b.y[q], z = b.z[q];
+        const double yr = (y - f->yc) / f->delta;
+        const double env = f->du * std::exp(-yr * yr);
+        for (int m = 0; m < f->n_modes; ++m) {
+          const double ph = two_pi * (f->kx[m] * (x - f->x0) / f->lx + f->kz[m] * (z - f->z0) / f->lz);
+          const double phi = env * (f->trig[m] ? std::cos(ph) : std::sin(ph));
+#ifdef ORC_DUAL
+          const real a(f->amp[m], f->amp_dot ? f->amp_dot[m] : 0.0);
+#else
+          const real a = f->amp[m];
+#endif
+          b.bv[p + (1 + f->comp[m]) * s] += a * phi;
+        }
+      }
+  return 0;
+}
+
+// Momentum thickness of the differentiation workload (no counterpart in the
+// reference): theta = sum_j w_j rho_j (u1 - U_j)(U_j - u2) / (rho0 du^2), where
+// rho_j = <rho>, U_j = <rho u>/<rho> average over the nodes i < ni, k < nk of
+// row j, and w_j = dy (dy/2 at j = 0 and my-1, the trapezoid rule).
+double orc_momentum_thickness(void* ctx, int block, double u1, double u2, double rho0, double du, double dy,
+                              int ni, int nk, double* dtheta) {
+  auto* c = static_cast<Ctx*>(ctx);
+  const Block& b = c->blocks[block];
+  const long long s = b.sz;
+  real theta = 0.0;
+  for (int j = 0; j < b.my; ++j) {
+    real s0 = 0.0, s1 = 0.0;
+    for (int k = 0; k < nk; ++k)
+      for (int i = 0; i < ni; ++i) {
+        const long long p = b.id(i, j, k);
+        s0 += b.cv[p];
+        s1 += b.cv[p + s];
+      }
+    const real rho = s0 / (double)((long long)ni * nk);
+    const real U = s1 / s0;
+    const double w = (j == 0 || j == b.my - 1) ? 0.5 * dy : dy;
+    theta += w * rho * (u1 - U) * (U - u2) / (rho0 * du * du);
+  }
+  if (dtheta) *dtheta = tangent_of(theta);
+  return value_of(theta);
 }
 
 int orc_cv_from_bv(void* ctx, int block) { cv_from_bv(static_cast<Ctx*>(ctx)->blocks[block]); return 0; }
@@ -950,9 +1041,9 @@ int orc_store_last_step(void* ctx) {
       for (int j = 0; j < b.my; ++j)
         for (int i = 0; i < b.mx; ++i) {
           const long long p = b.id(i, j, k), q = b.id0(i, j, k);
-          const double u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+          const real u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
           b.bv_last[q] = b.bv[p];
-          b.bv_last[q + n0] = std::sqrt(u * u + v * v + w * w);
+          b.bv_last[q + n0] = sqrt(u * u + v * v + w * w);
           b.bv_last[q + 2 * n0] = b.bv[p + 4 * s];
           b.bv_last[q + 3 * n0] = b.bv[p + 5 * s];
         }
@@ -964,7 +1055,7 @@ int orc_store_last_step(void* ctx) {
 // src/Residual.cuh:75-122 (unscaled: res_scale division is left to the caller).
 int orc_residual(void* ctx, double res[4]) {
   auto* c = static_cast<Ctx*>(ctx);
-  double acc[4] = {0, 0, 0, 0};
+  real acc[4] = {0, 0, 0, 0};
   long long n_total = 0;
   for (auto& b : c->blocks) {
     const long long s = b.sz, n0 = b.n0();
@@ -973,23 +1064,45 @@ int orc_residual(void* ctx, double res[4]) {
       for (int j = 0; j < b.my; ++j)
         for (int i = 0; i < b.mx; ++i) {
           const long long p = b.id(i, j, k), q = b.id0(i, j, k);
-          const double u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
-          const double vel = std::sqrt(u * u + v * v + w * w);
+          const real u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+          const real vel = sqrt(u * u + v * v + w * w);
           acc[0] += (b.bv[p] - b.bv_last[q]) * (b.bv[p] - b.bv_last[q]);
           acc[1] += (vel - b.bv_last[q + n0]) * (vel - b.bv_last[q + n0]);
           acc[2] += (b.bv[p + 4 * s] - b.bv_last[q + 2 * n0]) * (b.bv[p + 4 * s] - b.bv_last[q + 2 * n0]);
           acc[3] += (b.bv[p + 5 * s] - b.bv_last[q + 3 * n0]) * (b.bv[p + 5 * s] - b.bv_last[q + 3 * n0]);
         }
   }
-  for (int l = 0; l < 4; ++l) res[l] = std::sqrt(acc[l] / (double)n_total);
+  for (int l = 0; l < 4; ++l) res[l] = std::sqrt(value_of(acc[l]) / (double)n_total);
   return 0;
 }
 
-double orc_weno5z(const double* vp, const double* vm, double eps) { return weno5z(vp, vm, eps); }
-double orc_weno7z(const double* vp, const double* vm, double eps) { return weno7z(vp, vm, eps); }
-double orc_weno5js(const double* vp, const double* vm, double eps) { return weno5js(vp, vm, eps); }
-void orc_lf_split(const double* q, int n_var, const double* k3, double jac, double* fp, double* fm) {
-  lf_split(q, n_var, k3, jac, fp, fm);
+double orc_weno5z(const double* vp, const double* vm, double eps) {
+  real a[5], b[5];
+  std::copy(vp, vp + 5, a);
+  std::copy(vm, vm + 5, b);
+  return value_of(weno5z(a, b, eps));
 }
+double orc_weno7z(const double* vp, const double* vm, double eps) {
+  real a[7], b[7];
+  std::copy(vp, vp + 7, a);
+  std::copy(vm, vm + 7, b);
+  return value_of(weno7z(a, b, eps));
+}
+double orc_weno5js(const double* vp, const double* vm, double eps) {
+  real a[5], b[5];
+  std::copy(vp, vp + 5, a);
+  std::copy(vm, vm + 5, b);
+  return value_of(weno5js(a, b, eps));
+}
+void orc_lf_split(const double* q, int n_var, const double* k3, double jac, double* fp, double* fm) {
+  std::vector<real> qq(q, q + n_var + 2), p(n_var), m(n_var);
+  lf_split(qq.data(), n_var, k3, jac, p.data(), m.data());
+  for (int l = 0; l < n_var; ++l) {
+    fp[l] = value_of(p[l]);
+    fm[l] = value_of(m[l]);
+  }
+}
+
+int orc_scalar_width(void) { return kWidth; }
 
 }  // extern "C"

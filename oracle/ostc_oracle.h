@@ -76,6 +76,30 @@ int     orc_mixing_layer_ic(void* ctx, int block, const orc_ml_streams* st, doub
                             int gi0, int gj0, int gk0, int period_i, int period_j, int period_k,
                             uint64_t seed, double amplitude);
 
+/* Fourier-mode velocity perturbation of the differentiation workload:
+ * bv[1 + comp[m]] += amp[m] * du * exp(-((y - yc)/delta)^2)
+ *                    * (trig[m] ? cos : sin)(2 pi (kx[m] (x - x0)/lx + kz[m] (z - z0)/lz))
+ * over the padded domain (coordinates from the grid arrays).  amp_dot (may be
+ * NULL) gives the amplitude tangents in the tangent build. */
+typedef struct {
+  int           n_modes;
+  const int*    comp;
+  const int*    kx;
+  const int*    kz;
+  const int*    trig;
+  const double* amp;
+  const double* amp_dot;
+  double        x0, z0, lx, lz, delta, yc, du;
+} orc_fourier;
+int     orc_fourier_perturbation(void* ctx, int block, const orc_fourier* f);
+/* theta = sum_j w_j <rho>_j (u1 - U_j)(U_j - u2) / (rho0 du^2), U_j = <rho u>_j/<rho>_j,
+ * averages over nodes i < ni, k < nk; w_j = dy, dy/2 at the two end rows.
+ * dtheta (may be NULL) receives the tangent in the tangent build, else 0. */
+double  orc_momentum_thickness(void* ctx, int block, double u1, double u2, double rho0, double du,
+                               double dy, int ni, int nk, double* dtheta);
+/* 1: state arrays hold doubles; 2: {value, tangent} pairs (tangent build) */
+int     orc_scalar_width(void);
+
 /* RK3 prologue/epilogue and whole steps (src/Driver.cu:115-199, src/RK.cuh:20-228) */
 int     orc_initialize(void* ctx);
 int     orc_step(void* ctx, int n_steps, double* last_dt);

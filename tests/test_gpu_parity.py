@@ -141,3 +141,33 @@ def test_batched_launch_configs_match_oracle(split):
         e = rel_err(interior(s.get(b, pkg.FIELD_BV), g), interior(o.array(b, O.BV), g))
         assert np.all(e <= BV_TOL), (b, e)
     s.close()
+
+
+_C0_CACHE = {}
+
+
+def _config0_oracle(scheme):
+    """Oracle state after 100 SSP-RK3 steps of the 64^3 single-block mixing
+    layer (periodic x/z, outflow y, CFL 0.5)."""
+    if scheme not in _C0_CACHE:
+        case = cases.single_block_case(64, scheme=scheme)
+        o = oracle_for(case)
+        dt = o.step(100)
+        _C0_CACHE[scheme] = (case, dt, o.array(0, O.BV).copy())
+    return _C0_CACHE[scheme]
+
+
+@pytest.mark.parametrize("scheme", [51, 50], ids=["wenoz5", "weno5js"])
+@pytest.mark.parametrize("mode", [pkg.MODE_PLAIN, pkg.MODE_BATCHED], ids=["plain", "batched"])
+def test_single_block_64_cubed_100_steps(scheme, mode):
+    print(f"[gpu] 64^3, 100 steps, scheme {scheme}, mode {mode}", flush=True)
+    case, dt_o, bv_o = _config0_oracle(scheme)
+    s = _solver(case)
+    s.initialize()
+    dt_g = s.rk3(100, mode)
+    g = s.ngg
+    e = rel_err(interior(s.get(0, pkg.FIELD_BV), g), interior(bv_o, g))
+    print(f"  dt gpu {dt_g:.15e} oracle {dt_o:.15e}; primitive rel err {e}", flush=True)
+    assert abs(dt_g - dt_o) <= 1e-12 * dt_o
+    assert np.all(e <= BV_TOL), e
+    s.close()
