@@ -24,6 +24,8 @@
 #ifndef OPENSTC_ABI_H
 #define OPENSTC_ABI_H
 #include <stdint.h>
+
+#include "openstc/mech.h"
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -245,6 +247,102 @@ int ostc_adjoint_forward(ostc_ctx ctx, int n_steps, int checkpoint_every, const 
 int ostc_adjoint_backward(ostc_ctx ctx, const ostc_fourier* f, double* grad);
 /* Milliseconds of the last forward and backward calls (HIP events). */
 int ostc_adjoint_timing(ostc_ctx ctx, double* forward_ms, double* backward_ms);
+
+/* ---------------------------------------------------- finite-rate mixtures
+ * The reference's MixtureModel::FR instantiation of the same operators
+ * (species = 1, reaction = 0/1; laminar): NASA-7 thermodynamics,
+ * mixture-averaged transport (src/Transport.cu:43-136), finite-rate
+ * chemistry (src/FiniteRateChem.cu:7-195), the temperature Newton iteration
+ * (src/FieldOperation.cu:3-40), component-wise (51) and characteristic (52)
+ * WENO-Z5 with Lax-Friedrichs-type splitting (src/WENO.cu:697-1143,
+ * :1455-1536), the positivity-preserving limiter (:1538-1580) and viscous
+ * fluxes with species diffusion (src/ViscousScheme.cuh:23-909).
+ * n_var = 5 + n_spec; the species counts compiled into the library are
+ * reported by ostc_fr_supported (others return OSTC_ERR_UNSUPPORTED).
+ *
+ * Extra planes of a block (reference layout): sv [n_spec] (mass fractions),
+ * thermal conductivity, cp, gamma, acoustic speed, rho_D [n_spec]; cv/qn/dq
+ * carry n_var planes.  All state stays on the device; a step makes no host
+ * synchronisation, so OSTC_FR_MODE_GRAPH captures it once and replays it. */
+typedef struct ostc_fr_ctx_s* ostc_fr_ctx;
+
+typedef struct {
+  int inviscid_scheme;       /* 51 WENO-Z5 component-wise, 52 WENO-Z5 characteristic */
+  int viscous_order;         /* 0 or 2 */
+  int reaction;              /* 1: finite-rate chemistry source */
+  int positive_preserving;   /* positive_preserving: limiter on the species fluxes */
+  int grad_p_diffusion;      /* gradPInDiffusionFlux */
+  int fixed_time_step;
+  double dt, cfl;
+  double prandtl;            /* prandtl_number (viscous spectral radius of the time step) */
+  double rho_ref, v_ref;     /* reference state: weno_eps_scale = (rho_ref v_ref)^2 */
+} ostc_fr_setup;
+
+#define OSTC_FR_FIELD_CV 0
+#define OSTC_FR_FIELD_BV 1
+#define OSTC_FR_FIELD_SV 2
+#define OSTC_FR_FIELD_QN 3
+#define OSTC_FR_FIELD_DQ 4
+#define OSTC_FR_FIELD_MUL 5
+#define OSTC_FR_FIELD_LAMBDA 6
+#define OSTC_FR_FIELD_CP 7
+#define OSTC_FR_FIELD_GAMMA 8
+#define OSTC_FR_FIELD_C 9
+#define OSTC_FR_FIELD_MACH 10
+#define OSTC_FR_FIELD_RHOD 11
+#define OSTC_FR_FIELD_JAC 12
+#define OSTC_FR_FIELD_METRIC 13
+#define OSTC_FR_FIELD_DT_LOCAL 14
+
+#define OSTC_FR_MODE_DIRECT 0   /* kernels launched one by one */
+#define OSTC_FR_MODE_GRAPH 1    /* one step captured in a hipGraph, replayed */
+#define OSTC_FR_MODE_TIMED 2    /* direct, with HIP events around every kernel (ostc_fr_kernel_ms) */
+
+/* kernel kinds of ostc_fr_kernel_ms */
+#define OSTC_FR_K_SOURCE 0
+#define OSTC_FR_K_INVISCID_X 1
+#define OSTC_FR_K_INVISCID_Y 2
+#define OSTC_FR_K_INVISCID_Z 3
+#define OSTC_FR_K_VISCOUS_X 4
+#define OSTC_FR_K_VISCOUS_Y 5
+#define OSTC_FR_K_VISCOUS_Z 6
+#define OSTC_FR_K_UPDATE 7
+#define OSTC_FR_K_PROPS 8
+#define OSTC_FR_K_COUNT 9
+
+int ostc_fr_supported(int n_spec);  /* 1 if kernels for n_spec species are compiled in */
+/* Driver/Field construction for FR (src/Driver.cu:12-113, src/Field.cu:469-647) */
+int ostc_fr_create(const ostc_fr_setup* s, const ostc_mech_tables* mech, void* hip_stream, ostc_fr_ctx* out);
+int ostc_fr_destroy(ostc_fr_ctx ctx);
+int ostc_fr_add_block(ostc_fr_ctx ctx, const ostc_block_desc* d, int* block_id);
+int ostc_fr_add_boundary(ostc_fr_ctx ctx, int block, int axis, int side, int type);  /* outflow, periodic */
+int ostc_fr_add_inner_face(ostc_fr_ctx ctx, int block, int axis, int side, int target_block);
+int ostc_fr_finalize(ostc_fr_ctx ctx);
+int ostc_fr_set_geometry(ostc_fr_ctx ctx, int block, const double* jac, const double* metric, long long elem_stride,
+                         long long comp_stride, long long offset);
+int ostc_fr_set_field(ostc_fr_ctx ctx, int block, int field, const double* host);
+int ostc_fr_get_field(ostc_fr_ctx ctx, int block, int field, double* host);
+/* initialize_computation + compute_cv_from_bv for FR: cv from bv/sv, boundary
+ * conditions, inner-face exchange, properties */
+int ostc_fr_initialize(ostc_fr_ctx ctx);
+/* n_steps SSP-RK3 steps (src/RK.cuh:85-226, MixtureModel::FR) */
+int ostc_fr_step(ostc_fr_ctx ctx, int n_steps, int mode, double* last_dt);
+/* stage pieces: dq = J omega (source); inviscid x, y, z; viscous x, y, z,
+ * with the dt the positivity limiter reads (src/RK.cuh:113-139) */
+int ostc_fr_stage_rhs(ostc_fr_ctx ctx, int block, double dt);
+int ostc_fr_compute_dt(ostc_fr_ctx ctx, double* dt_out);   /* local_time_step + global MIN */
+int ostc_fr_stage_update(ostc_fr_ctx ctx, int block, int rk, double dt);
+int ostc_fr_update_properties(ostc_fr_ctx ctx, int block);
+/* timing of the last ostc_fr_step: device ms and launches; per kernel kind
+ * (OSTC_FR_K_*) the summed ms and launch counts of the last TIMED call */
+int ostc_fr_last_step_timing(ostc_fr_ctx ctx, double* ms, long long* launches);
+int ostc_fr_kernel_ms(ostc_fr_ctx ctx, double ms[OSTC_FR_K_COUNT], long long counts[OSTC_FR_K_COUNT]);
+int ostc_fr_sync(ostc_fr_ctx ctx);
+const char* ostc_fr_last_error(ostc_fr_ctx ctx);
+/* species of the mixing-layer initial state: sv[l] = y >= 0 ? y_upper[l] : y_lower[l]
+ * over the padded domain, y = y0 + j dy (src/Initialize.cu:259-325) */
+int ostc_util_mixing_layer_sv(int mx, int my, int mz, int ngg, int n_spec, double y0, double dy,
+                              const double* y_upper, const double* y_lower, double* sv);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,8 @@ Python callers (tests, benchmarks, the adjoint wrapper):
 * ``Solver.data_communication()``                      -- src/DataCommunication.cuh:29-58
 * ``Solver.update_physical_properties(b)``              -- src/FieldOperation.cuh:107-140
 * ``Solver.compute_residual()``                        -- src/Residual.cuh:75-187
+* ``mixture.FrSolver`` -- the same operators for finite-rate mixtures
+  (MixtureModel::FR), ``mixture.load_mechanism`` -- src/ChemData.cpp:11-948
 
 Where the reference aborts the run (MPI_Abort after cudaGetLastError), these
 raise ``OstcError`` carrying the library's message.  There is no CPU
@@ -139,6 +141,10 @@ SIGNATURES = {
     "ostc_adjoint_backward": (_I, [_P, C.POINTER(Fourier), _DP]),
     "ostc_adjoint_timing": (_I, [_P, _DP, _DP]),
 }
+
+# the finite-rate mixture path (MixtureModel::FR): FrSolver, load_mechanism
+mixture = importlib.import_module(__name__ + ".mixture")
+SIGNATURES.update(mixture.SIGNATURES)
 
 
 def lib():

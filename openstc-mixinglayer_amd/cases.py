@@ -62,27 +62,10 @@ def _cuts(lengths):
     return off
 
 
-def mixing_layer_case(x_cells, y_cells, z_cells, *, re_delta=500.0, ma_c=0.4,
-                      lx_over_delta=20.0, ly_over_delta=40.0, lz_over_delta=20.0,
-                      scheme=51, viscous_order=2, cfl=0.5, prandtl=0.72,
-                      seed=42, perturbation=1e-3, y_bc=BC_OUTFLOW):
-    """Temporal mixing layer of air on a tensor-product partition.
-
-    ``x_cells`` etc. are lists of per-block cell counts along each axis
-    (a block with n cells has n+1 nodes; neighbours share the interface
-    node).  x and z are periodic with period = total cells (node N-1 is
-    node 0); y uses ``y_bc`` on both ends.  Re_delta = rho*dU*delta/mu
-    fixes delta_omega; dU = u_upper - u_lower = 2*Mc*c for equal
-    temperatures.
-    """
-    st = air_streams(ma_c=ma_c, ma_upper=ma_c)
-    rho, t = st["upper"][0], st["upper"][5]
-    du = st["upper"][1] - st["lower"][1]
-    delta = re_delta * sutherland(t) / (rho * du)
-    nx, ny, nz = sum(x_cells), sum(y_cells), sum(z_cells)
-    dx = lx_over_delta * delta / nx
-    dy = ly_over_delta * delta / ny
-    dz = lz_over_delta * delta / nz
+def _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc):
+    """Blocks and faces of a tensor-product partition: x and z periodic
+    (inner faces across the period when there are several blocks along the
+    axis, a periodic face otherwise), y bounded by ``y_bc`` faces."""
     ox, oy, oz = _cuts(x_cells), _cuts(y_cells), _cuts(z_cells)
     blocks, bidx = [], {}
     for kb in range(len(z_cells)):
@@ -111,6 +94,31 @@ def mixing_layer_case(x_cells, y_cells, z_cells, *, re_delta=500.0, ma_c=0.4,
                     boundaries.append(dict(block=b, axis=axis, side=side, type=BC_PERIODIC))
                 else:
                     boundaries.append(dict(block=b, axis=axis, side=side, type=y_bc))
+    return blocks, boundaries, inner
+
+
+def mixing_layer_case(x_cells, y_cells, z_cells, *, re_delta=500.0, ma_c=0.4,
+                      lx_over_delta=20.0, ly_over_delta=40.0, lz_over_delta=20.0,
+                      scheme=51, viscous_order=2, cfl=0.5, prandtl=0.72,
+                      seed=42, perturbation=1e-3, y_bc=BC_OUTFLOW):
+    """Temporal mixing layer of air on a tensor-product partition.
+
+    ``x_cells`` etc. are lists of per-block cell counts along each axis
+    (a block with n cells has n+1 nodes; neighbours share the interface
+    node).  x and z are periodic with period = total cells (node N-1 is
+    node 0); y uses ``y_bc`` on both ends.  Re_delta = rho*dU*delta/mu
+    fixes delta_omega; dU = u_upper - u_lower = 2*Mc*c for equal
+    temperatures.
+    """
+    st = air_streams(ma_c=ma_c, ma_upper=ma_c)
+    rho, t = st["upper"][0], st["upper"][5]
+    du = st["upper"][1] - st["lower"][1]
+    delta = re_delta * sutherland(t) / (rho * du)
+    nx, ny, nz = sum(x_cells), sum(y_cells), sum(z_cells)
+    dx = lx_over_delta * delta / nx
+    dy = ly_over_delta * delta / ny
+    dz = lz_over_delta * delta / nz
+    blocks, boundaries, inner = _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc)
     # face order inside a block: as registered above (x-, x+, y-, y+, z-, z+)
     return dict(
         n_var=5, inviscid_scheme=scheme, viscous_order=viscous_order, cfl=cfl, prandtl=prandtl,
@@ -201,3 +209,112 @@ def fourier_case(n=128, n_steps=50, checkpoint_every=10, seed=3, n_modes=32, sch
     case["n_steps"] = n_steps
     case["checkpoint_every"] = checkpoint_every
     return case
+
+
+# ------------------------------------------------------ finite-rate mixtures
+def nasa_cp(tables, t):
+    """Species cp [J/(kg K)] at t (compute_cp, Combustion2Part,
+    src/ChemData.cpp:150-176) from mechanism tables."""
+    ns = tables.n_spec
+    cp = np.zeros(ns)
+    for i in range(ns):
+        tt = max(t, tables.t_low[i])
+        c = tables.low_coeff[i] if tt < tables.t_mid[i] else tables.high_coeff[i]
+        cp[i] = (c[0] + c[1] * tt + c[2] * tt ** 2 + c[3] * tt ** 3 + c[4] * tt ** 4) * R_U / tables.mw[i]
+    return cp
+
+
+def mixture_viscosity(tables, t, y):
+    """Wilke mixture viscosity of the host routine compute_viscosity,
+    src/Transport.cu:11-41."""
+    ns = tables.n_spec
+    mw = np.array(tables.mw[:ns])
+    mw_total = 1.0 / np.sum(np.asarray(y) / mw)
+    x = np.asarray(y) * mw_total / mw
+    vis = np.zeros(ns)
+    for i in range(ns):
+        t_dl = t * tables.lj_potent_inv[i]
+        vis[i] = tables.vis_coeff[i] * math.sqrt(t) / (1.147 * t_dl ** -0.145 + (t_dl + 0.5) ** -2)
+    mu = 0.0
+    for i in range(ns):
+        s = 0.0
+        for j in range(ns):
+            if i == j:
+                phi = 1.0
+            else:
+                num = 1 + math.sqrt(vis[i] / vis[j]) * tables.wj_div_wi_to_one4th[i][j]
+                phi = num * num * tables.sqrt_wi_div_wj_pl1_mul8[i][j]
+            s += phi * x[j]
+        mu += vis[i] * x[i] / s
+    return mu
+
+
+def mixture_streams(tables, y_upper, y_lower, t_upper, t_lower, p_upper=101325.0, p_lower=101325.0, ma_c=0.5):
+    """Stream states [rho,u,v,w,p,T] of a two-stream mixture,
+    get_mixing_layer_info with species (src/MixingLayer.cu:35-110),
+    ``upper_stream.mach`` = Mc, ``upper_faster`` = 1, so u_lower = -Mc c_lower
+    and the convective Mach number is Mc."""
+    out = {}
+    for side, y, t, p in (("upper", y_upper, t_upper, p_upper), ("lower", y_lower, t_lower, p_lower)):
+        y = np.asarray(y, dtype=np.float64)
+        mw_inv = float(np.sum(y / np.array(tables.mw[:tables.n_spec])))
+        cp = float(np.sum(nasa_cp(tables, t) * y))
+        gam = cp / (cp - R_U * mw_inv)
+        c = math.sqrt(gam * R_U * mw_inv * t)
+        out["c_" + side] = c
+        out[side] = [p / (R_U * mw_inv * t), 0.0, 0.0, 0.0, p, t]
+    out["upper"][1] = ma_c * out["c_upper"]
+    ma_l = (ma_c - ma_c) * out["c_upper"] / out["c_lower"] - ma_c
+    out["lower"][1] = ma_l * out["c_lower"]
+    return out
+
+
+def species_vector(tables, composition: dict):
+    names = [tables.spec_name[i].value.decode() for i in range(tables.n_spec)]
+    y = np.zeros(len(names))
+    for k, v in composition.items():
+        y[names.index(k)] = v
+    return list(y)
+
+
+FUEL_H2_N2 = {"H2": 0.05, "N2": 0.95}          # fuel stream, 300 K
+OXIDIZER_AIR = {"O2": 0.232, "N2": 0.768}      # oxidizer stream, 1500 K
+
+
+def reacting_case(x_cells, y_cells, z_cells, tables, *, mech_files=None, reaction=1, re_delta=500.0, ma_c=0.5,
+                  fuel=FUEL_H2_N2, oxidizer=OXIDIZER_AIR, t_fuel=300.0, t_oxidizer=1500.0,
+                  lx_over_delta=20.0, ly_over_delta=40.0, lz_over_delta=20.0, scheme=52, viscous_order=2,
+                  cfl=0.5, prandtl=0.72, positive_preserving=0, seed=42, perturbation=1e-3, y_bc=BC_OUTFLOW):
+    """Temporal H2/air mixing layer on a tensor-product partition: the fuel
+    (H2 in N2) stream above y = 0, the oxidizer (air) stream below, tanh
+    velocity profile with convective Mach ``ma_c``, species and temperature
+    steps at y = 0 (src/Initialize.cu:259-325), seeded velocity noise.
+    delta_omega follows from Re_delta = rho dU delta / mu of the fuel stream.
+    The reference's spatially developing variant needs its inflow boundary,
+    which this path does not include; the temporal layer uses periodic x/z."""
+    y_up = species_vector(tables, fuel)
+    y_lo = species_vector(tables, oxidizer)
+    st = mixture_streams(tables, y_up, y_lo, t_fuel, t_oxidizer, ma_c=ma_c)
+    rho = st["upper"][0]
+    du = st["upper"][1] - st["lower"][1]
+    delta = re_delta * mixture_viscosity(tables, t_fuel, y_up) / (rho * du)
+    nx, ny, nz = sum(x_cells), sum(y_cells), sum(z_cells)
+    dx = lx_over_delta * delta / nx
+    dy = ly_over_delta * delta / ny
+    dz = lz_over_delta * delta / nz
+    blocks, boundaries, inner = _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc)
+    ns = tables.n_spec
+    u1 = abs(st["upper"][1])
+    return dict(
+        n_var=5 + ns, n_spec=ns, inviscid_scheme=scheme, viscous_order=viscous_order, reaction=reaction,
+        positive_preserving=positive_preserving, cfl=cfl, prandtl=prandtl, fixed_time_step=0, dt=0.0,
+        rho_ref=rho, v_ref=u1 if u1 >= 1 else st["c_upper"],
+        streams=st, y_upper=y_up, y_lower=y_lo, delta_omega=delta, dx=dx, dy=dy, dz=dz,
+        periods=(nx, 0, nz), seed=seed, amplitude=perturbation * du, mech_files=mech_files,
+        blocks=blocks, boundaries=boundaries, inner_faces=inner,
+        n_cells=sum(b["mx"] * b["my"] * b["mz"] for b in blocks))
+
+
+def reacting_single_block(nx, ny, nz, tables, **kw):
+    """One block of nx x ny x nz nodes, periodic x/z, outflow y."""
+    return reacting_case([nx - 1], [ny - 1], [nz - 1], tables, **kw)

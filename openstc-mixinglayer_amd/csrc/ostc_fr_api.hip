@@ -1,0 +1,524 @@
+// ostc_fr_api.hip -- host runtime behind the finite-rate mixture entry points
+// of the C ABI (ostc_fr_*, include/openstc/abi.h): block and face registry,
+// device memory in the reference layout, the RK3 step of
+// RK3<MixtureModel::FR> (src/RK.cuh:85-226) with the dt kept on the device,
+// hipGraph capture of a step, and per-kernel event timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "openstc/abi.h"
+#include "ostc_fr.hpp"
+
+using namespace ostc::fr;
+
+namespace {
+
+struct FaceDesc { int axis, side, type; };
+struct InnerDesc { int axis, side, target; };
+
+struct HBlock {
+  int mx, my, mz;
+  FrBlockDev d{};
+  std::vector<FaceDesc> faces;
+  std::vector<InnerDesc> inner;
+};
+
+constexpr double kRkA[3] = {1.0, 0.75, 1.0 / 3.0};  // src/RK.cuh:10-14
+constexpr double kRkB[3] = {0.0, 0.25, 2.0 / 3.0};
+constexpr double kRkC[3] = {1.0, 0.25, 2.0 / 3.0};
+
+}  // namespace
+
+struct ostc_fr_ctx_s {
+  ostc_fr_setup s{};
+  ostc_mech_tables tab{};
+  Tab* d_tab = nullptr;
+  int ns = 0, nv = 5, ngg = 3;
+  hipStream_t st = nullptr;
+  bool own_stream = false;
+  std::vector<HBlock> blocks;
+  bool finalized = false;
+  double* d_dt = nullptr;          // dt of the current step (one double)
+  double weno_eps_scale = 1;
+  std::vector<void*> allocs;
+  std::string err;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long long graph_launches = 0;
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  double last_ms = 0;
+  long long last_launches = 0;
+  KernelTimer* timer = nullptr;
+  double kms[KT_COUNT] = {};
+  long long kcount[KT_COUNT] = {};
+};
+
+namespace {
+
+int fail(ostc_fr_ctx c, int code, const std::string& m) {
+  if (c) c->err = m;
+  return code;
+}
+
+int hipfail(ostc_fr_ctx c, hipError_t e, const char* what) {
+  return fail(c, OSTC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define FR_HIP(c, call)                                  \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) return hipfail((c), e_, #call); \
+  } while (0)
+
+// Device allocation zeroed on the context's stream.
+int dalloc(ostc_fr_ctx c, double** p, size_t n) {
+  FR_HIP(c, hipMalloc((void**)p, n * sizeof(double)));
+  c->allocs.push_back(*p);
+  FR_HIP(c, hipMemsetAsync(*p, 0, n * sizeof(double), c->st));
+  return OSTC_OK;
+}
+
+FrArgs args(ostc_fr_ctx c, int b) {
+  FrArgs a{};
+  a.B = c->blocks[b].d;
+  a.T = c->d_tab;
+  a.scheme = c->s.inviscid_scheme;
+  a.viscous = c->s.viscous_order != 0;
+  a.reaction = c->s.reaction != 0;
+  a.pp = c->s.positive_preserving != 0;
+  a.grad_p = c->s.grad_p_diffusion != 0;
+  a.cfl = c->s.cfl;
+  a.prandtl = c->s.prandtl;
+  a.weno_eps_scale = c->weno_eps_scale;
+  a.v_ref = c->s.v_ref;
+  a.dt = c->d_dt;
+  return a;
+}
+
+int check_ready(ostc_fr_ctx c) {
+  if (!c) return OSTC_ERR_ARG;
+  if (!c->finalized) return fail(c, OSTC_ERR_STATE, "context not finalized");
+  return OSTC_OK;
+}
+
+int bc_priority(int type) { return type == OSTC_BC_OUTFLOW ? 0 : 1; }  // src/BoundCond.cuh:1488-1678
+
+// Boundary conditions of one block (priority order, then registration
+// order), src/BoundCond.cuh:1488-1678.
+void apply_bc(ostc_fr_ctx c, int b, long long* n) {
+  std::vector<FaceDesc> fs = c->blocks[b].faces;
+  std::stable_sort(fs.begin(), fs.end(),
+                   [](const FaceDesc& x, const FaceDesc& y) { return bc_priority(x.type) < bc_priority(y.type); });
+  const FrArgs a = args(c, b);
+  for (const FaceDesc& f : fs) launch_bc(a, c->ns, f.axis, f.side, f.type, c->st, n);
+}
+
+// data_communication, inner faces in block order then face order
+// (src/DataCommunication.cuh:29-58).
+void exchange(ostc_fr_ctx c, long long* n) {
+  for (size_t b = 0; b < c->blocks.size(); ++b) {
+    const FrArgs a = args(c, (int)b);
+    for (const InnerDesc& f : c->blocks[b].inner)
+      launch_comm(a, c->blocks[f.target].d, c->ns, f.axis, f.side, c->st, n);
+  }
+}
+
+// One SSP-RK3 step, src/RK.cuh:85-226 (MixtureModel::FR, laminar).
+void step(ostc_fr_ctx c, long long* n, KernelTimer* tm) {
+  const int nb = (int)c->blocks.size();
+  if (c->s.fixed_time_step) launch_dt_set(c->d_dt, c->s.dt, c->st, n);
+  for (int b = 0; b < nb; ++b) {  // qn <- cv
+    const FrBlockDev& d = c->blocks[b].d;
+    (void)hipMemcpyAsync(d.qn, d.cv, (size_t)d.sz * c->nv * sizeof(double), hipMemcpyDeviceToDevice, c->st);
+    ++*n;
+  }
+  for (int rk = 0; rk < 3; ++rk) {
+    if (!c->s.fixed_time_step && rk == 0) {
+      // dt = global MIN of local_time_step (properties of the step's state)
+      launch_dt_set(c->d_dt, 1e6, c->st, n);
+      for (int b = 0; b < nb; ++b)
+        launch_local_dt(args(c, b), reinterpret_cast<unsigned long long*>(c->d_dt), c->st, n);
+    }
+    for (int b = 0; b < nb; ++b) {
+      const FrArgs a = args(c, b);
+      launch_rhs(a, c->ns, c->st, n, tm);
+      launch_update(a, c->ns, kRkA[rk], kRkB[rk], kRkC[rk], c->st, n, tm);
+      apply_bc(c, b, n);
+    }
+    exchange(c, n);
+    for (int b = 0; b < nb; ++b) launch_props(args(c, b), c->ns, c->st, n, tm);
+  }
+}
+
+void invalidate_graph(ostc_fr_ctx c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+}
+
+int build_graph(ostc_fr_ctx c) {
+  if (c->gexec) return OSTC_OK;
+  long long n = 0;
+  FR_HIP(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  step(c, &n, nullptr);
+  hipGraph_t g = nullptr;
+  FR_HIP(c, hipStreamEndCapture(c->st, &g));
+  c->graph = g;
+  FR_HIP(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+  c->graph_launches = n;
+  return OSTC_OK;
+}
+
+int field_ptr(ostc_fr_ctx c, int b, int f, double** p, size_t* n) {
+  const FrBlockDev& d = c->blocks[b].d;
+  const size_t sz = (size_t)d.sz, n0 = (size_t)d.n0;
+  switch (f) {
+    case OSTC_FR_FIELD_CV: *p = d.cv; *n = sz * c->nv; break;
+    case OSTC_FR_FIELD_BV: *p = d.bv; *n = sz * 6; break;
+    case OSTC_FR_FIELD_SV: *p = d.sv; *n = sz * c->ns; break;
+    case OSTC_FR_FIELD_QN: *p = d.qn; *n = sz * c->nv; break;
+    case OSTC_FR_FIELD_DQ: *p = d.dq; *n = n0 * c->nv; break;
+    case OSTC_FR_FIELD_MUL: *p = d.mul; *n = sz; break;
+    case OSTC_FR_FIELD_LAMBDA: *p = d.lam; *n = sz; break;
+    case OSTC_FR_FIELD_CP: *p = d.cp; *n = sz; break;
+    case OSTC_FR_FIELD_GAMMA: *p = d.gam; *n = sz; break;
+    case OSTC_FR_FIELD_C: *p = d.cc; *n = sz; break;
+    case OSTC_FR_FIELD_MACH: *p = d.mach; *n = sz; break;
+    case OSTC_FR_FIELD_RHOD: *p = d.rhod; *n = sz * c->ns; break;
+    case OSTC_FR_FIELD_JAC: *p = d.jac; *n = sz; break;
+    case OSTC_FR_FIELD_METRIC: *p = d.met; *n = sz * 9; break;
+    case OSTC_FR_FIELD_DT_LOCAL: *p = d.dtl; *n = n0; break;
+    default: return fail(c, OSTC_ERR_ARG, "unknown field");
+  }
+  return OSTC_OK;
+}
+
+int check_block(ostc_fr_ctx c, int b) {
+  if (b < 0 || b >= (int)c->blocks.size()) return fail(c, OSTC_ERR_ARG, "block index out of range");
+  return OSTC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ostc_fr_supported(int n_spec) { return supported_ns(n_spec) ? 1 : 0; }
+
+int ostc_fr_create(const ostc_fr_setup* s, const ostc_mech_tables* mech, void* stream, ostc_fr_ctx* out) {
+  if (!s || !mech || !out) return OSTC_ERR_ARG;
+  *out = nullptr;
+  if (s->inviscid_scheme != 51 && s->inviscid_scheme != 52) return OSTC_ERR_UNSUPPORTED;
+  if (s->viscous_order != 0 && s->viscous_order != 2) return OSTC_ERR_UNSUPPORTED;
+  if (!supported_ns(mech->n_spec)) return OSTC_ERR_UNSUPPORTED;
+  if (s->cfl <= 0 && !s->fixed_time_step) return OSTC_ERR_ARG;
+  if (s->fixed_time_step && !(s->dt > 0)) return OSTC_ERR_ARG;
+  auto* c = new ostc_fr_ctx_s();
+  c->s = *s;
+  c->tab = *mech;
+  c->ns = mech->n_spec;
+  c->nv = 5 + c->ns;
+  c->weno_eps_scale = (s->rho_ref * s->v_ref) * (s->rho_ref * s->v_ref);  // src/DParameter.cu:26-29
+  if (stream) {
+    c->st = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return OSTC_ERR_HIP;
+    }
+    c->own_stream = true;
+  }
+  *out = c;
+  return OSTC_OK;
+}
+
+int ostc_fr_destroy(ostc_fr_ctx c) {
+  if (!c) return OSTC_ERR_ARG;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  invalidate_graph(c);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->d_tab) (void)hipFree(c->d_tab);
+  if (c->ev_a) (void)hipEventDestroy(c->ev_a);
+  if (c->ev_b) (void)hipEventDestroy(c->ev_b);
+  if (c->timer) {
+    for (int i = 0; i < KernelTimer::kMax; ++i) (void)hipEventDestroy(c->timer->ev[i]);
+    delete c->timer;
+  }
+  if (c->own_stream) (void)hipStreamDestroy(c->st);
+  delete c;
+  return OSTC_OK;
+}
+
+int ostc_fr_add_block(ostc_fr_ctx c, const ostc_block_desc* d, int* id) {
+  if (!c || !d) return OSTC_ERR_ARG;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "blocks are fixed after finalize");
+  if (d->mz < 2) return fail(c, OSTC_ERR_UNSUPPORTED, "two-dimensional blocks are not supported");
+  if (d->mx < 2 || d->my < 2) return fail(c, OSTC_ERR_ARG, "block too small");
+  HBlock h;
+  h.mx = d->mx;
+  h.my = d->my;
+  h.mz = d->mz;
+  c->blocks.push_back(h);
+  if (id) *id = (int)c->blocks.size() - 1;
+  return OSTC_OK;
+}
+
+int ostc_fr_add_boundary(ostc_fr_ctx c, int b, int axis, int side, int type) {
+  if (!c) return OSTC_ERR_ARG;
+  if (int r = check_block(c, b)) return r;
+  if (axis < 0 || axis > 2 || (side != -1 && side != 1)) return fail(c, OSTC_ERR_ARG, "bad face");
+  if (type != OSTC_BC_OUTFLOW && type != OSTC_BC_PERIODIC)
+    return fail(c, OSTC_ERR_UNSUPPORTED, "mixture path: outflow and periodic faces only");
+  c->blocks[b].faces.push_back({axis, side, type});
+  return OSTC_OK;
+}
+
+int ostc_fr_add_inner_face(ostc_fr_ctx c, int b, int axis, int side, int target) {
+  if (!c) return OSTC_ERR_ARG;
+  if (int r = check_block(c, b)) return r;
+  if (int r = check_block(c, target)) return r;
+  if (axis < 0 || axis > 2 || (side != -1 && side != 1)) return fail(c, OSTC_ERR_ARG, "bad face");
+  c->blocks[b].inner.push_back({axis, side, target});
+  return OSTC_OK;
+}
+
+int ostc_fr_finalize(ostc_fr_ctx c) {
+  if (!c) return OSTC_ERR_ARG;
+  if (c->finalized) return OSTC_OK;
+  if (c->blocks.empty()) return fail(c, OSTC_ERR_STATE, "no blocks");
+  // inner faces must be conforming with the target's opposite face
+  for (auto& h : c->blocks)
+    for (auto& f : h.inner) {
+      const HBlock& t = c->blocks[f.target];
+      const int e[3] = {h.mx, h.my, h.mz}, te[3] = {t.mx, t.my, t.mz};
+      for (int a = 0; a < 3; ++a)
+        if (a != f.axis && e[a] != te[a]) return fail(c, OSTC_ERR_ARG, "inner face is not 1-to-1");
+      if (te[f.axis] < c->ngg + 1) return fail(c, OSTC_ERR_ARG, "target block thinner than ngg+1 layers");
+    }
+  FR_HIP(c, hipMalloc((void**)&c->d_tab, sizeof(Tab)));
+  FR_HIP(c, hipMemcpyAsync(c->d_tab, &c->tab, sizeof(Tab), hipMemcpyHostToDevice, c->st));
+  const int g = c->ngg, nv = c->nv, ns = c->ns;
+  for (auto& h : c->blocks) {
+    FrBlockDev& d = h.d;
+    d.mx = h.mx;
+    d.my = h.my;
+    d.mz = h.mz;
+    d.g = g;
+    d.sx = h.mx + 2 * g;
+    d.sy = h.my + 2 * g;
+    d.sz = (long long)d.sx * d.sy * (h.mz + 2 * g);
+    d.n0 = (long long)h.mx * h.my * h.mz;
+    const size_t sz = (size_t)d.sz, n0 = (size_t)d.n0;
+    int r = OSTC_OK;
+    if ((r = dalloc(c, &d.cv, sz * nv)) || (r = dalloc(c, &d.qn, sz * nv)) || (r = dalloc(c, &d.bv, sz * 6)) ||
+        (r = dalloc(c, &d.sv, sz * ns)) || (r = dalloc(c, &d.dq, n0 * nv)) || (r = dalloc(c, &d.mul, sz)) ||
+        (r = dalloc(c, &d.lam, sz)) || (r = dalloc(c, &d.cp, sz)) || (r = dalloc(c, &d.gam, sz)) ||
+        (r = dalloc(c, &d.cc, sz)) || (r = dalloc(c, &d.mach, sz)) || (r = dalloc(c, &d.rhod, sz * ns)) ||
+        (r = dalloc(c, &d.jac, sz)) || (r = dalloc(c, &d.met, sz * 9)) || (r = dalloc(c, &d.dtl, n0)))
+      return r;
+  }
+  if (int r = dalloc(c, &c->d_dt, 1)) return r;
+  long long n = 0;
+  launch_dt_set(c->d_dt, c->s.fixed_time_step ? c->s.dt : 1e6, c->st, &n);
+  FR_HIP(c, hipEventCreate(&c->ev_a));
+  FR_HIP(c, hipEventCreate(&c->ev_b));
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  c->finalized = true;
+  return OSTC_OK;
+}
+
+int ostc_fr_set_geometry(ostc_fr_ctx c, int b, const double* jac, const double* metric, long long es, long long cs,
+                         long long off) {
+  if (int r = check_ready(c)) return r;
+  if (int r = check_block(c, b)) return r;
+  if (!jac || !metric) return fail(c, OSTC_ERR_ARG, "null geometry");
+  const FrBlockDev& d = c->blocks[b].d;
+  std::vector<double> planes((size_t)d.sz * 9);
+  for (long long p = 0; p < d.sz; ++p)
+    for (int k = 0; k < 9; ++k) planes[(size_t)k * d.sz + p] = metric[p * es + k * cs + off];
+  FR_HIP(c, hipMemcpyAsync(d.jac, jac, (size_t)d.sz * sizeof(double), hipMemcpyHostToDevice, c->st));
+  FR_HIP(c, hipMemcpyAsync(d.met, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  invalidate_graph(c);
+  return OSTC_OK;
+}
+
+int ostc_fr_set_field(ostc_fr_ctx c, int b, int f, const double* host) {
+  if (int r = check_ready(c)) return r;
+  if (int r = check_block(c, b)) return r;
+  if (!host) return fail(c, OSTC_ERR_ARG, "null host pointer");
+  double* p = nullptr;
+  size_t n = 0;
+  if (int r = field_ptr(c, b, f, &p, &n)) return r;
+  FR_HIP(c, hipMemcpyAsync(p, host, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_fr_get_field(ostc_fr_ctx c, int b, int f, double* host) {
+  if (int r = check_ready(c)) return r;
+  if (int r = check_block(c, b)) return r;
+  if (!host) return fail(c, OSTC_ERR_ARG, "null host pointer");
+  double* p = nullptr;
+  size_t n = 0;
+  if (int r = field_ptr(c, b, f, &p, &n)) return r;
+  FR_HIP(c, hipMemcpyAsync(host, p, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_fr_initialize(ostc_fr_ctx c) {
+  if (int r = check_ready(c)) return r;
+  long long n = 0;
+  for (int b = 0; b < (int)c->blocks.size(); ++b) launch_cv_from_bv(args(c, b), c->ns, c->st, &n);
+  for (int b = 0; b < (int)c->blocks.size(); ++b) apply_bc(c, b, &n);
+  exchange(c, &n);
+  for (int b = 0; b < (int)c->blocks.size(); ++b) launch_props(args(c, b), c->ns, c->st, &n, nullptr);
+  FR_HIP(c, hipGetLastError());
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_fr_step(ostc_fr_ctx c, int n_steps, int mode, double* last_dt) {
+  if (int r = check_ready(c)) return r;
+  if (n_steps < 0) return fail(c, OSTC_ERR_ARG, "negative step count");
+  if (mode != OSTC_FR_MODE_DIRECT && mode != OSTC_FR_MODE_GRAPH && mode != OSTC_FR_MODE_TIMED)
+    return fail(c, OSTC_ERR_ARG, "unknown mode");
+  long long n = 0;
+  KernelTimer* tm = nullptr;
+  if (mode == OSTC_FR_MODE_TIMED) {
+    if (!c->timer) {
+      c->timer = new KernelTimer();
+      for (int i = 0; i < KernelTimer::kMax; ++i) FR_HIP(c, hipEventCreate(&c->timer->ev[i]));
+    }
+    tm = c->timer;
+    tm->n = 0;
+    tm->ok = true;
+  }
+  if (mode == OSTC_FR_MODE_GRAPH) {
+    if (int r = build_graph(c)) return r;
+  }
+  FR_HIP(c, hipEventRecord(c->ev_a, c->st));
+  for (int i = 0; i < n_steps; ++i) {
+    if (mode == OSTC_FR_MODE_GRAPH) {
+      FR_HIP(c, hipGraphLaunch(c->gexec, c->st));
+      n += c->graph_launches;
+    } else {
+      step(c, &n, tm);
+    }
+  }
+  FR_HIP(c, hipEventRecord(c->ev_b, c->st));
+  FR_HIP(c, hipGetLastError());
+  FR_HIP(c, hipEventSynchronize(c->ev_b));
+  float ms = 0;
+  FR_HIP(c, hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+  c->last_ms = ms;
+  c->last_launches = n;
+  if (tm) {
+    if (!tm->ok) return fail(c, OSTC_ERR_STATE, "too many timed launches in one call (use fewer steps)");
+    std::fill(c->kms, c->kms + KT_COUNT, 0.0);
+    std::fill(c->kcount, c->kcount + KT_COUNT, 0LL);
+    for (int e = 0; e + 1 < tm->n; e += 2) {
+      float k = 0;
+      FR_HIP(c, hipEventElapsedTime(&k, tm->ev[e], tm->ev[e + 1]));
+      c->kms[tm->kind[e / 2]] += k;
+      c->kcount[tm->kind[e / 2]] += 1;
+    }
+  }
+  if (last_dt) FR_HIP(c, hipMemcpy(last_dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost));
+  return OSTC_OK;
+}
+
+int ostc_fr_stage_rhs(ostc_fr_ctx c, int b, double dt) {
+  if (int r = check_ready(c)) return r;
+  if (int r = check_block(c, b)) return r;
+  long long n = 0;
+  launch_dt_set(c->d_dt, dt, c->st, &n);
+  launch_rhs(args(c, b), c->ns, c->st, &n, nullptr);
+  FR_HIP(c, hipGetLastError());
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_fr_compute_dt(ostc_fr_ctx c, double* dt_out) {
+  if (int r = check_ready(c)) return r;
+  long long n = 0;
+  launch_dt_set(c->d_dt, 1e6, c->st, &n);
+  for (int b = 0; b < (int)c->blocks.size(); ++b)
+    launch_local_dt(args(c, b), reinterpret_cast<unsigned long long*>(c->d_dt), c->st, &n);
+  FR_HIP(c, hipGetLastError());
+  double dt = 0;
+  FR_HIP(c, hipMemcpyAsync(&dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  if (dt_out) *dt_out = dt;
+  return OSTC_OK;
+}
+
+int ostc_fr_stage_update(ostc_fr_ctx c, int b, int rk, double dt) {
+  if (int r = check_ready(c)) return r;
+  if (int r = check_block(c, b)) return r;
+  if (rk < 0 || rk > 2) return fail(c, OSTC_ERR_ARG, "rk must be 0, 1 or 2");
+  long long n = 0;
+  launch_dt_set(c->d_dt, dt, c->st, &n);
+  launch_update(args(c, b), c->ns, kRkA[rk], kRkB[rk], kRkC[rk], c->st, &n, nullptr);
+  FR_HIP(c, hipGetLastError());
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_fr_update_properties(ostc_fr_ctx c, int b) {
+  if (int r = check_ready(c)) return r;
+  if (int r = check_block(c, b)) return r;
+  long long n = 0;
+  launch_props(args(c, b), c->ns, c->st, &n, nullptr);
+  FR_HIP(c, hipGetLastError());
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_fr_last_step_timing(ostc_fr_ctx c, double* ms, long long* launches) {
+  if (!c) return OSTC_ERR_ARG;
+  if (ms) *ms = c->last_ms;
+  if (launches) *launches = c->last_launches;
+  return OSTC_OK;
+}
+
+int ostc_fr_kernel_ms(ostc_fr_ctx c, double ms[OSTC_FR_K_COUNT], long long counts[OSTC_FR_K_COUNT]) {
+  if (!c) return OSTC_ERR_ARG;
+  static_assert(KT_COUNT == OSTC_FR_K_COUNT, "kernel kinds");
+  for (int k = 0; k < KT_COUNT; ++k) {
+    if (ms) ms[k] = c->kms[k];
+    if (counts) counts[k] = c->kcount[k];
+  }
+  return OSTC_OK;
+}
+
+int ostc_fr_sync(ostc_fr_ctx c) {
+  if (!c) return OSTC_ERR_ARG;
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+const char* ostc_fr_last_error(ostc_fr_ctx c) { return c ? c->err.c_str() : "null context"; }
+
+int ostc_util_mixing_layer_sv(int mx, int my, int mz, int g, int ns, double y0, double dy, const double* y_upper,
+                              const double* y_lower, double* sv) {
+  if (mx < 2 || my < 2 || mz < 2 || g < 1 || ns < 1 || !y_upper || !y_lower || !sv) return OSTC_ERR_ARG;
+  const long long sx = mx + 2 * g, sy = my + 2 * g, sz = sx * sy * (mz + 2 * g);
+  for (int k = -g; k < mz + g; ++k)
+    for (int j = -g; j < my + g; ++j) {
+      const double y = y0 + j * dy;
+      const double* Y = y >= 0 ? y_upper : y_lower;
+      for (int i = -g; i < mx + g; ++i) {
+        const long long p = ((long long)(k + g) * sy + (j + g)) * sx + (i + g);
+        for (int l = 0; l < ns; ++l) sv[p + l * sz] = Y[l];
+      }
+    }
+  return OSTC_OK;
+}
+
+}  // extern "C"

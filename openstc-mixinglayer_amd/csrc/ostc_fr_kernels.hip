@@ -1,0 +1,892 @@
+// ostc_fr_kernels.hip -- HIP kernels of the finite-rate mixture path
+// (the reference's MixtureModel::FR, laminar), for gfx950.
+//
+// Per RK stage and block: k_source (dq = J omega, or 0), three inviscid
+// sweeps, three viscous sweeps, k_update (SSP-RK3 + primitives + T-Newton),
+// then ghost ops and k_props over the padded domain.  dt lives on the device
+// (k_local_dt reduces it with 64-lane shuffles and one atomicMin per wave), so
+// a whole step has no host synchronisation and can be captured in a graph.
+//
+// Inviscid sweeps stage the split fluxes of a tile in LDS: a tile is TW
+// columns x (TS + 2 ngg) points along the sweep, one thread per face
+// (TS + 1 faces per column, 256 faces per tile).  Points are loaded with
+// consecutive threads on consecutive i (coalesced), F+ and F- are stored as
+// structure-of-arrays planes (bank-conflict-free face reads), face fluxes
+// overwrite the same LDS, and each cell subtracts its face difference from dq.
+#include <hip/hip_runtime.h>
+
+#include "ostc_fr.hpp"
+
+namespace ostc {
+namespace fr {
+namespace {
+
+constexpr int kT = 256;
+
+__device__ __forceinline__ long long pid(const FrBlockDev& B, int i, int j, int k) {
+  return ((long long)(k + B.g) * B.sy + (j + B.g)) * B.sx + (i + B.g);
+}
+__device__ __forceinline__ long long pid0(const FrBlockDev& B, int i, int j, int k) {
+  return ((long long)k * B.my + j) * B.mx + i;
+}
+
+// WENO-Z5, src/WENO.cu:1582-1617
+__device__ __forceinline__ double weno5z(const double* vp, const double* vm, double eps) {
+  constexpr double one6th{1.0 / 6};
+  double v0{one6th * (2 * vp[2] + 5 * vp[3] - vp[4])};
+  double v1{one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3])};
+  double v2{one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2])};
+  constexpr double c1312{13.0 / 12};
+  double b0 = c1312 * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
+              0.25 * (3 * vp[2] - 4 * vp[3] + vp[4]) * (3 * vp[2] - 4 * vp[3] + vp[4]);
+  double b1 = c1312 * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
+              0.25 * (vp[1] - vp[3]) * (vp[1] - vp[3]);
+  double b2 = c1312 * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
+              0.25 * (vp[0] - 4 * vp[1] + 3 * vp[2]) * (vp[0] - 4 * vp[1] + 3 * vp[2]);
+  double tau{(b0 - b2) * (b0 - b2)};
+  double a0{0.3 + 0.3 * tau / ((eps + b0) * (eps + b0))};
+  double a1{0.6 + 0.6 * tau / ((eps + b1) * (eps + b1))};
+  double a2{0.1 + 0.1 * tau / ((eps + b2) * (eps + b2))};
+  const double fp{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
+  v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
+  v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
+  v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
+  b0 = c1312 * (vm[2] + vm[4] - 2 * vm[3]) * (vm[2] + vm[4] - 2 * vm[3]) +
+       0.25 * (3 * vm[2] - 4 * vm[3] + vm[4]) * (3 * vm[2] - 4 * vm[3] + vm[4]);
+  b1 = c1312 * (vm[1] + vm[3] - 2 * vm[2]) * (vm[1] + vm[3] - 2 * vm[2]) + 0.25 * (vm[1] - vm[3]) * (vm[1] - vm[3]);
+  b2 = c1312 * (vm[0] + vm[2] - 2 * vm[1]) * (vm[0] + vm[2] - 2 * vm[1]) +
+       0.25 * (vm[0] - 4 * vm[1] + 3 * vm[2]) * (vm[0] - 4 * vm[1] + 3 * vm[2]);
+  tau = (b0 - b2) * (b0 - b2);
+  a0 = 0.1 + 0.1 * tau / ((eps + b0) * (eps + b0));
+  a1 = 0.6 + 0.6 * tau / ((eps + b1) * (eps + b1));
+  a2 = 0.3 + 0.3 * tau / ((eps + b2) * (eps + b2));
+  const double fm{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
+  return fp + fm;
+}
+
+// cv of one point from bv and sv (compute_cv_from_bv_1_point<FR>,
+// src/FieldOperation.cuh:78-105)
+template <int NS>
+__device__ __forceinline__ void cv_point(const FrArgs& A, long long p) {
+  const FrBlockDev& B = A.B;
+  const long long s = B.sz;
+  const double rho = B.bv[p];
+  B.cv[p] = rho;
+  B.cv[p + s] = rho * B.bv[p + s];
+  B.cv[p + 2 * s] = rho * B.bv[p + 2 * s];
+  B.cv[p + 3 * s] = rho * B.bv[p + 3 * s];
+  double Y[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    Y[l] = B.sv[p + l * s];
+    B.cv[p + (5 + l) * s] = rho * Y[l];
+  }
+  B.cv[p + 4 * s] =
+      total_energy<NS>(A.T, rho, B.bv[p + s], B.bv[p + 2 * s], B.bv[p + 3 * s], B.bv[p + 4 * s], B.bv[p + 5 * s], Y);
+}
+
+// ------------------------------------------------------------ point kernels
+template <int NS>
+__global__ __launch_bounds__(kT) void k_cv_from_bv(FrArgs A) {
+  const long long p = (long long)blockIdx.x * kT + threadIdx.x;
+  if (p < A.B.sz) cv_point<NS>(A, p);
+}
+
+// update_physical_properties<FR>, src/FieldOperation.cuh:107-140, over the
+// padded domain
+template <int NS>
+__global__ __launch_bounds__(kT) void k_props(FrArgs A) {
+  const FrBlockDev& B = A.B;
+  const long long p = (long long)blockIdx.x * kT + threadIdx.x;
+  if (p >= B.sz) return;
+  const long long s = B.sz;
+  const double t = B.bv[p + 5 * s];
+  const double u = B.bv[p + s], v = B.bv[p + 2 * s], w = B.bv[p + 3 * s];
+  const double V = sqrt(u * u + v * v + w * w);
+  double mw{0}, cp_tot{0}, cvv{0}, cp[NS], Y[NS];
+  heat_capacity<NS>(A.T, t, cp);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    Y[l] = B.sv[p + l * s];
+    mw += Y[l] / A.T->mw[l];
+    cp_tot += Y[l] * cp[l];
+    cvv += Y[l] * (cp[l] - kRu / A.T->mw[l]);
+  }
+  mw = 1 / mw;
+  const double gam = cp_tot / cvv;
+  const double c = sqrt(gam * kRu * t / mw);
+  B.cp[p] = cp_tot;
+  B.gam[p] = gam;
+  B.cc[p] = c;
+  double mu, lam, rd[NS];
+  transport<NS>(A.T, t, B.bv[p + 4 * s], B.bv[p], Y, mw, cp, &mu, &lam, rd);
+  B.mul[p] = mu;
+  B.lam[p] = lam;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) B.rhod[p + l * s] = rd[l];
+  B.mach[p] = V / c;
+}
+
+// dq = 0, then the chemical source J omega on the species rows
+// (cudaMemset + compute_source<FR>, src/RK.cuh:114-118, src/SourceTerm.cuh:11-35)
+template <int NS>
+__global__ __launch_bounds__(kT) void k_source(FrArgs A) {
+  const FrBlockDev& B = A.B;
+  const long long q = (long long)blockIdx.x * kT + threadIdx.x;
+  if (q >= B.n0) return;
+  const int i = (int)(q % B.mx), j = (int)((q / B.mx) % B.my), k = (int)(q / ((long long)B.mx * B.my));
+  const long long p = pid(B, i, j, k);
+  const long long n0 = B.n0;
+#pragma unroll
+  for (int l = 0; l < 5; ++l) B.dq[q + l * n0] = 0;
+  if (A.reaction) {
+    double Y[NS], om[NS];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) Y[l] = B.sv[p + l * B.sz];
+    chemistry<NS>(A.T, B.bv[p], B.bv[p + 5 * B.sz], Y, om);
+    const double J = B.jac[p];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) B.dq[q + (5 + l) * n0] = 0.0 + J * om[l];
+  } else {
+#pragma unroll
+    for (int l = 0; l < NS; ++l) B.dq[q + (5 + l) * n0] = 0;
+  }
+}
+
+__global__ void k_dt_set(double* dt, double v) { *dt = v; }
+
+// local_time_step<FR, Laminar>, src/TimeAdvanceFunc.cuh:28-83, and the MIN of
+// global_time_step: dt_local per cell, a 64-lane shuffle MIN per wave and one
+// 64-bit atomicMin on the bit pattern of a positive double.
+__global__ __launch_bounds__(kT) void k_local_dt(FrArgs A, unsigned long long* dt_bits) {
+  const FrBlockDev& B = A.B;
+  const long long q = (long long)blockIdx.x * kT + threadIdx.x;
+  double dtl = 1e6;
+  if (q < B.n0) {
+    const int i = (int)(q % B.mx), j = (int)((q / B.mx) % B.my), k = (int)(q / ((long long)B.mx * B.my));
+    const long long p = pid(B, i, j, k), s = B.sz;
+    auto m = [&](int r, int c) { return B.met[p + (3 * (r - 1) + (c - 1)) * s]; };
+    const double gxi = sqrt(m(1, 1) * m(1, 1) + m(1, 2) * m(1, 2) + m(1, 3) * m(1, 3));
+    const double geta = sqrt(m(2, 1) * m(2, 1) + m(2, 2) * m(2, 2) + m(2, 3) * m(2, 3));
+    const double gzeta = sqrt(m(3, 1) * m(3, 1) + m(3, 2) * m(3, 2) + m(3, 3) * m(3, 3));
+    const double u = B.bv[p + s], v = B.bv[p + 2 * s], w = B.bv[p + 3 * s];
+    const double U = u * m(1, 1) + v * m(1, 2) + w * m(1, 3);
+    const double V = u * m(2, 1) + v * m(2, 2) + w * m(2, 3);
+    const double W = u * m(3, 1) + v * m(3, 2) + w * m(3, 3);
+    const double a = B.cc[p];
+    const double r0 = fabs(U) + a * gxi, r1 = fabs(V) + a * geta, r2 = fabs(W) + a * gzeta;
+    const double sr_inv = r0 + r1 + r2;
+    const double coeff_1 = fmax(B.gam[p], 4.0 / 3.0) / B.bv[p];
+    const double coeff_2 = B.mul[p] / A.prandtl;
+    double sr_visc = gxi * gxi + geta * geta;
+    sr_visc += gzeta * gzeta;
+    sr_visc *= coeff_1 * coeff_2;
+    dtl = A.cfl / (sr_inv + sr_visc);
+    B.dtl[q] = dtl;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) dtl = fmin(dtl, __shfl_xor(dtl, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMin(dt_bits, (unsigned long long)__double_as_longlong(dtl));
+}
+
+// update_cv_and_bv_rk<FR>, src/RK.cuh:230-284 + compute_temperature_and_pressure
+template <int NS>
+__global__ __launch_bounds__(kT) void k_update(FrArgs A, double ra, double rb, double rc) {
+  const FrBlockDev& B = A.B;
+  const long long q = (long long)blockIdx.x * kT + threadIdx.x;
+  if (q >= B.n0) return;
+  const int i = (int)(q % B.mx), j = (int)((q / B.mx) % B.my), k = (int)(q / ((long long)B.mx * B.my));
+  const long long p = pid(B, i, j, k), s = B.sz, n0 = B.n0;
+  const double dt = *A.dt;
+  const double dt_div_jac = dt / B.jac[p];
+  double cv[5 + NS];
+#pragma unroll
+  for (int l = 0; l < 5 + NS; ++l) {
+    cv[l] = ra * B.qn[p + l * s] + rb * B.cv[p + l * s] + rc * dt_div_jac * B.dq[q + l * n0];
+    B.cv[p + l * s] = cv[l];
+  }
+  const double density_inv = 1.0 / cv[0];
+  const double u = cv[1] * density_inv, v = cv[2] * density_inv, w = cv[3] * density_inv;
+  B.bv[p] = cv[0];
+  B.bv[p + s] = u;
+  B.bv[p + 2 * s] = v;
+  B.bv[p + 3 * s] = w;
+  double Y[NS], mw{0};
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    Y[l] = cv[5 + l] * density_inv;
+    B.sv[p + l * s] = Y[l];
+  }
+#pragma unroll
+  for (int l = 0; l < NS; ++l) mw += Y[l] / A.T->mw[l];
+  mw = 1 / mw;
+  const double gas_const = kRu / mw;
+  const double e = cv[4] / cv[0] - 0.5 * (u * u + v * v + w * w);
+  const double t = temperature<NS>(A.T, e, Y, gas_const, B.bv[p + 5 * s]);
+  B.bv[p + 5 * s] = t;
+  B.bv[p + 4 * s] = cv[0] * t * gas_const;
+}
+
+// apply_outflow / apply_periodic with scalars, src/BoundCond.cuh:205-233,
+// :1443-1486: one thread per face point over the face's padded tangential
+// extent, looping over the ngg ghost layers.
+template <int NS>
+__global__ __launch_bounds__(kT) void k_bc(FrArgs A, int axis, int side, int type) {
+  const FrBlockDev& B = A.B;
+  const int ext[3] = {B.mx, B.my, B.mz};
+  const int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
+  const int n1 = ext[o1] + 2 * B.g, n2 = ext[o2] + 2 * B.g;
+  const int q = blockIdx.x * kT + threadIdx.x;
+  if (q >= n1 * n2) return;
+  int ix[3];
+  ix[axis] = side < 0 ? 0 : ext[axis] - 1;
+  ix[o1] = q % n1 - B.g;
+  ix[o2] = q / n1 - B.g;
+  int src[3] = {ix[0], ix[1], ix[2]};
+  if (type == 8) src[axis] = side < 0 ? ext[axis] - 1 : 0;
+  const long long s = B.sz;
+  for (int g = 1; g <= B.g; ++g) {
+    int gi[3] = {ix[0], ix[1], ix[2]}, si[3] = {src[0], src[1], src[2]};
+    gi[axis] += g * side;
+    if (type == 8) si[axis] += g * side;
+    const long long pg = pid(B, gi[0], gi[1], gi[2]), ps = pid(B, si[0], si[1], si[2]);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) B.bv[pg + l * s] = B.bv[ps + l * s];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) B.sv[pg + l * s] = B.sv[ps + l * s];
+    cv_point<NS>(A, pg);
+  }
+}
+
+// inner_communication with scalars, src/DataCommunication.cuh:60-105, for an
+// aligned 1-to-1 face: the shared node layer and ngg ghost layers over the
+// interior tangential extent.
+template <int NS>
+__global__ __launch_bounds__(kT) void k_comm(FrArgs A, FrBlockDev Tg, int axis, int side) {
+  const FrBlockDev& B = A.B;
+  const int ext[3] = {B.mx, B.my, B.mz}, text[3] = {Tg.mx, Tg.my, Tg.mz};
+  const int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
+  const int n1 = ext[o1], n2 = ext[o2];
+  const long long q = (long long)blockIdx.x * kT + threadIdx.x;
+  if (q >= (long long)n1 * n2 * (B.g + 1)) return;
+  const int layer = (int)(q / ((long long)n1 * n2));
+  const int r = (int)(q % ((long long)n1 * n2));
+  int ix[3], tx[3];
+  ix[axis] = (side < 0 ? 0 : ext[axis] - 1) + side * layer;
+  tx[axis] = (side < 0 ? text[axis] - 1 : 0) + side * layer;
+  ix[o1] = tx[o1] = r % n1;
+  ix[o2] = tx[o2] = r / n1;
+  const long long p = pid(B, ix[0], ix[1], ix[2]), pt = pid(Tg, tx[0], tx[1], tx[2]);
+  const long long s = B.sz, ts = Tg.sz;
+  if (layer == 0) {
+#pragma unroll
+    for (int l = 0; l < 6; ++l) {
+      const double ave{0.5 * (Tg.bv[pt + l * ts] + B.bv[p + l * s])};
+      B.bv[p + l * s] = ave;
+      Tg.bv[pt + l * ts] = ave;
+    }
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const double ave{0.5 * (Tg.sv[pt + l * ts] + B.sv[p + l * s])};
+      B.sv[p + l * s] = ave;
+      Tg.sv[pt + l * ts] = ave;
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < 6; ++l) B.bv[p + l * s] = Tg.bv[pt + l * ts];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) B.sv[p + l * s] = Tg.sv[pt + l * ts];
+  }
+  cv_point<NS>(A, p);
+}
+
+// ------------------------------------------------------------ tile geometry
+// DIR 0: TW = 4 columns along j, 64 faces per column; DIR 1, 2: TW = 16
+// columns along i, 16 faces per column.
+template <int DIR> struct Tile;
+template <> struct Tile<0> { static constexpr int TW = 4, F = 64; };
+template <> struct Tile<1> { static constexpr int TW = 16, F = 16; };
+template <> struct Tile<2> { static constexpr int TW = 16, F = 16; };
+
+// Coordinates of tile position (col, sweep index t) in block indices.
+template <int DIR>
+__device__ __forceinline__ void tile_idx(int col0, int t, int col, int third, int* idx) {
+  if (DIR == 0) {
+    idx[0] = t;
+    idx[1] = col0 + col;
+    idx[2] = third;
+  } else if (DIR == 1) {
+    idx[0] = col0 + col;
+    idx[1] = t;
+    idx[2] = third;
+  } else {
+    idx[0] = col0 + col;
+    idx[1] = third;
+    idx[2] = t;
+  }
+}
+
+// Positivity-preserving limiter on the species part of one face,
+// src/WENO.cu:1538-1580.
+template <int NS>
+__device__ __forceinline__ void pp_limit(double* fc, const double* f1, const double* ql, const double* qr, double jl,
+                                         double jr, double dt, int idx, int max_extent) {
+  const double alpha = 1.0 / 3.0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    double theta_p = 1.0, theta_m = 1.0;
+    if (idx > -1) {
+      const double up = 0.5 * alpha * ql[l] * jl - dt * fc[l];
+      if (up < 0) {
+        const double up_lf = 0.5 * alpha * ql[l] * jl - dt * f1[l];
+        if (fabs(up - up_lf) > 1e-20) {
+          theta_p = (0 - up_lf) / (up - up_lf);
+          if (theta_p > 1) theta_p = 1.0;
+          else if (theta_p < 0) theta_p = 0;
+        }
+      }
+    }
+    if (idx < max_extent - 1) {
+      const double um = 0.5 * alpha * qr[l] * jr + dt * fc[l];
+      if (um < 0) {
+        const double um_lf = 0.5 * alpha * qr[l] * jr + dt * f1[l];
+        if (fabs(um - um_lf) > 1e-20) {
+          theta_m = (0 - um_lf) / (um - um_lf);
+          if (theta_m > 1) theta_m = 1.0;
+          else if (theta_m < 0) theta_m = 0;
+        }
+      }
+    }
+    fc[l] = fmin(theta_p, theta_m) * (fc[l] - f1[l]) + f1[l];
+  }
+}
+
+// One inviscid sweep along DIR (compute_convective_term_weno_{x,y,z}<FR>,
+// src/WENO.cu:8-461) with the component-wise (51) or characteristic (52)
+// WENO-Z5 flux, src/WENO.cu:1455-1536 and :722-1143 (method 1).
+template <int DIR, int NS, int SCHEME>
+__global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
+  constexpr int NV = 5 + NS, G = 3;
+  constexpr int TW = Tile<DIR>::TW, F = Tile<DIR>::F, TS = F - 1, P = TS + 2 * G, NP = P * TW;
+  extern __shared__ double lds[];
+  double* Fp = lds;            // [NV][NP]
+  double* Fm = lds + NV * NP;  // [NV][NP]
+  const FrBlockDev& B = A.B;
+  const int ext[3] = {B.mx, B.my, B.mz};
+  const int m = ext[DIR];
+  const int across = DIR == 0 ? 1 : 0;
+  const int col0 = blockIdx.y * TW;
+  const int third = blockIdx.z;
+  const int t0 = blockIdx.x * TS;  // first cell of the tile along DIR
+  const long long s = B.sz;
+  const int tid = threadIdx.x;
+
+  // 1. split fluxes of the tile's points (cells t0-3 .. t0+TS+2)
+  for (int e = tid; e < NP; e += kT) {
+    int col, sp;
+    if (DIR == 0) {
+      col = e / P;
+      sp = e % P;
+    } else {
+      col = e % TW;
+      sp = e / TW;
+    }
+    const int t = t0 - G + sp;
+    int idx[3];
+    tile_idx<DIR>(col0, t, col, third, idx);
+    const int slot = col * P + sp;
+    if (idx[across] < ext[across] && t < m + G) {
+      const long long p = pid(B, idx[0], idx[1], idx[2]);
+      double q[NV + 2], kk[3];
+#pragma unroll
+      for (int l = 0; l < NV; ++l) q[l] = B.cv[p + l * s];
+      q[NV] = B.bv[p + 4 * s];
+      q[NV + 1] = B.cc[p];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) kk[c] = B.met[p + (3 * DIR + c) * s];
+      const double jac = B.jac[p];
+      // compute_flux, src/WENO.cu:697-720
+      const double Uk{(q[1] * kk[0] + q[2] * kk[1] + q[3] * kk[2]) / q[0]};
+      const double pk{q[NV]};
+      const double cGradK = q[NV + 1] * sqrt(kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2]);
+      const double lambda0 = fabs(Uk) + cGradK;
+      Fp[0 * NP + slot] = 0.5 * jac * (Uk * q[0] + lambda0 * q[0]);
+      Fp[1 * NP + slot] = 0.5 * jac * (Uk * q[1] + pk * kk[0] + lambda0 * q[1]);
+      Fp[2 * NP + slot] = 0.5 * jac * (Uk * q[2] + pk * kk[1] + lambda0 * q[2]);
+      Fp[3 * NP + slot] = 0.5 * jac * (Uk * q[3] + pk * kk[2] + lambda0 * q[3]);
+      Fp[4 * NP + slot] = 0.5 * jac * (Uk * (q[4] + pk) + lambda0 * q[4]);
+      Fm[0 * NP + slot] = 0.5 * jac * (Uk * q[0] - lambda0 * q[0]);
+      Fm[1 * NP + slot] = 0.5 * jac * (Uk * q[1] + pk * kk[0] - lambda0 * q[1]);
+      Fm[2 * NP + slot] = 0.5 * jac * (Uk * q[2] + pk * kk[1] - lambda0 * q[2]);
+      Fm[3 * NP + slot] = 0.5 * jac * (Uk * q[3] + pk * kk[2] - lambda0 * q[3]);
+      Fm[4 * NP + slot] = 0.5 * jac * (Uk * (q[4] + pk) - lambda0 * q[4]);
+#pragma unroll
+      for (int l = 5; l < NV; ++l) {
+        Fp[l * NP + slot] = 0.5 * jac * (Uk * q[l] + lambda0 * q[l]);
+        Fm[l * NP + slot] = 0.5 * jac * (Uk * q[l] - lambda0 * q[l]);
+      }
+    } else {
+#pragma unroll
+      for (int l = 0; l < NV; ++l) Fp[l * NP + slot] = Fm[l * NP + slot] = 0;
+    }
+  }
+  __syncthreads();
+
+  // 2. one face per thread: face between cells t0-1+f and t0+f
+  const int f = tid % F, col = tid / F;
+  const int tl = t0 - 1 + f;  // left cell of the face
+  int idx[3];
+  tile_idx<DIR>(col0, tl, col, third, idx);
+  const bool active = idx[across] < ext[across] && tl < m;
+  double fc[NV];
+  if (active) {
+    const int sl = col * P + (f + G - 1);  // LDS slot of the left cell
+    const long long pl = pid(B, idx[0], idx[1], idx[2]);
+    int idr[3] = {idx[0], idx[1], idx[2]};
+    idr[DIR] += 1;
+    const long long pr = pid(B, idr[0], idr[1], idr[2]);
+    double ml[3], mr[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ml[c] = B.met[pl + (3 * DIR + c) * s];
+      mr[c] = B.met[pr + (3 * DIR + c) * s];
+    }
+    const double jac1 = B.jac[pl], jac2 = B.jac[pr];
+    double f1[NS];
+    if (SCHEME == 52) {
+      // compute_weno_flux_ch<FR> with "method 1" flux splitting.  The five
+      // acoustic/entropy fields run as a rolled loop and their back
+      // projection (src/WENO.cu:1121-1142) is accumulated field by field in
+      // the reference's term order, which keeps the arithmetic identical and
+      // the live register set small.
+      CharFace<NS> C;
+      {
+        double cvl[NV + 2], cvr[NV + 2];
+#pragma unroll
+        for (int l = 0; l < NV; ++l) {
+          cvl[l] = B.cv[pl + l * s];
+          cvr[l] = B.cv[pr + l * s];
+        }
+        cvl[NV] = B.bv[pl + 4 * s];
+        cvr[NV] = B.bv[pr + 4 * s];
+        cvl[NV + 1] = B.cc[pl];
+        cvr[NV + 1] = B.cc[pr];
+        char_face<NS>(A.T, cvl, cvr, ml, mr, jac1, jac2, C);
+        if (A.pp) {
+          const double srThis = fabs((ml[0] * cvl[1] + ml[1] * cvl[2] + ml[2] * cvl[3]) / cvl[0] +
+                                     cvl[NV + 1] * sqrt(ml[0] * ml[0] + ml[1] * ml[1] + ml[2] * ml[2]));
+          const double srNext = fabs((mr[0] * cvr[1] + mr[1] * cvr[2] + mr[2] * cvr[3]) / cvr[0] +
+                                     cvr[NV + 1] * sqrt(mr[0] * mr[0] + mr[1] * mr[1] + mr[2] * mr[2]));
+#pragma unroll
+          for (int l = 0; l < NS; ++l)
+            f1[l] = 0.5 * (Fp[(l + 5) * NP + sl] + srThis * cvl[l + 5] * jac1) +
+                    0.5 * (Fp[(l + 5) * NP + sl + 1] - srNext * cvr[l + 5] * jac2);
+        }
+      }
+      constexpr double eps{1e-40};
+      const double eps_scaled = eps * A.weno_eps_scale * 0.25 * (C.kxJ * C.kxJ + C.kyJ * C.kyJ + C.kzJ * C.kzJ);
+      const double um = C.um, vm = C.vm, wm = C.wm, cm = C.cm, gm1 = C.gm1, kx = C.kx, ky = C.ky, kz = C.kz;
+      const double hm = C.hm, Ukb = C.Uk_bar;
+      double f0 = 0, f1a = 0, f2 = 0, f3 = 0, f4 = 0;  // fci[0..4] accumulated over the fields
+#pragma unroll 1
+      for (int l = 0; l < 5; ++l) {
+        double L[5];
+        const double cas = char_row<NS>(C, l, L);
+        double vPlus[5], vMinus[5];
+#pragma unroll 1
+        for (int r = 0; r < 5; ++r) {
+          double ap = 0, am = 0;
+#pragma unroll
+          for (int n = 0; n < 5; ++n) {
+            ap += L[n] * Fp[n * NP + sl - 2 + r];
+            am += L[n] * Fm[n * NP + sl - 1 + r];
+          }
+#pragma unroll
+          for (int n = 0; n < NS; ++n) {
+            ap += cas * C.alpha_l[n] * Fp[(5 + n) * NP + sl - 2 + r];
+            am += cas * C.alpha_l[n] * Fm[(5 + n) * NP + sl - 1 + r];
+          }
+#pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            vPlus[q] = (q == r) ? ap : vPlus[q];
+            vMinus[q] = (q == r) ? am : vMinus[q];
+          }
+        }
+        const double fch = weno5z(vPlus, vMinus, eps_scaled);
+        // column l of the back projection
+        double r0, r1, r2, r3, r4;
+        if (l == 0) {
+          r0 = 1; r1 = um - kx * cm; r2 = vm - ky * cm; r3 = wm - kz * cm; r4 = hm - Ukb * cm;
+        } else if (l == 1) {
+          r0 = kx; r1 = kx * um; r2 = kx * vm + kz * cm; r3 = kx * wm - ky * cm;
+          r4 = kx * (hm - cm * cm / gm1) + (kz * vm - ky * wm) * cm;
+        } else if (l == 2) {
+          r0 = ky; r1 = ky * um - kz * cm; r2 = ky * vm; r3 = ky * wm + kx * cm;
+          r4 = ky * (hm - cm * cm / gm1) + (kx * wm - kz * um) * cm;
+        } else if (l == 3) {
+          r0 = kz; r1 = kz * um + ky * cm; r2 = kz * vm - kx * cm; r3 = kz * wm;
+          r4 = kz * (hm - cm * cm / gm1) + (ky * um - kx * vm) * cm;
+        } else {
+          r0 = 1; r1 = um + kx * cm; r2 = vm + ky * cm; r3 = wm + kz * cm; r4 = hm + Ukb * cm;
+        }
+        if (l == 0) {
+          f0 = fch;  // 1 * fChar[0] is exact
+          f1a = r1 * fch;
+          f2 = r2 * fch;
+          f3 = r3 * fch;
+          f4 = r4 * fch;
+        } else {
+          f0 = (l == 4) ? f0 + fch : f0 + r0 * fch;
+          f1a = f1a + r1 * fch;
+          f2 = f2 + r2 * fch;
+          f3 = f3 + r3 * fch;
+          f4 = f4 + r4 * fch;
+        }
+      }
+      double add{0};
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        double vPlus[5], vMinus[5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+          vPlus[r] = -C.svm[l] * Fp[sl - 2 + r] + Fp[(5 + l) * NP + sl - 2 + r];
+          vMinus[r] = -C.svm[l] * Fm[sl - 1 + r] + Fm[(5 + l) * NP + sl - 1 + r];
+        }
+        const double fch = weno5z(vPlus, vMinus, eps_scaled);
+        fc[5 + l] = C.svm[l] * f0 + fch;
+        add += C.alpha_l[l] * fch;
+      }
+      fc[0] = f0;
+      fc[1] = f1a;
+      fc[2] = f2;
+      fc[3] = f3;
+      fc[4] = f4 - add * cm * cm / gm1;
+      if (A.pp) {
+        double ql[NS], qr[NS];
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          ql[l] = B.cv[pl + (5 + l) * s];
+          qr[l] = B.cv[pr + (5 + l) * s];
+        }
+        pp_limit<NS>(fc + 5, f1, ql, qr, jac1, jac2, *A.dt, tl, m);
+      }
+    } else {
+      constexpr double eps{1e-20};
+      const double eps_ref = eps * A.weno_eps_scale * 0.25 *
+                             ((ml[0] * jac1 + mr[0] * jac2) * (ml[0] * jac1 + mr[0] * jac2) +
+                              (ml[1] * jac1 + mr[1] * jac2) * (ml[1] * jac1 + mr[1] * jac2) +
+                              (ml[2] * jac1 + mr[2] * jac2) * (ml[2] * jac1 + mr[2] * jac2));
+      double eps_scaled[3];
+      eps_scaled[0] = eps_ref;
+      eps_scaled[1] = eps_ref * A.v_ref * A.v_ref;
+      eps_scaled[2] = eps_scaled[1] * A.v_ref * A.v_ref;
+      if (A.pp) {
+#pragma unroll
+        for (int l = 0; l < NS; ++l) f1[l] = 0.5 * Fp[(l + 5) * NP + sl] + 0.5 * Fm[(l + 5) * NP + sl + 1];
+      }
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        const double eh = (l == 1 || l == 2 || l == 3) ? eps_scaled[1] : (l == 4 ? eps_scaled[2] : eps_scaled[0]);
+        double vp[5], vm[5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+          vp[r] = Fp[l * NP + sl - 2 + r];
+          vm[r] = Fm[l * NP + sl - 1 + r];
+        }
+        fc[l] = weno5z(vp, vm, eh);
+      }
+      if (A.pp) {
+        double ql[NS], qr[NS];
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          ql[l] = B.cv[pl + (5 + l) * s];
+          qr[l] = B.cv[pr + (5 + l) * s];
+        }
+        pp_limit<NS>(fc + 5, f1, ql, qr, jac1, jac2, *A.dt, tl, m);
+      }
+    }
+  }
+  __syncthreads();  // every face has read its stencil; the LDS now holds face fluxes
+  double* fcs = lds;  // [NV][F * TW], face slot col * F + f
+  if (active) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) fcs[l * (F * TW) + col * F + f] = fc[l];
+  }
+  __syncthreads();
+
+  // 3. cell t0-1+f (f >= 1): dq -= fc(right) - fc(left)
+  if (f >= 1 && active) {
+    const long long q0 = pid0(B, idx[0], idx[1], idx[2]);
+    const long long n0 = B.n0;
+#pragma unroll
+    for (int l = 0; l < NV; ++l)
+      B.dq[q0 + l * n0] -= fcs[l * (F * TW) + col * F + f] - fcs[l * (F * TW) + col * F + f - 1];
+  }
+}
+
+// One viscous sweep along DIR: compute_{fv,gv,hv}_2nd_order<FR, Laminar>,
+// src/ViscousScheme.cuh:23-318, :320-616, :618-909; one thread per face,
+// face fluxes through LDS, dq += fv(right) - fv(left).
+template <int DIR, int NS>
+__global__ __launch_bounds__(kT) void k_viscous(FrArgs A) {
+  constexpr int NV = 5 + NS;
+  constexpr int TW = Tile<DIR>::TW, F = Tile<DIR>::F, TS = F - 1;
+  extern __shared__ double lds[];
+  const FrBlockDev& B = A.B;
+  const int ext[3] = {B.mx, B.my, B.mz};
+  const int m = ext[DIR];
+  const int across = DIR == 0 ? 1 : 0;
+  const int col0 = blockIdx.y * TW;
+  const int third = blockIdx.z;
+  const int t0 = blockIdx.x * TS;
+  const long long s = B.sz;
+  const int tid = threadIdx.x;
+  const int f = tid % F, col = tid / F;
+  const int tl = t0 - 1 + f;
+  int idx[3];
+  tile_idx<DIR>(col0, tl, col, third, idx);
+  const bool active = idx[across] < ext[across] && tl < m;
+  double fv[NV];
+  if (active) {
+    const int i = idx[0], j = idx[1], k = idx[2];
+    int o[3] = {0, 0, 0};
+    o[DIR] = 1;
+    const long long p = pid(B, i, j, k), p1 = pid(B, i + o[0], j + o[1], k + o[2]);
+    auto M = [&](long long pt, int r, int c) { return B.met[pt + (3 * (r - 1) + (c - 1)) * s]; };
+    const double xi_x = 0.5 * (M(p, 1, 1) + M(p1, 1, 1));
+    const double xi_y = 0.5 * (M(p, 1, 2) + M(p1, 1, 2));
+    const double xi_z = 0.5 * (M(p, 1, 3) + M(p1, 1, 3));
+    const double eta_x = 0.5 * (M(p, 2, 1) + M(p1, 2, 1));
+    const double eta_y = 0.5 * (M(p, 2, 2) + M(p1, 2, 2));
+    const double eta_z = 0.5 * (M(p, 2, 3) + M(p1, 2, 3));
+    const double zeta_x = 0.5 * (M(p, 3, 1) + M(p1, 3, 1));
+    const double zeta_y = 0.5 * (M(p, 3, 2) + M(p1, 3, 2));
+    const double zeta_z = 0.5 * (M(p, 3, 3) + M(p1, 3, 3));
+    // normal derivative: 2 points; tangential: average of the two 2-point
+    // central differences at the face's cells
+    auto deriv = [&](const double* fp_, double* d) {
+      auto Fv = [&](int ii, int jj, int kk) { return fp_[pid(B, ii, jj, kk)]; };
+      if (DIR == 0) {
+        d[0] = Fv(i + 1, j, k) - Fv(i, j, k);
+        d[1] = 0.25 * (Fv(i, j + 1, k) - Fv(i, j - 1, k) + Fv(i + 1, j + 1, k) - Fv(i + 1, j - 1, k));
+        d[2] = 0.25 * (Fv(i, j, k + 1) - Fv(i, j, k - 1) + Fv(i + 1, j, k + 1) - Fv(i + 1, j, k - 1));
+      } else if (DIR == 1) {
+        d[0] = 0.25 * (Fv(i + 1, j, k) - Fv(i - 1, j, k) + Fv(i + 1, j + 1, k) - Fv(i - 1, j + 1, k));
+        d[1] = Fv(i, j + 1, k) - Fv(i, j, k);
+        d[2] = 0.25 * (Fv(i, j, k + 1) - Fv(i, j, k - 1) + Fv(i, j + 1, k + 1) - Fv(i, j + 1, k - 1));
+      } else {
+        d[0] = 0.25 * (Fv(i + 1, j, k) - Fv(i - 1, j, k) + Fv(i + 1, j, k + 1) - Fv(i - 1, j, k + 1));
+        d[1] = 0.25 * (Fv(i, j + 1, k) - Fv(i, j - 1, k) + Fv(i, j + 1, k + 1) - Fv(i, j - 1, k + 1));
+        d[2] = Fv(i, j, k + 1) - Fv(i, j, k);
+      }
+    };
+    double du[3], dv[3], dw[3], dT[3];
+    deriv(B.bv + 1 * s, du);
+    deriv(B.bv + 2 * s, dv);
+    deriv(B.bv + 3 * s, dw);
+    deriv(B.bv + 5 * s, dT);
+    const double u_x = du[0] * xi_x + du[1] * eta_x + du[2] * zeta_x;
+    const double u_y = du[0] * xi_y + du[1] * eta_y + du[2] * zeta_y;
+    const double u_z = du[0] * xi_z + du[1] * eta_z + du[2] * zeta_z;
+    const double v_x = dv[0] * xi_x + dv[1] * eta_x + dv[2] * zeta_x;
+    const double v_y = dv[0] * xi_y + dv[1] * eta_y + dv[2] * zeta_y;
+    const double v_z = dv[0] * xi_z + dv[1] * eta_z + dv[2] * zeta_z;
+    const double w_x = dw[0] * xi_x + dw[1] * eta_x + dw[2] * zeta_x;
+    const double w_y = dw[0] * xi_y + dw[1] * eta_y + dw[2] * zeta_y;
+    const double w_z = dw[0] * xi_z + dw[1] * eta_z + dw[2] * zeta_z;
+    const double mul = 0.5 * (B.mul[p] + B.mul[p1]);
+    const double viscosity = mul + 0.0;
+    const double tau_xx = viscosity * (4 * u_x - 2 * v_y - 2 * w_z) / 3.0;
+    const double tau_yy = viscosity * (4 * v_y - 2 * u_x - 2 * w_z) / 3.0;
+    const double tau_zz = viscosity * (4 * w_z - 2 * u_x - 2 * v_y) / 3.0;
+    const double tau_xy = viscosity * (u_y + v_x);
+    const double tau_xz = viscosity * (u_z + w_x);
+    const double tau_yz = viscosity * (v_z + w_y);
+    constexpr int r = DIR + 1;
+    const double kx = 0.5 * (M(p, r, 1) * B.jac[p] + M(p1, r, 1) * B.jac[p1]);
+    const double ky = 0.5 * (M(p, r, 2) * B.jac[p] + M(p1, r, 2) * B.jac[p1]);
+    const double kz = 0.5 * (M(p, r, 3) * B.jac[p] + M(p1, r, 3) * B.jac[p1]);
+    fv[0] = 0;
+    fv[1] = kx * tau_xx + ky * tau_xy + kz * tau_xz;
+    fv[2] = kx * tau_xy + ky * tau_yy + kz * tau_yz;
+    fv[3] = kx * tau_xz + ky * tau_yz + kz * tau_zz;
+    const double um = 0.5 * (B.bv[p + s] + B.bv[p1 + s]);
+    const double vm = 0.5 * (B.bv[p + 2 * s] + B.bv[p1 + 2 * s]);
+    const double wm = 0.5 * (B.bv[p + 3 * s] + B.bv[p1 + 3 * s]);
+    const double t_x = dT[0] * xi_x + dT[1] * eta_x + dT[2] * zeta_x;
+    const double t_y = dT[0] * xi_y + dT[1] * eta_y + dT[2] * zeta_y;
+    const double t_z = dT[0] * xi_z + dT[1] * eta_z + dT[2] * zeta_z;
+    const double conductivity = 0.5 * (B.lam[p] + B.lam[p1]);
+    fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + conductivity * (kx * t_x + ky * t_y + kz * t_z);
+    double diffusivity[NS], yk[NS], ddf[NS];
+    double sum_grad_over_w{0}, sum_rhoDkYk{0}, corr{0}, mw_tot{0};
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      yk[l] = 0.5 * (B.sv[p + l * s] + B.sv[p1 + l * s]);
+      diffusivity[l] = 0.5 * (B.rhod[p + l * s] + B.rhod[p1 + l * s]) + 0.0;
+      double dy[3];
+      deriv(B.sv + l * s, dy);
+      const double y_x = dy[0] * xi_x + dy[1] * eta_x + dy[2] * zeta_x;
+      const double y_y = dy[0] * xi_y + dy[1] * eta_y + dy[2] * zeta_y;
+      const double y_z = dy[0] * xi_z + dy[1] * eta_z + dy[2] * zeta_z;
+      const double gy = kx * y_x + ky * y_y + kz * y_z;
+      ddf[l] = gy;
+      corr += diffusivity[l] * gy;
+      sum_grad_over_w += gy / A.T->mw[l];
+      mw_tot += yk[l] / A.T->mw[l];
+      sum_rhoDkYk += diffusivity[l] * yk[l];
+    }
+    mw_tot = 1.0 / mw_tot;
+    corr -= mw_tot * sum_rhoDkYk * sum_grad_over_w;
+    if (A.grad_p) {
+      double dp[3];
+      deriv(B.bv + 4 * s, dp);
+      const double p_x{dp[0] * xi_x + dp[1] * eta_x + dp[2] * zeta_x};
+      const double p_y{dp[0] * xi_y + dp[1] * eta_y + dp[2] * zeta_y};
+      const double p_z{dp[0] * xi_z + dp[1] * eta_z + dp[2] * zeta_z};
+      const double gp = (kx * p_x + ky * p_y + kz * p_z) / (0.5 * (B.bv[p1 + 4 * s] + B.bv[p + 4 * s]));
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        ddf[l] += (mw_tot / A.T->mw[l] - 1) * yk[l] * gp;
+        corr += (mw_tot / A.T->mw[l] - 1) * yk[l] * gp * diffusivity[l];
+      }
+    }
+    double h[NS];
+    const double tm = 0.5 * (B.bv[p + 5 * s] + B.bv[p1 + 5 * s]);
+    enthalpy<NS>(A.T, tm, h);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const double flux{diffusivity[l] * (ddf[l] - mw_tot * yk[l] * sum_grad_over_w) - yk[l] * corr};
+      fv[5 + l] = flux;
+      fv[4] += h[l] * flux;
+    }
+#pragma unroll
+    for (int l = 0; l < NV; ++l) lds[l * (F * TW) + col * F + f] = fv[l];
+  }
+  __syncthreads();
+  if (f >= 1 && active) {
+    const long long q0 = pid0(B, idx[0], idx[1], idx[2]);
+    const long long n0 = B.n0;
+#pragma unroll
+    for (int l = 0; l < NV; ++l)
+      B.dq[q0 + l * n0] += lds[l * (F * TW) + col * F + f] - lds[l * (F * TW) + col * F + f - 1];
+  }
+}
+
+unsigned nblk(long long n) { return (unsigned)((n + kT - 1) / kT); }
+
+template <int NS>
+void launch_rhs_t(const FrArgs& A, hipStream_t st, long long* n, KernelTimer* tm) {
+  const FrBlockDev& B = A.B;
+  const int ext[3] = {B.mx, B.my, B.mz};
+  auto rec = [&](int which, bool end) {
+    if (tm) tm->mark(which, end, st);
+  };
+  rec(KT_SOURCE, false);
+  hipLaunchKernelGGL(k_source<NS>, dim3(nblk(B.n0)), dim3(kT), 0, st, A);
+  rec(KT_SOURCE, true);
+  ++*n;
+  auto inv = [&](auto dirc) {
+    constexpr int D = decltype(dirc)::value;
+    constexpr int TW = Tile<D>::TW, F = Tile<D>::F, TS = F - 1, P = TS + 6;
+    const int across = D == 0 ? 1 : 0, third = D == 2 ? 1 : 2;
+    dim3 grid((ext[D] + TS - 1) / TS, (ext[across] + TW - 1) / TW, ext[third]);
+    const size_t lds = (size_t)2 * (5 + NS) * P * TW * sizeof(double);
+    rec(KT_INV_X + D, false);
+    if (A.scheme == 52)
+      hipLaunchKernelGGL((k_inviscid<D, NS, 52>), grid, dim3(kT), lds, st, A);
+    else
+      hipLaunchKernelGGL((k_inviscid<D, NS, 51>), grid, dim3(kT), lds, st, A);
+    rec(KT_INV_X + D, true);
+    ++*n;
+  };
+  inv(std::integral_constant<int, 0>{});
+  inv(std::integral_constant<int, 1>{});
+  inv(std::integral_constant<int, 2>{});
+  if (A.viscous) {
+    auto vis = [&](auto dirc) {
+      constexpr int D = decltype(dirc)::value;
+      constexpr int TW = Tile<D>::TW, F = Tile<D>::F, TS = F - 1;
+      const int across = D == 0 ? 1 : 0, third = D == 2 ? 1 : 2;
+      dim3 grid((ext[D] + TS - 1) / TS, (ext[across] + TW - 1) / TW, ext[third]);
+      const size_t lds = (size_t)(5 + NS) * F * TW * sizeof(double);
+      rec(KT_VIS_X + D, false);
+      hipLaunchKernelGGL((k_viscous<D, NS>), grid, dim3(kT), lds, st, A);
+      rec(KT_VIS_X + D, true);
+      ++*n;
+    };
+    vis(std::integral_constant<int, 0>{});
+    vis(std::integral_constant<int, 1>{});
+    vis(std::integral_constant<int, 2>{});
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+bool supported_ns(int ns) { return ns == 9; }
+
+#define OSTC_FR_DISPATCH(NSV, CALL) \
+  switch (NSV) {                    \
+    case 9: {                       \
+      constexpr int NS = 9;         \
+      CALL;                         \
+    } break;                        \
+    default: break;                 \
+  }
+
+void launch_rhs(const FrArgs& A, int ns, hipStream_t st, long long* n, KernelTimer* tm) {
+  OSTC_FR_DISPATCH(ns, launch_rhs_t<NS>(A, st, n, tm))
+}
+
+void launch_props(const FrArgs& A, int ns, hipStream_t st, long long* n, KernelTimer* tm) {
+  if (tm) tm->mark(KT_PROPS, false, st);
+  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_props<NS>, dim3(nblk(A.B.sz)), dim3(kT), 0, st, A))
+  if (tm) tm->mark(KT_PROPS, true, st);
+  ++*n;
+}
+
+void launch_cv_from_bv(const FrArgs& A, int ns, hipStream_t st, long long* n) {
+  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_cv_from_bv<NS>, dim3(nblk(A.B.sz)), dim3(kT), 0, st, A))
+  ++*n;
+}
+
+void launch_update(const FrArgs& A, int ns, double ra, double rb, double rc, hipStream_t st, long long* n,
+                   KernelTimer* tm) {
+  if (tm) tm->mark(KT_UPDATE, false, st);
+  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_update<NS>, dim3(nblk(A.B.n0)), dim3(kT), 0, st, A, ra, rb, rc))
+  if (tm) tm->mark(KT_UPDATE, true, st);
+  ++*n;
+}
+
+void launch_local_dt(const FrArgs& A, unsigned long long* dt_bits, hipStream_t st, long long* n) {
+  hipLaunchKernelGGL(k_local_dt, dim3(nblk(A.B.n0)), dim3(kT), 0, st, A, dt_bits);
+  ++*n;
+}
+
+void launch_dt_set(double* dt, double v, hipStream_t st, long long* n) {
+  hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(1), 0, st, dt, v);
+  ++*n;
+}
+
+void launch_bc(const FrArgs& A, int ns, int axis, int side, int type, hipStream_t st, long long* n) {
+  const int ext[3] = {A.B.mx, A.B.my, A.B.mz};
+  const long long cnt = (long long)(ext[(axis + 1) % 3] + 2 * A.B.g) * (ext[(axis + 2) % 3] + 2 * A.B.g);
+  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_bc<NS>, dim3(nblk(cnt)), dim3(kT), 0, st, A, axis, side, type))
+  ++*n;
+}
+
+void launch_comm(const FrArgs& A, const FrBlockDev& Tg, int ns, int axis, int side, hipStream_t st, long long* n) {
+  const int ext[3] = {A.B.mx, A.B.my, A.B.mz};
+  const long long cnt = (long long)ext[(axis + 1) % 3] * ext[(axis + 2) % 3] * (A.B.g + 1);
+  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_comm<NS>, dim3(nblk(cnt)), dim3(kT), 0, st, A, Tg, axis, side))
+  ++*n;
+}
+
+size_t inviscid_lds_bytes(int ns, int dir) {
+  const int tw = dir == 0 ? Tile<0>::TW : Tile<1>::TW, f = dir == 0 ? Tile<0>::F : Tile<1>::F;
+  return (size_t)2 * (5 + ns) * (f - 1 + 6) * tw * sizeof(double);
+}
+
+}  // namespace fr
+}  // namespace ostc

@@ -1,0 +1,235 @@
+"""ctypes binding of the finite-rate mixture oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+import this module.  The oracle runs on tables read by `mech.load` (this
+directory's own reader), never by the product's reader.  Geometry and the
+velocity/thermodynamic initial state come from the air oracle's routines
+(uniform grid metric, seeded tanh profile), species from the two streams.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+import mech
+import oracle as OA
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libostc_oracle_fr.so")
+R_U = 8314.462618
+
+(CV, BV, SV, QN, DQ, MUL, LAMBDA, CP, GAMMA, C_SOUND, MACH, RHOD, JAC, METRIC, DT_LOCAL) = range(15)
+
+
+class _Setup(C.Structure):
+    _fields_ = [("inviscid_scheme", C.c_int), ("viscous_order", C.c_int), ("reaction", C.c_int),
+                ("positive_preserving", C.c_int), ("grad_p_diffusion", C.c_int), ("fixed_time_step", C.c_int),
+                ("dt", C.c_double), ("cfl", C.c_double), ("prandtl", C.c_double),
+                ("rho_ref", C.c_double), ("v_ref", C.c_double), ("n_threads", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            OA.build(force=True)
+        L = C.CDLL(LIB_PATH)
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        dp, tp = C.POINTER(C.c_double), C.POINTER(mech.MechTables)
+        L.orf_create.restype = vp
+        L.orf_create.argtypes = [C.POINTER(_Setup), tp]
+        L.orf_destroy.argtypes = [vp]
+        L.orf_add_block.argtypes = [vp, i, i, i]
+        L.orf_ngg.argtypes = [vp]
+        L.orf_add_boundary.argtypes = [vp, i, i, i, i]
+        L.orf_add_inner_face.argtypes = [vp, i, i, i, i]
+        L.orf_array.restype = dp
+        L.orf_array.argtypes = [vp, i, i, C.POINTER(C.c_longlong)]
+        L.orf_initialize.argtypes = [vp]
+        L.orf_step.argtypes = [vp, i, dp]
+        for n in ("orf_cv_from_bv", "orf_properties", "orf_zero_dq", "orf_source", "orf_viscous", "orf_local_dt",
+                  "orf_apply_bc"):
+            getattr(L, n).argtypes = [vp, i]
+        L.orf_inviscid.argtypes = [vp, i, d]
+        L.orf_global_dt.restype = d
+        L.orf_global_dt.argtypes = [vp]
+        L.orf_update.argtypes = [vp, i, d, i]
+        L.orf_communicate.argtypes = [vp]
+        L.orf_enthalpy_cp.argtypes = [tp, d, dp, dp]
+        L.orf_gibbs_rt.argtypes = [tp, d, dp]
+        L.orf_chemical_source.argtypes = [tp, d, d, dp, dp, dp, dp]
+        L.orf_transport.argtypes = [tp, d, d, d, dp, dp, dp, dp]
+        L.orf_temperature.restype = d
+        L.orf_temperature.argtypes = [tp, d, dp, d, C.POINTER(C.c_int)]
+        L.orf_char_matrices.argtypes = [tp, dp, dp, dp, dp, d, d, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def tables(mech_file, therm_file, tran_file, reaction=1):
+    return mech.load(mech_file, therm_file, tran_file, reaction)
+
+
+# ----------------------------------------------------------- point routines
+def enthalpy_cp(T, t):
+    h, cp = np.zeros(mech.MAX_SPEC), np.zeros(mech.MAX_SPEC)
+    lib().orf_enthalpy_cp(C.byref(T), t, _dp(h), _dp(cp))
+    return h[:T.n_spec], cp[:T.n_spec]
+
+
+def gibbs_rt(T, t):
+    g = np.zeros(mech.MAX_SPEC)
+    lib().orf_gibbs_rt(C.byref(T), t, _dp(g))
+    return g[:T.n_spec]
+
+
+def chemical_source(T, rho, t, y):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    om, kf, kb = np.zeros(mech.MAX_SPEC), np.zeros(mech.MAX_REAC), np.zeros(mech.MAX_REAC)
+    lib().orf_chemical_source(C.byref(T), rho, t, _dp(y), _dp(om), _dp(kf), _dp(kb))
+    return om[:T.n_spec], kf[:T.n_reac], kb[:T.n_reac]
+
+
+def transport(T, t, p, rho, y):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    mu, lam, rd = np.zeros(1), np.zeros(1), np.zeros(mech.MAX_SPEC)
+    lib().orf_transport(C.byref(T), t, p, rho, _dp(y), _dp(mu), _dp(lam), _dp(rd))
+    return mu[0], lam[0], rd[:T.n_spec]
+
+
+def temperature(T, e, y, t_start):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    it = C.c_int(0)
+    t = lib().orf_temperature(C.byref(T), e, _dp(y), t_start, C.byref(it))
+    return t, it.value
+
+
+def char_matrices(T, ql, qr, kl, kr, jl, jr):
+    nv = 5 + T.n_spec
+    L, R = np.zeros(nv * nv), np.zeros(nv * nv)
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (ql, qr, kl, kr)]
+    lib().orf_char_matrices(C.byref(T), *[_dp(a) for a in args], jl, jr, _dp(L), _dp(R))
+    return L.reshape(nv, nv), R.reshape(nv, nv)
+
+
+def mixture_streams(T, case):
+    """Stream states of a two-stream mixture (get_mixing_layer_info with
+    species, src/MixingLayer.cu:35-110): [rho,u,v,w,p,T] and Y of the upper
+    (fuel) and lower (oxidizer) streams, upper_faster with ma_upper = ma_c."""
+    out = {}
+    cs = {}
+    for side in ("upper", "lower"):
+        y = np.array(case["y_" + side], dtype=np.float64)
+        t, p = case["t_" + side], case["p_" + side]
+        _, cp = enthalpy_cp(T, t)
+        mw_inv = 0.0
+        cpm = 0.0
+        for i in range(T.n_spec):
+            mw_inv += y[i] / T.mw[i]
+            cpm += cp[i] * y[i]
+        gam = cpm / (cpm - R_U * mw_inv)
+        cs[side] = math.sqrt(gam * R_U * mw_inv * t)
+        out[side] = [p / (R_U * mw_inv * t), 0.0, 0.0, 0.0, p, t]
+    ma_c = case["ma_c"]
+    ma_l = (ma_c - ma_c) * cs["upper"] / cs["lower"] - ma_c
+    out["upper"][1] = ma_c * cs["upper"]
+    out["lower"][1] = ma_l * cs["lower"]
+    return out
+
+
+class FrOracle:
+    """A set of blocks of the finite-rate mixture path on the host."""
+
+    def __init__(self, case, T=None, n_threads=0):
+        self.case = case
+        self.T = T if T is not None else tables(*case["mech_files"], reaction=case["reaction"])
+        self.L = L = lib()
+        s = _Setup(case["inviscid_scheme"], case["viscous_order"], case["reaction"],
+                   int(case.get("positive_preserving", 0)), int(case.get("grad_p_diffusion", 0)),
+                   int(case.get("fixed_time_step", 0)), float(case.get("dt", 0.0)), case["cfl"],
+                   case["prandtl"], case["rho_ref"], case["v_ref"], n_threads)
+        self.ctx = L.orf_create(C.byref(s), C.byref(self.T))
+        if not self.ctx:
+            raise ValueError("unsupported setup")
+        self.ngg = L.orf_ngg(self.ctx)
+        self.ns = self.T.n_spec
+        for b in case["blocks"]:
+            L.orf_add_block(self.ctx, b["mx"], b["my"], b["mz"])
+        for f in case["boundaries"]:
+            L.orf_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"])
+        for f in case["inner_faces"]:
+            L.orf_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"])
+
+    def close(self):
+        if self.ctx:
+            self.L.orf_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def array(self, block, which):
+        n = C.c_longlong(0)
+        ptr = self.L.orf_array(self.ctx, block, which, C.byref(n))
+        flat = np.ctypeslib.as_array(ptr, shape=(n.value,))
+        b = self.case["blocks"][block]
+        g = self.ngg
+        if which in (DQ, DT_LOCAL):
+            shape = (b["mz"], b["my"], b["mx"])
+        else:
+            shape = (b["mz"] + 2 * g, b["my"] + 2 * g, b["mx"] + 2 * g)
+        return flat.reshape((-1,) + shape)
+
+    def setup_mixing_layer(self):
+        """Geometry and the initial state of every block: the air oracle's
+        uniform-grid metric and seeded tanh/velocity-perturbation profile on
+        the mixture stream states, species from the stream of each node
+        (y >= 0 upper), src/Initialize.cu:259-325."""
+        case = self.case
+        st = case["streams"]
+        ao = OA.Oracle(dict(case, n_var=5, inviscid_scheme=51), n_threads=1)
+        streams = OA._Streams((C.c_double * 6)(*st["upper"]), (C.c_double * 6)(*st["lower"]))
+        g = self.ngg
+        for bi, b in enumerate(case["blocks"]):
+            ao.L.orc_uniform_grid(ao.ctx, bi, b["x0"], b["y0"], b["z0"], case["dx"], case["dy"], case["dz"])
+            ao.L.orc_compute_metric(ao.ctx, bi)
+            px, py, pz = case["periods"]
+            ao.L.orc_mixing_layer_ic(ao.ctx, bi, C.byref(streams), case["delta_omega"], b["gi0"], b["gj0"],
+                                     b["gk0"], px, py, pz, case["seed"], case["amplitude"])
+            self.array(bi, JAC)[...] = ao.array(bi, OA.JAC)
+            self.array(bi, METRIC)[...] = ao.array(bi, OA.METRIC)
+            self.array(bi, BV)[...] = ao.array(bi, OA.BV)
+            yv = b["y0"] + (np.arange(-g, b["my"] + g)) * case["dy"]
+            sv = self.array(bi, SV)
+            up = yv >= 0
+            for l in range(self.ns):
+                sv[l][:, up, :] = case["y_upper"][l]
+                sv[l][:, ~up, :] = case["y_lower"][l]
+        del ao
+
+    def initialize(self):
+        self.L.orf_initialize(self.ctx)
+
+    def step(self, n=1):
+        dt = C.c_double(0)
+        self.L.orf_step(self.ctx, n, C.byref(dt))
+        return dt.value
+
+    def rhs(self, block, dt=0.0):
+        """dq of one block at its current state: source, inviscid, viscous."""
+        L = self.L
+        L.orf_zero_dq(self.ctx, block)
+        L.orf_source(self.ctx, block)
+        L.orf_inviscid(self.ctx, block, dt)
+        L.orf_viscous(self.ctx, block)
+        return self.array(block, DQ)

@@ -1,0 +1,86 @@
+/*
+ * ostc_oracle_fr.h -- C interface of the CPU oracle for the finite-rate
+ * mixture path (the reference's MixtureModel::FR with laminar transport).
+ *
+ * TEST INFRASTRUCTURE ONLY.  A host-C++ (OpenMP over cells) restatement of
+ * the reference's mixture routines: NASA-7 thermodynamics, mixture-averaged
+ * transport, finite-rate chemistry, the temperature Newton iteration,
+ * component-wise and characteristic WENO-Z fluxes with the positivity
+ * limiter, second-order viscous fluxes with species diffusion, the SSP-RK3
+ * update, boundary copies and inner-face exchange.  Nothing in the product
+ * links, loads or calls it; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may use it.
+ *
+ * Layout: as the air oracle (ostc_oracle.h) -- per-variable planes, i
+ * fastest, g = ngg ghost layers on cv [n_var], bv [6], sv [n_spec], qn, the
+ * property planes and geometry; g = 0 for dq [n_var] and dt_local.
+ * n_var = 5 + n_spec.
+ */
+#ifndef OSTC_ORACLE_FR_H
+#define OSTC_ORACLE_FR_H
+#include <stdint.h>
+#include "openstc/mech.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int    inviscid_scheme;      /* 51 WENO-Z5 component-wise, 52 WENO-Z5 characteristic */
+  int    viscous_order;        /* 0 or 2 */
+  int    reaction;             /* 1: finite-rate chemistry source */
+  int    positive_preserving;  /* 1: limiter on the species fluxes */
+  int    grad_p_diffusion;     /* gradPInDiffusionFlux */
+  int    fixed_time_step;
+  double dt, cfl;
+  double prandtl;              /* prandtl_number (viscous spectral radius of the time step) */
+  double rho_ref, v_ref;       /* weno_eps_scale = (rho_ref v_ref)^2 */
+  int    n_threads;
+} orf_setup;
+
+/* arrays */
+enum { ORF_CV = 0, ORF_BV, ORF_SV, ORF_QN, ORF_DQ, ORF_MUL, ORF_LAMBDA, ORF_CP, ORF_GAMMA, ORF_C,
+       ORF_MACH, ORF_RHOD, ORF_JAC, ORF_METRIC, ORF_DT_LOCAL };
+
+void*   orf_create(const orf_setup* s, const ostc_mech_tables* t);
+void    orf_destroy(void* ctx);
+int     orf_add_block(void* ctx, int mx, int my, int mz);
+int     orf_ngg(void* ctx);
+int     orf_add_boundary(void* ctx, int block, int axis, int side, int type);  /* 6 outflow, 8 periodic */
+int     orf_add_inner_face(void* ctx, int block, int axis, int side, int target_block);
+double* orf_array(void* ctx, int block, int which, long long* n_out);
+
+int     orf_initialize(void* ctx);            /* BCs, exchange, properties, cv from bv */
+int     orf_step(void* ctx, int n_steps, double* last_dt);
+/* stage pieces */
+int     orf_cv_from_bv(void* ctx, int block);
+int     orf_properties(void* ctx, int block);
+int     orf_zero_dq(void* ctx, int block);
+int     orf_source(void* ctx, int block);
+int     orf_inviscid(void* ctx, int block, double dt);
+int     orf_viscous(void* ctx, int block);
+int     orf_local_dt(void* ctx, int block);
+double  orf_global_dt(void* ctx);
+int     orf_update(void* ctx, int block, double dt, int rk);
+int     orf_apply_bc(void* ctx, int block);
+int     orf_communicate(void* ctx);
+
+/* point routines, for closed-form and identity tests */
+void    orf_enthalpy_cp(const ostc_mech_tables* t, double temp, double* h, double* cp);
+void    orf_gibbs_rt(const ostc_mech_tables* t, double temp, double* g);
+/* omega [ns] (kg/m3/s) for density rho, temperature t, mass fractions y */
+void    orf_chemical_source(const ostc_mech_tables* t, double rho, double temp, const double* y, double* omega,
+                            double* kf, double* kb);
+/* mu, lambda, rho_D [ns] of a mixture at (T, p, rho, Y) */
+void    orf_transport(const ostc_mech_tables* t, double temp, double p, double rho, const double* y, double* mu,
+                      double* lambda, double* rho_d);
+/* T-Newton: returns T, *iters receives the iteration count */
+double  orf_temperature(const ostc_mech_tables* t, double e, const double* y, double t_start, int* iters);
+/* left (5+ns)x(5+ns) characteristic matrix rows L and the back projection R
+ * of compute_weno_flux_ch at a face, for the L R = I identity */
+int     orf_char_matrices(const ostc_mech_tables* t, const double* ql, const double* qr, const double* kl,
+                          const double* kr, double jl, double jr, double* L, double* R);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,6 +1,7 @@
 """The C-ABI library loads on a machine without a GPU and exports every
-function that include/openstc/abi.h declares (no compute calls)."""
+function that include/openstc/*.h declares (no compute calls)."""
 import ctypes
+import glob
 import os
 import re
 
@@ -8,7 +9,7 @@ from _util import ROOT, pkg
 
 
 def declared_functions():
-    text = open(os.path.join(ROOT, "include", "openstc", "abi.h")).read()
+    text = "".join(open(p).read() for p in sorted(glob.glob(os.path.join(ROOT, "include", "openstc", "*.h"))))
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(ostc_[a-z_0-9]+)\s*\(", text)))
 
