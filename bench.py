@@ -101,6 +101,72 @@ def adjoint_leg(n: int, steps: int, every: int, fd_modes=(0, 1), h=1e-6):
                         f"momentum thickness"}
 
 
+def reacting_leg(x_cells, y_cells, z_cells, steps: int, warmup: int, label: str, cpu_sample=None):
+    """Reacting H2/air temporal mixing layer (9 species, 19 reactions,
+    characteristic WENO-Z5 + LF-type splitting, mixture-averaged transport,
+    finite-rate chemistry, SSP-RK3 CFL 0.5) through the finite-rate path:
+    `steps` graph-replayed steps after `warmup`, then one step with HIP events
+    around every kernel for the per-kernel breakdown."""
+    mx = pkg.mixture
+    tables = mx.load_mechanism(*mx.H2_AIR_FILES)
+    case = pkg.cases.reacting_case(x_cells, y_cells, z_cells, tables, scheme=52)
+    cells = case["n_cells"]
+    print(f"[bench] {label}: {len(case['blocks'])} blocks, {cells} cells", file=sys.stderr, flush=True)
+    s = mx.FrSolver(case, tables)
+    s.setup_mixing_layer()
+    s.initialize()
+    s.rk3(warmup, mx.MODE_GRAPH)
+    s.sync()
+    t0 = time.perf_counter()
+    s.rk3(steps, mx.MODE_GRAPH, want_dt=False)
+    s.sync()
+    wall = time.perf_counter() - t0
+    s.rk3(1, mx.MODE_TIMED, want_dt=False)
+    kms = s.kernel_ms()
+    dt_last = s.rk3(1, mx.MODE_GRAPH)
+    bv = s.get(0, mx.FIELD_BV)
+    finite = bool(np.isfinite(bv).all())
+    s.close()
+    value = cells * steps / wall / 1e6
+    total_k = sum(v[0] for v in kms.values())
+    top = max(kms.items(), key=lambda kv: kv[1][0])
+    print(f"[bench] {label}: {wall * 1e3 / steps:.2f} ms/step, {value:.1f} Mcell-updates/s, dt {dt_last:.3e}, "
+          f"finite {finite}; kernels (ms/step): " +
+          ", ".join(f"{k} {v[0]:.2f}" for k, v in kms.items()), file=sys.stderr, flush=True)
+    out = {"value": value, "unit": "Mcell-updates/s", "ms_per_step": wall * 1e3 / steps, "steps": steps,
+           "warmup": warmup, "cells": cells, "blocks": len(case["blocks"]), "state_finite": finite,
+           "kernel_ms_per_step": {k: v[0] for k, v in kms.items()},
+           "kernel_launches_per_step": {k: int(v[1]) for k, v in kms.items()},
+           "dominant_kernel": top[0], "dominant_kernel_share": top[1][0] / total_k,
+           "workload": f"{label}: temporal H2/air mixing layer, 9 species / 19 reactions, WENO-Z5 characteristic "
+                       f"+ LF-type splitting, mixture-averaged transport, finite-rate chemistry, SSP-RK3 CFL 0.5"}
+    if cpu_sample is not None:
+        out["cpu_baseline"] = reacting_cpu_baseline(tables, *cpu_sample)
+    return out
+
+
+def reacting_cpu_baseline(tables, n, steps):
+    """The CPU oracle of the finite-rate path on a bounded sample: `steps`
+    RK3 steps of an n x n/2 x n/2 block of the same reacting layer."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mech
+    import oracle_fr as OF
+
+    threads = min(os.cpu_count() or 1, 16)
+    to = mech.load(*pkg.mixture.H2_AIR_FILES)
+    case = pkg.cases.reacting_single_block(n, n // 2, n // 2, tables, scheme=52)
+    o = OF.FrOracle(case, to, n_threads=threads)
+    o.setup_mixing_layer()
+    o.initialize()
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    cells = case["n_cells"]
+    return {"value": cells * steps / dt / 1e6, "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} RK3 steps of a {n}x{n // 2}x{n // 2} block ({cells} cells) of the same reacting "
+                      f"layer, {dt:.1f} s on {threads} OpenMP threads"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +180,9 @@ def main():
     ap.add_argument("--adjoint-n", type=int, default=128)
     ap.add_argument("--adjoint-steps", type=int, default=50)
     ap.add_argument("--adjoint-every", type=int, default=10)
+    ap.add_argument("--no-reacting", action="store_true")
+    ap.add_argument("--reacting-steps", type=int, default=5)
+    ap.add_argument("--no-large", action="store_true", help="skip the 134M-cell reacting case")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -182,6 +251,20 @@ def main():
     if not a.no_adjoint:
         adjoint = adjoint_leg(a.adjoint_n, a.adjoint_steps, a.adjoint_every)
 
+    reacting = None
+    if not a.no_reacting:
+        reacting = {"single_block_512x256x128": reacting_leg(
+            [511], [255], [127], a.reacting_steps, 2, "512x256x128 single block",
+            cpu_sample=None if a.no_cpu else (64, 1))}
+        if not a.no_large:
+            rng = np.random.default_rng(42)
+            xs = [int(v) for v in rng.integers(384, 640, size=1)]
+            ys = [int(v) for v in rng.integers(192, 320, size=1)]
+            zs = [int(v) for v in rng.integers(96, 160, size=1)]
+            reacting["large_1024x512x256_8_blocks"] = reacting_leg(
+                [xs[0], 1023 - xs[0]], [ys[0], 511 - ys[0]], [zs[0], 255 - zs[0]], a.reacting_steps, 1,
+                "1024x512x256 in 8 uneven blocks")
+
     cpu = None
     if not a.no_cpu:
         print(f"[bench] cpu baseline: {a.cpu_steps} oracle steps", file=sys.stderr, flush=True)
@@ -197,7 +280,7 @@ def main():
                    "tile": list(pkg.tile_shapes()[tile_index]), "workgroups_per_cu": tile_waves,
                    "split_stage": bool(tile_split)},
         "value_graph": value_graph, "plain_value": plain_value, "speedup_vs_plain": value / plain_value,
-        "roofline": roofline, "cpu_baseline": cpu, "adjoint": adjoint,
+        "roofline": roofline, "cpu_baseline": cpu, "adjoint": adjoint, "reacting": reacting,
     }
     print(json.dumps(out))
 
