@@ -183,11 +183,18 @@ def main():
     ap.add_argument("--no-reacting", action="store_true")
     ap.add_argument("--reacting-steps", type=int, default=5)
     ap.add_argument("--no-large", action="store_true", help="skip the 134M-cell reacting case")
+    ap.add_argument("--reacting-only", action="store_true",
+                    help="run only the 512x256x128 reacting case and print its result (profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
               "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
         sys.exit(2)
+
+    if a.reacting_only:
+        r = reacting_leg([511], [255], [127], a.reacting_steps, 2, "512x256x128 single block")
+        print(json.dumps({"reacting": {"single_block_512x256x128": r}}))
+        return
 
     case = pkg.cases.batched_case(seed=42)
     cells = case["n_cells"]
@@ -255,7 +262,7 @@ def main():
     if not a.no_reacting:
         reacting = {"single_block_512x256x128": reacting_leg(
             [511], [255], [127], a.reacting_steps, 2, "512x256x128 single block",
-            cpu_sample=None if a.no_cpu else (64, 1))}
+            cpu_sample=None if a.no_cpu else (96, 2))}
         if not a.no_large:
             rng = np.random.default_rng(42)
             xs = [int(v) for v in rng.integers(384, 640, size=1)]

@@ -165,9 +165,23 @@ __device__ __forceinline__ void transport(const Tab* __restrict__ T, double t, d
   }
 }
 
-// arrhenius, src/FiniteRateChem.cu:104-106
+// arrhenius, src/FiniteRateChem.cu:104-106: A * T^b * exp(-Ea/T), evaluated
+// as (A * pow(T, b)) * exp(-Ea / T).  A zero exponent or activation energy
+// makes its factor exactly 1, and multiplying by 1 is exact, so those factors
+// are skipped (the tables make the test wave-uniform).
 __device__ __forceinline__ double arrhenius(double t, double A, double b, double Ea) {
-  return A * pow(t, b) * exp(-Ea / t);
+  double r = A;
+  if (b != 0) r = A * pow(t, b);
+  if (Ea != 0) r = r * exp(-Ea / t);
+  return r;
+}
+
+// c^nu for a stoichiometric coefficient: pow(c, 0) = 1 and pow(c, 1) = c
+// exactly; c^2 is the correctly rounded square.
+__device__ __forceinline__ double stoich_pow(double c, int nu) {
+  if (nu == 1) return c;
+  if (nu == 2) return c * c;
+  return pow(c, (double)nu);
 }
 
 // finite_rate_chemistry (explicit), src/FiniteRateChem.cu:7-58 with
@@ -239,15 +253,19 @@ __device__ void chemistry(const Tab* __restrict__ T, double rho, double t, const
         double d_gibbs{0};
 #pragma unroll
         for (int l = 0; l < NS; ++l) d_gibbs += g[l] * (T->stoi_b[i][l] - T->stoi_f[i][l]);
-        const double kc{pow(temp_t, (double)T->order[i]) * exp(-d_gibbs)};
+        const int ord = T->order[i];
+        const double tp = ord == 0 ? 1.0 : (ord == 1 ? temp_t : pow(temp_t, (double)ord));
+        const double kc{tp * exp(-d_gibbs)};
         kb = kf / kc;
       }
     }
+    // rate of progress: factors with a zero coefficient are exactly 1
     double q1 = 1.0, q2 = type != 0 ? 1.0 : 0.0;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-      q1 *= pow(c[j], (double)T->stoi_f[i][j]);
-      if (type != 0) q2 *= pow(c[j], (double)T->stoi_b[i][j]);
+      const int nf = T->stoi_f[i][j], nb = T->stoi_b[i][j];
+      if (nf != 0) q1 *= stoich_pow(c[j], nf);
+      if (type != 0 && nb != 0) q2 *= stoich_pow(c[j], nb);
     }
     q1 *= kf;
     if (type != 0) q2 *= kb;
