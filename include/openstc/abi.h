@@ -267,12 +267,15 @@ int ostc_adjoint_timing(ostc_ctx ctx, double* forward_ms, double* backward_ms);
 typedef struct ostc_fr_ctx_s* ostc_fr_ctx;
 
 typedef struct {
-  int inviscid_scheme;       /* 51 WENO-Z5 component-wise, 52 WENO-Z5 characteristic */
+  int inviscid_scheme;       /* 51 WENO-Z5 component-wise, 52 WENO-Z5 characteristic,
+                                3 AUSM+, 4 HLLC on reconstructed primitives (inviscid_type 0) */
   int viscous_order;         /* 0 or 2 */
   int reaction;              /* 1: finite-rate chemistry source */
   int positive_preserving;   /* positive_preserving: limiter on the species fluxes */
   int grad_p_diffusion;      /* gradPInDiffusionFlux */
   int fixed_time_step;
+  int reconstruction;        /* schemes 3/4: 1 first order, 2 MUSCL, 3 NND2, minmod limiter
+                                (src/Reconstruction.cu:5-46, src/Limiter.cuh:7-41) */
   double dt, cfl;
   double prandtl;            /* prandtl_number (viscous spectral radius of the time step) */
   double rho_ref, v_ref;     /* reference state: weno_eps_scale = (rho_ref v_ref)^2 */

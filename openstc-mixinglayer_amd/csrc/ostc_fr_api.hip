@@ -91,6 +91,7 @@ FrArgs args(ostc_fr_ctx c, int b) {
   a.reaction = c->s.reaction != 0;
   a.pp = c->s.positive_preserving != 0;
   a.grad_p = c->s.grad_p_diffusion != 0;
+  a.recon = c->s.reconstruction;
   a.cfl = c->s.cfl;
   a.prandtl = c->s.prandtl;
   a.weno_eps_scale = c->weno_eps_scale;
@@ -212,7 +213,10 @@ int ostc_fr_supported(int n_spec) { return supported_ns(n_spec) ? 1 : 0; }
 int ostc_fr_create(const ostc_fr_setup* s, const ostc_mech_tables* mech, void* stream, ostc_fr_ctx* out) {
   if (!s || !mech || !out) return OSTC_ERR_ARG;
   *out = nullptr;
-  if (s->inviscid_scheme != 51 && s->inviscid_scheme != 52) return OSTC_ERR_UNSUPPORTED;
+  if (s->inviscid_scheme != 51 && s->inviscid_scheme != 52 && s->inviscid_scheme != 3 && s->inviscid_scheme != 4)
+    return OSTC_ERR_UNSUPPORTED;
+  if ((s->inviscid_scheme == 3 || s->inviscid_scheme == 4) && (s->reconstruction < 1 || s->reconstruction > 3))
+    return OSTC_ERR_UNSUPPORTED;
   if (s->viscous_order != 0 && s->viscous_order != 2) return OSTC_ERR_UNSUPPORTED;
   if (!supported_ns(mech->n_spec)) return OSTC_ERR_UNSUPPORTED;
   if (s->cfl <= 0 && !s->fixed_time_step) return OSTC_ERR_ARG;

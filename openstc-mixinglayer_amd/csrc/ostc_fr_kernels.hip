@@ -15,6 +15,8 @@
 // overwrite the same LDS, and each cell subtracts its face difference from dq.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ostc_fr.hpp"
 
 namespace ostc {
@@ -624,6 +626,95 @@ __global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
   }
 }
 
+// One inviscid sweep along DIR with reconstructed primitives and an
+// approximate Riemann solver: compute_convective_term_pv_1D<FR>,
+// src/InviscidScheme.cu:50-165 (AUSM+ for scheme 3, HLLC for 4).  The tile
+// holds rho, u, v, w, p, Y_k of TS + 4 points per column in LDS.
+template <int DIR, int NS>
+__global__ __launch_bounds__(kT) void k_inviscid_pv(FrArgs A) {
+  constexpr int NV = 5 + NS;
+  constexpr int TW = Tile<DIR>::TW, F = Tile<DIR>::F, TS = F - 1, P = TS + 4, NP = P * TW;
+  extern __shared__ double lds[];
+  const FrBlockDev& B = A.B;
+  const int ext[3] = {B.mx, B.my, B.mz};
+  const int m = ext[DIR];
+  const int across = DIR == 0 ? 1 : 0;
+  const int col0 = blockIdx.y * TW;
+  const int third = blockIdx.z;
+  const int t0 = blockIdx.x * TS;
+  const long long s = B.sz;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NP; e += kT) {  // points t0-2 .. t0+TS+1
+    int col, sp;
+    if (DIR == 0) {
+      col = e / P;
+      sp = e % P;
+    } else {
+      col = e % TW;
+      sp = e / TW;
+    }
+    const int t = t0 - 2 + sp;
+    int idx[3];
+    tile_idx<DIR>(col0, t, col, third, idx);
+    const int slot = col * P + sp;
+    if (idx[across] < ext[across] && t < m + 2) {
+      const long long p = pid(B, idx[0], idx[1], idx[2]);
+#pragma unroll
+      for (int l = 0; l < 5; ++l) lds[l * NP + slot] = B.bv[p + l * s];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) lds[(5 + l) * NP + slot] = B.sv[p + l * s];
+    } else {
+#pragma unroll
+      for (int l = 0; l < NV; ++l) lds[l * NP + slot] = 1.0;
+    }
+  }
+  __syncthreads();
+  const int f = tid % F, col = tid / F;
+  const int tl = t0 - 1 + f;
+  int idx[3];
+  tile_idx<DIR>(col0, tl, col, third, idx);
+  const bool active = idx[across] < ext[across] && tl < m;
+  double fc[NV];
+  if (active) {
+    const int sl = col * P + f + 1;  // slot of the left cell
+    double pl[NV + 2], pr[NV + 2];
+#pragma unroll
+    for (int l = 0; l < NV; ++l)
+      reconstruct1(A.recon, lds[l * NP + sl - 1], lds[l * NP + sl], lds[l * NP + sl + 1], lds[l * NP + sl + 2], &pl[l],
+                   &pr[l]);
+    pv_energy<NS>(A.T, pl);
+    pv_energy<NS>(A.T, pr);
+    const long long p0 = pid(B, idx[0], idx[1], idx[2]);
+    int idr[3] = {idx[0], idx[1], idx[2]};
+    idr[DIR] += 1;
+    const long long p1 = pid(B, idr[0], idr[1], idr[2]);
+    double ml[3], mr[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ml[c] = B.met[p0 + (3 * DIR + c) * s];
+      mr[c] = B.met[p1 + (3 * DIR + c) * s];
+    }
+    if (A.scheme == 3)
+      ausm_plus<NS>(pl, pr, ml, mr, B.jac[p0], B.jac[p1], fc);
+    else
+      hllc<NS>(A.T, pl, pr, ml, mr, B.jac[p0], B.jac[p1], fc);
+  }
+  __syncthreads();
+  double* fcs = lds;
+  if (active) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) fcs[l * (F * TW) + col * F + f] = fc[l];
+  }
+  __syncthreads();
+  if (f >= 1 && active) {
+    const long long q0 = pid0(B, idx[0], idx[1], idx[2]);
+    const long long n0 = B.n0;
+#pragma unroll
+    for (int l = 0; l < NV; ++l)
+      B.dq[q0 + l * n0] -= fcs[l * (F * TW) + col * F + f] - fcs[l * (F * TW) + col * F + f - 1];
+  }
+}
+
 // One viscous sweep along DIR: compute_{fv,gv,hv}_2nd_order<FR, Laminar>,
 // src/ViscousScheme.cuh:23-318, :320-616, :618-909; one thread per face,
 // face fluxes through LDS, dq += fv(right) - fv(left).
@@ -795,8 +886,11 @@ void launch_rhs_t(const FrArgs& A, hipStream_t st, long long* n, KernelTimer* tm
     rec(KT_INV_X + D, false);
     if (A.scheme == 52)
       hipLaunchKernelGGL((k_inviscid<D, NS, 52>), grid, dim3(kT), lds, st, A);
-    else
+    else if (A.scheme == 51)
       hipLaunchKernelGGL((k_inviscid<D, NS, 51>), grid, dim3(kT), lds, st, A);
+    else  // primitive reconstruction + Riemann solver (3, 4): TS + 4 points, or F faces of fluxes
+      hipLaunchKernelGGL((k_inviscid_pv<D, NS>), grid, dim3(kT),
+                         (size_t)(5 + NS) * std::max(TS + 4, F) * TW * sizeof(double), st, A);
     rec(KT_INV_X + D, true);
     ++*n;
   };

@@ -59,7 +59,7 @@ class FrSetup(C.Structure):
     """ostc_fr_setup (include/openstc/abi.h)."""
     _fields_ = [("inviscid_scheme", C.c_int), ("viscous_order", C.c_int), ("reaction", C.c_int),
                 ("positive_preserving", C.c_int), ("grad_p_diffusion", C.c_int), ("fixed_time_step", C.c_int),
-                ("dt", C.c_double), ("cfl", C.c_double), ("prandtl", C.c_double),
+                ("reconstruction", C.c_int), ("dt", C.c_double), ("cfl", C.c_double), ("prandtl", C.c_double),
                 ("rho_ref", C.c_double), ("v_ref", C.c_double)]
 
 
@@ -134,7 +134,8 @@ class FrSolver:
         self.nv = 5 + self.ns
         s = FrSetup(case["inviscid_scheme"], case["viscous_order"], case["reaction"],
                     int(case.get("positive_preserving", 0)), int(case.get("grad_p_diffusion", 0)),
-                    int(case.get("fixed_time_step", 0)), float(case.get("dt", 0.0)), case["cfl"], case["prandtl"],
+                    int(case.get("fixed_time_step", 0)), int(case.get("reconstruction", 2)),
+                    float(case.get("dt", 0.0)), case["cfl"], case["prandtl"],
                     case["rho_ref"], case["v_ref"])
         ctx = C.c_void_p()
         r = L.ostc_fr_create(C.byref(s), C.byref(self.T), stream, C.byref(ctx))

@@ -27,7 +27,7 @@ R_U = 8314.462618
 class _Setup(C.Structure):
     _fields_ = [("inviscid_scheme", C.c_int), ("viscous_order", C.c_int), ("reaction", C.c_int),
                 ("positive_preserving", C.c_int), ("grad_p_diffusion", C.c_int), ("fixed_time_step", C.c_int),
-                ("dt", C.c_double), ("cfl", C.c_double), ("prandtl", C.c_double),
+                ("reconstruction", C.c_int), ("dt", C.c_double), ("cfl", C.c_double), ("prandtl", C.c_double),
                 ("rho_ref", C.c_double), ("v_ref", C.c_double), ("n_threads", C.c_int)]
 
 
@@ -68,6 +68,8 @@ def lib():
         L.orf_temperature.restype = d
         L.orf_temperature.argtypes = [tp, d, dp, d, C.POINTER(C.c_int)]
         L.orf_char_matrices.argtypes = [tp, dp, dp, dp, dp, d, d, dp, dp]
+        L.orf_riemann.argtypes = [tp, i, dp, dp, dp, dp, d, d, dp]
+        L.orf_reconstruct.argtypes = [dp, i, i, dp, dp]
         _lib = L
     return _lib
 
@@ -122,6 +124,23 @@ def char_matrices(T, ql, qr, kl, kr, jl, jr):
     return L.reshape(nv, nv), R.reshape(nv, nv)
 
 
+def riemann(T, scheme, ql, qr, kl, kr, jl, jr):
+    """AUSM+ (3) or HLLC (4) flux between primitive states [rho,u,v,w,p,Y]."""
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (ql, qr, kl, kr)]
+    fc = np.zeros(5 + T.n_spec)
+    lib().orf_riemann(C.byref(T), scheme, *[_dp(a) for a in args], jl, jr, _dp(fc))
+    return fc
+
+
+def reconstruct(pv, recon):
+    """Left/right states of the face between points 1 and 2 of pv[4, nv]."""
+    pv = np.ascontiguousarray(pv, dtype=np.float64)
+    nv = pv.shape[1]
+    pl, pr = np.zeros(nv), np.zeros(nv)
+    lib().orf_reconstruct(_dp(pv), nv, recon, _dp(pl), _dp(pr))
+    return pl, pr
+
+
 def mixture_streams(T, case):
     """Stream states of a two-stream mixture (get_mixing_layer_info with
     species, src/MixingLayer.cu:35-110): [rho,u,v,w,p,T] and Y of the upper
@@ -156,7 +175,8 @@ class FrOracle:
         self.L = L = lib()
         s = _Setup(case["inviscid_scheme"], case["viscous_order"], case["reaction"],
                    int(case.get("positive_preserving", 0)), int(case.get("grad_p_diffusion", 0)),
-                   int(case.get("fixed_time_step", 0)), float(case.get("dt", 0.0)), case["cfl"],
+                   int(case.get("fixed_time_step", 0)), int(case.get("reconstruction", 2)),
+                   float(case.get("dt", 0.0)), case["cfl"],
                    case["prandtl"], case["rho_ref"], case["v_ref"], n_threads)
         self.ctx = L.orf_create(C.byref(s), C.byref(self.T))
         if not self.ctx:

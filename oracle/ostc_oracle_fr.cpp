@@ -665,10 +665,253 @@ void pp_limiter(int ns, double* fc, const double* f1, const double* ql, const do
   }
 }
 
+// ------------------------------------------- primitive reconstruction path
+// (inviscid_type 0: compute_convective_term_pv, src/InviscidScheme.cu:12-215)
+
+// gxl::sgn (src/gxl_lib/Math.hpp:13-15): -1 for negative, +1 otherwise (also 0)
+inline int sgn(double a) { return a < 0 ? -1 : 1; }
+// minmod limiter, src/Limiter.cuh:7-15
+inline double minmod(double a, double b) { return 0.5 * (sgn(a) + sgn(b)) * std::min(abs(a), abs(b)); }
+
+// first_order_reconstruct / MUSCL_reconstruct / NND2_reconstruct,
+// src/Reconstruction.cu:5-46; pv holds nv values per point, face between
+// points `is` and `is`+1.
+void reconstruct(const double* pv, int nv, int is, int recon, double* pl, double* pr) {
+  for (int l = 0; l < nv; ++l) {
+    const double a0 = pv[(is - 1) * nv + l], a1 = pv[is * nv + l], a2 = pv[(is + 1) * nv + l];
+    if (recon == 2 || recon == 3) {
+      const double a3 = pv[(is + 2) * nv + l];
+      const double delta_i{a1 - a0};
+      const double delta_i1{a2 - a1};
+      const double delta_i2{a3 - a2};
+      if (recon == 2) {
+        constexpr double kappa{1.0 / 3.0};
+        const double neg_l = minmod(delta_i, delta_i1), pos_l = minmod(delta_i1, delta_i);
+        const double neg_r = minmod(delta_i1, delta_i2), pos_r = minmod(delta_i2, delta_i1);
+        pl[l] = a1 + 0.25 * ((1 - kappa) * neg_l + (1 + kappa) * pos_l);
+        pr[l] = a2 - 0.25 * ((1 - kappa) * pos_r + (1 + kappa) * neg_r);
+      } else {
+        pl[l] = a1 + 0.5 * minmod(delta_i, delta_i1);
+        pr[l] = a2 - 0.5 * minmod(delta_i1, delta_i2);
+      }
+    } else {
+      pl[l] = a1;
+      pr[l] = a2;
+    }
+  }
+}
+
+// total energy and gamma of a reconstructed mixture state (the mixture part
+// of reconstruction<>, src/InviscidScheme.cu:180-209): q[nv] = E, q[nv+1] = gamma
+void pv_energy(const Tab& T, int nv, double* q) {
+  const int ns = T.n_spec;
+  double e = 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  double mw_inv{0.0};
+  for (int l = 0; l < ns; ++l) mw_inv += q[5 + l] / T.mw[l];
+  const double t = q[4] / (q[0] * kRu * mw_inv);
+  double h[MS], cpi[MS];
+  enthalpy_cp(T, t, h, cpi);
+  double cp{0}, cv{0};
+  for (int l = 0; l < ns; ++l) {
+    cp += cpi[l] * q[l + 5];
+    cv += q[l + 5] * (cpi[l] - kRu / T.mw[l]);
+    e += h[l] * q[l + 5];
+  }
+  q[nv] = q[0] * e - q[4];
+  q[nv + 1] = cp / cv;
+}
+
+// riemannSolver_ausmPlus (mixture), src/RiemannSolver.cuh:9-96
+void ausm_plus(int nv, const double* pl, const double* pr, const double* ml, const double* mr, double jl, double jr,
+               double* fci) {
+  const double k1 = 0.5 * (jl * ml[0] + jr * mr[0]);
+  const double k2 = 0.5 * (jl * ml[1] + jr * mr[1]);
+  const double k3 = 0.5 * (jl * ml[2] + jr * mr[2]);
+  const double grad_k_div_jac = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
+  const double ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) / grad_k_div_jac;
+  const double ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) / grad_k_div_jac;
+  const double p_l = pl[4], p_r = pr[4], rho_l = pl[0], rho_r = pr[0];
+  const double gam_l = pl[nv + 1], gam_r = pr[nv + 1];
+  const double c = 0.5 * (sqrt(gam_l * p_l / rho_l) + sqrt(gam_r * p_r / rho_r));
+  const double mach_l = ul / c, mach_r = ur / c;
+  double mlp, mrn, plp, prn;
+  constexpr double alpha{3 / 16.0};
+  if (abs(mach_l) > 1) {
+    mlp = 0.5 * (mach_l + abs(mach_l));
+    plp = mlp / mach_l;
+  } else {
+    const double a = (mach_l + 1) * (mach_l + 1) * 0.25;
+    const double b = (mach_l * mach_l - 1) * (mach_l * mach_l - 1);
+    mlp = a + 0.125 * b;
+    plp = a * (2 - mach_l) + alpha * mach_l * b;
+  }
+  if (abs(mach_r) > 1) {
+    mrn = 0.5 * (mach_r - abs(mach_r));
+    prn = mrn / mach_r;
+  } else {
+    const double a = (mach_r - 1) * (mach_r - 1) * 0.25;
+    const double b = (mach_r * mach_r - 1) * (mach_r * mach_r - 1);
+    mrn = -a - 0.125 * b;
+    prn = a * (2 + mach_r) - alpha * mach_r * b;
+  }
+  const double p_coeff = plp * p_l + prn * p_r;
+  const double m_half = mlp + mrn;
+  const double mach_pos = 0.5 * (m_half + abs(m_half));
+  const double mach_neg = 0.5 * (m_half - abs(m_half));
+  const double mass_flux_half = c * (rho_l * mach_pos + rho_r * mach_neg);
+  const double coeff = mass_flux_half * grad_k_div_jac;
+  const double* q = mass_flux_half >= 0 ? pl : pr;
+  fci[0] = coeff;
+  fci[1] = coeff * q[1] + p_coeff * k1;
+  fci[2] = coeff * q[2] + p_coeff * k2;
+  fci[3] = coeff * q[3] + p_coeff * k3;
+  fci[4] = coeff * (q[nv] + q[4]) / q[0];
+  for (int l = 5; l < nv; ++l) fci[l] = coeff * q[l];
+}
+
+// riemannSolver_hllc (mixture), src/RiemannSolver.cuh:98-230
+void hllc(const Tab& T, int nv, const double* pl, const double* pr, const double* ml, const double* mr, double jl,
+          double jr, double* fci) {
+  const int ns = T.n_spec;
+  const double rl_c{sqrt(pl[0]) / (sqrt(pl[0]) + sqrt(pr[0]))}, rr_c{sqrt(pr[0]) / (sqrt(pl[0]) + sqrt(pr[0]))};
+  const double u_tilde{pl[1] * rl_c + pr[1] * rr_c};
+  const double v_tilde{pl[2] * rl_c + pr[2] * rr_c};
+  const double w_tilde{pl[3] * rl_c + pr[3] * rr_c};
+  double svm[MS];
+  for (int l = 0; l < ns; ++l) svm[l] = rl_c * pl[l + 5] + rr_c * pr[l + 5];
+  double mw_inv{0};
+  for (int l = 0; l < ns; ++l) mw_inv += svm[l] / T.mw[l];
+  const double tl{pl[4] / pl[0]};
+  const double tr{pr[4] / pr[0]};
+  const double t{(rl_c * tl + rr_c * tr) / (kRu * mw_inv)};
+  double cp_i[MS], h_i[MS];
+  enthalpy_cp(T, t, h_i, cp_i);
+  double cv{0}, cp{0};
+  for (int l = 0; l < ns; ++l) {
+    cp += cp_i[l] * svm[l];
+    cv += svm[l] * (cp_i[l] - kRu / T.mw[l]);
+  }
+  const double gamma = cp / cv;
+  const double c_tilde = sqrt(gamma * kRu * mw_inv * t);
+  const double kx = 0.5 * (ml[0] + mr[0]);
+  const double ky = 0.5 * (ml[1] + mr[1]);
+  const double kz = 0.5 * (ml[2] + mr[2]);
+  const double gradK = sqrt(kx * kx + ky * ky + kz * kz);
+  const double U_tilde_bar{(kx * u_tilde + ky * v_tilde + kz * w_tilde) / gradK};
+  const double jac_ave{0.5 * (jl + jr)};
+  const double gamma_l = pl[nv + 1], gamma_r = pr[nv + 1];
+  const double Ul{kx * pl[1] + ky * pl[2] + kz * pl[3]};
+  const double cl{sqrt(gamma_l * pl[4] / pl[0])};
+  const double sl{std::min(Ul / gradK - cl, U_tilde_bar - c_tilde)};
+  if (sl >= 0) {
+    const double rhoUk{pl[0] * Ul};
+    fci[0] = jac_ave * rhoUk;
+    fci[1] = jac_ave * (rhoUk * pl[1] + pl[4] * kx);
+    fci[2] = jac_ave * (rhoUk * pl[2] + pl[4] * ky);
+    fci[3] = jac_ave * (rhoUk * pl[3] + pl[4] * kz);
+    fci[4] = jac_ave * ((pl[nv] + pl[4]) * Ul);
+    for (int l = 5; l < nv; ++l) fci[l] = jac_ave * rhoUk * pl[l];
+    return;
+  }
+  const double Ur{kx * pr[1] + ky * pr[2] + kz * pr[3]};
+  const double cr{sqrt(gamma_r * pr[4] / pr[0])};
+  const double sr{std::max(Ur / gradK + cr, U_tilde_bar + c_tilde)};
+  if (sr < 0) {
+    const double rhoUk{pr[0] * Ur};
+    fci[0] = jac_ave * rhoUk;
+    fci[1] = jac_ave * (rhoUk * pr[1] + pr[4] * kx);
+    fci[2] = jac_ave * (rhoUk * pr[2] + pr[4] * ky);
+    fci[3] = jac_ave * (rhoUk * pr[3] + pr[4] * kz);
+    fci[4] = jac_ave * ((pr[nv] + pr[4]) * Ur);
+    for (int l = 5; l < nv; ++l) fci[l] = jac_ave * rhoUk * pr[l];
+    return;
+  }
+  const double sm{((pr[0] * Ur * (sr - Ur / gradK) - pl[0] * Ul * (sl - Ul / gradK)) / gradK + pl[4] - pr[4]) /
+                  (pr[0] * (sr - Ur / gradK) - pl[0] * (sl - Ul / gradK))};
+  const double pm{pl[0] * (sl - Ul / gradK) * (sm - Ul / gradK) + pl[4]};
+  if (sm >= 0) {
+    const double pCoeff{1.0 / (sl - sm)};
+    const double QlCoeff{jac_ave * pCoeff * sm * (sl * gradK - Ul) * pl[0]};
+    fci[0] = QlCoeff;
+    const double dP{(sl * pm - sm * pl[4]) * pCoeff * jac_ave};
+    fci[1] = QlCoeff * pl[1] + dP * kx;
+    fci[2] = QlCoeff * pl[2] + dP * ky;
+    fci[3] = QlCoeff * pl[3] + dP * kz;
+    fci[4] = QlCoeff * pl[nv] / pl[0] + pCoeff * jac_ave * (sl * pm * sm * gradK - sm * pl[4] * Ul);
+    for (int l = 5; l < nv; ++l) fci[l] = QlCoeff * pl[l];
+  } else {
+    const double pCoeff{1.0 / (sr - sm)};
+    const double QrCoeff{jac_ave * pCoeff * sm * (sr * gradK - Ur) * pr[0]};
+    fci[0] = QrCoeff;
+    const double dP{(sr * pm - sm * pr[4]) * pCoeff * jac_ave};
+    fci[1] = QrCoeff * pr[1] + dP * kx;
+    fci[2] = QrCoeff * pr[2] + dP * ky;
+    fci[3] = QrCoeff * pr[3] + dP * kz;
+    fci[4] = QrCoeff * pr[nv] / pr[0] + pCoeff * jac_ave * (sr * pm * sm * gradK - sm * pr[4] * Ur);
+    for (int l = 5; l < nv; ++l) fci[l] = QrCoeff * pr[l];
+  }
+}
+
+// compute_convective_term_pv_1D<FR>, src/InviscidScheme.cu:50-165: per
+// direction x, y, z, reconstructed primitives -> Riemann flux,
+// dq(i) -= fc(i+1/2) - fc(i-1/2).
+void inviscid_pv(const Ctx& c, Block& b) {
+  const int nv = c.nv, ns = c.ns, g = b.g;
+  const long long s = b.sz, n0 = b.n0();
+  const int ext[3] = {b.mx, b.my, b.mz};
+  for (int dir = 0; dir < 3; ++dir) {
+    const int m = ext[dir];
+    const int oa = (dir == 0) ? 1 : 0, ob = (dir == 2) ? 1 : 2;
+    const int na = ext[oa], nb = ext[ob];
+#pragma omp parallel
+    {
+      const int np = m + 2 * g;
+      std::vector<double> PV(np * nv), K(np * 3), J(np), fc((m + 1) * nv);
+#pragma omp for collapse(2) schedule(static)
+      for (int bb = 0; bb < nb; ++bb)
+        for (int aa = 0; aa < na; ++aa) {
+          int idx[3];
+          idx[oa] = aa;
+          idx[ob] = bb;
+          for (int t = -g; t < m + g; ++t) {
+            idx[dir] = t;
+            const long long p = b.id(idx[0], idx[1], idx[2]);
+            const int ps = t + g;
+            for (int l = 0; l < 5; ++l) PV[ps * nv + l] = b.bv[p + l * s];
+            for (int l = 0; l < ns; ++l) PV[ps * nv + 5 + l] = b.sv[p + l * s];
+            for (int cc = 0; cc < 3; ++cc) K[ps * 3 + cc] = b.metric[p + (3 * dir + cc) * s];
+            J[ps] = b.jac[p];
+          }
+          for (int t = -1; t < m; ++t) {
+            const int is = t + g;
+            double pl[5 + MS + 2], pr[5 + MS + 2];
+            reconstruct(PV.data(), nv, is, c.s.reconstruction, pl, pr);
+            pv_energy(c.T, nv, pl);
+            pv_energy(c.T, nv, pr);
+            double* fci = &fc[(t + 1) * nv];
+            if (c.s.inviscid_scheme == 3)
+              ausm_plus(nv, pl, pr, &K[is * 3], &K[(is + 1) * 3], J[is], J[is + 1], fci);
+            else
+              hllc(c.T, nv, pl, pr, &K[is * 3], &K[(is + 1) * 3], J[is], J[is + 1], fci);
+          }
+          for (int t = 0; t < m; ++t) {
+            idx[dir] = t;
+            const long long p0 = b.id0(idx[0], idx[1], idx[2]);
+            for (int l = 0; l < nv; ++l) b.dq[p0 + l * n0] -= fc[(t + 1) * nv + l] - fc[t * nv + l];
+          }
+        }
+    }
+  }
+}
+
 // compute_convective_term_weno_{x,y,z}<FR> with compute_weno_flux_cp (51)
 // or compute_weno_flux_ch (52, "method 1"), src/WENO.cu:8-461, :722-1143,
 // :1455-1536.  dq(i) -= fc(i+1/2) - fc(i-1/2) for x, y, z in that order.
 void inviscid(const Ctx& c, Block& b, double dt) {
+  if (c.s.inviscid_scheme == 3 || c.s.inviscid_scheme == 4) {
+    inviscid_pv(c, b);
+    return;
+  }
   const int nv = c.nv, ns = c.ns, g = b.g, scheme = c.s.inviscid_scheme;
   const long long s = b.sz, n0 = b.n0();
   const int ext[3] = {b.mx, b.my, b.mz};
@@ -1096,7 +1339,8 @@ extern "C" {
 
 void* orf_create(const orf_setup* s, const ostc_mech_tables* t) {
   if (!s || !t) return nullptr;
-  if (s->inviscid_scheme != 51 && s->inviscid_scheme != 52) return nullptr;
+  if (s->inviscid_scheme != 51 && s->inviscid_scheme != 52 && s->inviscid_scheme != 3 && s->inviscid_scheme != 4)
+    return nullptr;
   Ctx* c = new Ctx();
   c->s = *s;
   c->T = *t;
@@ -1228,6 +1472,25 @@ double orf_temperature(const ostc_mech_tables* t, double e, const double* y, dou
   for (int l = 0; l < t->n_spec; ++l) mw += y[l] / t->mw[l];
   return temperature(*t, e, y, kRu * mw, t_start, iters);
 }
+int orf_riemann(const ostc_mech_tables* t, int scheme, const double* ql, const double* qr, const double* kl,
+                const double* kr, double jl, double jr, double* fc) {
+  const int nv = 5 + t->n_spec;
+  double pl[5 + MS + 2], pr[5 + MS + 2];
+  for (int l = 0; l < nv; ++l) {
+    pl[l] = ql[l];
+    pr[l] = qr[l];
+  }
+  pv_energy(*t, nv, pl);
+  pv_energy(*t, nv, pr);
+  if (scheme == 3) ausm_plus(nv, pl, pr, kl, kr, jl, jr, fc);
+  else hllc(*t, nv, pl, pr, kl, kr, jl, jr, fc);
+  return 0;
+}
+
+void orf_reconstruct(const double* pv, int nv, int recon, double* pl, double* pr) {
+  reconstruct(pv, nv, 1, recon, pl, pr);
+}
+
 int orf_char_matrices(const ostc_mech_tables* t, const double* ql, const double* qr, const double* kl,
                       const double* kr, double jl, double jr, double* L, double* R) {
   const int ns = t->n_spec, nv = 5 + ns;

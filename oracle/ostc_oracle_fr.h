@@ -25,12 +25,14 @@ extern "C" {
 #endif
 
 typedef struct {
-  int    inviscid_scheme;      /* 51 WENO-Z5 component-wise, 52 WENO-Z5 characteristic */
+  int    inviscid_scheme;      /* 51 WENO-Z5 component-wise, 52 WENO-Z5 characteristic,
+                                  3 AUSM+, 4 HLLC on reconstructed primitives */
   int    viscous_order;        /* 0 or 2 */
   int    reaction;             /* 1: finite-rate chemistry source */
   int    positive_preserving;  /* 1: limiter on the species fluxes */
   int    grad_p_diffusion;     /* gradPInDiffusionFlux */
   int    fixed_time_step;
+  int    reconstruction;       /* schemes 3/4: 1 first order, 2 MUSCL, 3 NND2 (minmod limiter) */
   double dt, cfl;
   double prandtl;              /* prandtl_number (viscous spectral radius of the time step) */
   double rho_ref, v_ref;       /* weno_eps_scale = (rho_ref v_ref)^2 */
@@ -75,6 +77,12 @@ void    orf_transport(const ostc_mech_tables* t, double temp, double p, double r
                       double* lambda, double* rho_d);
 /* T-Newton: returns T, *iters receives the iteration count */
 double  orf_temperature(const ostc_mech_tables* t, double e, const double* y, double t_start, int* iters);
+/* Riemann flux (scheme 3 AUSM+, 4 HLLC) between primitive states q = [rho, u,
+ * v, w, p, Y_1..Y_ns], metric rows kl/kr and Jacobians jl/jr */
+int     orf_riemann(const ostc_mech_tables* t, int scheme, const double* ql, const double* qr, const double* kl,
+                    const double* kr, double jl, double jr, double* fc);
+/* reconstruction of the face between points 1 and 2 of pv[4][nv] */
+void    orf_reconstruct(const double* pv, int nv, int recon, double* pl, double* pr);
 /* left (5+ns)x(5+ns) characteristic matrix rows L and the back projection R
  * of compute_weno_flux_ch at a face, for the L R = I identity */
 int     orf_char_matrices(const ostc_mech_tables* t, const double* ql, const double* qr, const double* kl,

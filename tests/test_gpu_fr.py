@@ -215,3 +215,32 @@ def test_steps_reacting_mixture(tables):
     s.rk3(5, MX.MODE_GRAPH)
     _compare_state(o, s, 1, 1e-9, "reacting, 5 steps")
     s.close()
+
+
+@pytest.mark.parametrize("scheme", [3, 4], ids=["ausm+", "hllc"])
+@pytest.mark.parametrize("recon", [1, 2, 3], ids=["1st", "muscl", "nnd2"])
+def test_stage_residual_primitive_reconstruction(tables, scheme, recon):
+    print(f"[gpu] fr dq, primitive reconstruction {recon}, Riemann solver {scheme}", flush=True)
+    case = _case(tables, scheme=scheme)
+    case["reconstruction"] = recon
+    for label, state in (("layer", None), ("premixed", _mixed_state(case, tables, 31 + recon))):
+        o, s = _pair(case, tables) if state is None else _pair_state(case, tables, *state)
+        dq_o = o.rhs(0, 2e-9).copy()
+        s.compute_rhs(0, 2e-9)
+        err = rel_err(s.get(0, MX.FIELD_DQ), dq_o)
+        print(f"  {label} dq per-variable:", " ".join(f"{e:.1e}" for e in err), flush=True)
+        assert np.all(err[:5] <= 1e-12)
+        assert np.all(err[5:] <= 1e-10)
+        s.close()
+
+
+@pytest.mark.parametrize("scheme", [3, 4], ids=["ausm+", "hllc"])
+def test_steps_primitive_reconstruction(tables, scheme):
+    print(f"[gpu] fr 10 steps, MUSCL + Riemann solver {scheme}", flush=True)
+    case = _case(tables, scheme=scheme)
+    case["reconstruction"] = 2
+    o, s = _pair(case, tables)
+    o.step(10)
+    s.rk3(10, MX.MODE_GRAPH)
+    _compare_state(o, s, 1, 1e-9, "10 steps")
+    s.close()

@@ -359,3 +359,101 @@ def test_species_diffusion_fluxes_sum_to_zero(T):
     s = dq[5:].sum(axis=0)
     assert np.max(np.abs(s)) <= 1e-12 * np.max(np.abs(dq[5:]))
     assert np.all(dq[0] == 0)
+
+
+# ------------------------------------------ primitive reconstruction path
+def _prim(T, rng, u=None, rho=None, p=None, y=None):
+    y = _random_mixture(rng) if y is None else y
+    u = rng.uniform(-150, 150, 3) if u is None else np.asarray(u, dtype=float)
+    return np.concatenate([[rng.uniform(0.3, 1.5) if rho is None else rho], u,
+                           [rng.uniform(5e4, 2e5) if p is None else p], y])
+
+
+def _physical_flux(T, q, k, jac):
+    """J (rho U, rho u U + p kx, ..., (E + p) U, rho Y U), U = k . u."""
+    rho, u, p, y = q[0], q[1:4], q[4], q[5:]
+    mw_inv = np.sum(y / np.array(T.mw[:9]))
+    t = p / (rho * R_U * mw_inv)
+    h, _ = OF.enthalpy_cp(T, t)
+    E = rho * (0.5 * u @ u + h @ y) - p
+    U = k @ u
+    return jac * np.concatenate([[rho * U], rho * u * U + p * k, [(E + p) * U], rho * y * U])
+
+
+@pytest.mark.parametrize("scheme", [3, 4])
+def test_riemann_flux_is_consistent(T, scheme):
+    """F(q, q) is the physical flux (AUSM+: M+ + M- = M, P+ + P- = 1)."""
+    rng = np.random.default_rng(scheme)
+    for _ in range(10):
+        q = _prim(T, rng)
+        k = rng.normal(size=3)
+        f = OF.riemann(T, scheme, q, q, k, k, 0.8, 0.8)
+        ref = _physical_flux(T, q, k, 0.8)
+        assert np.max(np.abs(f - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("scheme", [3, 4])
+def test_supersonic_flow_takes_the_upwind_flux(T, scheme):
+    rng = np.random.default_rng(10 + scheme)
+    k = np.array([1.0, 0.0, 0.0])
+    ql = _prim(T, rng, u=[2500.0, 10.0, -5.0])
+    qr = _prim(T, rng, u=[2300.0, -20.0, 8.0])
+    f = OF.riemann(T, scheme, ql, qr, k, k, 1.0, 1.0)
+    ref = _physical_flux(T, ql, k, 1.0)
+    assert np.max(np.abs(f - ref)) <= 1e-12 * np.max(np.abs(ref))
+    f = OF.riemann(T, scheme, qr * np.array([1, -1, -1, -1] + [1] * 10), ql * np.array([1, -1, -1, -1] + [1] * 10),
+                   k, k, 1.0, 1.0)
+    ref = _physical_flux(T, ql * np.array([1, -1, -1, -1] + [1] * 10), k, 1.0)
+    assert np.max(np.abs(f - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_hllc_resolves_an_isolated_contact(T):
+    """Same u and p, different rho and Y: the HLLC flux is the upwind
+    physical flux (the contact wave is captured exactly)."""
+    rng = np.random.default_rng(12)
+    k = np.array([0.6, 0.8, 0.0])
+    for un in (40.0, -40.0):
+        u = un * k
+        ql = _prim(T, rng, u=u, p=1e5)
+        qr = _prim(T, rng, u=u, p=1e5)
+        f = OF.riemann(T, 4, ql, qr, k, k, 1.0, 1.0)
+        ref = _physical_flux(T, ql if un > 0 else qr, k, 1.0)
+        assert np.max(np.abs(f - ref)) <= 1e-11 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("recon", [2, 3])
+def test_limited_reconstruction_is_exact_on_linear_data(recon):
+    pv = np.outer(np.arange(4.0), np.linspace(-2, 3, 14)) + np.linspace(1, 5, 14)
+    pl, pr = OF.reconstruct(pv, recon)
+    mid = 0.5 * (pv[1] + pv[2])
+    assert np.max(np.abs(pl - mid)) <= 1e-14 * np.max(np.abs(mid))
+    assert np.max(np.abs(pr - mid)) <= 1e-14 * np.max(np.abs(mid))
+
+
+@pytest.mark.parametrize("recon", [1, 2, 3])
+def test_reconstruction_is_first_order_at_extrema(recon):
+    pv = np.array([[1.0], [3.0], [2.0], [0.5]])  # extremum at point 1 and a monotone drop to point 3
+    pl, pr = OF.reconstruct(pv, recon)
+    assert pl[0] == 3.0  # minmod of opposite-signed slopes is 0
+    if recon == 1:
+        assert pr[0] == 2.0
+    else:
+        assert 0.5 <= pr[0] <= 3.0
+
+
+@pytest.mark.parametrize("scheme", [3, 4])
+@pytest.mark.parametrize("recon", [1, 2, 3])
+def test_pv_path_preserves_free_stream_and_conserves(T, scheme, recon):
+    case = _periodic_case(T, scheme=scheme)
+    case["reconstruction"] = recon
+    o = _oracle_with(T, case, np.random.default_rng(6), uniform=True)
+    dq = o.rhs(0, 1e-9)
+    cv = o.array(0, OF.CV)
+    for l in range(14):
+        assert np.max(np.abs(dq[l])) <= 2e-12 * np.max(np.abs(cv[l])), l
+    o = _oracle_with(T, case, np.random.default_rng(7))
+    o.L.orf_apply_bc(o.ctx, 0)
+    o.L.orf_properties(o.ctx, 0)
+    dq = o.rhs(0, 1e-9)[:, :-1, :-1, :-1]
+    for l in range(14):
+        assert abs(dq[l].sum()) <= 1e-12 * np.abs(dq[l]).sum(), l
