@@ -23,7 +23,7 @@ class _Setup(C.Structure):
     _fields_ = [("n_var", C.c_int), ("inviscid_scheme", C.c_int), ("viscous_order", C.c_int),
                 ("fixed_time_step", C.c_int), ("dt", C.c_double), ("cfl", C.c_double),
                 ("prandtl", C.c_double), ("rho_ref", C.c_double), ("v_ref", C.c_double),
-                ("n_threads", C.c_int)]
+                ("n_threads", C.c_int), ("reconstruction", C.c_int)]
 
 
 class _Streams(C.Structure):
@@ -87,6 +87,8 @@ def lib(tangent: bool = False):
             getattr(L, name).restype = d
             getattr(L, name).argtypes = [dp, dp, d]
         L.orc_lf_split.argtypes = [dp, i, dp, d, dp, dp]
+        L.orc_riemann.argtypes = [i, dp, dp, dp, dp, d, d, dp]
+        L.orc_char_matrices.argtypes = [dp, dp, dp, dp, d, d, dp, dp]
         L.orc_fourier_perturbation.argtypes = [vp, i, C.POINTER(_Fourier)]
         L.orc_momentum_thickness.restype = d
         L.orc_momentum_thickness.argtypes = [vp, i, d, d, d, d, d, i, i, dp]
@@ -107,7 +109,7 @@ class Oracle:
         self.L = L = lib(tangent)
         s = _Setup(case["n_var"], case["inviscid_scheme"], case["viscous_order"],
                    int(case.get("fixed_time_step", 0)), float(case.get("dt", 0.0)), case["cfl"],
-                   case["prandtl"], case["rho_ref"], case["v_ref"], n_threads)
+                   case["prandtl"], case["rho_ref"], case["v_ref"], n_threads, int(case.get("reconstruction", 2)))
         self.case = case
         self.ctx = L.orc_create(C.byref(s))
         self.ngg = L.orc_ngg(self.ctx)
@@ -234,6 +236,23 @@ def weno5js(vp, vm, eps):
     vp = np.ascontiguousarray(vp, dtype=np.float64)
     vm = np.ascontiguousarray(vm, dtype=np.float64)
     return lib().orc_weno5js(_dp(vp), _dp(vm), float(eps))
+
+
+def riemann(scheme, ql, qr, kl, kr, jl, jr):
+    """LF (1) / AUSM+ (3) / HLLC (4) flux of air between primitive states [rho,u,v,w,p]."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (ql, qr, kl, kr)]
+    fc = np.zeros(5)
+    lib().orc_riemann(scheme, *[_dp(x) for x in a], float(jl), float(jr), _dp(fc))
+    return fc
+
+
+def char_matrices(ql, qr, kl, kr, jl, jr):
+    """Left and right eigenvector matrices of the air characteristic WENO at a
+    face between states [cv(5), p]."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (ql, qr, kl, kr)]
+    L, R = np.zeros(25), np.zeros(25)
+    lib().orc_char_matrices(*[_dp(x) for x in a], float(jl), float(jr), _dp(L), _dp(R))
+    return L.reshape(5, 5), R.reshape(5, 5)
 
 
 def lf_split(q, k3, jac):

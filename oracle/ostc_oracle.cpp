@@ -332,10 +332,311 @@ void local_dt(const Ctx& c, Block& b) {
       }
 }
 
+
+// ------------------------------------------------ characteristic WENO (air)
+// compute_weno_flux_ch<Air>, "method 1", src/WENO.cu:1147-1453: Roe-averaged
+// left eigenvectors at the face (LR filled as L, then as R), the split fluxes
+// of the five points projected, WENO-Z5 per field, projected back.
+struct CharAir {
+  real L[5][5], R[5][5];
+};
+
+void char_air(const real* cvl, const real* cvr, const double* m1, const double* m2, double jac1, double jac2,
+              CharAir& M) {
+  const real rlc{sqrt(cvl[0]) / (sqrt(cvl[0]) + sqrt(cvr[0]))};
+  const real rrc{sqrt(cvr[0]) / (sqrt(cvl[0]) + sqrt(cvr[0]))};
+  const real um{rlc * cvl[1] / cvl[0] + rrc * cvr[1] / cvr[0]};
+  const real vm{rlc * cvl[2] / cvl[0] + rrc * cvr[2] / cvr[0]};
+  const real wm{rlc * cvl[3] / cvl[0] + rrc * cvr[3] / cvr[0]};
+  const real ekm{0.5 * (um * um + vm * vm + wm * wm)};
+  constexpr double gm1{kGammaAir - 1};
+  const real hl{(cvl[4] + cvl[5]) / cvl[0]};
+  const real hr{(cvr[4] + cvr[5]) / cvr[0]};
+  const real hm{rlc * hl + rrc * hr};
+  const real cm2{gm1 * (hm - ekm)};
+  const real cm{sqrt(cm2)};
+  double kx{(jac1 * m1[0] + jac2 * m2[0]) / (jac1 + jac2)};
+  double ky{(jac1 * m1[1] + jac2 * m2[1]) / (jac1 + jac2)};
+  double kz{(jac1 * m1[2] + jac2 * m2[2]) / (jac1 + jac2)};
+  const double gradK{std::sqrt(kx * kx + ky * ky + kz * kz)};
+  kx /= gradK;
+  ky /= gradK;
+  kz /= gradK;
+  const real Uk_bar{kx * um + ky * vm + kz * wm};
+  const real alpha{gm1 * ekm};
+  auto& L = M.L;
+  L[0][0] = (alpha + Uk_bar * cm) / cm2 * 0.5;
+  L[0][1] = -(gm1 * um + kx * cm) / cm2 * 0.5;
+  L[0][2] = -(gm1 * vm + ky * cm) / cm2 * 0.5;
+  L[0][3] = -(gm1 * wm + kz * cm) / cm2 * 0.5;
+  L[0][4] = gm1 / cm2 * 0.5;
+  L[1][0] = kx * (1 - alpha / cm2) - (kz * vm - ky * wm) / cm;
+  L[1][1] = kx * gm1 * um / cm2;
+  L[1][2] = (kx * gm1 * vm + kz * cm) / cm2;
+  L[1][3] = (kx * gm1 * wm - ky * cm) / cm2;
+  L[1][4] = -kx * gm1 / cm2;
+  L[2][0] = ky * (1 - alpha / cm2) - (kx * wm - kz * um) / cm;
+  L[2][1] = (ky * gm1 * um - kz * cm) / cm2;
+  L[2][2] = ky * gm1 * vm / cm2;
+  L[2][3] = (ky * gm1 * wm + kx * cm) / cm2;
+  L[2][4] = -ky * gm1 / cm2;
+  L[3][0] = kz * (1 - alpha / cm2) - (ky * um - kx * vm) / cm;
+  L[3][1] = (kz * gm1 * um + ky * cm) / cm2;
+  L[3][2] = (kz * gm1 * vm - kx * cm) / cm2;
+  L[3][3] = kz * gm1 * wm / cm2;
+  L[3][4] = -kz * gm1 / cm2;
+  L[4][0] = (alpha - Uk_bar * cm) / cm2 * 0.5;
+  L[4][1] = -(gm1 * um - kx * cm) / cm2 * 0.5;
+  L[4][2] = -(gm1 * vm - ky * cm) / cm2 * 0.5;
+  L[4][3] = -(gm1 * wm - kz * cm) / cm2 * 0.5;
+  L[4][4] = gm1 / cm2 * 0.5;
+  auto& R = M.R;
+  R[0][0] = 1.0;
+  R[0][1] = kx;
+  R[0][2] = ky;
+  R[0][3] = kz;
+  R[0][4] = 1.0;
+  R[1][0] = um - kx * cm;
+  R[1][1] = kx * um;
+  R[1][2] = ky * um - kz * cm;
+  R[1][3] = kz * um + ky * cm;
+  R[1][4] = um + kx * cm;
+  R[2][0] = vm - ky * cm;
+  R[2][1] = kx * vm + kz * cm;
+  R[2][2] = ky * vm;
+  R[2][3] = kz * vm - kx * cm;
+  R[2][4] = vm + ky * cm;
+  R[3][0] = wm - kz * cm;
+  R[3][1] = kx * wm - ky * cm;
+  R[3][2] = ky * wm + kx * cm;
+  R[3][3] = kz * wm;
+  R[3][4] = wm + kz * cm;
+  R[4][0] = hm - Uk_bar * cm;
+  R[4][1] = kx * alpha / gm1 + (kz * vm - ky * wm) * cm;
+  R[4][2] = ky * alpha / gm1 + (kx * wm - kz * um) * cm;
+  R[4][3] = kz * alpha / gm1 + (ky * um - kx * vm) * cm;
+  R[4][4] = hm + Uk_bar * cm;
+}
+
+// ---------------------------------------- primitive reconstruction (air)
+// gxl::sgn (src/gxl_lib/Math.hpp:13-15): -1 for negative, +1 otherwise
+inline int sgn(const real& a) { return a < 0.0 ? -1 : 1; }
+// minmod, src/Limiter.cuh:7-15
+inline real minmod(const real& a, const real& b) {
+  return 0.5 * (sgn(a) + sgn(b)) * std::min(abs(a), abs(b));
+}
+
+// src/Reconstruction.cu:5-46, face between pencil points is and is+1;
+// pv holds 5 values per point; pl/pr get [rho,u,v,w,p,E] (E of
+// reconstruction<Air>, src/InviscidScheme.cu:210-215)
+void reconstruct_air(const real* pv, int is, int recon, real* pl, real* pr) {
+  for (int l = 0; l < 5; ++l) {
+    const real a0 = pv[(is - 1) * 5 + l], a1 = pv[is * 5 + l], a2 = pv[(is + 1) * 5 + l];
+    if (recon == 2 || recon == 3) {
+      const real a3 = pv[(is + 2) * 5 + l];
+      const real delta_i{a1 - a0}, delta_i1{a2 - a1}, delta_i2{a3 - a2};
+      if (recon == 2) {
+        constexpr double kappa{1.0 / 3.0};
+        const real neg_l = minmod(delta_i, delta_i1), pos_l = minmod(delta_i1, delta_i);
+        const real neg_r = minmod(delta_i1, delta_i2), pos_r = minmod(delta_i2, delta_i1);
+        pl[l] = a1 + 0.25 * ((1 - kappa) * neg_l + (1 + kappa) * pos_l);
+        pr[l] = a2 - 0.25 * ((1 - kappa) * pos_r + (1 + kappa) * neg_r);
+      } else {
+        pl[l] = a1 + 0.5 * minmod(delta_i, delta_i1);
+        pr[l] = a2 - 0.5 * minmod(delta_i1, delta_i2);
+      }
+    } else {
+      pl[l] = a1;
+      pr[l] = a2;
+    }
+  }
+  const real el = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+  const real er = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+  pl[5] = el * pl[0] + pl[4] / (kGammaAir - 1);
+  pr[5] = er * pr[0] + pr[4] / (kGammaAir - 1);
+}
+
+// riemannSolver_ausmPlus<Air>, src/RiemannSolver.cuh:9-96
+void ausm_air(const real* pl, const real* pr, const double* ml, const double* mr, double jl, double jr, real* fci) {
+  const double k1 = 0.5 * (jl * ml[0] + jr * mr[0]);
+  const double k2 = 0.5 * (jl * ml[1] + jr * mr[1]);
+  const double k3 = 0.5 * (jl * ml[2] + jr * mr[2]);
+  const double grad_k_div_jac = std::sqrt(k1 * k1 + k2 * k2 + k3 * k3);
+  const real ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) / grad_k_div_jac;
+  const real ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) / grad_k_div_jac;
+  const real p_l = pl[4], p_r = pr[4], rho_l = pl[0], rho_r = pr[0];
+  const real c = 0.5 * (sqrt(kGammaAir * p_l / rho_l) + sqrt(kGammaAir * p_r / rho_r));
+  const real mach_l = ul / c, mach_r = ur / c;
+  real mlp, mrn, plp, prn;
+  constexpr double alpha{3 / 16.0};
+  if (abs(mach_l) > 1.0) {
+    mlp = 0.5 * (mach_l + abs(mach_l));
+    plp = mlp / mach_l;
+  } else {
+    const real a = (mach_l + 1) * (mach_l + 1) * 0.25;
+    const real b = (mach_l * mach_l - 1) * (mach_l * mach_l - 1);
+    mlp = a + 0.125 * b;
+    plp = a * (2 - mach_l) + alpha * mach_l * b;
+  }
+  if (abs(mach_r) > 1.0) {
+    mrn = 0.5 * (mach_r - abs(mach_r));
+    prn = mrn / mach_r;
+  } else {
+    const real a = (mach_r - 1) * (mach_r - 1) * 0.25;
+    const real b = (mach_r * mach_r - 1) * (mach_r * mach_r - 1);
+    mrn = -a - 0.125 * b;
+    prn = a * (2 + mach_r) - alpha * mach_r * b;
+  }
+  const real p_coeff = plp * p_l + prn * p_r;
+  const real m_half = mlp + mrn;
+  const real mach_pos = 0.5 * (m_half + abs(m_half));
+  const real mach_neg = 0.5 * (m_half - abs(m_half));
+  const real mass_flux_half = c * (rho_l * mach_pos + rho_r * mach_neg);
+  const real coeff = mass_flux_half * grad_k_div_jac;
+  const real* q = mass_flux_half >= 0.0 ? pl : pr;
+  fci[0] = coeff;
+  fci[1] = coeff * q[1] + p_coeff * k1;
+  fci[2] = coeff * q[2] + p_coeff * k2;
+  fci[3] = coeff * q[3] + p_coeff * k3;
+  fci[4] = coeff * (q[5] + q[4]) / q[0];
+}
+
+// riemannSolver_hllc<Air>, src/RiemannSolver.cuh:98-230
+void hllc_air(const real* pl, const real* pr, const double* ml, const double* mr, double jl, double jr, real* fci) {
+  const real rl_c{sqrt(pl[0]) / (sqrt(pl[0]) + sqrt(pr[0]))}, rr_c{sqrt(pr[0]) / (sqrt(pl[0]) + sqrt(pr[0]))};
+  const real u_tilde{pl[1] * rl_c + pr[1] * rr_c};
+  const real v_tilde{pl[2] * rl_c + pr[2] * rr_c};
+  const real w_tilde{pl[3] * rl_c + pr[3] * rr_c};
+  const double gamma{kGammaAir};
+  const real hl{(pl[5] + pl[4]) / pl[0]};
+  const real hr{(pr[5] + pr[4]) / pr[0]};
+  const real h_tilde{hl * rl_c + hr * rr_c};
+  const real ek_tilde{0.5 * (u_tilde * u_tilde + v_tilde * v_tilde + w_tilde * w_tilde)};
+  const real c_tilde = sqrt((gamma - 1) * (h_tilde - ek_tilde));
+  const double kx = 0.5 * (ml[0] + mr[0]);
+  const double ky = 0.5 * (ml[1] + mr[1]);
+  const double kz = 0.5 * (ml[2] + mr[2]);
+  const double gradK = std::sqrt(kx * kx + ky * ky + kz * kz);
+  const real U_tilde_bar{(kx * u_tilde + ky * v_tilde + kz * w_tilde) / gradK};
+  const double jac_ave{0.5 * (jl + jr)};
+  const real Ul{kx * pl[1] + ky * pl[2] + kz * pl[3]};
+  const real cl{sqrt(kGammaAir * pl[4] / pl[0])};
+  const real sl{std::min(Ul / gradK - cl, U_tilde_bar - c_tilde)};
+  const real Ur{kx * pr[1] + ky * pr[2] + kz * pr[3]};
+  const real cr{sqrt(kGammaAir * pr[4] / pr[0])};
+  const real sr{std::max(Ur / gradK + cr, U_tilde_bar + c_tilde)};
+  if (sl >= 0.0 || sr < 0.0) {
+    const real* q = sl >= 0.0 ? pl : pr;
+    const real U = sl >= 0.0 ? Ul : Ur;
+    const real rhoUk{q[0] * U};
+    fci[0] = jac_ave * rhoUk;
+    fci[1] = jac_ave * (rhoUk * q[1] + q[4] * kx);
+    fci[2] = jac_ave * (rhoUk * q[2] + q[4] * ky);
+    fci[3] = jac_ave * (rhoUk * q[3] + q[4] * kz);
+    fci[4] = jac_ave * ((q[5] + q[4]) * U);
+    return;
+  }
+  const real sm{((pr[0] * Ur * (sr - Ur / gradK) - pl[0] * Ul * (sl - Ul / gradK)) / gradK + pl[4] - pr[4]) /
+                (pr[0] * (sr - Ur / gradK) - pl[0] * (sl - Ul / gradK))};
+  const real pm{pl[0] * (sl - Ul / gradK) * (sm - Ul / gradK) + pl[4]};
+  const bool left = sm >= 0.0;
+  const real* q = left ? pl : pr;
+  const real sw = left ? sl : sr;
+  const real U = left ? Ul : Ur;
+  const real pCoeff{1.0 / (sw - sm)};
+  const real QCoeff{jac_ave * pCoeff * sm * (sw * gradK - U) * q[0]};
+  fci[0] = QCoeff;
+  const real dP{(sw * pm - sm * q[4]) * pCoeff * jac_ave};
+  fci[1] = QCoeff * q[1] + dP * kx;
+  fci[2] = QCoeff * q[2] + dP * ky;
+  fci[3] = QCoeff * q[3] + dP * kz;
+  fci[4] = QCoeff * q[5] / q[0] + pCoeff * jac_ave * (sw * pm * sm * gradK - sm * q[4] * U);
+}
+
+// riemannSolver_laxFriedrich<Air>, src/RiemannSolver.cuh:426-461
+void lf_air(const real* pl, const real* pr, const double* ml, const double* mr, double jl, double jr, real* fci) {
+  const double kx = 0.5 * (ml[0] + mr[0]);
+  const double ky = 0.5 * (ml[1] + mr[1]);
+  const double kz = 0.5 * (ml[2] + mr[2]);
+  const double gradK = std::sqrt(kx * kx + ky * ky + kz * kz);
+  const real Ukl{pl[1] * kx + pl[2] * ky + pl[3] * kz};
+  const real Ukr{pr[1] * kx + pr[2] * ky + pr[3] * kz};
+  const real cl{sqrt(kGammaAir * pl[4] / pl[0])};
+  const real cr{sqrt(kGammaAir * pr[4] / pr[0])};
+  const real spectral_radius{std::max(abs(Ukl) + cl * gradK, abs(Ukr) + cr * gradK)};
+  const double half_jac_ave{0.5 * 0.5 * (jl + jr)};
+  const real rhoUl{pl[0] * Ukl};
+  const real rhoUr{pr[0] * Ukr};
+  fci[0] = (rhoUl + rhoUr - spectral_radius * (pr[0] - pl[0])) * half_jac_ave;
+  fci[1] = (rhoUl * pl[1] + rhoUr * pr[1] + kx * (pl[4] + pr[4]) - spectral_radius * (pr[1] * pr[0] - pl[1] * pl[0])) *
+           half_jac_ave;
+  fci[2] = (rhoUl * pl[2] + rhoUr * pr[2] + ky * (pl[4] + pr[4]) - spectral_radius * (pr[2] * pr[0] - pl[2] * pl[0])) *
+           half_jac_ave;
+  fci[3] = (rhoUl * pl[3] + rhoUr * pr[3] + kz * (pl[4] + pr[4]) - spectral_radius * (pr[3] * pr[0] - pl[3] * pl[0])) *
+           half_jac_ave;
+  fci[4] = ((pl[5] + pl[4]) * Ukl + (pr[5] + pr[4]) * Ukr - spectral_radius * (pr[5] - pl[5])) * half_jac_ave;
+}
+
+void riemann_air(int scheme, const real* pl, const real* pr, const double* ml, const double* mr, double jl, double jr,
+                 real* fci) {
+  if (scheme == 1) lf_air(pl, pr, ml, mr, jl, jr, fci);
+  else if (scheme == 3) ausm_air(pl, pr, ml, mr, jl, jr, fci);
+  else hllc_air(pl, pr, ml, mr, jl, jr, fci);
+}
+
+// compute_convective_term_pv_1D<Air>, src/InviscidScheme.cu:50-165
+void inviscid_pv(const Ctx& c, Block& b) {
+  const int g = b.g;
+  const long long s = b.sz, n0 = b.n0();
+  const int ext[3] = {b.mx, b.my, b.mz};
+  for (int dir = 0; dir < 3; ++dir) {
+    const int m = ext[dir];
+    const int oa = (dir == 0) ? 1 : 0, ob = (dir == 2) ? 1 : 2;
+    const int na = ext[oa], nb = ext[ob];
+#pragma omp parallel
+    {
+      const int np = m + 2 * g;
+      std::vector<real> PV(np * 5), fc((m + 1) * 5);
+      std::vector<double> K(np * 3), J(np);
+#pragma omp for collapse(2) schedule(static)
+      for (int bb = 0; bb < nb; ++bb)
+        for (int aa = 0; aa < na; ++aa) {
+          int idx[3];
+          idx[oa] = aa;
+          idx[ob] = bb;
+          for (int t = -g; t < m + g; ++t) {
+            idx[dir] = t;
+            const long long p = b.id(idx[0], idx[1], idx[2]);
+            const int ps = t + g;
+            for (int l = 0; l < 5; ++l) PV[ps * 5 + l] = b.bv[p + l * s];
+            for (int cc = 0; cc < 3; ++cc) K[ps * 3 + cc] = b.metric[p + (3 * dir + cc) * s];
+            J[ps] = b.jac[p];
+          }
+          for (int t = -1; t < m; ++t) {
+            const int is = t + g;
+            real pl[6], pr[6];
+            reconstruct_air(PV.data(), is, c.s.reconstruction, pl, pr);
+            riemann_air(c.s.inviscid_scheme, pl, pr, &K[is * 3], &K[(is + 1) * 3], J[is], J[is + 1],
+                        &fc[(t + 1) * 5]);
+          }
+          for (int t = 0; t < m; ++t) {
+            idx[dir] = t;
+            const long long p0 = b.id0(idx[0], idx[1], idx[2]);
+            for (int l = 0; l < 5; ++l) b.dq[p0 + l * n0] -= fc[(t + 1) * 5 + l] - fc[t * 5 + l];
+          }
+        }
+    }
+  }
+}
+
 // compute_convective_term_weno_{x,y,z} + compute_weno_flux_cp,
 // src/WENO.cu:8-461 (pencil kernels), :1455-1536 (cp flux), :463-491 (launcher).
 // dq(i) -= fc(i+1/2) - fc(i-1/2), directions x, y, z in that order.
 void inviscid(const Ctx& c, Block& b) {
+  if (c.s.inviscid_scheme == 1 || c.s.inviscid_scheme == 3 || c.s.inviscid_scheme == 4) {
+    inviscid_pv(c, b);
+    return;
+  }
   const int nv = c.s.n_var, g = b.g;
   const int scheme = c.s.inviscid_scheme;
   const long long s = b.sz;
@@ -380,6 +681,39 @@ void inviscid(const Ctx& c, Block& b) {
             const double jac1{b.jac[p1]}, jac2{b.jac[p2]};
             const double* m1 = &kj[is * 3];
             const double* m2 = &kj[(is + 1) * 3];
+            if (scheme == 52) {
+              // compute_weno_flux_ch<Air>, "method 1", src/WENO.cu:1147-1453
+              real cvl[6], cvr[6];
+              for (int l = 0; l < 5; ++l) {
+                cvl[l] = b.cv[p1 + l * s];
+                cvr[l] = b.cv[p2 + l * s];
+              }
+              cvl[5] = b.bv[p1 + 4 * s];
+              cvr[5] = b.bv[p2 + 4 * s];
+              CharAir M;
+              char_air(cvl, cvr, m1, m2, jac1, jac2, M);
+              constexpr double eps{1e-40};
+              const double eps_scaled = eps * c.weno_eps_scale * 0.25 *
+                                        ((m1[0] * jac1 + m2[0] * jac2) * (m1[0] * jac1 + m2[0] * jac2) +
+                                         (m1[1] * jac1 + m2[1] * jac2) * (m1[1] * jac1 + m2[1] * jac2) +
+                                         (m1[2] * jac1 + m2[2] * jac2) * (m1[2] * jac1 + m2[2] * jac2));
+              real fChar[5];
+              for (int l = 0; l < 5; ++l) {
+                real vPlus[5] = {}, vMinus[5] = {};
+                for (int r = 0; r < 5; ++r)
+                  for (int n = 0; n < 5; ++n) {
+                    vPlus[r] += M.L[l][n] * Fp[(is - 2 + r) * nv + n];
+                    vMinus[r] += M.L[l][n] * Fm[(is - 1 + r) * nv + n];
+                  }
+                fChar[l] = weno5z(vPlus, vMinus, eps_scaled);
+              }
+              for (int l = 0; l < 5; ++l) {
+                real f{0};
+                for (int q = 0; q < 5; ++q) f += M.R[l][q] * fChar[q];
+                fc[(t + 1) * nv + l] = f;
+              }
+              continue;
+            }
             constexpr double eps{1e-20};
             const double eps_ref = eps * c.weno_eps_scale * 0.25 *
                                    ((m1[0] * jac1 + m2[0] * jac2) * (m1[0] * jac1 + m2[0] * jac2) +
@@ -1094,6 +1428,39 @@ double orc_weno5js(const double* vp, const double* vm, double eps) {
   std::copy(vm, vm + 5, b);
   return value_of(weno5js(a, b, eps));
 }
+int orc_riemann(int scheme, const double* ql, const double* qr, const double* kl, const double* kr, double jl,
+                double jr, double* fc) {
+  real pl[6], pr[6], f[5];
+  real pv[4 * 5];
+  for (int l = 0; l < 5; ++l) {
+    pv[5 + l] = ql[l];
+    pv[10 + l] = qr[l];
+    pv[l] = ql[l];
+    pv[15 + l] = qr[l];
+  }
+  reconstruct_air(pv, 1, 1, pl, pr);
+  riemann_air(scheme, pl, pr, kl, kr, jl, jr, f);
+  for (int l = 0; l < 5; ++l) fc[l] = value_of(f[l]);
+  return 0;
+}
+
+int orc_char_matrices(const double* ql, const double* qr, const double* kl, const double* kr, double jl, double jr,
+                      double* L, double* R) {
+  real a[6], b[6];
+  for (int l = 0; l < 6; ++l) {
+    a[l] = ql[l];
+    b[l] = qr[l];
+  }
+  CharAir M;
+  char_air(a, b, kl, kr, jl, jr, M);
+  for (int r = 0; r < 5; ++r)
+    for (int q = 0; q < 5; ++q) {
+      L[r * 5 + q] = value_of(M.L[r][q]);
+      R[r * 5 + q] = value_of(M.R[r][q]);
+    }
+  return 0;
+}
+
 void orc_lf_split(const double* q, int n_var, const double* k3, double jac, double* fp, double* fm) {
   std::vector<real> qq(q, q + n_var + 2), p(n_var), m(n_var);
   lf_split(qq.data(), n_var, k3, jac, p.data(), m.data());

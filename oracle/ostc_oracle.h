@@ -26,7 +26,8 @@ extern "C" {
 
 typedef struct {
   int    n_var;            /* 5 for air */
-  int    inviscid_scheme;  /* 51 WENO-Z5 cp, 71 WENO-Z7 cp, 50 WENO5-JS cp (not in the reference) */
+  int    inviscid_scheme;  /* 51 WENO-Z5 cp, 52 WENO-Z5 characteristic, 71 WENO-Z7 cp, 50 WENO5-JS cp
+                              (not in the reference); 1 LF, 3 AUSM+, 4 HLLC on reconstructed primitives */
   int    viscous_order;    /* 0 inviscid, 2 second-order central */
   int    fixed_time_step;  /* 1: use dt below; 0: CFL-based global dt */
   double dt;
@@ -35,6 +36,7 @@ typedef struct {
   double rho_ref;          /* rho_inf of the reference state */
   double v_ref;            /* v_inf of the reference state */
   int    n_threads;        /* OpenMP threads, <=0: library default */
+  int    reconstruction;   /* schemes 1/3/4: 1 first order, 2 MUSCL, 3 NND2 (minmod) */
 } orc_setup;
 
 /* Mixing-layer stream states, as produced by get_mixing_layer_info for air
@@ -123,6 +125,13 @@ int     orc_residual(void* ctx, double res[4]);
 double  orc_weno5z(const double* vp, const double* vm, double eps);
 double  orc_weno7z(const double* vp, const double* vm, double eps);
 double  orc_weno5js(const double* vp, const double* vm, double eps);
+/* Riemann flux between primitive states [rho,u,v,w,p] (scheme 1 LF, 3 AUSM+,
+ * 4 HLLC), metric rows kl/kr, Jacobians jl/jr; and the characteristic WENO-Z5
+ * left/right matrices of a face between conservative states q = [cv(5), p]. */
+int     orc_riemann(int scheme, const double* ql, const double* qr, const double* kl, const double* kr, double jl,
+                    double jr, double* fc);
+int     orc_char_matrices(const double* ql, const double* qr, const double* kl, const double* kr, double jl,
+                          double jr, double* L, double* R);
 void    orc_lf_split(const double* q /* n_var cv, p, c */, int n_var, const double* k3, double jac,
                      double* fp, double* fm);
 
