@@ -66,7 +66,9 @@ typedef struct ostc_ctx_s* ostc_ctx;
  * Sutherland viscosity) is the mixture model on this path. */
 typedef struct {
   int n_var;            /* 5 */
-  int inviscid_scheme;  /* 51 WENO-Z5 cp, 71 WENO-Z7 cp, 50 WENO5-JS cp (no reference counterpart) */
+  int inviscid_scheme;  /* 51 WENO-Z5 cp, 71 WENO-Z7 cp, 50 WENO5-JS cp (no reference counterpart);
+                           OSTC_MODE_PLAIN only: 52 WENO-Z5 characteristic (src/WENO.cu:1147-1453),
+                           1 LF, 3 AUSM+, 4 HLLC on reconstructed primitives (src/InviscidScheme.cu:12-215) */
   int viscous_order;    /* 0 inviscid, 2 second-order central */
   int fixed_time_step;  /* 1: use dt below, 0: CFL-based global dt */
   double dt;
@@ -74,6 +76,8 @@ typedef struct {
   double prandtl;       /* prandtl_number */
   double rho_ref;       /* rho_inf of the reference state (src/BoundCond.cpp:300-321) */
   double v_ref;         /* v_inf of the reference state */
+  int reconstruction;   /* schemes 1/3/4: 1 first order, 2 MUSCL, 3 NND2, minmod limiter
+                           (src/Reconstruction.cu:5-46, src/Limiter.cuh:7-41) */
 } ostc_setup;
 
 typedef struct {

@@ -46,7 +46,8 @@ class OstcError(RuntimeError):
 class _Setup(C.Structure):
     _fields_ = [("n_var", C.c_int), ("inviscid_scheme", C.c_int), ("viscous_order", C.c_int),
                 ("fixed_time_step", C.c_int), ("dt", C.c_double), ("cfl", C.c_double),
-                ("prandtl", C.c_double), ("rho_ref", C.c_double), ("v_ref", C.c_double)]
+                ("prandtl", C.c_double), ("rho_ref", C.c_double), ("v_ref", C.c_double),
+                ("reconstruction", C.c_int)]
 
 
 class _BlockDesc(C.Structure):
@@ -178,7 +179,7 @@ class Solver:
         self.case = case
         s = _Setup(case["n_var"], case["inviscid_scheme"], case["viscous_order"],
                    int(case.get("fixed_time_step", 0)), float(case.get("dt", 0.0)), case["cfl"],
-                   case["prandtl"], case["rho_ref"], case["v_ref"])
+                   case["prandtl"], case["rho_ref"], case["v_ref"], int(case.get("reconstruction", 2)))
         ctx = C.c_void_p()
         r = L.ostc_create(C.byref(s), stream, C.byref(ctx))
         if r != 0:

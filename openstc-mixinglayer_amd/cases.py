@@ -100,7 +100,7 @@ def _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc
 def mixing_layer_case(x_cells, y_cells, z_cells, *, re_delta=500.0, ma_c=0.4,
                       lx_over_delta=20.0, ly_over_delta=40.0, lz_over_delta=20.0,
                       scheme=51, viscous_order=2, cfl=0.5, prandtl=0.72,
-                      seed=42, perturbation=1e-3, y_bc=BC_OUTFLOW):
+                      seed=42, perturbation=1e-3, y_bc=BC_OUTFLOW, reconstruction=2):
     """Temporal mixing layer of air on a tensor-product partition.
 
     ``x_cells`` etc. are lists of per-block cell counts along each axis
@@ -108,7 +108,8 @@ def mixing_layer_case(x_cells, y_cells, z_cells, *, re_delta=500.0, ma_c=0.4,
     node).  x and z are periodic with period = total cells (node N-1 is
     node 0); y uses ``y_bc`` on both ends.  Re_delta = rho*dU*delta/mu
     fixes delta_omega; dU = u_upper - u_lower = 2*Mc*c for equal
-    temperatures.
+    temperatures.  ``reconstruction`` (1 first order, 2 MUSCL, 3 NND2)
+    applies to the primitive-variable schemes 1/3/4.
     """
     st = air_streams(ma_c=ma_c, ma_upper=ma_c)
     rho, t = st["upper"][0], st["upper"][5]
@@ -122,7 +123,7 @@ def mixing_layer_case(x_cells, y_cells, z_cells, *, re_delta=500.0, ma_c=0.4,
     # face order inside a block: as registered above (x-, x+, y-, y+, z-, z+)
     return dict(
         n_var=5, inviscid_scheme=scheme, viscous_order=viscous_order, cfl=cfl, prandtl=prandtl,
-        fixed_time_step=0, dt=0.0,
+        fixed_time_step=0, dt=0.0, reconstruction=reconstruction,
         rho_ref=st["upper"][0], v_ref=abs(st["upper"][1]) if abs(st["upper"][1]) >= 1 else st["c_upper"],
         streams=st, delta_omega=delta, dx=dx, dy=dy, dz=dz,
         periods=(nx, 0, nz), seed=seed, amplitude=perturbation * du,

@@ -392,6 +392,16 @@ int upload_blocks(ostc_ctx c) {
   return OSTC_OK;
 }
 
+// The fused batched kernel and the reverse mode implement the component-wise
+// WENO schemes 50/51/71; the characteristic WENO (52) and the primitive
+// reconstruction + Riemann schemes (1/3/4) run on the per-block kernels.
+bool fused_scheme(const ostc_ctx c) { return c->P.scheme == 51 || c->P.scheme == 50 || c->P.scheme == 71; }
+
+int need_fused(ostc_ctx c) {
+  if (fused_scheme(c)) return OSTC_OK;
+  return fail(c, OSTC_ERR_UNSUPPORTED, "inviscid_scheme 52/1/3/4 run with OSTC_MODE_PLAIN and the stage operators only");
+}
+
 int set_tiles(ostc_ctx c, int index, int n_wg) {
   const TileShape t = fused_shape(c->P.scheme, index);
   const int lds = fused_lds_bytes(c->P.scheme, index);
@@ -583,7 +593,9 @@ int ostc_create(const ostc_setup* s, void* hip_stream, ostc_ctx* out) {
   if (!s || !out) return OSTC_ERR_ARG;
   *out = nullptr;
   if (s->n_var != 5) return OSTC_ERR_UNSUPPORTED;
-  if (s->inviscid_scheme != 51 && s->inviscid_scheme != 50 && s->inviscid_scheme != 71) return OSTC_ERR_UNSUPPORTED;
+  const int sc = s->inviscid_scheme;
+  if (sc != 51 && sc != 50 && sc != 71 && sc != 52 && sc != 1 && sc != 3 && sc != 4) return OSTC_ERR_UNSUPPORTED;
+  if ((sc == 1 || sc == 3 || sc == 4) && (s->reconstruction < 1 || s->reconstruction > 3)) return OSTC_ERR_UNSUPPORTED;
   if (s->viscous_order != 0 && s->viscous_order != 2) return OSTC_ERR_UNSUPPORTED;
   if (!(s->cfl > 0) || !(s->prandtl > 0)) return OSTC_ERR_ARG;
   auto* c = new ostc_ctx_s;
@@ -592,6 +604,7 @@ int ostc_create(const ostc_setup* s, void* hip_stream, ostc_ctx* out) {
   c->P.n_var = 5;
   c->P.scheme = s->inviscid_scheme;
   c->P.visc = s->viscous_order == 2;
+  c->P.recon = s->reconstruction;
   c->P.cfl = s->cfl;
   c->P.prandtl = s->prandtl;
   c->P.weno_eps_scale = s->rho_ref * s->v_ref * s->rho_ref * s->v_ref;  // src/DParameter.cu:26-29
@@ -845,6 +858,7 @@ int ostc_step(ostc_ctx c, int n_steps, int mode, double* last_dt) {
       if (r) return r;
     }
   } else if (mode == OSTC_MODE_BATCHED) {
+    if ((r = need_fused(c))) return r;
     if (!c->graph_valid) {
       r = build_graph(c);
       if (r) return r;
@@ -854,6 +868,7 @@ int ostc_step(ostc_ctx c, int n_steps, int mode, double* last_dt) {
     if (last_dt && !c->s.fixed_time_step)
       HIPCHK(c, hipMemcpyAsync(&dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->st));
   } else if (mode == OSTC_MODE_BATCHED_TIMED) {
+    if ((r = need_fused(c))) return r;
     while ((long long)c->evpool.size() < 6ll * n_steps) {
       hipEvent_t e;
       HIPCHK(c, hipEventCreate(&e));
@@ -922,6 +937,7 @@ int ostc_stage_rhs(ostc_ctx c, int b) {
 int ostc_batched_rhs(ostc_ctx c, int b) {
   int r = check_block(c, b);
   if (r) return r;
+  if ((r = need_fused(c))) return r;
   FusedArgs a = fused_args(c, 0, 1, 0);
   a.target_block = b;
   if (fused_launch(c->tile_index, 1, a, c->n_wg, c->st) < 0)
@@ -1036,6 +1052,7 @@ int ostc_tile_shape(int index, int* tx, int* ty, int* tz) {
 int ostc_set_tile(ostc_ctx c, int index, int waves_per_cu, int split) {
   int r = check_ready(c);
   if (r) return r;
+  if ((r = need_fused(c))) return r;
   if (index < 0 || index >= fused_shape_count(c->P.scheme)) return fail(c, OSTC_ERR_ARG, "tile index out of range");
   // waves_per_cu <= 0: one workgroup per tile; otherwise a persistent grid of
   // waves_per_cu workgroups per compute unit, with XCD-contiguous chunks.
@@ -1057,6 +1074,7 @@ int ostc_tile_config(ostc_ctx c, int* index, int* waves_per_cu, int* split) {
 int ostc_autotune(ostc_ctx c, int reps, int* best_index, double* best_ms) {
   int r = check_ready(c);
   if (r) return r;
+  if ((r = need_fused(c))) return r;
   if (reps < 1) reps = 3;
   double best = 1e30;
   int bi = c->tile_index, bw = 0, bs = 0;

@@ -49,11 +49,49 @@ __global__ __launch_bounds__(64) void k_weno_pencil(DevBlock B, Params P, int sl
     ijk[DIR] = L + 1;
     const long long p2 = pidx(B, ijk[0], ijk[1], ijk[2]);
     const double j1 = B.jac[p1], j2 = B.jac[p2];
+    const int sL = tid + R - 1;  // LDS index of cell L
+    if constexpr (SCHEME == 52) {
+      // compute_weno_flux_ch<Air>, "method 1" (src/WENO.cu:1147-1453): the
+      // five split fluxes of each side projected on the left eigenvectors of
+      // the Roe-averaged face state, WENO-Z5 per characteristic field, and the
+      // face flux projected back with the right eigenvectors.
+      double cvl[6], cvr[6];
+      for (int l = 0; l < 5; ++l) {
+        cvl[l] = cv[p1 + l * s];
+        cvr[l] = cv[p2 + l * s];
+      }
+      cvl[5] = bv[p1 + 4 * s];
+      cvr[5] = bv[p2 + 4 * s];
+      const double m1[3] = {B.met[p1 + 3 * DIR * s], B.met[p1 + (3 * DIR + 1) * s], B.met[p1 + (3 * DIR + 2) * s]};
+      const double m2[3] = {B.met[p2 + 3 * DIR * s], B.met[p2 + (3 * DIR + 1) * s], B.met[p2 + (3 * DIR + 2) * s]};
+      CharAir M;
+      char_air(cvl, cvr, m1, m2, j1, j2, M);
+      constexpr double eps{1e-40};
+      const double ax = m1[0] * j1 + m2[0] * j2, ay = m1[1] * j1 + m2[1] * j2, az = m1[2] * j1 + m2[2] * j2;
+      const double e = eps * P.weno_eps_scale * 0.25 * (ax * ax + ay * ay + az * az);
+      double fChar[5];
+      for (int l = 0; l < 5; ++l) {
+        double vp[5], vm[5];
+        for (int r = 0; r < 5; ++r) {
+          vp[r] = M.L[l][0] * sFp[0][sL - 2 + r];
+          vm[r] = M.L[l][0] * sFm[0][sL - 1 + r];
+          for (int n = 1; n < 5; ++n) {
+            vp[r] += M.L[l][n] * sFp[n][sL - 2 + r];
+            vm[r] += M.L[l][n] * sFm[n][sL - 1 + r];
+          }
+        }
+        fChar[l] = weno5z(vp, vm, e);
+      }
+      for (int l = 0; l < 5; ++l) {
+        double f = M.R[l][0] * fChar[0];
+        for (int q = 1; q < 5; ++q) f += M.R[l][q] * fChar[q];
+        sFc[l][tid] = f;
+      }
+    } else {
     double eps3[3];
     weno_eps(P, B.met[p1 + 3 * DIR * s] * j1, B.met[p1 + (3 * DIR + 1) * s] * j1, B.met[p1 + (3 * DIR + 2) * s] * j1,
              B.met[p2 + 3 * DIR * s] * j2, B.met[p2 + (3 * DIR + 1) * s] * j2, B.met[p2 + (3 * DIR + 2) * s] * j2,
              eps3);
-    const int sL = tid + R - 1;  // LDS index of cell L
     for (int l = 0; l < 5; ++l) {
       double vp[W], vm[W];
       for (int r = 0; r < W; ++r) {
@@ -63,6 +101,60 @@ __global__ __launch_bounds__(64) void k_weno_pencil(DevBlock B, Params P, int sl
       const double e = l == 0 ? eps3[0] : (l == 4 ? eps3[2] : eps3[1]);
       sFc[l][tid] = weno_face<SCHEME>(vp, vm, e);
     }
+    }
+  }
+  __syncthreads();
+  if (tid > 0 && L <= m - 1) {
+    int ijk[3] = {base[0], base[1], base[2]};
+    ijk[DIR] = L;
+    const long long p0 = pidx0(B, ijk[0], ijk[1], ijk[2]);
+    for (int l = 0; l < 5; ++l) B.dq[p0 + l * B.n0] -= sFc[l][tid] - sFc[l][tid - 1];
+  }
+}
+
+// compute_convective_term_pv_1D<Air> (src/InviscidScheme.cu:50-165): the
+// primitive variables of 67 pencil points staged in LDS, MUSCL / NND2 /
+// first-order reconstruction (src/Reconstruction.cu:5-46) and an LF, AUSM+ or
+// HLLC flux (src/RiemannSolver.cuh) at 64 faces, 63 cells updated.
+template <int DIR>
+__global__ __launch_bounds__(64) void k_pv_pencil(DevBlock B, Params P, int slot) {
+  constexpr int NP = 67;  // points c0-2 .. c0+64
+  __shared__ double sPv[5][NP], sK[3][NP], sJ[NP], sFc[5][64];
+  const int m = DIR == 0 ? B.mx : (DIR == 1 ? B.my : B.mz);
+  const int c0 = 63 * blockIdx.x;
+  const int tid = threadIdx.x;
+  int base[3];
+  if (DIR == 0) { base[1] = blockIdx.y; base[2] = blockIdx.z; }
+  if (DIR == 1) { base[0] = blockIdx.y; base[2] = blockIdx.z; }
+  if (DIR == 2) { base[0] = blockIdx.y; base[1] = blockIdx.z; }
+  const double* bv = B.bv[slot];
+  const long long s = B.sz;
+  for (int pt = tid; pt < NP; pt += 64) {
+    const int pos = c0 - 2 + pt;
+    if (pos > m + 1) continue;  // the faces need points -2 .. m+1 (ngg >= 2)
+    int ijk[3] = {base[0], base[1], base[2]};
+    ijk[DIR] = pos;
+    const long long p = pidx(B, ijk[0], ijk[1], ijk[2]);
+    for (int l = 0; l < 5; ++l) sPv[l][pt] = bv[p + l * s];
+    for (int c = 0; c < 3; ++c) sK[c][pt] = B.met[p + (3 * DIR + c) * s];
+    sJ[pt] = B.jac[p];
+  }
+  __syncthreads();
+  const int L = c0 - 1 + tid;  // face L+1/2 between LDS points tid+1 and tid+2
+  if (L <= m - 1) {
+    double pl[6], pr[6];
+    for (int l = 0; l < 5; ++l)
+      reconstruct_var(P.recon, sPv[l][tid], sPv[l][tid + 1], sPv[l][tid + 2], sPv[l][tid + 3], &pl[l], &pr[l]);
+    // total energy of the reconstructed states, reconstruction<Air> (src/InviscidScheme.cu:210-215)
+    const double el = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+    const double er = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+    pl[5] = el * pl[0] + pl[4] / (kGamma - 1);
+    pr[5] = er * pr[0] + pr[4] / (kGamma - 1);
+    const double ml[3] = {sK[0][tid + 1], sK[1][tid + 1], sK[2][tid + 1]};
+    const double mr[3] = {sK[0][tid + 2], sK[1][tid + 2], sK[2][tid + 2]};
+    double f[5];
+    riemann_air(P.scheme, pl, pr, ml, mr, sJ[tid + 1], sJ[tid + 2], f);
+    for (int l = 0; l < 5; ++l) sFc[l][tid] = f[l];
   }
   __syncthreads();
   if (tid > 0 && L <= m - 1) {
@@ -330,7 +422,13 @@ static void launch_weno(const DevBlock& B, const Params& P, hipStream_t st, long
 void plain_rhs(const DevBlock& B, const Params& P, hipStream_t st, long long* n) {
   if (P.scheme == 71) launch_weno<71>(B, P, st, n);
   else if (P.scheme == 50) launch_weno<50>(B, P, st, n);
-  else launch_weno<51>(B, P, st, n);
+  else if (P.scheme == 52) launch_weno<52>(B, P, st, n);
+  else if (P.scheme == 1 || P.scheme == 3 || P.scheme == 4) {
+    hipLaunchKernelGGL(k_pv_pencil<0>, pencil_grid(B, 0), dim3(64), 0, st, B, P, 0);
+    hipLaunchKernelGGL(k_pv_pencil<1>, pencil_grid(B, 1), dim3(64), 0, st, B, P, 0);
+    hipLaunchKernelGGL(k_pv_pencil<2>, pencil_grid(B, 2), dim3(64), 0, st, B, P, 0);
+    *n += 3;
+  } else launch_weno<51>(B, P, st, n);
   if (P.visc) {
     hipLaunchKernelGGL(k_visc_pencil<0>, pencil_grid(B, 0), dim3(64), 0, st, B, P, 0);
     hipLaunchKernelGGL(k_visc_pencil<1>, pencil_grid(B, 1), dim3(64), 0, st, B, P, 0);
