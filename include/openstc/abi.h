@@ -105,6 +105,33 @@ int ostc_add_block(ostc_ctx ctx, const ostc_block_desc* d, int* block_id);
 /* Boundary face (Boundary::register_boundary, src/Mesh.cpp:14-32):
  * axis 0/1/2, side -1 (index 0) or +1 (index m-1), type OSTC_BC_*. */
 int ostc_add_boundary(ostc_ctx ctx, int block, int axis, int side, int type);
+
+/* Far-field state of one boundary label: the members of FarField after its
+ * constructor (src/BoundCond.h:129-149).  sv: mass fractions (mixture
+ * contexts; ignored for air). */
+typedef struct {
+  double density, u, v, w, pressure, temperature;
+  double acoustic_speed, entropy, specific_heat_ratio, mw;
+  double sv[OSTC_MAX_SPEC];
+} ostc_farfield;
+
+/* Inputs of the FarField constructor (src/BoundCond.cpp:586-700) with the
+ * reference's defaults for keys not given: mach, pressure, temperature,
+ * density, reynolds_number = -1, velocity = 0, (u, v, w) = (1, 0, 0) as the
+ * velocity direction.  Combinations: (mach, pressure, temperature),
+ * (density, velocity, pressure), (mach, temperature, reynolds_number). */
+typedef struct {
+  double mach, pressure, temperature, velocity, density, u, v, w, reynolds_number;
+} ostc_farfield_spec;
+
+/* FarField::FarField for air (n_spec = 0, laminar), src/BoundCond.cpp:586-700.
+ * Host-only; no context needed. */
+int ostc_farfield_state_air(const ostc_farfield_spec* in, ostc_farfield* out);
+
+/* Farfield face (label 4) with the state of its label: apply_farfield<Air>,
+ * src/BoundCond.cuh:755-1022 (characteristic state from the face point's
+ * normal Mach number).  Replaces ostc_add_boundary(..., OSTC_BC_FARFIELD). */
+int ostc_add_farfield(ostc_ctx ctx, int block, int axis, int side, const ostc_farfield* ff);
 /* 1-to-1 conforming inner face with aligned index directions
  * (InnerFace::register_boundary, src/Mesh.cpp:38-176).  Register each side
  * of an interface, as the reference's inner N.txt files do. */

@@ -50,6 +50,38 @@ class _Setup(C.Structure):
                 ("reconstruction", C.c_int)]
 
 
+class Farfield(C.Structure):
+    """ostc_farfield: FarField members after its constructor (src/BoundCond.h:129-149)."""
+    _fields_ = [("density", C.c_double), ("u", C.c_double), ("v", C.c_double), ("w", C.c_double),
+                ("pressure", C.c_double), ("temperature", C.c_double), ("acoustic_speed", C.c_double),
+                ("entropy", C.c_double), ("specific_heat_ratio", C.c_double), ("mw", C.c_double),
+                ("sv", C.c_double * 16)]
+
+
+class FarfieldSpec(C.Structure):
+    """ostc_farfield_spec: FarField constructor inputs (src/BoundCond.cpp:586-700)."""
+    _fields_ = [("mach", C.c_double), ("pressure", C.c_double), ("temperature", C.c_double),
+                ("velocity", C.c_double), ("density", C.c_double), ("u", C.c_double), ("v", C.c_double),
+                ("w", C.c_double), ("reynolds_number", C.c_double)]
+
+
+def farfield_spec(**kw) -> FarfieldSpec:
+    """Constructor inputs with the reference's defaults for keys not given."""
+    d = dict(mach=-1.0, pressure=-1.0, temperature=-1.0, velocity=0.0, density=-1.0, u=1.0, v=0.0, w=0.0,
+             reynolds_number=-1.0)
+    d.update(kw)
+    return FarfieldSpec(**{k: float(v) for k, v in d.items()})
+
+
+def farfield_state_air(**kw) -> Farfield:
+    """FarField::FarField for air from keyword inputs (mach, pressure, ...)."""
+    out = Farfield()
+    r = lib().ostc_farfield_state_air(C.byref(farfield_spec(**kw)), C.byref(out))
+    if r != 0:
+        raise OstcError(f"bad far-field specification ({r})")
+    return out
+
+
 class _BlockDesc(C.Structure):
     _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int)]
 
@@ -104,6 +136,8 @@ SIGNATURES = {
     "ostc_destroy": (_I, [_P]),
     "ostc_add_block": (_I, [_P, C.POINTER(_BlockDesc), C.POINTER(_I)]),
     "ostc_add_boundary": (_I, [_P, _I, _I, _I, _I]),
+    "ostc_add_farfield": (_I, [_P, _I, _I, _I, C.POINTER(Farfield)]),
+    "ostc_farfield_state_air": (_I, [C.POINTER(FarfieldSpec), C.POINTER(Farfield)]),
     "ostc_add_inner_face": (_I, [_P, _I, _I, _I, _I]),
     "ostc_finalize": (_I, [_P]),
     "ostc_ngg": (_I, [_P, C.POINTER(_I)]),
@@ -191,7 +225,11 @@ class Solver:
         for b in case["blocks"]:
             self._chk(L.ostc_add_block(self.ctx, C.byref(_BlockDesc(b["mx"], b["my"], b["mz"])), None))
         for f in case["boundaries"]:
-            self._chk(L.ostc_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"]))
+            if f["type"] == 4:
+                ff = farfield_state_air(**f["farfield"])
+                self._chk(L.ostc_add_farfield(self.ctx, f["block"], f["axis"], f["side"], C.byref(ff)))
+            else:
+                self._chk(L.ostc_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"]))
         for f in case["inner_faces"]:
             self._chk(L.ostc_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"]))
         self._chk(L.ostc_finalize(self.ctx))

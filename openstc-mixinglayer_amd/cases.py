@@ -97,6 +97,21 @@ def _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc
     return blocks, boundaries, inner
 
 
+def stream_farfield(state, c):
+    """FarField inputs (mach, pressure, temperature, velocity direction) of a
+    stream state [rho, u, v, w, p, T] moving along x with sound speed c."""
+    u = state[1]
+    return dict(mach=abs(u) / c, pressure=state[4], temperature=state[5], u=-1.0 if u < 0 else 1.0, v=0.0, w=0.0)
+
+
+def _attach_farfield(boundaries, upper, lower):
+    """Farfield faces (label 4) on y take the stream they face: y+ the upper
+    stream, y- the lower one."""
+    for f in boundaries:
+        if f["type"] == BC_FARFIELD:
+            f["farfield"] = dict(upper if f["side"] > 0 else lower)
+
+
 def mixing_layer_case(x_cells, y_cells, z_cells, *, re_delta=500.0, ma_c=0.4,
                       lx_over_delta=20.0, ly_over_delta=40.0, lz_over_delta=20.0,
                       scheme=51, viscous_order=2, cfl=0.5, prandtl=0.72,
@@ -109,7 +124,8 @@ def mixing_layer_case(x_cells, y_cells, z_cells, *, re_delta=500.0, ma_c=0.4,
     node 0); y uses ``y_bc`` on both ends.  Re_delta = rho*dU*delta/mu
     fixes delta_omega; dU = u_upper - u_lower = 2*Mc*c for equal
     temperatures.  ``reconstruction`` (1 first order, 2 MUSCL, 3 NND2)
-    applies to the primitive-variable schemes 1/3/4.
+    applies to the primitive-variable schemes 1/3/4.  With ``y_bc`` =
+    BC_FARFIELD the y faces carry the far-field state of the stream they face.
     """
     st = air_streams(ma_c=ma_c, ma_upper=ma_c)
     rho, t = st["upper"][0], st["upper"][5]
@@ -120,6 +136,8 @@ def mixing_layer_case(x_cells, y_cells, z_cells, *, re_delta=500.0, ma_c=0.4,
     dy = ly_over_delta * delta / ny
     dz = lz_over_delta * delta / nz
     blocks, boundaries, inner = _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc)
+    _attach_farfield(boundaries, stream_farfield(st["upper"], st["c_upper"]),
+                     stream_farfield(st["lower"], st["c_lower"]))
     # face order inside a block: as registered above (x-, x+, y-, y+, z-, z+)
     return dict(
         n_var=5, inviscid_scheme=scheme, viscous_order=viscous_order, cfl=cfl, prandtl=prandtl,

@@ -15,7 +15,7 @@ using namespace ostc;
 
 namespace {
 
-struct FaceDesc { int axis, side, type; };
+struct FaceDesc { int axis, side, type, ff; };  // ff: index of the far-field state (type 4)
 struct InnerDesc { int axis, side, target; };
 struct Box { int lo[3], hi[3]; };  // inclusive bounds
 
@@ -75,6 +75,8 @@ struct ostc_ctx_s {
   bool own_stream = false;
   int cus = 256;
   std::vector<HBlock> blocks;
+  std::vector<ostc_farfield> farfields;  // states of the farfield faces, in registration order
+  double* d_ff = nullptr;
   bool finalized = false;
   DevBlock* d_blocks = nullptr;
   std::vector<DevOp> ops;
@@ -184,16 +186,26 @@ int build_ops(ostc_ctx c) {
     for (const FaceDesc& fd : f) {
       const int a = fd.axis, m = B.m[a];
       const int idx = fd.side < 0 ? 0 : m - 1;
-      DevOp op = make_op(fd.type == OSTC_BC_PERIODIC ? OP_BC_PERIODIC : OP_BC_OUTFLOW, b,
-                         full_tangential(B, a, idx, idx, G));
+      const int ot = fd.type == OSTC_BC_PERIODIC ? OP_BC_PERIODIC
+                     : fd.type == OSTC_BC_FARFIELD ? OP_BC_FARFIELD : OP_BC_OUTFLOW;
+      DevOp op = make_op(ot, b, full_tangential(B, a, idx, idx, G));
       op.axis = a;
       op.side = fd.side;
+      if (fd.type == OSTC_BC_FARFIELD) op.ff = c->d_ff + (size_t)fd.ff * kFarfieldDoubles;
       bc_of_block[b].push_back(op);
       // ghost cells this op reads before the same stage can have written them
       Box rd = fd.type == OSTC_BC_PERIODIC
                    ? (fd.side < 0 ? full_tangential(B, a, m - 1 - G, m - 2, G) : full_tangential(B, a, 1, G, G))
                    : full_tangential(B, a, idx, idx, G);
       for (const Box& rb : minus_interior(rd, B.m)) rim_ops.push_back(make_op(OP_COPY_BV, b, rb));
+      if (fd.type == OSTC_BC_FARFIELD) {
+        // the farfield state replaces the face points after the fused update
+        // computed their properties: recompute them with the ghost layers
+        Box fp;
+        for (int d = 0; d < 3; ++d) { fp.lo[d] = 0; fp.hi[d] = B.m[d] - 1; }
+        fp.lo[a] = fp.hi[a] = idx;
+        props_ops.push_back(make_op(OP_PROPS, b, fp));
+      }
     }
     Box pad;
     for (int d = 0; d < 3; ++d) { pad.lo[d] = -G; pad.hi[d] = B.m[d] - 1 + G; }
@@ -660,9 +672,64 @@ int ostc_add_boundary(ostc_ctx c, int block, int axis, int side, int type) {
   if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_finalize");
   if (block < 0 || block >= (int)c->blocks.size() || axis < 0 || axis > 2 || (side != -1 && side != 1))
     return fail(c, OSTC_ERR_ARG, "bad boundary face");
+  if (type == OSTC_BC_FARFIELD)
+    return fail(c, OSTC_ERR_ARG, "farfield faces need their state: use ostc_add_farfield");
   if (type != OSTC_BC_OUTFLOW && type != OSTC_BC_PERIODIC)
-    return fail(c, OSTC_ERR_UNSUPPORTED, "boundary type not implemented (outflow 6 and periodic 8 are)");
-  c->blocks[block].faces.push_back({axis, side, type});
+    return fail(c, OSTC_ERR_UNSUPPORTED, "boundary type not implemented (farfield 4, outflow 6, periodic 8 are)");
+  c->blocks[block].faces.push_back({axis, side, type, -1});
+  return OSTC_OK;
+}
+
+int ostc_add_farfield(ostc_ctx c, int block, int axis, int side, const ostc_farfield* ff) {
+  if (!c) return OSTC_ERR_ARG;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_finalize");
+  if (block < 0 || block >= (int)c->blocks.size() || axis < 0 || axis > 2 || (side != -1 && side != 1) || !ff)
+    return fail(c, OSTC_ERR_ARG, "bad farfield face");
+  if (!(ff->density > 0) || !(ff->pressure > 0) || !(ff->acoustic_speed > 0) || !(ff->specific_heat_ratio > 1) ||
+      !(ff->entropy > 0))
+    return fail(c, OSTC_ERR_ARG, "far-field state: density, pressure, acoustic speed, entropy > 0 and gamma > 1");
+  c->blocks[block].faces.push_back({axis, side, OSTC_BC_FARFIELD, (int)c->farfields.size()});
+  c->farfields.push_back(*ff);
+  return OSTC_OK;
+}
+
+// FarField::FarField for air (n_spec = 0, laminar), src/BoundCond.cpp:586-700.
+int ostc_farfield_state_air(const ostc_farfield_spec* in, ostc_farfield* out) {
+  if (!in || !out) return OSTC_ERR_ARG;
+  ostc_farfield f{};
+  double mach = in->mach, pressure = in->pressure, temperature = in->temperature, velocity = in->velocity,
+         density = in->density;
+  double u = in->u, v = in->v, w = in->w;
+  const double mw = kMwAir;
+  if (temperature < 0) {
+    if (!(pressure > 0) || !(density > 0)) return OSTC_ERR_ARG;
+    temperature = pressure * mw / (density * kRu);
+  }
+  const double viscosity = 1.7894e-5 * std::pow(temperature / 288.16, 1.5) * (288.16 + 110) / (temperature + 110);
+  const double gamma = kGamma;
+  const double acoustic_speed = std::sqrt(gamma * kRu / mw * temperature);
+  if (mach < 0) mach = velocity / acoustic_speed;
+  else velocity = mach * acoustic_speed;
+  u *= velocity;
+  v *= velocity;
+  w *= velocity;
+  if (pressure < 0) {
+    if (!(in->reynolds_number > 0) || !(velocity > 0)) return OSTC_ERR_ARG;
+    density = viscosity * in->reynolds_number / velocity;
+    pressure = density * temperature * kRu / mw;
+  }
+  if (density < 0) density = pressure * mw / (kRu * temperature);
+  f.density = density;
+  f.u = u;
+  f.v = v;
+  f.w = w;
+  f.pressure = pressure;
+  f.temperature = temperature;
+  f.acoustic_speed = acoustic_speed;
+  f.entropy = pressure / std::pow(density, gamma);
+  f.specific_heat_ratio = gamma;
+  f.mw = mw;
+  *out = f;
   return OSTC_OK;
 }
 
@@ -713,6 +780,15 @@ int ostc_finalize(ostc_ctx c) {
     HIPCHK(c, hipMalloc(&v, c->blocks.size() * sizeof(DevBlock)));
     c->allocs.push_back(v);
     c->d_blocks = static_cast<DevBlock*>(v);
+  }
+  if (!c->farfields.empty()) {
+    static_assert(sizeof(ostc_farfield) == kFarfieldDoubles * sizeof(double), "far-field layout");
+    void* v = nullptr;
+    HIPCHK(c, hipMalloc(&v, c->farfields.size() * sizeof(ostc_farfield)));
+    c->allocs.push_back(v);
+    c->d_ff = static_cast<double*>(v);
+    HIPCHK(c, hipMemcpy(c->d_ff, c->farfields.data(), c->farfields.size() * sizeof(ostc_farfield),
+                        hipMemcpyHostToDevice));
   }
   int r = dalloc(c, &c->d_scratch, 1 + 4 * (size_t)c->n_first);
   if (!r) r = dalloc(c, &c->d_dt, 1);

@@ -14,7 +14,13 @@ enum OpType : int {
   OP_PROPS = 3,        // update_physical_properties<Air>, src/FieldOperation.cuh:107-140
   OP_CV_FROM_BV = 4,   // compute_cv_from_bv, src/FieldOperation.cuh:40-76
   OP_COPY_BV = 5,      // bv of slot `src` -> slot `slot` (ghost rims read by boundary ops)
+  OP_BC_FARFIELD = 6,  // apply_farfield<Air>, src/BoundCond.cuh:755-1022
 };
+
+// Doubles of one far-field state on the device, the layout of ostc_farfield:
+// density, u, v, w, pressure, temperature, acoustic_speed, entropy,
+// specific_heat_ratio, mw, sv[16].
+constexpr int kFarfieldDoubles = 26;
 
 struct DevOp {
   int type;
@@ -27,6 +33,7 @@ struct DevOp {
   int flags;           // OP_COMM bit 0: also recompute the target's cv on the shared layer
   long long begin;     // first point of this op in its launch range
   long long count;     // n[0]*n[1]*n[2]
+  const double* ff;    // OP_BC_FARFIELD: the state of the face's label (kFarfieldDoubles)
 };
 
 // One op point.  Every op reads and writes the ring slot `slot` of cv/bv/mul.
@@ -66,6 +73,69 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
                                  ? pidx(B, so[0] + g * d[0], so[1] + g * d[1], so[2] + g * d[2])
                                  : pidx(B, i, j, k);
         for (int l = 0; l < 6; ++l) bv[pg + l * s] = bv[ps + l * s];
+        cv_point(B, pg);
+      }
+      break;
+    }
+    case OP_BC_FARFIELD: {
+      // Each face point (and ngg beyond on the tangential directions) decides
+      // from its own normal Mach number: supersonic inflow takes the far field
+      // on the face point and its ghosts, supersonic outflow copies the face
+      // point to the ghosts, subsonic faces take the state from the outgoing
+      // Riemann invariant of the face point and the incoming one of the far
+      // field, with the far field's entropy and tangential velocity on
+      // inflow and the face point's on outflow.
+      const double* F = op.ff;
+      const long long p = pidx(B, i, j, k);
+      double nx = B.met[p + (3 * op.axis) * s], ny = B.met[p + (3 * op.axis + 1) * s],
+             nz = B.met[p + (3 * op.axis + 2) * s];
+      const double grad_n_inv = op.side / sqrt(nx * nx + ny * ny + nz * nz);
+      nx *= grad_n_inv;
+      ny *= grad_n_inv;
+      nz *= grad_n_inv;
+      const double u_b = bv[p + s], v_b = bv[p + 2 * s], w_b = bv[p + 3 * s];
+      const double u_face = nx * u_b + ny * v_b + nz * w_b;
+      const double p_b = bv[p + 4 * s], rho_b = bv[p];
+      const double a_b = sqrt(kGamma * p_b / rho_b);
+      const double mach_b = u_face / a_b;
+      double st[6];
+      bool face_too = true;
+      if (mach_b <= -1) {
+        for (int l = 0; l < 6; ++l) st[l] = F[l];
+      } else if (mach_b >= 1) {
+        for (int l = 0; l < 6; ++l) st[l] = bv[p + l * s];
+        face_too = false;
+      } else {
+        const double riemann_pos = u_face + 2 * a_b / (kGamma - 1);
+        const double u_inf = F[1] * nx + F[2] * ny + F[3] * nz;
+        const double riemann_neg = u_inf - 2 * F[6] / (F[8] - 1);
+        const double Un = 0.5 * (riemann_pos + riemann_neg);
+        const double c_b = 0.25 * (kGamma - 1) * (riemann_pos - riemann_neg);
+        double s_b;
+        if (mach_b <= 0) {
+          s_b = F[7];
+          st[1] = F[1] + (Un - u_inf) * nx;
+          st[2] = F[2] + (Un - u_inf) * ny;
+          st[3] = F[3] + (Un - u_inf) * nz;
+        } else {
+          s_b = p_b / pow(rho_b, kGamma);
+          st[1] = u_b + (Un - u_face) * nx;
+          st[2] = v_b + (Un - u_face) * ny;
+          st[3] = w_b + (Un - u_face) * nz;
+        }
+        st[0] = pow(c_b * c_b / (kGamma * s_b), 1 / (kGamma - 1));
+        st[4] = st[0] * c_b * c_b / kGamma;
+        st[5] = st[4] * kMwAir / (st[0] * kRu);
+      }
+      int d[3] = {0, 0, 0};
+      d[op.axis] = op.side;
+      if (face_too) {
+        for (int l = 0; l < 6; ++l) bv[p + l * s] = st[l];
+        cv_point(B, p);
+      }
+      for (int g = 1; g <= B.g; ++g) {
+        const long long pg = pidx(B, i + g * d[0], j + g * d[1], k + g * d[2]);
+        for (int l = 0; l < 6; ++l) bv[pg + l * s] = st[l];
         cv_point(B, pg);
       }
       break;

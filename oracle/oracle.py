@@ -30,6 +30,35 @@ class _Streams(C.Structure):
     _fields_ = [("upper", C.c_double * 6), ("lower", C.c_double * 6)]
 
 
+class Farfield(C.Structure):
+    """FarField members after its constructor (src/BoundCond.h:129-149)."""
+    _fields_ = [("density", C.c_double), ("u", C.c_double), ("v", C.c_double), ("w", C.c_double),
+                ("pressure", C.c_double), ("temperature", C.c_double), ("acoustic_speed", C.c_double),
+                ("entropy", C.c_double), ("specific_heat_ratio", C.c_double), ("mw", C.c_double),
+                ("sv", C.c_double * 16)]
+
+
+class FarfieldSpec(C.Structure):
+    """FarField constructor inputs; -1 = not given (velocity 0, direction (1, 0, 0))."""
+    _fields_ = [("mach", C.c_double), ("pressure", C.c_double), ("temperature", C.c_double),
+                ("velocity", C.c_double), ("density", C.c_double), ("u", C.c_double), ("v", C.c_double),
+                ("w", C.c_double), ("reynolds_number", C.c_double)]
+
+
+def farfield_spec(**kw) -> FarfieldSpec:
+    d = dict(mach=-1.0, pressure=-1.0, temperature=-1.0, velocity=0.0, density=-1.0, u=1.0, v=0.0, w=0.0,
+             reynolds_number=-1.0)
+    d.update(kw)
+    return FarfieldSpec(**{k: float(v) for k, v in d.items()})
+
+
+def farfield_state_air(**kw) -> Farfield:
+    out = Farfield()
+    if lib().orc_farfield_state_air(C.byref(farfield_spec(**kw)), C.byref(out)) != 0:
+        raise ValueError("bad far-field specification")
+    return out
+
+
 class _Fourier(C.Structure):
     _fields_ = [("n_modes", C.c_int), ("comp", C.c_void_p), ("kx", C.c_void_p), ("kz", C.c_void_p),
                 ("trig", C.c_void_p), ("amp", C.c_void_p), ("amp_dot", C.c_void_p),
@@ -64,6 +93,8 @@ def lib(tangent: bool = False):
         L.orc_ngg.argtypes = [vp]
         L.orc_add_boundary.argtypes = [vp, i, i, i, i]
         L.orc_add_inner_face.argtypes = [vp, i, i, i, i]
+        L.orc_add_farfield.argtypes = [vp, i, i, i, C.POINTER(Farfield)]
+        L.orc_farfield_state_air.argtypes = [C.POINTER(FarfieldSpec), C.POINTER(Farfield)]
         L.orc_array.restype = dp
         L.orc_array.argtypes = [vp, i, i, C.POINTER(C.c_longlong)]
         L.orc_uniform_grid.argtypes = [vp, i, d, d, d, d, d, d]
@@ -120,7 +151,11 @@ class Oracle:
                 raise ValueError("bad block")
             self.nblocks += 1
         for f in case["boundaries"]:
-            L.orc_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"])
+            if f["type"] == 4:
+                ff = farfield_state_air(**f["farfield"])
+                L.orc_add_farfield(self.ctx, f["block"], f["axis"], f["side"], C.byref(ff))
+            else:
+                L.orc_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"])
         for f in case["inner_faces"]:
             L.orc_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"])
 

@@ -62,6 +62,7 @@ constexpr double kRkC[3] = {1.0, 0.25, 2.0 / 3.0};
 
 struct Face {       // physical boundary face, src/Mesh.cpp:14-32
   int axis, side, type;
+  orc_farfield ff;  // type ORC_BC_FARFIELD: the state of its label
 };
 
 struct InnerFace {  // aligned 1-to-1 inner face, src/Mesh.cpp:38-176
@@ -924,12 +925,95 @@ void apply_face(Block& b, const Face& f) {
       }
 }
 
+// apply_farfield<Air>, src/BoundCond.cuh:755-1022.  Each point of the face
+// (ngg points beyond it on the tangential directions) decides from its own
+// normal Mach number: supersonic inflow takes the far-field state on the face
+// point and its ghosts, supersonic outflow copies the face point to the
+// ghosts, subsonic faces take the state from the two Riemann invariants
+// (outgoing from the face point, incoming from the far field) with the
+// far-field entropy and tangential velocity on inflow, the face point's on
+// outflow.
+void apply_farfield(Block& b, const Face& f) {
+  const int g = b.g;
+  const long long s = b.sz;
+  const int ext[3] = {b.mx, b.my, b.mz};
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = -g;
+    hi[a] = ext[a] - 1 + g;
+  }
+  lo[f.axis] = hi[f.axis] = (f.side < 0) ? 0 : ext[f.axis] - 1;
+  int dir[3] = {0, 0, 0};
+  dir[f.axis] = f.side;
+  const orc_farfield& ff = f.ff;
+  constexpr double gamma_air{kGammaAir};
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = lo[2]; k <= hi[2]; ++k)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int i = lo[0]; i <= hi[0]; ++i) {
+        const long long p = b.id(i, j, k);
+        double nx = b.metric[p + (3 * f.axis) * s], ny = b.metric[p + (3 * f.axis + 1) * s],
+               nz = b.metric[p + (3 * f.axis + 2) * s];
+        const double grad_n_inv = f.side / std::sqrt(nx * nx + ny * ny + nz * nz);
+        nx *= grad_n_inv;
+        ny *= grad_n_inv;
+        nz *= grad_n_inv;
+        const real u_b{b.bv[p + s]}, v_b{b.bv[p + 2 * s]}, w_b{b.bv[p + 3 * s]};
+        const real u_face{nx * u_b + ny * v_b + nz * w_b};
+        const real p_b{b.bv[p + 4 * s]}, rho_b{b.bv[p]};
+        const real a_b{sqrt(gamma_air * p_b / rho_b)};
+        const real mach_b{u_face / a_b};
+        real st[6];
+        bool face_too = true;
+        if (mach_b <= -1) {
+          st[0] = ff.density; st[1] = ff.u; st[2] = ff.v; st[3] = ff.w;
+          st[4] = ff.pressure; st[5] = ff.temperature;
+        } else if (mach_b >= 1) {
+          for (int l = 0; l < 6; ++l) st[l] = b.bv[p + l * s];
+          face_too = false;
+        } else {
+          const real riemann_pos{u_face + 2 * a_b / (gamma_air - 1)};
+          const double u_inf{ff.u * nx + ff.v * ny + ff.w * nz};
+          const double riemann_neg{u_inf - 2 * ff.acoustic_speed / (ff.specific_heat_ratio - 1)};
+          const real Un{0.5 * (riemann_pos + riemann_neg)};
+          const real c_b{0.25 * (gamma_air - 1) * (riemann_pos - riemann_neg)};
+          real s_b;
+          if (mach_b <= 0) {
+            s_b = ff.entropy;
+            st[1] = ff.u + (Un - u_inf) * nx;
+            st[2] = ff.v + (Un - u_inf) * ny;
+            st[3] = ff.w + (Un - u_inf) * nz;
+          } else {
+            s_b = p_b / pow(rho_b, gamma_air);
+            st[1] = u_b + (Un - u_face) * nx;
+            st[2] = v_b + (Un - u_face) * ny;
+            st[3] = w_b + (Un - u_face) * nz;
+          }
+          st[0] = pow(c_b * c_b / (gamma_air * s_b), 1 / (gamma_air - 1));
+          st[4] = st[0] * c_b * c_b / gamma_air;
+          st[5] = st[4] * kMwAir / (st[0] * kRu);
+        }
+        if (face_too) {
+          for (int l = 0; l < 6; ++l) b.bv[p + l * s] = st[l];
+          cv_from_bv_point(b, p);
+        }
+        for (int gg = 1; gg <= g; ++gg) {
+          const long long pg = b.id(i + gg * dir[0], j + gg * dir[1], k + gg * dir[2]);
+          for (int l = 0; l < 6; ++l) b.bv[pg + l * s] = st[l];
+          cv_from_bv_point(b, pg);
+        }
+      }
+}
+
 void apply_bc(Block& b) {
   // priority: farfield(4), outflow(6), ..., periodic(8) (src/BoundCond.cuh:1488-1678)
   const int order[] = {ORC_BC_FARFIELD, ORC_BC_OUTFLOW, ORC_BC_PERIODIC};
   for (int t : order)
     for (const Face& f : b.faces)
-      if (f.type == t) apply_face(b, f);
+      if (f.type == t) {
+        if (t == ORC_BC_FARFIELD) apply_farfield(b, f);
+        else apply_face(b, f);
+      }
 }
 
 // inner_communication, src/DataCommunication.cuh:60-105, for an aligned face.
@@ -1052,7 +1136,51 @@ int orc_add_block(void* ctx, int mx, int my, int mz) {
 int orc_add_boundary(void* ctx, int block, int axis, int side, int type) {
   auto* c = static_cast<Ctx*>(ctx);
   if (block < 0 || block >= (int)c->blocks.size()) return -1;
-  c->blocks[block].faces.push_back(Face{axis, side, type});
+  c->blocks[block].faces.push_back(Face{axis, side, type, orc_farfield{}});
+  return 0;
+}
+
+int orc_add_farfield(void* ctx, int block, int axis, int side, const orc_farfield* ff) {
+  auto* c = static_cast<Ctx*>(ctx);
+  if (block < 0 || block >= (int)c->blocks.size() || !ff) return -1;
+  Face f{axis, side, ORC_BC_FARFIELD, *ff};
+  c->blocks[block].faces.push_back(f);
+  return 0;
+}
+
+// FarField::FarField for air (n_spec = 0, laminar), src/BoundCond.cpp:586-700
+int orc_farfield_state_air(const orc_farfield_spec* in, orc_farfield* out) {
+  if (!in || !out) return -1;
+  orc_farfield f{};
+  double mach = in->mach, pressure = in->pressure, temperature = in->temperature, velocity = in->velocity,
+         density = in->density;
+  double u = in->u, v = in->v, w = in->w;
+  const double mw = kMwAir;
+  if (temperature < 0) temperature = pressure * mw / (density * kRu);
+  const double viscosity = value_of(sutherland(temperature));
+  const double gamma = kGammaAir;
+  const double acoustic_speed = std::sqrt(gamma * kRu / mw * temperature);
+  if (mach < 0) mach = velocity / acoustic_speed;
+  else velocity = mach * acoustic_speed;
+  u *= velocity;
+  v *= velocity;
+  w *= velocity;
+  if (pressure < 0) {
+    density = viscosity * in->reynolds_number / velocity;
+    pressure = density * temperature * kRu / mw;
+  }
+  if (density < 0) density = pressure * mw / (kRu * temperature);
+  f.density = density;
+  f.u = u;
+  f.v = v;
+  f.w = w;
+  f.pressure = pressure;
+  f.temperature = temperature;
+  f.acoustic_speed = acoustic_speed;
+  f.entropy = pressure / std::pow(density, gamma);
+  f.specific_heat_ratio = gamma;
+  f.mw = mw;
+  *out = f;
   return 0;
 }
 

@@ -51,6 +51,24 @@ enum { ORC_CV = 0, ORC_BV, ORC_QN, ORC_DQ, ORC_MUL, ORC_MACH, ORC_JAC, ORC_METRI
 
 enum { ORC_BC_FARFIELD = 4, ORC_BC_OUTFLOW = 6, ORC_BC_PERIODIC = 8 };
 
+/* Far-field state of one boundary label: the members of FarField after its
+ * constructor (src/BoundCond.h:129-149).  sv holds mass fractions on the
+ * mixture path. */
+typedef struct {
+  double density, u, v, w, pressure, temperature;
+  double acoustic_speed, entropy, specific_heat_ratio, mw;
+  double sv[16];
+} orc_farfield;
+
+/* Constructor inputs of FarField (src/BoundCond.cpp:586-700) with the
+ * reference's defaults for keys not given: mach, pressure, temperature,
+ * density, reynolds_number = -1, velocity = 0, (u, v, w) = (1, 0, 0), the
+ * velocity direction.  Combinations: (mach, pressure, temperature),
+ * (density, velocity, pressure), (mach, temperature, reynolds_number). */
+typedef struct {
+  double mach, pressure, temperature, velocity, density, u, v, w, reynolds_number;
+} orc_farfield_spec;
+
 void*   orc_create(const orc_setup* s);
 void    orc_destroy(void* ctx);
 int     orc_add_block(void* ctx, int mx, int my, int mz);
@@ -64,6 +82,10 @@ int     orc_add_boundary(void* ctx, int block, int axis, int side, int type);
  * side of an interface is registered separately, as in the reference's
  * inner N.txt files (src/Mesh.cpp:38-176). */
 int     orc_add_inner_face(void* ctx, int block, int axis, int side, int target_block);
+/* farfield face (label 4) with its state, apply_farfield<Air> (src/BoundCond.cuh:755-1022) */
+int     orc_add_farfield(void* ctx, int block, int axis, int side, const orc_farfield* ff);
+/* FarField constructor for air (n_spec = 0), src/BoundCond.cpp:586-700 */
+int     orc_farfield_state_air(const orc_farfield_spec* in, orc_farfield* out);
 double* orc_array(void* ctx, int block, int which, long long* n_out);
 
 /* setup */
