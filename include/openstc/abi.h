@@ -351,6 +351,14 @@ int ostc_fr_destroy(ostc_fr_ctx ctx);
 int ostc_fr_add_block(ostc_fr_ctx ctx, const ostc_block_desc* d, int* block_id);
 int ostc_fr_add_boundary(ostc_fr_ctx ctx, int block, int axis, int side, int type);  /* outflow, periodic */
 int ostc_fr_add_inner_face(ostc_fr_ctx ctx, int block, int axis, int side, int target_block);
+/* Farfield face (label 4) of a mixture, apply_farfield<FR> (src/BoundCond.cuh:755-1022): the subsonic
+ * state iterates gamma with cp of the boundary temperature and composition. */
+int ostc_fr_add_farfield(ostc_fr_ctx ctx, int block, int axis, int side, const ostc_farfield* ff);
+/* FarField::FarField with species (src/BoundCond.cpp:586-700): mass fractions y[n_spec]; the
+ * (mach, temperature, reynolds_number) combination uses the host mixture viscosity
+ * (compute_viscosity, src/Transport.cu:11-41).  Host-only. */
+int ostc_fr_farfield_state(const ostc_mech_tables* tables, const ostc_farfield_spec* in, const double* y,
+                           ostc_farfield* out);
 int ostc_fr_finalize(ostc_fr_ctx ctx);
 int ostc_fr_set_geometry(ostc_fr_ctx ctx, int block, const double* jac, const double* metric, long long elem_stride,
                          long long comp_stride, long long offset);

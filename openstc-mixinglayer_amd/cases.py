@@ -322,6 +322,11 @@ def reacting_case(x_cells, y_cells, z_cells, tables, *, mech_files=None, reactio
     dy = ly_over_delta * delta / ny
     dz = lz_over_delta * delta / nz
     blocks, boundaries, inner = _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc)
+    _attach_farfield(boundaries, stream_farfield(st["upper"], st["c_upper"]),
+                     stream_farfield(st["lower"], st["c_lower"]))
+    for f in boundaries:
+        if f["type"] == BC_FARFIELD:
+            f["farfield_y"] = [float(v) for v in (y_up if f["side"] > 0 else y_lo)]
     ns = tables.n_spec
     u1 = abs(st["upper"][1])
     return dict(

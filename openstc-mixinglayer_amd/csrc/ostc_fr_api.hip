@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -17,7 +18,7 @@ using namespace ostc::fr;
 
 namespace {
 
-struct FaceDesc { int axis, side, type; };
+struct FaceDesc { int axis, side, type, ff; };  // ff: index of the far-field state (type 4)
 struct InnerDesc { int axis, side, target; };
 
 struct HBlock {
@@ -41,6 +42,8 @@ struct ostc_fr_ctx_s {
   hipStream_t st = nullptr;
   bool own_stream = false;
   std::vector<HBlock> blocks;
+  std::vector<ostc_farfield> farfields;  // states of the farfield faces, in registration order
+  double* d_ff = nullptr;
   bool finalized = false;
   double* d_dt = nullptr;          // dt of the current step (one double)
   double weno_eps_scale = 1;
@@ -106,7 +109,8 @@ int check_ready(ostc_fr_ctx c) {
   return OSTC_OK;
 }
 
-int bc_priority(int type) { return type == OSTC_BC_OUTFLOW ? 0 : 1; }  // src/BoundCond.cuh:1488-1678
+// src/BoundCond.cuh:1488-1678: farfield, outflow, ..., periodic
+int bc_priority(int type) { return type == OSTC_BC_FARFIELD ? 0 : (type == OSTC_BC_OUTFLOW ? 1 : 2); }
 
 // Boundary conditions of one block (priority order, then registration
 // order), src/BoundCond.cuh:1488-1678.
@@ -115,7 +119,9 @@ void apply_bc(ostc_fr_ctx c, int b, long long* n) {
   std::stable_sort(fs.begin(), fs.end(),
                    [](const FaceDesc& x, const FaceDesc& y) { return bc_priority(x.type) < bc_priority(y.type); });
   const FrArgs a = args(c, b);
-  for (const FaceDesc& f : fs) launch_bc(a, c->ns, f.axis, f.side, f.type, c->st, n);
+  for (const FaceDesc& f : fs)
+    launch_bc(a, c->ns, f.axis, f.side, f.type, f.type == OSTC_BC_FARFIELD ? c->d_ff + (size_t)f.ff * 26 : nullptr,
+              c->st, n);
 }
 
 // data_communication, inner faces in block order then face order
@@ -275,9 +281,123 @@ int ostc_fr_add_boundary(ostc_fr_ctx c, int b, int axis, int side, int type) {
   if (!c) return OSTC_ERR_ARG;
   if (int r = check_block(c, b)) return r;
   if (axis < 0 || axis > 2 || (side != -1 && side != 1)) return fail(c, OSTC_ERR_ARG, "bad face");
+  if (type == OSTC_BC_FARFIELD)
+    return fail(c, OSTC_ERR_ARG, "farfield faces need their state: use ostc_fr_add_farfield");
   if (type != OSTC_BC_OUTFLOW && type != OSTC_BC_PERIODIC)
-    return fail(c, OSTC_ERR_UNSUPPORTED, "mixture path: outflow and periodic faces only");
-  c->blocks[b].faces.push_back({axis, side, type});
+    return fail(c, OSTC_ERR_UNSUPPORTED, "mixture path: farfield, outflow and periodic faces only");
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_fr_finalize");
+  c->blocks[b].faces.push_back({axis, side, type, -1});
+  return OSTC_OK;
+}
+
+int ostc_fr_add_farfield(ostc_fr_ctx c, int b, int axis, int side, const ostc_farfield* ff) {
+  if (!c) return OSTC_ERR_ARG;
+  if (int r = check_block(c, b)) return r;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_fr_finalize");
+  if (axis < 0 || axis > 2 || (side != -1 && side != 1) || !ff) return fail(c, OSTC_ERR_ARG, "bad face");
+  if (!(ff->density > 0) || !(ff->pressure > 0) || !(ff->acoustic_speed > 0) || !(ff->specific_heat_ratio > 1) ||
+      !(ff->entropy > 0) || !(ff->mw > 0))
+    return fail(c, OSTC_ERR_ARG, "far-field state: density, pressure, acoustic speed, entropy, mw > 0, gamma > 1");
+  c->blocks[b].faces.push_back({axis, side, OSTC_BC_FARFIELD, (int)c->farfields.size()});
+  c->farfields.push_back(*ff);
+  return OSTC_OK;
+}
+
+// compute_viscosity (host), src/Transport.cu:11-41
+static double host_mixture_viscosity(const ostc_mech_tables& T, double temperature, double mw_total,
+                                     const double* Y) {
+  const int ns = T.n_spec;
+  double x[OSTC_MAX_SPEC], vis[OSTC_MAX_SPEC];
+  for (int i = 0; i < ns; ++i) {
+    x[i] = Y[i] * mw_total / T.mw[i];
+    const double t_dl{temperature * T.lj_potent_inv[i]};
+    const double collision_integral{1.147 * std::pow(t_dl, -0.145) + std::pow(t_dl + 0.5, -2)};
+    vis[i] = T.vis_coeff[i] * std::sqrt(temperature) / collision_integral;
+  }
+  double viscosity{0};
+  for (int i = 0; i < ns; ++i) {
+    double vis_temp{0};
+    for (int j = 0; j < ns; ++j) {
+      double phi = 1.0;
+      if (i != j) {
+        const double numerator{1 + std::sqrt(vis[i] / vis[j]) * T.wj_div_wi_to_one4th[i][j]};
+        phi = numerator * numerator * T.sqrt_wi_div_wj_pl1_mul8[i][j];
+      }
+      vis_temp += phi * x[j];
+    }
+    viscosity += vis[i] * x[i] / vis_temp;
+  }
+  return viscosity;
+}
+
+// Species::compute_cp (Combustion2Part), src/ChemData.cpp:108-150
+static void host_cp(const ostc_mech_tables& T, double temp, double* cp) {
+  const double t2{temp * temp}, t3{t2 * temp}, t4{t3 * temp};
+  for (int i = 0; i < T.n_spec; ++i) {
+    if (temp < T.t_low[i]) {
+      const double tt = T.t_low[i];
+      const double tt2{tt * tt}, tt3{tt2 * tt}, tt4{tt3 * tt};
+      const double* c = T.low_coeff[i];
+      cp[i] = c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
+    } else {
+      const double* c = temp < T.t_mid[i] ? T.low_coeff[i] : T.high_coeff[i];
+      cp[i] = c[0] + c[1] * temp + c[2] * t2 + c[3] * t3 + c[4] * t4;
+    }
+    cp[i] *= kRu / T.mw[i];
+  }
+}
+
+// FarField::FarField with species (laminar), src/BoundCond.cpp:586-700
+int ostc_fr_farfield_state(const ostc_mech_tables* tables, const ostc_farfield_spec* in, const double* y,
+                           ostc_farfield* out) {
+  if (!tables || !in || !y || !out) return OSTC_ERR_ARG;
+  const ostc_mech_tables& T = *tables;
+  const int ns = T.n_spec;
+  if (ns < 1 || ns > OSTC_MAX_SPEC) return OSTC_ERR_ARG;
+  ostc_farfield f{};
+  double mach = in->mach, pressure = in->pressure, temperature = in->temperature, velocity = in->velocity,
+         density = in->density;
+  double u = in->u, v = in->v, w = in->w;
+  for (int i = 0; i < ns; ++i) f.sv[i] = y[i];
+  double mw = 0;
+  for (int i = 0; i < ns; ++i) mw += f.sv[i] / T.mw[i];
+  mw = 1 / mw;
+  if (temperature < 0) {
+    if (!(pressure > 0) || !(density > 0)) return OSTC_ERR_ARG;
+    temperature = pressure * mw / (density * kRu);
+  }
+  const double viscosity = host_mixture_viscosity(T, temperature, mw, f.sv);
+  double cpi[OSTC_MAX_SPEC];
+  host_cp(T, temperature, cpi);
+  double cp{0}, cv{0};
+  for (int i = 0; i < ns; ++i) {
+    cp += f.sv[i] * cpi[i];
+    cv += f.sv[i] * (cpi[i] - kRu / T.mw[i]);
+  }
+  const double gamma = cp / cv;
+  const double acoustic_speed = std::sqrt(gamma * kRu / mw * temperature);
+  if (mach < 0) mach = velocity / acoustic_speed;
+  else velocity = mach * acoustic_speed;
+  u *= velocity;
+  v *= velocity;
+  w *= velocity;
+  if (pressure < 0) {
+    if (!(in->reynolds_number > 0) || !(velocity > 0)) return OSTC_ERR_ARG;
+    density = viscosity * in->reynolds_number / velocity;
+    pressure = density * temperature * kRu / mw;
+  }
+  if (density < 0) density = pressure * mw / (kRu * temperature);
+  f.density = density;
+  f.u = u;
+  f.v = v;
+  f.w = w;
+  f.pressure = pressure;
+  f.temperature = temperature;
+  f.acoustic_speed = acoustic_speed;
+  f.entropy = pressure / std::pow(density, gamma);
+  f.specific_heat_ratio = gamma;
+  f.mw = mw;
+  *out = f;
   return OSTC_OK;
 }
 
@@ -326,6 +446,13 @@ int ostc_fr_finalize(ostc_fr_ctx c) {
       return r;
   }
   if (int r = dalloc(c, &c->d_dt, 1)) return r;
+  if (!c->farfields.empty()) {
+    static_assert(sizeof(ostc_farfield) == 26 * sizeof(double), "far-field layout");
+    if (int r = dalloc(c, &c->d_ff, c->farfields.size() * 26)) return r;
+    FR_HIP(c, hipMemcpyAsync(c->d_ff, c->farfields.data(), c->farfields.size() * sizeof(ostc_farfield),
+                             hipMemcpyHostToDevice, c->st));
+    FR_HIP(c, hipStreamSynchronize(c->st));
+  }
   long long n = 0;
   launch_dt_set(c->d_dt, c->s.fixed_time_step ? c->s.dt : 1e6, c->st, &n);
   FR_HIP(c, hipEventCreate(&c->ev_a));
@@ -379,6 +506,10 @@ int ostc_fr_initialize(ostc_fr_ctx c) {
   if (int r = check_ready(c)) return r;
   long long n = 0;
   for (int b = 0; b < (int)c->blocks.size(); ++b) launch_cv_from_bv(args(c, b), c->ns, c->st, &n);
+  // farfield faces read gamma of their face points: give the property fields
+  // the initial state's values before the first boundary pass
+  if (!c->farfields.empty())
+    for (int b = 0; b < (int)c->blocks.size(); ++b) launch_props(args(c, b), c->ns, c->st, &n, nullptr);
   for (int b = 0; b < (int)c->blocks.size(); ++b) apply_bc(c, b, &n);
   exchange(c, &n);
   for (int b = 0; b < (int)c->blocks.size(); ++b) launch_props(args(c, b), c->ns, c->st, &n, nullptr);

@@ -232,8 +232,109 @@ __global__ __launch_bounds__(kT) void k_update(FrArgs A, double ra, double rb, d
 // apply_outflow / apply_periodic with scalars, src/BoundCond.cuh:205-233,
 // :1443-1486: one thread per face point over the face's padded tangential
 // extent, looping over the ngg ghost layers.
+// apply_farfield<FR>, src/BoundCond.cuh:755-1022, at face point ix: the
+// face point's normal Mach number (gamma from the property field) picks
+// supersonic inflow (the far field on the face point and its ghosts),
+// supersonic outflow (copy to the ghosts) or the subsonic state of the two
+// Riemann invariants, whose gamma is iterated from 1.4 with cp of the
+// boundary temperature and composition until two values agree to 1e-4.
+// F: density, u, v, w, pressure, temperature, acoustic_speed, entropy,
+// specific_heat_ratio, mw, sv[16].
 template <int NS>
-__global__ __launch_bounds__(kT) void k_bc(FrArgs A, int axis, int side, int type) {
+__device__ void farfield_point(const FrArgs& A, int axis, int side, const double* __restrict__ F, const int* ix) {
+  const FrBlockDev& B = A.B;
+  const long long s = B.sz;
+  const long long p = pid(B, ix[0], ix[1], ix[2]);
+  double nx = B.met[p + (3 * axis) * s], ny = B.met[p + (3 * axis + 1) * s], nz = B.met[p + (3 * axis + 2) * s];
+  const double grad_n_inv = side / sqrt(nx * nx + ny * ny + nz * nz);
+  nx *= grad_n_inv;
+  ny *= grad_n_inv;
+  nz *= grad_n_inv;
+  const double u_b = B.bv[p + s], v_b = B.bv[p + 2 * s], w_b = B.bv[p + 3 * s];
+  const double u_face = nx * u_b + ny * v_b + nz * w_b;
+  double sv_b[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) sv_b[l] = B.sv[p + l * s];
+  const double gamma_b = B.gam[p];
+  double mw_inv{0};
+#pragma unroll
+  for (int l = 0; l < NS; ++l) mw_inv += sv_b[l] / A.T->mw[l];
+  double mw = 1.0 / mw_inv;
+  const double p_b = B.bv[p + 4 * s], rho_b = B.bv[p];
+  const double a_b = sqrt(gamma_b * p_b / rho_b);
+  const double mach_b = u_face / a_b;
+  double st[6];
+  bool face_too = true;
+  if (mach_b <= -1) {
+#pragma unroll
+    for (int l = 0; l < 6; ++l) st[l] = F[l];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) sv_b[l] = F[10 + l];
+  } else if (mach_b >= 1) {
+#pragma unroll
+    for (int l = 0; l < 6; ++l) st[l] = B.bv[p + l * s];
+    face_too = false;
+  } else {
+    const double riemann_pos = u_face + 2 * a_b / (gamma_b - 1);
+    const double u_inf = F[1] * nx + F[2] * ny + F[3] * nz;
+    const double riemann_neg = u_inf - 2 * F[6] / (F[8] - 1);
+    const double Un = 0.5 * (riemann_pos + riemann_neg);
+    if (mach_b < 0) {
+      st[1] = F[1] + (Un - u_inf) * nx;
+      st[2] = F[2] + (Un - u_inf) * ny;
+      st[3] = F[3] + (Un - u_inf) * nz;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) sv_b[l] = F[10 + l];
+      mw = F[9];
+    } else {
+      st[1] = u_b + (Un - u_face) * nx;
+      st[2] = v_b + (Un - u_face) * ny;
+      st[3] = w_b + (Un - u_face) * nz;
+    }
+    double gamma{1.4}, err{1};
+    double density = 0, pressure = 0, temperature = 0;
+    // the iteration is a fixed point of a smooth contraction; the bound only
+    // guarantees every lane leaves the loop
+    for (int it = 0; it < 1000 && err > 1e-4; ++it) {
+      const double gamma_last = gamma;
+      const double c_b = 0.25 * (gamma - 1) * (riemann_pos - riemann_neg);
+      const double s_b = mach_b <= 0 ? F[7] : p_b / pow(rho_b, gamma);
+      density = pow(c_b * c_b / (gamma * s_b), 1 / (gamma - 1));
+      pressure = density * c_b * c_b / gamma;
+      temperature = pressure * mw / (density * kRu);
+      double cp[NS];
+      heat_capacity<NS>(A.T, temperature, cp);
+      double cp_tot{0};
+#pragma unroll
+      for (int l = 0; l < NS; ++l) cp_tot += cp[l] * sv_b[l];
+      gamma = cp_tot / (cp_tot - kRu / mw);
+      err = fabs(1 - gamma / gamma_last);
+    }
+    st[0] = density;
+    st[4] = pressure;
+    st[5] = temperature;
+  }
+  if (face_too) {
+#pragma unroll
+    for (int l = 0; l < 6; ++l) B.bv[p + l * s] = st[l];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) B.sv[p + l * s] = sv_b[l];
+    cv_point<NS>(A, p);
+  }
+  for (int g = 1; g <= B.g; ++g) {
+    int gi[3] = {ix[0], ix[1], ix[2]};
+    gi[axis] += g * side;
+    const long long pg = pid(B, gi[0], gi[1], gi[2]);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) B.bv[pg + l * s] = st[l];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) B.sv[pg + l * s] = sv_b[l];
+    cv_point<NS>(A, pg);
+  }
+}
+
+template <int NS>
+__global__ __launch_bounds__(kT) void k_bc(FrArgs A, int axis, int side, int type, const double* ff) {
   const FrBlockDev& B = A.B;
   const int ext[3] = {B.mx, B.my, B.mz};
   const int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
@@ -244,6 +345,10 @@ __global__ __launch_bounds__(kT) void k_bc(FrArgs A, int axis, int side, int typ
   ix[axis] = side < 0 ? 0 : ext[axis] - 1;
   ix[o1] = q % n1 - B.g;
   ix[o2] = q / n1 - B.g;
+  if (type == 4) {
+    farfield_point<NS>(A, axis, side, ff, ix);
+    return;
+  }
   int src[3] = {ix[0], ix[1], ix[2]};
   if (type == 8) src[axis] = side < 0 ? ext[axis] - 1 : 0;
   const long long s = B.sz;
@@ -963,10 +1068,11 @@ void launch_dt_set(double* dt, double v, hipStream_t st, long long* n) {
   ++*n;
 }
 
-void launch_bc(const FrArgs& A, int ns, int axis, int side, int type, hipStream_t st, long long* n) {
+void launch_bc(const FrArgs& A, int ns, int axis, int side, int type, const double* ff, hipStream_t st,
+               long long* n) {
   const int ext[3] = {A.B.mx, A.B.my, A.B.mz};
   const long long cnt = (long long)(ext[(axis + 1) % 3] + 2 * A.B.g) * (ext[(axis + 2) % 3] + 2 * A.B.g);
-  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_bc<NS>, dim3(nblk(cnt)), dim3(kT), 0, st, A, axis, side, type))
+  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_bc<NS>, dim3(nblk(cnt)), dim3(kT), 0, st, A, axis, side, type, ff))
   ++*n;
 }
 

@@ -69,6 +69,10 @@ class _BlockDesc(C.Structure):
 
 _I, _D, _P, _LL = C.c_int, C.c_double, C.c_void_p, C.c_longlong
 _DP = C.POINTER(C.c_double)
+def _pkg():
+    return importlib.import_module(__name__.rsplit(".", 1)[0])
+
+
 SIGNATURES = {
     "ostc_mech_load": (_I, [C.c_char_p, C.c_char_p, C.c_char_p, _I, C.POINTER(MechTables), C.c_char_p, _I]),
     "ostc_fr_supported": (_I, [_I]),
@@ -77,6 +81,9 @@ SIGNATURES = {
     "ostc_fr_add_block": (_I, [_P, C.POINTER(_BlockDesc), C.POINTER(_I)]),
     "ostc_fr_add_boundary": (_I, [_P, _I, _I, _I, _I]),
     "ostc_fr_add_inner_face": (_I, [_P, _I, _I, _I, _I]),
+    "ostc_fr_add_farfield": (_I, [_P, _I, _I, _I, C.POINTER(_pkg().Farfield)]),
+    "ostc_fr_farfield_state": (_I, [C.POINTER(MechTables), C.POINTER(_pkg().FarfieldSpec), _DP,
+                                    C.POINTER(_pkg().Farfield)]),
     "ostc_fr_finalize": (_I, [_P]),
     "ostc_fr_set_geometry": (_I, [_P, _I, _DP, _DP, _LL, _LL, _LL]),
     "ostc_fr_set_field": (_I, [_P, _I, _I, _DP]),
@@ -93,10 +100,6 @@ SIGNATURES = {
     "ostc_fr_last_error": (C.c_char_p, [_P]),
     "ostc_util_mixing_layer_sv": (_I, [_I, _I, _I, _I, _I, _D, _D, _DP, _DP, _DP]),
 }
-
-
-def _pkg():
-    return importlib.import_module(__name__.rsplit(".", 1)[0])
 
 
 def _dp(a):
@@ -119,6 +122,19 @@ def load_mechanism(mech_file=H2_AIR_FILES[0], therm_file=H2_AIR_FILES[1], tran_f
 
 def supported(n_spec: int) -> bool:
     return bool(_pkg().lib().ostc_fr_supported(n_spec))
+
+
+def farfield_state(tables: MechTables, y, **kw):
+    """FarField constructor with species (src/BoundCond.cpp:586-700): mass
+    fractions ``y`` and keyword inputs (mach, pressure, temperature, ...)."""
+    pkg = _pkg()
+    yy = np.zeros(MAX_SPEC)
+    yy[:tables.n_spec] = y
+    out = pkg.Farfield()
+    r = pkg.lib().ostc_fr_farfield_state(C.byref(tables), C.byref(pkg.farfield_spec(**kw)), _dp(yy), C.byref(out))
+    if r != 0:
+        raise pkg.OstcError(f"bad far-field specification ({r})")
+    return out
 
 
 class FrSolver:
@@ -146,7 +162,11 @@ class FrSolver:
         for b in case["blocks"]:
             self._chk(L.ostc_fr_add_block(self.ctx, C.byref(_BlockDesc(b["mx"], b["my"], b["mz"])), None))
         for f in case["boundaries"]:
-            self._chk(L.ostc_fr_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"]))
+            if f["type"] == 4:
+                ff = farfield_state(self.T, f["farfield_y"], **f["farfield"])
+                self._chk(L.ostc_fr_add_farfield(self.ctx, f["block"], f["axis"], f["side"], C.byref(ff)))
+            else:
+                self._chk(L.ostc_fr_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"]))
         for f in case["inner_faces"]:
             self._chk(L.ostc_fr_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"]))
         self._chk(L.ostc_fr_finalize(self.ctx))

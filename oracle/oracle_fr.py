@@ -70,6 +70,8 @@ def lib():
         L.orf_char_matrices.argtypes = [tp, dp, dp, dp, dp, d, d, dp, dp]
         L.orf_riemann.argtypes = [tp, i, dp, dp, dp, dp, d, d, dp]
         L.orf_reconstruct.argtypes = [dp, i, i, dp, dp]
+        L.orf_farfield_state.argtypes = [tp, C.POINTER(OA.FarfieldSpec), dp, C.POINTER(OA.Farfield)]
+        L.orf_add_farfield.argtypes = [vp, i, i, i, C.POINTER(OA.Farfield)]
         _lib = L
     return _lib
 
@@ -83,6 +85,16 @@ def tables(mech_file, therm_file, tran_file, reaction=1):
 
 
 # ----------------------------------------------------------- point routines
+def farfield_state(T, y, **kw):
+    """FarField constructor with species: mass fractions y, keyword inputs
+    (mach, pressure, temperature, ...) as for air."""
+    yy = np.zeros(mech.MAX_SPEC)
+    yy[:T.n_spec] = y
+    out = OA.Farfield()
+    lib().orf_farfield_state(C.byref(T), C.byref(OA.farfield_spec(**kw)), _dp(yy), C.byref(out))
+    return out
+
+
 def enthalpy_cp(T, t):
     h, cp = np.zeros(mech.MAX_SPEC), np.zeros(mech.MAX_SPEC)
     lib().orf_enthalpy_cp(C.byref(T), t, _dp(h), _dp(cp))
@@ -186,7 +198,11 @@ class FrOracle:
         for b in case["blocks"]:
             L.orf_add_block(self.ctx, b["mx"], b["my"], b["mz"])
         for f in case["boundaries"]:
-            L.orf_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"])
+            if f["type"] == 4:
+                ff = farfield_state(self.T, f["farfield_y"], **f["farfield"])
+                L.orf_add_farfield(self.ctx, f["block"], f["axis"], f["side"], C.byref(ff))
+            else:
+                L.orf_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"])
         for f in case["inner_faces"]:
             L.orf_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"])
 

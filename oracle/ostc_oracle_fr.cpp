@@ -502,6 +502,7 @@ void char_back(const CharFace& F, int ns, const double* fChar, double* fci) {
 // --------------------------------------------------------------- structures
 struct Face {
   int axis, side, type;
+  orf_farfield ff;  // type 4: the state of its label
 };
 struct InnerFace {
   int axis, side, target;
@@ -1215,8 +1216,93 @@ void update(const Ctx& c, Block& b, double dt, int rk) {
       }
 }
 
-int bc_priority(int type) {  // src/BoundCond.cuh:1488-1678: outflow before periodic
-  return type == 6 ? 0 : 1;
+int bc_priority(int type) {  // src/BoundCond.cuh:1488-1678: farfield, outflow, ..., periodic
+  return type == 4 ? 0 : (type == 6 ? 1 : 2);
+}
+
+// apply_farfield<FR>, src/BoundCond.cuh:755-1022, one face point (i, j, k)
+// and its ngg ghosts.  The face point's gamma comes from the property field;
+// the subsonic state iterates gamma (from 1.4) until two successive values
+// agree to 1e-4, with cp of the boundary temperature and composition.
+void farfield_point(const Ctx& c, Block& b, const Face& f, int i, int j, int k) {
+  const long long s = b.sz;
+  const int ns = c.ns;
+  const orf_farfield& ff = f.ff;
+  int dir[3] = {0, 0, 0};
+  dir[f.axis] = f.side;
+  const long long p = b.id(i, j, k);
+  double nx = b.metric[p + (3 * f.axis) * s], ny = b.metric[p + (3 * f.axis + 1) * s],
+         nz = b.metric[p + (3 * f.axis + 2) * s];
+  const double grad_n_inv = f.side / std::sqrt(nx * nx + ny * ny + nz * nz);
+  nx *= grad_n_inv;
+  ny *= grad_n_inv;
+  nz *= grad_n_inv;
+  const double u_b{b.bv[p + s]}, v_b{b.bv[p + 2 * s]}, w_b{b.bv[p + 3 * s]};
+  const double u_face{nx * u_b + ny * v_b + nz * w_b};
+  double sv_b[MS];
+  for (int l = 0; l < ns; ++l) sv_b[l] = b.sv[p + l * s];
+  const double gamma_b{b.gam[p]};
+  double mw_inv{0};
+  for (int l = 0; l < ns; ++l) mw_inv += sv_b[l] / c.T.mw[l];
+  double mw = 1.0 / mw_inv;
+  const double p_b{b.bv[p + 4 * s]}, rho_b{b.bv[p]};
+  const double a_b{std::sqrt(gamma_b * p_b / rho_b)};
+  const double mach_b{u_face / a_b};
+  double st[6];
+  bool face_too = true;
+  if (mach_b <= -1) {
+    st[0] = ff.density; st[1] = ff.u; st[2] = ff.v; st[3] = ff.w; st[4] = ff.pressure; st[5] = ff.temperature;
+    for (int l = 0; l < ns; ++l) sv_b[l] = ff.sv[l];
+  } else if (mach_b >= 1) {
+    for (int l = 0; l < 6; ++l) st[l] = b.bv[p + l * s];
+    face_too = false;
+  } else {
+    const double riemann_pos{u_face + 2 * a_b / (gamma_b - 1)};
+    const double u_inf{ff.u * nx + ff.v * ny + ff.w * nz};
+    const double riemann_neg{u_inf - 2 * ff.acoustic_speed / (ff.specific_heat_ratio - 1)};
+    const double Un{0.5 * (riemann_pos + riemann_neg)};
+    if (mach_b < 0) {
+      st[1] = ff.u + (Un - u_inf) * nx;
+      st[2] = ff.v + (Un - u_inf) * ny;
+      st[3] = ff.w + (Un - u_inf) * nz;
+      for (int l = 0; l < ns; ++l) sv_b[l] = ff.sv[l];
+      mw = ff.mw;
+    } else {
+      st[1] = u_b + (Un - u_face) * nx;
+      st[2] = v_b + (Un - u_face) * ny;
+      st[3] = w_b + (Un - u_face) * nz;
+    }
+    double gamma{1.4}, err{1}, gamma_last;
+    double density = 0, pressure = 0, temperature = 0;
+    while (err > 1e-4) {
+      gamma_last = gamma;
+      const double c_b{0.25 * (gamma - 1) * (riemann_pos - riemann_neg)};
+      const double s_b = mach_b <= 0 ? ff.entropy : p_b / std::pow(rho_b, gamma);
+      density = std::pow(c_b * c_b / (gamma * s_b), 1 / (gamma - 1));
+      pressure = density * c_b * c_b / gamma;
+      temperature = pressure * mw / (density * kRu);
+      double cp[MS];
+      heat_capacity(c.T, temperature, cp);
+      double cp_tot{0};
+      for (int l = 0; l < ns; ++l) cp_tot += cp[l] * sv_b[l];
+      gamma = cp_tot / (cp_tot - kRu / mw);
+      err = std::abs(1 - gamma / gamma_last);
+    }
+    st[0] = density;
+    st[4] = pressure;
+    st[5] = temperature;
+  }
+  if (face_too) {
+    for (int l = 0; l < 6; ++l) b.bv[p + l * s] = st[l];
+    for (int l = 0; l < ns; ++l) b.sv[p + l * s] = sv_b[l];
+    cv_point(c, b, p);
+  }
+  for (int g = 1; g <= b.g; ++g) {
+    const long long pg = b.id(i + g * dir[0], j + g * dir[1], k + g * dir[2]);
+    for (int l = 0; l < 6; ++l) b.bv[pg + l * s] = st[l];
+    for (int l = 0; l < ns; ++l) b.sv[pg + l * s] = sv_b[l];
+    cv_point(c, b, pg);
+  }
 }
 
 // apply_outflow / apply_periodic with scalars, src/BoundCond.cuh:205-233, :1443-1486
@@ -1237,6 +1323,10 @@ void apply_bc(const Ctx& c, Block& b) {
         ix[a] = face_idx;
         ix[o1] = t1;
         ix[o2] = t2;
+        if (f.type == 4) {
+          farfield_point(c, b, f, ix[0], ix[1], ix[2]);
+          continue;
+        }
         int src[3] = {ix[0], ix[1], ix[2]};
         if (f.type == 8) src[a] = f.side < 0 ? ext[a] - 1 : 0;
         for (int g = 1; g <= b.g; ++g) {
@@ -1389,7 +1479,86 @@ int orf_add_block(void* ctx, int mx, int my, int mz) {
 int orf_ngg(void* ctx) { return C(ctx)->ngg; }
 
 int orf_add_boundary(void* ctx, int block, int axis, int side, int type) {
-  C(ctx)->blocks[block].faces.push_back({axis, side, type});
+  C(ctx)->blocks[block].faces.push_back({axis, side, type, orf_farfield{}});
+  return 0;
+}
+
+int orf_add_farfield(void* ctx, int block, int axis, int side, const orf_farfield* ff) {
+  Face f{axis, side, 4, *ff};
+  C(ctx)->blocks[block].faces.push_back(f);
+  return 0;
+}
+
+// compute_viscosity (host), src/Transport.cu:11-41
+double mixture_viscosity(const Tab& T, double temperature, double mw_total, const double* Y) {
+  const int ns = T.n_spec;
+  double x[MS], vis[MS];
+  for (int i = 0; i < ns; ++i) {
+    x[i] = Y[i] * mw_total / T.mw[i];
+    const double t_dl{temperature * T.lj_potent_inv[i]};
+    const double collision_integral{1.147 * std::pow(t_dl, -0.145) + std::pow(t_dl + 0.5, -2)};
+    vis[i] = T.vis_coeff[i] * std::sqrt(temperature) / collision_integral;
+  }
+  double viscosity{0};
+  for (int i = 0; i < ns; ++i) {
+    double vis_temp{0};
+    for (int j = 0; j < ns; ++j) {
+      double phi = 1.0;
+      if (i != j) {
+        const double numerator{1 + std::sqrt(vis[i] / vis[j]) * T.wj_div_wi_to_one4th[i][j]};
+        phi = numerator * numerator * T.sqrt_wi_div_wj_pl1_mul8[i][j];
+      }
+      vis_temp += phi * x[j];
+    }
+    viscosity += vis[i] * x[i] / vis_temp;
+  }
+  return viscosity;
+}
+
+// FarField::FarField with species (laminar), src/BoundCond.cpp:586-700
+int orf_farfield_state(const ostc_mech_tables* t, const orf_farfield_spec* in, const double* y, orf_farfield* out) {
+  const Tab& T = *t;
+  const int ns = T.n_spec;
+  orf_farfield f{};
+  double mach = in->mach, pressure = in->pressure, temperature = in->temperature, velocity = in->velocity,
+         density = in->density;
+  double u = in->u, v = in->v, w = in->w;
+  for (int i = 0; i < ns; ++i) f.sv[i] = y[i];
+  double mw = 0;
+  for (int i = 0; i < ns; ++i) mw += f.sv[i] / T.mw[i];
+  mw = 1 / mw;
+  if (temperature < 0) temperature = pressure * mw / (density * kRu);
+  const double viscosity = mixture_viscosity(T, temperature, mw, f.sv);
+  double cpi[MS];
+  heat_capacity(T, temperature, cpi);
+  double cp{0}, cv{0};
+  for (int i = 0; i < ns; ++i) {
+    cp += f.sv[i] * cpi[i];
+    cv += f.sv[i] * (cpi[i] - kRu / T.mw[i]);
+  }
+  const double gamma = cp / cv;
+  const double acoustic_speed = std::sqrt(gamma * kRu / mw * temperature);
+  if (mach < 0) mach = velocity / acoustic_speed;
+  else velocity = mach * acoustic_speed;
+  u *= velocity;
+  v *= velocity;
+  w *= velocity;
+  if (pressure < 0) {
+    density = viscosity * in->reynolds_number / velocity;
+    pressure = density * temperature * kRu / mw;
+  }
+  if (density < 0) density = pressure * mw / (kRu * temperature);
+  f.density = density;
+  f.u = u;
+  f.v = v;
+  f.w = w;
+  f.pressure = pressure;
+  f.temperature = temperature;
+  f.acoustic_speed = acoustic_speed;
+  f.entropy = pressure / std::pow(density, gamma);
+  f.specific_heat_ratio = gamma;
+  f.mw = mw;
+  *out = f;
   return 0;
 }
 
@@ -1426,6 +1595,9 @@ double* orf_array(void* ctx, int block, int which, long long* n) {
 int orf_initialize(void* ctx) {
   Ctx* c = C(ctx);
   for (auto& b : c->blocks) cv_from_bv(*c, b);
+  // the farfield faces read gamma of their face points; the reference reads
+  // a property field nothing has written yet, here it holds the initial state's
+  for (auto& b : c->blocks) properties(*c, b);
   for (auto& b : c->blocks) apply_bc(*c, b);
   communicate(*c);
   for (auto& b : c->blocks) properties(*c, b);

@@ -49,9 +49,26 @@ int     orf_add_block(void* ctx, int mx, int my, int mz);
 int     orf_ngg(void* ctx);
 int     orf_add_boundary(void* ctx, int block, int axis, int side, int type);  /* 6 outflow, 8 periodic */
 int     orf_add_inner_face(void* ctx, int block, int axis, int side, int target_block);
+
+/* Far-field state of one label (FarField after its constructor,
+ * src/BoundCond.h:129-149) and the constructor's inputs (-1 = not given,
+ * velocity 0, direction (1, 0, 0)); the same layouts as the air oracle's. */
+typedef struct {
+  double density, u, v, w, pressure, temperature;
+  double acoustic_speed, entropy, specific_heat_ratio, mw;
+  double sv[OSTC_MAX_SPEC];
+} orf_farfield;
+typedef struct {
+  double mach, pressure, temperature, velocity, density, u, v, w, reynolds_number;
+} orf_farfield_spec;
+/* FarField::FarField with species (src/BoundCond.cpp:586-700): mass fractions y */
+int     orf_farfield_state(const ostc_mech_tables* t, const orf_farfield_spec* in, const double* y,
+                           orf_farfield* out);
+/* farfield face (label 4), apply_farfield<FR> (src/BoundCond.cuh:755-1022) */
+int     orf_add_farfield(void* ctx, int block, int axis, int side, const orf_farfield* ff);
 double* orf_array(void* ctx, int block, int which, long long* n_out);
 
-int     orf_initialize(void* ctx);            /* BCs, exchange, properties, cv from bv */
+int     orf_initialize(void* ctx);            /* cv from bv, properties, BCs, exchange, properties */
 int     orf_step(void* ctx, int n_steps, double* last_dt);
 /* stage pieces */
 int     orf_cv_from_bv(void* ctx, int block);
