@@ -167,6 +167,39 @@ def reacting_cpu_baseline(tables, n, steps):
                       f"layer, {dt:.1f} s on {threads} OpenMP threads"}
 
 
+def measured_traffic(tile, split: int, waves: int):
+    """HBM bytes per launch of the fused kernel for this tile configuration,
+    from the FETCH_SIZE / WRITE_SIZE counter runs summarised in
+    profiles/fused_rhs_update_batched24_pmc.txt (the same 24-block case), or
+    None when the configuration was not measured."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "fused_rhs_update_batched24_pmc.txt")
+    if waves != 0 or not os.path.exists(path):
+        return None
+    want = f"traffic tile={tile[0]}x{tile[1]}x{tile[2]} split={int(split)} "
+    with open(path) as f:
+        for line in f:
+            if line.startswith(want):
+                fields = dict(kv.split("=") for kv in line.split()[1:])
+                return int(fields["bytes"])
+    return None
+
+
+def copy_probe(mib: int):
+    """Five launches of an elementwise FP64 copy (b = a * 1) of ``mib`` MiB:
+    each reads and writes exactly mib MiB, the reference figure that turns
+    the FETCH_SIZE / WRITE_SIZE counters of a profiled run into bytes."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8,
+                      "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,11 +218,17 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 134M-cell reacting case")
     ap.add_argument("--reacting-only", action="store_true",
                     help="run only the 512x256x128 reacting case and print its result (profiling)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain FP64 copy of MIB MiB five times (calibrates HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: this build runs the path on a "
+              "single GPU and does not shard it.", file=sys.stderr)
         sys.exit(2)
+
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
 
     if a.reacting_only:
         r = reacting_leg([511], [255], [127], a.reacting_steps, 2, "512x256x128 single block")
@@ -248,8 +287,11 @@ def main():
     avg_kernel_s = avg_kernel_ms / 1e3
     bytes_per_launch = BYTES_PER_CELL_STAGE * cells
     achieved = bytes_per_launch / avg_kernel_s / 1e9
+    traffic = measured_traffic(pkg.tile_shapes()[tile_index], tile_split, tile_waves)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "traffic_source": "profiles/fused_rhs_update_batched24_pmc.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                  "copy-calibrated, bytes per launch)" if traffic else None,
                 "kernel": "k_fused (RHS+update, one RK stage over all blocks)",
                 "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_kernel_s * 1e3}
     s.close()

@@ -85,7 +85,7 @@ struct ostc_ctx_s {
   Group props, rim;
   std::vector<std::vector<int>> plain_bc;
   std::vector<int> plain_comm;
-  int tile_index = 0, n_wg = 0, xcd_remap = 0, n_tiles = 0, waves = 0, split = 0;
+  int tile_index = 0, n_wg = 0, xcd_remap = 1, n_tiles = 0, waves = 0, split = 0;
   int* d_tile_block = nullptr;
   double* d_scratch = nullptr;
   int n_first = 1024;
@@ -1131,8 +1131,9 @@ int ostc_set_tile(ostc_ctx c, int index, int waves_per_cu, int split) {
   if ((r = need_fused(c))) return r;
   if (index < 0 || index >= fused_shape_count(c->P.scheme)) return fail(c, OSTC_ERR_ARG, "tile index out of range");
   // waves_per_cu <= 0: one workgroup per tile; otherwise a persistent grid of
-  // waves_per_cu workgroups per compute unit, with XCD-contiguous chunks.
-  c->xcd_remap = waves_per_cu > 0 ? 1 : 0;
+  // waves_per_cu workgroups per compute unit.  Either way the workgroups of
+  // one XCD take consecutive tiles.
+  c->xcd_remap = 1;
   c->waves = waves_per_cu > 0 ? waves_per_cu : 0;
   c->split = split ? 1 : 0;
   return set_tiles(c, index, waves_per_cu > 0 ? waves_per_cu * c->cus : 0);

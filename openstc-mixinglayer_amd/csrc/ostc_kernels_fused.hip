@@ -316,19 +316,26 @@ __global__ __launch_bounds__(TX * TY * TZ) __attribute__((amdgpu_waves_per_eu(WP
   t.ti = tid % TX;
   t.tj = (tid / TX) % TY;
   t.tk = tid / (TX * TY);
-  // tile assignment: plain stride over the work-list, or (xcd_remap)
-  // contiguous chunks for workgroups that share an XCD (round-robin dispatch
-  // puts workgroup w on XCD w % 8).
+  // tile assignment.  Round-robin dispatch puts workgroup w on XCD w % 8;
+  // with xcd_remap the workgroups of one XCD take consecutive work items
+  // (XCD x owns items [x q + min(x, r), ...) of G = 8 q + r), so tiles that
+  // run together on one XCD are neighbours in the block and share their
+  // halo points through that XCD's L2.  One workgroup per tile: item = tile;
+  // a persistent grid: item = a chunk of consecutive tiles.
   const int G = gridDim.x;
+  int lid = blockIdx.x;
+  if (A.xcd_remap) {
+    const int x = blockIdx.x % 8, q = G / 8, r = G % 8;
+    lid = x * q + min(x, r) + blockIdx.x / 8;
+  }
   int first, step, last;
-  if (A.xcd_remap && (G % 8) == 0) {
-    const int lid = (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
+  if (G < A.n_tiles && A.xcd_remap) {
     const int chunk = (A.n_tiles + G - 1) / G;
     first = lid * chunk;
     last = min(A.n_tiles, first + chunk);
     step = 1;
   } else {
-    first = blockIdx.x;
+    first = lid;
     last = A.n_tiles;
     step = G;
   }
