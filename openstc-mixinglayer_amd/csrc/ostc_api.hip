@@ -405,13 +405,14 @@ int upload_blocks(ostc_ctx c) {
 }
 
 // The fused batched kernel and the reverse mode implement the component-wise
-// WENO schemes 50/51/71; the characteristic WENO (52) and the primitive
+// WENO schemes 50/51/71; the characteristic WENO (52, 72) and the primitive
 // reconstruction + Riemann schemes (1/3/4) run on the per-block kernels.
 bool fused_scheme(const ostc_ctx c) { return c->P.scheme == 51 || c->P.scheme == 50 || c->P.scheme == 71; }
 
 int need_fused(ostc_ctx c) {
   if (fused_scheme(c)) return OSTC_OK;
-  return fail(c, OSTC_ERR_UNSUPPORTED, "inviscid_scheme 52/1/3/4 run with OSTC_MODE_PLAIN and the stage operators only");
+  return fail(c, OSTC_ERR_UNSUPPORTED,
+              "inviscid_scheme 52/72/1/3/4 run with OSTC_MODE_PLAIN and the stage operators only");
 }
 
 int set_tiles(ostc_ctx c, int index, int n_wg) {
@@ -606,13 +607,14 @@ int ostc_create(const ostc_setup* s, void* hip_stream, ostc_ctx* out) {
   *out = nullptr;
   if (s->n_var != 5) return OSTC_ERR_UNSUPPORTED;
   const int sc = s->inviscid_scheme;
-  if (sc != 51 && sc != 50 && sc != 71 && sc != 52 && sc != 1 && sc != 3 && sc != 4) return OSTC_ERR_UNSUPPORTED;
+  if (sc != 51 && sc != 50 && sc != 71 && sc != 52 && sc != 72 && sc != 1 && sc != 3 && sc != 4)
+    return OSTC_ERR_UNSUPPORTED;
   if ((sc == 1 || sc == 3 || sc == 4) && (s->reconstruction < 1 || s->reconstruction > 3)) return OSTC_ERR_UNSUPPORTED;
   if (s->viscous_order != 0 && s->viscous_order != 2) return OSTC_ERR_UNSUPPORTED;
   if (!(s->cfl > 0) || !(s->prandtl > 0)) return OSTC_ERR_ARG;
   auto* c = new ostc_ctx_s;
   c->s = *s;
-  c->ngg = s->inviscid_scheme == 71 ? 4 : 3;
+  c->ngg = (sc == 71 || sc == 72) ? 4 : 3;  // src/Parameter.cpp:227-238
   c->P.n_var = 5;
   c->P.scheme = s->inviscid_scheme;
   c->P.visc = s->viscous_order == 2;

@@ -50,11 +50,12 @@ __global__ __launch_bounds__(64) void k_weno_pencil(DevBlock B, Params P, int sl
     const long long p2 = pidx(B, ijk[0], ijk[1], ijk[2]);
     const double j1 = B.jac[p1], j2 = B.jac[p2];
     const int sL = tid + R - 1;  // LDS index of cell L
-    if constexpr (SCHEME == 52) {
+    if constexpr (SCHEME == 52 || SCHEME == 72) {
       // compute_weno_flux_ch<Air>, "method 1" (src/WENO.cu:1147-1453): the
-      // five split fluxes of each side projected on the left eigenvectors of
-      // the Roe-averaged face state, WENO-Z5 per characteristic field, and the
-      // face flux projected back with the right eigenvectors.
+      // W split fluxes of each side projected on the left eigenvectors of
+      // the Roe-averaged face state, WENO-Z5 (52) or WENO-Z7 (72) per
+      // characteristic field, and the face flux projected back with the right
+      // eigenvectors.
       double cvl[6], cvr[6];
       for (int l = 0; l < 5; ++l) {
         cvl[l] = cv[p1 + l * s];
@@ -71,16 +72,17 @@ __global__ __launch_bounds__(64) void k_weno_pencil(DevBlock B, Params P, int sl
       const double e = eps * P.weno_eps_scale * 0.25 * (ax * ax + ay * ay + az * az);
       double fChar[5];
       for (int l = 0; l < 5; ++l) {
-        double vp[5], vm[5];
-        for (int r = 0; r < 5; ++r) {
-          vp[r] = M.L[l][0] * sFp[0][sL - 2 + r];
-          vm[r] = M.L[l][0] * sFm[0][sL - 1 + r];
+        double vp[W], vm[W];
+        for (int r = 0; r < W; ++r) {
+          vp[r] = M.L[l][0] * sFp[0][sL - W / 2 + r];
+          vm[r] = M.L[l][0] * sFm[0][sL - W / 2 + 1 + r];
           for (int n = 1; n < 5; ++n) {
-            vp[r] += M.L[l][n] * sFp[n][sL - 2 + r];
-            vm[r] += M.L[l][n] * sFm[n][sL - 1 + r];
+            vp[r] += M.L[l][n] * sFp[n][sL - W / 2 + r];
+            vm[r] += M.L[l][n] * sFm[n][sL - W / 2 + 1 + r];
           }
         }
-        fChar[l] = weno5z(vp, vm, e);
+        if constexpr (SCHEME == 72) fChar[l] = weno7z(vp, vm, e);
+        else fChar[l] = weno5z(vp, vm, e);
       }
       for (int l = 0; l < 5; ++l) {
         double f = M.R[l][0] * fChar[0];
@@ -423,6 +425,7 @@ void plain_rhs(const DevBlock& B, const Params& P, hipStream_t st, long long* n)
   if (P.scheme == 71) launch_weno<71>(B, P, st, n);
   else if (P.scheme == 50) launch_weno<50>(B, P, st, n);
   else if (P.scheme == 52) launch_weno<52>(B, P, st, n);
+  else if (P.scheme == 72) launch_weno<72>(B, P, st, n);
   else if (P.scheme == 1 || P.scheme == 3 || P.scheme == 4) {
     hipLaunchKernelGGL(k_pv_pencil<0>, pencil_grid(B, 0), dim3(64), 0, st, B, P, 0);
     hipLaunchKernelGGL(k_pv_pencil<1>, pencil_grid(B, 1), dim3(64), 0, st, B, P, 0);

@@ -682,8 +682,12 @@ void inviscid(const Ctx& c, Block& b) {
             const double jac1{b.jac[p1]}, jac2{b.jac[p2]};
             const double* m1 = &kj[is * 3];
             const double* m2 = &kj[(is + 1) * 3];
-            if (scheme == 52) {
-              // compute_weno_flux_ch<Air>, "method 1", src/WENO.cu:1147-1453
+            if (scheme == 52 || scheme == 72) {
+              // compute_weno_flux_ch<Air>, "method 1", src/WENO.cu:1147-1453;
+              // 72 projects 7 points and applies WENO-Z7 to the five fields
+              // (:1309-1320; the reference's field loop there runs to 7 over
+              // its 5x5 matrix and 5-entry fChar, past their ends -- only
+              // the five defined fields enter the back projection)
               real cvl[6], cvr[6];
               for (int l = 0; l < 5; ++l) {
                 cvl[l] = b.cv[p1 + l * s];
@@ -700,6 +704,16 @@ void inviscid(const Ctx& c, Block& b) {
                                          (m1[2] * jac1 + m2[2] * jac2) * (m1[2] * jac1 + m2[2] * jac2));
               real fChar[5];
               for (int l = 0; l < 5; ++l) {
+                if (scheme == 72) {
+                  real vPlus[7] = {}, vMinus[7] = {};
+                  for (int r = 0; r < 7; ++r)
+                    for (int n = 0; n < 5; ++n) {
+                      vPlus[r] += M.L[l][n] * Fp[(is - 3 + r) * nv + n];
+                      vMinus[r] += M.L[l][n] * Fm[(is - 2 + r) * nv + n];
+                    }
+                  fChar[l] = weno7z(vPlus, vMinus, eps_scaled);
+                  continue;
+                }
                 real vPlus[5] = {}, vMinus[5] = {};
                 for (int r = 0; r < 5; ++r)
                   for (int n = 0; n < 5; ++n) {

@@ -26,7 +26,8 @@ extern "C" {
 
 typedef struct {
   int    n_var;            /* 5 for air */
-  int    inviscid_scheme;  /* 51 WENO-Z5 cp, 52 WENO-Z5 characteristic, 71 WENO-Z7 cp, 50 WENO5-JS cp
+  int    inviscid_scheme;  /* 51 WENO-Z5 cp, 52 WENO-Z5 characteristic, 71 WENO-Z7 cp, 72 WENO-Z7
+                              characteristic, 50 WENO5-JS cp
                               (not in the reference); 1 LF, 3 AUSM+, 4 HLLC on reconstructed primitives */
   int    viscous_order;    /* 0 inviscid, 2 second-order central */
   int    fixed_time_step;  /* 1: use dt below; 0: CFL-based global dt */

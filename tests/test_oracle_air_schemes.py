@@ -1,4 +1,4 @@
-"""Pins of the air oracle's characteristic WENO-Z5 (scheme 52) and primitive
+"""Pins of the air oracle's characteristic WENO-Z5 and WENO-Z7 (schemes 52, 72) and primitive
 reconstruction + Riemann solver path (schemes 1 LF, 3 AUSM+, 4 HLLC) (CPU only).
 
 * L R = I for the air eigenvector pair at a face with equal states;
@@ -97,7 +97,7 @@ def _case(scheme, recon=2, n=10):
     return c
 
 
-@pytest.mark.parametrize("scheme,recon", [(52, 2), (1, 1), (1, 2), (3, 2), (3, 3), (4, 1), (4, 2), (4, 3)])
+@pytest.mark.parametrize("scheme,recon", [(52, 2), (72, 2), (1, 1), (1, 2), (3, 2), (3, 3), (4, 1), (4, 2), (4, 3)])
 def test_free_stream_and_conservation(scheme, recon):
     o = O.Oracle(_case(scheme, recon))
     exact_grid(o)
@@ -125,3 +125,44 @@ def test_free_stream_and_conservation(scheme, recon):
     assert np.all(np.abs(tot) <= 1e-13 * scale), (tot, scale)
     assert np.all(scale > 0)
 
+
+
+
+def _wave_dq(scheme, n):
+    """dq of a small-amplitude plane wave along (1, 2, -1) in index space,
+    n - 1 nodes per period on every axis."""
+    o = O.Oracle(_case(scheme, 2, n))
+    exact_grid(o)
+    g = o.ngg
+    idx = np.arange(-g, n + g)
+    ph = 2 * np.pi * (idx % (n - 1)) / (n - 1)
+    Z, Y, X = np.meshgrid(ph, ph, ph, indexing="ij")
+    W = X + 2 * Y - Z
+    set_state(o, 1.2 * (1 + 1e-2 * np.sin(W)), 40.0 + 0.5 * np.cos(W), -20.0 + 0.3 * np.sin(W + 1),
+              10.0 + 0 * W, 1e5 * (1 + 1e-2 * np.sin(W + 0.3)))
+    L = O.lib()
+    L.orc_apply_bc(o.ctx, 0)
+    L.orc_properties(o.ctx, 0)
+    o.rhs(0)
+    return o.array(0, O.DQ).copy()
+
+
+def test_characteristic_and_componentwise_weno_converge_together():
+    """In the smooth limit WENO-Z weights tend to the linear ones, where a
+    per-face linear projection commutes with the reconstruction: the
+    characteristic residuals (52, 72) approach the component-wise ones (51,
+    71) at the schemes' order.  A stencil shifted by one point would sit at
+    O(h k) ~ 0.5 instead."""
+    err = {}
+    for n in (20, 30):
+        d = {s: _wave_dq(s, n) for s in (51, 52, 71, 72)}
+        for l in range(5):
+            scale = np.max(np.abs(d[71][l]))
+            e72 = np.max(np.abs(d[72][l] - d[71][l])) / scale
+            e52 = np.max(np.abs(d[52][l] - d[51][l])) / scale
+            err[(n, l)] = (e72, e52)
+            assert e72 <= 5e-3 and e52 <= 2e-2, (n, l, e72, e52)
+            assert np.max(np.abs(np.roll(d[72][l], 1, axis=1) - d[71][l])) / scale > 0.2
+    for l in range(5):
+        assert err[(30, l)][0] <= 0.1 * err[(20, l)][0], (l, err)
+        assert err[(30, l)][1] <= 0.3 * err[(20, l)][1], (l, err)
