@@ -298,7 +298,7 @@ int ostc_adjoint_timing(ostc_ctx ctx, double* forward_ms, double* backward_ms);
 typedef struct ostc_fr_ctx_s* ostc_fr_ctx;
 
 typedef struct {
-  int inviscid_scheme;       /* 51 WENO-Z5 component-wise, 52 WENO-Z5 characteristic,
+  int inviscid_scheme;       /* 51 / 71 WENO-Z5 / WENO-Z7 component-wise, 52 / 72 characteristic,
                                 3 AUSM+, 4 HLLC on reconstructed primitives (inviscid_type 0) */
   int viscous_order;         /* 0 or 2 */
   int reaction;              /* 1: finite-rate chemistry source */
@@ -351,6 +351,8 @@ int ostc_fr_destroy(ostc_fr_ctx ctx);
 int ostc_fr_add_block(ostc_fr_ctx ctx, const ostc_block_desc* d, int* block_id);
 int ostc_fr_add_boundary(ostc_fr_ctx ctx, int block, int axis, int side, int type);  /* outflow, periodic */
 int ostc_fr_add_inner_face(ostc_fr_ctx ctx, int block, int axis, int side, int target_block);
+/* ghost layers of every array of the context: 3, or 4 for the WENO-Z7 schemes 71/72 (src/Parameter.cpp:227-238) */
+int ostc_fr_ngg(ostc_fr_ctx ctx, int* ngg);
 /* Farfield face (label 4) of a mixture, apply_farfield<FR> (src/BoundCond.cuh:755-1022): the subsonic
  * state iterates gamma with cp of the boundary temperature and composition. */
 int ostc_fr_add_farfield(ostc_fr_ctx ctx, int block, int axis, int side, const ostc_farfield* ff);

@@ -219,8 +219,8 @@ int ostc_fr_supported(int n_spec) { return supported_ns(n_spec) ? 1 : 0; }
 int ostc_fr_create(const ostc_fr_setup* s, const ostc_mech_tables* mech, void* stream, ostc_fr_ctx* out) {
   if (!s || !mech || !out) return OSTC_ERR_ARG;
   *out = nullptr;
-  if (s->inviscid_scheme != 51 && s->inviscid_scheme != 52 && s->inviscid_scheme != 3 && s->inviscid_scheme != 4)
-    return OSTC_ERR_UNSUPPORTED;
+  const int sc = s->inviscid_scheme;
+  if (sc != 51 && sc != 52 && sc != 71 && sc != 72 && sc != 3 && sc != 4) return OSTC_ERR_UNSUPPORTED;
   if ((s->inviscid_scheme == 3 || s->inviscid_scheme == 4) && (s->reconstruction < 1 || s->reconstruction > 3))
     return OSTC_ERR_UNSUPPORTED;
   if (s->viscous_order != 0 && s->viscous_order != 2) return OSTC_ERR_UNSUPPORTED;
@@ -232,6 +232,7 @@ int ostc_fr_create(const ostc_fr_setup* s, const ostc_mech_tables* mech, void* s
   c->tab = *mech;
   c->ns = mech->n_spec;
   c->nv = 5 + c->ns;
+  c->ngg = (sc == 71 || sc == 72) ? 4 : 3;  // src/Parameter.cpp:227-238
   c->weno_eps_scale = (s->rho_ref * s->v_ref) * (s->rho_ref * s->v_ref);  // src/DParameter.cu:26-29
   if (stream) {
     c->st = (hipStream_t)stream;
@@ -398,6 +399,12 @@ int ostc_fr_farfield_state(const ostc_mech_tables* tables, const ostc_farfield_s
   f.specific_heat_ratio = gamma;
   f.mw = mw;
   *out = f;
+  return OSTC_OK;
+}
+
+int ostc_fr_ngg(ostc_fr_ctx c, int* ngg) {
+  if (!c || !ngg) return OSTC_ERR_ARG;
+  *ngg = c->ngg;
   return OSTC_OK;
 }
 

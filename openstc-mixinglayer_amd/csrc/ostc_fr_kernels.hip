@@ -66,6 +66,64 @@ __device__ __forceinline__ double weno5z(const double* vp, const double* vm, dou
   return fp + fm;
 }
 
+// WENO-Z7, src/WENO.cu:1619-1693.  vp = F+ at i-3..i+3, vm = F- at i-2..i+4.
+__device__ __forceinline__ double weno7z(const double* vp, const double* vm, double eps) {
+  constexpr double one6th{1.0 / 6}, d12{13.0 / 12.0}, d13{1043.0 / 960}, d14{1.0 / 12};
+  constexpr double c0{1.0 / 35}, c1{12.0 / 35}, c2{18.0 / 35}, c3{4.0 / 35}, one12th{1.0 / 12};
+  double s1 = one6th * (-2 * vp[0] + 9 * vp[1] - 18 * vp[2] + 11 * vp[3]);
+  double s2 = -vp[0] + 4 * vp[1] - 5 * vp[2] + 2 * vp[3];
+  double s3 = -vp[0] + 3 * vp[1] - 3 * vp[2] + vp[3];
+  double b0 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = one6th * (vp[1] - 6 * vp[2] + 3 * vp[3] + 2 * vp[4]);
+  s2 = vp[2] - 2 * vp[3] + vp[4];
+  s3 = -vp[1] + 3 * vp[2] - 3 * vp[3] + vp[4];
+  double b1 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = one6th * (-2 * vp[2] - 3 * vp[3] + 6 * vp[4] - vp[5]);
+  s3 = -vp[2] + 3 * vp[3] - 3 * vp[4] + vp[5];
+  double b2 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = one6th * (-11 * vp[3] + 18 * vp[4] - 9 * vp[5] + 2 * vp[6]);
+  s2 = 2 * vp[3] - 5 * vp[4] + 4 * vp[5] - vp[6];
+  s3 = -vp[3] + 3 * vp[4] - 3 * vp[5] + vp[6];
+  double b3 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  double tau = (b0 - b3) * (b0 - b3);
+  double a0 = c0 + c0 * tau / ((eps + b0) * (eps + b0));
+  double a1 = c1 + c1 * tau / ((eps + b1) * (eps + b1));
+  double a2 = c2 + c2 * tau / ((eps + b2) * (eps + b2));
+  double a3 = c3 + c3 * tau / ((eps + b3) * (eps + b3));
+  double v0 = one12th * (-3 * vp[0] + 13 * vp[1] - 23 * vp[2] + 25 * vp[3]);
+  double v1 = one12th * (vp[1] - 5 * vp[2] + 13 * vp[3] + 3 * vp[4]);
+  double v2 = one12th * (-vp[2] + 7 * vp[3] + 7 * vp[4] - vp[5]);
+  double v3 = one12th * (3 * vp[3] + 13 * vp[4] - 5 * vp[5] + vp[6]);
+  const double fp = (a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3);
+
+  s1 = one6th * (-2 * vm[6] + 9 * vm[5] - 18 * vm[4] + 11 * vm[3]);
+  s2 = -vm[6] + 4 * vm[5] - 5 * vm[4] + 2 * vm[3];
+  s3 = -vm[6] + 3 * vm[5] - 3 * vm[4] + vm[3];
+  b0 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = one6th * (vm[5] - 6 * vm[4] + 3 * vm[3] + 2 * vm[2]);
+  s2 = vm[4] - 2 * vm[3] + vm[2];
+  s3 = -vm[5] + 3 * vm[4] - 3 * vm[3] + vm[2];
+  b1 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = one6th * (-2 * vm[4] - 3 * vm[3] + 6 * vm[2] - vm[1]);
+  s3 = -vm[4] + 3 * vm[3] - 3 * vm[2] + vm[1];
+  b2 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = one6th * (-11 * vm[3] + 18 * vm[2] - 9 * vm[1] + 2 * vm[0]);
+  s2 = 2 * vm[3] - 5 * vm[2] + 4 * vm[1] - vm[0];
+  s3 = -vm[3] + 3 * vm[2] - 3 * vm[1] + vm[0];
+  b3 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  tau = (b0 - b3) * (b0 - b3);
+  a0 = c0 + c0 * tau / ((eps + b0) * (eps + b0));
+  a1 = c1 + c1 * tau / ((eps + b1) * (eps + b1));
+  a2 = c2 + c2 * tau / ((eps + b2) * (eps + b2));
+  a3 = c3 + c3 * tau / ((eps + b3) * (eps + b3));
+  v0 = one12th * (-3 * vm[6] + 13 * vm[5] - 23 * vm[4] + 25 * vm[3]);
+  v1 = one12th * (vm[5] - 5 * vm[4] + 13 * vm[3] + 3 * vm[2]);
+  v2 = one12th * (-vm[4] + 7 * vm[3] + 7 * vm[2] - vm[1]);
+  v3 = one12th * (3 * vm[3] + 13 * vm[2] - 5 * vm[1] + vm[0]);
+  const double fm = (a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3);
+  return fp + fm;
+}
+
 // cv of one point from bv and sv (compute_cv_from_bv_1_point<FR>,
 // src/FieldOperation.cuh:78-105)
 template <int NS>
@@ -473,7 +531,9 @@ __device__ __forceinline__ void pp_limit(double* fc, const double* f1, const dou
 // WENO-Z5 flux, src/WENO.cu:1455-1536 and :722-1143 (method 1).
 template <int DIR, int NS, int SCHEME>
 __global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
-  constexpr int NV = 5 + NS, G = 3;
+  // WENO-Z5 (51, 52): 3 ghost layers, 5-point candidates; WENO-Z7 (71, 72): 4 and 7
+  constexpr int G = (SCHEME == 71 || SCHEME == 72) ? 4 : 3, W = 2 * G - 1;
+  constexpr int NV = 5 + NS;
   constexpr int TW = Tile<DIR>::TW, F = Tile<DIR>::F, TS = F - 1, P = TS + 2 * G, NP = P * TW;
   extern __shared__ double lds[];
   double* Fp = lds;            // [NV][NP]
@@ -560,8 +620,9 @@ __global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
     }
     const double jac1 = B.jac[pl], jac2 = B.jac[pr];
     double f1[NS];
-    if (SCHEME == 52) {
-      // compute_weno_flux_ch<FR> with "method 1" flux splitting.  The five
+    if (SCHEME == 52 || SCHEME == 72) {
+      // compute_weno_flux_ch<FR> with "method 1" flux splitting (72: 7 points
+      // and WENO-Z7, src/WENO.cu:898-965).  The five
       // acoustic/entropy fields run as a rolled loop and their back
       // projection (src/WENO.cu:1121-1142) is accumulated field by field in
       // the reference's term order, which keeps the arithmetic identical and
@@ -599,27 +660,29 @@ __global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
       for (int l = 0; l < 5; ++l) {
         double L[5];
         const double cas = char_row<NS>(C, l, L);
-        double vPlus[5], vMinus[5];
+        double vPlus[W], vMinus[W];
 #pragma unroll 1
-        for (int r = 0; r < 5; ++r) {
+        for (int r = 0; r < W; ++r) {
           double ap = 0, am = 0;
 #pragma unroll
           for (int n = 0; n < 5; ++n) {
-            ap += L[n] * Fp[n * NP + sl - 2 + r];
-            am += L[n] * Fm[n * NP + sl - 1 + r];
+            ap += L[n] * Fp[n * NP + sl - W / 2 + r];
+            am += L[n] * Fm[n * NP + sl - W / 2 + 1 + r];
           }
 #pragma unroll
           for (int n = 0; n < NS; ++n) {
-            ap += cas * C.alpha_l[n] * Fp[(5 + n) * NP + sl - 2 + r];
-            am += cas * C.alpha_l[n] * Fm[(5 + n) * NP + sl - 1 + r];
+            ap += cas * C.alpha_l[n] * Fp[(5 + n) * NP + sl - W / 2 + r];
+            am += cas * C.alpha_l[n] * Fm[(5 + n) * NP + sl - W / 2 + 1 + r];
           }
 #pragma unroll
-          for (int q = 0; q < 5; ++q) {
+          for (int q = 0; q < W; ++q) {
             vPlus[q] = (q == r) ? ap : vPlus[q];
             vMinus[q] = (q == r) ? am : vMinus[q];
           }
         }
-        const double fch = weno5z(vPlus, vMinus, eps_scaled);
+        double fch;
+        if constexpr (W == 7) fch = weno7z(vPlus, vMinus, eps_scaled);
+        else fch = weno5z(vPlus, vMinus, eps_scaled);
         // column l of the back projection
         double r0, r1, r2, r3, r4;
         if (l == 0) {
@@ -653,13 +716,15 @@ __global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
       double add{0};
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
-        double vPlus[5], vMinus[5];
+        double vPlus[W], vMinus[W];
 #pragma unroll
-        for (int r = 0; r < 5; ++r) {
-          vPlus[r] = -C.svm[l] * Fp[sl - 2 + r] + Fp[(5 + l) * NP + sl - 2 + r];
-          vMinus[r] = -C.svm[l] * Fm[sl - 1 + r] + Fm[(5 + l) * NP + sl - 1 + r];
+        for (int r = 0; r < W; ++r) {
+          vPlus[r] = -C.svm[l] * Fp[sl - W / 2 + r] + Fp[(5 + l) * NP + sl - W / 2 + r];
+          vMinus[r] = -C.svm[l] * Fm[sl - W / 2 + 1 + r] + Fm[(5 + l) * NP + sl - W / 2 + 1 + r];
         }
-        const double fch = weno5z(vPlus, vMinus, eps_scaled);
+        double fch;
+        if constexpr (W == 7) fch = weno7z(vPlus, vMinus, eps_scaled);
+        else fch = weno5z(vPlus, vMinus, eps_scaled);
         fc[5 + l] = C.svm[l] * f0 + fch;
         add += C.alpha_l[l] * fch;
       }
@@ -694,13 +759,14 @@ __global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
 #pragma unroll
       for (int l = 0; l < NV; ++l) {
         const double eh = (l == 1 || l == 2 || l == 3) ? eps_scaled[1] : (l == 4 ? eps_scaled[2] : eps_scaled[0]);
-        double vp[5], vm[5];
+        double vp[W], vm[W];
 #pragma unroll
-        for (int r = 0; r < 5; ++r) {
-          vp[r] = Fp[l * NP + sl - 2 + r];
-          vm[r] = Fm[l * NP + sl - 1 + r];
+        for (int r = 0; r < W; ++r) {
+          vp[r] = Fp[l * NP + sl - W / 2 + r];
+          vm[r] = Fm[l * NP + sl - W / 2 + 1 + r];
         }
-        fc[l] = weno5z(vp, vm, eh);
+        if constexpr (W == 7) fc[l] = weno7z(vp, vm, eh);  // compute_weno_flux_cp, 71 (src/WENO.cu:1516-1534)
+        else fc[l] = weno5z(vp, vm, eh);
       }
       if (A.pp) {
         double ql[NS], qr[NS];
@@ -984,7 +1050,8 @@ void launch_rhs_t(const FrArgs& A, hipStream_t st, long long* n, KernelTimer* tm
   ++*n;
   auto inv = [&](auto dirc) {
     constexpr int D = decltype(dirc)::value;
-    constexpr int TW = Tile<D>::TW, F = Tile<D>::F, TS = F - 1, P = TS + 6;
+    constexpr int TW = Tile<D>::TW, F = Tile<D>::F, TS = F - 1;
+    const int P = TS + 2 * (A.scheme == 71 || A.scheme == 72 ? 4 : 3);
     const int across = D == 0 ? 1 : 0, third = D == 2 ? 1 : 2;
     dim3 grid((ext[D] + TS - 1) / TS, (ext[across] + TW - 1) / TW, ext[third]);
     const size_t lds = (size_t)2 * (5 + NS) * P * TW * sizeof(double);
@@ -993,6 +1060,10 @@ void launch_rhs_t(const FrArgs& A, hipStream_t st, long long* n, KernelTimer* tm
       hipLaunchKernelGGL((k_inviscid<D, NS, 52>), grid, dim3(kT), lds, st, A);
     else if (A.scheme == 51)
       hipLaunchKernelGGL((k_inviscid<D, NS, 51>), grid, dim3(kT), lds, st, A);
+    else if (A.scheme == 72)
+      hipLaunchKernelGGL((k_inviscid<D, NS, 72>), grid, dim3(kT), lds, st, A);
+    else if (A.scheme == 71)
+      hipLaunchKernelGGL((k_inviscid<D, NS, 71>), grid, dim3(kT), lds, st, A);
     else  // primitive reconstruction + Riemann solver (3, 4): TS + 4 points, or F faces of fluxes
       hipLaunchKernelGGL((k_inviscid_pv<D, NS>), grid, dim3(kT),
                          (size_t)(5 + NS) * std::max(TS + 4, F) * TW * sizeof(double), st, A);
@@ -1081,11 +1152,6 @@ void launch_comm(const FrArgs& A, const FrBlockDev& Tg, int ns, int axis, int si
   const long long cnt = (long long)ext[(axis + 1) % 3] * ext[(axis + 2) % 3] * (A.B.g + 1);
   OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_comm<NS>, dim3(nblk(cnt)), dim3(kT), 0, st, A, Tg, axis, side))
   ++*n;
-}
-
-size_t inviscid_lds_bytes(int ns, int dir) {
-  const int tw = dir == 0 ? Tile<0>::TW : Tile<1>::TW, f = dir == 0 ? Tile<0>::F : Tile<1>::F;
-  return (size_t)2 * (5 + ns) * (f - 1 + 6) * tw * sizeof(double);
 }
 
 }  // namespace fr

@@ -82,6 +82,7 @@ SIGNATURES = {
     "ostc_fr_add_boundary": (_I, [_P, _I, _I, _I, _I]),
     "ostc_fr_add_inner_face": (_I, [_P, _I, _I, _I, _I]),
     "ostc_fr_add_farfield": (_I, [_P, _I, _I, _I, C.POINTER(_pkg().Farfield)]),
+    "ostc_fr_ngg": (_I, [_P, C.POINTER(_I)]),
     "ostc_fr_farfield_state": (_I, [C.POINTER(MechTables), C.POINTER(_pkg().FarfieldSpec), _DP,
                                     C.POINTER(_pkg().Farfield)]),
     "ostc_fr_finalize": (_I, [_P]),
@@ -158,7 +159,9 @@ class FrSolver:
         if r != 0:
             raise pkg.OstcError(f"ostc_fr_create failed ({r})")
         self.ctx = ctx
-        self.ngg = 3
+        g = C.c_int()
+        self._chk(L.ostc_fr_ngg(self.ctx, C.byref(g)))
+        self.ngg = g.value
         for b in case["blocks"]:
             self._chk(L.ostc_fr_add_block(self.ctx, C.byref(_BlockDesc(b["mx"], b["my"], b["mz"])), None))
         for f in case["boundaries"]:

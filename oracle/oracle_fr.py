@@ -233,7 +233,8 @@ class FrOracle:
         (y >= 0 upper), src/Initialize.cu:259-325."""
         case = self.case
         st = case["streams"]
-        ao = OA.Oracle(dict(case, n_var=5, inviscid_scheme=51), n_threads=1)
+        # an air oracle with the same ghost depth provides grid and profile
+        ao = OA.Oracle(dict(case, n_var=5, inviscid_scheme=71 if self.ngg == 4 else 51), n_threads=1)
         streams = OA._Streams((C.c_double * 6)(*st["upper"]), (C.c_double * 6)(*st["lower"]))
         g = self.ngg
         for bi, b in enumerate(case["blocks"]):

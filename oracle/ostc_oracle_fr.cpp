@@ -349,6 +349,66 @@ double weno5z(const double* vp, const double* vm, double eps) {
   return fp + fm;
 }
 
+// WENO-Z7, src/WENO.cu:1619-1693
+double weno7z(const double* vp, const double* vm, double eps) {
+  constexpr double one6th{1.0 / 6};
+  constexpr double d12{13.0 / 12.0}, d13{1043.0 / 960}, d14{1.0 / 12};
+  double s1{one6th * (-2 * vp[0] + 9 * vp[1] - 18 * vp[2] + 11 * vp[3])};
+  double s2{-vp[0] + 4 * vp[1] - 5 * vp[2] + 2 * vp[3]};
+  double s3{-vp[0] + 3 * vp[1] - 3 * vp[2] + vp[3]};
+  double b0{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
+  s1 = one6th * (vp[1] - 6 * vp[2] + 3 * vp[3] + 2 * vp[4]);
+  s2 = vp[2] - 2 * vp[3] + vp[4];
+  s3 = -vp[1] + 3 * vp[2] - 3 * vp[3] + vp[4];
+  double b1{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
+  s1 = one6th * (-2 * vp[2] - 3 * vp[3] + 6 * vp[4] - vp[5]);
+  s3 = -vp[2] + 3 * vp[3] - 3 * vp[4] + vp[5];
+  double b2{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
+  s1 = one6th * (-11 * vp[3] + 18 * vp[4] - 9 * vp[5] + 2 * vp[6]);
+  s2 = 2 * vp[3] - 5 * vp[4] + 4 * vp[5] - vp[6];
+  s3 = -vp[3] + 3 * vp[4] - 3 * vp[5] + vp[6];
+  double b3{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
+  double tau{(b0 - b3) * (b0 - b3)};
+  constexpr double c0{1.0 / 35}, c1{12.0 / 35}, c2{18.0 / 35}, c3{4.0 / 35};
+  double a0{c0 + c0 * tau / ((eps + b0) * (eps + b0))};
+  double a1{c1 + c1 * tau / ((eps + b1) * (eps + b1))};
+  double a2{c2 + c2 * tau / ((eps + b2) * (eps + b2))};
+  double a3{c3 + c3 * tau / ((eps + b3) * (eps + b3))};
+  constexpr double one12th{1.0 / 12};
+  double v0{one12th * (-3 * vp[0] + 13 * vp[1] - 23 * vp[2] + 25 * vp[3])};
+  double v1{one12th * (vp[1] - 5 * vp[2] + 13 * vp[3] + 3 * vp[4])};
+  double v2{one12th * (-vp[2] + 7 * vp[3] + 7 * vp[4] - vp[5])};
+  double v3{one12th * (3 * vp[3] + 13 * vp[4] - 5 * vp[5] + vp[6])};
+  const double fp{(a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3)};
+
+  s1 = one6th * (-2 * vm[6] + 9 * vm[5] - 18 * vm[4] + 11 * vm[3]);
+  s2 = -vm[6] + 4 * vm[5] - 5 * vm[4] + 2 * vm[3];
+  s3 = -vm[6] + 3 * vm[5] - 3 * vm[4] + vm[3];
+  b0 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = one6th * (vm[5] - 6 * vm[4] + 3 * vm[3] + 2 * vm[2]);
+  s2 = vm[4] - 2 * vm[3] + vm[2];
+  s3 = -vm[5] + 3 * vm[4] - 3 * vm[3] + vm[2];
+  b1 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = one6th * (-2 * vm[4] - 3 * vm[3] + 6 * vm[2] - vm[1]);
+  s3 = -vm[4] + 3 * vm[3] - 3 * vm[2] + vm[1];
+  b2 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = one6th * (-11 * vm[3] + 18 * vm[2] - 9 * vm[1] + 2 * vm[0]);
+  s2 = 2 * vm[3] - 5 * vm[2] + 4 * vm[1] - vm[0];
+  s3 = -vm[3] + 3 * vm[2] - 3 * vm[1] + vm[0];
+  b3 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  tau = (b0 - b3) * (b0 - b3);
+  a0 = c0 + c0 * tau / ((eps + b0) * (eps + b0));
+  a1 = c1 + c1 * tau / ((eps + b1) * (eps + b1));
+  a2 = c2 + c2 * tau / ((eps + b2) * (eps + b2));
+  a3 = c3 + c3 * tau / ((eps + b3) * (eps + b3));
+  v0 = one12th * (-3 * vm[6] + 13 * vm[5] - 23 * vm[4] + 25 * vm[3]);
+  v1 = one12th * (vm[5] - 5 * vm[4] + 13 * vm[3] + 3 * vm[2]);
+  v2 = one12th * (-vm[4] + 7 * vm[3] + 7 * vm[2] - vm[1]);
+  v3 = one12th * (3 * vm[3] + 13 * vm[2] - 5 * vm[1] + vm[0]);
+  const double fm{(a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3)};
+  return fp + fm;
+}
+
 // compute_flux (LF split), src/WENO.cu:697-720; q = [cv(0..nv-1), p, c]
 void lf_split(const double* q, int nv, const double* m, double jac, double* fp, double* fm) {
   const double Uk{(q[1] * m[0] + q[2] * m[1] + q[3] * m[2]) / q[0]};
@@ -950,7 +1010,7 @@ void inviscid(const Ctx& c, Block& b, double dt) {
             const double jac1 = J[is], jac2 = J[is + 1];
             double* fci = &fc[(t + 1) * nv];
             double* f1i = &f1[(t + 1) * MS];
-            if (scheme == 52) {
+            if (scheme == 52 || scheme == 72) {
               const double* cvl = &Q[is * (nv + 2)];
               const double* cvr = &Q[(is + 1) * (nv + 2)];
               CharFace F;
@@ -967,6 +1027,34 @@ void inviscid(const Ctx& c, Block& b, double dt) {
                            0.5 * (Fp[(is + 1) * nv + l + 5] - srNext * cvr[l + 5] * jac2);
               }
               double fChar[5 + MS];
+              if (scheme == 72) {
+                // WENO-Z7 characteristic, src/WENO.cu:898-965: 7 points
+                // i-3..i+3 (F+) and i-2..i+4 (F-) for every field
+                for (int l = 0; l < 5; ++l) {
+                  double L[5];
+                  const double cas = char_row(F, l, L);
+                  double vPlus[7] = {}, vMinus[7] = {};
+                  for (int r = 0; r < 7; ++r) {
+                    for (int n = 0; n < 5; ++n) {
+                      vPlus[r] += L[n] * Fp[(is - 3 + r) * nv + n];
+                      vMinus[r] += L[n] * Fm[(is - 2 + r) * nv + n];
+                    }
+                    for (int n = 0; n < ns; ++n) {
+                      vPlus[r] += cas * F.alpha_l[n] * Fp[(is - 3 + r) * nv + 5 + n];
+                      vMinus[r] += cas * F.alpha_l[n] * Fm[(is - 2 + r) * nv + 5 + n];
+                    }
+                  }
+                  fChar[l] = weno7z(vPlus, vMinus, eps_scaled);
+                }
+                for (int l = 0; l < ns; ++l) {
+                  double vPlus[7], vMinus[7];
+                  for (int r = 0; r < 7; ++r) {
+                    vPlus[r] = -F.svm[l] * Fp[(is - 3 + r) * nv] + Fp[(is - 3 + r) * nv + 5 + l];
+                    vMinus[r] = -F.svm[l] * Fm[(is - 2 + r) * nv] + Fm[(is - 2 + r) * nv + 5 + l];
+                  }
+                  fChar[5 + l] = weno7z(vPlus, vMinus, eps_scaled);
+                }
+              } else {
               for (int l = 0; l < 5; ++l) {
                 double L[5];
                 const double cas = char_row(F, l, L);
@@ -991,6 +1079,7 @@ void inviscid(const Ctx& c, Block& b, double dt) {
                 }
                 fChar[5 + l] = weno5z(vPlus, vMinus, eps_scaled);
               }
+              }
               char_back(F, ns, fChar, fci);
             } else {
               constexpr double eps{1e-20};
@@ -1008,6 +1097,16 @@ void inviscid(const Ctx& c, Block& b, double dt) {
                 double eh{eps_scaled[0]};
                 if (l == 1 || l == 2 || l == 3) eh = eps_scaled[1];
                 else if (l == 4) eh = eps_scaled[2];
+                if (scheme == 71) {
+                  // WENO-Z7 component-wise, src/WENO.cu:1516-1534
+                  double vp[7], vm[7];
+                  for (int r = 0; r < 7; ++r) {
+                    vp[r] = Fp[(is - 3 + r) * nv + l];
+                    vm[r] = Fm[(is - 2 + r) * nv + l];
+                  }
+                  fci[l] = weno7z(vp, vm, eh);
+                  continue;
+                }
                 double vp[5], vm[5];
                 for (int r = 0; r < 5; ++r) {
                   vp[r] = Fp[(is - 2 + r) * nv + l];
@@ -1429,14 +1528,14 @@ extern "C" {
 
 void* orf_create(const orf_setup* s, const ostc_mech_tables* t) {
   if (!s || !t) return nullptr;
-  if (s->inviscid_scheme != 51 && s->inviscid_scheme != 52 && s->inviscid_scheme != 3 && s->inviscid_scheme != 4)
-    return nullptr;
+  const int sc = s->inviscid_scheme;
+  if (sc != 51 && sc != 52 && sc != 71 && sc != 72 && sc != 3 && sc != 4) return nullptr;
   Ctx* c = new Ctx();
   c->s = *s;
   c->T = *t;
   c->ns = t->n_spec;
   c->nv = 5 + c->ns;
-  c->ngg = 3;
+  c->ngg = (s->inviscid_scheme == 71 || s->inviscid_scheme == 72) ? 4 : 3;  // src/Parameter.cpp:227-238
   c->weno_eps_scale = (s->rho_ref * s->v_ref) * (s->rho_ref * s->v_ref);
 #ifdef _OPENMP
   if (s->n_threads > 0) omp_set_num_threads(s->n_threads);

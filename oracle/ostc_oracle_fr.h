@@ -25,7 +25,7 @@ extern "C" {
 #endif
 
 typedef struct {
-  int    inviscid_scheme;      /* 51 WENO-Z5 component-wise, 52 WENO-Z5 characteristic,
+  int    inviscid_scheme;      /* 51 / 71 WENO-Z5 / WENO-Z7 component-wise, 52 / 72 characteristic,
                                   3 AUSM+, 4 HLLC on reconstructed primitives */
   int    viscous_order;        /* 0 or 2 */
   int    reaction;             /* 1: finite-rate chemistry source */

@@ -83,7 +83,7 @@ def test_stage_residual(tables, scheme, viscous, reaction, pp):
 
 
 def _compare_state(o, s, nblocks, tol, label):
-    g = 3
+    g = s.ngg
     worst = 0.0
     for b in range(nblocks):
         for fo, fg in ((OF.BV, MX.FIELD_BV), (OF.SV, MX.FIELD_SV)):
@@ -144,7 +144,7 @@ def _mixed_state(case, tables, seed, radicals=1.0, t_mean=1400.0):
     explicit chemistry cannot step; 1e-5: traces that it can)."""
     rng = np.random.default_rng(seed)
     b = case["blocks"][0]
-    g = 3
+    g = 4 if case["inviscid_scheme"] in (71, 72) else 3
     shp = (b["mz"] + 2 * g, b["my"] + 2 * g, b["mx"] + 2 * g)
     k, j, i = np.meshgrid(*(np.arange(n) for n in shp), indexing="ij")
     ph = rng.uniform(0, 2 * np.pi, (16, 3))
@@ -243,4 +243,38 @@ def test_steps_primitive_reconstruction(tables, scheme):
     o.step(10)
     s.rk3(10, MX.MODE_GRAPH)
     _compare_state(o, s, 1, 1e-9, "10 steps")
+    s.close()
+
+
+@pytest.mark.parametrize("scheme", [71, 72], ids=["wenoz7-cp", "wenoz7-char"])
+@pytest.mark.parametrize("reaction", [0, 1])
+def test_stage_residual_weno7(tables, scheme, reaction):
+    """WENO-Z7 component-wise (71) and characteristic (72) on 4 ghost layers."""
+    print(f"[gpu] fr dq scheme {scheme} reaction {reaction}", flush=True)
+    case = _case(tables, scheme=scheme, reaction=reaction, positive_preserving=1)
+    for label, state in (("layer", None), ("premixed", _mixed_state(case, tables, 41 + scheme))):
+        o, s = _pair(case, tables) if state is None else _pair_state(case, tables, *state)
+        assert s.ngg == 4 and o.ngg == 4
+        dq_o = o.rhs(0, 2e-9).copy()
+        s.compute_rhs(0, 2e-9)
+        err = rel_err(s.get(0, MX.FIELD_DQ), dq_o)
+        print(f"  {label} dq per-variable:", " ".join(f"{e:.1e}" for e in err), flush=True)
+        assert np.all(err[:5] <= 1e-12)
+        assert np.all(err[5:] <= (1e-10 if reaction else 1e-12))
+        s.close()
+
+
+@pytest.mark.parametrize("scheme", [71, 72], ids=["wenoz7-cp", "wenoz7-char"])
+def test_steps_weno7(tables, scheme):
+    print(f"[gpu] fr 10 steps scheme {scheme}, then 5 steps on a 2 x 2 x 2 partition", flush=True)
+    o, s = _pair(_case(tables, scheme=scheme, positive_preserving=1), tables)
+    o.step(10)
+    s.rk3(10, MX.MODE_GRAPH)
+    _compare_state(o, s, 1, 1e-9, f"scheme {scheme}, 10 steps")
+    s.close()
+    case = pkg.cases.reacting_case([7, 6], [8, 9], [5, 6], tables[0], scheme=scheme)
+    o, s = _pair(case, tables)
+    o.step(5)
+    s.rk3(5, MX.MODE_GRAPH)
+    _compare_state(o, s, len(case["blocks"]), 1e-9, f"scheme {scheme}, 2x2x2, 5 steps")
     s.close()

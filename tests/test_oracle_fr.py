@@ -297,7 +297,8 @@ def _load_state(o, rng, uniform=False):
     else:
         # period n - 1 nodes on every axis (node n-1 repeats node 0, as the
         # periodic faces assume)
-        n, g = shp[0] - 6, 3
+        g = o.ngg
+        n = shp[0] - 2 * g
         per = np.arange(-g, n + g) % (n - 1)
         k, j, i = np.meshgrid(per, per, per, indexing="ij")
         ph = [rng.uniform(0, 2 * np.pi, 3) for _ in range(20)]
@@ -324,7 +325,7 @@ def _oracle_with(T, case, rng, uniform=False):
     return o
 
 
-@pytest.mark.parametrize("scheme", [51, 52])
+@pytest.mark.parametrize("scheme", [51, 52, 71, 72])
 def test_free_stream_is_preserved(T, scheme):
     o = _oracle_with(T, _periodic_case(T, scheme=scheme), np.random.default_rng(2), uniform=True)
     dq = o.rhs(0, 1e-9)
@@ -333,7 +334,7 @@ def test_free_stream_is_preserved(T, scheme):
         assert np.max(np.abs(dq[l])) <= 1e-12 * np.max(np.abs(cv[l])) * 2.0, l
 
 
-@pytest.mark.parametrize("scheme", [51, 52])
+@pytest.mark.parametrize("scheme", [51, 52, 71, 72])
 def test_periodic_residual_telescopes(T, scheme):
     """Fully periodic box, chemistry off: dq summed over the distinct nodes
     (node n-1 repeats node 0) is zero for every conserved variable -- the
@@ -457,3 +458,50 @@ def test_pv_path_preserves_free_stream_and_conserves(T, scheme, recon):
     dq = o.rhs(0, 1e-9)[:, :-1, :-1, :-1]
     for l in range(14):
         assert abs(dq[l].sum()) <= 1e-12 * np.abs(dq[l]).sum(), l
+
+
+def _wave_dq(T, scheme, n):
+    """Mixture residual of a small-amplitude plane wave along (1, 2, -1) in
+    index space, n - 1 nodes per period, viscous fluxes and chemistry off."""
+    o = OF.FrOracle(_periodic_case(T, n=n, scheme=scheme, viscous=0), T, n_threads=4)
+    jac, met = o.array(0, OF.JAC), o.array(0, OF.METRIC)
+    jac[...] = 0.125
+    met[...] = 0.0
+    met[0], met[4], met[8] = 2.0, 2.0, 2.0
+    g = o.ngg
+    per = np.arange(-g, n + g) % (n - 1)
+    k, j, i = np.meshgrid(per, per, per, indexing="ij")
+    w = 2 * np.pi * (i + 2 * j - k) / (n - 1)
+    y = np.array([0.01, 0.02, 0.05, 0.001, 0.002, 0.01, 0.2, 0.02, 0.687])
+    sv = o.array(0, OF.SV)
+    bv = o.array(0, OF.BV)
+    for l in range(9):
+        sv[l] = y[l] * (1 + 1e-2 * np.sin(w + 0.1 * l))
+    sv[...] /= sv.sum(axis=0)
+    bv[0] = 0.5 * (1 + 1e-2 * np.sin(w))
+    bv[1], bv[2], bv[3] = 40 + 0.5 * np.cos(w), -20 + 0.3 * np.sin(w + 1), 10 + 0 * w
+    bv[5] = 1300 * (1 + 1e-2 * np.sin(w + 0.3))
+    mw_inv = np.tensordot(1 / np.array(T.mw[:9]), sv, axes=1)
+    bv[4] = bv[0] * R_U * mw_inv * bv[5]
+    o.L.orf_cv_from_bv(o.ctx, 0)
+    o.L.orf_properties(o.ctx, 0)
+    return o.rhs(0, 1e-9).copy()
+
+
+def test_mixture_characteristic_and_componentwise_weno_converge_together(T):
+    """Smooth limit: the characteristic residuals (52, 72) approach the
+    component-wise ones (51, 71) at the schemes' order, for the flow rows and
+    the species rows; a stencil shifted by one point stays at O(h k)."""
+    err = {}
+    for n in (20, 30):
+        d = {s: _wave_dq(T, s, n) for s in (51, 52, 71, 72)}
+        for l in range(14):
+            scale = np.max(np.abs(d[71][l]))
+            e72 = np.max(np.abs(d[72][l] - d[71][l])) / scale
+            e52 = np.max(np.abs(d[52][l] - d[51][l])) / scale
+            err[(n, l)] = (e72, e52)
+            assert e72 <= 1e-2 and e52 <= 3e-2, (n, l, e72, e52)
+            assert np.max(np.abs(np.roll(d[72][l], 1, axis=1) - d[71][l])) / scale > 0.2
+    for l in range(14):
+        assert err[(30, l)][0] <= 0.1 * err[(20, l)][0], (l, err[(20, l)], err[(30, l)])
+        assert err[(30, l)][1] <= 0.3 * err[(20, l)][1], (l, err[(20, l)], err[(30, l)])
