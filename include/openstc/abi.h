@@ -353,6 +353,10 @@ int ostc_fr_add_boundary(ostc_fr_ctx ctx, int block, int axis, int side, int typ
 int ostc_fr_add_inner_face(ostc_fr_ctx ctx, int block, int axis, int side, int target_block);
 /* ghost layers of every array of the context: 3, or 4 for the WENO-Z7 schemes 71/72 (src/Parameter.cpp:227-238) */
 int ostc_fr_ngg(ostc_fr_ctx ctx, int* ngg);
+/* store_last_step / compute_residual (src/TimeAdvanceFunc.cu:6-37, src/Residual.cuh:12-187): unscaled
+ * RMS of the change of rho, |V|, p and T over every block's interior since the last store. */
+int ostc_fr_store_last_step(ostc_fr_ctx ctx);
+int ostc_fr_residual(ostc_fr_ctx ctx, double res[4]);
 /* Farfield face (label 4) of a mixture, apply_farfield<FR> (src/BoundCond.cuh:755-1022): the subsonic
  * state iterates gamma with cp of the boundary temperature and composition. */
 int ostc_fr_add_farfield(ostc_fr_ctx ctx, int block, int axis, int side, const ostc_farfield* ff);

@@ -28,6 +28,7 @@ struct HBlock {
   std::vector<InnerDesc> inner;
 };
 
+constexpr int kResidualBlocks = 256;                  // workgroups of the residual reduction
 constexpr double kRkA[3] = {1.0, 0.75, 1.0 / 3.0};  // src/RK.cuh:10-14
 constexpr double kRkB[3] = {0.0, 0.25, 2.0 / 3.0};
 constexpr double kRkC[3] = {1.0, 0.25, 2.0 / 3.0};
@@ -46,6 +47,7 @@ struct ostc_fr_ctx_s {
   double* d_ff = nullptr;
   bool finalized = false;
   double* d_dt = nullptr;          // dt of the current step (one double)
+  double* d_res = nullptr;         // per-workgroup partial sums of the residual
   double weno_eps_scale = 1;
   std::vector<void*> allocs;
   std::string err;
@@ -449,10 +451,12 @@ int ostc_fr_finalize(ostc_fr_ctx c) {
         (r = dalloc(c, &d.sv, sz * ns)) || (r = dalloc(c, &d.dq, n0 * nv)) || (r = dalloc(c, &d.mul, sz)) ||
         (r = dalloc(c, &d.lam, sz)) || (r = dalloc(c, &d.cp, sz)) || (r = dalloc(c, &d.gam, sz)) ||
         (r = dalloc(c, &d.cc, sz)) || (r = dalloc(c, &d.mach, sz)) || (r = dalloc(c, &d.rhod, sz * ns)) ||
-        (r = dalloc(c, &d.jac, sz)) || (r = dalloc(c, &d.met, sz * 9)) || (r = dalloc(c, &d.dtl, n0)))
+        (r = dalloc(c, &d.jac, sz)) || (r = dalloc(c, &d.met, sz * 9)) || (r = dalloc(c, &d.dtl, n0)) ||
+        (r = dalloc(c, &d.bvl, n0 * 4)))
       return r;
   }
   if (int r = dalloc(c, &c->d_dt, 1)) return r;
+  if (int r = dalloc(c, &c->d_res, 4 * kResidualBlocks)) return r;
   if (!c->farfields.empty()) {
     static_assert(sizeof(ostc_farfield) == 26 * sizeof(double), "far-field layout");
     if (int r = dalloc(c, &c->d_ff, c->farfields.size() * 26)) return r;
@@ -619,6 +623,33 @@ int ostc_fr_update_properties(ostc_fr_ctx c, int b) {
   launch_props(args(c, b), c->ns, c->st, &n, nullptr);
   FR_HIP(c, hipGetLastError());
   FR_HIP(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_fr_store_last_step(ostc_fr_ctx c) {
+  if (int r = check_ready(c)) return r;
+  for (auto& B : c->blocks) launch_store_last(B.d, c->st);
+  FR_HIP(c, hipGetLastError());
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_fr_residual(ostc_fr_ctx c, double res[4]) {
+  if (int r = check_ready(c)) return r;
+  if (!res) return OSTC_ERR_ARG;
+  double acc[4] = {0, 0, 0, 0};
+  long long n_total = 0;
+  std::vector<double> h(4 * kResidualBlocks);
+  for (auto& B : c->blocks) {
+    launch_residual(B.d, c->d_res, kResidualBlocks, c->st);
+    FR_HIP(c, hipMemcpyAsync(h.data(), c->d_res, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    FR_HIP(c, hipStreamSynchronize(c->st));
+    for (int q = 0; q < kResidualBlocks; ++q)
+      for (int l = 0; l < 4; ++l) acc[l] += h[4 * q + l];
+    n_total += B.d.n0;
+  }
+  for (int l = 0; l < 4; ++l) res[l] = std::sqrt(acc[l] / (double)n_total);
+  if (std::isnan(res[0] + res[1] + res[2] + res[3])) return fail(c, OSTC_ERR_NUMERIC, "NaN in the residual");
   return OSTC_OK;
 }
 

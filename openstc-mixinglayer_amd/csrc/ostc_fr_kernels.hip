@@ -1154,5 +1154,55 @@ void launch_comm(const FrArgs& A, const FrBlockDev& Tg, int ns, int axis, int si
   ++*n;
 }
 
+// store_last_step and compute_square_of_dbv + reduction_of_dv_squared<4>
+// (src/TimeAdvanceFunc.cu:6-37, src/Residual.cuh:12-69) of the interior.
+__global__ __launch_bounds__(256) void k_store_last(FrBlockDev B) {
+  const long long t = blockIdx.x * 256ll + threadIdx.x;
+  if (t >= B.n0) return;
+  const int i = (int)(t % B.mx), j = (int)((t / B.mx) % B.my), k = (int)(t / ((long long)B.mx * B.my));
+  const long long p = pid(B, i, j, k), s = B.sz;
+  const double u = B.bv[p + s], v = B.bv[p + 2 * s], w = B.bv[p + 3 * s];
+  B.bvl[t] = B.bv[p];
+  B.bvl[t + B.n0] = sqrt(u * u + v * v + w * w);
+  B.bvl[t + 2 * B.n0] = B.bv[p + 4 * s];
+  B.bvl[t + 3 * B.n0] = B.bv[p + 5 * s];
+}
+
+__global__ __launch_bounds__(256) void k_residual(FrBlockDev B, double* out4) {
+  __shared__ double part[4][4];
+  double acc[4] = {0, 0, 0, 0};
+  const long long s = B.sz;
+  for (long long t = blockIdx.x * 256ll + threadIdx.x; t < B.n0; t += 256ll * gridDim.x) {
+    const int i = (int)(t % B.mx), j = (int)((t / B.mx) % B.my), k = (int)(t / ((long long)B.mx * B.my));
+    const long long p = pid(B, i, j, k);
+    const double u = B.bv[p + s], v = B.bv[p + 2 * s], w = B.bv[p + 3 * s];
+    const double vel = sqrt(u * u + v * v + w * w);
+    const double d0 = B.bv[p] - B.bvl[t], d1 = vel - B.bvl[t + B.n0];
+    const double d2 = B.bv[p + 4 * s] - B.bvl[t + 2 * B.n0], d3 = B.bv[p + 5 * s] - B.bvl[t + 3 * B.n0];
+    acc[0] += d0 * d0;
+    acc[1] += d1 * d1;
+    acc[2] += d2 * d2;
+    acc[3] += d3 * d3;
+  }
+  for (int l = 0; l < 4; ++l) {
+    double v = acc[l];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) part[l][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int l = threadIdx.x;
+    out4[blockIdx.x * 4 + l] = part[l][0] + part[l][1] + part[l][2] + part[l][3];
+  }
+}
+
+void launch_store_last(const FrBlockDev& B, hipStream_t st) {
+  hipLaunchKernelGGL(k_store_last, dim3((unsigned)((B.n0 + 255) / 256)), dim3(256), 0, st, B);
+}
+
+void launch_residual(const FrBlockDev& B, double* out4, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(k_residual, dim3(nblk), dim3(256), 0, st, B, out4);
+}
+
 }  // namespace fr
 }  // namespace ostc

@@ -83,6 +83,8 @@ SIGNATURES = {
     "ostc_fr_add_inner_face": (_I, [_P, _I, _I, _I, _I]),
     "ostc_fr_add_farfield": (_I, [_P, _I, _I, _I, C.POINTER(_pkg().Farfield)]),
     "ostc_fr_ngg": (_I, [_P, C.POINTER(_I)]),
+    "ostc_fr_store_last_step": (_I, [_P]),
+    "ostc_fr_residual": (_I, [_P, _DP]),
     "ostc_fr_farfield_state": (_I, [C.POINTER(MechTables), C.POINTER(_pkg().FarfieldSpec), _DP,
                                     C.POINTER(_pkg().Farfield)]),
     "ostc_fr_finalize": (_I, [_P]),
@@ -269,6 +271,17 @@ class FrSolver:
 
     def update_physical_properties(self, b: int):
         self._chk(self.L.ostc_fr_update_properties(self.ctx, b))
+
+    def store_last_step(self):
+        """store_last_step, src/TimeAdvanceFunc.cu:6-21."""
+        self._chk(self.L.ostc_fr_store_last_step(self.ctx))
+
+    def compute_residual(self) -> np.ndarray:
+        """Unscaled RMS change of rho, |V|, p, T since store_last_step
+        (src/Residual.cuh:12-187)."""
+        r = np.zeros(4)
+        self._chk(self.L.ostc_fr_residual(self.ctx, _dp(r)))
+        return r
 
     def last_step_timing(self):
         ms, n = C.c_double(), C.c_longlong()

@@ -72,6 +72,8 @@ def lib():
         L.orf_reconstruct.argtypes = [dp, i, i, dp, dp]
         L.orf_farfield_state.argtypes = [tp, C.POINTER(OA.FarfieldSpec), dp, C.POINTER(OA.Farfield)]
         L.orf_add_farfield.argtypes = [vp, i, i, i, C.POINTER(OA.Farfield)]
+        L.orf_store_last_step.argtypes = [vp]
+        L.orf_residual.argtypes = [vp, dp]
         _lib = L
     return _lib
 
@@ -256,6 +258,14 @@ class FrOracle:
 
     def initialize(self):
         self.L.orf_initialize(self.ctx)
+
+    def store_last_step(self):
+        self.L.orf_store_last_step(self.ctx)
+
+    def residual(self):
+        r = np.zeros(4)
+        self.L.orf_residual(self.ctx, _dp(r))
+        return r
 
     def step(self, n=1):
         dt = C.c_double(0)

@@ -571,7 +571,7 @@ struct InnerFace {
 struct Block {
   int mx, my, mz, g;
   long long sx, sy, sz;
-  std::vector<double> cv, bv, sv, qn, dq, mul, lam, cp, gam, cc, mach, rhod, jac, metric, dtl;
+  std::vector<double> cv, bv, sv, qn, dq, mul, lam, cp, gam, cc, mach, rhod, jac, metric, dtl, bv_last;
   std::vector<Face> faces;
   std::vector<InnerFace> inner;
   long long id(int i, int j, int k) const { return (k + g) * sx * sy + (j + g) * sx + (i + g); }
@@ -1561,6 +1561,7 @@ int orf_add_block(void* ctx, int mx, int my, int mz) {
   b.bv.assign(sz * 6, 0);
   b.sv.assign(sz * c->ns, 0);
   b.dq.assign(n0 * c->nv, 0);
+  b.bv_last.assign(n0 * 4, 0);
   b.mul.assign(sz, 0);
   b.lam.assign(sz, 0);
   b.cp.assign(sz, 0);
@@ -1689,6 +1690,46 @@ double* orf_array(void* ctx, int block, int which, long long* n) {
   }
   if (n) *n = (long long)v->size();
   return v->data();
+}
+
+// store_last_step and compute_residual (src/TimeAdvanceFunc.cu:6-37,
+// src/Residual.cuh:12-187): RMS over every block's interior of the change of
+// rho, |V|, p and T since the last store.
+int orf_store_last_step(void* ctx) {
+  for (auto& b : C(ctx)->blocks) {
+    const long long s = b.sz, n0 = b.n0();
+    for (int k = 0; k < b.mz; ++k)
+      for (int j = 0; j < b.my; ++j)
+        for (int i = 0; i < b.mx; ++i) {
+          const long long p = b.id(i, j, k), q = b.id0(i, j, k);
+          const double u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+          b.bv_last[q] = b.bv[p];
+          b.bv_last[q + n0] = std::sqrt(u * u + v * v + w * w);
+          b.bv_last[q + 2 * n0] = b.bv[p + 4 * s];
+          b.bv_last[q + 3 * n0] = b.bv[p + 5 * s];
+        }
+  }
+  return 0;
+}
+
+int orf_residual(void* ctx, double res[4]) {
+  double acc[4] = {0, 0, 0, 0};
+  long long n_total = 0;
+  for (auto& b : C(ctx)->blocks) {
+    const long long s = b.sz, n0 = b.n0();
+    n_total += n0;
+    for (int k = 0; k < b.mz; ++k)
+      for (int j = 0; j < b.my; ++j)
+        for (int i = 0; i < b.mx; ++i) {
+          const long long p = b.id(i, j, k), q = b.id0(i, j, k);
+          const double u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+          const double d[4] = {b.bv[p] - b.bv_last[q], std::sqrt(u * u + v * v + w * w) - b.bv_last[q + n0],
+                               b.bv[p + 4 * s] - b.bv_last[q + 2 * n0], b.bv[p + 5 * s] - b.bv_last[q + 3 * n0]};
+          for (int l = 0; l < 4; ++l) acc[l] += d[l] * d[l];
+        }
+  }
+  for (int l = 0; l < 4; ++l) res[l] = std::sqrt(acc[l] / (double)n_total);
+  return 0;
 }
 
 int orf_initialize(void* ctx) {

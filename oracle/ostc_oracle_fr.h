@@ -82,6 +82,9 @@ double  orf_global_dt(void* ctx);
 int     orf_update(void* ctx, int block, double dt, int rk);
 int     orf_apply_bc(void* ctx, int block);
 int     orf_communicate(void* ctx);
+/* store_last_step / compute_residual: RMS change of rho, |V|, p, T since the store */
+int     orf_store_last_step(void* ctx);
+int     orf_residual(void* ctx, double res[4]);
 
 /* point routines, for closed-form and identity tests */
 void    orf_enthalpy_cp(const ostc_mech_tables* t, double temp, double* h, double* cp);

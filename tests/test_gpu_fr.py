@@ -278,3 +278,25 @@ def test_steps_weno7(tables, scheme):
     s.rk3(5, MX.MODE_GRAPH)
     _compare_state(o, s, len(case["blocks"]), 1e-9, f"scheme {scheme}, 2x2x2, 5 steps")
     s.close()
+
+
+def test_residual_matches_oracle(tables):
+    """ostc_fr_store_last_step / ostc_fr_residual on identical states (the
+    initial layer, then the oracle's state two steps later): only the
+    summation order differs."""
+    print("[gpu] fr residual", flush=True)
+    case = pkg.cases.reacting_case([7, 6], [8, 9], [5, 6], tables[0], scheme=52)
+    o, s = _pair(case, tables)
+    nb = len(case["blocks"])
+    for b in range(nb):
+        s.set(b, MX.FIELD_BV, o.array(b, OF.BV))
+    s.store_last_step()
+    o.store_last_step()
+    o.step(2)
+    for b in range(nb):
+        s.set(b, MX.FIELD_BV, o.array(b, OF.BV))
+    got, want = s.compute_residual(), o.residual()
+    print(f"  residual gpu {got} oracle {want}", flush=True)
+    np.testing.assert_allclose(got, want, rtol=1e-12)
+    assert np.all(want > 0)
+    s.close()

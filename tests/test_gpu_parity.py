@@ -171,3 +171,24 @@ def test_single_block_64_cubed_100_steps(scheme, mode):
     assert abs(dt_g - dt_o) <= 1e-12 * dt_o
     assert np.all(e <= BV_TOL), e
     s.close()
+
+
+def test_residual_matches_oracle():
+    """ostc_store_last_step / ostc_residual (src/TimeAdvanceFunc.cu:6-37,
+    src/Residual.cuh:12-187) on identical states: only the summation order
+    differs."""
+    print("[gpu] residual of two oracle states", flush=True)
+    case = MULTI
+    nb = len(case["blocks"])
+    o = oracle_for(case)
+    s = _solver(case)
+    _load_oracle_state(s, o, nb)
+    s.store_last_step()
+    assert O.lib().orc_store_last_step(o.ctx) == 0
+    o.step(2)
+    _load_oracle_state(s, o, nb)
+    got, want = s.compute_residual(), o.residual()
+    print(f"  residual gpu {got} oracle {want}", flush=True)
+    np.testing.assert_allclose(got, want, rtol=1e-12)
+    assert np.all(want > 0)
+    s.close()

@@ -505,3 +505,26 @@ def test_mixture_characteristic_and_componentwise_weno_converge_together(T):
     for l in range(14):
         assert err[(30, l)][0] <= 0.1 * err[(20, l)][0], (l, err[(20, l)], err[(30, l)])
         assert err[(30, l)][1] <= 0.3 * err[(20, l)][1], (l, err[(20, l)], err[(30, l)])
+
+
+def test_mixture_residual_is_the_rms_of_the_primitive_changes(T):
+    """compute_residual with species: sqrt(sum(dv^2) / N) of rho, |V|, p, T
+    over the interior, against numpy, after two steps of the mixing layer."""
+    case = pkg.cases.reacting_case([7, 6], [8, 9], [5, 6], T, scheme=52)
+    o = OF.FrOracle(case, T, n_threads=4)
+    o.setup_mixing_layer()
+    o.initialize()
+    o.store_last_step()
+    g = o.ngg
+    before = [o.array(b, OF.BV)[:, g:-g, g:-g, g:-g].copy() for b in range(len(case["blocks"]))]
+    o.step(2)
+    acc, n = np.zeros(4), 0
+    for b, old in enumerate(before):
+        new = o.array(b, OF.BV)[:, g:-g, g:-g, g:-g]
+        vel = lambda q: np.sqrt(q[1] ** 2 + q[2] ** 2 + q[3] ** 2)
+        acc += [np.sum((new[0] - old[0]) ** 2), np.sum((vel(new) - vel(old)) ** 2),
+                np.sum((new[4] - old[4]) ** 2), np.sum((new[5] - old[5]) ** 2)]
+        n += new[0].size
+    want = np.sqrt(acc / n)
+    np.testing.assert_allclose(o.residual(), want, rtol=1e-12)
+    assert np.all(want > 0)

@@ -209,3 +209,68 @@ def test_oracle_matches_its_golden_fixture():
     dt = o.step(3)
     assert dt == ref["dt"]
     np.testing.assert_array_equal(o.array(0, O.BV), ref["bv_after3"])
+
+
+@pytest.mark.parametrize("scheme", [51, 52, 71, 3])
+def test_rhs_is_equivariant_under_axis_permutation(scheme):
+    """Swapping x and y in the state (u <-> v) swaps them in the residual
+    (rho u <-> rho v rows, i <-> j indices), inviscid and viscous fluxes
+    together, on a periodic box with the uniform metric.  The directions are
+    accumulated in a different order, so agreement is to round-off."""
+    n = 12
+    case = box_case(n, scheme=scheme)
+    case["reconstruction"] = 2
+
+    def residual(swap):
+        o = O.Oracle(case)
+        exact_grid(o)
+        g = o.ngg
+        idx = np.arange(-g, n + g)
+        ph = 2 * np.pi * (idx % (n - 1)) / (n - 1)
+        Z, Y, X = np.meshgrid(ph, ph, ph, indexing="ij")
+        if swap:
+            X, Y = Y, X
+        rho = 1.2 + 0.1 * np.sin(X + 2 * Y) * np.cos(Z)
+        u = 50 + 20 * np.cos(Y) + 5 * np.sin(X + Z)
+        v = 10 * np.sin(Z - X) + 3 * np.cos(Y)
+        w = -8 * np.cos(2 * Y + Z)
+        p = 1e5 + 3e3 * np.sin(X) * np.sin(2 * Y)
+        if swap:
+            u, v = v, u
+        set_state(o, rho, u, v, w, p)
+        L = O.lib()
+        L.orc_apply_bc(o.ctx, 0)
+        L.orc_properties(o.ctx, 0)
+        o.rhs(0)
+        return o.array(0, O.DQ).copy()
+
+    a, b = residual(False), residual(True)
+    b = b[[0, 2, 1, 3, 4]].transpose(0, 1, 3, 2)  # rows rho u <-> rho v, axes j <-> i
+    for l in range(5):
+        scale = np.max(np.abs(a[l]))
+        assert np.max(np.abs(a[l] - b[l])) <= 1e-12 * scale, (l, np.max(np.abs(a[l] - b[l])) / scale)
+
+
+def test_residual_is_the_rms_of_the_primitive_changes():
+    """compute_residual (src/Residual.cuh:12-187): sqrt(sum(dv^2) / N) over the
+    interior of every block for rho, |V|, p and T, against numpy."""
+    case = cases.small_multiblock_case()
+    o = O.Oracle(case)
+    o.setup_mixing_layer()
+    o.initialize()
+    L = O.lib()
+    assert L.orc_store_last_step(o.ctx) == 0
+    g = o.ngg
+    before = [o.array(b, O.BV)[:, g:-g, g:-g, g:-g].copy() for b in range(len(case["blocks"]))]
+    o.step(2)
+    acc, n = np.zeros(4), 0
+    for b, old in enumerate(before):
+        new = o.array(b, O.BV)[:, g:-g, g:-g, g:-g]
+        vel = lambda q: np.sqrt(q[1] ** 2 + q[2] ** 2 + q[3] ** 2)
+        acc += [np.sum((new[0] - old[0]) ** 2), np.sum((vel(new) - vel(old)) ** 2),
+                np.sum((new[4] - old[4]) ** 2), np.sum((new[5] - old[5]) ** 2)]
+        n += new[0].size
+    want = np.sqrt(acc / n)
+    got = o.residual()
+    np.testing.assert_allclose(got, want, rtol=1e-12)
+    assert np.all(want > 0)
