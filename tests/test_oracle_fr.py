@@ -528,3 +528,51 @@ def test_mixture_residual_is_the_rms_of_the_primitive_changes(T):
     want = np.sqrt(acc / n)
     np.testing.assert_allclose(o.residual(), want, rtol=1e-12)
     assert np.all(want > 0)
+
+
+# ------------------------------------------------------------ falloff rates
+def _copy(T):
+    return type(T).from_buffer_copy(T)
+
+
+def _third_body_conc(T, r, rho, y):
+    """[M] of reaction r in mol/cm^3 (c_k = rho Y_k / W_k * 1e-3)."""
+    return sum(T.third_body_coeff[r][k] * rho * y[k] / T.mw[k] * 1e-3 for k in range(T.n_spec))
+
+
+@pytest.mark.parametrize("r", [8, 14])
+def test_falloff_rates_reduce_to_their_limits(T, r):
+    """Lindemann: 1/kf = 1/(k0 [M]) + 1/k_inf, so two densities give k0 and
+    k_inf, which must be the low- and high-pressure Arrhenius laws of the
+    tables.  Troe with alpha = 1 (and the file's T3 = 1e-30, T1 = 1e30) has
+    F_cent = 1 and equals Lindemann; with the file's alpha, at the reduced
+    pressure where log10 Pr = -c = 0.4 + 0.67 log10 F_cent the broadening
+    factor is F_cent = alpha exactly."""
+    rng = np.random.default_rng(r)
+    y = _random_mixture(rng)
+    t = 1350.0
+    lind = _copy(T)
+    lind.label[r] = 5
+    kf = {}
+    for rho in (0.02, 0.5):
+        kf[rho] = OF.chemical_source(lind, rho, t, y)[1][r]
+    m1, m2 = _third_body_conc(T, r, 0.02, y), _third_body_conc(T, r, 0.5, y)
+    k0 = (1 / m1 - 1 / m2) / (1 / kf[0.02] - 1 / kf[0.5])
+    k_inf = 1 / (1 / kf[0.5] - 1 / (k0 * m2))
+    arr = lambda A, b, Ea: A * t ** b * math.exp(-Ea / t)
+    assert math.isclose(k0, arr(T.A2[r], T.b2[r], T.Ea2[r]), rel_tol=1e-9)
+    assert math.isclose(k_inf, arr(T.A[r], T.b[r], T.Ea[r]), rel_tol=1e-9)
+    # alpha = 1: Troe == Lindemann, bit for bit (F = 10**0)
+    troe1 = _copy(T)
+    troe1.troe_alpha[r] = 1.0
+    assert T.label[r] == 6
+    for rho in (0.02, 0.5):
+        assert OF.chemical_source(troe1, rho, t, y)[1][r] == kf[rho]
+    # the file's alpha at log10 Pr = -c
+    alpha = T.troe_alpha[r]
+    c = -0.4 - 0.67 * math.log10(alpha)
+    m_star = 10 ** (-c) * k_inf / k0
+    rho_star = 0.5 * m_star / m2
+    kt = OF.chemical_source(T, rho_star, t, y)[1][r]
+    kl = OF.chemical_source(lind, rho_star, t, y)[1][r]
+    assert math.isclose(kt / kl, alpha, rel_tol=1e-10), (kt / kl, alpha)

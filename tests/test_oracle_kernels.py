@@ -153,3 +153,28 @@ def test_lf_split_identities():
         np.testing.assert_allclose(fp - fm, jac * lam * q[:5], rtol=0, atol=1e-14 * scale)
         # the upwind half carries no left-going characteristic: F+ >= 0 for rho
         assert fp[0] >= 0 >= fm[0]
+
+
+@pytest.mark.parametrize("order", [5, 7])
+def test_flux_difference_converges_at_the_design_order(order):
+    """Finite-difference WENO: (h(i+1/2) - h(i-1/2)) / dx with the upwind half
+    on point values of f = sin(x) (downwind half zero) approximates f'(x_i);
+    in smooth data the WENO-Z weights reach the optimal ones fast enough that
+    halving dx divides the error by about 2^order."""
+    fn = O.weno5z if order == 5 else O.weno7z
+    w = order
+    errs = []
+    for n in (16, 32, 64):
+        dx = 2 * np.pi / n
+        x = np.arange(-w, n + w) * dx + 0.3
+        f = np.sin(x)
+        zero = np.zeros(w)
+
+        def h(i):  # face i+1/2 of point index i (offset w in the padded array)
+            c = i + w
+            return fn(f[c - w // 2:c + w // 2 + 1], zero, 1e-40)
+
+        d = np.array([(h(i) - h(i - 1)) / dx for i in range(n)])
+        errs.append(np.max(np.abs(d - np.cos(x[w:n + w]))))
+    rates = [np.log2(errs[k] / errs[k + 1]) for k in range(2)]
+    assert rates[1] > order - 0.5, (errs, rates)
