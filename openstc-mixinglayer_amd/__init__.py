@@ -179,6 +179,8 @@ SIGNATURES = {
 
 # the finite-rate mixture path (MixtureModel::FR): FrSolver, load_mechanism
 mixture = importlib.import_module(__name__ + ".mixture")
+# the reverse mode as a torch.autograd.Function (imports torch on first use)
+autodiff = importlib.import_module(__name__ + ".autodiff")
 SIGNATURES.update(mixture.SIGNATURES)
 
 

@@ -169,3 +169,21 @@ def test_reverse_sweep_rejects_what_it_does_not_cover():
     with pytest.raises(pkg.OstcError, match="has not run"):
         s.adjoint_backward()
     s.close()
+
+
+def test_torch_autograd_wrapper_returns_the_adjoint_gradient():
+    """autodiff.momentum_thickness: theta and amp.grad from torch's backward()
+    are the library's forward and reverse sweeps, bit for bit."""
+    import torch
+
+    device = "cpu"
+    case = cases.fourier_case(12, n_steps=4, checkpoint_every=2)
+    print("[gpu] torch autograd wrapper", flush=True)
+    th_ref, g_ref = _gpu_gradient(case, 2)
+    s = pkg.Solver(case)
+    amp = torch.tensor(case["modes"]["amp"], dtype=torch.float64, device=device, requires_grad=True)
+    th = pkg.autodiff.momentum_thickness(amp, s)
+    (2.0 * th).backward()
+    assert th.item() == th_ref
+    np.testing.assert_array_equal(amp.grad.cpu().numpy(), 2.0 * g_ref)
+    s.close()
