@@ -209,6 +209,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tile", type=int, default=-1, metavar="INDEX",
+                    help="use this entry of the tile table (one workgroup per tile) instead of the autotuner")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-n", type=int, default=128)
     ap.add_argument("--adjoint-steps", type=int, default=50)
@@ -241,7 +243,9 @@ def main():
     s = pkg.Solver(case)
     s.setup_mixing_layer()
     s.initialize()
-    if not a.no_tune:
+    if a.tile >= 0:
+        s.set_tile(a.tile, 0, 0)
+    elif not a.no_tune:
         best = s.autotune(3)
         print(f"[bench] autotune: best {best[1]:.3f} ms/stage", file=sys.stderr, flush=True)
     tile_index, tile_waves, tile_split = s.tile_config()

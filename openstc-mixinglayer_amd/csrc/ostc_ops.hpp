@@ -43,6 +43,10 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
   const int j = op.lo[1] + (int)((local / op.n[0]) % op.n[1]);
   const int k = op.lo[2] + (int)(local / ((long long)op.n[0] * op.n[1]));
   const DevBlock& B = blocks[op.block];
+  // unit step along the op's axis (selects, not a private array indexed at
+  // run time, which would live in scratch memory)
+  const int ax = op.axis;
+  const int di = ax == 0 ? op.side : 0, dj = ax == 1 ? op.side : 0, dk = ax == 2 ? op.side : 0;
   double* bv = B.bv[slot];
   double* cv = B.cv[slot];
   const long long s = B.sz;
@@ -60,18 +64,18 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
   switch (op.type) {
     case OP_BC_OUTFLOW:
     case OP_BC_PERIODIC: {
-      int d[3] = {0, 0, 0};
-      d[op.axis] = op.side;
-      int so[3] = {i, j, k};
+      int si = i, sj = j, sk = k;
       if (op.type == OP_BC_PERIODIC) {
-        const int m = op.axis == 0 ? B.mx : (op.axis == 1 ? B.my : B.mz);
-        so[op.axis] = op.side < 0 ? m - 1 : 0;
+        const int m = ax == 0 ? B.mx : (ax == 1 ? B.my : B.mz);
+        const int src = op.side < 0 ? m - 1 : 0;
+        si = ax == 0 ? src : i;
+        sj = ax == 1 ? src : j;
+        sk = ax == 2 ? src : k;
       }
       for (int g = 1; g <= B.g; ++g) {
-        const long long pg = pidx(B, i + g * d[0], j + g * d[1], k + g * d[2]);
-        const long long ps = (op.type == OP_BC_PERIODIC)
-                                 ? pidx(B, so[0] + g * d[0], so[1] + g * d[1], so[2] + g * d[2])
-                                 : pidx(B, i, j, k);
+        const long long pg = pidx(B, i + g * di, j + g * dj, k + g * dk);
+        const long long ps = (op.type == OP_BC_PERIODIC) ? pidx(B, si + g * di, sj + g * dj, sk + g * dk)
+                                                         : pidx(B, i, j, k);
         for (int l = 0; l < 6; ++l) bv[pg + l * s] = bv[ps + l * s];
         cv_point(B, pg);
       }
@@ -127,14 +131,13 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
         st[4] = st[0] * c_b * c_b / kGamma;
         st[5] = st[4] * kMwAir / (st[0] * kRu);
       }
-      int d[3] = {0, 0, 0};
-      d[op.axis] = op.side;
       if (face_too) {
         for (int l = 0; l < 6; ++l) bv[p + l * s] = st[l];
         cv_point(B, p);
       }
       for (int g = 1; g <= B.g; ++g) {
-        const long long pg = pidx(B, i + g * d[0], j + g * d[1], k + g * d[2]);
+        const long long pg = pidx(B, i + g * di, j + g * dj, k + g * dk);
+#pragma unroll
         for (int l = 0; l < 6; ++l) bv[pg + l * s] = st[l];
         cv_point(B, pg);
       }
@@ -142,10 +145,11 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
     }
     case OP_COMM: {
       const DevBlock& T = blocks[op.target];
-      int t[3] = {i, j, k};
-      const int idx_a = op.axis == 0 ? i : (op.axis == 1 ? j : k);
-      t[op.axis] = op.tstart + (idx_a - op.lo[op.axis]);
-      const long long p = pidx(B, i, j, k), pt = pidx(T, t[0], t[1], t[2]);
+      const int idx_a = ax == 0 ? i : (ax == 1 ? j : k);
+      const int lo_a = ax == 0 ? op.lo[0] : (ax == 1 ? op.lo[1] : op.lo[2]);
+      const int ta = op.tstart + (idx_a - lo_a);
+      const long long p = pidx(B, i, j, k);
+      const long long pt = pidx(T, ax == 0 ? ta : i, ax == 1 ? ta : j, ax == 2 ? ta : k);
       double* tbv = T.bv[slot];
       const long long ts = T.sz;
       if (idx_a == op.shared) {
