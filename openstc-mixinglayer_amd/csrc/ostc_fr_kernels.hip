@@ -600,7 +600,10 @@ __global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
   __syncthreads();
 
   // 2. one face per thread: face between cells t0-1+f and t0+f
-  const int f = tid % F, col = tid / F;
+  // face f of column col; along x consecutive lanes take consecutive faces,
+  // along y and z consecutive columns (consecutive i), so the global loads of
+  // a wave are contiguous in both cases
+  const int f = DIR == 0 ? tid % F : tid / TW, col = DIR == 0 ? tid / F : tid % TW;
   const int tl = t0 - 1 + f;  // left cell of the face
   int idx[3];
   tile_idx<DIR>(col0, tl, col, third, idx);
@@ -840,7 +843,10 @@ __global__ __launch_bounds__(kT) void k_inviscid_pv(FrArgs A) {
     }
   }
   __syncthreads();
-  const int f = tid % F, col = tid / F;
+  // face f of column col; along x consecutive lanes take consecutive faces,
+  // along y and z consecutive columns (consecutive i), so the global loads of
+  // a wave are contiguous in both cases
+  const int f = DIR == 0 ? tid % F : tid / TW, col = DIR == 0 ? tid / F : tid % TW;
   const int tl = t0 - 1 + f;
   int idx[3];
   tile_idx<DIR>(col0, tl, col, third, idx);
@@ -903,7 +909,10 @@ __global__ __launch_bounds__(kT) void k_viscous(FrArgs A) {
   const int t0 = blockIdx.x * TS;
   const long long s = B.sz;
   const int tid = threadIdx.x;
-  const int f = tid % F, col = tid / F;
+  // face f of column col; along x consecutive lanes take consecutive faces,
+  // along y and z consecutive columns (consecutive i), so the global loads of
+  // a wave are contiguous in both cases
+  const int f = DIR == 0 ? tid % F : tid / TW, col = DIR == 0 ? tid / F : tid % TW;
   const int tl = t0 - 1 + f;
   int idx[3];
   tile_idx<DIR>(col0, tl, col, third, idx);
