@@ -526,29 +526,19 @@ __device__ __forceinline__ void pp_limit(double* fc, const double* f1, const dou
   }
 }
 
-// One inviscid sweep along DIR (compute_convective_term_weno_{x,y,z}<FR>,
-// src/WENO.cu:8-461) with the component-wise (51) or characteristic (52)
-// WENO-Z5 flux, src/WENO.cu:1455-1536 and :722-1143 (method 1).
-template <int DIR, int NS, int SCHEME>
-__global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
-  // WENO-Z5 (51, 52): 3 ghost layers, 5-point candidates; WENO-Z7 (71, 72): 4 and 7
-  constexpr int G = (SCHEME == 71 || SCHEME == 72) ? 4 : 3, W = 2 * G - 1;
-  constexpr int NV = 5 + NS;
-  constexpr int TW = Tile<DIR>::TW, F = Tile<DIR>::F, TS = F - 1, P = TS + 2 * G, NP = P * TW;
-  extern __shared__ double lds[];
-  double* Fp = lds;            // [NV][NP]
-  double* Fm = lds + NV * NP;  // [NV][NP]
+// Point-local Lax-Friedrichs split fluxes F+ and F- (compute_flux,
+// src/WENO.cu:697-720) of the TW columns x P points of a sweep tile, cells
+// t0-G .. t0+P-G-1 along DIR, into LDS planes [NV][P * TW]; points outside
+// the padded block hold zeros.
+template <int DIR, int NS, int G, int TW, int P>
+__device__ __forceinline__ void split_fluxes(const FrArgs& A, double* Fp, double* Fm, int col0, int third, int t0) {
+  constexpr int NV = 5 + NS, NP = P * TW;
   const FrBlockDev& B = A.B;
   const int ext[3] = {B.mx, B.my, B.mz};
   const int m = ext[DIR];
   const int across = DIR == 0 ? 1 : 0;
-  const int col0 = blockIdx.y * TW;
-  const int third = blockIdx.z;
-  const int t0 = blockIdx.x * TS;  // first cell of the tile along DIR
   const long long s = B.sz;
   const int tid = threadIdx.x;
-
-  // 1. split fluxes of the tile's points (cells t0-3 .. t0+TS+2)
   for (int e = tid; e < NP; e += kT) {
     int col, sp;
     if (DIR == 0) {
@@ -597,6 +587,33 @@ __global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
       for (int l = 0; l < NV; ++l) Fp[l * NP + slot] = Fm[l * NP + slot] = 0;
     }
   }
+}
+
+// One inviscid sweep along DIR (compute_convective_term_weno_{x,y,z}<FR>,
+// src/WENO.cu:8-461) with the component-wise WENO-Z5 (51) or WENO-Z7 (71)
+// flux, src/WENO.cu:1455-1536; the characteristic schemes run in
+// k_inviscid_ch.
+template <int DIR, int NS, int SCHEME>
+__global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
+  // WENO-Z5 (51, 52): 3 ghost layers, 5-point candidates; WENO-Z7 (71, 72): 4 and 7
+  constexpr int G = (SCHEME == 71 || SCHEME == 72) ? 4 : 3, W = 2 * G - 1;
+  constexpr int NV = 5 + NS;
+  constexpr int TW = Tile<DIR>::TW, F = Tile<DIR>::F, TS = F - 1, P = TS + 2 * G, NP = P * TW;
+  extern __shared__ double lds[];
+  double* Fp = lds;            // [NV][NP]
+  double* Fm = lds + NV * NP;  // [NV][NP]
+  const FrBlockDev& B = A.B;
+  const int ext[3] = {B.mx, B.my, B.mz};
+  const int m = ext[DIR];
+  const int across = DIR == 0 ? 1 : 0;
+  const int col0 = blockIdx.y * TW;
+  const int third = blockIdx.z;
+  const int t0 = blockIdx.x * TS;  // first cell of the tile along DIR
+  const long long s = B.sz;
+  const int tid = threadIdx.x;
+
+  // 1. split fluxes of the tile's points (cells t0-G .. t0+TS+G-1)
+  split_fluxes<DIR, NS, G, TW, P>(A, Fp, Fm, col0, third, t0);
   __syncthreads();
 
   // 2. one face per thread: face between cells t0-1+f and t0+f
@@ -623,129 +640,7 @@ __global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
     }
     const double jac1 = B.jac[pl], jac2 = B.jac[pr];
     double f1[NS];
-    if (SCHEME == 52 || SCHEME == 72) {
-      // compute_weno_flux_ch<FR> with "method 1" flux splitting (72: 7 points
-      // and WENO-Z7, src/WENO.cu:898-965).  The five
-      // acoustic/entropy fields run as a rolled loop and their back
-      // projection (src/WENO.cu:1121-1142) is accumulated field by field in
-      // the reference's term order, which keeps the arithmetic identical and
-      // the live register set small.
-      CharFace<NS> C;
-      {
-        double cvl[NV + 2], cvr[NV + 2];
-#pragma unroll
-        for (int l = 0; l < NV; ++l) {
-          cvl[l] = B.cv[pl + l * s];
-          cvr[l] = B.cv[pr + l * s];
-        }
-        cvl[NV] = B.bv[pl + 4 * s];
-        cvr[NV] = B.bv[pr + 4 * s];
-        cvl[NV + 1] = B.cc[pl];
-        cvr[NV + 1] = B.cc[pr];
-        char_face<NS>(A.T, cvl, cvr, ml, mr, jac1, jac2, C);
-        if (A.pp) {
-          const double srThis = fabs((ml[0] * cvl[1] + ml[1] * cvl[2] + ml[2] * cvl[3]) / cvl[0] +
-                                     cvl[NV + 1] * sqrt(ml[0] * ml[0] + ml[1] * ml[1] + ml[2] * ml[2]));
-          const double srNext = fabs((mr[0] * cvr[1] + mr[1] * cvr[2] + mr[2] * cvr[3]) / cvr[0] +
-                                     cvr[NV + 1] * sqrt(mr[0] * mr[0] + mr[1] * mr[1] + mr[2] * mr[2]));
-#pragma unroll
-          for (int l = 0; l < NS; ++l)
-            f1[l] = 0.5 * (Fp[(l + 5) * NP + sl] + srThis * cvl[l + 5] * jac1) +
-                    0.5 * (Fp[(l + 5) * NP + sl + 1] - srNext * cvr[l + 5] * jac2);
-        }
-      }
-      constexpr double eps{1e-40};
-      const double eps_scaled = eps * A.weno_eps_scale * 0.25 * (C.kxJ * C.kxJ + C.kyJ * C.kyJ + C.kzJ * C.kzJ);
-      const double um = C.um, vm = C.vm, wm = C.wm, cm = C.cm, gm1 = C.gm1, kx = C.kx, ky = C.ky, kz = C.kz;
-      const double hm = C.hm, Ukb = C.Uk_bar;
-      double f0 = 0, f1a = 0, f2 = 0, f3 = 0, f4 = 0;  // fci[0..4] accumulated over the fields
-#pragma unroll 1
-      for (int l = 0; l < 5; ++l) {
-        double L[5];
-        const double cas = char_row<NS>(C, l, L);
-        double vPlus[W], vMinus[W];
-#pragma unroll 1
-        for (int r = 0; r < W; ++r) {
-          double ap = 0, am = 0;
-#pragma unroll
-          for (int n = 0; n < 5; ++n) {
-            ap += L[n] * Fp[n * NP + sl - W / 2 + r];
-            am += L[n] * Fm[n * NP + sl - W / 2 + 1 + r];
-          }
-#pragma unroll
-          for (int n = 0; n < NS; ++n) {
-            ap += cas * C.alpha_l[n] * Fp[(5 + n) * NP + sl - W / 2 + r];
-            am += cas * C.alpha_l[n] * Fm[(5 + n) * NP + sl - W / 2 + 1 + r];
-          }
-#pragma unroll
-          for (int q = 0; q < W; ++q) {
-            vPlus[q] = (q == r) ? ap : vPlus[q];
-            vMinus[q] = (q == r) ? am : vMinus[q];
-          }
-        }
-        double fch;
-        if constexpr (W == 7) fch = weno7z(vPlus, vMinus, eps_scaled);
-        else fch = weno5z(vPlus, vMinus, eps_scaled);
-        // column l of the back projection
-        double r0, r1, r2, r3, r4;
-        if (l == 0) {
-          r0 = 1; r1 = um - kx * cm; r2 = vm - ky * cm; r3 = wm - kz * cm; r4 = hm - Ukb * cm;
-        } else if (l == 1) {
-          r0 = kx; r1 = kx * um; r2 = kx * vm + kz * cm; r3 = kx * wm - ky * cm;
-          r4 = kx * (hm - cm * cm / gm1) + (kz * vm - ky * wm) * cm;
-        } else if (l == 2) {
-          r0 = ky; r1 = ky * um - kz * cm; r2 = ky * vm; r3 = ky * wm + kx * cm;
-          r4 = ky * (hm - cm * cm / gm1) + (kx * wm - kz * um) * cm;
-        } else if (l == 3) {
-          r0 = kz; r1 = kz * um + ky * cm; r2 = kz * vm - kx * cm; r3 = kz * wm;
-          r4 = kz * (hm - cm * cm / gm1) + (ky * um - kx * vm) * cm;
-        } else {
-          r0 = 1; r1 = um + kx * cm; r2 = vm + ky * cm; r3 = wm + kz * cm; r4 = hm + Ukb * cm;
-        }
-        if (l == 0) {
-          f0 = fch;  // 1 * fChar[0] is exact
-          f1a = r1 * fch;
-          f2 = r2 * fch;
-          f3 = r3 * fch;
-          f4 = r4 * fch;
-        } else {
-          f0 = (l == 4) ? f0 + fch : f0 + r0 * fch;
-          f1a = f1a + r1 * fch;
-          f2 = f2 + r2 * fch;
-          f3 = f3 + r3 * fch;
-          f4 = f4 + r4 * fch;
-        }
-      }
-      double add{0};
-#pragma unroll
-      for (int l = 0; l < NS; ++l) {
-        double vPlus[W], vMinus[W];
-#pragma unroll
-        for (int r = 0; r < W; ++r) {
-          vPlus[r] = -C.svm[l] * Fp[sl - W / 2 + r] + Fp[(5 + l) * NP + sl - W / 2 + r];
-          vMinus[r] = -C.svm[l] * Fm[sl - W / 2 + 1 + r] + Fm[(5 + l) * NP + sl - W / 2 + 1 + r];
-        }
-        double fch;
-        if constexpr (W == 7) fch = weno7z(vPlus, vMinus, eps_scaled);
-        else fch = weno5z(vPlus, vMinus, eps_scaled);
-        fc[5 + l] = C.svm[l] * f0 + fch;
-        add += C.alpha_l[l] * fch;
-      }
-      fc[0] = f0;
-      fc[1] = f1a;
-      fc[2] = f2;
-      fc[3] = f3;
-      fc[4] = f4 - add * cm * cm / gm1;
-      if (A.pp) {
-        double ql[NS], qr[NS];
-#pragma unroll
-        for (int l = 0; l < NS; ++l) {
-          ql[l] = B.cv[pl + (5 + l) * s];
-          qr[l] = B.cv[pr + (5 + l) * s];
-        }
-        pp_limit<NS>(fc + 5, f1, ql, qr, jac1, jac2, *A.dt, tl, m);
-      }
-    } else {
+    {
       constexpr double eps{1e-20};
       const double eps_ref = eps * A.weno_eps_scale * 0.25 *
                              ((ml[0] * jac1 + mr[0] * jac2) * (ml[0] * jac1 + mr[0] * jac2) +
@@ -797,6 +692,213 @@ __global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
 #pragma unroll
     for (int l = 0; l < NV; ++l)
       B.dq[q0 + l * n0] -= fcs[l * (F * TW) + col * F + f] - fcs[l * (F * TW) + col * F + f - 1];
+  }
+}
+
+// Tiles of the characteristic sweep: 128 faces per 256-thread workgroup,
+// two threads per face.
+template <int DIR> struct TileCh;
+template <> struct TileCh<0> { static constexpr int TW = 2, F = 64; };
+template <> struct TileCh<1> { static constexpr int TW = 8, F = 16; };
+template <> struct TileCh<2> { static constexpr int TW = 8, F = 16; };
+
+// One characteristic WENO sweep along DIR (compute_weno_flux_ch<FR>, "method
+// 1", src/WENO.cu:722-1143; 72: 7 points and WENO-Z7).  Two threads share a
+// face: waves 0-1 (half 0) project and reconstruct the acoustic/entropy fields
+// 0-2 and the first NS/2 species fields, waves 2-3 (half 1) fields 3-4 and the
+// other species; the characteristic fluxes meet in LDS, half 0 back-projects
+// the five flow components and half 1 the species (with the positivity
+// limiter), each in the reference's term order, so the face flux is the
+// same as one thread computing every field.  Both halves form the face's Roe
+// state themselves.
+template <int DIR, int NS, int SCHEME>
+__global__ __launch_bounds__(kT) void k_inviscid_ch(FrArgs A) {
+  constexpr int G = SCHEME == 72 ? 4 : 3, W = 2 * G - 1;
+  constexpr int NV = 5 + NS, SH = NS / 2;  // species fields < SH belong to half 0
+  constexpr int TW = TileCh<DIR>::TW, F = TileCh<DIR>::F, TS = F - 1, P = TS + 2 * G, NP = P * TW;
+  constexpr int NF = F * TW;  // faces of the tile (128)
+  extern __shared__ double lds[];
+  double* Fp = lds;                // [NV][NP]
+  double* Fm = lds + NV * NP;      // [NV][NP]
+  double* fch = lds + 2 * NV * NP; // [NV][NF] characteristic fluxes of every face
+  const FrBlockDev& B = A.B;
+  const int ext[3] = {B.mx, B.my, B.mz};
+  const int m = ext[DIR];
+  const int across = DIR == 0 ? 1 : 0;
+  const int col0 = blockIdx.y * TW;
+  const int third = blockIdx.z;
+  const int t0 = blockIdx.x * TS;
+  const long long s = B.sz;
+  const int tid = threadIdx.x;
+
+  split_fluxes<DIR, NS, G, TW, P>(A, Fp, Fm, col0, third, t0);
+  __syncthreads();
+
+  const int half = tid / NF, q = tid % NF;  // wave-uniform half
+  const int f = DIR == 0 ? q % F : q / TW, col = DIR == 0 ? q / F : q % TW;
+  const int tl = t0 - 1 + f;
+  int idx[3];
+  tile_idx<DIR>(col0, tl, col, third, idx);
+  const bool active = idx[across] < ext[across] && tl < m;
+  double fc[NV];
+  CharFace<NS> C;
+  long long pl = 0, pr = 0;
+  double jac1 = 0, jac2 = 0, f1[NS];
+  const int sl = col * P + (f + G - 1);  // LDS slot of the left cell
+  if (active) {
+    pl = pid(B, idx[0], idx[1], idx[2]);
+    int idr[3] = {idx[0], idx[1], idx[2]};
+    idr[DIR] += 1;
+    pr = pid(B, idr[0], idr[1], idr[2]);
+    double ml[3], mr[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ml[c] = B.met[pl + (3 * DIR + c) * s];
+      mr[c] = B.met[pr + (3 * DIR + c) * s];
+    }
+    jac1 = B.jac[pl];
+    jac2 = B.jac[pr];
+    {
+      double cvl[NV + 2], cvr[NV + 2];
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        cvl[l] = B.cv[pl + l * s];
+        cvr[l] = B.cv[pr + l * s];
+      }
+      cvl[NV] = B.bv[pl + 4 * s];
+      cvr[NV] = B.bv[pr + 4 * s];
+      cvl[NV + 1] = B.cc[pl];
+      cvr[NV + 1] = B.cc[pr];
+      char_face<NS>(A.T, cvl, cvr, ml, mr, jac1, jac2, C);
+      if (A.pp && half == 1) {
+        const double srThis = fabs((ml[0] * cvl[1] + ml[1] * cvl[2] + ml[2] * cvl[3]) / cvl[0] +
+                                   cvl[NV + 1] * sqrt(ml[0] * ml[0] + ml[1] * ml[1] + ml[2] * ml[2]));
+        const double srNext = fabs((mr[0] * cvr[1] + mr[1] * cvr[2] + mr[2] * cvr[3]) / cvr[0] +
+                                   cvr[NV + 1] * sqrt(mr[0] * mr[0] + mr[1] * mr[1] + mr[2] * mr[2]));
+#pragma unroll
+        for (int l = 0; l < NS; ++l)
+          f1[l] = 0.5 * (Fp[(l + 5) * NP + sl] + srThis * cvl[l + 5] * jac1) +
+                  0.5 * (Fp[(l + 5) * NP + sl + 1] - srNext * cvr[l + 5] * jac2);
+      }
+    }
+    constexpr double eps{1e-40};
+    const double eps_scaled = eps * A.weno_eps_scale * 0.25 * (C.kxJ * C.kxJ + C.kyJ * C.kyJ + C.kzJ * C.kzJ);
+    // acoustic and entropy fields of this half
+    const int l_lo = half == 0 ? 0 : 3, l_hi = half == 0 ? 3 : 5;
+#pragma unroll 1
+    for (int l = l_lo; l < l_hi; ++l) {
+      double L[5];
+      const double cas = char_row<NS>(C, l, L);
+      double vPlus[W], vMinus[W];
+#pragma unroll 1
+      for (int r = 0; r < W; ++r) {
+        double ap = 0, am = 0;
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+          ap += L[n] * Fp[n * NP + sl - W / 2 + r];
+          am += L[n] * Fm[n * NP + sl - W / 2 + 1 + r];
+        }
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+          ap += cas * C.alpha_l[n] * Fp[(5 + n) * NP + sl - W / 2 + r];
+          am += cas * C.alpha_l[n] * Fm[(5 + n) * NP + sl - W / 2 + 1 + r];
+        }
+#pragma unroll
+        for (int qq = 0; qq < W; ++qq) {
+          vPlus[qq] = (qq == r) ? ap : vPlus[qq];
+          vMinus[qq] = (qq == r) ? am : vMinus[qq];
+        }
+      }
+      double v;
+      if constexpr (W == 7) v = weno7z(vPlus, vMinus, eps_scaled);
+      else v = weno5z(vPlus, vMinus, eps_scaled);
+      fch[l * NF + q] = v;
+    }
+    // species fields of this half (unrolled: CharFace stays in registers)
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      if ((l < SH) != (half == 0)) continue;
+      double vPlus[W], vMinus[W];
+      const double svm = C.svm[l];
+#pragma unroll
+      for (int r = 0; r < W; ++r) {
+        vPlus[r] = -svm * Fp[sl - W / 2 + r] + Fp[(5 + l) * NP + sl - W / 2 + r];
+        vMinus[r] = -svm * Fm[sl - W / 2 + 1 + r] + Fm[(5 + l) * NP + sl - W / 2 + 1 + r];
+      }
+      double v;
+      if constexpr (W == 7) v = weno7z(vPlus, vMinus, eps_scaled);
+      else v = weno5z(vPlus, vMinus, eps_scaled);
+      fch[(5 + l) * NF + q] = v;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    // back projection (src/WENO.cu:1121-1142), fields accumulated in order
+    const double um = C.um, vm = C.vm, wm = C.wm, cm = C.cm, gm1 = C.gm1, kx = C.kx, ky = C.ky, kz = C.kz;
+    const double hm = C.hm, Ukb = C.Uk_bar;
+    const double c0 = fch[0 * NF + q], c1 = fch[1 * NF + q], c2 = fch[2 * NF + q], c3 = fch[3 * NF + q],
+                 c4 = fch[4 * NF + q];
+    double f0 = c0;  // 1 * fChar[0] is exact
+    f0 = f0 + kx * c1;
+    f0 = f0 + ky * c2;
+    f0 = f0 + kz * c3;
+    f0 = f0 + c4;
+    if (half == 0) {
+      double f1a = (um - kx * cm) * c0, f2 = (vm - ky * cm) * c0, f3 = (wm - kz * cm) * c0,
+             f4 = (hm - Ukb * cm) * c0;
+      f1a = f1a + kx * um * c1;
+      f2 = f2 + (kx * vm + kz * cm) * c1;
+      f3 = f3 + (kx * wm - ky * cm) * c1;
+      f4 = f4 + (kx * (hm - cm * cm / gm1) + (kz * vm - ky * wm) * cm) * c1;
+      f1a = f1a + (ky * um - kz * cm) * c2;
+      f2 = f2 + ky * vm * c2;
+      f3 = f3 + (ky * wm + kx * cm) * c2;
+      f4 = f4 + (ky * (hm - cm * cm / gm1) + (kx * wm - kz * um) * cm) * c2;
+      f1a = f1a + (kz * um + ky * cm) * c3;
+      f2 = f2 + (kz * vm - kx * cm) * c3;
+      f3 = f3 + kz * wm * c3;
+      f4 = f4 + (kz * (hm - cm * cm / gm1) + (ky * um - kx * vm) * cm) * c3;
+      f1a = f1a + (um + kx * cm) * c4;
+      f2 = f2 + (vm + ky * cm) * c4;
+      f3 = f3 + (wm + kz * cm) * c4;
+      f4 = f4 + (hm + Ukb * cm) * c4;
+      double add{0};
+#pragma unroll
+      for (int l = 0; l < NS; ++l) add += C.alpha_l[l] * fch[(5 + l) * NF + q];
+      fc[0] = f0;
+      fc[1] = f1a;
+      fc[2] = f2;
+      fc[3] = f3;
+      fc[4] = f4 - add * cm * cm / gm1;
+    } else {
+#pragma unroll
+      for (int l = 0; l < NS; ++l) fc[5 + l] = C.svm[l] * f0 + fch[(5 + l) * NF + q];
+      if (A.pp) {
+        double ql[NS], qr[NS];
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          ql[l] = B.cv[pl + (5 + l) * s];
+          qr[l] = B.cv[pr + (5 + l) * s];
+        }
+        pp_limit<NS>(fc + 5, f1, ql, qr, jac1, jac2, *A.dt, tl, m);
+      }
+    }
+  }
+  __syncthreads();  // every face has read F+/F- and fch; the LDS now holds face fluxes
+  double* fcs = lds;  // [NV][NF], face slot col * F + f
+  const int v_lo = half == 0 ? 0 : 5, v_hi = half == 0 ? 5 : NV;
+  if (active) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l)
+      if (l >= v_lo && l < v_hi) fcs[l * NF + col * F + f] = fc[l];
+  }
+  __syncthreads();
+  if (f >= 1 && active) {
+    const long long q0 = pid0(B, idx[0], idx[1], idx[2]);
+    const long long n0 = B.n0;
+#pragma unroll
+    for (int l = 0; l < NV; ++l)
+      if (l >= v_lo && l < v_hi) B.dq[q0 + l * n0] -= fcs[l * NF + col * F + f] - fcs[l * NF + col * F + f - 1];
   }
 }
 
@@ -1065,12 +1167,17 @@ void launch_rhs_t(const FrArgs& A, hipStream_t st, long long* n, KernelTimer* tm
     dim3 grid((ext[D] + TS - 1) / TS, (ext[across] + TW - 1) / TW, ext[third]);
     const size_t lds = (size_t)2 * (5 + NS) * P * TW * sizeof(double);
     rec(KT_INV_X + D, false);
-    if (A.scheme == 52)
-      hipLaunchKernelGGL((k_inviscid<D, NS, 52>), grid, dim3(kT), lds, st, A);
-    else if (A.scheme == 51)
+    if (A.scheme == 52 || A.scheme == 72) {
+      constexpr int TWc = TileCh<D>::TW, Fc = TileCh<D>::F, TSc = Fc - 1;
+      const int Pc = TSc + 2 * (A.scheme == 72 ? 4 : 3);
+      dim3 gridc((ext[D] + TSc - 1) / TSc, (ext[across] + TWc - 1) / TWc, ext[third]);
+      const size_t ldsc = ((size_t)2 * (5 + NS) * Pc * TWc + (size_t)(5 + NS) * Fc * TWc) * sizeof(double);
+      if (A.scheme == 52)
+        hipLaunchKernelGGL((k_inviscid_ch<D, NS, 52>), gridc, dim3(kT), ldsc, st, A);
+      else
+        hipLaunchKernelGGL((k_inviscid_ch<D, NS, 72>), gridc, dim3(kT), ldsc, st, A);
+    } else if (A.scheme == 51)
       hipLaunchKernelGGL((k_inviscid<D, NS, 51>), grid, dim3(kT), lds, st, A);
-    else if (A.scheme == 72)
-      hipLaunchKernelGGL((k_inviscid<D, NS, 72>), grid, dim3(kT), lds, st, A);
     else if (A.scheme == 71)
       hipLaunchKernelGGL((k_inviscid<D, NS, 71>), grid, dim3(kT), lds, st, A);
     else  // primitive reconstruction + Riemann solver (3, 4): TS + 4 points, or F faces of fluxes
