@@ -39,6 +39,7 @@ struct ostc_fr_ctx_s {
   ostc_fr_setup s{};
   ostc_mech_tables tab{};
   Tab* d_tab = nullptr;
+  double* d_tlog = nullptr;        // log tables of the transport kernel (16 + 256 doubles)
   int ns = 0, nv = 5, ngg = 3;
   hipStream_t st = nullptr;
   bool own_stream = false;
@@ -91,6 +92,7 @@ FrArgs args(ostc_fr_ctx c, int b) {
   FrArgs a{};
   a.B = c->blocks[b].d;
   a.T = c->d_tab;
+  a.tlog = c->d_tlog;
   a.scheme = c->s.inviscid_scheme;
   a.viscous = c->s.viscous_order != 0;
   a.reaction = c->s.reaction != 0;
@@ -434,6 +436,18 @@ int ostc_fr_finalize(ostc_fr_ctx c) {
     }
   FR_HIP(c, hipMalloc((void**)&c->d_tab, sizeof(Tab)));
   FR_HIP(c, hipMemcpyAsync(c->d_tab, &c->tab, sizeof(Tab), hipMemcpyHostToDevice, c->st));
+  {
+    // logarithms of the temperature scales of the collision integrals
+    static_assert(OSTC_MAX_SPEC == 16, "log table layout");
+    std::vector<double> h(16 + 16 * 16, 0.0);  // the copy below completes before h goes out of scope
+    for (int l = 0; l < c->ns; ++l) h[l] = std::log(c->tab.lj_potent_inv[l]);
+    for (int m = 0; m < c->ns; ++m)
+      for (int n = 0; n < c->ns; ++n)
+        if (m != n) h[16 + 16 * m + n] = std::log(c->tab.kb_over_eps_jk[m][n]);
+    if (int r = dalloc(c, &c->d_tlog, h.size())) return r;
+    FR_HIP(c, hipMemcpyAsync(c->d_tlog, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    FR_HIP(c, hipStreamSynchronize(c->st));
+  }
   const int g = c->ngg, nv = c->nv, ns = c->ns;
   for (auto& h : c->blocks) {
     FrBlockDev& d = h.d;

@@ -106,17 +106,22 @@ __device__ __forceinline__ void gibbs_rt(const Tab* __restrict__ T, double t, do
 
 // compute_transport_property, src/Transport.cu:43-136, with compute_Omega_D
 // (:168-170).  Only the off-diagonal binary diffusivities are formed: the
-// mixture-averaged rho D_k never reads the diagonal.
+// mixture-averaged rho D_k never reads the diagonal.  The reference's powers
+// are evaluated as exp(b (log T + log c)) with log c tabulated per species and
+// species pair (tlog: log(lj_potent_inv[l]), then log(kb_over_eps_jk[m][n])
+// at 16 + 16 m + n), and (x + 0.5)^-2 as 1 / ((x + 0.5)^2): one logarithm per
+// point instead of 54 pow calls, within a few ulp of pow.
 template <int NS>
-__device__ __forceinline__ void transport(const Tab* __restrict__ T, double t, double p, double rho,
-                                          const double* Y, double mw_total, const double* cp, double* mu_out,
-                                          double* lambda_out, double* rho_d) {
+__device__ __forceinline__ void transport(const Tab* __restrict__ T, const double* __restrict__ tlog, double t,
+                                          double p, double rho, const double* Y, double mw_total, const double* cp,
+                                          double* mu_out, double* lambda_out, double* rho_d) {
   double X[NS], vis[NS];
+  const double log_t = log(t);
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     X[l] = Y[l] * mw_total / T->mw[l];
     const double t_dl{t * T->lj_potent_inv[l]};
-    const double collision_integral{1.147 * pow(t_dl, -0.145) + pow(t_dl + 0.5, -2.0)};
+    const double collision_integral{1.147 * exp(-0.145 * (log_t + tlog[l])) + 1.0 / ((t_dl + 0.5) * (t_dl + 0.5))};
     vis[l] = T->vis_coeff[l] * sqrt(t) / collision_integral;
   }
   double viscosity = 0, conductivity = 0;
@@ -146,7 +151,7 @@ __device__ __forceinline__ void transport(const Tab* __restrict__ T, double t, d
 #pragma unroll
     for (int n = m + 1; n < NS; ++n) {
       const double t_red{t * T->kb_over_eps_jk[m][n]};
-      const double om = 1.0 / pow(t_red, 0.145) + 1.0 / ((t_red + 0.5) * (t_red + 0.5));
+      const double om = exp(-0.145 * (log_t + tlog[16 + 16 * m + n])) + 1.0 / ((t_red + 0.5) * (t_red + 0.5));
       const double d = T->binary_diffusivity_coeff[m][n] * t_3over2_over_p / om;
       dij[m][n] = d;
       dij[n][m] = d;

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kT) void k_props(FrArgs A) {
   B.gam[p] = gam;
   B.cc[p] = c;
   double mu, lam, rd[NS];
-  transport<NS>(A.T, t, B.bv[p + 4 * s], B.bv[p], Y, mw, cp, &mu, &lam, rd);
+  transport<NS>(A.T, A.tlog, t, B.bv[p + 4 * s], B.bv[p], Y, mw, cp, &mu, &lam, rd);
   B.mul[p] = mu;
   B.lam[p] = lam;
 #pragma unroll
