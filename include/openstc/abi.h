@@ -136,6 +136,22 @@ int ostc_add_farfield(ostc_ctx ctx, int block, int axis, int side, const ostc_fa
  * (InnerFace::register_boundary, src/Mesh.cpp:38-176).  Register each side
  * of an interface, as the reference's inner N.txt files do. */
 int ostc_add_inner_face(ostc_ctx ctx, int block, int axis, int side, int target_block);
+/* Connectivity records of the reference's input files (Mesh::read_boundary,
+ * Mesh::read_inner_interface, src/Mesh.cpp:790-843).
+ * ostc_add_boundary_range: one boundary N.txt line, range = {x1, x2, y1, y2,
+ * z1, z2} in 0-based node indices, the normal axis having equal start and end
+ * (Boundary::register_boundary, src/Mesh.cpp:14-32); the face covers that node
+ * range widened by ngg on the tangential axes.  type OSTC_BC_*; ff is the
+ * far-field state of type OSTC_BC_FARFIELD (NULL otherwise).
+ * ostc_add_inner_face_range: one inner N.txt line pair, 1-based node indices of
+ * this block's face (range) and of the target block's face (target_range), the
+ * tangential axis that pairs with the other side's marked axis given as
+ * negative indices; the two faces may have any relative orientation (index
+ * axes permuted and reversed; InnerFace::register_boundary,
+ * src/Mesh.cpp:38-176).  target_block is 0-based (the file's id minus 1). */
+int ostc_add_boundary_range(ostc_ctx ctx, int block, const int range[6], int type, const ostc_farfield* ff);
+int ostc_add_inner_face_range(ostc_ctx ctx, int block, const int range[6], const int target_range[6],
+                              int target_block);
 /* Allocates device memory and builds the work-lists; blocks and faces are
  * fixed afterwards. */
 int ostc_finalize(ostc_ctx ctx);
@@ -205,6 +221,11 @@ const char* ostc_last_error(ostc_ctx ctx);
  * velocity perturbation amplitude*U(-1,1) from SplitMix64(seed, node, comp). */
 int ostc_util_uniform_geometry(int mx, int my, int mz, int ngg, double x0, double y0, double z0,
                                double dx, double dy, double dz, double* jac, double* metric);
+/* Block::compute_jac_metric (src/Mesh.cpp:241-502) from node coordinates: x, y, z
+ * on the domain padded by ngg+1 ghost layers, jac and metric (SoA) on the
+ * domain padded by ngg.  OSTC_ERR_NUMERIC on a non-positive Jacobian. */
+int ostc_util_metric(int mx, int my, int mz, int ngg, const double* x, const double* y, const double* z,
+                     double* jac, double* metric);
 int ostc_util_mixing_layer_bv(int mx, int my, int mz, int ngg, double y0, double dy,
                               const double upper[6], const double lower[6], double delta_omega,
                               int gi0, int gj0, int gk0, int period_i, int period_j, int period_k,
@@ -351,6 +372,14 @@ int ostc_fr_destroy(ostc_fr_ctx ctx);
 int ostc_fr_add_block(ostc_fr_ctx ctx, const ostc_block_desc* d, int* block_id);
 int ostc_fr_add_boundary(ostc_fr_ctx ctx, int block, int axis, int side, int type);  /* outflow, periodic */
 int ostc_fr_add_inner_face(ostc_fr_ctx ctx, int block, int axis, int side, int target_block);
+/* Connectivity records of the reference's input files, as ostc_add_boundary_range
+ * and ostc_add_inner_face_range (src/Mesh.cpp:14-176, :790-843): a boundary
+ * N.txt line (0-based node range, ff for a farfield label) and an inner N.txt
+ * line pair (1-based, the paired tangential axis marked negative; any relative
+ * orientation of the two blocks' index axes). */
+int ostc_fr_add_boundary_range(ostc_fr_ctx ctx, int block, const int range[6], int type, const ostc_farfield* ff);
+int ostc_fr_add_inner_face_range(ostc_fr_ctx ctx, int block, const int range[6], const int target_range[6],
+                                 int target_block);
 /* ghost layers of every array of the context: 3, or 4 for the WENO-Z7 schemes 71/72 (src/Parameter.cpp:227-238) */
 int ostc_fr_ngg(ostc_fr_ctx ctx, int* ngg);
 /* store_last_step / compute_residual (src/TimeAdvanceFunc.cu:6-37, src/Residual.cuh:12-187): unscaled

@@ -139,6 +139,8 @@ SIGNATURES = {
     "ostc_add_farfield": (_I, [_P, _I, _I, _I, C.POINTER(Farfield)]),
     "ostc_farfield_state_air": (_I, [C.POINTER(FarfieldSpec), C.POINTER(Farfield)]),
     "ostc_add_inner_face": (_I, [_P, _I, _I, _I, _I]),
+    "ostc_add_boundary_range": (_I, [_P, _I, C.POINTER(_I), _I, C.POINTER(Farfield)]),
+    "ostc_add_inner_face_range": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), _I]),
     "ostc_finalize": (_I, [_P]),
     "ostc_ngg": (_I, [_P, C.POINTER(_I)]),
     "ostc_zone_view": (_I, [_P, _I, C.POINTER(Zone)]),
@@ -167,6 +169,7 @@ SIGNATURES = {
     "ostc_sync": (_I, [_P]),
     "ostc_last_error": (C.c_char_p, [_P]),
     "ostc_util_uniform_geometry": (_I, [_I, _I, _I, _I, _D, _D, _D, _D, _D, _D, _DP, _DP]),
+    "ostc_util_metric": (_I, [_I, _I, _I, _I, _DP, _DP, _DP, _DP, _DP]),
     "ostc_util_mixing_layer_bv": (_I, [_I, _I, _I, _I, _D, _D, _DP, _DP, _D, _I, _I, _I, _I, _I, _I,
                                        C.c_uint64, _D, _DP]),
     "ostc_rhs_vjp": (_I, [_P, _I, _P, _P]),
@@ -179,6 +182,8 @@ SIGNATURES = {
 
 # the finite-rate mixture path (MixtureModel::FR): FrSolver, load_mechanism
 mixture = importlib.import_module(__name__ + ".mixture")
+# readers and writers of the reference's connectivity files
+mesh_io = importlib.import_module(__name__ + ".mesh_io")
 # the reverse mode as a torch.autograd.Function (imports torch on first use)
 autodiff = importlib.import_module(__name__ + ".autodiff")
 SIGNATURES.update(mixture.SIGNATURES)
@@ -201,6 +206,25 @@ def lib():
 
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(_DP)
+
+
+def ints(v):
+    """A C int array of a connectivity record."""
+    return (C.c_int * len(v))(*[int(x) for x in v])
+
+
+def metric_from_coordinates(mx: int, my: int, mz: int, g: int, x: np.ndarray, y: np.ndarray, z: np.ndarray):
+    """(jac, metric) on the domain padded by g from node coordinates on the
+    domain padded by g+1 (Block::compute_jac_metric, src/Mesh.cpp:241-502)."""
+    shp = (mz + 2 * g, my + 2 * g, mx + 2 * g)
+    xs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z)]
+    if any(a.size != (mz + 2 * g + 2) * (my + 2 * g + 2) * (mx + 2 * g + 2) for a in xs):
+        raise OstcError("coordinates must cover the domain padded by g+1 layers")
+    jac, met = np.empty(shp), np.empty((9, *shp))
+    r = lib().ostc_util_metric(mx, my, mz, g, *(_dp(a) for a in xs), _dp(jac), _dp(met))
+    if r != 0:
+        raise OstcError(f"ostc_util_metric failed ({r}): non-positive Jacobian")
+    return jac, met
 
 
 def padded_shape(blk: dict, g: int):
@@ -227,13 +251,20 @@ class Solver:
         for b in case["blocks"]:
             self._chk(L.ostc_add_block(self.ctx, C.byref(_BlockDesc(b["mx"], b["my"], b["mz"])), None))
         for f in case["boundaries"]:
-            if f["type"] == 4:
-                ff = farfield_state_air(**f["farfield"])
+            ff = farfield_state_air(**f["farfield"]) if f["type"] == 4 else None
+            if "range" in f:  # a boundary N.txt record
+                self._chk(L.ostc_add_boundary_range(self.ctx, f["block"], ints(f["range"]), f["type"],
+                                                    None if ff is None else C.byref(ff)))
+            elif ff is not None:
                 self._chk(L.ostc_add_farfield(self.ctx, f["block"], f["axis"], f["side"], C.byref(ff)))
             else:
                 self._chk(L.ostc_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"]))
         for f in case["inner_faces"]:
-            self._chk(L.ostc_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"]))
+            if "range" in f:  # an inner N.txt line pair
+                self._chk(L.ostc_add_inner_face_range(self.ctx, f["block"], ints(f["range"]),
+                                                      ints(f["target_range"]), f["target"]))
+            else:
+                self._chk(L.ostc_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"]))
         self._chk(L.ostc_finalize(self.ctx))
 
     def _chk(self, r):

@@ -342,3 +342,102 @@ def reacting_case(x_cells, y_cells, z_cells, tables, *, mech_files=None, reactio
 def reacting_single_block(nx, ny, nz, tables, **kw):
     """One block of nx x ny x nz nodes, periodic x/z, outflow y."""
     return reacting_case([nx - 1], [ny - 1], [nz - 1], tables, **kw)
+
+
+# ------------------------------------------------------------------ rotated pair
+ROTATED_AXES = ((0, -1, 0), (-1, 0, 0), (0, 0, -1))  # global (i, j, k) from local (p, q, r), reversed
+
+
+def rotated_pair(case: dict, cut: int, split_k: int | None = None) -> dict:
+    """The single-block case ``case`` (periodic x and z, y faces of one type)
+    cut at global node i = ``cut`` into two blocks given by the records of
+    the reference's connectivity files.
+
+    Block 0 keeps nodes i <= cut on the global index axes.  Block 1 holds
+    nodes i >= cut with local (p, q, r) = (ny-1-j, nx-1-i, nz-1-k): index
+    axes permuted and all three reversed (a right-handed frame).  Boundary
+    faces are boundary N.txt ranges (block 0's z-min face in two pieces whose
+    ranges share the node line i = cut // 2, widened by ngg as the reference
+    widens them, so both pieces write the ghosts near that line);
+    inner faces are inner N.txt line pairs for the interface at i = cut
+    (in two pieces split at 1-based k = ``split_k`` when given) and for the
+    periodic closure at i = 0 (node nx-1 is node 0).  ``frames`` records
+    each block's map to global node indices: global = origin + axes @ local.
+    """
+    blk = case["blocks"][0]
+    nx, ny, nz = blk["mx"], blk["my"], blk["mz"]
+    if len(case["blocks"]) != 1 or not 3 <= cut <= nx - 4:
+        raise ValueError("rotated_pair takes a single-block case and an interior cut")
+    nq = nx - cut
+    by_face = {(f["axis"], f["side"]): f for f in case["boundaries"]}
+    y_lo, y_hi = by_face[(1, -1)], by_face[(1, 1)]
+
+    def bnd(b, rng, like):
+        f = dict(block=b, range=list(rng), type=like["type"])
+        for key in ("farfield", "farfield_y"):
+            if key in like:
+                f[key] = like[key]
+        return f
+
+    c1 = cut // 2
+    per = dict(type=BC_PERIODIC)
+    boundaries = [
+        bnd(0, (0, cut, 0, 0, 0, nz - 1), y_lo), bnd(0, (0, cut, ny - 1, ny - 1, 0, nz - 1), y_hi),
+        bnd(0, (0, c1, 0, ny - 1, 0, 0), per), bnd(0, (c1, cut, 0, ny - 1, 0, 0), per),
+        bnd(0, (0, cut, 0, ny - 1, nz - 1, nz - 1), per),
+        # block 1: p = 0 is global y = ny-1 (the upper face), p = ny-1 the lower one
+        bnd(1, (0, 0, 0, nq - 1, 0, nz - 1), y_hi), bnd(1, (ny - 1, ny - 1, 0, nq - 1, 0, nz - 1), y_lo),
+        bnd(1, (0, ny - 1, 0, nq - 1, 0, 0), per), bnd(1, (0, ny - 1, 0, nq - 1, nz - 1, nz - 1), per),
+    ]
+    pieces = [(1, nz)] if split_k is None else [(1, split_k), (split_k, nz)]
+    inner = [dict(block=0, range=[1, 1, -1, -ny, 1, nz], target_range=[-ny, -1, 1, 1, nz, 1], target=1)]
+    for k0, k1 in pieces:
+        inner.append(dict(block=0, range=[cut + 1, cut + 1, -1, -ny, k0, k1],
+                          target_range=[-ny, -1, nq, nq, nz + 1 - k0, nz + 1 - k1], target=1))
+    inner.append(dict(block=1, range=[-ny, -1, 1, 1, nz, 1], target_range=[1, 1, -1, -ny, 1, nz], target=0))
+    for k0, k1 in pieces:
+        inner.append(dict(block=1, range=[-ny, -1, nq, nq, nz + 1 - k0, nz + 1 - k1],
+                          target_range=[cut + 1, cut + 1, -1, -ny, k0, k1], target=0))
+    blocks = [dict(blk, mx=cut + 1, my=ny, mz=nz, origin=(0, 0, 0), axes=((1, 0, 0), (0, 1, 0), (0, 0, 1))),
+              dict(blk, mx=ny, my=nq, mz=nz, origin=(nx - 1, ny - 1, nz - 1), axes=ROTATED_AXES)]
+    out = dict(case)
+    out.update(blocks=blocks, boundaries=boundaries, inner_faces=inner,
+               n_cells=sum(b["mx"] * b["my"] * b["mz"] for b in blocks), global_shape=(nx, ny, nz))
+    return out
+
+
+def _global_index(blk: dict, p, q, r):
+    o, a = blk["origin"], blk["axes"]
+    return tuple(o[d] + a[d][0] * p + a[d][1] * q + a[d][2] * r for d in range(3))
+
+
+def _local_grid(blk: dict, g: int):
+    p = np.arange(-g, blk["mx"] + g)
+    q = np.arange(-g, blk["my"] + g)
+    r = np.arange(-g, blk["mz"] + g)
+    R, Q, P = np.meshgrid(r, q, p, indexing="ij")
+    return _global_index(blk, P, Q, R)
+
+
+def block_coordinates(case: dict, b: int, g: int):
+    """Node coordinates (x, y, z) of block b on its domain padded by g+1
+    layers, arrays (k, j, i), from the global uniform grid of the pair."""
+    blk = case["blocks"][b]
+    i, j, k = _local_grid(blk, g + 1)
+    return blk["x0"] + i * case["dx"], blk["y0"] + j * case["dy"], blk["z0"] + k * case["dz"]
+
+
+def block_from_global(case: dict, b: int, arr: np.ndarray, g: int) -> np.ndarray:
+    """Block b's padded array (planes, k, j, i) from a global single-block
+    padded array with g ghost layers."""
+    i, j, k = _local_grid(case["blocks"][b], g)
+    return arr[:, k + g, j + g, i + g]
+
+
+def global_from_block(case: dict, b: int, arr: np.ndarray, g: int):
+    """(global (k, j, i) index arrays, values (planes, n)) of block b's
+    interior nodes; arr is padded by g (g = 0 for interior-only arrays)."""
+    blk = case["blocks"][b]
+    i, j, k = _local_grid(blk, 0)
+    inner = arr[:, g:g + blk["mz"], g:g + blk["my"], g:g + blk["mx"]] if g else arr
+    return (k.ravel(), j.ravel(), i.ravel()), inner.reshape(arr.shape[0], -1)

@@ -2,6 +2,7 @@
 // block and face registry, device memory, the ghost-layer work-lists, the
 // per-block (reference-shaped) step and the batched, graph-captured step.
 #include "openstc/abi.h"
+#include "ostc_faces.hpp"
 #include "ostc_kernels.hpp"
 
 #include <algorithm>
@@ -15,12 +16,15 @@ using namespace ostc;
 
 namespace {
 
-struct FaceDesc { int axis, side, type, ff; };  // ff: index of the far-field state (type 4)
-struct InnerDesc { int axis, side, target; };
+struct FaceDesc {
+  int type, ff;  // ff: index of the far-field state (type 4)
+  FaceRange r;   // the face layer and its tangential range widened by ngg
+};
 struct Box { int lo[3], hi[3]; };  // inclusive bounds
 
 bool box_ok(const Box& b) { return b.lo[0] <= b.hi[0] && b.lo[1] <= b.hi[1] && b.lo[2] <= b.hi[2]; }
 bool box_hit(const Box& a, const Box& b) {
+  if (!box_ok(a) || !box_ok(b)) return false;
   for (int d = 0; d < 3; ++d)
     if (a.hi[d] < b.lo[d] || b.hi[d] < a.lo[d]) return false;
   return true;
@@ -48,7 +52,7 @@ std::vector<Box> minus_interior(Box r, const int m[3]) {
 struct HBlock {
   int m[3];
   std::vector<FaceDesc> faces;
-  std::vector<InnerDesc> inner;
+  std::vector<FaceMap> inner;
   DevBlock d{};
 };
 
@@ -156,15 +160,37 @@ DevOp make_op(int type, int block, const Box& b) {
   return op;
 }
 
-Box full_tangential(const HBlock& B, int axis, int lo, int hi, int ext) {
+// The points of a boundary face's range with the normal index running lo..hi.
+Box face_box(const FaceRange& r, int lo, int hi) {
   Box b;
   for (int d = 0; d < 3; ++d) {
-    b.lo[d] = -ext;
-    b.hi[d] = B.m[d] - 1 + ext;
+    b.lo[d] = r.lo[d];
+    b.hi[d] = r.hi[d];
   }
-  b.lo[axis] = lo;
-  b.hi[axis] = hi;
+  b.lo[r.axis] = lo;
+  b.hi[r.axis] = hi;
   return b;
+}
+
+// Image of a source box of an inner face in the target block.
+Box map_box(const FaceMap& m, const Box& b) {
+  Box t;
+  for (int d = 0; d < 3; ++d) {
+    const int c = m.code[d], p = (c < 0 ? -c : c) - 1;
+    const int x = m.off[d] + (c < 0 ? -b.lo[p] : b.lo[p]), y = m.off[d] + (c < 0 ? -b.hi[p] : b.hi[p]);
+    t.lo[d] = std::min(x, y);
+    t.hi[d] = std::max(x, y);
+  }
+  return t;
+}
+
+Box box_and(const Box& a, const Box& b) {
+  Box r;
+  for (int d = 0; d < 3; ++d) {
+    r.lo[d] = std::max(a.lo[d], b.lo[d]);
+    r.hi[d] = std::min(a.hi[d], b.hi[d]);
+  }
+  return r;
 }
 
 // Work-lists for the ghost layers.  Boundary ops of one block keep the
@@ -184,26 +210,28 @@ int build_ops(ostc_ctx c) {
     std::stable_sort(f.begin(), f.end(),
                      [](const FaceDesc& x, const FaceDesc& y) { return bc_priority(x.type) < bc_priority(y.type); });
     for (const FaceDesc& fd : f) {
-      const int a = fd.axis, m = B.m[a];
-      const int idx = fd.side < 0 ? 0 : m - 1;
+      const int a = fd.r.axis, m = B.m[a];
+      const int idx = fd.r.side < 0 ? 0 : m - 1;
       const int ot = fd.type == OSTC_BC_PERIODIC ? OP_BC_PERIODIC
                      : fd.type == OSTC_BC_FARFIELD ? OP_BC_FARFIELD : OP_BC_OUTFLOW;
-      DevOp op = make_op(ot, b, full_tangential(B, a, idx, idx, G));
+      DevOp op = make_op(ot, b, face_box(fd.r, idx, idx));
       op.axis = a;
-      op.side = fd.side;
+      op.side = fd.r.side;
       if (fd.type == OSTC_BC_FARFIELD) op.ff = c->d_ff + (size_t)fd.ff * kFarfieldDoubles;
       bc_of_block[b].push_back(op);
       // ghost cells this op reads before the same stage can have written them
       Box rd = fd.type == OSTC_BC_PERIODIC
-                   ? (fd.side < 0 ? full_tangential(B, a, m - 1 - G, m - 2, G) : full_tangential(B, a, 1, G, G))
-                   : full_tangential(B, a, idx, idx, G);
+                   ? (fd.r.side < 0 ? face_box(fd.r, m - 1 - G, m - 2) : face_box(fd.r, 1, G))
+                   : face_box(fd.r, idx, idx);
       for (const Box& rb : minus_interior(rd, B.m)) rim_ops.push_back(make_op(OP_COPY_BV, b, rb));
       if (fd.type == OSTC_BC_FARFIELD) {
         // the farfield state replaces the face points after the fused update
         // computed their properties: recompute them with the ghost layers
-        Box fp;
-        for (int d = 0; d < 3; ++d) { fp.lo[d] = 0; fp.hi[d] = B.m[d] - 1; }
-        fp.lo[a] = fp.hi[a] = idx;
+        Box fp = face_box(fd.r, idx, idx);
+        for (int d = 0; d < 3; ++d) {
+          fp.lo[d] = std::max(fp.lo[d], 0);
+          fp.hi[d] = std::min(fp.hi[d], B.m[d] - 1);
+        }
         props_ops.push_back(make_op(OP_PROPS, b, fp));
       }
     }
@@ -216,97 +244,97 @@ int build_ops(ostc_ctx c) {
   // face of a two-sided interface into
   //  * a bulk part, the face without a frame of ngg+1 cells along its
   //    tangential rims: no other face's op reads or writes these cells, so
-  //    all bulk parts run in one launch; the side with side > 0 averages the
+  //    all bulk parts run in one launch; one side of the pair (the high face
+  //    of a low/high pair, else the face registered first) averages the
   //    shared node layer and recomputes cv in both blocks, the other side
   //    copies its ghost layers only;
   //  * the frame, whose cells neighbouring faces also touch: frame ops keep
   //    the reference's order wherever their cell sets conflict (level =
   //    1 + the highest level of an earlier conflicting frame op).
-  struct FaceGeom { int A, T, a, side; Box box, shA, shT, layT; bool paired; };
+  // Ghost layers read only the target's interior layers beyond its shared
+  // layer, which no inner-face op writes.
+  struct FaceGeom { int A, T; FaceMap M; Box box, shA; int partner; };
   std::vector<FaceGeom> geo;
   for (int b = 0; b < nb; ++b) {
-    HBlock& A = c->blocks[b];
-    for (const InnerDesc& f : A.inner) {
-      HBlock& T = c->blocks[f.target];
-      const int a = f.axis;
-      for (int d = 0; d < 3; ++d)
-        if (d != a && A.m[d] != T.m[d])
-          return fail(c, OSTC_ERR_ARG, "inner face: tangential extents of the two blocks differ");
+    for (const FaceMap& f : c->blocks[b].inner) {
       FaceGeom F;
-      F.A = b; F.T = f.target; F.a = a; F.side = f.side;
+      F.A = b;
+      F.T = f.target;
+      F.M = f;
       for (int d = 0; d < 3; ++d) {
-        F.box.lo[d] = F.shA.lo[d] = F.shT.lo[d] = F.layT.lo[d] = 0;
-        F.box.hi[d] = F.shA.hi[d] = F.shT.hi[d] = F.layT.hi[d] = A.m[d] - 1;
+        F.box.lo[d] = F.shA.lo[d] = f.lo[d];
+        F.box.hi[d] = F.shA.hi[d] = f.hi[d];
       }
-      const int mA = A.m[a], mT = T.m[a];
-      if (f.side > 0) {
-        F.box.lo[a] = mA - 1; F.box.hi[a] = mA - 1 + G;
-        F.shA.lo[a] = F.shA.hi[a] = mA - 1;
-        F.shT.lo[a] = F.shT.hi[a] = 0;
-        F.layT.lo[a] = 0; F.layT.hi[a] = G;
-      } else {
-        F.box.lo[a] = -G; F.box.hi[a] = 0;
-        F.shA.lo[a] = F.shA.hi[a] = 0;
-        F.shT.lo[a] = F.shT.hi[a] = mT - 1;
-        F.layT.lo[a] = mT - 1 - G; F.layT.hi[a] = mT - 1;
-      }
-      F.paired = false;
-      for (const InnerDesc& g : T.inner)
-        if (g.target == b && g.axis == a && g.side == -f.side) F.paired = true;
+      F.shA.lo[f.axis] = F.shA.hi[f.axis] = f.shared;
+      F.partner = -1;
       geo.push_back(F);
       props_ops.push_back(make_op(OP_PROPS, b, F.shA));  // shared layers after averaging
+    }
+  }
+  // the partner of a face: the target's face back to this block over the
+  // same shared nodes
+  for (size_t q = 0; q < geo.size(); ++q) {
+    const Box shT = map_box(geo[q].M, geo[q].shA);
+    for (size_t r = 0; r < geo.size() && geo[q].partner < 0; ++r) {
+      if (r == q || geo[r].A != geo[q].T || geo[r].T != geo[q].A) continue;
+      const Box& o = geo[r].shA;
+      bool same = true;
+      for (int d = 0; d < 3; ++d) same = same && o.lo[d] == shT.lo[d] && o.hi[d] == shT.hi[d];
+      if (same) geo[q].partner = (int)r;
     }
   }
   auto comm_op = [&](const FaceGeom& F, const Box& bx) {
     DevOp op = make_op(OP_COMM, F.A, bx);
     op.target = F.T;
-    op.axis = F.a;
-    op.side = F.side;
-    op.shared = F.side > 0 ? c->blocks[F.A].m[F.a] - 1 : 0;
-    op.tstart = F.side > 0 ? 0 : c->blocks[F.T].m[F.a] - 1 - G;
-    // tstart refers to lo[axis] of the full face box
-    op.tstart += bx.lo[F.a] - F.box.lo[F.a];
+    op.axis = F.M.axis;
+    op.side = F.M.side;
+    op.shared = F.M.shared;
+    for (int d = 0; d < 3; ++d) {
+      op.tcode[d] = F.M.code[d];
+      op.toff[d] = F.M.off[d];
+    }
     return op;
-  };
-  auto restrict_tangential = [](Box b, const Box& to, int a) {
-    for (int d = 0; d < 3; ++d)
-      if (d != a) { b.lo[d] = to.lo[d]; b.hi[d] = to.hi[d]; }
-    return b;
   };
   std::vector<DevOp> plain_ops, bulk_ops;
   struct CommOp { DevOp op; std::vector<Access> w, r; int level; };
   std::vector<CommOp> edge;
-  for (const FaceGeom& F : geo) {
+  for (size_t q = 0; q < geo.size(); ++q) {
+    const FaceGeom& F = geo[q];
     plain_ops.push_back(comm_op(F, F.box));
-    const HBlock& A = c->blocks[F.A];
-    const int W = G + 1;
+    const int a = F.M.axis, W = G + 1;
     Box core = F.box;
     for (int d = 0; d < 3; ++d)
-      if (d != F.a) { core.lo[d] = W; core.hi[d] = A.m[d] - 1 - W; }
+      if (d != a) { core.lo[d] = F.box.lo[d] + W; core.hi[d] = F.box.hi[d] - W; }
     std::vector<Box> frame;
-    if (F.paired && box_ok(core)) {
+    if (F.partner >= 0 && box_ok(core)) {
+      const FaceGeom& P = geo[F.partner];
+      const bool averages = F.M.side != P.M.side ? F.M.side > 0 : (int)q < F.partner;
       Box cb = core;
-      if (F.side < 0) cb.hi[F.a] = -1;  // ghost layers only; the partner averages
+      if (!averages) {  // ghost layers only; the partner averages
+        if (F.M.side < 0) cb.hi[a] = F.M.shared - 1;
+        else cb.lo[a] = F.M.shared + 1;
+      }
       DevOp op = comm_op(F, cb);
-      if (F.side > 0) op.flags = 1;     // recompute the target block's cv on the shared layer
+      if (averages) op.flags = 1;  // recompute the target block's cv on the shared layer
       bulk_ops.push_back(op);
       // frame = box minus core, as slabs along the first tangential axis
       // and then the second
-      const int t1 = F.a == 0 ? 1 : 0, t2 = F.a == 2 ? 1 : 2;
+      const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
       Box r = F.box;
-      Box p = r; p.hi[t1] = W - 1; frame.push_back(p);
-      p = r; p.lo[t1] = A.m[t1] - W; frame.push_back(p);
-      r.lo[t1] = W; r.hi[t1] = A.m[t1] - 1 - W;
-      p = r; p.hi[t2] = W - 1; frame.push_back(p);
-      p = r; p.lo[t2] = A.m[t2] - W; frame.push_back(p);
+      Box p = r; p.hi[t1] = F.box.lo[t1] + W - 1; frame.push_back(p);
+      p = r; p.lo[t1] = F.box.hi[t1] - W + 1; frame.push_back(p);
+      r.lo[t1] = core.lo[t1]; r.hi[t1] = core.hi[t1];
+      p = r; p.hi[t2] = F.box.lo[t2] + W - 1; frame.push_back(p);
+      p = r; p.lo[t2] = F.box.hi[t2] - W + 1; frame.push_back(p);
     } else {
       frame.push_back(F.box);
     }
     for (const Box& eb : frame) {
       CommOp co;
       co.op = comm_op(F, eb);
-      co.w = {{F.A, eb}, {F.T, restrict_tangential(F.shT, eb, F.a)}};
-      co.r = {{F.T, restrict_tangential(F.layT, eb, F.a)}, {F.A, restrict_tangential(F.shA, eb, F.a)}};
+      const Box sh = box_and(F.shA, eb);
+      co.w = {{F.A, eb}, {F.T, map_box(F.M, sh)}};
+      co.r = {{F.T, map_box(F.M, eb)}, {F.A, sh}};
       co.level = 0;
       for (auto& e : edge) {
         auto meet = [](const std::vector<Access>& x, const std::vector<Access>& y) {
@@ -669,30 +697,54 @@ int ostc_add_block(ostc_ctx c, const ostc_block_desc* d, int* id) {
   return OSTC_OK;
 }
 
-int ostc_add_boundary(ostc_ctx c, int block, int axis, int side, int type) {
+// Registers a boundary face of `type` over the points of `r`; `ff` is the
+// far-field state of type 4.
+static int add_face(ostc_ctx c, int block, const FaceRange& r, int type, const ostc_farfield* ff) {
+  if (type == OSTC_BC_FARFIELD) {
+    if (!ff) return fail(c, OSTC_ERR_ARG, "farfield faces need their state");
+    if (!(ff->density > 0) || !(ff->pressure > 0) || !(ff->acoustic_speed > 0) || !(ff->specific_heat_ratio > 1) ||
+        !(ff->entropy > 0))
+      return fail(c, OSTC_ERR_ARG, "far-field state: density, pressure, acoustic speed, entropy > 0 and gamma > 1");
+    c->blocks[block].faces.push_back({OSTC_BC_FARFIELD, (int)c->farfields.size(), r});
+    c->farfields.push_back(*ff);
+    return OSTC_OK;
+  }
+  if (type != OSTC_BC_OUTFLOW && type != OSTC_BC_PERIODIC)
+    return fail(c, OSTC_ERR_UNSUPPORTED,
+                "boundary label " + std::to_string(type) + " not implemented (farfield 4, outflow 6, periodic 8 are)");
+  c->blocks[block].faces.push_back({type, -1, r});
+  return OSTC_OK;
+}
+
+static int check_face_args(ostc_ctx c, int block, int axis, int side) {
   if (!c) return OSTC_ERR_ARG;
   if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_finalize");
   if (block < 0 || block >= (int)c->blocks.size() || axis < 0 || axis > 2 || (side != -1 && side != 1))
     return fail(c, OSTC_ERR_ARG, "bad boundary face");
-  if (type == OSTC_BC_FARFIELD)
-    return fail(c, OSTC_ERR_ARG, "farfield faces need their state: use ostc_add_farfield");
-  if (type != OSTC_BC_OUTFLOW && type != OSTC_BC_PERIODIC)
-    return fail(c, OSTC_ERR_UNSUPPORTED, "boundary type not implemented (farfield 4, outflow 6, periodic 8 are)");
-  c->blocks[block].faces.push_back({axis, side, type, -1});
   return OSTC_OK;
 }
 
+int ostc_add_boundary(ostc_ctx c, int block, int axis, int side, int type) {
+  if (int r = check_face_args(c, block, axis, side)) return r;
+  if (type == OSTC_BC_FARFIELD)
+    return fail(c, OSTC_ERR_ARG, "farfield faces need their state: use ostc_add_farfield");
+  return add_face(c, block, whole_face(axis, side, c->blocks[block].m, c->ngg), type, nullptr);
+}
+
 int ostc_add_farfield(ostc_ctx c, int block, int axis, int side, const ostc_farfield* ff) {
+  if (int r = check_face_args(c, block, axis, side)) return r;
+  if (!ff) return fail(c, OSTC_ERR_ARG, "bad farfield face");
+  return add_face(c, block, whole_face(axis, side, c->blocks[block].m, c->ngg), OSTC_BC_FARFIELD, ff);
+}
+
+int ostc_add_boundary_range(ostc_ctx c, int block, const int range[6], int type, const ostc_farfield* ff) {
   if (!c) return OSTC_ERR_ARG;
   if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_finalize");
-  if (block < 0 || block >= (int)c->blocks.size() || axis < 0 || axis > 2 || (side != -1 && side != 1) || !ff)
-    return fail(c, OSTC_ERR_ARG, "bad farfield face");
-  if (!(ff->density > 0) || !(ff->pressure > 0) || !(ff->acoustic_speed > 0) || !(ff->specific_heat_ratio > 1) ||
-      !(ff->entropy > 0))
-    return fail(c, OSTC_ERR_ARG, "far-field state: density, pressure, acoustic speed, entropy > 0 and gamma > 1");
-  c->blocks[block].faces.push_back({axis, side, OSTC_BC_FARFIELD, (int)c->farfields.size()});
-  c->farfields.push_back(*ff);
-  return OSTC_OK;
+  if (block < 0 || block >= (int)c->blocks.size() || !range) return fail(c, OSTC_ERR_ARG, "bad boundary face");
+  FaceRange r{};
+  std::string err;
+  if (!boundary_range(range, c->blocks[block].m, c->ngg, &r, &err)) return fail(c, OSTC_ERR_ARG, err);
+  return add_face(c, block, r, type, ff);
 }
 
 // FarField::FarField for air (n_spec = 0, laminar), src/BoundCond.cpp:586-700.
@@ -741,7 +793,26 @@ int ostc_add_inner_face(ostc_ctx c, int block, int axis, int side, int target) {
   const int nb = (int)c->blocks.size();
   if (block < 0 || block >= nb || target < 0 || target >= nb || axis < 0 || axis > 2 || (side != -1 && side != 1))
     return fail(c, OSTC_ERR_ARG, "bad inner face");
-  c->blocks[block].inner.push_back({axis, side, target});
+  const HBlock &A = c->blocks[block], &T = c->blocks[target];
+  for (int d = 0; d < 3; ++d)
+    if (d != axis && A.m[d] != T.m[d])
+      return fail(c, OSTC_ERR_ARG, "inner face: tangential extents of the two blocks differ");
+  if (T.m[axis] < c->ngg + 1) return fail(c, OSTC_ERR_ARG, "inner face: target block thinner than ngg+1 layers");
+  c->blocks[block].inner.push_back(aligned_face(axis, side, A.m, T.m, c->ngg, target));
+  return OSTC_OK;
+}
+
+int ostc_add_inner_face_range(ostc_ctx c, int block, const int range[6], const int target_range[6], int target) {
+  if (!c) return OSTC_ERR_ARG;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_finalize");
+  const int nb = (int)c->blocks.size();
+  if (block < 0 || block >= nb || target < 0 || target >= nb || !range || !target_range)
+    return fail(c, OSTC_ERR_ARG, "bad inner face");
+  FaceMap m{};
+  std::string err;
+  if (!inner_face_map(range, target_range, c->blocks[block].m, c->blocks[target].m, c->ngg, target, &m, &err))
+    return fail(c, OSTC_ERR_ARG, err);
+  c->blocks[block].inner.push_back(m);
   return OSTC_OK;
 }
 
@@ -1479,6 +1550,47 @@ int ostc_util_uniform_geometry(int mx, int my, int mz, int g, double x0, double 
         const double dxd1 = (X(i + 1) - X(i - 1)) * 0.5, dyd1 = (Y(j) - Y(j)) * 0.5, dzd1 = (Z(k) - Z(k)) * 0.5;
         const double dxd2 = (X(i) - X(i)) * 0.5, dyd2 = (Y(j + 1) - Y(j - 1)) * 0.5, dzd2 = (Z(k) - Z(k)) * 0.5;
         const double dxd3 = (X(i) - X(i)) * 0.5, dyd3 = (Y(j) - Y(j)) * 0.5, dzd3 = (Z(k + 1) - Z(k - 1)) * 0.5;
+        const double J = dxd1 * dyd2 * dzd3 + dxd2 * dyd3 * dzd1 + dxd3 * dyd1 * dzd2 - dxd1 * dyd3 * dzd2 -
+                         dxd2 * dyd1 * dzd3 - dxd3 * dyd2 * dzd1;
+        if (!(J > 0)) return OSTC_ERR_NUMERIC;
+        const long long p = ((long long)(k + g) * sy + (j + g)) * sx + (i + g);
+        jac[p] = J;
+        metric[p + 0 * sz] = (dyd2 * dzd3 - dyd3 * dzd2) / J;
+        metric[p + 1 * sz] = (dxd3 * dzd2 - dxd2 * dzd3) / J;
+        metric[p + 2 * sz] = (dxd2 * dyd3 - dxd3 * dyd2) / J;
+        metric[p + 3 * sz] = (dyd3 * dzd1 - dyd1 * dzd3) / J;
+        metric[p + 4 * sz] = (dxd1 * dzd3 - dxd3 * dzd1) / J;
+        metric[p + 5 * sz] = (dxd3 * dyd1 - dxd1 * dyd3) / J;
+        metric[p + 6 * sz] = (dyd1 * dzd2 - dyd2 * dzd1) / J;
+        metric[p + 7 * sz] = (dxd2 * dzd1 - dxd1 * dzd2) / J;
+        metric[p + 8 * sz] = (dxd1 * dyd2 - dxd2 * dyd1) / J;
+      }
+  return OSTC_OK;
+}
+
+// Block::compute_jac_metric (src/Mesh.cpp:241-502) from node coordinates:
+// x, y, z on the domain padded by g+1 ghost layers (the half-central
+// differences of the outermost ghost layer read one layer beyond it), jac and
+// metric SoA on the domain padded by g.
+int ostc_util_metric(int mx, int my, int mz, int g, const double* x, const double* y, const double* z, double* jac,
+                     double* metric) {
+  if (mx < 2 || my < 2 || mz < 2 || g < 1 || !x || !y || !z || !jac || !metric) return OSTC_ERR_ARG;
+  const long long sx = mx + 2 * g, sy = my + 2 * g, sz = sx * sy * (mz + 2 * g);
+  const int G = g + 1;
+  const long long cx = mx + 2 * G, cy = my + 2 * G;
+  auto at = [&](const double* a, int i, int j, int k) { return a[((long long)(k + G) * cy + (j + G)) * cx + (i + G)]; };
+  for (int k = -g; k < mz + g; ++k)
+    for (int j = -g; j < my + g; ++j)
+      for (int i = -g; i < mx + g; ++i) {
+        const double dxd1 = (at(x, i + 1, j, k) - at(x, i - 1, j, k)) * 0.5,
+                     dyd1 = (at(y, i + 1, j, k) - at(y, i - 1, j, k)) * 0.5,
+                     dzd1 = (at(z, i + 1, j, k) - at(z, i - 1, j, k)) * 0.5;
+        const double dxd2 = (at(x, i, j + 1, k) - at(x, i, j - 1, k)) * 0.5,
+                     dyd2 = (at(y, i, j + 1, k) - at(y, i, j - 1, k)) * 0.5,
+                     dzd2 = (at(z, i, j + 1, k) - at(z, i, j - 1, k)) * 0.5;
+        const double dxd3 = (at(x, i, j, k + 1) - at(x, i, j, k - 1)) * 0.5,
+                     dyd3 = (at(y, i, j, k + 1) - at(y, i, j, k - 1)) * 0.5,
+                     dzd3 = (at(z, i, j, k + 1) - at(z, i, j, k - 1)) * 0.5;
         const double J = dxd1 * dyd2 * dzd3 + dxd2 * dyd3 * dzd1 + dxd3 * dyd1 * dzd2 - dxd1 * dyd3 * dzd2 -
                          dxd2 * dyd1 * dzd3 - dxd3 * dyd2 * dzd1;
         if (!(J > 0)) return OSTC_ERR_NUMERIC;

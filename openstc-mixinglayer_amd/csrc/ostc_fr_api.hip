@@ -12,20 +12,27 @@
 #include <vector>
 
 #include "openstc/abi.h"
+#include "ostc_faces.hpp"
 #include "ostc_fr.hpp"
 
 using namespace ostc::fr;
+using ostc::FaceMap;
+using ostc::FaceRange;
 
 namespace {
 
-struct FaceDesc { int axis, side, type, ff; };  // ff: index of the far-field state (type 4)
-struct InnerDesc { int axis, side, target; };
+struct FaceDesc {
+  int type, ff;  // ff: index of the far-field state (type 4)
+  FaceRange r;   // the face layer and its tangential range widened by ngg
+};
 
 struct HBlock {
   int mx, my, mz;
   FrBlockDev d{};
   std::vector<FaceDesc> faces;
-  std::vector<InnerDesc> inner;
+  std::vector<FaceMap> inner;
+  int e[3];  // mx, my, mz
+  const int* ext() const { return e; }
 };
 
 constexpr int kResidualBlocks = 256;                  // workgroups of the residual reduction
@@ -123,9 +130,16 @@ void apply_bc(ostc_fr_ctx c, int b, long long* n) {
   std::stable_sort(fs.begin(), fs.end(),
                    [](const FaceDesc& x, const FaceDesc& y) { return bc_priority(x.type) < bc_priority(y.type); });
   const FrArgs a = args(c, b);
-  for (const FaceDesc& f : fs)
-    launch_bc(a, c->ns, f.axis, f.side, f.type, f.type == OSTC_BC_FARFIELD ? c->d_ff + (size_t)f.ff * 26 : nullptr,
-              c->st, n);
+  for (const FaceDesc& f : fs) {
+    FaceBox fb{};
+    for (int d = 0; d < 3; ++d) {
+      fb.lo[d] = f.r.lo[d];
+      fb.n[d] = f.r.hi[d] - f.r.lo[d] + 1;
+    }
+    fb.axis = f.r.axis;
+    fb.side = f.r.side;
+    launch_bc(a, c->ns, fb, f.type, f.type == OSTC_BC_FARFIELD ? c->d_ff + (size_t)f.ff * 26 : nullptr, c->st, n);
+  }
 }
 
 // data_communication, inner faces in block order then face order
@@ -133,8 +147,18 @@ void apply_bc(ostc_fr_ctx c, int b, long long* n) {
 void exchange(ostc_fr_ctx c, long long* n) {
   for (size_t b = 0; b < c->blocks.size(); ++b) {
     const FrArgs a = args(c, (int)b);
-    for (const InnerDesc& f : c->blocks[b].inner)
-      launch_comm(a, c->blocks[f.target].d, c->ns, f.axis, f.side, c->st, n);
+    for (const FaceMap& f : c->blocks[b].inner) {
+      CommMap m{};
+      for (int d = 0; d < 3; ++d) {
+        m.lo[d] = f.lo[d];
+        m.n[d] = f.hi[d] - f.lo[d] + 1;
+        m.code[d] = f.code[d];
+        m.off[d] = f.off[d];
+      }
+      m.axis = f.axis;
+      m.shared = f.shared;
+      launch_comm(a, c->blocks[f.target].d, c->ns, m, c->st, n);
+    }
   }
 }
 
@@ -274,11 +298,32 @@ int ostc_fr_add_block(ostc_fr_ctx c, const ostc_block_desc* d, int* id) {
   if (d->mz < 2) return fail(c, OSTC_ERR_UNSUPPORTED, "two-dimensional blocks are not supported");
   if (d->mx < 2 || d->my < 2) return fail(c, OSTC_ERR_ARG, "block too small");
   HBlock h;
-  h.mx = d->mx;
-  h.my = d->my;
-  h.mz = d->mz;
+  h.mx = h.e[0] = d->mx;
+  h.my = h.e[1] = d->my;
+  h.mz = h.e[2] = d->mz;
   c->blocks.push_back(h);
   if (id) *id = (int)c->blocks.size() - 1;
+  return OSTC_OK;
+}
+
+// Registers a boundary face of `type` over the points of `r`; `ff` is the
+// far-field state of type 4.
+static int add_face(ostc_fr_ctx c, int b, const FaceRange& r, int type, const ostc_farfield* ff) {
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_fr_finalize");
+  if (type == OSTC_BC_FARFIELD) {
+    if (!ff) return fail(c, OSTC_ERR_ARG, "farfield faces need their state: use ostc_fr_add_farfield");
+    if (!(ff->density > 0) || !(ff->pressure > 0) || !(ff->acoustic_speed > 0) ||
+        !(ff->specific_heat_ratio > 1) || !(ff->entropy > 0) || !(ff->mw > 0))
+      return fail(c, OSTC_ERR_ARG,
+                  "far-field state: density, pressure, acoustic speed, entropy, mw > 0, gamma > 1");
+    c->blocks[b].faces.push_back({OSTC_BC_FARFIELD, (int)c->farfields.size(), r});
+    c->farfields.push_back(*ff);
+    return OSTC_OK;
+  }
+  if (type != OSTC_BC_OUTFLOW && type != OSTC_BC_PERIODIC)
+    return fail(c, OSTC_ERR_UNSUPPORTED, "mixture path: boundary label " + std::to_string(type) +
+                                             " not implemented (farfield 4, outflow 6, periodic 8 are)");
+  c->blocks[b].faces.push_back({type, -1, r});
   return OSTC_OK;
 }
 
@@ -288,24 +333,24 @@ int ostc_fr_add_boundary(ostc_fr_ctx c, int b, int axis, int side, int type) {
   if (axis < 0 || axis > 2 || (side != -1 && side != 1)) return fail(c, OSTC_ERR_ARG, "bad face");
   if (type == OSTC_BC_FARFIELD)
     return fail(c, OSTC_ERR_ARG, "farfield faces need their state: use ostc_fr_add_farfield");
-  if (type != OSTC_BC_OUTFLOW && type != OSTC_BC_PERIODIC)
-    return fail(c, OSTC_ERR_UNSUPPORTED, "mixture path: farfield, outflow and periodic faces only");
-  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_fr_finalize");
-  c->blocks[b].faces.push_back({axis, side, type, -1});
-  return OSTC_OK;
+  return add_face(c, b, ostc::whole_face(axis, side, c->blocks[b].ext(), c->ngg), type, nullptr);
 }
 
 int ostc_fr_add_farfield(ostc_fr_ctx c, int b, int axis, int side, const ostc_farfield* ff) {
   if (!c) return OSTC_ERR_ARG;
   if (int r = check_block(c, b)) return r;
-  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_fr_finalize");
   if (axis < 0 || axis > 2 || (side != -1 && side != 1) || !ff) return fail(c, OSTC_ERR_ARG, "bad face");
-  if (!(ff->density > 0) || !(ff->pressure > 0) || !(ff->acoustic_speed > 0) || !(ff->specific_heat_ratio > 1) ||
-      !(ff->entropy > 0) || !(ff->mw > 0))
-    return fail(c, OSTC_ERR_ARG, "far-field state: density, pressure, acoustic speed, entropy, mw > 0, gamma > 1");
-  c->blocks[b].faces.push_back({axis, side, OSTC_BC_FARFIELD, (int)c->farfields.size()});
-  c->farfields.push_back(*ff);
-  return OSTC_OK;
+  return add_face(c, b, ostc::whole_face(axis, side, c->blocks[b].ext(), c->ngg), OSTC_BC_FARFIELD, ff);
+}
+
+int ostc_fr_add_boundary_range(ostc_fr_ctx c, int b, const int range[6], int type, const ostc_farfield* ff) {
+  if (!c) return OSTC_ERR_ARG;
+  if (int r = check_block(c, b)) return r;
+  if (!range) return fail(c, OSTC_ERR_ARG, "bad face");
+  FaceRange fr{};
+  std::string err;
+  if (!ostc::boundary_range(range, c->blocks[b].ext(), c->ngg, &fr, &err)) return fail(c, OSTC_ERR_ARG, err);
+  return add_face(c, b, fr, type, ff);
 }
 
 // compute_viscosity (host), src/Transport.cu:11-41
@@ -416,8 +461,28 @@ int ostc_fr_add_inner_face(ostc_fr_ctx c, int b, int axis, int side, int target)
   if (!c) return OSTC_ERR_ARG;
   if (int r = check_block(c, b)) return r;
   if (int r = check_block(c, target)) return r;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_fr_finalize");
   if (axis < 0 || axis > 2 || (side != -1 && side != 1)) return fail(c, OSTC_ERR_ARG, "bad face");
-  c->blocks[b].inner.push_back({axis, side, target});
+  const HBlock &h = c->blocks[b], &t = c->blocks[target];
+  for (int a = 0; a < 3; ++a)
+    if (a != axis && h.ext()[a] != t.ext()[a]) return fail(c, OSTC_ERR_ARG, "inner face is not 1-to-1");
+  if (t.ext()[axis] < c->ngg + 1) return fail(c, OSTC_ERR_ARG, "target block thinner than ngg+1 layers");
+  c->blocks[b].inner.push_back(ostc::aligned_face(axis, side, h.ext(), t.ext(), c->ngg, target));
+  return OSTC_OK;
+}
+
+int ostc_fr_add_inner_face_range(ostc_fr_ctx c, int b, const int range[6], const int target_range[6], int target) {
+  if (!c) return OSTC_ERR_ARG;
+  if (int r = check_block(c, b)) return r;
+  if (int r = check_block(c, target)) return r;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_fr_finalize");
+  if (!range || !target_range) return fail(c, OSTC_ERR_ARG, "bad face");
+  FaceMap m{};
+  std::string err;
+  if (!ostc::inner_face_map(range, target_range, c->blocks[b].ext(), c->blocks[target].ext(), c->ngg, target, &m,
+                            &err))
+    return fail(c, OSTC_ERR_ARG, err);
+  c->blocks[b].inner.push_back(m);
   return OSTC_OK;
 }
 
@@ -425,15 +490,6 @@ int ostc_fr_finalize(ostc_fr_ctx c) {
   if (!c) return OSTC_ERR_ARG;
   if (c->finalized) return OSTC_OK;
   if (c->blocks.empty()) return fail(c, OSTC_ERR_STATE, "no blocks");
-  // inner faces must be conforming with the target's opposite face
-  for (auto& h : c->blocks)
-    for (auto& f : h.inner) {
-      const HBlock& t = c->blocks[f.target];
-      const int e[3] = {h.mx, h.my, h.mz}, te[3] = {t.mx, t.my, t.mz};
-      for (int a = 0; a < 3; ++a)
-        if (a != f.axis && e[a] != te[a]) return fail(c, OSTC_ERR_ARG, "inner face is not 1-to-1");
-      if (te[f.axis] < c->ngg + 1) return fail(c, OSTC_ERR_ARG, "target block thinner than ngg+1 layers");
-    }
   FR_HIP(c, hipMalloc((void**)&c->d_tab, sizeof(Tab)));
   FR_HIP(c, hipMemcpyAsync(c->d_tab, &c->tab, sizeof(Tab), hipMemcpyHostToDevice, c->st));
   {

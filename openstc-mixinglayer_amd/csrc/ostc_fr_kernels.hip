@@ -392,17 +392,16 @@ __device__ void farfield_point(const FrArgs& A, int axis, int side, const double
 }
 
 template <int NS>
-__global__ __launch_bounds__(kT) void k_bc(FrArgs A, int axis, int side, int type, const double* ff) {
+__global__ __launch_bounds__(kT) void k_bc(FrArgs A, FaceBox fb, int type, const double* ff) {
   const FrBlockDev& B = A.B;
   const int ext[3] = {B.mx, B.my, B.mz};
-  const int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
-  const int n1 = ext[o1] + 2 * B.g, n2 = ext[o2] + 2 * B.g;
+  const int axis = fb.axis, side = fb.side;
   const int q = blockIdx.x * kT + threadIdx.x;
-  if (q >= n1 * n2) return;
+  if (q >= fb.n[0] * fb.n[1] * fb.n[2]) return;
   int ix[3];
-  ix[axis] = side < 0 ? 0 : ext[axis] - 1;
-  ix[o1] = q % n1 - B.g;
-  ix[o2] = q / n1 - B.g;
+  ix[0] = fb.lo[0] + q % fb.n[0];
+  ix[1] = fb.lo[1] + (q / fb.n[0]) % fb.n[1];
+  ix[2] = fb.lo[2] + q / (fb.n[0] * fb.n[1]);
   if (type == 4) {
     farfield_point<NS>(A, axis, side, ff, ix);
     return;
@@ -423,27 +422,28 @@ __global__ __launch_bounds__(kT) void k_bc(FrArgs A, int axis, int side, int typ
   }
 }
 
-// inner_communication with scalars, src/DataCommunication.cuh:60-105, for an
-// aligned 1-to-1 face: the shared node layer and ngg ghost layers over the
-// interior tangential extent.
+// inner_communication with scalars, src/DataCommunication.cuh:60-105: the
+// shared node layer takes the mean of both blocks (in both), the ghost layers
+// the target's values.
 template <int NS>
-__global__ __launch_bounds__(kT) void k_comm(FrArgs A, FrBlockDev Tg, int axis, int side) {
+__global__ __launch_bounds__(kT) void k_comm(FrArgs A, FrBlockDev Tg, CommMap m) {
   const FrBlockDev& B = A.B;
-  const int ext[3] = {B.mx, B.my, B.mz}, text[3] = {Tg.mx, Tg.my, Tg.mz};
-  const int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
-  const int n1 = ext[o1], n2 = ext[o2];
   const long long q = (long long)blockIdx.x * kT + threadIdx.x;
-  if (q >= (long long)n1 * n2 * (B.g + 1)) return;
-  const int layer = (int)(q / ((long long)n1 * n2));
-  const int r = (int)(q % ((long long)n1 * n2));
-  int ix[3], tx[3];
-  ix[axis] = (side < 0 ? 0 : ext[axis] - 1) + side * layer;
-  tx[axis] = (side < 0 ? text[axis] - 1 : 0) + side * layer;
-  ix[o1] = tx[o1] = r % n1;
-  ix[o2] = tx[o2] = r / n1;
-  const long long p = pid(B, ix[0], ix[1], ix[2]), pt = pid(Tg, tx[0], tx[1], tx[2]);
+  if (q >= (long long)m.n[0] * m.n[1] * m.n[2]) return;
+  const int i = m.lo[0] + (int)(q % m.n[0]);
+  const int j = m.lo[1] + (int)((q / m.n[0]) % m.n[1]);
+  const int k = m.lo[2] + (int)(q / ((long long)m.n[0] * m.n[1]));
+  auto src_index = [&](int code) {
+    const int a = (code < 0 ? -code : code) - 1;
+    const int v = a == 0 ? i : (a == 1 ? j : k);
+    return code < 0 ? -v : v;
+  };
+  const int tx0 = m.off[0] + src_index(m.code[0]), tx1 = m.off[1] + src_index(m.code[1]),
+            tx2 = m.off[2] + src_index(m.code[2]);
+  const bool shared = (m.axis == 0 ? i : (m.axis == 1 ? j : k)) == m.shared;
+  const long long p = pid(B, i, j, k), pt = pid(Tg, tx0, tx1, tx2);
   const long long s = B.sz, ts = Tg.sz;
-  if (layer == 0) {
+  if (shared) {
 #pragma unroll
     for (int l = 0; l < 6; ++l) {
       const double ave{0.5 * (Tg.bv[pt + l * ts] + B.bv[p + l * s])};
@@ -1255,18 +1255,16 @@ void launch_dt_set(double* dt, double v, hipStream_t st, long long* n) {
   ++*n;
 }
 
-void launch_bc(const FrArgs& A, int ns, int axis, int side, int type, const double* ff, hipStream_t st,
+void launch_bc(const FrArgs& A, int ns, const FaceBox& fb, int type, const double* ff, hipStream_t st,
                long long* n) {
-  const int ext[3] = {A.B.mx, A.B.my, A.B.mz};
-  const long long cnt = (long long)(ext[(axis + 1) % 3] + 2 * A.B.g) * (ext[(axis + 2) % 3] + 2 * A.B.g);
-  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_bc<NS>, dim3(nblk(cnt)), dim3(kT), 0, st, A, axis, side, type, ff))
+  const long long cnt = (long long)fb.n[0] * fb.n[1] * fb.n[2];
+  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_bc<NS>, dim3(nblk(cnt)), dim3(kT), 0, st, A, fb, type, ff))
   ++*n;
 }
 
-void launch_comm(const FrArgs& A, const FrBlockDev& Tg, int ns, int axis, int side, hipStream_t st, long long* n) {
-  const int ext[3] = {A.B.mx, A.B.my, A.B.mz};
-  const long long cnt = (long long)ext[(axis + 1) % 3] * ext[(axis + 2) % 3] * (A.B.g + 1);
-  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_comm<NS>, dim3(nblk(cnt)), dim3(kT), 0, st, A, Tg, axis, side))
+void launch_comm(const FrArgs& A, const FrBlockDev& Tg, int ns, const CommMap& m, hipStream_t st, long long* n) {
+  const long long cnt = (long long)m.n[0] * m.n[1] * m.n[2];
+  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_comm<NS>, dim3(nblk(cnt)), dim3(kT), 0, st, A, Tg, m))
   ++*n;
 }
 

@@ -29,7 +29,8 @@ struct DevOp {
   int lo[3];           // box origin (block indices, ghosts negative)
   int n[3];            // box extents
   int shared;          // OP_COMM: index along axis of the shared node layer
-  int tstart;          // OP_COMM: target index matching lo[axis]
+  int tcode[3];        // OP_COMM: target index along d = toff[d] + sgn * (index along p), tcode[d] = sgn * (p + 1)
+  int toff[3];
   int flags;           // OP_COMM bit 0: also recompute the target's cv on the shared layer
   long long begin;     // first point of this op in its launch range
   long long count;     // n[0]*n[1]*n[2]
@@ -146,10 +147,14 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
     case OP_COMM: {
       const DevBlock& T = blocks[op.target];
       const int idx_a = ax == 0 ? i : (ax == 1 ? j : k);
-      const int lo_a = ax == 0 ? op.lo[0] : (ax == 1 ? op.lo[1] : op.lo[2]);
-      const int ta = op.tstart + (idx_a - lo_a);
+      auto src_index = [&](int code) {
+        const int q = (code < 0 ? -code : code) - 1;
+        const int v = q == 0 ? i : (q == 1 ? j : k);
+        return code < 0 ? -v : v;
+      };
       const long long p = pidx(B, i, j, k);
-      const long long pt = pidx(T, ax == 0 ? ta : i, ax == 1 ? ta : j, ax == 2 ? ta : k);
+      const long long pt = pidx(T, op.toff[0] + src_index(op.tcode[0]), op.toff[1] + src_index(op.tcode[1]),
+                                op.toff[2] + src_index(op.tcode[2]));
       double* tbv = T.bv[slot];
       const long long ts = T.sz;
       if (idx_a == op.shared) {

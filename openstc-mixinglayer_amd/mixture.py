@@ -82,6 +82,8 @@ SIGNATURES = {
     "ostc_fr_add_boundary": (_I, [_P, _I, _I, _I, _I]),
     "ostc_fr_add_inner_face": (_I, [_P, _I, _I, _I, _I]),
     "ostc_fr_add_farfield": (_I, [_P, _I, _I, _I, C.POINTER(_pkg().Farfield)]),
+    "ostc_fr_add_boundary_range": (_I, [_P, _I, C.POINTER(_I), _I, C.POINTER(_pkg().Farfield)]),
+    "ostc_fr_add_inner_face_range": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), _I]),
     "ostc_fr_ngg": (_I, [_P, C.POINTER(_I)]),
     "ostc_fr_store_last_step": (_I, [_P]),
     "ostc_fr_residual": (_I, [_P, _DP]),
@@ -166,14 +168,22 @@ class FrSolver:
         self.ngg = g.value
         for b in case["blocks"]:
             self._chk(L.ostc_fr_add_block(self.ctx, C.byref(_BlockDesc(b["mx"], b["my"], b["mz"])), None))
+        ints = _pkg().ints
         for f in case["boundaries"]:
-            if f["type"] == 4:
-                ff = farfield_state(self.T, f["farfield_y"], **f["farfield"])
+            ff = farfield_state(self.T, f["farfield_y"], **f["farfield"]) if f["type"] == 4 else None
+            if "range" in f:  # a boundary N.txt record
+                self._chk(L.ostc_fr_add_boundary_range(self.ctx, f["block"], ints(f["range"]), f["type"],
+                                                       None if ff is None else C.byref(ff)))
+            elif ff is not None:
                 self._chk(L.ostc_fr_add_farfield(self.ctx, f["block"], f["axis"], f["side"], C.byref(ff)))
             else:
                 self._chk(L.ostc_fr_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"]))
         for f in case["inner_faces"]:
-            self._chk(L.ostc_fr_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"]))
+            if "range" in f:  # an inner N.txt line pair
+                self._chk(L.ostc_fr_add_inner_face_range(self.ctx, f["block"], ints(f["range"]),
+                                                         ints(f["target_range"]), f["target"]))
+            else:
+                self._chk(L.ostc_fr_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"]))
         self._chk(L.ostc_fr_finalize(self.ctx))
 
     def _chk(self, r):

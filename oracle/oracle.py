@@ -93,6 +93,8 @@ def lib(tangent: bool = False):
         L.orc_ngg.argtypes = [vp]
         L.orc_add_boundary.argtypes = [vp, i, i, i, i]
         L.orc_add_inner_face.argtypes = [vp, i, i, i, i]
+        L.orc_add_boundary_range.argtypes = [vp, i, C.POINTER(C.c_int), i, C.c_void_p]
+        L.orc_add_inner_face_range.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), i]
         L.orc_add_farfield.argtypes = [vp, i, i, i, C.POINTER(Farfield)]
         L.orc_farfield_state_air.argtypes = [C.POINTER(FarfieldSpec), C.POINTER(Farfield)]
         L.orc_array.restype = dp
@@ -128,6 +130,11 @@ def lib(tangent: bool = False):
     return _libs[tangent]
 
 
+def _ints(v):
+    """A C int[6] of a connectivity record."""
+    return (C.c_int * len(v))(*[int(x) for x in v])
+
+
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
@@ -151,13 +158,24 @@ class Oracle:
                 raise ValueError("bad block")
             self.nblocks += 1
         for f in case["boundaries"]:
-            if f["type"] == 4:
-                ff = farfield_state_air(**f["farfield"])
-                L.orc_add_farfield(self.ctx, f["block"], f["axis"], f["side"], C.byref(ff))
+            ff = farfield_state_air(**f["farfield"]) if f["type"] == 4 else None
+            if "range" in f:
+                r = L.orc_add_boundary_range(self.ctx, f["block"], _ints(f["range"]), f["type"],
+                                             None if ff is None else C.cast(C.byref(ff), C.c_void_p))
+            elif ff is not None:
+                r = L.orc_add_farfield(self.ctx, f["block"], f["axis"], f["side"], C.byref(ff))
             else:
-                L.orc_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"])
+                r = L.orc_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"])
+            if r != 0:
+                raise ValueError(f"bad boundary face {f}")
         for f in case["inner_faces"]:
-            L.orc_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"])
+            if "range" in f:
+                r = L.orc_add_inner_face_range(self.ctx, f["block"], _ints(f["range"]), _ints(f["target_range"]),
+                                               f["target"])
+            else:
+                r = L.orc_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"])
+            if r != 0:
+                raise ValueError(f"bad inner face {f}")
 
     def __del__(self):
         try:

@@ -49,6 +49,8 @@ def lib():
         L.orf_ngg.argtypes = [vp]
         L.orf_add_boundary.argtypes = [vp, i, i, i, i]
         L.orf_add_inner_face.argtypes = [vp, i, i, i, i]
+        L.orf_add_inner_face_range.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), i]
+        L.orf_add_boundary_range.argtypes = [vp, i, C.POINTER(C.c_int), i, C.c_void_p]
         L.orf_array.restype = dp
         L.orf_array.argtypes = [vp, i, i, C.POINTER(C.c_longlong)]
         L.orf_initialize.argtypes = [vp]
@@ -200,13 +202,24 @@ class FrOracle:
         for b in case["blocks"]:
             L.orf_add_block(self.ctx, b["mx"], b["my"], b["mz"])
         for f in case["boundaries"]:
-            if f["type"] == 4:
-                ff = farfield_state(self.T, f["farfield_y"], **f["farfield"])
-                L.orf_add_farfield(self.ctx, f["block"], f["axis"], f["side"], C.byref(ff))
+            ff = farfield_state(self.T, f["farfield_y"], **f["farfield"]) if f["type"] == 4 else None
+            if "range" in f:
+                r = L.orf_add_boundary_range(self.ctx, f["block"], OA._ints(f["range"]), f["type"],
+                                             None if ff is None else C.cast(C.byref(ff), C.c_void_p))
+            elif ff is not None:
+                r = L.orf_add_farfield(self.ctx, f["block"], f["axis"], f["side"], C.byref(ff))
             else:
-                L.orf_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"])
+                r = L.orf_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"])
+            if r != 0:
+                raise ValueError(f"bad boundary face {f}")
         for f in case["inner_faces"]:
-            L.orf_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"])
+            if "range" in f:
+                r = L.orf_add_inner_face_range(self.ctx, f["block"], OA._ints(f["range"]),
+                                               OA._ints(f["target_range"]), f["target"])
+            else:
+                r = L.orf_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"])
+            if r != 0:
+                raise ValueError(f"bad inner face {f}")
 
     def close(self):
         if self.ctx:

@@ -14,6 +14,7 @@
 // Sutherland and dt closed forms, viscous linear-profile exactness, and an
 // independent numpy restatement of the whole RHS on tiny grids.
 #include "ostc_oracle.h"
+#include "ostc_oracle_faces.h"
 
 #include <algorithm>
 #include <cmath>
@@ -62,12 +63,11 @@ constexpr double kRkC[3] = {1.0, 0.25, 2.0 / 3.0};
 
 struct Face {       // physical boundary face, src/Mesh.cpp:14-32
   int axis, side, type;
-  orc_farfield ff;  // type ORC_BC_FARFIELD: the state of its label
+  orc_farfield ff;   // type ORC_BC_FARFIELD: the state of its label
+  int lo[3], hi[3];  // the face layer; tangential node range widened by ngg
 };
 
-struct InnerFace {  // aligned 1-to-1 inner face, src/Mesh.cpp:38-176
-  int axis, side, target;
-};
+using InnerFace = orc_faces::Inner;  // 1-to-1 inner face, src/Mesh.cpp:38-176
 
 struct Block {
   int mx, my, mz, g;
@@ -912,12 +912,7 @@ void apply_face(Block& b, const Face& f) {
   const int g = b.g;
   const long long s = b.sz;
   const int ext[3] = {b.mx, b.my, b.mz};
-  int lo[3], hi[3];
-  for (int a = 0; a < 3; ++a) {
-    lo[a] = -g;
-    hi[a] = ext[a] - 1 + g;
-  }
-  lo[f.axis] = hi[f.axis] = (f.side < 0) ? 0 : ext[f.axis] - 1;
+  const int *lo = f.lo, *hi = f.hi;
   int dir[3] = {0, 0, 0};
   dir[f.axis] = f.side;
 #pragma omp parallel for collapse(2) schedule(static)
@@ -950,13 +945,7 @@ void apply_face(Block& b, const Face& f) {
 void apply_farfield(Block& b, const Face& f) {
   const int g = b.g;
   const long long s = b.sz;
-  const int ext[3] = {b.mx, b.my, b.mz};
-  int lo[3], hi[3];
-  for (int a = 0; a < 3; ++a) {
-    lo[a] = -g;
-    hi[a] = ext[a] - 1 + g;
-  }
-  lo[f.axis] = hi[f.axis] = (f.side < 0) ? 0 : ext[f.axis] - 1;
+  const int *lo = f.lo, *hi = f.hi;
   int dir[3] = {0, 0, 0};
   dir[f.axis] = f.side;
   const orc_farfield& ff = f.ff;
@@ -1030,35 +1019,22 @@ void apply_bc(Block& b) {
       }
 }
 
-// inner_communication, src/DataCommunication.cuh:60-105, for an aligned face.
+// inner_communication, src/DataCommunication.cuh:60-105: the shared node
+// layer takes the mean of both blocks (in both), the ghost layers take the
+// target's values.
 void communicate_face(Ctx& c, Block& b, const InnerFace& f) {
   Block& t = c.blocks[f.target];
-  const int g = b.g;
   const long long s = b.sz, ts = t.sz;
-  const int ext[3] = {b.mx, b.my, b.mz};
-  const int text[3] = {t.mx, t.my, t.mz};
-  // layers n = 0..g along the face normal; n indexes from range_start
-  int lo[3], hi[3];
-  for (int a = 0; a < 3; ++a) {
-    lo[a] = 0;
-    hi[a] = ext[a] - 1;
-  }
-  const int a = f.axis;
-  if (f.side > 0) { lo[a] = ext[a] - 1; hi[a] = ext[a] - 1 + g; }
-  else { lo[a] = -g; hi[a] = 0; }
-  const int shared = (f.side > 0) ? lo[a] : hi[a];
-  // target index along the axis: side>0 -> target low face starting at 0;
-  // side<0 -> target high face starting at m_t-1-g
-  const int tstart = (f.side > 0) ? 0 : text[a] - 1 - g;
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int k = lo[2]; k <= hi[2]; ++k)
-    for (int j = lo[1]; j <= hi[1]; ++j)
-      for (int i = lo[0]; i <= hi[0]; ++i) {
-        int idx[3] = {i, j, k};
-        int tid[3] = {i, j, k};
-        tid[a] = tstart + (idx[a] - lo[a]);
-        const long long p = b.id(i, j, k), pt = t.id(tid[0], tid[1], tid[2]);
-        if (idx[a] == shared) {
+  for (int n2 = 0; n2 < f.n_point[2]; ++n2)
+    for (int n1 = 0; n1 < f.n_point[1]; ++n1)
+      for (int n0 = 0; n0 < f.n_point[0]; ++n0) {
+        const int n[3] = {n0, n1, n2};
+        int idx[3], tid[3];
+        bool shared;
+        orc_faces::point_of(f, n, idx, tid, &shared);
+        const long long p = b.id(idx[0], idx[1], idx[2]), pt = t.id(tid[0], tid[1], tid[2]);
+        if (shared) {
           for (int l = 0; l < 6; ++l) {
             const real ave{0.5 * (t.bv[pt + l * ts] + b.bv[p + l * s])};
             b.bv[p + l * s] = ave;
@@ -1116,6 +1092,30 @@ uint64_t splitmix64(uint64_t x) {
   return z ^ (z >> 31);
 }
 
+// A whole face as the boundary N.txt record that names it.
+void whole_face_record(const Block& b, int axis, int side, int r[6]) {
+  const int ext[3] = {b.mx, b.my, b.mz};
+  for (int i = 0; i < 3; ++i) {
+    r[2 * i] = 0;
+    r[2 * i + 1] = ext[i] - 1;
+  }
+  r[2 * axis] = r[2 * axis + 1] = side < 0 ? 0 : ext[axis] - 1;
+}
+
+int add_face(Ctx* c, int block, const int range[6], int type, const orc_farfield* ff) {
+  Block& b = c->blocks[block];
+  const int ext[3] = {b.mx, b.my, b.mz};
+  orc_faces::Boundary r;
+  if (!orc_faces::register_boundary(range, c->ngg, ext, &r)) return -1;
+  Face f{r.face, r.direction, type, ff ? *ff : orc_farfield{}, {}, {}};
+  for (int i = 0; i < 3; ++i) {
+    f.lo[i] = r.range_start[i];
+    f.hi[i] = r.range_end[i];
+  }
+  b.faces.push_back(f);
+  return 0;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ C ABI
@@ -1150,16 +1150,23 @@ int orc_add_block(void* ctx, int mx, int my, int mz) {
 int orc_add_boundary(void* ctx, int block, int axis, int side, int type) {
   auto* c = static_cast<Ctx*>(ctx);
   if (block < 0 || block >= (int)c->blocks.size()) return -1;
-  c->blocks[block].faces.push_back(Face{axis, side, type, orc_farfield{}});
-  return 0;
+  int r[6];
+  whole_face_record(c->blocks[block], axis, side, r);
+  return add_face(c, block, r, type, nullptr);
 }
 
 int orc_add_farfield(void* ctx, int block, int axis, int side, const orc_farfield* ff) {
   auto* c = static_cast<Ctx*>(ctx);
   if (block < 0 || block >= (int)c->blocks.size() || !ff) return -1;
-  Face f{axis, side, ORC_BC_FARFIELD, *ff};
-  c->blocks[block].faces.push_back(f);
-  return 0;
+  int r[6];
+  whole_face_record(c->blocks[block], axis, side, r);
+  return add_face(c, block, r, ORC_BC_FARFIELD, ff);
+}
+
+int orc_add_boundary_range(void* ctx, int block, const int range[6], int type, const orc_farfield* ff) {
+  auto* c = static_cast<Ctx*>(ctx);
+  if (block < 0 || block >= (int)c->blocks.size() || (type == ORC_BC_FARFIELD && !ff)) return -1;
+  return add_face(c, block, range, type, ff);
 }
 
 // FarField::FarField for air (n_spec = 0, laminar), src/BoundCond.cpp:586-700
@@ -1198,11 +1205,26 @@ int orc_farfield_state_air(const orc_farfield_spec* in, orc_farfield* out) {
   return 0;
 }
 
+int orc_add_inner_face_range(void* ctx, int block, const int range[6], const int target_range[6], int target) {
+  auto* c = static_cast<Ctx*>(ctx);
+  if (block < 0 || block >= (int)c->blocks.size() || target < 0 || target >= (int)c->blocks.size()) return -1;
+  const Block &b = c->blocks[block], &t = c->blocks[target];
+  const int ext[3] = {b.mx, b.my, b.mz}, text[3] = {t.mx, t.my, t.mz};
+  InnerFace f;
+  if (!orc_faces::register_inner(range, target_range, c->ngg, target, &f)) return -1;
+  if (!orc_faces::inner_in_bounds(f, ext, text, c->ngg)) return -1;
+  c->blocks[block].inner.push_back(f);
+  return 0;
+}
+
 int orc_add_inner_face(void* ctx, int block, int axis, int side, int target) {
   auto* c = static_cast<Ctx*>(ctx);
   if (block < 0 || block >= (int)c->blocks.size() || target < 0 || target >= (int)c->blocks.size()) return -1;
-  c->blocks[block].inner.push_back(InnerFace{axis, side, target});
-  return 0;
+  const Block &b = c->blocks[block], &t = c->blocks[target];
+  const int ext[3] = {b.mx, b.my, b.mz}, text[3] = {t.mx, t.my, t.mz};
+  int src[6], tgt[6];
+  orc_faces::aligned_records(axis, side, ext, text, src, tgt);
+  return orc_add_inner_face_range(ctx, block, src, tgt, target);
 }
 
 double* orc_array(void* ctx, int block, int which, long long* n_out) {

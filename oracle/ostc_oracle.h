@@ -83,6 +83,12 @@ int     orc_add_boundary(void* ctx, int block, int axis, int side, int type);
  * side of an interface is registered separately, as in the reference's
  * inner N.txt files (src/Mesh.cpp:38-176). */
 int     orc_add_inner_face(void* ctx, int block, int axis, int side, int target_block);
+/* Connectivity records: one boundary N.txt line (0-based node range, ff for
+ * type ORC_BC_FARFIELD) and one inner N.txt line pair (1-based, the paired
+ * tangential axis marked negative; target_block 0-based). */
+int     orc_add_boundary_range(void* ctx, int block, const int range[6], int type, const orc_farfield* ff);
+int     orc_add_inner_face_range(void* ctx, int block, const int range[6], const int target_range[6],
+                                 int target_block);
 /* farfield face (label 4) with its state, apply_farfield<Air> (src/BoundCond.cuh:755-1022) */
 int     orc_add_farfield(void* ctx, int block, int axis, int side, const orc_farfield* ff);
 /* FarField constructor for air (n_spec = 0), src/BoundCond.cpp:586-700 */

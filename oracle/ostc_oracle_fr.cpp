@@ -22,6 +22,7 @@
 // five fields (src/WENO.cu:861-873), which for the first face of each pencil
 // is outside its shared-memory arrays.
 #include "ostc_oracle_fr.h"
+#include "ostc_oracle_faces.h"
 
 #include <algorithm>
 #include <cmath>
@@ -562,11 +563,10 @@ void char_back(const CharFace& F, int ns, const double* fChar, double* fci) {
 // --------------------------------------------------------------- structures
 struct Face {
   int axis, side, type;
-  orf_farfield ff;  // type 4: the state of its label
+  orf_farfield ff;   // type 4: the state of its label
+  int lo[3], hi[3];  // the face layer; tangential node range widened by ngg (src/Mesh.cpp:14-32)
 };
-struct InnerFace {
-  int axis, side, target;
-};
+using InnerFace = orc_faces::Inner;  // 1-to-1 inner face, src/Mesh.cpp:38-176
 
 struct Block {
   int mx, my, mz, g;
@@ -1414,10 +1414,10 @@ void apply_bc(const Ctx& c, Block& b) {
   const int ext[3] = {b.mx, b.my, b.mz};
   for (const Face& f : fs) {
     const int a = f.axis, o1 = (a + 1) % 3, o2 = (a + 2) % 3;
-    const int face_idx = f.side < 0 ? 0 : ext[a] - 1;
+    const int face_idx = f.lo[a];
 #pragma omp parallel for collapse(2) schedule(static)
-    for (int t2 = -b.g; t2 < ext[o2] + b.g; ++t2)
-      for (int t1 = -b.g; t1 < ext[o1] + b.g; ++t1) {
+    for (int t2 = f.lo[o2]; t2 <= f.hi[o2]; ++t2)
+      for (int t1 = f.lo[o1]; t1 <= f.hi[o1]; ++t1) {
         int ix[3];
         ix[a] = face_idx;
         ix[o1] = t1;
@@ -1441,31 +1441,25 @@ void apply_bc(const Ctx& c, Block& b) {
   }
 }
 
-// inner_communication with scalars, src/DataCommunication.cuh:60-105 (aligned
-// 1-to-1 faces; each face covers the shared node layer plus ngg ghost layers
-// over the interior tangential extent, src/Mesh.cpp:38-176)
+// inner_communication with scalars, src/DataCommunication.cuh:60-105: per
+// face (blocks, then faces) the shared node layer takes the mean of both
+// blocks, the ghost layers the target's values.
 void communicate(Ctx& c) {
   for (int bi = 0; bi < (int)c.blocks.size(); ++bi) {
     Block& b = c.blocks[bi];
     for (const InnerFace& f : b.inner) {
       Block& t = c.blocks[f.target];
-      const int a = f.axis, o1 = (a + 1) % 3, o2 = (a + 2) % 3;
-      const int ext[3] = {b.mx, b.my, b.mz}, text[3] = {t.mx, t.my, t.mz};
-      const int shared = f.side < 0 ? 0 : ext[a] - 1;
-      const int tshared = f.side < 0 ? text[a] - 1 : 0;
       const long long s = b.sz, ts = t.sz;
-      for (int layer = 0; layer <= b.g; ++layer) {
-        const int ia = shared + f.side * layer;
-        const int ta = tshared + f.side * layer;
-        for (int t2 = 0; t2 < ext[o2]; ++t2)
-          for (int t1 = 0; t1 < ext[o1]; ++t1) {
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int n2 = 0; n2 < f.n_point[2]; ++n2)
+        for (int n1 = 0; n1 < f.n_point[1]; ++n1)
+          for (int n0 = 0; n0 < f.n_point[0]; ++n0) {
+            const int n[3] = {n0, n1, n2};
             int ix[3], tx[3];
-            ix[a] = ia;
-            tx[a] = ta;
-            ix[o1] = tx[o1] = t1;
-            ix[o2] = tx[o2] = t2;
+            bool shared;
+            orc_faces::point_of(f, n, ix, tx, &shared);
             const long long p = b.id(ix[0], ix[1], ix[2]), pt = t.id(tx[0], tx[1], tx[2]);
-            if (layer == 0) {
+            if (shared) {
               for (int l = 0; l < 6; ++l) {
                 const double ave{0.5 * (t.bv[pt + l * ts] + b.bv[p + l * s])};
                 b.bv[p + l * s] = ave;
@@ -1482,7 +1476,6 @@ void communicate(Ctx& c) {
             }
             cv_point(c, b, p);
           }
-      }
     }
   }
 }
@@ -1521,6 +1514,31 @@ void step(Ctx& c, double* dt_io) {
 }
 
 Ctx* C(void* p) { return static_cast<Ctx*>(p); }
+
+// Registers a boundary N.txt record (Boundary::register_boundary, src/Mesh.cpp:14-32).
+int add_face(Ctx* c, int block, const int range[6], int type, const orf_farfield* ff) {
+  if (block < 0 || block >= (int)c->blocks.size() || (type == 4 && !ff)) return -1;
+  Block& b = c->blocks[block];
+  const int ext[3] = {b.mx, b.my, b.mz};
+  orc_faces::Boundary r;
+  if (!orc_faces::register_boundary(range, c->ngg, ext, &r)) return -1;
+  Face f{r.face, r.direction, type, ff ? *ff : orf_farfield{}, {}, {}};
+  for (int i = 0; i < 3; ++i) {
+    f.lo[i] = r.range_start[i];
+    f.hi[i] = r.range_end[i];
+  }
+  b.faces.push_back(f);
+  return 0;
+}
+
+void whole_face_record(const Block& b, int axis, int side, int r[6]) {
+  const int ext[3] = {b.mx, b.my, b.mz};
+  for (int i = 0; i < 3; ++i) {
+    r[2 * i] = 0;
+    r[2 * i + 1] = ext[i] - 1;
+  }
+  r[2 * axis] = r[2 * axis + 1] = side < 0 ? 0 : ext[axis] - 1;
+}
 
 }  // namespace
 
@@ -1579,14 +1597,19 @@ int orf_add_block(void* ctx, int mx, int my, int mz) {
 int orf_ngg(void* ctx) { return C(ctx)->ngg; }
 
 int orf_add_boundary(void* ctx, int block, int axis, int side, int type) {
-  C(ctx)->blocks[block].faces.push_back({axis, side, type, orf_farfield{}});
-  return 0;
+  int r[6];
+  whole_face_record(C(ctx)->blocks[block], axis, side, r);
+  return add_face(C(ctx), block, r, type, nullptr);
 }
 
 int orf_add_farfield(void* ctx, int block, int axis, int side, const orf_farfield* ff) {
-  Face f{axis, side, 4, *ff};
-  C(ctx)->blocks[block].faces.push_back(f);
-  return 0;
+  int r[6];
+  whole_face_record(C(ctx)->blocks[block], axis, side, r);
+  return add_face(C(ctx), block, r, 4, ff);
+}
+
+int orf_add_boundary_range(void* ctx, int block, const int range[6], int type, const orf_farfield* ff) {
+  return add_face(C(ctx), block, range, type, ff);
 }
 
 // compute_viscosity (host), src/Transport.cu:11-41
@@ -1662,9 +1685,26 @@ int orf_farfield_state(const ostc_mech_tables* t, const orf_farfield_spec* in, c
   return 0;
 }
 
-int orf_add_inner_face(void* ctx, int block, int axis, int side, int target) {
-  C(ctx)->blocks[block].inner.push_back({axis, side, target});
+int orf_add_inner_face_range(void* ctx, int block, const int range[6], const int target_range[6], int target) {
+  Ctx* c = C(ctx);
+  if (block < 0 || block >= (int)c->blocks.size() || target < 0 || target >= (int)c->blocks.size()) return -1;
+  const Block &b = c->blocks[block], &t = c->blocks[target];
+  const int ext[3] = {b.mx, b.my, b.mz}, text[3] = {t.mx, t.my, t.mz};
+  InnerFace f;
+  if (!orc_faces::register_inner(range, target_range, c->ngg, target, &f)) return -1;
+  if (!orc_faces::inner_in_bounds(f, ext, text, c->ngg)) return -1;
+  c->blocks[block].inner.push_back(f);
   return 0;
+}
+
+int orf_add_inner_face(void* ctx, int block, int axis, int side, int target) {
+  Ctx* c = C(ctx);
+  if (block < 0 || block >= (int)c->blocks.size() || target < 0 || target >= (int)c->blocks.size()) return -1;
+  const Block &b = c->blocks[block], &t = c->blocks[target];
+  const int ext[3] = {b.mx, b.my, b.mz}, text[3] = {t.mx, t.my, t.mz};
+  int src[6], tgt[6];
+  orc_faces::aligned_records(axis, side, ext, text, src, tgt);
+  return orf_add_inner_face_range(ctx, block, src, tgt, target);
 }
 
 double* orf_array(void* ctx, int block, int which, long long* n) {

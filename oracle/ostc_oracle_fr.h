@@ -49,6 +49,9 @@ int     orf_add_block(void* ctx, int mx, int my, int mz);
 int     orf_ngg(void* ctx);
 int     orf_add_boundary(void* ctx, int block, int axis, int side, int type);  /* 6 outflow, 8 periodic */
 int     orf_add_inner_face(void* ctx, int block, int axis, int side, int target_block);
+/* connectivity records: a boundary N.txt line, an inner N.txt line pair */
+int     orf_add_inner_face_range(void* ctx, int block, const int range[6], const int target_range[6],
+                                 int target_block);
 
 /* Far-field state of one label (FarField after its constructor,
  * src/BoundCond.h:129-149) and the constructor's inputs (-1 = not given,
@@ -66,6 +69,7 @@ int     orf_farfield_state(const ostc_mech_tables* t, const orf_farfield_spec* i
                            orf_farfield* out);
 /* farfield face (label 4), apply_farfield<FR> (src/BoundCond.cuh:755-1022) */
 int     orf_add_farfield(void* ctx, int block, int axis, int side, const orf_farfield* ff);
+int     orf_add_boundary_range(void* ctx, int block, const int range[6], int type, const orf_farfield* ff);
 double* orf_array(void* ctx, int block, int which, long long* n_out);
 
 int     orf_initialize(void* ctx);            /* cv from bv, properties, BCs, exchange, properties */

@@ -32,3 +32,20 @@ def oracle_for(case, steps_init=True):
     if steps_init:
         o.initialize()
     return o
+
+
+def oracle_pair(pair, bv_global, tangent=False):
+    """The oracle on the rotated two-block case: node coordinates of each
+    block's frame, metrics from them (Block::compute_jac_metric), the initial
+    primitives mapped from a global single-block padded array, then
+    initialize (boundary conditions, exchange, properties, cv)."""
+    o = O.Oracle(pair, tangent=tangent)
+    g = o.ngg
+    L = O.lib(tangent)
+    for b in range(len(pair["blocks"])):
+        for which, arr in zip((O.X, O.Y, O.Z), cases.block_coordinates(pair, b, g)):
+            o.array(b, which)[0][...] = arr
+        assert L.orc_compute_metric(o.ctx, b) == 0
+        o.array(b, O.BV)[...] = cases.block_from_global(pair, b, bv_global, g)
+    o.initialize()
+    return o
