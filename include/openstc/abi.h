@@ -41,6 +41,7 @@ extern "C" {
 #define OSTC_BC_FARFIELD 4
 #define OSTC_BC_OUTFLOW 6
 #define OSTC_BC_PERIODIC 8
+#define OSTC_BC_INFLOW 5
 
 /* time-step modes of ostc_step */
 #define OSTC_MODE_PLAIN 0    /* per-block launches shaped like the reference */
@@ -152,6 +153,31 @@ int ostc_add_inner_face(ostc_ctx ctx, int block, int axis, int side, int target_
 int ostc_add_boundary_range(ostc_ctx ctx, int block, const int range[6], int type, const ostc_farfield* ff);
 int ostc_add_inner_face_range(ostc_ctx ctx, int block, const int range[6], const int target_range[6],
                               int target_block);
+
+/* Inflow face (label 5), apply_inflow, src/BoundCond.cuh:235-612, with the
+ * members of Inflow after its constructor (src/BoundCond.cpp:10-321).
+ * inflow_type 2 (mixing layer): u = (u1+u2)/2 + (u1-u2)/2 tanh(2y/delta_omega)
+ * at every face point, the other primitives (and mass fractions) of the
+ * upper stream where y >= 0 and of the lower stream below; inflow_type 0
+ * (constant): the upper state everywhere.  The ghost layers repeat the face
+ * point.  fluctuation_type 1 (white noise) adds intensity*|V| times standard
+ * normal deviates to u and v (type 2, only where |y| < 4 delta_omega) or to u
+ * (type 0).  The reference draws them from cuRAND states seeded by the wall
+ * clock; here they come from a counter-based generator (SplitMix64 finalizer
+ * and the Box-Muller transform) keyed by seed, the face point and the count
+ * of the block's boundary-condition passes, so a run is reproducible. */
+typedef struct {
+  int inflow_type;            /* 2 mixing layer, 0 constant */
+  int fluctuation_type;       /* 0 none, 1 white noise */
+  double delta_omega;
+  double fluctuation_intensity;
+  unsigned long long seed;
+  double upper[6], lower[6];  /* rho, u, v, w, p, T */
+  double sv_upper[OSTC_MAX_SPEC], sv_lower[OSTC_MAX_SPEC];
+} ostc_inflow;
+/* range: a boundary N.txt record of the face (0-based, as ostc_add_boundary_range);
+ * y: the block's node y coordinates on the padded domain (host, the bv plane layout). */
+int ostc_add_inflow(ostc_ctx ctx, int block, const int range[6], const ostc_inflow* in, const double* y);
 /* Allocates device memory and builds the work-lists; blocks and faces are
  * fixed afterwards. */
 int ostc_finalize(ostc_ctx ctx);
@@ -380,6 +406,8 @@ int ostc_fr_add_inner_face(ostc_fr_ctx ctx, int block, int axis, int side, int t
 int ostc_fr_add_boundary_range(ostc_fr_ctx ctx, int block, const int range[6], int type, const ostc_farfield* ff);
 int ostc_fr_add_inner_face_range(ostc_fr_ctx ctx, int block, const int range[6], const int target_range[6],
                                  int target_block);
+/* Inflow face (label 5) of a mixture, apply_inflow with mass fractions; as ostc_add_inflow. */
+int ostc_fr_add_inflow(ostc_fr_ctx ctx, int block, const int range[6], const ostc_inflow* in, const double* y);
 /* ghost layers of every array of the context: 3, or 4 for the WENO-Z7 schemes 71/72 (src/Parameter.cpp:227-238) */
 int ostc_fr_ngg(ostc_fr_ctx ctx, int* ngg);
 /* store_last_step / compute_residual (src/TimeAdvanceFunc.cu:6-37, src/Residual.cuh:12-187): unscaled

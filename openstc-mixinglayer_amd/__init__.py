@@ -58,6 +58,23 @@ class Farfield(C.Structure):
                 ("sv", C.c_double * 16)]
 
 
+class Inflow(C.Structure):
+    """ostc_inflow: Inflow members (src/BoundCond.cpp:10-321) and the white-noise seed."""
+    _fields_ = [("inflow_type", C.c_int), ("fluctuation_type", C.c_int), ("delta_omega", C.c_double),
+                ("fluctuation_intensity", C.c_double), ("seed", C.c_uint64), ("upper", C.c_double * 6),
+                ("lower", C.c_double * 6), ("sv_upper", C.c_double * 16), ("sv_lower", C.c_double * 16)]
+
+
+def inflow_struct(d: dict) -> Inflow:
+    """An ostc_inflow from a case's ``inflow`` record."""
+    sv_u = list(d.get("sv_upper", [])) + [0.0] * (16 - len(d.get("sv_upper", [])))
+    sv_l = list(d.get("sv_lower", [])) + [0.0] * (16 - len(d.get("sv_lower", [])))
+    return Inflow(int(d.get("inflow_type", 2)), int(d.get("fluctuation_type", 0)), float(d.get("delta_omega", 0.0)),
+                  float(d.get("fluctuation_intensity", 0.0)), int(d.get("seed", 0)),
+                  (C.c_double * 6)(*d["upper"]), (C.c_double * 6)(*d["lower"]),
+                  (C.c_double * 16)(*sv_u), (C.c_double * 16)(*sv_l))
+
+
 class FarfieldSpec(C.Structure):
     """ostc_farfield_spec: FarField constructor inputs (src/BoundCond.cpp:586-700)."""
     _fields_ = [("mach", C.c_double), ("pressure", C.c_double), ("temperature", C.c_double),
@@ -141,6 +158,7 @@ SIGNATURES = {
     "ostc_add_inner_face": (_I, [_P, _I, _I, _I, _I]),
     "ostc_add_boundary_range": (_I, [_P, _I, C.POINTER(_I), _I, C.POINTER(Farfield)]),
     "ostc_add_inner_face_range": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), _I]),
+    "ostc_add_inflow": (_I, [_P, _I, C.POINTER(_I), C.POINTER(Inflow), _DP]),
     "ostc_finalize": (_I, [_P]),
     "ostc_ngg": (_I, [_P, C.POINTER(_I)]),
     "ostc_zone_view": (_I, [_P, _I, C.POINTER(Zone)]),
@@ -251,6 +269,12 @@ class Solver:
         for b in case["blocks"]:
             self._chk(L.ostc_add_block(self.ctx, C.byref(_BlockDesc(b["mx"], b["my"], b["mz"])), None))
         for f in case["boundaries"]:
+            if f["type"] == 5:  # inflow: its state and the y of the block's nodes
+                blk = case["blocks"][f["block"]]
+                y = np.ascontiguousarray(cases.node_coordinates(case, f["block"], self.ngg)[1])
+                self._chk(L.ostc_add_inflow(self.ctx, f["block"], ints(cases.face_range(blk, f)),
+                                            C.byref(inflow_struct(f["inflow"])), _dp(y)))
+                continue
             ff = farfield_state_air(**f["farfield"]) if f["type"] == 4 else None
             if "range" in f:  # a boundary N.txt record
                 self._chk(L.ostc_add_boundary_range(self.ctx, f["block"], ints(f["range"]), f["type"],

@@ -28,7 +28,7 @@ import numpy as np
 R_U = 8314.462618          # src/Constants.h:19
 GAMMA_AIR = 1.4            # src/Constants.h:23
 MW_AIR = 28.9647           # src/Constants.h:28
-BC_FARFIELD, BC_OUTFLOW, BC_PERIODIC = 4, 6, 8
+BC_FARFIELD, BC_INFLOW, BC_OUTFLOW, BC_PERIODIC = 4, 5, 6, 8
 
 
 def sutherland(t: float) -> float:
@@ -62,10 +62,11 @@ def _cuts(lengths):
     return off
 
 
-def _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc):
+def _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc, x_open=False):
     """Blocks and faces of a tensor-product partition: x and z periodic
     (inner faces across the period when there are several blocks along the
-    axis, a periodic face otherwise), y bounded by ``y_bc`` faces."""
+    axis, a periodic face otherwise), y bounded by ``y_bc`` faces; with
+    ``x_open`` the x ends are an inflow face (x-) and an outflow face (x+)."""
     ox, oy, oz = _cuts(x_cells), _cuts(y_cells), _cuts(z_cells)
     blocks, bidx = [], {}
     for kb in range(len(z_cells)):
@@ -84,7 +85,7 @@ def _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc
             for side in (-1, 1):
                 q = list(pos)
                 q[axis] += side
-                periodic = axis != 1
+                periodic = axis == 2 or (axis == 0 and not x_open)
                 if 0 <= q[axis] < nb[axis]:
                     inner.append(dict(block=b, axis=axis, side=side, target=bidx[tuple(q)]))
                 elif periodic and nb[axis] > 1:
@@ -92,6 +93,8 @@ def _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc
                     inner.append(dict(block=b, axis=axis, side=side, target=bidx[tuple(q)]))
                 elif periodic:
                     boundaries.append(dict(block=b, axis=axis, side=side, type=BC_PERIODIC))
+                elif axis == 0:
+                    boundaries.append(dict(block=b, axis=axis, side=side, type=BC_INFLOW if side < 0 else BC_OUTFLOW))
                 else:
                     boundaries.append(dict(block=b, axis=axis, side=side, type=y_bc))
     return blocks, boundaries, inner
@@ -102,6 +105,18 @@ def stream_farfield(state, c):
     stream state [rho, u, v, w, p, T] moving along x with sound speed c."""
     u = state[1]
     return dict(mach=abs(u) / c, pressure=state[4], temperature=state[5], u=-1.0 if u < 0 else 1.0, v=0.0, w=0.0)
+
+
+def _attach_inflow(boundaries, st, delta, intensity, seed, sv_upper=(), sv_lower=()):
+    """The mixing-layer inflow (inflow_type 2) of the two streams on the
+    inflow faces, white noise of ``intensity`` (fluctuation_type 1) when
+    it is positive."""
+    for f in boundaries:
+        if f["type"] == BC_INFLOW:
+            f["inflow"] = dict(inflow_type=2, fluctuation_type=1 if intensity > 0 else 0, delta_omega=delta,
+                               fluctuation_intensity=intensity, seed=seed,
+                               upper=[float(v) for v in st["upper"]], lower=[float(v) for v in st["lower"]],
+                               sv_upper=[float(v) for v in sv_upper], sv_lower=[float(v) for v in sv_lower])
 
 
 def _attach_farfield(boundaries, upper, lower):
@@ -115,7 +130,8 @@ def _attach_farfield(boundaries, upper, lower):
 def mixing_layer_case(x_cells, y_cells, z_cells, *, re_delta=500.0, ma_c=0.4,
                       lx_over_delta=20.0, ly_over_delta=40.0, lz_over_delta=20.0,
                       scheme=51, viscous_order=2, cfl=0.5, prandtl=0.72,
-                      seed=42, perturbation=1e-3, y_bc=BC_OUTFLOW, reconstruction=2):
+                      seed=42, perturbation=1e-3, y_bc=BC_OUTFLOW, reconstruction=2,
+                      spatial=False, inflow_noise=1e-3):
     """Temporal mixing layer of air on a tensor-product partition.
 
     ``x_cells`` etc. are lists of per-block cell counts along each axis
@@ -126,6 +142,9 @@ def mixing_layer_case(x_cells, y_cells, z_cells, *, re_delta=500.0, ma_c=0.4,
     temperatures.  ``reconstruction`` (1 first order, 2 MUSCL, 3 NND2)
     applies to the primitive-variable schemes 1/3/4.  With ``y_bc`` =
     BC_FARFIELD the y faces carry the far-field state of the stream they face.
+    ``spatial``: the spatially developing layer -- x- is the mixing-layer
+    inflow (src/BoundCond.cuh:420-482) with white noise of relative
+    intensity ``inflow_noise``, x+ an outflow face, x not periodic.
     """
     st = air_streams(ma_c=ma_c, ma_upper=ma_c)
     rho, t = st["upper"][0], st["upper"][5]
@@ -135,16 +154,18 @@ def mixing_layer_case(x_cells, y_cells, z_cells, *, re_delta=500.0, ma_c=0.4,
     dx = lx_over_delta * delta / nx
     dy = ly_over_delta * delta / ny
     dz = lz_over_delta * delta / nz
-    blocks, boundaries, inner = _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc)
+    blocks, boundaries, inner = _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc,
+                                           x_open=spatial)
     _attach_farfield(boundaries, stream_farfield(st["upper"], st["c_upper"]),
                      stream_farfield(st["lower"], st["c_lower"]))
+    _attach_inflow(boundaries, st, delta, inflow_noise, seed + 1)
     # face order inside a block: as registered above (x-, x+, y-, y+, z-, z+)
     return dict(
         n_var=5, inviscid_scheme=scheme, viscous_order=viscous_order, cfl=cfl, prandtl=prandtl,
         fixed_time_step=0, dt=0.0, reconstruction=reconstruction,
         rho_ref=st["upper"][0], v_ref=abs(st["upper"][1]) if abs(st["upper"][1]) >= 1 else st["c_upper"],
         streams=st, delta_omega=delta, dx=dx, dy=dy, dz=dz,
-        periods=(nx, 0, nz), seed=seed, amplitude=perturbation * du,
+        periods=(0 if spatial else nx, 0, nz), seed=seed, amplitude=perturbation * du,
         blocks=blocks, boundaries=boundaries, inner_faces=inner,
         n_cells=sum(b["mx"] * b["my"] * b["mz"] for b in blocks))
 
@@ -303,14 +324,17 @@ OXIDIZER_AIR = {"O2": 0.232, "N2": 0.768}      # oxidizer stream, 1500 K
 def reacting_case(x_cells, y_cells, z_cells, tables, *, mech_files=None, reaction=1, re_delta=500.0, ma_c=0.5,
                   fuel=FUEL_H2_N2, oxidizer=OXIDIZER_AIR, t_fuel=300.0, t_oxidizer=1500.0,
                   lx_over_delta=20.0, ly_over_delta=40.0, lz_over_delta=20.0, scheme=52, viscous_order=2,
-                  cfl=0.5, prandtl=0.72, positive_preserving=0, seed=42, perturbation=1e-3, y_bc=BC_OUTFLOW):
+                  cfl=0.5, prandtl=0.72, positive_preserving=0, seed=42, perturbation=1e-3, y_bc=BC_OUTFLOW,
+                  spatial=False, inflow_noise=1e-3):
     """Temporal H2/air mixing layer on a tensor-product partition: the fuel
     (H2 in N2) stream above y = 0, the oxidizer (air) stream below, tanh
     velocity profile with convective Mach ``ma_c``, species and temperature
     steps at y = 0 (src/Initialize.cu:259-325), seeded velocity noise.
     delta_omega follows from Re_delta = rho dU delta / mu of the fuel stream.
-    The reference's spatially developing variant needs its inflow boundary,
-    which this path does not include; the temporal layer uses periodic x/z."""
+    ``spatial``: the spatially developing layer -- x- is the mixing-layer
+    inflow of the two streams (src/BoundCond.cuh:420-482) with white noise
+    of relative intensity ``inflow_noise``, x+ an outflow face; otherwise
+    the temporal layer, periodic in x."""
     y_up = species_vector(tables, fuel)
     y_lo = species_vector(tables, oxidizer)
     st = mixture_streams(tables, y_up, y_lo, t_fuel, t_oxidizer, ma_c=ma_c)
@@ -321,9 +345,11 @@ def reacting_case(x_cells, y_cells, z_cells, tables, *, mech_files=None, reactio
     dx = lx_over_delta * delta / nx
     dy = ly_over_delta * delta / ny
     dz = lz_over_delta * delta / nz
-    blocks, boundaries, inner = _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc)
+    blocks, boundaries, inner = _partition(x_cells, y_cells, z_cells, dx, dy, dz, delta, ly_over_delta, y_bc,
+                                           x_open=spatial)
     _attach_farfield(boundaries, stream_farfield(st["upper"], st["c_upper"]),
                      stream_farfield(st["lower"], st["c_lower"]))
+    _attach_inflow(boundaries, st, delta, inflow_noise, seed + 1, y_up, y_lo)
     for f in boundaries:
         if f["type"] == BC_FARFIELD:
             f["farfield_y"] = [float(v) for v in (y_up if f["side"] > 0 else y_lo)]
@@ -334,7 +360,7 @@ def reacting_case(x_cells, y_cells, z_cells, tables, *, mech_files=None, reactio
         positive_preserving=positive_preserving, cfl=cfl, prandtl=prandtl, fixed_time_step=0, dt=0.0,
         rho_ref=rho, v_ref=u1 if u1 >= 1 else st["c_upper"],
         streams=st, y_upper=y_up, y_lower=y_lo, delta_omega=delta, dx=dx, dy=dy, dz=dz,
-        periods=(nx, 0, nz), seed=seed, amplitude=perturbation * du, mech_files=mech_files,
+        periods=(0 if spatial else nx, 0, nz), seed=seed, amplitude=perturbation * du, mech_files=mech_files,
         blocks=blocks, boundaries=boundaries, inner_faces=inner,
         n_cells=sum(b["mx"] * b["my"] * b["mz"] for b in blocks))
 
@@ -407,7 +433,8 @@ def rotated_pair(case: dict, cut: int, split_k: int | None = None) -> dict:
 
 
 def _global_index(blk: dict, p, q, r):
-    o, a = blk["origin"], blk["axes"]
+    o = blk.get("origin", (0, 0, 0))
+    a = blk.get("axes", ((1, 0, 0), (0, 1, 0), (0, 0, 1)))
     return tuple(o[d] + a[d][0] * p + a[d][1] * q + a[d][2] * r for d in range(3))
 
 
@@ -419,12 +446,33 @@ def _local_grid(blk: dict, g: int):
     return _global_index(blk, P, Q, R)
 
 
-def block_coordinates(case: dict, b: int, g: int):
-    """Node coordinates (x, y, z) of block b on its domain padded by g+1
-    layers, arrays (k, j, i), from the global uniform grid of the pair."""
+def node_coordinates(case: dict, b: int, g: int):
+    """Node coordinates (x, y, z) of block b on its domain padded by g
+    layers, arrays (k, j, i), from the uniform grid of the case (a block
+    with ``origin``/``axes`` maps its local indices to the grid's first)."""
     blk = case["blocks"][b]
-    i, j, k = _local_grid(blk, g + 1)
+    i, j, k = _local_grid(blk, g)
     return blk["x0"] + i * case["dx"], blk["y0"] + j * case["dy"], blk["z0"] + k * case["dz"]
+
+
+def block_coordinates(case: dict, b: int, g: int):
+    """node_coordinates on the domain padded by g+1 layers (the layers the
+    metric's central differences read)."""
+    return node_coordinates(case, b, g + 1)
+
+
+def face_range(blk: dict, f: dict):
+    """The boundary N.txt record of a boundary face: its ``range``, or the
+    whole face of ``axis``/``side``."""
+    if "range" in f:
+        return list(f["range"])
+    ext = (blk["mx"], blk["my"], blk["mz"])
+    r = []
+    for d in range(3):
+        r += [0, ext[d] - 1]
+    a = f["axis"]
+    r[2 * a] = r[2 * a + 1] = 0 if f["side"] < 0 else ext[a] - 1
+    return r
 
 
 def block_from_global(case: dict, b: int, arr: np.ndarray, g: int) -> np.ndarray:

@@ -17,8 +17,10 @@ using namespace ostc;
 namespace {
 
 struct FaceDesc {
-  int type, ff;  // ff: index of the far-field state (type 4)
+  int type, ff;  // ff: index of the far-field state (type 4) or of the inflow state (type 5)
   FaceRange r;   // the face layer and its tangential range widened by ngg
+  std::vector<double> y;  // type 5: y of the face points, i fastest over r
+  size_t y_off = 0;       // type 5: offset of y in d_inflow
 };
 struct Box { int lo[3], hi[3]; };  // inclusive bounds
 
@@ -64,6 +66,7 @@ int bc_priority(int type) {  // src/BoundCond.cuh:1488-1678 (farfield, outflow, 
   switch (type) {
     case OSTC_BC_FARFIELD: return 0;
     case OSTC_BC_OUTFLOW: return 1;
+    case OSTC_BC_INFLOW: return 2;
     case OSTC_BC_PERIODIC: return 7;
     default: return 99;
   }
@@ -81,6 +84,10 @@ struct ostc_ctx_s {
   std::vector<HBlock> blocks;
   std::vector<ostc_farfield> farfields;  // states of the farfield faces, in registration order
   double* d_ff = nullptr;
+  std::vector<double> inflows;           // packed inflow states (kInflowDoubles each)
+  bool inflow_noise = false;             // some inflow face has white noise
+  double* d_inflow = nullptr;            // inflow states, then the y of every inflow face point
+  unsigned long long* d_pass = nullptr;  // boundary-condition passes per block
   bool finalized = false;
   DevBlock* d_blocks = nullptr;
   std::vector<DevOp> ops;
@@ -212,20 +219,27 @@ int build_ops(ostc_ctx c) {
     for (const FaceDesc& fd : f) {
       const int a = fd.r.axis, m = B.m[a];
       const int idx = fd.r.side < 0 ? 0 : m - 1;
-      const int ot = fd.type == OSTC_BC_PERIODIC ? OP_BC_PERIODIC
-                     : fd.type == OSTC_BC_FARFIELD ? OP_BC_FARFIELD : OP_BC_OUTFLOW;
+      const int ot = fd.type == OSTC_BC_PERIODIC   ? OP_BC_PERIODIC
+                     : fd.type == OSTC_BC_FARFIELD ? OP_BC_FARFIELD
+                     : fd.type == OSTC_BC_INFLOW   ? OP_BC_INFLOW
+                                                   : OP_BC_OUTFLOW;
       DevOp op = make_op(ot, b, face_box(fd.r, idx, idx));
       op.axis = a;
       op.side = fd.r.side;
       if (fd.type == OSTC_BC_FARFIELD) op.ff = c->d_ff + (size_t)fd.ff * kFarfieldDoubles;
+      if (fd.type == OSTC_BC_INFLOW) {
+        op.ff = c->d_inflow + (size_t)fd.ff * kInflowDoubles;
+        op.aux = c->d_inflow + fd.y_off;
+        op.pass = c->d_pass;
+      }
       bc_of_block[b].push_back(op);
       // ghost cells this op reads before the same stage can have written them
       Box rd = fd.type == OSTC_BC_PERIODIC
                    ? (fd.r.side < 0 ? face_box(fd.r, m - 1 - G, m - 2) : face_box(fd.r, 1, G))
                    : face_box(fd.r, idx, idx);
       for (const Box& rb : minus_interior(rd, B.m)) rim_ops.push_back(make_op(OP_COPY_BV, b, rb));
-      if (fd.type == OSTC_BC_FARFIELD) {
-        // the farfield state replaces the face points after the fused update
+      if (fd.type == OSTC_BC_FARFIELD || fd.type == OSTC_BC_INFLOW) {
+        // the boundary state replaces the face points after the fused update
         // computed their properties: recompute them with the ghost layers
         Box fp = face_box(fd.r, idx, idx);
         for (int d = 0; d < 3; ++d) {
@@ -502,6 +516,7 @@ void launch_op(ostc_ctx c, int idx, int slot, long long* n) {
 // ---------------------------------------------------- per-block path pieces
 int plain_bc(ostc_ctx c, int b, long long* n) {
   for (int idx : c->plain_bc[b]) launch_op(c, idx, 0, n);
+  if (c->inflow_noise) count_pass(c->d_pass, b, b + 1, c->st, n);
   return OSTC_OK;
 }
 
@@ -570,6 +585,7 @@ int batched_stage(ostc_ctx c, int rk, long long* n, int ev_base) {
   if (events) HIPCHK(c, hipEventRecord(c->evpool[ev_base + 2 * rk + 1], c->st));
   launch_group(c, c->rim, out, in, n);
   for (auto& g : c->bc_rounds) launch_group(c, g, out, out, n);
+  if (c->inflow_noise) count_pass(c->d_pass, 0, (int)c->blocks.size(), c->st, n);
   for (auto& g : c->comm_levels) launch_group(c, g, out, out, n);
   launch_group(c, c->props, out, out, n);
   return OSTC_OK;
@@ -709,9 +725,10 @@ static int add_face(ostc_ctx c, int block, const FaceRange& r, int type, const o
     c->farfields.push_back(*ff);
     return OSTC_OK;
   }
+  if (type == OSTC_BC_INFLOW) return fail(c, OSTC_ERR_ARG, "inflow faces (label 5) need their state: use ostc_add_inflow");
   if (type != OSTC_BC_OUTFLOW && type != OSTC_BC_PERIODIC)
-    return fail(c, OSTC_ERR_UNSUPPORTED,
-                "boundary label " + std::to_string(type) + " not implemented (farfield 4, outflow 6, periodic 8 are)");
+    return fail(c, OSTC_ERR_UNSUPPORTED, "boundary label " + std::to_string(type) +
+                                             " not implemented (farfield 4, inflow 5, outflow 6, periodic 8 are)");
   c->blocks[block].faces.push_back({type, -1, r});
   return OSTC_OK;
 }
@@ -735,6 +752,32 @@ int ostc_add_farfield(ostc_ctx c, int block, int axis, int side, const ostc_farf
   if (int r = check_face_args(c, block, axis, side)) return r;
   if (!ff) return fail(c, OSTC_ERR_ARG, "bad farfield face");
   return add_face(c, block, whole_face(axis, side, c->blocks[block].m, c->ngg), OSTC_BC_FARFIELD, ff);
+}
+
+int ostc_add_inflow(ostc_ctx c, int block, const int range[6], const ostc_inflow* in, const double* y) {
+  if (!c) return OSTC_ERR_ARG;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_finalize");
+  if (block < 0 || block >= (int)c->blocks.size() || !range || !in || !y) return fail(c, OSTC_ERR_ARG, "bad inflow face");
+  if ((in->inflow_type != 0 && in->inflow_type != 2) || (in->fluctuation_type != 0 && in->fluctuation_type != 1))
+    return fail(c, OSTC_ERR_UNSUPPORTED, "inflow_type 0/2 and fluctuation_type 0/1 are implemented");
+  if (in->inflow_type == 2 && !(in->delta_omega > 0)) return fail(c, OSTC_ERR_ARG, "inflow: delta_omega > 0");
+  for (const double* S : {in->upper, in->lower})
+    if (!(S[0] > 0) || !(S[4] > 0) || !(S[5] > 0)) return fail(c, OSTC_ERR_ARG, "inflow: rho, p, T > 0");
+  const HBlock& B = c->blocks[block];
+  FaceRange r{};
+  std::string err;
+  if (!boundary_range(range, B.m, c->ngg, &r, &err)) return fail(c, OSTC_ERR_ARG, err);
+  FaceDesc fd{OSTC_BC_INFLOW, (int)(c->inflows.size() / kInflowDoubles), r, {}};
+  const int G = c->ngg;
+  const long long sx = B.m[0] + 2 * G, sy = B.m[1] + 2 * G;
+  for (int k = r.lo[2]; k <= r.hi[2]; ++k)
+    for (int j = r.lo[1]; j <= r.hi[1]; ++j)
+      for (int i = r.lo[0]; i <= r.hi[0]; ++i) fd.y.push_back(y[((long long)(k + G) * sy + (j + G)) * sx + (i + G)]);
+  c->inflows.resize(c->inflows.size() + kInflowDoubles);
+  pack_inflow(*in, c->inflows.data() + c->inflows.size() - kInflowDoubles);
+  if (in->fluctuation_type == 1) c->inflow_noise = true;
+  c->blocks[block].faces.push_back(std::move(fd));
+  return OSTC_OK;
 }
 
 int ostc_add_boundary_range(ostc_ctx c, int block, const int range[6], int type, const ostc_farfield* ff) {
@@ -862,6 +905,24 @@ int ostc_finalize(ostc_ctx c) {
     c->d_ff = static_cast<double*>(v);
     HIPCHK(c, hipMemcpy(c->d_ff, c->farfields.data(), c->farfields.size() * sizeof(ostc_farfield),
                         hipMemcpyHostToDevice));
+  }
+  if (!c->inflows.empty()) {
+    std::vector<double> h = c->inflows;
+    for (auto& B : c->blocks)
+      for (auto& f : B.faces)
+        if (f.type == OSTC_BC_INFLOW) {
+          f.y_off = h.size();
+          h.insert(h.end(), f.y.begin(), f.y.end());
+        }
+    int r = dalloc(c, &c->d_inflow, h.size());
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(c->d_inflow, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  }
+  {
+    double* p = nullptr;  // one 8-byte counter per block
+    if (int r = dalloc(c, &p, c->blocks.size())) return r;
+    c->d_pass = reinterpret_cast<unsigned long long*>(p);
   }
   int r = dalloc(c, &c->d_scratch, 1 + 4 * (size_t)c->n_first);
   if (!r) r = dalloc(c, &c->d_dt, 1);
@@ -1407,6 +1468,9 @@ int ostc_adjoint_forward(ostc_ctx c, int n_steps, int every, const ostc_theta* t
   if (r) return r;
   if (c->blocks.size() != 1 || !c->blocks[0].inner.empty())
     return fail(c, OSTC_ERR_UNSUPPORTED, "the RK3 reverse sweep covers a single block without inner faces");
+  for (const FaceDesc& f : c->blocks[0].faces)
+    if (f.type != OSTC_BC_OUTFLOW && f.type != OSTC_BC_PERIODIC)
+      return fail(c, OSTC_ERR_UNSUPPORTED, "the RK3 reverse sweep covers outflow and periodic faces");
   if (!c->s.fixed_time_step) return fail(c, OSTC_ERR_UNSUPPORTED, "the RK3 reverse sweep needs fixed_time_step = 1");
   if (c->P.scheme != 51 && c->P.scheme != 50)
     return fail(c, OSTC_ERR_UNSUPPORTED, "reverse mode covers inviscid_scheme 51 and 50");

@@ -14,6 +14,7 @@
 #include "openstc/abi.h"
 #include "ostc_faces.hpp"
 #include "ostc_fr.hpp"
+#include "ostc_inflow.hpp"
 
 using namespace ostc::fr;
 using ostc::FaceMap;
@@ -22,8 +23,10 @@ using ostc::FaceRange;
 namespace {
 
 struct FaceDesc {
-  int type, ff;  // ff: index of the far-field state (type 4)
+  int type, ff;  // ff: index of the far-field state (type 4) or of the inflow state (type 5)
   FaceRange r;   // the face layer and its tangential range widened by ngg
+  std::vector<double> y;  // type 5: y of the face points, i fastest over r
+  size_t y_off = 0;       // type 5: offset of y in d_inflow
 };
 
 struct HBlock {
@@ -53,6 +56,10 @@ struct ostc_fr_ctx_s {
   std::vector<HBlock> blocks;
   std::vector<ostc_farfield> farfields;  // states of the farfield faces, in registration order
   double* d_ff = nullptr;
+  std::vector<double> inflows;           // packed inflow states (kInflowDoubles each)
+  bool inflow_noise = false;             // some inflow face has white noise
+  double* d_inflow = nullptr;            // inflow states, then the y of every inflow face point
+  unsigned long long* d_pass = nullptr;  // boundary-condition passes per block
   bool finalized = false;
   double* d_dt = nullptr;          // dt of the current step (one double)
   double* d_res = nullptr;         // per-workgroup partial sums of the residual
@@ -120,8 +127,10 @@ int check_ready(ostc_fr_ctx c) {
   return OSTC_OK;
 }
 
-// src/BoundCond.cuh:1488-1678: farfield, outflow, ..., periodic
-int bc_priority(int type) { return type == OSTC_BC_FARFIELD ? 0 : (type == OSTC_BC_OUTFLOW ? 1 : 2); }
+// src/BoundCond.cuh:1488-1678: farfield, outflow, inflow, ..., periodic
+int bc_priority(int type) {
+  return type == OSTC_BC_FARFIELD ? 0 : (type == OSTC_BC_OUTFLOW ? 1 : (type == OSTC_BC_INFLOW ? 2 : 3));
+}
 
 // Boundary conditions of one block (priority order, then registration
 // order), src/BoundCond.cuh:1488-1678.
@@ -138,8 +147,13 @@ void apply_bc(ostc_fr_ctx c, int b, long long* n) {
     }
     fb.axis = f.r.axis;
     fb.side = f.r.side;
-    launch_bc(a, c->ns, fb, f.type, f.type == OSTC_BC_FARFIELD ? c->d_ff + (size_t)f.ff * 26 : nullptr, c->st, n);
+    const double* ff = f.type == OSTC_BC_FARFIELD ? c->d_ff + (size_t)f.ff * 26
+                       : f.type == OSTC_BC_INFLOW ? c->d_inflow + (size_t)f.ff * ostc::kInflowDoubles
+                                                  : nullptr;
+    launch_bc(a, c->ns, fb, f.type, ff, f.type == OSTC_BC_INFLOW ? c->d_inflow + f.y_off : nullptr, c->d_pass + b,
+              c->st, n);
   }
+  if (c->inflow_noise) launch_count_pass(c->d_pass + b, c->st, n);
 }
 
 // data_communication, inner faces in block order then face order
@@ -320,9 +334,11 @@ static int add_face(ostc_fr_ctx c, int b, const FaceRange& r, int type, const os
     c->farfields.push_back(*ff);
     return OSTC_OK;
   }
+  if (type == OSTC_BC_INFLOW)
+    return fail(c, OSTC_ERR_ARG, "inflow faces (label 5) need their state: use ostc_fr_add_inflow");
   if (type != OSTC_BC_OUTFLOW && type != OSTC_BC_PERIODIC)
     return fail(c, OSTC_ERR_UNSUPPORTED, "mixture path: boundary label " + std::to_string(type) +
-                                             " not implemented (farfield 4, outflow 6, periodic 8 are)");
+                                             " not implemented (farfield 4, inflow 5, outflow 6, periodic 8 are)");
   c->blocks[b].faces.push_back({type, -1, r});
   return OSTC_OK;
 }
@@ -341,6 +357,33 @@ int ostc_fr_add_farfield(ostc_fr_ctx c, int b, int axis, int side, const ostc_fa
   if (int r = check_block(c, b)) return r;
   if (axis < 0 || axis > 2 || (side != -1 && side != 1) || !ff) return fail(c, OSTC_ERR_ARG, "bad face");
   return add_face(c, b, ostc::whole_face(axis, side, c->blocks[b].ext(), c->ngg), OSTC_BC_FARFIELD, ff);
+}
+
+int ostc_fr_add_inflow(ostc_fr_ctx c, int b, const int range[6], const ostc_inflow* in, const double* y) {
+  if (!c) return OSTC_ERR_ARG;
+  if (int r = check_block(c, b)) return r;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_fr_finalize");
+  if (!range || !in || !y) return fail(c, OSTC_ERR_ARG, "bad inflow face");
+  if ((in->inflow_type != 0 && in->inflow_type != 2) || (in->fluctuation_type != 0 && in->fluctuation_type != 1))
+    return fail(c, OSTC_ERR_UNSUPPORTED, "inflow_type 0/2 and fluctuation_type 0/1 are implemented");
+  if (in->inflow_type == 2 && !(in->delta_omega > 0)) return fail(c, OSTC_ERR_ARG, "inflow: delta_omega > 0");
+  for (const double* S : {in->upper, in->lower})
+    if (!(S[0] > 0) || !(S[4] > 0) || !(S[5] > 0)) return fail(c, OSTC_ERR_ARG, "inflow: rho, p, T > 0");
+  const HBlock& h = c->blocks[b];
+  FaceRange fr{};
+  std::string err;
+  if (!ostc::boundary_range(range, h.ext(), c->ngg, &fr, &err)) return fail(c, OSTC_ERR_ARG, err);
+  FaceDesc fd{OSTC_BC_INFLOW, (int)(c->inflows.size() / ostc::kInflowDoubles), fr, {}, 0};
+  const int G = c->ngg;
+  const long long sx = h.mx + 2 * G, sy = h.my + 2 * G;
+  for (int k = fr.lo[2]; k <= fr.hi[2]; ++k)
+    for (int j = fr.lo[1]; j <= fr.hi[1]; ++j)
+      for (int i = fr.lo[0]; i <= fr.hi[0]; ++i) fd.y.push_back(y[((long long)(k + G) * sy + (j + G)) * sx + (i + G)]);
+  c->inflows.resize(c->inflows.size() + ostc::kInflowDoubles);
+  ostc::pack_inflow(*in, c->inflows.data() + c->inflows.size() - ostc::kInflowDoubles);
+  if (in->fluctuation_type == 1) c->inflow_noise = true;
+  c->blocks[b].faces.push_back(std::move(fd));
+  return OSTC_OK;
 }
 
 int ostc_fr_add_boundary_range(ostc_fr_ctx c, int b, const int range[6], int type, const ostc_farfield* ff) {
@@ -533,6 +576,23 @@ int ostc_fr_finalize(ostc_fr_ctx c) {
     FR_HIP(c, hipMemcpyAsync(c->d_ff, c->farfields.data(), c->farfields.size() * sizeof(ostc_farfield),
                              hipMemcpyHostToDevice, c->st));
     FR_HIP(c, hipStreamSynchronize(c->st));
+  }
+  if (!c->inflows.empty()) {
+    std::vector<double> h = c->inflows;  // states, then the y arrays of the inflow faces
+    for (auto& B : c->blocks)
+      for (auto& f : B.faces)
+        if (f.type == OSTC_BC_INFLOW) {
+          f.y_off = h.size();
+          h.insert(h.end(), f.y.begin(), f.y.end());
+        }
+    if (int r = dalloc(c, &c->d_inflow, h.size())) return r;
+    FR_HIP(c, hipMemcpyAsync(c->d_inflow, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    FR_HIP(c, hipStreamSynchronize(c->st));
+  }
+  {
+    double* p = nullptr;  // one 8-byte counter per block
+    if (int r = dalloc(c, &p, c->blocks.size())) return r;
+    c->d_pass = reinterpret_cast<unsigned long long*>(p);
   }
   long long n = 0;
   launch_dt_set(c->d_dt, c->s.fixed_time_step ? c->s.dt : 1e6, c->st, &n);

@@ -18,6 +18,7 @@
 #include <algorithm>
 
 #include "ostc_fr.hpp"
+#include "ostc_inflow.hpp"
 
 namespace ostc {
 namespace fr {
@@ -392,7 +393,8 @@ __device__ void farfield_point(const FrArgs& A, int axis, int side, const double
 }
 
 template <int NS>
-__global__ __launch_bounds__(kT) void k_bc(FrArgs A, FaceBox fb, int type, const double* ff) {
+__global__ __launch_bounds__(kT) void k_bc(FrArgs A, FaceBox fb, int type, const double* ff, const double* y,
+                                           const unsigned long long* pass) {
   const FrBlockDev& B = A.B;
   const int ext[3] = {B.mx, B.my, B.mz};
   const int axis = fb.axis, side = fb.side;
@@ -404,6 +406,23 @@ __global__ __launch_bounds__(kT) void k_bc(FrArgs A, FaceBox fb, int type, const
   ix[2] = fb.lo[2] + q / (fb.n[0] * fb.n[1]);
   if (type == 4) {
     farfield_point<NS>(A, axis, side, ff, ix);
+    return;
+  }
+  if (type == 5) {  // apply_inflow, src/BoundCond.cuh:235-612
+    double st[6];
+    const bool upper = ostc::inflow_state(ff, y[q], *pass, ostc::inflow_point(axis, ix[0], ix[1], ix[2], B.mx, B.my, B.g), st);
+    const double* sv_in = ff + (upper ? ostc::kInflowSvUpper : ostc::kInflowSvLower);
+    const long long s = B.sz;
+    for (int g = 0; g <= B.g; ++g) {
+      int gi[3] = {ix[0], ix[1], ix[2]};
+      gi[axis] += g * side;
+      const long long pg = pid(B, gi[0], gi[1], gi[2]);
+#pragma unroll
+      for (int l = 0; l < 6; ++l) B.bv[pg + l * s] = st[l];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) B.sv[pg + l * s] = sv_in[l];
+      cv_point<NS>(A, pg);
+    }
     return;
   }
   int src[3] = {ix[0], ix[1], ix[2]};
@@ -1255,10 +1274,21 @@ void launch_dt_set(double* dt, double v, hipStream_t st, long long* n) {
   ++*n;
 }
 
-void launch_bc(const FrArgs& A, int ns, const FaceBox& fb, int type, const double* ff, hipStream_t st,
-               long long* n) {
+void launch_bc(const FrArgs& A, int ns, const FaceBox& fb, int type, const double* ff, const double* y,
+               const unsigned long long* pass, hipStream_t st, long long* n) {
   const long long cnt = (long long)fb.n[0] * fb.n[1] * fb.n[2];
-  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_bc<NS>, dim3(nblk(cnt)), dim3(kT), 0, st, A, fb, type, ff))
+  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_bc<NS>, dim3(nblk(cnt)), dim3(kT), 0, st, A, fb, type, ff, y, pass))
+  ++*n;
+}
+
+// One more boundary-condition pass of a block: the counter that keys the
+// inflow's white noise (a kernel, so that a captured graph repeats it).
+__global__ void k_count_pass(unsigned long long* pass) {
+  if (threadIdx.x == 0) *pass += 1;
+}
+
+void launch_count_pass(unsigned long long* pass, hipStream_t st, long long* n) {
+  hipLaunchKernelGGL(k_count_pass, dim3(1), dim3(64), 0, st, pass);
   ++*n;
 }
 

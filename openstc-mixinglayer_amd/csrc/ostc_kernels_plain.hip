@@ -476,6 +476,20 @@ void ops_launch(const DevBlock* d_blocks, int n_blocks, const DevOp* d_ops, int 
   *n += 1;
 }
 
+// One more boundary-condition pass for blocks lo..hi-1 (the counter that
+// keys the inflow's white noise).  A kernel rather than a memory operation, so
+// that it records into a captured graph like the ops it follows.
+__global__ void k_count_pass(unsigned long long* pass, int lo, int hi) {
+  const int b = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b < hi) pass[b] += 1;
+}
+
+void count_pass(unsigned long long* pass, int lo, int hi, hipStream_t st, long long* n) {
+  if (hi <= lo) return;
+  hipLaunchKernelGGL(k_count_pass, dim3((unsigned)((hi - lo + 63) / 64)), dim3(64), 0, st, pass, lo, hi);
+  *n += 1;
+}
+
 void store_last(const DevBlock& B, hipStream_t st) {
   hipLaunchKernelGGL(k_store_last, dim3((unsigned)((B.n0 + 255) / 256)), dim3(256), 0, st, B);
 }

@@ -4,6 +4,7 @@
 // per launch (the batched path).
 #pragma once
 #include "ostc_device.hpp"
+#include "ostc_inflow.hpp"
 
 namespace ostc {
 
@@ -15,6 +16,7 @@ enum OpType : int {
   OP_CV_FROM_BV = 4,   // compute_cv_from_bv, src/FieldOperation.cuh:40-76
   OP_COPY_BV = 5,      // bv of slot `src` -> slot `slot` (ghost rims read by boundary ops)
   OP_BC_FARFIELD = 6,  // apply_farfield<Air>, src/BoundCond.cuh:755-1022
+  OP_BC_INFLOW = 7,    // apply_inflow<Air>, src/BoundCond.cuh:235-612
 };
 
 // Doubles of one far-field state on the device, the layout of ostc_farfield:
@@ -34,7 +36,10 @@ struct DevOp {
   int flags;           // OP_COMM bit 0: also recompute the target's cv on the shared layer
   long long begin;     // first point of this op in its launch range
   long long count;     // n[0]*n[1]*n[2]
-  const double* ff;    // OP_BC_FARFIELD: the state of the face's label (kFarfieldDoubles)
+  const double* ff;    // OP_BC_FARFIELD: the state of the face's label (kFarfieldDoubles);
+                       // OP_BC_INFLOW: the inflow state (kInflowDoubles)
+  const double* aux;   // OP_BC_INFLOW: y of every op point (the face layer), in op point order
+  const unsigned long long* pass;  // OP_BC_INFLOW: boundary-condition passes per block so far
 };
 
 // One op point.  Every op reads and writes the ring slot `slot` of cv/bv/mul.
@@ -79,6 +84,16 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
                                                          : pidx(B, i, j, k);
         for (int l = 0; l < 6; ++l) bv[pg + l * s] = bv[ps + l * s];
         cv_point(B, pg);
+      }
+      break;
+    }
+    case OP_BC_INFLOW: {
+      double st[6];
+      inflow_state(op.ff, op.aux[local], op.pass[op.block], inflow_point(ax, i, j, k, B.mx, B.my, B.g), st);
+      for (int g = 0; g <= B.g; ++g) {
+        const long long q = pidx(B, i + g * di, j + g * dj, k + g * dk);
+        for (int l = 0; l < 6; ++l) bv[q + l * s] = st[l];
+        cv_point(B, q);
       }
       break;
     }

@@ -84,6 +84,7 @@ SIGNATURES = {
     "ostc_fr_add_farfield": (_I, [_P, _I, _I, _I, C.POINTER(_pkg().Farfield)]),
     "ostc_fr_add_boundary_range": (_I, [_P, _I, C.POINTER(_I), _I, C.POINTER(_pkg().Farfield)]),
     "ostc_fr_add_inner_face_range": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), _I]),
+    "ostc_fr_add_inflow": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_pkg().Inflow), _DP]),
     "ostc_fr_ngg": (_I, [_P, C.POINTER(_I)]),
     "ostc_fr_store_last_step": (_I, [_P]),
     "ostc_fr_residual": (_I, [_P, _DP]),
@@ -170,6 +171,13 @@ class FrSolver:
             self._chk(L.ostc_fr_add_block(self.ctx, C.byref(_BlockDesc(b["mx"], b["my"], b["mz"])), None))
         ints = _pkg().ints
         for f in case["boundaries"]:
+            if f["type"] == 5:  # inflow: its state and the y of the block's nodes
+                blk = case["blocks"][f["block"]]
+                y = np.ascontiguousarray(pkg.cases.node_coordinates(case, f["block"], self.ngg)[1])
+                self._chk(L.ostc_fr_add_inflow(self.ctx, f["block"], ints(pkg.cases.face_range(blk, f)),
+                                               C.byref(pkg.inflow_struct(f["inflow"])),
+                                               y.ctypes.data_as(C.POINTER(C.c_double))))
+                continue
             ff = farfield_state(self.T, f["farfield_y"], **f["farfield"]) if f["type"] == 4 else None
             if "range" in f:  # a boundary N.txt record
                 self._chk(L.ostc_fr_add_boundary_range(self.ctx, f["block"], ints(f["range"]), f["type"],

@@ -95,6 +95,7 @@ def lib(tangent: bool = False):
         L.orc_add_inner_face.argtypes = [vp, i, i, i, i]
         L.orc_add_boundary_range.argtypes = [vp, i, C.POINTER(C.c_int), i, C.c_void_p]
         L.orc_add_inner_face_range.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), i]
+        L.orc_add_inflow.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(Inflow)]
         L.orc_add_farfield.argtypes = [vp, i, i, i, C.POINTER(Farfield)]
         L.orc_farfield_state_air.argtypes = [C.POINTER(FarfieldSpec), C.POINTER(Farfield)]
         L.orc_array.restype = dp
@@ -130,6 +131,33 @@ def lib(tangent: bool = False):
     return _libs[tangent]
 
 
+class Inflow(C.Structure):
+    """orc_inflow (oracle/ostc_oracle.h)."""
+    _fields_ = [("inflow_type", C.c_int), ("fluctuation_type", C.c_int), ("delta_omega", C.c_double),
+                ("fluctuation_intensity", C.c_double), ("seed", C.c_uint64), ("upper", C.c_double * 6),
+                ("lower", C.c_double * 6), ("sv_upper", C.c_double * 16), ("sv_lower", C.c_double * 16)]
+
+
+def inflow_struct(d: dict) -> Inflow:
+    pad = lambda v: list(v) + [0.0] * (16 - len(v))
+    return Inflow(int(d.get("inflow_type", 2)), int(d.get("fluctuation_type", 0)), float(d.get("delta_omega", 0.0)),
+                  float(d.get("fluctuation_intensity", 0.0)), int(d.get("seed", 0)),
+                  (C.c_double * 6)(*d["upper"]), (C.c_double * 6)(*d["lower"]),
+                  (C.c_double * 16)(*pad(d.get("sv_upper", []))), (C.c_double * 16)(*pad(d.get("sv_lower", []))))
+
+
+def face_record(blk: dict, f: dict):
+    """The boundary N.txt record of a face: its ``range`` or the whole face."""
+    if "range" in f:
+        return list(f["range"])
+    ext = (blk["mx"], blk["my"], blk["mz"])
+    r = []
+    for d in range(3):
+        r += [0, ext[d] - 1]
+    r[2 * f["axis"]] = r[2 * f["axis"] + 1] = 0 if f["side"] < 0 else ext[f["axis"]] - 1
+    return r
+
+
 def _ints(v):
     """A C int[6] of a connectivity record."""
     return (C.c_int * len(v))(*[int(x) for x in v])
@@ -158,6 +186,12 @@ class Oracle:
                 raise ValueError("bad block")
             self.nblocks += 1
         for f in case["boundaries"]:
+            if f["type"] == 5:
+                r = L.orc_add_inflow(self.ctx, f["block"], _ints(face_record(case["blocks"][f["block"]], f)),
+                                     C.byref(inflow_struct(f["inflow"])))
+                if r != 0:
+                    raise ValueError(f"bad inflow face {f}")
+                continue
             ff = farfield_state_air(**f["farfield"]) if f["type"] == 4 else None
             if "range" in f:
                 r = L.orc_add_boundary_range(self.ctx, f["block"], _ints(f["range"]), f["type"],

@@ -51,6 +51,7 @@ def lib():
         L.orf_add_inner_face.argtypes = [vp, i, i, i, i]
         L.orf_add_inner_face_range.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), i]
         L.orf_add_boundary_range.argtypes = [vp, i, C.POINTER(C.c_int), i, C.c_void_p]
+        L.orf_add_inflow.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(OA.Inflow), C.POINTER(C.c_double)]
         L.orf_array.restype = dp
         L.orf_array.argtypes = [vp, i, i, C.POINTER(C.c_longlong)]
         L.orf_initialize.argtypes = [vp]
@@ -202,6 +203,19 @@ class FrOracle:
         for b in case["blocks"]:
             L.orf_add_block(self.ctx, b["mx"], b["my"], b["mz"])
         for f in case["boundaries"]:
+            if f["type"] == 5:
+                blk = case["blocks"][f["block"]]
+                g = L.orf_ngg(self.ctx)
+                jj = np.arange(-g, blk["my"] + g)
+                y = np.ascontiguousarray(np.broadcast_to((blk["y0"] + jj * case["dy"])[None, :, None],
+                                                         (blk["mz"] + 2 * g, blk["my"] + 2 * g, blk["mx"] + 2 * g)))
+                if "axes" in blk:
+                    raise ValueError("inflow faces on blocks with their own frame: give the y array")
+                r = L.orf_add_inflow(self.ctx, f["block"], OA._ints(OA.face_record(blk, f)),
+                                     C.byref(OA.inflow_struct(f["inflow"])), _dp(y))
+                if r != 0:
+                    raise ValueError(f"bad inflow face {f}")
+                continue
             ff = farfield_state(self.T, f["farfield_y"], **f["farfield"]) if f["type"] == 4 else None
             if "range" in f:
                 r = L.orf_add_boundary_range(self.ctx, f["block"], OA._ints(f["range"]), f["type"],

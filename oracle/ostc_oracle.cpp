@@ -15,6 +15,7 @@
 // independent numpy restatement of the whole RHS on tiny grids.
 #include "ostc_oracle.h"
 #include "ostc_oracle_faces.h"
+#include "ostc_oracle_inflow.h"
 
 #include <algorithm>
 #include <cmath>
@@ -65,6 +66,7 @@ struct Face {       // physical boundary face, src/Mesh.cpp:14-32
   int axis, side, type;
   orc_farfield ff;   // type ORC_BC_FARFIELD: the state of its label
   int lo[3], hi[3];  // the face layer; tangential node range widened by ngg
+  orc_inflow in;     // type ORC_BC_INFLOW: the inflow of its label
 };
 
 using InnerFace = orc_faces::Inner;  // 1-to-1 inner face, src/Mesh.cpp:38-176
@@ -76,6 +78,7 @@ struct Block {
   std::vector<double> jac, metric, x, y, z;
   std::vector<Face> faces;
   std::vector<InnerFace> inner;
+  unsigned long long bc_passes = 0;  // boundary-condition passes so far (keys the inflow noise)
 
   long long id(int i, int j, int k) const { return (k + g) * sx * sy + (j + g) * sx + (i + g); }
   long long id0(int i, int j, int k) const { return (long long)k * mx * my + (long long)j * mx + i; }
@@ -1008,15 +1011,76 @@ void apply_farfield(Block& b, const Face& f) {
       }
 }
 
+// apply_inflow<Air>, src/BoundCond.cuh:235-612 (inflow_type 2: the
+// mixing-layer profile, :420-482; otherwise the constant state, :483-522),
+// fluctuation_type 1 (white noise, the random states indexed by the point's
+// position in the padded face plane).  The face point and its ghost layers
+// take the same state.
+void apply_inflow(Block& b, const Face& f) {
+  const orc_inflow& in = f.in;
+  const int g = b.g;
+  const long long s = b.sz;
+  int dir[3] = {0, 0, 0};
+  dir[f.axis] = f.side;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = f.lo[2]; k <= f.hi[2]; ++k)
+    for (int j = f.lo[1]; j <= f.hi[1]; ++j)
+      for (int i = f.lo[0]; i <= f.hi[0]; ++i) {
+        const double y = b.y[b.idg(i, j, k)];
+        double u;
+        bool upper = true;
+        if (in.inflow_type == 2) {
+          u = 0.5 * (in.upper[1] + in.lower[1]) + 0.5 * (in.upper[1] - in.lower[1]) * std::tanh(2 * y / in.delta_omega);
+          upper = y >= 0;
+        } else {
+          u = in.upper[1];
+        }
+        const double* S = upper ? in.upper : in.lower;
+        const double density = S[0], w = S[3], p = S[4], T = S[5];
+        double v = S[2];
+        if (in.fluctuation_type == 1) {
+          const double vel = std::sqrt(u * u + v * v + w * w);
+          const uint64_t point = f.axis == 0 ? (uint64_t)(k + g) * (b.my + 2 * g) + (j + g)
+                                 : f.axis == 1 ? (uint64_t)(k + g) * (b.mx + 2 * g) + (i + g)
+                                               : (uint64_t)(j + g) * (b.mx + 2 * g) + (i + g);
+          const double rms = in.fluctuation_intensity;
+          if (in.inflow_type == 2) {
+            if (y < 4 * in.delta_omega && y > -4 * in.delta_omega) {
+              double n1, n2;
+              orc_inflow_noise::normal_pair(in.seed, b.bc_passes, point, &n1, &n2);
+              u += n1 * rms * vel;
+              v += n2 * rms * vel;
+            }
+          } else {
+            double n1, n2;
+            orc_inflow_noise::normal_pair(in.seed, b.bc_passes, point, &n1, &n2);
+            u += n1 * rms * vel;
+          }
+        }
+        for (int gg = 0; gg <= g; ++gg) {
+          const long long q = b.id(i + gg * dir[0], j + gg * dir[1], k + gg * dir[2]);
+          b.bv[q] = density;
+          b.bv[q + s] = u;
+          b.bv[q + 2 * s] = v;
+          b.bv[q + 3 * s] = w;
+          b.bv[q + 4 * s] = p;
+          b.bv[q + 5 * s] = T;
+          cv_from_bv_point(b, q);
+        }
+      }
+}
+
 void apply_bc(Block& b) {
-  // priority: farfield(4), outflow(6), ..., periodic(8) (src/BoundCond.cuh:1488-1678)
-  const int order[] = {ORC_BC_FARFIELD, ORC_BC_OUTFLOW, ORC_BC_PERIODIC};
+  // priority: farfield(4), outflow(6), inflow(5), ..., periodic(8) (src/BoundCond.cuh:1488-1678)
+  const int order[] = {ORC_BC_FARFIELD, ORC_BC_OUTFLOW, ORC_BC_INFLOW, ORC_BC_PERIODIC};
   for (int t : order)
     for (const Face& f : b.faces)
       if (f.type == t) {
         if (t == ORC_BC_FARFIELD) apply_farfield(b, f);
+        else if (t == ORC_BC_INFLOW) apply_inflow(b, f);
         else apply_face(b, f);
       }
+  ++b.bc_passes;
 }
 
 // inner_communication, src/DataCommunication.cuh:60-105: the shared node
@@ -1107,7 +1171,7 @@ int add_face(Ctx* c, int block, const int range[6], int type, const orc_farfield
   const int ext[3] = {b.mx, b.my, b.mz};
   orc_faces::Boundary r;
   if (!orc_faces::register_boundary(range, c->ngg, ext, &r)) return -1;
-  Face f{r.face, r.direction, type, ff ? *ff : orc_farfield{}, {}, {}};
+  Face f{r.face, r.direction, type, ff ? *ff : orc_farfield{}, {}, {}, {}};
   for (int i = 0; i < 3; ++i) {
     f.lo[i] = r.range_start[i];
     f.hi[i] = r.range_end[i];
@@ -1161,6 +1225,14 @@ int orc_add_farfield(void* ctx, int block, int axis, int side, const orc_farfiel
   int r[6];
   whole_face_record(c->blocks[block], axis, side, r);
   return add_face(c, block, r, ORC_BC_FARFIELD, ff);
+}
+
+int orc_add_inflow(void* ctx, int block, const int range[6], const orc_inflow* in) {
+  auto* c = static_cast<Ctx*>(ctx);
+  if (block < 0 || block >= (int)c->blocks.size() || !in) return -1;
+  if (add_face(c, block, range, ORC_BC_INFLOW, nullptr) != 0) return -1;
+  c->blocks[block].faces.back().in = *in;
+  return 0;
 }
 
 int orc_add_boundary_range(void* ctx, int block, const int range[6], int type, const orc_farfield* ff) {

@@ -50,7 +50,18 @@ typedef struct {
 enum { ORC_CV = 0, ORC_BV, ORC_QN, ORC_DQ, ORC_MUL, ORC_MACH, ORC_JAC, ORC_METRIC,
        ORC_DT_LOCAL, ORC_X, ORC_Y, ORC_Z, ORC_BV_LAST };
 
-enum { ORC_BC_FARFIELD = 4, ORC_BC_OUTFLOW = 6, ORC_BC_PERIODIC = 8 };
+enum { ORC_BC_FARFIELD = 4, ORC_BC_INFLOW = 5, ORC_BC_OUTFLOW = 6, ORC_BC_PERIODIC = 8 };
+
+/* Inflow of one boundary label (Inflow members, src/BoundCond.cpp:10-321):
+ * inflow_type 2 mixing layer / 0 constant, fluctuation_type 0 none / 1 white
+ * noise from the counter-based generator seeded by `seed`. */
+typedef struct {
+  int inflow_type, fluctuation_type;
+  double delta_omega, fluctuation_intensity;
+  unsigned long long seed;
+  double upper[6], lower[6];
+  double sv_upper[16], sv_lower[16];
+} orc_inflow;
 
 /* Far-field state of one boundary label: the members of FarField after its
  * constructor (src/BoundCond.h:129-149).  sv holds mass fractions on the
@@ -87,6 +98,8 @@ int     orc_add_inner_face(void* ctx, int block, int axis, int side, int target_
  * type ORC_BC_FARFIELD) and one inner N.txt line pair (1-based, the paired
  * tangential axis marked negative; target_block 0-based). */
 int     orc_add_boundary_range(void* ctx, int block, const int range[6], int type, const orc_farfield* ff);
+/* Inflow face (label 5) over a boundary N.txt range; y from the block's own coordinates. */
+int     orc_add_inflow(void* ctx, int block, const int range[6], const orc_inflow* in);
 int     orc_add_inner_face_range(void* ctx, int block, const int range[6], const int target_range[6],
                                  int target_block);
 /* farfield face (label 4) with its state, apply_farfield<Air> (src/BoundCond.cuh:755-1022) */

@@ -23,6 +23,7 @@
 // is outside its shared-memory arrays.
 #include "ostc_oracle_fr.h"
 #include "ostc_oracle_faces.h"
+#include "ostc_oracle_inflow.h"
 
 #include <algorithm>
 #include <cmath>
@@ -565,6 +566,8 @@ struct Face {
   int axis, side, type;
   orf_farfield ff;   // type 4: the state of its label
   int lo[3], hi[3];  // the face layer; tangential node range widened by ngg (src/Mesh.cpp:14-32)
+  orf_inflow in;     // type 5: the inflow of its label
+  std::vector<double> y;  // type 5: y of the face points, i fastest over lo..hi
 };
 using InnerFace = orc_faces::Inner;  // 1-to-1 inner face, src/Mesh.cpp:38-176
 
@@ -574,6 +577,7 @@ struct Block {
   std::vector<double> cv, bv, sv, qn, dq, mul, lam, cp, gam, cc, mach, rhod, jac, metric, dtl, bv_last;
   std::vector<Face> faces;
   std::vector<InnerFace> inner;
+  unsigned long long bc_passes = 0;  // boundary-condition passes so far (keys the inflow noise)
   long long id(int i, int j, int k) const { return (k + g) * sx * sy + (j + g) * sx + (i + g); }
   long long id0(int i, int j, int k) const { return (long long)k * mx * my + (long long)j * mx + i; }
   long long n0() const { return (long long)mx * my * mz; }
@@ -1315,8 +1319,61 @@ void update(const Ctx& c, Block& b, double dt, int rk) {
       }
 }
 
-int bc_priority(int type) {  // src/BoundCond.cuh:1488-1678: farfield, outflow, ..., periodic
-  return type == 4 ? 0 : (type == 6 ? 1 : 2);
+int bc_priority(int type) {  // src/BoundCond.cuh:1488-1678: farfield, outflow, inflow, ..., periodic
+  return type == 4 ? 0 : (type == 6 ? 1 : (type == 5 ? 2 : 3));
+}
+
+// apply_inflow with mass fractions, src/BoundCond.cuh:235-612: inflow_type 2
+// (the mixing-layer profile, :420-482) or the constant state (:483-522),
+// white noise (fluctuation_type 1) keyed by the point's position in the
+// padded face plane; the face point and its ghost layers take the same state.
+void inflow_point(const Ctx& c, Block& b, const Face& f, int i, int j, int k, double y) {
+  const orf_inflow& in = f.in;
+  const int g = b.g;
+  const long long s = b.sz;
+  double u;
+  bool upper = true;
+  if (in.inflow_type == 2) {
+    u = 0.5 * (in.upper[1] + in.lower[1]) + 0.5 * (in.upper[1] - in.lower[1]) * std::tanh(2 * y / in.delta_omega);
+    upper = y >= 0;
+  } else {
+    u = in.upper[1];
+  }
+  const double* S = upper ? in.upper : in.lower;
+  const double* sv = upper ? in.sv_upper : in.sv_lower;
+  double v = S[2];
+  const double w = S[3];
+  if (in.fluctuation_type == 1) {
+    const double vel = std::sqrt(u * u + v * v + w * w);
+    const uint64_t point = f.axis == 0 ? (uint64_t)(k + g) * (b.my + 2 * g) + (j + g)
+                           : f.axis == 1 ? (uint64_t)(k + g) * (b.mx + 2 * g) + (i + g)
+                                         : (uint64_t)(j + g) * (b.mx + 2 * g) + (i + g);
+    const double rms = in.fluctuation_intensity;
+    double n1, n2;
+    if (in.inflow_type == 2) {
+      if (y < 4 * in.delta_omega && y > -4 * in.delta_omega) {
+        orc_inflow_noise::normal_pair(in.seed, b.bc_passes, point, &n1, &n2);
+        u += n1 * rms * vel;
+        v += n2 * rms * vel;
+      }
+    } else {
+      orc_inflow_noise::normal_pair(in.seed, b.bc_passes, point, &n1, &n2);
+      u += n1 * rms * vel;
+    }
+  }
+  int dir[3] = {0, 0, 0};
+  dir[f.axis] = f.side;
+  for (int gg = 0; gg <= g; ++gg) {
+    const long long q = b.id(i + gg * dir[0], j + gg * dir[1], k + gg * dir[2]);
+    b.bv[q] = S[0];
+    b.bv[q + s] = u;
+    b.bv[q + 2 * s] = v;
+    b.bv[q + 3 * s] = w;
+    b.bv[q + 4 * s] = S[4];
+    b.bv[q + 5 * s] = S[5];
+    for (int l = 0; l < c.ns; ++l) b.sv[q + l * s] = sv[l];
+    cv_point(c, b, q);
+  }
 }
 
 // apply_farfield<FR>, src/BoundCond.cuh:755-1022, one face point (i, j, k)
@@ -1426,6 +1483,12 @@ void apply_bc(const Ctx& c, Block& b) {
           farfield_point(c, b, f, ix[0], ix[1], ix[2]);
           continue;
         }
+        if (f.type == 5) {
+          const long long q = ((long long)(ix[2] - f.lo[2]) * (f.hi[1] - f.lo[1] + 1) + (ix[1] - f.lo[1])) *
+                                  (f.hi[0] - f.lo[0] + 1) + (ix[0] - f.lo[0]);
+          inflow_point(c, b, f, ix[0], ix[1], ix[2], f.y[q]);
+          continue;
+        }
         int src[3] = {ix[0], ix[1], ix[2]};
         if (f.type == 8) src[a] = f.side < 0 ? ext[a] - 1 : 0;
         for (int g = 1; g <= b.g; ++g) {
@@ -1439,6 +1502,7 @@ void apply_bc(const Ctx& c, Block& b) {
         }
       }
   }
+  ++b.bc_passes;
 }
 
 // inner_communication with scalars, src/DataCommunication.cuh:60-105: per
@@ -1522,7 +1586,7 @@ int add_face(Ctx* c, int block, const int range[6], int type, const orf_farfield
   const int ext[3] = {b.mx, b.my, b.mz};
   orc_faces::Boundary r;
   if (!orc_faces::register_boundary(range, c->ngg, ext, &r)) return -1;
-  Face f{r.face, r.direction, type, ff ? *ff : orf_farfield{}, {}, {}};
+  Face f{r.face, r.direction, type, ff ? *ff : orf_farfield{}, {}, {}, {}, {}};
   for (int i = 0; i < 3; ++i) {
     f.lo[i] = r.range_start[i];
     f.hi[i] = r.range_end[i];
@@ -1606,6 +1670,20 @@ int orf_add_farfield(void* ctx, int block, int axis, int side, const orf_farfiel
   int r[6];
   whole_face_record(C(ctx)->blocks[block], axis, side, r);
   return add_face(C(ctx), block, r, 4, ff);
+}
+
+int orf_add_inflow(void* ctx, int block, const int range[6], const orf_inflow* in, const double* y) {
+  Ctx* c = C(ctx);
+  if (!in || !y || add_face(c, block, range, 5, nullptr) != 0) return -1;
+  Block& b = c->blocks[block];
+  Face& f = b.faces.back();
+  f.in = *in;
+  const long long sx = b.mx + 2 * b.g, sy = b.my + 2 * b.g;
+  for (int k = f.lo[2]; k <= f.hi[2]; ++k)
+    for (int j = f.lo[1]; j <= f.hi[1]; ++j)
+      for (int i = f.lo[0]; i <= f.hi[0]; ++i)
+        f.y.push_back(y[((long long)(k + b.g) * sy + (j + b.g)) * sx + (i + b.g)]);
+  return 0;
 }
 
 int orf_add_boundary_range(void* ctx, int block, const int range[6], int type, const orf_farfield* ff) {

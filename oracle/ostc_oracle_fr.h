@@ -70,6 +70,16 @@ int     orf_farfield_state(const ostc_mech_tables* t, const orf_farfield_spec* i
 /* farfield face (label 4), apply_farfield<FR> (src/BoundCond.cuh:755-1022) */
 int     orf_add_farfield(void* ctx, int block, int axis, int side, const orf_farfield* ff);
 int     orf_add_boundary_range(void* ctx, int block, const int range[6], int type, const orf_farfield* ff);
+/* Inflow face (label 5): the state in the layout of orc_inflow (oracle/ostc_oracle.h) and
+ * the block's node y coordinates on the padded domain (the bv plane layout). */
+typedef struct {
+  int inflow_type, fluctuation_type;
+  double delta_omega, fluctuation_intensity;
+  unsigned long long seed;
+  double upper[6], lower[6];
+  double sv_upper[16], sv_lower[16];
+} orf_inflow;
+int     orf_add_inflow(void* ctx, int block, const int range[6], const orf_inflow* in, const double* y);
 double* orf_array(void* ctx, int block, int which, long long* n_out);
 
 int     orf_initialize(void* ctx);            /* cv from bv, properties, BCs, exchange, properties */
