@@ -27,7 +27,7 @@ from __future__ import annotations
 import os
 
 # boundary labels with an implementation here (src/BoundCond.cuh:1488-1678)
-FARFIELD, OUTFLOW, PERIODIC = 4, 6, 8
+FARFIELD, INFLOW, OUTFLOW, PERIODIC = 4, 5, 6, 8
 
 
 def file_name(kind: str, rank: int = 0) -> str:
@@ -140,14 +140,16 @@ def write_parallel(path: str, n_block: int):
 
 
 def connectivity(directory: str, n_block: int, rank: int = 0, labels: dict | None = None,
-                 farfields: dict | None = None):
+                 farfields: dict | None = None, inflows: dict | None = None):
     """(boundaries, inner_faces) of a case dict from the three files of one
     rank.  ``labels`` maps a boundary label to its type (default: 4 farfield,
-    6 outflow, 8 periodic, the labels of src/BoundCond.cuh:1488-1678);
-    ``farfields`` maps a farfield label to its state inputs
-    (``farfield_state_air`` / ``FrSolver`` keys)."""
-    labels = {FARFIELD: FARFIELD, OUTFLOW: OUTFLOW, PERIODIC: PERIODIC} if labels is None else labels
+    5 inflow, 6 outflow, 8 periodic, the labels of
+    src/BoundCond.cuh:1488-1678); ``farfields`` maps a farfield label to its
+    state inputs (``farfield_state_air`` / ``FrSolver`` keys), ``inflows`` an
+    inflow label to its ``ostc_inflow`` record (``inflow_struct`` keys)."""
+    labels = {FARFIELD: FARFIELD, INFLOW: INFLOW, OUTFLOW: OUTFLOW, PERIODIC: PERIODIC} if labels is None else labels
     farfields = farfields or {}
+    inflows = inflows or {}
     bnd = read_boundary(os.path.join(directory, file_name("boundary", rank)), n_block)
     inn = read_inner(os.path.join(directory, file_name("inner", rank)), n_block)
     par_path = os.path.join(directory, file_name("parallel", rank))
@@ -162,6 +164,10 @@ def connectivity(directory: str, n_block: int, rank: int = 0, labels: dict | Non
             if label not in farfields:
                 raise ValueError(f"farfield label {label} needs its state")
             f["farfield"] = dict(farfields[label])
+        if f["type"] == INFLOW:
+            if label not in inflows:
+                raise ValueError(f"inflow label {label} needs its state")
+            f["inflow"] = dict(inflows[label])
         boundaries.append(f)
     inner = [dict(block=blk, range=list(src), target_range=list(tgt), target=t) for blk, src, tgt, t in inn]
     return boundaries, inner

@@ -6,7 +6,7 @@ oracle on the same records.
 Tolerances (per variable, max|gpu - oracle| / max|oracle|): primitives (and
 mass fractions) after n RK3 steps <= 1e-9, as for the aligned partitions.
 The ABI refuses malformed records with OSTC_ERR_ARG and boundary labels
-without an implementation with OSTC_ERR_UNSUPPORTED.
+without an implementation (here 2, the wall) with OSTC_ERR_UNSUPPORTED.
 """
 import ctypes as C
 
@@ -129,8 +129,10 @@ def test_malformed_records_are_refused():
     for src, tgt in bad:
         assert L.ostc_add_inner_face_range(c, 0, ints(src), ints(tgt), 1) == -1, (src, tgt)
     assert L.ostc_add_inner_face_range(c, 0, ints(ok_src), ints(ok_tgt), 1) == 0
-    assert L.ostc_add_boundary_range(c, 0, ints([0, cut, 0, 0, 0, nz - 1]), 5, None) == -5
-    assert "label 5" in L.ostc_last_error(c).decode()
+    assert L.ostc_add_boundary_range(c, 0, ints([0, cut, 0, 0, 0, nz - 1]), 2, None) == -5  # wall
+    assert "label 2" in L.ostc_last_error(c).decode()
+    assert L.ostc_add_boundary_range(c, 0, ints([0, cut, 0, 0, 0, nz - 1]), 5, None) == -1  # inflow needs its state
+    assert "ostc_add_inflow" in L.ostc_last_error(c).decode()
     assert L.ostc_add_boundary_range(c, 0, ints([0, cut, 3, 3, 0, nz - 1]), 6, None) == -1
     assert L.ostc_add_boundary_range(c, 0, ints([0, cut + 1, 0, 0, 0, nz - 1]), 6, None) == -1
     assert L.ostc_add_boundary_range(c, 0, ints([0, cut, 0, 0, 0, nz - 1]), 4, None) == -1  # farfield needs a state

@@ -33,12 +33,15 @@ def test_reads_the_reference_connectivity_files():
     assert bnd[3] == (0, [550, 550, 0, 100, 0, 80], 6)
     assert mesh_io.read_inner(os.path.join(FIX, mesh_io.file_name("inner")), 1) == []
     assert mesh_io.read_parallel(os.path.join(FIX, mesh_io.file_name("parallel")), 1) == []
-    # label 5 (the mixing-layer inflow) has no implementation on this path
-    with pytest.raises(ValueError, match="label 5"):
+    # label 5 (the mixing-layer inflow) needs its state
+    with pytest.raises(ValueError, match="inflow label 5"):
         mesh_io.connectivity(FIX, 1)
-    b, inner = mesh_io.connectivity(FIX, 1, labels={5: 6, 6: 6, 8: 8})
+    state = dict(inflow_type=2, delta_omega=1e-3, upper=[1, 2, 0, 0, 1e5, 300], lower=[1, -2, 0, 0, 1e5, 300])
+    b, inner = mesh_io.connectivity(FIX, 1, inflows={5: state})
     assert len(b) == 6 and inner == []
-    assert b[2] == dict(block=0, range=[0, 0, 0, 100, 0, 80], type=6, label=5)
+    assert b[2] == dict(block=0, range=[0, 0, 0, 100, 0, 80], type=5, label=5, inflow=state)
+    with pytest.raises(ValueError, match="label 8 has no implementation"):
+        mesh_io.connectivity(FIX, 1, labels={5: 5, 6: 6}, inflows={5: state})
 
 
 def _pair(y_bc=cases.BC_OUTFLOW, viscous_order=0, scheme=51):
