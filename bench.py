@@ -101,15 +101,17 @@ def adjoint_leg(n: int, steps: int, every: int, fd_modes=(0, 1), h=1e-6):
                         f"momentum thickness"}
 
 
-def reacting_leg(x_cells, y_cells, z_cells, steps: int, warmup: int, label: str, cpu_sample=None):
-    """Reacting H2/air temporal mixing layer (9 species, 19 reactions,
-    characteristic WENO-Z5 + LF-type splitting, mixture-averaged transport,
-    finite-rate chemistry, SSP-RK3 CFL 0.5) through the finite-rate path:
-    `steps` graph-replayed steps after `warmup`, then one step with HIP events
-    around every kernel for the per-kernel breakdown."""
+def reacting_leg(x_cells, y_cells, z_cells, steps: int, warmup: int, label: str, cpu_sample=None, spatial=False):
+    """Reacting H2/air mixing layer (9 species, 19 reactions, characteristic
+    WENO-Z5 + LF-type splitting, mixture-averaged transport, finite-rate
+    chemistry, SSP-RK3 CFL 0.5) through the finite-rate path: `steps`
+    graph-replayed steps after `warmup`, then one step with HIP events around
+    every kernel for the per-kernel breakdown.  spatial: the spatially
+    developing layer (mixing-layer inflow with 1e-3 white noise at x-,
+    outflow at x+); otherwise the temporal layer (periodic x)."""
     mx = pkg.mixture
     tables = mx.load_mechanism(*mx.H2_AIR_FILES)
-    case = pkg.cases.reacting_case(x_cells, y_cells, z_cells, tables, scheme=52)
+    case = pkg.cases.reacting_case(x_cells, y_cells, z_cells, tables, scheme=52, spatial=spatial)
     cells = case["n_cells"]
     print(f"[bench] {label}: {len(case['blocks'])} blocks, {cells} cells", file=sys.stderr, flush=True)
     s = mx.FrSolver(case, tables)
@@ -138,14 +140,15 @@ def reacting_leg(x_cells, y_cells, z_cells, steps: int, warmup: int, label: str,
            "kernel_ms_per_step": {k: v[0] for k, v in kms.items()},
            "kernel_launches_per_step": {k: int(v[1]) for k, v in kms.items()},
            "dominant_kernel": top[0], "dominant_kernel_share": top[1][0] / total_k,
-           "workload": f"{label}: temporal H2/air mixing layer, 9 species / 19 reactions, WENO-Z5 characteristic "
+           "workload": f"{label}: {'spatially developing (inflow x-, outflow x+)' if spatial else 'temporal'} "
+                       f"H2/air mixing layer, 9 species / 19 reactions, WENO-Z5 characteristic "
                        f"+ LF-type splitting, mixture-averaged transport, finite-rate chemistry, SSP-RK3 CFL 0.5"}
     if cpu_sample is not None:
-        out["cpu_baseline"] = reacting_cpu_baseline(tables, *cpu_sample)
+        out["cpu_baseline"] = reacting_cpu_baseline(tables, *cpu_sample, spatial=spatial)
     return out
 
 
-def reacting_cpu_baseline(tables, n, steps):
+def reacting_cpu_baseline(tables, n, steps, spatial=False):
     """The CPU oracle of the finite-rate path on a bounded sample: `steps`
     RK3 steps of an n x n/2 x n/2 block of the same reacting layer."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -154,7 +157,7 @@ def reacting_cpu_baseline(tables, n, steps):
 
     threads = min(os.cpu_count() or 1, 16)
     to = mech.load(*pkg.mixture.H2_AIR_FILES)
-    case = pkg.cases.reacting_single_block(n, n // 2, n // 2, tables, scheme=52)
+    case = pkg.cases.reacting_single_block(n, n // 2, n // 2, tables, scheme=52, spatial=spatial)
     o = OF.FrOracle(case, to, n_threads=threads)
     o.setup_mixing_layer()
     o.initialize()
@@ -233,7 +236,7 @@ def main():
         return
 
     if a.reacting_only:
-        r = reacting_leg([511], [255], [127], a.reacting_steps, 2, "512x256x128 single block")
+        r = reacting_leg([511], [255], [127], a.reacting_steps, 2, "512x256x128 single block", spatial=True)
         print(json.dumps({"reacting": {"single_block_512x256x128": r}}))
         return
 
@@ -308,7 +311,7 @@ def main():
     if not a.no_reacting:
         reacting = {"single_block_512x256x128": reacting_leg(
             [511], [255], [127], a.reacting_steps, 2, "512x256x128 single block",
-            cpu_sample=None if a.no_cpu else (96, 2))}
+            cpu_sample=None if a.no_cpu else (96, 2), spatial=True)}
         if not a.no_large:
             rng = np.random.default_rng(42)
             xs = [int(v) for v in rng.integers(384, 640, size=1)]
