@@ -408,6 +408,31 @@ int ostc_fr_add_inner_face_range(ostc_fr_ctx ctx, int block, const int range[6],
                                  int target_block);
 /* Inflow face (label 5) of a mixture, apply_inflow with mass fractions; as ostc_add_inflow. */
 int ostc_fr_add_inflow(ostc_fr_ctx ctx, int block, const int range[6], const ostc_inflow* in, const double* y);
+
+/* Sponge layer (src/SpongeLayer.cu:28-172): at the start of every step the
+ * running mean of cv over the steps so far is updated, mean = (mean n + cv) / (n + 1)
+ * (update_sponge_layer_value), and in every stage after the chemistry source
+ * dq -= sigma (cv - mean) on every conservative variable (sponge_layer_source),
+ * sigma = sum over the enabled sides of sigma_d S(xi), xi = (s - start_d)/(end_d - start_d)
+ * clipped at 1 where the coordinate s lies beyond start_d (below it on the
+ * minus sides x-, y-, z-, above it on the plus sides), S(xi) = 6 xi^5 - 15 xi^4 + 10 xi^3
+ * (sponge_function 0).  sigma_d = spongeCoefficient * v_char / |start_d - end_d|
+ * (DParameter, src/DParameter.cu:258-282).  The mean starts at zero (a new run). */
+typedef struct {
+  int enabled[6];             /* x-, x+, y-, y+, z-, z+ (sponge_layer_direction 0..5) */
+  double start[6], end[6];    /* spongeXMinusStart, spongeXMinusEnd, spongeXPlusStart, ... */
+  double sigma[6];            /* sponge_sigma0..5 */
+  int function;               /* sponge_function: 0 */
+  int jacobian_scaled;        /* 1: dq -= (sigma J)(cv - mean), the scaling of every other source term
+                                 (the chemistry's J omega, src/FiniteRateChem.cu:36), so that the
+                                 relaxation rate is sigma; 0: the reference's term as written, without J
+                                 (src/SpongeLayer.cu:155), whose rate is sigma / J */
+} ostc_sponge;
+/* Before ostc_fr_finalize, per block: x, y, z are the block's node coordinates
+ * on the padded domain (host, the bv plane layout); the library keeps sigma of
+ * every interior node. */
+int ostc_fr_set_sponge(ostc_fr_ctx ctx, int block, const ostc_sponge* sp, const double* x, const double* y,
+                       const double* z);
 /* ghost layers of every array of the context: 3, or 4 for the WENO-Z7 schemes 71/72 (src/Parameter.cpp:227-238) */
 int ostc_fr_ngg(ostc_fr_ctx ctx, int* ngg);
 /* store_last_step / compute_residual (src/TimeAdvanceFunc.cu:6-37, src/Residual.cuh:12-187): unscaled

@@ -489,3 +489,32 @@ def global_from_block(case: dict, b: int, arr: np.ndarray, g: int):
     i, j, k = _local_grid(blk, 0)
     inner = arr[:, g:g + blk["mz"], g:g + blk["my"], g:g + blk["mx"]] if g else arr
     return (k.ravel(), j.ravel(), i.ravel()), inner.reshape(arr.shape[0], -1)
+
+
+def sponge_layer(case: dict, sides=(1, 2, 3), depth=0.2, coefficient=1.0, v_char=None,
+                 jacobian_scaled=1) -> dict:
+    """A sponge layer for ``case`` (setup.txt keys sponge_layer_direction,
+    sponge*Start/End, spongeCoefficient): on each listed side (0 x-, 1 x+,
+    2 y-, 3 y+, 4 z-, 5 z+) it ramps up over the outer ``depth`` fraction of
+    the domain; sigma_d = coefficient * v_char / |start - end|
+    (src/DParameter.cu:258-282), v_char defaulting to the velocity
+    difference of the two streams.  jacobian_scaled 1 multiplies the term
+    by the Jacobian (relaxation rate sigma); 0 keeps the reference's term,
+    whose rate is sigma / J."""
+    b = case["blocks"]
+    lo = [min(blk[o] for blk in b) for o in ("x0", "y0", "z0")]
+    hi = [max(blk[o] + (blk[m] - 1) * case[d] for blk in b) for o, m, d in
+          (("x0", "mx", "dx"), ("y0", "my", "dy"), ("z0", "mz", "dz"))]
+    st = case["streams"]
+    v = abs(st["upper"][1] - st["lower"][1]) if v_char is None else v_char
+    enabled, start, end, sigma = [0] * 6, [0.0] * 6, [0.0] * 6, [0.0] * 6
+    for side in sides:
+        a = side // 2
+        width = depth * (hi[a] - lo[a])
+        if side % 2 == 0:   # minus side: start inside, end at the boundary (start > end)
+            start[side], end[side] = lo[a] + width, lo[a]
+        else:
+            start[side], end[side] = hi[a] - width, hi[a]
+        enabled[side] = 1
+        sigma[side] = coefficient * v / abs(start[side] - end[side])
+    return dict(enabled=enabled, start=start, end=end, sigma=sigma, function=0, jacobian_scaled=jacobian_scaled)

@@ -34,6 +34,8 @@ struct HBlock {
   FrBlockDev d{};
   std::vector<FaceDesc> faces;
   std::vector<FaceMap> inner;
+  std::vector<double> sp_sigma;  // sponge sigma of every interior node (empty: none)
+  bool sp_jac = true;            // the sponge term carries the Jacobian
   int e[3];  // mx, my, mz
   const int* ext() const { return e; }
 };
@@ -60,6 +62,8 @@ struct ostc_fr_ctx_s {
   bool inflow_noise = false;             // some inflow face has white noise
   double* d_inflow = nullptr;            // inflow states, then the y of every inflow face point
   unsigned long long* d_pass = nullptr;  // boundary-condition passes per block
+  bool sponge = false;                   // a sponge layer is set on some block
+  unsigned long long* d_sp_iter = nullptr;  // steps averaged into the sponge mean
   bool finalized = false;
   double* d_dt = nullptr;          // dt of the current step (one double)
   double* d_res = nullptr;         // per-workgroup partial sums of the residual
@@ -184,6 +188,11 @@ void step(ostc_fr_ctx c, long long* n, KernelTimer* tm) {
     const FrBlockDev& d = c->blocks[b].d;
     (void)hipMemcpyAsync(d.qn, d.cv, (size_t)d.sz * c->nv * sizeof(double), hipMemcpyDeviceToDevice, c->st);
     ++*n;
+  }
+  if (c->sponge) {  // update_sponge_layer_value, then update_sponge_iter (src/RK.cuh:104-109)
+    for (int b = 0; b < nb; ++b)
+      if (c->blocks[b].d.sp_sigma) launch_sponge_mean(args(c, b), c->ns, c->d_sp_iter, c->st, n);
+    launch_count_pass(c->d_sp_iter, c->st, n);
   }
   for (int rk = 0; rk < 3; ++rk) {
     if (!c->s.fixed_time_step && rk == 0) {
@@ -383,6 +392,49 @@ int ostc_fr_add_inflow(ostc_fr_ctx c, int b, const int range[6], const ostc_infl
   ostc::pack_inflow(*in, c->inflows.data() + c->inflows.size() - ostc::kInflowDoubles);
   if (in->fluctuation_type == 1) c->inflow_noise = true;
   c->blocks[b].faces.push_back(std::move(fd));
+  return OSTC_OK;
+}
+
+// sponge_layer_source's sigma at one node (src/SpongeLayer.cu:68-160): the
+// sides in the order x-, x+, y-, y+, z-, z+, S(xi) of sponge_function 0.
+static double sponge_sigma_at(const ostc_sponge& sp, double x, double y, double z) {
+  double sigma = 0;
+  const double s3[3] = {x, y, z};
+  for (int d = 0; d < 6; ++d) {
+    if (!sp.enabled[d]) continue;
+    const double v = s3[d / 2], x1 = sp.start[d];
+    if ((d % 2 == 0) ? !(v < x1) : !(v > x1)) continue;
+    double xi = (v - x1) / (sp.end[d] - x1);
+    if (xi > 1) xi = 1;
+    const double xi3 = xi * xi * xi;
+    const double S = 6 * xi3 * xi * xi - 15 * xi3 * xi + 10 * xi3;
+    sigma += sp.sigma[d] * S;
+  }
+  return sigma;
+}
+
+int ostc_fr_set_sponge(ostc_fr_ctx c, int b, const ostc_sponge* sp, const double* x, const double* y,
+                       const double* z) {
+  if (!c) return OSTC_ERR_ARG;
+  if (int r = check_block(c, b)) return r;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "the sponge layer is fixed after ostc_fr_finalize");
+  if (!sp || !x || !y || !z) return fail(c, OSTC_ERR_ARG, "bad sponge layer");
+  if (sp->function != 0) return fail(c, OSTC_ERR_UNSUPPORTED, "sponge_function 0 is implemented");
+  for (int d = 0; d < 6; ++d)
+    if (sp->enabled[d] && !(std::fabs(sp->start[d] - sp->end[d]) >= 1e-10))
+      return fail(c, OSTC_ERR_ARG, "sponge layer: start and end of an enabled side coincide");
+  HBlock& h = c->blocks[b];
+  const int G = c->ngg;
+  const long long sx = h.mx + 2 * G, sy = h.my + 2 * G;
+  h.sp_sigma.assign((size_t)h.mx * h.my * h.mz, 0.0);
+  h.sp_jac = sp->jacobian_scaled != 0;
+  for (int k = 0; k < h.mz; ++k)
+    for (int j = 0; j < h.my; ++j)
+      for (int i = 0; i < h.mx; ++i) {
+        const long long p = ((long long)(k + G) * sy + (j + G)) * sx + (i + G);
+        h.sp_sigma[((size_t)k * h.my + j) * h.mx + i] = sponge_sigma_at(*sp, x[p], y[p], z[p]);
+      }
+  c->sponge = true;
   return OSTC_OK;
 }
 
@@ -590,9 +642,19 @@ int ostc_fr_finalize(ostc_fr_ctx c) {
     FR_HIP(c, hipStreamSynchronize(c->st));
   }
   {
-    double* p = nullptr;  // one 8-byte counter per block
-    if (int r = dalloc(c, &p, c->blocks.size())) return r;
+    double* p = nullptr;  // one 8-byte counter per block, then the sponge step counter
+    if (int r = dalloc(c, &p, c->blocks.size() + 1)) return r;
     c->d_pass = reinterpret_cast<unsigned long long*>(p);
+    c->d_sp_iter = c->d_pass + c->blocks.size();
+  }
+  for (auto& h : c->blocks) {
+    if (h.sp_sigma.empty()) continue;
+    if (int r = dalloc(c, &h.d.sp_sigma, h.sp_sigma.size())) return r;
+    if (int r = dalloc(c, &h.d.sp_mean, h.sp_sigma.size() * c->nv)) return r;
+    h.d.sp_jac = h.sp_jac ? 1 : 0;
+    FR_HIP(c, hipMemcpyAsync(h.d.sp_sigma, h.sp_sigma.data(), h.sp_sigma.size() * sizeof(double),
+                             hipMemcpyHostToDevice, c->st));
+    FR_HIP(c, hipStreamSynchronize(c->st));
   }
   long long n = 0;
   launch_dt_set(c->d_dt, c->s.fixed_time_step ? c->s.dt : 1e6, c->st, &n);

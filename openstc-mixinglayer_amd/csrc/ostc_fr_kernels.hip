@@ -1167,6 +1167,20 @@ __global__ __launch_bounds__(kT) void k_viscous(FrArgs A) {
 
 unsigned nblk(long long n) { return (unsigned)((n + kT - 1) / kT); }
 
+// sponge_layer_source (src/SpongeLayer.cu:68-160), after the chemistry source
+// of the same stage: dq -= sigma (cv - mean) on every conservative variable,
+// sigma times the Jacobian when sp_jac is set (as the chemistry's J omega).
+template <int NS>
+__global__ __launch_bounds__(kT) void k_sponge_source(FrBlockDev B) {
+  const long long t = (long long)blockIdx.x * kT + threadIdx.x;
+  if (t >= B.n0) return;
+  const int i = (int)(t % B.mx), j = (int)((t / B.mx) % B.my), k = (int)(t / ((long long)B.mx * B.my));
+  const long long p = pid(B, i, j, k);
+  const double sigma = B.sp_jac ? B.sp_sigma[t] * B.jac[p] : B.sp_sigma[t];
+#pragma unroll
+  for (int l = 0; l < 5 + NS; ++l) B.dq[t + l * B.n0] -= sigma * (B.cv[p + l * B.sz] - B.sp_mean[t + l * B.n0]);
+}
+
 template <int NS>
 void launch_rhs_t(const FrArgs& A, hipStream_t st, long long* n, KernelTimer* tm) {
   const FrBlockDev& B = A.B;
@@ -1176,8 +1190,12 @@ void launch_rhs_t(const FrArgs& A, hipStream_t st, long long* n, KernelTimer* tm
   };
   rec(KT_SOURCE, false);
   hipLaunchKernelGGL(k_source<NS>, dim3(nblk(B.n0)), dim3(kT), 0, st, A);
-  rec(KT_SOURCE, true);
   ++*n;
+  if (B.sp_sigma) {
+    hipLaunchKernelGGL(k_sponge_source<NS>, dim3(nblk(B.n0)), dim3(kT), 0, st, B);
+    ++*n;
+  }
+  rec(KT_SOURCE, true);
   auto inv = [&](auto dirc) {
     constexpr int D = decltype(dirc)::value;
     constexpr int TW = Tile<D>::TW, F = Tile<D>::F, TS = F - 1;
@@ -1289,6 +1307,25 @@ __global__ void k_count_pass(unsigned long long* pass) {
 
 void launch_count_pass(unsigned long long* pass, hipStream_t st, long long* n) {
   hipLaunchKernelGGL(k_count_pass, dim3(1), dim3(64), 0, st, pass);
+  ++*n;
+}
+
+template <int NS>
+__global__ __launch_bounds__(kT) void k_sponge_mean(FrBlockDev B, const unsigned long long* iter) {
+  const long long t = (long long)blockIdx.x * kT + threadIdx.x;
+  if (t >= B.n0) return;
+  const int i = (int)(t % B.mx), j = (int)((t / B.mx) % B.my), k = (int)(t / ((long long)B.mx * B.my));
+  const long long p = pid(B, i, j, k);
+  const double n = (double)*iter;
+#pragma unroll
+  for (int l = 0; l < 5 + NS; ++l) {
+    double& m = B.sp_mean[t + l * B.n0];
+    m = (m * n + B.cv[p + l * B.sz]) / (n + 1);
+  }
+}
+
+void launch_sponge_mean(const FrArgs& A, int ns, const unsigned long long* iter, hipStream_t st, long long* n) {
+  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_sponge_mean<NS>, dim3(nblk(A.B.n0)), dim3(kT), 0, st, A.B, iter))
   ++*n;
 }
 

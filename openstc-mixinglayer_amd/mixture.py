@@ -85,6 +85,7 @@ SIGNATURES = {
     "ostc_fr_add_boundary_range": (_I, [_P, _I, C.POINTER(_I), _I, C.POINTER(_pkg().Farfield)]),
     "ostc_fr_add_inner_face_range": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), _I]),
     "ostc_fr_add_inflow": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_pkg().Inflow), _DP]),
+    "ostc_fr_set_sponge": (_I, [_P, _I, C.c_void_p, _DP, _DP, _DP]),
     "ostc_fr_ngg": (_I, [_P, C.POINTER(_I)]),
     "ostc_fr_store_last_step": (_I, [_P]),
     "ostc_fr_residual": (_I, [_P, _DP]),
@@ -143,6 +144,18 @@ def farfield_state(tables: MechTables, y, **kw):
     return out
 
 
+class Sponge(C.Structure):
+    """ostc_sponge: the sponge layer's sides, ranges and strengths (src/SpongeLayer.cu:28-172)."""
+    _fields_ = [("enabled", C.c_int * 6), ("start", C.c_double * 6), ("end", C.c_double * 6),
+                ("sigma", C.c_double * 6), ("function", C.c_int), ("jacobian_scaled", C.c_int)]
+
+
+def sponge_struct(d: dict) -> Sponge:
+    return Sponge((C.c_int * 6)(*[int(v) for v in d["enabled"]]), (C.c_double * 6)(*d["start"]),
+                  (C.c_double * 6)(*d["end"]), (C.c_double * 6)(*d["sigma"]), int(d.get("function", 0)),
+                  int(d.get("jacobian_scaled", 1)))
+
+
 class FrSolver:
     """Blocks of a finite-rate mixture on one GPU (the reference's layout and
     operators, MixtureModel::FR)."""
@@ -192,6 +205,12 @@ class FrSolver:
                                                          ints(f["target_range"]), f["target"]))
             else:
                 self._chk(L.ostc_fr_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"]))
+        if "sponge" in case:  # the sponge layer of every block, from its node coordinates
+            sp = sponge_struct(case["sponge"])
+            for b in range(len(case["blocks"])):
+                xyz = [np.ascontiguousarray(a) for a in pkg.cases.node_coordinates(case, b, self.ngg)]
+                self._chk(L.ostc_fr_set_sponge(self.ctx, b, C.cast(C.byref(sp), C.c_void_p),
+                                               *(a.ctypes.data_as(C.POINTER(C.c_double)) for a in xyz)))
         self._chk(L.ostc_fr_finalize(self.ctx))
 
     def _chk(self, r):

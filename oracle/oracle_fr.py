@@ -52,6 +52,8 @@ def lib():
         L.orf_add_inner_face_range.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), i]
         L.orf_add_boundary_range.argtypes = [vp, i, C.POINTER(C.c_int), i, C.c_void_p]
         L.orf_add_inflow.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(OA.Inflow), C.POINTER(C.c_double)]
+        L.orf_set_sponge.argtypes = [vp, i, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
         L.orf_array.restype = dp
         L.orf_array.argtypes = [vp, i, i, C.POINTER(C.c_longlong)]
         L.orf_initialize.argtypes = [vp]
@@ -79,6 +81,12 @@ def lib():
         L.orf_residual.argtypes = [vp, dp]
         _lib = L
     return _lib
+
+
+class Sponge(C.Structure):
+    """orf_sponge (oracle/ostc_oracle_fr.h)."""
+    _fields_ = [("enabled", C.c_int * 6), ("start", C.c_double * 6), ("end", C.c_double * 6),
+                ("sigma", C.c_double * 6), ("function", C.c_int), ("jacobian_scaled", C.c_int)]
 
 
 def _dp(a):
@@ -226,6 +234,21 @@ class FrOracle:
                 r = L.orf_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"])
             if r != 0:
                 raise ValueError(f"bad boundary face {f}")
+        if "sponge" in case:
+            sp = case["sponge"]
+            raw = Sponge((C.c_int * 6)(*[int(v) for v in sp["enabled"]]), (C.c_double * 6)(*sp["start"]),
+                         (C.c_double * 6)(*sp["end"]), (C.c_double * 6)(*sp["sigma"]), int(sp.get("function", 0)),
+                         int(sp.get("jacobian_scaled", 1)))
+            g = L.orf_ngg(self.ctx)
+            for b, blk in enumerate(case["blocks"]):
+                if "axes" in blk:
+                    raise ValueError("sponge layers on blocks with their own frame are not set up here")
+                kk, jj, ii = np.meshgrid(np.arange(-g, blk["mz"] + g), np.arange(-g, blk["my"] + g),
+                                         np.arange(-g, blk["mx"] + g), indexing="ij")
+                xyz = [np.ascontiguousarray(blk[o] + idx * case[d]) for o, idx, d in
+                       (("x0", ii, "dx"), ("y0", jj, "dy"), ("z0", kk, "dz"))]
+                if L.orf_set_sponge(self.ctx, b, C.cast(C.byref(raw), C.c_void_p), *(_dp(a) for a in xyz)) != 0:
+                    raise ValueError("bad sponge layer")
         for f in case["inner_faces"]:
             if "range" in f:
                 r = L.orf_add_inner_face_range(self.ctx, f["block"], OA._ints(f["range"]),

@@ -578,6 +578,10 @@ struct Block {
   std::vector<Face> faces;
   std::vector<InnerFace> inner;
   unsigned long long bc_passes = 0;  // boundary-condition passes so far (keys the inflow noise)
+  bool sponge = false;                      // sponge layer on this block
+  orf_sponge sp{};
+  std::vector<double> xs, ys, zs;           // interior node coordinates (sponge layer)
+  std::vector<double> sponge_mean_cv;       // [nv][n0]
   long long id(int i, int j, int k) const { return (k + g) * sx * sy + (j + g) * sx + (i + g); }
   long long id0(int i, int j, int k) const { return (long long)k * mx * my + (long long)j * mx + i; }
   long long n0() const { return (long long)mx * my * mz; }
@@ -590,6 +594,7 @@ struct Ctx {
   double weno_eps_scale = 1;
   double dt_cur = 0;
   std::vector<Block> blocks;
+  unsigned long long sponge_iter = 0;  // steps averaged into the sponge means
 };
 
 // compute_total_energy<FR> + compute_cv_from_bv_1_point, src/FieldOperation.cuh:17-105
@@ -654,7 +659,60 @@ void properties(const Ctx& c, Block& b) {
 }
 
 // compute_source<FR> -> finite_rate_chemistry, src/SourceTerm.cuh:11-35
+// sponge_function 0 (Nektar++), src/SpongeLayer.cu:162-170
+double sponge_function(double xi) {
+  const double xi3 = xi * xi * xi;
+  return 6 * xi3 * xi * xi - 15 * xi3 * xi + 10 * xi3;
+}
+
+// sponge_layer_source, src/SpongeLayer.cu:68-160: sigma from the sides that
+// are on (x-, x+, y-, y+, z-, z+ in this order), then dq -= sigma (cv - mean).
+void sponge_source(const Ctx& c, Block& b) {
+  const orf_sponge& sp = b.sp;
+  const long long s = b.sz, n0 = b.n0();
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = b.id(i, j, k), p0 = b.id0(i, j, k);
+        const double pos[3] = {b.xs[p0], b.ys[p0], b.zs[p0]};
+        double sigma = 0;
+        for (int side = 0; side < 6; ++side) {
+          if (!sp.enabled[side]) continue;
+          const double x = pos[side / 2], x1 = sp.start[side];
+          const bool inside = side % 2 == 0 ? x < x1 : x > x1;
+          if (!inside) continue;
+          const double x2 = sp.end[side];
+          double xi = (x - x1) / (x2 - x1);
+          if (xi > 1) xi = 1;
+          sigma += sp.sigma[side] * sponge_function(xi);
+        }
+        // the reference's term carries no Jacobian (its rate is sigma / J);
+        // jacobian_scaled multiplies it in, as the chemistry source does
+        if (sp.jacobian_scaled) sigma = sigma * b.jac[p];
+        for (int l = 0; l < c.nv; ++l)
+          b.dq[p0 + l * n0] -= sigma * (b.cv[p + l * s] - b.sponge_mean_cv[p0 + l * n0]);
+      }
+}
+
+// update_sponge_layer_value, src/SpongeLayer.cu:28-49 (the scalar counters
+// equal the step counter in a new run)
+void sponge_mean(const Ctx& c, Block& b) {
+  const long long s = b.sz, n0 = b.n0();
+  const double n = static_cast<double>(c.sponge_iter);
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = b.id(i, j, k), p0 = b.id0(i, j, k);
+        for (int l = 0; l < c.nv; ++l) {
+          double& m = b.sponge_mean_cv[p0 + l * n0];
+          m = (m * n + b.cv[p + l * s]) / (n + 1);
+        }
+      }
+}
+
 void source(const Ctx& c, Block& b) {
+  if (b.sponge && (!c.s.reaction || c.T.n_reac == 0)) sponge_source(c, b);
   if (!c.s.reaction || c.T.n_reac == 0) return;
   const long long s = b.sz, n0 = b.n0();
 #pragma omp parallel for collapse(2) schedule(static)
@@ -667,6 +725,7 @@ void source(const Ctx& c, Block& b) {
         chemistry(c.T, b.bv[p], b.bv[p + 5 * s], Y, om, nullptr, nullptr);
         for (int l = 0; l < c.ns; ++l) b.dq[p0 + (l + 5) * n0] += b.jac[p] * om[l];
       }
+  if (b.sponge) sponge_source(c, b);
 }
 
 // local_time_step<FR, Laminar>, src/TimeAdvanceFunc.cuh:28-83
@@ -1555,6 +1614,13 @@ double global_dt(Ctx& c) {
 void step(Ctx& c, double* dt_io) {
   double dt = c.s.fixed_time_step ? c.s.dt : c.dt_cur;
   for (auto& b : c.blocks) b.qn = b.cv;
+  bool any_sponge = false;
+  for (auto& b : c.blocks)
+    if (b.sponge) {
+      sponge_mean(c, b);
+      any_sponge = true;
+    }
+  if (any_sponge) ++c.sponge_iter;
   for (int rk = 0; rk < 3; ++rk) {
     for (auto& b : c.blocks) {
       std::fill(b.dq.begin(), b.dq.end(), 0.0);
@@ -1683,6 +1749,28 @@ int orf_add_inflow(void* ctx, int block, const int range[6], const orf_inflow* i
     for (int j = f.lo[1]; j <= f.hi[1]; ++j)
       for (int i = f.lo[0]; i <= f.hi[0]; ++i)
         f.y.push_back(y[((long long)(k + b.g) * sy + (j + b.g)) * sx + (i + b.g)]);
+  return 0;
+}
+
+int orf_set_sponge(void* ctx, int block, const orf_sponge* sp, const double* x, const double* y, const double* z) {
+  Ctx* c = C(ctx);
+  if (block < 0 || block >= (int)c->blocks.size() || !sp || !x || !y || !z) return -1;
+  Block& b = c->blocks[block];
+  b.sponge = true;
+  b.sp = *sp;
+  const long long n0 = b.n0(), sx = b.mx + 2 * b.g, sy = b.my + 2 * b.g;
+  b.xs.assign(n0, 0);
+  b.ys.assign(n0, 0);
+  b.zs.assign(n0, 0);
+  b.sponge_mean_cv.assign(n0 * c->nv, 0.0);
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = ((long long)(k + b.g) * sy + (j + b.g)) * sx + (i + b.g);
+        b.xs[b.id0(i, j, k)] = x[p];
+        b.ys[b.id0(i, j, k)] = y[p];
+        b.zs[b.id0(i, j, k)] = z[p];
+      }
   return 0;
 }
 

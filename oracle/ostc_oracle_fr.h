@@ -80,6 +80,15 @@ typedef struct {
   double sv_upper[16], sv_lower[16];
 } orf_inflow;
 int     orf_add_inflow(void* ctx, int block, const int range[6], const orf_inflow* in, const double* y);
+/* Sponge layer of one block (src/SpongeLayer.cu:28-172): x, y, z on the padded domain. */
+typedef struct {
+  int enabled[6];
+  double start[6], end[6], sigma[6];
+  int function;
+  int jacobian_scaled;  /* 1: the term times the Jacobian; 0: the reference's term as written */
+} orf_sponge;
+int     orf_set_sponge(void* ctx, int block, const orf_sponge* sp, const double* x, const double* y,
+                       const double* z);
 double* orf_array(void* ctx, int block, int which, long long* n_out);
 
 int     orf_initialize(void* ctx);            /* cv from bv, properties, BCs, exchange, properties */
