@@ -202,6 +202,8 @@ SIGNATURES = {
 mixture = importlib.import_module(__name__ + ".mixture")
 # readers and writers of the reference's connectivity files
 mesh_io = importlib.import_module(__name__ + ".mesh_io")
+# checkpoints in the reference's Tecplot (#!TDV112) format
+checkpoint = importlib.import_module(__name__ + ".checkpoint")
 # the reverse mode as a torch.autograd.Function (imports torch on first use)
 autodiff = importlib.import_module(__name__ + ".autodiff")
 SIGNATURES.update(mixture.SIGNATURES)
