@@ -24,9 +24,10 @@ data, per zone
 (src/Residual.cuh:140-150), read back on restart (src/Driver.cu:33-39).
 
 A restart loads density, u, v, w, pressure, temperature (and the mass
-fractions) of every block's nodes and calls initialize(): boundary
-conditions, exchange, properties and cv from the primitives, as the
-reference's restart does (src/Initialize.cuh:103-352).
+fractions) of every block's nodes -- and the ghost layers the file holds --
+and calls initialize(): boundary conditions, exchange, properties and cv
+from the primitives, as the reference's restart does
+(src/Initialize.cuh:103-352).
 """
 from __future__ import annotations
 
@@ -156,14 +157,19 @@ def read_residual_scale(path: str):
         return [float(v) for v in f.read().split()[:4]]
 
 
-def _interior(a, g):
-    return a[..., g:a.shape[-3] - g, g:a.shape[-2] - g, g:a.shape[-1] - g]
+def _inner(a, g, keep=0):
+    """The nodes of a padded array with `keep` of its g ghost layers."""
+    c = g - keep
+    return a[..., c:a.shape[-3] - c, c:a.shape[-2] - c, c:a.shape[-1] - c]
 
 
-def save(solver, path: str, solution_time: float = 0.0):
+def save(solver, path: str, solution_time: float = 0.0, ghosts: int = 0):
     """flowfield.plt of a Solver or mixture FrSolver: node coordinates of the
     case's grid, the primitives, Mach and (for a mixture) the mass fractions
-    under the species names of its mechanism, on every block's nodes."""
+    under the species names of its mechanism, on every block's nodes and
+    `ghosts` ghost layers (the reference's ngg_output, 0 by default).  Edge
+    and corner ghost nodes take no part in the exchange, so only a file with
+    all ngg ghost layers restarts a run where it stopped, to round-off."""
     from . import cases
     mx = getattr(solver, "T", None) is not None
     species = solver.T.species() if mx else []
@@ -176,13 +182,15 @@ def save(solver, path: str, solution_time: float = 0.0):
         f_sv = None
     names = BASE_NAMES + list(species)
     zones = []
+    if not 0 <= ghosts <= g:
+        raise ValueError(f"ghosts must be in 0..{g}")
     for b in range(len(solver.case["blocks"])):
-        xyz = cases.node_coordinates(solver.case, b, 0)
-        bv = _interior(solver.get(b, f_bv), g)
-        mach = _interior(solver.get(b, f_mach), g)[0]
+        xyz = cases.node_coordinates(solver.case, b, ghosts)
+        bv = _inner(solver.get(b, f_bv), g, ghosts)
+        mach = _inner(solver.get(b, f_mach), g, ghosts)[0]
         arrays = list(xyz) + [bv[l] for l in range(6)] + [mach]
         if species:
-            sv = _interior(solver.get(b, f_sv), g)
+            sv = _inner(solver.get(b, f_sv), g, ghosts)
             arrays += [sv[l] for l in range(len(species))]
         zones.append(arrays)
     write_plt(path, names, zones, solution_time)
@@ -205,16 +213,18 @@ def load(solver, path: str):
     if len(data["zones"]) != len(solver.case["blocks"]):
         raise ValueError(f"{path}: {len(data['zones'])} zones for {len(solver.case['blocks'])} blocks")
     for b, zone in enumerate(data["zones"]):
+        blk = solver.case["blocks"][b]
+        keep = (zone["shape"][2] - blk["mx"]) // 2   # ghost layers in the file
         bv = solver.get(b, f_bv)
-        inner = _interior(bv, g)
-        if inner.shape[1:] != zone["shape"]:
-            raise ValueError(f"{path}: zone {b} has shape {zone['shape']}, block {inner.shape[1:]}")
+        inner = _inner(bv, g, keep) if 0 <= keep <= g else None
+        if inner is None or inner.shape[1:] != zone["shape"]:
+            raise ValueError(f"{path}: zone {b} has shape {zone['shape']}, block {bv.shape[1:]} with {g} ghosts")
         for l, nm in enumerate(BASE_NAMES[3:9]):
             inner[l] = zone[nm]
         solver.set(b, f_bv, bv)
         if species:
             sv = solver.get(b, f_sv)
-            si = _interior(sv, g)
+            si = _inner(sv, g, keep)
             for l, nm in enumerate(species):
                 si[l] = zone[nm]
             solver.set(b, f_sv, sv)
