@@ -428,6 +428,11 @@ typedef struct {
                                  relaxation rate is sigma; 0: the reference's term as written, without J
                                  (src/SpongeLayer.cu:155), whose rate is sigma / J */
 } ostc_sponge;
+/* Air contexts: the same sponge layer on the five conservative variables,
+ * with OSTC_MODE_PLAIN and the stage operators (the fused batched kernel
+ * refuses it); before ostc_finalize, per block, as ostc_fr_set_sponge. */
+int ostc_set_sponge(ostc_ctx ctx, int block, const ostc_sponge* sp, const double* x, const double* y,
+                    const double* z);
 /* Before ostc_fr_finalize, per block: x, y, z are the block's node coordinates
  * on the padded domain (host, the bv plane layout); the library keeps sigma of
  * every interior node. */

@@ -159,6 +159,7 @@ SIGNATURES = {
     "ostc_add_boundary_range": (_I, [_P, _I, C.POINTER(_I), _I, C.POINTER(Farfield)]),
     "ostc_add_inner_face_range": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), _I]),
     "ostc_add_inflow": (_I, [_P, _I, C.POINTER(_I), C.POINTER(Inflow), _DP]),
+    "ostc_set_sponge": (_I, [_P, _I, _P, _DP, _DP, _DP]),
     "ostc_finalize": (_I, [_P]),
     "ostc_ngg": (_I, [_P, C.POINTER(_I)]),
     "ostc_zone_view": (_I, [_P, _I, C.POINTER(Zone)]),
@@ -291,6 +292,11 @@ class Solver:
                                                       ints(f["target_range"]), f["target"]))
             else:
                 self._chk(L.ostc_add_inner_face(self.ctx, f["block"], f["axis"], f["side"], f["target"]))
+        if "sponge" in case:  # the sponge layer of every block, from its node coordinates
+            sp = mixture.sponge_struct(case["sponge"])
+            for b in range(len(case["blocks"])):
+                xyz = [np.ascontiguousarray(a) for a in cases.node_coordinates(case, b, self.ngg)]
+                self._chk(L.ostc_set_sponge(self.ctx, b, C.cast(C.byref(sp), C.c_void_p), *(_dp(a) for a in xyz)))
         self._chk(L.ostc_finalize(self.ctx))
 
     def _chk(self, r):

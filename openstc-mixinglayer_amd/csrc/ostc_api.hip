@@ -3,6 +3,7 @@
 // per-block (reference-shaped) step and the batched, graph-captured step.
 #include "openstc/abi.h"
 #include "ostc_faces.hpp"
+#include "ostc_sponge.hpp"
 #include "ostc_kernels.hpp"
 
 #include <algorithm>
@@ -55,6 +56,8 @@ struct HBlock {
   int m[3];
   std::vector<FaceDesc> faces;
   std::vector<FaceMap> inner;
+  std::vector<double> sp_sigma;  // sponge sigma of every interior node (empty: none)
+  int sp_jac = 1;
   DevBlock d{};
 };
 
@@ -88,6 +91,8 @@ struct ostc_ctx_s {
   bool inflow_noise = false;             // some inflow face has white noise
   double* d_inflow = nullptr;            // inflow states, then the y of every inflow face point
   unsigned long long* d_pass = nullptr;  // boundary-condition passes per block
+  bool sponge = false;                   // a sponge layer is set on some block
+  unsigned long long* d_sp_iter = nullptr;  // steps averaged into the sponge mean
   bool finalized = false;
   DevBlock* d_blocks = nullptr;
   std::vector<DevOp> ops;
@@ -452,6 +457,7 @@ int upload_blocks(ostc_ctx c) {
 bool fused_scheme(const ostc_ctx c) { return c->P.scheme == 51 || c->P.scheme == 50 || c->P.scheme == 71; }
 
 int need_fused(ostc_ctx c) {
+  if (c->sponge) return fail(c, OSTC_ERR_UNSUPPORTED, "the sponge layer runs with OSTC_MODE_PLAIN");
   if (fused_scheme(c)) return OSTC_OK;
   return fail(c, OSTC_ERR_UNSUPPORTED,
               "inviscid_scheme 52/72/1/3/4 run with OSTC_MODE_PLAIN and the stage operators only");
@@ -544,6 +550,11 @@ int plain_step(ostc_ctx c, double* dt_io, long long* n) {
   for (auto& B : c->blocks) {
     HIPCHK(c, hipMemcpyAsync(B.d.cv[1], B.d.cv[0], 5 * B.d.sz * sizeof(double), hipMemcpyDeviceToDevice, c->st));
     ++*n;
+  }
+  if (c->sponge) {  // update_sponge_layer_value, then update_sponge_iter (src/RK.cuh:104-109)
+    for (auto& B : c->blocks)
+      if (B.d.sp_sigma) sponge_mean(B.d, c->d_sp_iter, c->st, n);
+    count_pass(c->d_sp_iter, 0, 1, c->st, n);
   }
   double dt = *dt_io;
   for (int rk = 0; rk < 3; ++rk) {
@@ -780,6 +791,31 @@ int ostc_add_inflow(ostc_ctx c, int block, const int range[6], const ostc_inflow
   return OSTC_OK;
 }
 
+int ostc_set_sponge(ostc_ctx c, int block, const ostc_sponge* sp, const double* x, const double* y,
+                    const double* z) {
+  if (!c) return OSTC_ERR_ARG;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "the sponge layer is fixed after ostc_finalize");
+  if (block < 0 || block >= (int)c->blocks.size() || !sp || !x || !y || !z)
+    return fail(c, OSTC_ERR_ARG, "bad sponge layer");
+  if (sp->function != 0) return fail(c, OSTC_ERR_UNSUPPORTED, "sponge_function 0 is implemented");
+  for (int d = 0; d < 6; ++d)
+    if (sp->enabled[d] && !(std::fabs(sp->start[d] - sp->end[d]) >= 1e-10))
+      return fail(c, OSTC_ERR_ARG, "sponge layer: start and end of an enabled side coincide");
+  HBlock& B = c->blocks[block];
+  const int G = c->ngg;
+  const long long sx = B.m[0] + 2 * G, sy = B.m[1] + 2 * G;
+  B.sp_sigma.assign((size_t)B.m[0] * B.m[1] * B.m[2], 0.0);
+  B.sp_jac = sp->jacobian_scaled != 0;
+  for (int k = 0; k < B.m[2]; ++k)
+    for (int j = 0; j < B.m[1]; ++j)
+      for (int i = 0; i < B.m[0]; ++i) {
+        const long long p = ((long long)(k + G) * sy + (j + G)) * sx + (i + G);
+        B.sp_sigma[((size_t)k * B.m[1] + j) * B.m[0] + i] = sponge_sigma_at(*sp, x[p], y[p], z[p]);
+      }
+  c->sponge = true;
+  return OSTC_OK;
+}
+
 int ostc_add_boundary_range(ostc_ctx c, int block, const int range[6], int type, const ostc_farfield* ff) {
   if (!c) return OSTC_ERR_ARG;
   if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_finalize");
@@ -920,9 +956,22 @@ int ostc_finalize(ostc_ctx c) {
     HIPCHK(c, hipStreamSynchronize(c->st));
   }
   {
-    double* p = nullptr;  // one 8-byte counter per block
-    if (int r = dalloc(c, &p, c->blocks.size())) return r;
+    double* p = nullptr;  // one 8-byte counter per block, then the sponge step counter
+    if (int r = dalloc(c, &p, c->blocks.size() + 1)) return r;
     c->d_pass = reinterpret_cast<unsigned long long*>(p);
+    c->d_sp_iter = c->d_pass + c->blocks.size();
+  }
+  for (auto& B : c->blocks) {
+    if (B.sp_sigma.empty()) continue;
+    double *sig = nullptr, *mean = nullptr;
+    if (int r = dalloc(c, &sig, B.sp_sigma.size())) return r;
+    if (int r = dalloc(c, &mean, 5 * B.sp_sigma.size())) return r;
+    HIPCHK(c, hipMemcpyAsync(sig, B.sp_sigma.data(), B.sp_sigma.size() * sizeof(double), hipMemcpyHostToDevice,
+                             c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    B.d.sp_sigma = sig;
+    B.d.sp_mean = mean;
+    B.d.sp_jac = B.sp_jac;
   }
   int r = dalloc(c, &c->d_scratch, 1 + 4 * (size_t)c->n_first);
   if (!r) r = dalloc(c, &c->d_dt, 1);
@@ -1471,6 +1520,7 @@ int ostc_adjoint_forward(ostc_ctx c, int n_steps, int every, const ostc_theta* t
   for (const FaceDesc& f : c->blocks[0].faces)
     if (f.type != OSTC_BC_OUTFLOW && f.type != OSTC_BC_PERIODIC)
       return fail(c, OSTC_ERR_UNSUPPORTED, "the RK3 reverse sweep covers outflow and periodic faces");
+  if (c->sponge) return fail(c, OSTC_ERR_UNSUPPORTED, "the RK3 reverse sweep does not cover the sponge layer");
   if (!c->s.fixed_time_step) return fail(c, OSTC_ERR_UNSUPPORTED, "the RK3 reverse sweep needs fixed_time_step = 1");
   if (c->P.scheme != 51 && c->P.scheme != 50)
     return fail(c, OSTC_ERR_UNSUPPORTED, "reverse mode covers inviscid_scheme 51 and 50");

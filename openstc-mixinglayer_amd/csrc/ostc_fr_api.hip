@@ -15,6 +15,7 @@
 #include "ostc_faces.hpp"
 #include "ostc_fr.hpp"
 #include "ostc_inflow.hpp"
+#include "ostc_sponge.hpp"
 
 using namespace ostc::fr;
 using ostc::FaceMap;
@@ -395,24 +396,6 @@ int ostc_fr_add_inflow(ostc_fr_ctx c, int b, const int range[6], const ostc_infl
   return OSTC_OK;
 }
 
-// sponge_layer_source's sigma at one node (src/SpongeLayer.cu:68-160): the
-// sides in the order x-, x+, y-, y+, z-, z+, S(xi) of sponge_function 0.
-static double sponge_sigma_at(const ostc_sponge& sp, double x, double y, double z) {
-  double sigma = 0;
-  const double s3[3] = {x, y, z};
-  for (int d = 0; d < 6; ++d) {
-    if (!sp.enabled[d]) continue;
-    const double v = s3[d / 2], x1 = sp.start[d];
-    if ((d % 2 == 0) ? !(v < x1) : !(v > x1)) continue;
-    double xi = (v - x1) / (sp.end[d] - x1);
-    if (xi > 1) xi = 1;
-    const double xi3 = xi * xi * xi;
-    const double S = 6 * xi3 * xi * xi - 15 * xi3 * xi + 10 * xi3;
-    sigma += sp.sigma[d] * S;
-  }
-  return sigma;
-}
-
 int ostc_fr_set_sponge(ostc_fr_ctx c, int b, const ostc_sponge* sp, const double* x, const double* y,
                        const double* z) {
   if (!c) return OSTC_ERR_ARG;
@@ -432,7 +415,7 @@ int ostc_fr_set_sponge(ostc_fr_ctx c, int b, const ostc_sponge* sp, const double
     for (int j = 0; j < h.my; ++j)
       for (int i = 0; i < h.mx; ++i) {
         const long long p = ((long long)(k + G) * sy + (j + G)) * sx + (i + G);
-        h.sp_sigma[((size_t)k * h.my + j) * h.mx + i] = sponge_sigma_at(*sp, x[p], y[p], z[p]);
+        h.sp_sigma[((size_t)k * h.my + j) * h.mx + i] = ostc::sponge_sigma_at(*sp, x[p], y[p], z[p]);
       }
   c->sponge = true;
   return OSTC_OK;

@@ -12,6 +12,7 @@ void plain_local_dt(const DevBlock& B, const Params& P, hipStream_t st, long lon
 void min_reduce(const double* in, long long len, double* scratch, int n_first, hipStream_t st, long long* n);
 void padded_pass(const DevBlock& B, int slot, int what, hipStream_t st, long long* n);
 void count_pass(unsigned long long* pass, int lo, int hi, hipStream_t st, long long* n);
+void sponge_mean(const DevBlock& B, const unsigned long long* iter, hipStream_t st, long long* n);
 void ops_launch(const DevBlock* d_blocks, int n_blocks, const DevOp* d_ops, int op_lo, int op_hi, long long total, int slot,
                 int src, hipStream_t st, long long* n);
 void store_last(const DevBlock& B, hipStream_t st);

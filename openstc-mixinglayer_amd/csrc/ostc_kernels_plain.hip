@@ -421,7 +421,41 @@ static void launch_weno(const DevBlock& B, const Params& P, hipStream_t st, long
   *n += 3;
 }
 
+// sponge_layer_source (src/SpongeLayer.cu:68-160), first term of the stage's
+// dq as in compute_source: dq -= sigma (cv - mean), sigma times the Jacobian
+// when sp_jac is set.
+__global__ __launch_bounds__(256) void k_sponge_source(DevBlock B) {
+  const long long t = blockIdx.x * 256ll + threadIdx.x;
+  if (t >= B.n0) return;
+  const int i = (int)(t % B.mx), j = (int)((t / B.mx) % B.my), k = (int)(t / ((long long)B.mx * B.my));
+  const long long p = pidx(B, i, j, k);
+  const double sigma = B.sp_jac ? B.sp_sigma[t] * B.jac[p] : B.sp_sigma[t];
+  for (int l = 0; l < 5; ++l) B.dq[t + l * B.n0] -= sigma * (B.cv[0][p + l * B.sz] - B.sp_mean[t + l * B.n0]);
+}
+
+// update_sponge_layer_value (src/SpongeLayer.cu:28-49): mean = (mean n + cv) / (n + 1).
+__global__ __launch_bounds__(256) void k_sponge_mean(DevBlock B, const unsigned long long* iter) {
+  const long long t = blockIdx.x * 256ll + threadIdx.x;
+  if (t >= B.n0) return;
+  const int i = (int)(t % B.mx), j = (int)((t / B.mx) % B.my), k = (int)(t / ((long long)B.mx * B.my));
+  const long long p = pidx(B, i, j, k);
+  const double n = (double)*iter;
+  for (int l = 0; l < 5; ++l) {
+    double& m = B.sp_mean[t + l * B.n0];
+    m = (m * n + B.cv[0][p + l * B.sz]) / (n + 1);
+  }
+}
+
+void sponge_mean(const DevBlock& B, const unsigned long long* iter, hipStream_t st, long long* n) {
+  hipLaunchKernelGGL(k_sponge_mean, dim3((unsigned)((B.n0 + 255) / 256)), dim3(256), 0, st, B, iter);
+  *n += 1;
+}
+
 void plain_rhs(const DevBlock& B, const Params& P, hipStream_t st, long long* n) {
+  if (B.sp_sigma) {
+    hipLaunchKernelGGL(k_sponge_source, dim3((unsigned)((B.n0 + 255) / 256)), dim3(256), 0, st, B);
+    *n += 1;
+  }
   if (P.scheme == 71) launch_weno<71>(B, P, st, n);
   else if (P.scheme == 50) launch_weno<50>(B, P, st, n);
   else if (P.scheme == 52) launch_weno<52>(B, P, st, n);

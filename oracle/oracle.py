@@ -96,6 +96,8 @@ def lib(tangent: bool = False):
         L.orc_add_boundary_range.argtypes = [vp, i, C.POINTER(C.c_int), i, C.c_void_p]
         L.orc_add_inner_face_range.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), i]
         L.orc_add_inflow.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(Inflow)]
+        L.orc_set_sponge.argtypes = [vp, i, C.c_void_p]
+        L.orc_source.argtypes = [vp, i]
         L.orc_add_farfield.argtypes = [vp, i, i, i, C.POINTER(Farfield)]
         L.orc_farfield_state_air.argtypes = [C.POINTER(FarfieldSpec), C.POINTER(Farfield)]
         L.orc_array.restype = dp
@@ -129,6 +131,12 @@ def lib(tangent: bool = False):
         L.orc_scalar_width.restype = i
         _libs[tangent] = L
     return _libs[tangent]
+
+
+class Sponge(C.Structure):
+    """orc_sponge (oracle/ostc_oracle.h)."""
+    _fields_ = [("enabled", C.c_int * 6), ("start", C.c_double * 6), ("end", C.c_double * 6),
+                ("sigma", C.c_double * 6), ("function", C.c_int), ("jacobian_scaled", C.c_int)]
 
 
 class Inflow(C.Structure):
@@ -202,6 +210,13 @@ class Oracle:
                 r = L.orc_add_boundary(self.ctx, f["block"], f["axis"], f["side"], f["type"])
             if r != 0:
                 raise ValueError(f"bad boundary face {f}")
+        if "sponge" in case:
+            sp = case["sponge"]
+            st = Sponge((C.c_int * 6)(*[int(v) for v in sp["enabled"]]), (C.c_double * 6)(*sp["start"]),
+                        (C.c_double * 6)(*sp["end"]), (C.c_double * 6)(*sp["sigma"]), int(sp.get("function", 0)),
+                        int(sp.get("jacobian_scaled", 1)))
+            for b in range(len(case["blocks"])):
+                L.orc_set_sponge(self.ctx, b, C.cast(C.byref(st), C.c_void_p))
         for f in case["inner_faces"]:
             if "range" in f:
                 r = L.orc_add_inner_face_range(self.ctx, f["block"], _ints(f["range"]), _ints(f["target_range"]),
@@ -286,9 +301,10 @@ class Oracle:
 
     # -- stage-level operators ----------------------------------------
     def rhs(self, b: int):
-        """dq = 0; inviscid; viscous (src/RK.cuh:113-139)."""
+        """dq = 0; source (the sponge layer); inviscid; viscous (src/RK.cuh:113-139)."""
         L = self.L
         L.orc_zero_dq(self.ctx, b)
+        L.orc_source(self.ctx, b)
         L.orc_inviscid(self.ctx, b)
         L.orc_viscous(self.ctx, b)
 

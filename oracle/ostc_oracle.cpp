@@ -79,6 +79,9 @@ struct Block {
   std::vector<Face> faces;
   std::vector<InnerFace> inner;
   unsigned long long bc_passes = 0;  // boundary-condition passes so far (keys the inflow noise)
+  bool sponge = false;               // sponge layer on this block
+  orc_sponge sp{};
+  std::vector<real> sponge_mean_cv;  // [5][n0]
 
   long long id(int i, int j, int k) const { return (k + g) * sx * sy + (j + g) * sx + (i + g); }
   long long id0(int i, int j, int k) const { return (long long)k * mx * my + (long long)j * mx + i; }
@@ -96,6 +99,7 @@ struct Ctx {
   std::vector<Block> blocks;
   double weno_eps_scale;
   double dt_cur = 1e6;
+  unsigned long long sponge_iter = 0;  // steps averaged into the sponge means
 };
 
 inline double sqr(double a) { return a * a; }
@@ -1070,6 +1074,49 @@ void apply_inflow(Block& b, const Face& f) {
       }
 }
 
+// sponge_layer_source<Air>, src/SpongeLayer.cu:68-160, called from
+// compute_source before the inviscid terms: sigma from the enabled sides
+// (x-, x+, y-, y+, z-, z+ in this order; sponge_function 0), then
+// dq -= sigma (cv - mean), sigma times J when jacobian_scaled.
+void sponge_source(Block& b) {
+  const orc_sponge& sp = b.sp;
+  const long long s = b.sz, n0 = (long long)b.mx * b.my * b.mz;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = b.id(i, j, k), t = ((long long)k * b.my + j) * b.mx + i, q = b.idg(i, j, k);
+        const double pos[3] = {b.x[q], b.y[q], b.z[q]};
+        double sigma = 0;
+        for (int side = 0; side < 6; ++side) {
+          if (!sp.enabled[side]) continue;
+          const double x = pos[side / 2], x1 = sp.start[side];
+          if (!(side % 2 == 0 ? x < x1 : x > x1)) continue;
+          double xi = (x - x1) / (sp.end[side] - x1);
+          if (xi > 1) xi = 1;
+          const double xi3 = xi * xi * xi;
+          sigma += sp.sigma[side] * (6 * xi3 * xi * xi - 15 * xi3 * xi + 10 * xi3);
+        }
+        if (sp.jacobian_scaled) sigma = sigma * b.jac[p];
+        for (int l = 0; l < 5; ++l) b.dq[t + l * n0] -= sigma * (b.cv[p + l * s] - b.sponge_mean_cv[t + l * n0]);
+      }
+}
+
+// update_sponge_layer_value, src/SpongeLayer.cu:28-49
+void sponge_mean(Block& b, unsigned long long iter) {
+  const long long s = b.sz, n0 = (long long)b.mx * b.my * b.mz;
+  const double n = static_cast<double>(iter);
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = b.id(i, j, k), t = ((long long)k * b.my + j) * b.mx + i;
+        for (int l = 0; l < 5; ++l) {
+          real& m = b.sponge_mean_cv[t + l * n0];
+          m = (m * n + b.cv[p + l * s]) / (n + 1);
+        }
+      }
+}
+
 void apply_bc(Block& b) {
   // priority: farfield(4), outflow(6), inflow(5), ..., periodic(8) (src/BoundCond.cuh:1488-1678)
   const int order[] = {ORC_BC_FARFIELD, ORC_BC_OUTFLOW, ORC_BC_INFLOW, ORC_BC_PERIODIC};
@@ -1225,6 +1272,22 @@ int orc_add_farfield(void* ctx, int block, int axis, int side, const orc_farfiel
   int r[6];
   whole_face_record(c->blocks[block], axis, side, r);
   return add_face(c, block, r, ORC_BC_FARFIELD, ff);
+}
+
+int orc_set_sponge(void* ctx, int block, const orc_sponge* sp) {
+  auto* c = static_cast<Ctx*>(ctx);
+  if (block < 0 || block >= (int)c->blocks.size() || !sp) return -1;
+  Block& b = c->blocks[block];
+  b.sponge = true;
+  b.sp = *sp;
+  b.sponge_mean_cv.assign((size_t)5 * b.mx * b.my * b.mz, real{0.0});
+  return 0;
+}
+
+int orc_source(void* ctx, int block) {
+  Block& b = static_cast<Ctx*>(ctx)->blocks[block];
+  if (b.sponge) sponge_source(b);
+  return 0;
 }
 
 int orc_add_inflow(void* ctx, int block, const int range[6], const orc_inflow* in) {
@@ -1581,6 +1644,13 @@ int orc_step(void* ctx, int n_steps, double* last_dt) {
   double dt = c->s.fixed_time_step ? c->s.dt : 1e6;
   for (int st = 0; st < n_steps; ++st) {
     for (auto& b : c->blocks) b.qn = b.cv;
+    bool any_sponge = false;
+    for (auto& b : c->blocks)
+      if (b.sponge) {
+        sponge_mean(b, c->sponge_iter);
+        any_sponge = true;
+      }
+    if (any_sponge) ++c->sponge_iter;
     for (int rk = 0; rk < 3; ++rk) {
       for (auto& b : c->blocks) std::fill(b.dq.begin(), b.dq.end(), 0.0);
       if (!c->s.fixed_time_step && rk == 0) {
@@ -1588,6 +1658,7 @@ int orc_step(void* ctx, int n_steps, double* last_dt) {
         dt = global_dt(*c);
       }
       for (auto& b : c->blocks) {
+        if (b.sponge) sponge_source(b);
         inviscid(*c, b);
         viscous(*c, b);
         update(*c, b, dt, rk);

@@ -100,6 +100,17 @@ int     orc_add_inner_face(void* ctx, int block, int axis, int side, int target_
 int     orc_add_boundary_range(void* ctx, int block, const int range[6], int type, const orc_farfield* ff);
 /* Inflow face (label 5) over a boundary N.txt range; y from the block's own coordinates. */
 int     orc_add_inflow(void* ctx, int block, const int range[6], const orc_inflow* in);
+/* Sponge layer of one block (src/SpongeLayer.cu:28-172), sigma from the block's own coordinates;
+ * the layout of orf_sponge (oracle/ostc_oracle_fr.h). */
+typedef struct {
+  int enabled[6];
+  double start[6], end[6], sigma[6];
+  int function;
+  int jacobian_scaled;
+} orc_sponge;
+int     orc_set_sponge(void* ctx, int block, const orc_sponge* sp);
+/* compute_source of air: the sponge term of the stage (nothing without a sponge layer) */
+int     orc_source(void* ctx, int block);
 int     orc_add_inner_face_range(void* ctx, int block, const int range[6], const int target_range[6],
                                  int target_block);
 /* farfield face (label 4) with its state, apply_farfield<Air> (src/BoundCond.cuh:755-1022) */

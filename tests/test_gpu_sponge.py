@@ -53,3 +53,23 @@ def test_unscaled_sponge_rhs():
     print(f"  dq rel err {e}", flush=True)
     assert np.all(e <= 1e-10), e   # the chemistry rows: <= 1e-10 (the source gate)
     s.close()
+
+
+def test_air_sponge_plain_steps_and_batched_refusal():
+    from _util import O, oracle_for
+    print("[gpu] air spatial layer with sponge, plain, 6 steps", flush=True)
+    case = cases.mixing_layer_case([19], [15], [11], spatial=True, inflow_noise=1e-2)
+    case["sponge"] = cases.sponge_layer(case, sides=(1, 2, 3), depth=0.3, coefficient=2.0)
+    o = oracle_for(case)
+    s = pkg.Solver(case)
+    s.setup_mixing_layer()
+    s.initialize()
+    dt_o = o.step(6)
+    dt_g = s.rk3(6, pkg.MODE_PLAIN)
+    assert abs(dt_g - dt_o) <= 1e-12 * dt_o, (dt_g, dt_o)
+    e = rel_err(s.get(0, pkg.FIELD_BV), o.array(0, O.BV))
+    print(f"  bv rel err {e}", flush=True)
+    assert np.all(e <= 1e-9), e
+    with pytest.raises(pkg.OstcError, match="sponge"):
+        s.rk3(1, pkg.MODE_BATCHED)
+    s.close()

@@ -138,3 +138,24 @@ def test_running_mean_of_the_step_starts(T):
     scale = np.max(np.abs(want), axis=(1, 2, 3), keepdims=True)
     assert np.all(np.max(np.abs(dq - want), axis=(1, 2, 3), keepdims=True) <= 1e-12 * scale)
     assert np.any(want != 0)
+
+
+def test_air_sponge_source_reads_back_sigma_j():
+    from _util import O
+    case = cases.mixing_layer_case([15], [19], [7], spatial=True, inflow_noise=0.0)
+    case["sponge"] = cases.sponge_layer(case, sides=(1, 2, 3), depth=0.3, coefficient=2.0, jacobian_scaled=1)
+    o = O.Oracle(case)
+    o.setup_mixing_layer()
+    o.initialize()
+    g = o.ngg
+    L = O.lib()
+    L.orc_zero_dq(o.ctx, 0)
+    L.orc_source(o.ctx, 0)
+    dq = o.array(0, O.DQ)
+    cv = o.array(0, O.CV)[:, g:-g, g:-g, g:-g]
+    jac = o.array(0, O.JAC)[0, g:-g, g:-g, g:-g]
+    want, _ = _expected_sigma(case)
+    for l in range(5):
+        nz = np.abs(cv[l]) > 0
+        np.testing.assert_allclose(-dq[l][nz], (want * jac)[nz] * cv[l][nz], rtol=1e-13, atol=0)
+    assert np.any(want > 0) and np.any(want == 0)
