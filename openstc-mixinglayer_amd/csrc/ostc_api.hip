@@ -460,7 +460,7 @@ int need_fused(ostc_ctx c) {
   if (c->sponge) return fail(c, OSTC_ERR_UNSUPPORTED, "the sponge layer runs with OSTC_MODE_PLAIN");
   if (fused_scheme(c)) return OSTC_OK;
   return fail(c, OSTC_ERR_UNSUPPORTED,
-              "inviscid_scheme 52/72/1/3/4 run with OSTC_MODE_PLAIN and the stage operators only");
+              "inviscid_scheme 52/72/1/2/3/4 run with OSTC_MODE_PLAIN and the stage operators only");
 }
 
 int set_tiles(ostc_ctx c, int index, int n_wg) {
@@ -662,9 +662,10 @@ int ostc_create(const ostc_setup* s, void* hip_stream, ostc_ctx* out) {
   *out = nullptr;
   if (s->n_var != 5) return OSTC_ERR_UNSUPPORTED;
   const int sc = s->inviscid_scheme;
-  if (sc != 51 && sc != 50 && sc != 71 && sc != 52 && sc != 72 && sc != 1 && sc != 3 && sc != 4)
+  if (sc != 51 && sc != 50 && sc != 71 && sc != 52 && sc != 72 && sc != 1 && sc != 2 && sc != 3 && sc != 4)
     return OSTC_ERR_UNSUPPORTED;
-  if ((sc == 1 || sc == 3 || sc == 4) && (s->reconstruction < 1 || s->reconstruction > 3)) return OSTC_ERR_UNSUPPORTED;
+  if ((sc == 1 || sc == 2 || sc == 3 || sc == 4) && (s->reconstruction < 1 || s->reconstruction > 3))
+    return OSTC_ERR_UNSUPPORTED;
   if (s->viscous_order != 0 && s->viscous_order != 2) return OSTC_ERR_UNSUPPORTED;
   if (!(s->cfl > 0) || !(s->prandtl > 0)) return OSTC_ERR_ARG;
   auto* c = new ostc_ctx_s;

@@ -155,7 +155,22 @@ __global__ __launch_bounds__(64) void k_pv_pencil(DevBlock B, Params P, int slot
     const double ml[3] = {sK[0][tid + 1], sK[1][tid + 1], sK[2][tid + 1]};
     const double mr[3] = {sK[0][tid + 2], sK[1][tid + 2], sK[2][tid + 2]};
     double f[5];
-    riemann_air(P.scheme, pl, pr, ml, mr, sJ[tid + 1], sJ[tid + 2], f);
+    if (P.scheme == 2) {
+      // entropy-fix deltas of the face's two points from their full metric
+      double dlr[2];
+      for (int q = 0; q < 2; ++q) {
+        int ijk[3] = {base[0], base[1], base[2]};
+        ijk[DIR] = L + q;
+        const long long p = pidx(B, ijk[0], ijk[1], ijk[2]);
+        double mm[9], qq[5];
+        for (int cc = 0; cc < 9; ++cc) mm[cc] = B.met[p + cc * s];
+        for (int l = 0; l < 5; ++l) qq[l] = sPv[l][tid + 1 + q];
+        dlr[q] = entropy_delta_air(qq, mm);
+      }
+      roe_air(pl, pr, ml, mr, sJ[tid + 1], sJ[tid + 2], dlr[0], dlr[1], f);
+    } else {
+      riemann_air(P.scheme, pl, pr, ml, mr, sJ[tid + 1], sJ[tid + 2], f);
+    }
     for (int l = 0; l < 5; ++l) sFc[l][tid] = f[l];
   }
   __syncthreads();
@@ -460,7 +475,7 @@ void plain_rhs(const DevBlock& B, const Params& P, hipStream_t st, long long* n)
   else if (P.scheme == 50) launch_weno<50>(B, P, st, n);
   else if (P.scheme == 52) launch_weno<52>(B, P, st, n);
   else if (P.scheme == 72) launch_weno<72>(B, P, st, n);
-  else if (P.scheme == 1 || P.scheme == 3 || P.scheme == 4) {
+  else if (P.scheme == 1 || P.scheme == 2 || P.scheme == 3 || P.scheme == 4) {
     hipLaunchKernelGGL(k_pv_pencil<0>, pencil_grid(B, 0), dim3(64), 0, st, B, P, 0);
     hipLaunchKernelGGL(k_pv_pencil<1>, pencil_grid(B, 1), dim3(64), 0, st, B, P, 0);
     hipLaunchKernelGGL(k_pv_pencil<2>, pencil_grid(B, 2), dim3(64), 0, st, B, P, 0);

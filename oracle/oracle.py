@@ -124,6 +124,7 @@ def lib(tangent: bool = False):
             getattr(L, name).argtypes = [dp, dp, d]
         L.orc_lf_split.argtypes = [dp, i, dp, d, dp, dp]
         L.orc_riemann.argtypes = [i, dp, dp, dp, dp, d, d, dp]
+        L.orc_roe.argtypes = [dp, dp, dp, dp, d, d, d, d, dp]
         L.orc_char_matrices.argtypes = [dp, dp, dp, dp, d, d, dp, dp]
         L.orc_fourier_perturbation.argtypes = [vp, i, C.POINTER(_Fourier)]
         L.orc_momentum_thickness.restype = d
@@ -346,6 +347,19 @@ def riemann(scheme, ql, qr, kl, kr, jl, jr):
     a = [np.ascontiguousarray(x, dtype=np.float64) for x in (ql, qr, kl, kr)]
     fc = np.zeros(5)
     lib().orc_riemann(scheme, *[_dp(x) for x in a], float(jl), float(jr), _dp(fc))
+    return fc
+
+
+def roe(ql, qr, kl, kr, jl, jr, dl=0.0, dr=0.0):
+    """Roe flux of air between primitive states [rho,u,v,w,p] with the two
+    points' entropy-fix deltas."""
+    def full(q):
+        q = np.asarray(q, dtype=np.float64)
+        e = 0.5 * q[0] * (q[1] ** 2 + q[2] ** 2 + q[3] ** 2) + q[4] / 0.4
+        return np.ascontiguousarray(np.concatenate([q, [e]]))
+    a = [full(ql), full(qr)] + [np.ascontiguousarray(x, dtype=np.float64) for x in (kl, kr)]
+    fc = np.zeros(5)
+    lib().orc_roe(*[_dp(x) for x in a], float(jl), float(jr), float(dl), float(dr), _dp(fc))
     return fc
 
 

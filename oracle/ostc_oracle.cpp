@@ -585,6 +585,98 @@ void lf_air(const real* pl, const real* pr, const double* ml, const double* mr, 
   fci[4] = ((pl[5] + pl[4]) * Ukl + (pr[5] + pr[4]) * Ukr - spectral_radius * (pr[5] - pl[5])) * half_jac_ave;
 }
 
+// compute_entropy_fix_delta<Air>, src/InviscidScheme.cu:421-453 (3D),
+// entropy_fix_factor 0.125 (the default, src/Parameter.cpp:471-507).
+real entropy_delta(const real* q, const double* m) {
+  const real U = abs(q[1] * m[0] + q[2] * m[1] + q[3] * m[2]);
+  const real V = abs(q[1] * m[3] + q[2] * m[4] + q[3] * m[5]);
+  const real W = abs(q[1] * m[6] + q[2] * m[7] + q[3] * m[8]);
+  const double kx = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  const double ky = std::sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
+  const double kz = std::sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
+  const real c = sqrt(kGammaAir * q[4] / q[0]);
+  return 0.125 * (U + V + W + c * (kx + ky + kz) / 3.0);
+}
+
+// riemannSolver_Roe<Air>, src/RiemannSolver.cuh:279-417, with
+// compute_half_sum_left_right_flux :234-277.
+void roe_air(const real* pl, const real* pr, const double* ml, const double* mr, double jl, double jr, real del_l,
+             real del_r, real* fci) {
+  double JacKx = jl * ml[0], JacKy = jl * ml[1], JacKz = jl * ml[2];
+  real Uk = pl[1] * JacKx + pl[2] * JacKy + pl[3] * JacKz;
+  real coeff = Uk * pl[0];
+  fci[0] = 0.5 * coeff;
+  fci[1] = 0.5 * (coeff * pl[1] + pl[4] * JacKx);
+  fci[2] = 0.5 * (coeff * pl[2] + pl[4] * JacKy);
+  fci[3] = 0.5 * (coeff * pl[3] + pl[4] * JacKz);
+  fci[4] = 0.5 * Uk * (pl[4] + pl[5]);
+  JacKx = jr * mr[0];
+  JacKy = jr * mr[1];
+  JacKz = jr * mr[2];
+  Uk = pr[1] * JacKx + pr[2] * JacKy + pr[3] * JacKz;
+  coeff = Uk * pr[0];
+  fci[0] += 0.5 * coeff;
+  fci[1] += 0.5 * (coeff * pr[1] + pr[4] * JacKx);
+  fci[2] += 0.5 * (coeff * pr[2] + pr[4] * JacKy);
+  fci[3] += 0.5 * (coeff * pr[3] + pr[4] * JacKz);
+  fci[4] += 0.5 * Uk * (pr[4] + pr[5]);
+
+  const real dl = sqrt(pl[0]), dr = sqrt(pr[0]);
+  const real inv_denominator = 1.0 / (dl + dr);
+  const real u = (dl * pl[1] + dr * pr[1]) * inv_denominator;
+  const real v = (dl * pl[2] + dr * pr[2]) * inv_denominator;
+  const real w = (dl * pl[3] + dr * pr[3]) * inv_denominator;
+  const real ek = 0.5 * (u * u + v * v + w * w);
+  const real hl = (pl[5] + pl[4]) / pl[0];
+  const real hr = (pr[5] + pr[4]) / pr[0];
+  const real h = (dl * hl + dr * hr) * inv_denominator;
+  const double gamma = kGammaAir;
+  const real c = sqrt((gamma - 1) * (h - ek));
+  double kx = 0.5 * (ml[0] + mr[0]);
+  double ky = 0.5 * (ml[1] + mr[1]);
+  double kz = 0.5 * (ml[2] + mr[2]);
+  const double gradK = std::sqrt(kx * kx + ky * ky + kz * kz);
+  real Ukk = kx * u + ky * v + kz * w;
+  real characteristic[3] = {Ukk - gradK * c, Ukk, Ukk + gradK * c};
+  const real delta_ave = 0.5 * (del_l + del_r);
+  for (auto& cc : characteristic) {
+    cc = abs(cc);
+    if (cc < delta_ave) cc = 0.5 * (cc * cc / delta_ave + delta_ave);
+  }
+  kx /= gradK;
+  ky /= gradK;
+  kz /= gradK;
+  Ukk /= gradK;
+  const double jac_ave = 0.5 * (jl + jr);
+  real dq[5];
+  dq[0] = jac_ave * (pr[0] - pl[0]);
+  for (int l = 1; l < 5; ++l) dq[l] = jac_ave * (pr[0] * pr[l] - pl[0] * pl[l]);
+  dq[4] = jac_ave * (pr[5] - pl[5]);
+  real c1 = (gamma - 1) * (ek * dq[0] - u * dq[1] - v * dq[2] - w * dq[3] + dq[4]) / (c * c);
+  real c2 = (kx * dq[1] + ky * dq[2] + kz * dq[3] - Ukk * dq[0]) / c;
+  real c3 = dq[0] - c1;
+  real LDq[5];
+  LDq[0] = 0.5 * (c1 - c2);
+  LDq[1] = kx * c3 - ((kz * v - ky * w) * dq[0] - kz * dq[2] + ky * dq[3]) / c;
+  LDq[2] = ky * c3 - ((kx * w - kz * u) * dq[0] - kx * dq[3] + kz * dq[1]) / c;
+  LDq[3] = kz * c3 - ((ky * u - kx * v) * dq[0] - ky * dq[1] + kx * dq[2]) / c;
+  LDq[4] = 0.5 * (c1 + c2);
+  real bb[5];
+  bb[0] = -characteristic[0] * LDq[0];
+  for (int l = 1; l < 5; ++l) bb[l] = -characteristic[1] * LDq[l];
+  bb[4] = -characteristic[2] * LDq[4];
+  const real c0 = kx * bb[1] + ky * bb[2] + kz * bb[3];
+  c1 = c0 + bb[0] + bb[4];
+  c2 = c * (bb[4] - bb[0]);
+  c3 = 0;
+  fci[0] += 0.5 * c1;
+  fci[1] += 0.5 * (u * c1 + kx * c2 - c * (kz * bb[2] - ky * bb[3]));
+  fci[2] += 0.5 * (v * c1 + ky * c2 - c * (kx * bb[3] - kz * bb[1]));
+  fci[3] += 0.5 * (w * c1 + kz * c2 - c * (ky * bb[1] - kx * bb[2]));
+  fci[4] += 0.5 * (h * c1 + Ukk * c2 - c * c * c0 / (gamma - 1) +
+                   c * ((kz * v - ky * w) * bb[1] + (kx * w - kz * u) * bb[2] + (ky * u - kx * v) * bb[3]) + c3);
+}
+
 void riemann_air(int scheme, const real* pl, const real* pr, const double* ml, const double* mr, double jl, double jr,
                  real* fci) {
   if (scheme == 1) lf_air(pl, pr, ml, mr, jl, jr, fci);
@@ -604,7 +696,7 @@ void inviscid_pv(const Ctx& c, Block& b) {
 #pragma omp parallel
     {
       const int np = m + 2 * g;
-      std::vector<real> PV(np * 5), fc((m + 1) * 5);
+      std::vector<real> PV(np * 5), fc((m + 1) * 5), D(np);
       std::vector<double> K(np * 3), J(np);
 #pragma omp for collapse(2) schedule(static)
       for (int bb = 0; bb < nb; ++bb)
@@ -619,13 +711,21 @@ void inviscid_pv(const Ctx& c, Block& b) {
             for (int l = 0; l < 5; ++l) PV[ps * 5 + l] = b.bv[p + l * s];
             for (int cc = 0; cc < 3; ++cc) K[ps * 3 + cc] = b.metric[p + (3 * dir + cc) * s];
             J[ps] = b.jac[p];
+            if (c.s.inviscid_scheme == 2) {
+              double mm[9];
+              for (int cc = 0; cc < 9; ++cc) mm[cc] = b.metric[p + cc * s];
+              D[ps] = entropy_delta(&PV[ps * 5], mm);
+            }
           }
           for (int t = -1; t < m; ++t) {
             const int is = t + g;
             real pl[6], pr[6];
             reconstruct_air(PV.data(), is, c.s.reconstruction, pl, pr);
-            riemann_air(c.s.inviscid_scheme, pl, pr, &K[is * 3], &K[(is + 1) * 3], J[is], J[is + 1],
-                        &fc[(t + 1) * 5]);
+            if (c.s.inviscid_scheme == 2)
+              roe_air(pl, pr, &K[is * 3], &K[(is + 1) * 3], J[is], J[is + 1], D[is], D[is + 1], &fc[(t + 1) * 5]);
+            else
+              riemann_air(c.s.inviscid_scheme, pl, pr, &K[is * 3], &K[(is + 1) * 3], J[is], J[is + 1],
+                          &fc[(t + 1) * 5]);
           }
           for (int t = 0; t < m; ++t) {
             idx[dir] = t;
@@ -641,7 +741,8 @@ void inviscid_pv(const Ctx& c, Block& b) {
 // src/WENO.cu:8-461 (pencil kernels), :1455-1536 (cp flux), :463-491 (launcher).
 // dq(i) -= fc(i+1/2) - fc(i-1/2), directions x, y, z in that order.
 void inviscid(const Ctx& c, Block& b) {
-  if (c.s.inviscid_scheme == 1 || c.s.inviscid_scheme == 3 || c.s.inviscid_scheme == 4) {
+  if (c.s.inviscid_scheme == 1 || c.s.inviscid_scheme == 2 || c.s.inviscid_scheme == 3 ||
+      c.s.inviscid_scheme == 4) {
     inviscid_pv(c, b);
     return;
   }
@@ -1765,6 +1866,18 @@ int orc_char_matrices(const double* ql, const double* qr, const double* kl, cons
       L[r * 5 + q] = value_of(M.L[r][q]);
       R[r * 5 + q] = value_of(M.R[r][q]);
     }
+  return 0;
+}
+
+int orc_roe(const double* ql, const double* qr, const double* kl, const double* kr, double jl, double jr, double dl,
+            double dr, double* f) {
+  real a[6], b[6], out[5];
+  for (int l = 0; l < 6; ++l) {
+    a[l] = ql[l];
+    b[l] = qr[l];
+  }
+  roe_air(a, b, kl, kr, jl, jr, dl, dr, out);
+  for (int l = 0; l < 5; ++l) f[l] = value_of(out[l]);
   return 0;
 }
 

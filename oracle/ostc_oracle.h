@@ -181,6 +181,9 @@ double  orc_weno5js(const double* vp, const double* vm, double eps);
 /* Riemann flux between primitive states [rho,u,v,w,p] (scheme 1 LF, 3 AUSM+,
  * 4 HLLC), metric rows kl/kr, Jacobians jl/jr; and the characteristic WENO-Z5
  * left/right matrices of a face between conservative states q = [cv(5), p]. */
+/* riemannSolver_Roe<Air> on reconstructed states [rho,u,v,w,p,E] with the two points' entropy-fix deltas */
+int     orc_roe(const double* ql, const double* qr, const double* kl, const double* kr, double jl, double jr,
+                double dl, double dr, double* f);
 int     orc_riemann(int scheme, const double* ql, const double* qr, const double* kl, const double* kr, double jl,
                     double jr, double* fc);
 int     orc_char_matrices(const double* ql, const double* qr, const double* kl, const double* kr, double jl,

@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 DQ_TOL = 1e-12
 BV_TOL = 1e-9
 
-SCHEMES = [(52, 2), (72, 2), (1, 1), (1, 2), (3, 2), (3, 3), (4, 1), (4, 2), (4, 3)]
+SCHEMES = [(52, 2), (72, 2), (1, 1), (1, 2), (2, 1), (2, 2), (3, 2), (3, 3), (4, 1), (4, 2), (4, 3)]
 IDS = [f"s{s}-r{r}" for s, r in SCHEMES]
 
 
