@@ -69,7 +69,8 @@ typedef struct {
   int n_var;            /* 5 */
   int inviscid_scheme;  /* 51 WENO-Z5 cp, 71 WENO-Z7 cp, 50 WENO5-JS cp (no reference counterpart);
                            OSTC_MODE_PLAIN only: 52 / 72 WENO-Z5 / WENO-Z7 characteristic (src/WENO.cu:1147-1453),
-                           1 LF, 3 AUSM+, 4 HLLC on reconstructed primitives (src/InviscidScheme.cu:12-215) */
+                           1 LF, 2 Roe (entropy fix 0.125), 3 AUSM+, 4 HLLC on reconstructed primitives
+                           (src/InviscidScheme.cu:12-215, src/RiemannSolver.cuh:279-417) */
   int viscous_order;    /* 0 inviscid, 2 second-order central */
   int fixed_time_step;  /* 1: use dt below, 0: CFL-based global dt */
   double dt;
