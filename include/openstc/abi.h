@@ -347,7 +347,8 @@ typedef struct ostc_fr_ctx_s* ostc_fr_ctx;
 
 typedef struct {
   int inviscid_scheme;       /* 51 / 71 WENO-Z5 / WENO-Z7 component-wise, 52 / 72 characteristic,
-                                3 AUSM+, 4 HLLC on reconstructed primitives (inviscid_type 0) */
+                                2 Roe (entropy fix 0.125), 3 AUSM+, 4 HLLC on reconstructed primitives
+                                (inviscid_type 0) */
   int viscous_order;         /* 0 or 2 */
   int reaction;              /* 1: finite-rate chemistry source */
   int positive_preserving;   /* positive_preserving: limiter on the species fluxes */

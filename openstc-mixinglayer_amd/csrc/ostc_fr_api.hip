@@ -272,8 +272,8 @@ int ostc_fr_create(const ostc_fr_setup* s, const ostc_mech_tables* mech, void* s
   if (!s || !mech || !out) return OSTC_ERR_ARG;
   *out = nullptr;
   const int sc = s->inviscid_scheme;
-  if (sc != 51 && sc != 52 && sc != 71 && sc != 72 && sc != 3 && sc != 4) return OSTC_ERR_UNSUPPORTED;
-  if ((s->inviscid_scheme == 3 || s->inviscid_scheme == 4) && (s->reconstruction < 1 || s->reconstruction > 3))
+  if (sc != 51 && sc != 52 && sc != 71 && sc != 72 && sc != 2 && sc != 3 && sc != 4) return OSTC_ERR_UNSUPPORTED;
+  if ((sc == 2 || sc == 3 || sc == 4) && (s->reconstruction < 1 || s->reconstruction > 3))
     return OSTC_ERR_UNSUPPORTED;
   if (s->viscous_order != 0 && s->viscous_order != 2) return OSTC_ERR_UNSUPPORTED;
   if (!supported_ns(mech->n_spec)) return OSTC_ERR_UNSUPPORTED;

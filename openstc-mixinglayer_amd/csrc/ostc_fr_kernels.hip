@@ -992,7 +992,17 @@ __global__ __launch_bounds__(kT) void k_inviscid_pv(FrArgs A) {
       ml[c] = B.met[p0 + (3 * DIR + c) * s];
       mr[c] = B.met[p1 + (3 * DIR + c) * s];
     }
-    if (A.scheme == 3)
+    if (A.scheme == 2) {
+      double mm[9], delta[2];
+      const long long pts[2] = {p0, p1};
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) mm[c] = B.met[pts[q] + c * s];
+        delta[q] = entropy_delta_fr(B.bv[pts[q] + s], B.bv[pts[q] + 2 * s], B.bv[pts[q] + 3 * s], B.cc[pts[q]], mm);
+      }
+      roe_fr<NS>(A.T, pl, pr, ml, mr, B.jac[p0], B.jac[p1], delta[0], delta[1], fc);
+    } else if (A.scheme == 3)
       ausm_plus<NS>(pl, pr, ml, mr, B.jac[p0], B.jac[p1], fc);
     else
       hllc<NS>(A.T, pl, pr, ml, mr, B.jac[p0], B.jac[p1], fc);

@@ -74,6 +74,7 @@ def lib():
         L.orf_temperature.argtypes = [tp, d, dp, d, C.POINTER(C.c_int)]
         L.orf_char_matrices.argtypes = [tp, dp, dp, dp, dp, d, d, dp, dp]
         L.orf_riemann.argtypes = [tp, i, dp, dp, dp, dp, d, d, dp]
+        L.orf_roe.argtypes = [tp, dp, dp, dp, dp, d, d, d, d, dp]
         L.orf_reconstruct.argtypes = [dp, i, i, dp, dp]
         L.orf_farfield_state.argtypes = [tp, C.POINTER(OA.FarfieldSpec), dp, C.POINTER(OA.Farfield)]
         L.orf_add_farfield.argtypes = [vp, i, i, i, C.POINTER(OA.Farfield)]
@@ -154,6 +155,14 @@ def riemann(T, scheme, ql, qr, kl, kr, jl, jr):
     args = [np.ascontiguousarray(a, dtype=np.float64) for a in (ql, qr, kl, kr)]
     fc = np.zeros(5 + T.n_spec)
     lib().orf_riemann(C.byref(T), scheme, *[_dp(a) for a in args], jl, jr, _dp(fc))
+    return fc
+
+
+def roe(T, ql, qr, kl, kr, jl, jr, delta_l=0.0, delta_r=0.0):
+    """Roe flux with the entropy fix (scheme 2) between primitive states."""
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (ql, qr, kl, kr)]
+    fc = np.zeros(5 + T.n_spec)
+    lib().orf_roe(C.byref(T), *[_dp(a) for a in args], jl, jr, delta_l, delta_r, _dp(fc))
     return fc
 
 

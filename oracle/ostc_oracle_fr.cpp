@@ -976,6 +976,112 @@ void hllc(const Tab& T, int nv, const double* pl, const double* pr, const double
   }
 }
 
+// compute_entropy_fix_delta (mixture, 3D), src/InviscidScheme.cu:420-453:
+// the factor 0.125 times (|U| + |V| + |W| + c (|grad xi| + |grad eta| +
+// |grad zeta|) / 3) at one point, m the 9 metric entries.
+double entropy_delta(double u, double v, double w, double c, const double* m) {
+  const double U = fabs(u * m[0] + v * m[1] + w * m[2]);
+  const double V = fabs(u * m[3] + v * m[4] + w * m[5]);
+  const double W = fabs(u * m[6] + v * m[7] + w * m[8]);
+  const double kx = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  const double ky = sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
+  const double kz = sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
+  return 0.125 * (U + V + W + c * (kx + ky + kz) / 3.0);
+}
+
+// riemannSolver_Roe (mixture), src/RiemannSolver.cuh:234-417:
+// compute_half_sum_left_right_flux (:234-276) plus -1/2 R |Lambda| L dQ.
+// The Roe state is the sqrt(rho)-weighted mean of u, v, w, H, Y_k and p/rho;
+// T = mean(p/rho) / (R_u sum Y_k / W_k); gamma, c, h_k from NASA-7 at T.
+// |Lambda| of U - c|k|, U, U + c|k| with Harten's fix of width
+// (delta_l + delta_r) / 2.  pl, pr: rho, u, v, w, p, Y_k, E.
+void roe(const Tab& T, int nv, const double* pl, const double* pr, const double* ml, const double* mr, double jl,
+         double jr, double delta_l, double delta_r, double* fci) {
+  const int ns = T.n_spec;
+  const double* ps[2] = {pl, pr};
+  const double* ms[2] = {ml, mr};
+  const double js[2] = {jl, jr};
+  for (int l = 0; l < nv; ++l) fci[l] = 0;
+  for (int side = 0; side < 2; ++side) {
+    const double* q = ps[side];
+    const double ax = js[side] * ms[side][0], ay = js[side] * ms[side][1], az = js[side] * ms[side][2];
+    const double Uk = q[1] * ax + q[2] * ay + q[3] * az;
+    const double mflux = Uk * q[0];
+    fci[0] += 0.5 * mflux;
+    fci[1] += 0.5 * (mflux * q[1] + q[4] * ax);
+    fci[2] += 0.5 * (mflux * q[2] + q[4] * ay);
+    fci[3] += 0.5 * (mflux * q[3] + q[4] * az);
+    fci[4] += 0.5 * Uk * (q[4] + q[nv]);
+    for (int l = 5; l < nv; ++l) fci[l] += 0.5 * mflux * q[l];
+  }
+  const double wl = sqrt(pl[0]), wr = sqrt(pr[0]);
+  const double wsum_inv = 1.0 / (wl + wr);
+  auto avg = [&](double a, double b) { return (wl * a + wr * b) * wsum_inv; };
+  const double u = avg(pl[1], pr[1]), v = avg(pl[2], pr[2]), w = avg(pl[3], pr[3]);
+  const double ek = 0.5 * (u * u + v * v + w * w);
+  const double H = avg((pl[nv] + pl[4]) / pl[0], (pr[nv] + pr[4]) / pr[0]);
+  double Y[MS], hk[MS], cpk[MS];
+  double r_mix{0};
+  for (int l = 0; l < ns; ++l) {
+    Y[l] = avg(pl[5 + l], pr[5 + l]);
+    r_mix += Y[l] / T.mw[l];
+  }
+  const double t = avg(pl[4] / pl[0], pr[4] / pr[0]) / (kRu * r_mix);
+  enthalpy_cp(T, t, hk, cpk);
+  double cp_m{0}, cv_m{0};
+  for (int l = 0; l < ns; ++l) {
+    cp_m += cpk[l] * Y[l];
+    cv_m += Y[l] * (cpk[l] - kRu / T.mw[l]);
+  }
+  const double gm = cp_m / cv_m, gm1 = gm - 1;
+  const double c = sqrt(gm * kRu * r_mix * t);
+  const double W_mix = 1.0 / r_mix;
+  double nx = 0.5 * (ml[0] + mr[0]), ny = 0.5 * (ml[1] + mr[1]), nz = 0.5 * (ml[2] + mr[2]);
+  const double kn = sqrt(nx * nx + ny * ny + nz * nz);
+  double Un = nx * u + ny * v + nz * w;
+  double lam[3] = {Un - kn * c, Un, Un + kn * c};
+  const double dlt = 0.5 * (delta_l + delta_r);
+  for (double& x : lam) {
+    x = fabs(x);
+    if (x < dlt) x = 0.5 * (x * x / dlt + dlt);
+  }
+  nx /= kn;
+  ny /= kn;
+  nz /= kn;
+  Un /= kn;
+  // jump of the conservative variables (times the mean Jacobian)
+  const double jm = 0.5 * (jl + jr);
+  double dQ[5 + MS];
+  dQ[0] = jm * (pr[0] - pl[0]);
+  for (int l = 1; l < nv; ++l) dQ[l] = jm * (pr[0] * pr[l] - pl[0] * pl[l]);
+  dQ[4] = jm * (pr[nv] - pl[nv]);
+  // wave strengths L dQ
+  double a1 = gm1 * (ek * dQ[0] - u * dQ[1] - v * dQ[2] - w * dQ[3] + dQ[4]) / (c * c);
+  const double a2 = (nx * dQ[1] + ny * dQ[2] + nz * dQ[3] - Un * dQ[0]) / c;
+  for (int l = 0; l < ns; ++l) a1 += (W_mix / T.mw[l] - hk[l] * gm1 / (c * c)) * dQ[5 + l];
+  const double a3 = dQ[0] - a1;
+  double bw[5 + MS];
+  bw[0] = -lam[0] * (0.5 * (a1 - a2));
+  bw[1] = -lam[1] * (nx * a3 - ((nz * v - ny * w) * dQ[0] - nz * dQ[2] + ny * dQ[3]) / c);
+  bw[2] = -lam[1] * (ny * a3 - ((nx * w - nz * u) * dQ[0] - nx * dQ[3] + nz * dQ[1]) / c);
+  bw[3] = -lam[1] * (nz * a3 - ((ny * u - nx * v) * dQ[0] - ny * dQ[1] + nx * dQ[2]) / c);
+  bw[4] = -lam[2] * (0.5 * (a1 + a2));
+  for (int l = 0; l < ns; ++l) bw[5 + l] = -lam[1] * (dQ[5 + l] - Y[l] * dQ[0]);
+  // R |Lambda| L dQ
+  const double e0 = nx * bw[1] + ny * bw[2] + nz * bw[3];
+  const double e1 = e0 + bw[0] + bw[4];
+  const double e2 = c * (bw[4] - bw[0]);
+  double e3 = 0;
+  for (int l = 0; l < ns; ++l) e3 += (hk[l] - W_mix / T.mw[l] * c * c / gm1) * bw[5 + l];
+  fci[0] += 0.5 * e1;
+  fci[1] += 0.5 * (u * e1 + nx * e2 - c * (nz * bw[2] - ny * bw[3]));
+  fci[2] += 0.5 * (v * e1 + ny * e2 - c * (nx * bw[3] - nz * bw[1]));
+  fci[3] += 0.5 * (w * e1 + nz * e2 - c * (ny * bw[1] - nx * bw[2]));
+  fci[4] += 0.5 * (H * e1 + Un * e2 - c * c * e0 / gm1 +
+                   c * ((nz * v - ny * w) * bw[1] + (nx * w - nz * u) * bw[2] + (ny * u - nx * v) * bw[3]) + e3);
+  for (int l = 0; l < ns; ++l) fci[5 + l] += 0.5 * (bw[5 + l] + Y[l] * e1);
+}
+
 // compute_convective_term_pv_1D<FR>, src/InviscidScheme.cu:50-165: per
 // direction x, y, z, reconstructed primitives -> Riemann flux,
 // dq(i) -= fc(i+1/2) - fc(i-1/2).
@@ -990,7 +1096,7 @@ void inviscid_pv(const Ctx& c, Block& b) {
 #pragma omp parallel
     {
       const int np = m + 2 * g;
-      std::vector<double> PV(np * nv), K(np * 3), J(np), fc((m + 1) * nv);
+      std::vector<double> PV(np * nv), K(np * 3), J(np), D(np), fc((m + 1) * nv);
 #pragma omp for collapse(2) schedule(static)
       for (int bb = 0; bb < nb; ++bb)
         for (int aa = 0; aa < na; ++aa) {
@@ -1005,6 +1111,11 @@ void inviscid_pv(const Ctx& c, Block& b) {
             for (int l = 0; l < ns; ++l) PV[ps * nv + 5 + l] = b.sv[p + l * s];
             for (int cc = 0; cc < 3; ++cc) K[ps * 3 + cc] = b.metric[p + (3 * dir + cc) * s];
             J[ps] = b.jac[p];
+            if (c.s.inviscid_scheme == 2) {
+              double mm[9];
+              for (int cc = 0; cc < 9; ++cc) mm[cc] = b.metric[p + cc * s];
+              D[ps] = entropy_delta(b.bv[p + s], b.bv[p + 2 * s], b.bv[p + 3 * s], b.cc[p], mm);
+            }
           }
           for (int t = -1; t < m; ++t) {
             const int is = t + g;
@@ -1013,7 +1124,9 @@ void inviscid_pv(const Ctx& c, Block& b) {
             pv_energy(c.T, nv, pl);
             pv_energy(c.T, nv, pr);
             double* fci = &fc[(t + 1) * nv];
-            if (c.s.inviscid_scheme == 3)
+            if (c.s.inviscid_scheme == 2)
+              roe(c.T, nv, pl, pr, &K[is * 3], &K[(is + 1) * 3], J[is], J[is + 1], D[is], D[is + 1], fci);
+            else if (c.s.inviscid_scheme == 3)
               ausm_plus(nv, pl, pr, &K[is * 3], &K[(is + 1) * 3], J[is], J[is + 1], fci);
             else
               hllc(c.T, nv, pl, pr, &K[is * 3], &K[(is + 1) * 3], J[is], J[is + 1], fci);
@@ -1032,7 +1145,7 @@ void inviscid_pv(const Ctx& c, Block& b) {
 // or compute_weno_flux_ch (52, "method 1"), src/WENO.cu:8-461, :722-1143,
 // :1455-1536.  dq(i) -= fc(i+1/2) - fc(i-1/2) for x, y, z in that order.
 void inviscid(const Ctx& c, Block& b, double dt) {
-  if (c.s.inviscid_scheme == 3 || c.s.inviscid_scheme == 4) {
+  if (c.s.inviscid_scheme == 2 || c.s.inviscid_scheme == 3 || c.s.inviscid_scheme == 4) {
     inviscid_pv(c, b);
     return;
   }
@@ -1677,7 +1790,7 @@ extern "C" {
 void* orf_create(const orf_setup* s, const ostc_mech_tables* t) {
   if (!s || !t) return nullptr;
   const int sc = s->inviscid_scheme;
-  if (sc != 51 && sc != 52 && sc != 71 && sc != 72 && sc != 3 && sc != 4) return nullptr;
+  if (sc != 51 && sc != 52 && sc != 71 && sc != 72 && sc != 2 && sc != 3 && sc != 4) return nullptr;
   Ctx* c = new Ctx();
   c->s = *s;
   c->T = *t;
@@ -2002,6 +2115,22 @@ int orf_riemann(const ostc_mech_tables* t, int scheme, const double* ql, const d
   pv_energy(*t, nv, pr);
   if (scheme == 3) ausm_plus(nv, pl, pr, kl, kr, jl, jr, fc);
   else hllc(*t, nv, pl, pr, kl, kr, jl, jr, fc);
+  return 0;
+}
+
+// one Roe face flux from the primitives rho, u, v, w, p, Y_k of both sides,
+// the face-normal metric rows, the Jacobians and the entropy-fix deltas
+int orf_roe(const ostc_mech_tables* t, const double* ql, const double* qr, const double* kl, const double* kr,
+            double jl, double jr, double delta_l, double delta_r, double* fc) {
+  const int nv = 5 + t->n_spec;
+  double pl[5 + MS + 2], pr[5 + MS + 2];
+  for (int l = 0; l < nv; ++l) {
+    pl[l] = ql[l];
+    pr[l] = qr[l];
+  }
+  pv_energy(*t, nv, pl);
+  pv_energy(*t, nv, pr);
+  roe(*t, nv, pl, pr, kl, kr, jl, jr, delta_l, delta_r, fc);
   return 0;
 }
 

@@ -124,6 +124,10 @@ double  orf_temperature(const ostc_mech_tables* t, double e, const double* y, do
  * v, w, p, Y_1..Y_ns], metric rows kl/kr and Jacobians jl/jr */
 int     orf_riemann(const ostc_mech_tables* t, int scheme, const double* ql, const double* qr, const double* kl,
                     const double* kr, double jl, double jr, double* fc);
+/* Roe flux with Harten's entropy fix (scheme 2) between primitive states q;
+ * delta_l/delta_r: the entropy-fix widths of the two points */
+int     orf_roe(const ostc_mech_tables* t, const double* ql, const double* qr, const double* kl, const double* kr,
+                double jl, double jr, double delta_l, double delta_r, double* fc);
 /* reconstruction of the face between points 1 and 2 of pv[4][nv] */
 void    orf_reconstruct(const double* pv, int nv, int recon, double* pl, double* pr);
 /* left (5+ns)x(5+ns) characteristic matrix rows L and the back projection R

@@ -217,7 +217,7 @@ def test_steps_reacting_mixture(tables):
     s.close()
 
 
-@pytest.mark.parametrize("scheme", [3, 4], ids=["ausm+", "hllc"])
+@pytest.mark.parametrize("scheme", [2, 3, 4], ids=["roe", "ausm+", "hllc"])
 @pytest.mark.parametrize("recon", [1, 2, 3], ids=["1st", "muscl", "nnd2"])
 def test_stage_residual_primitive_reconstruction(tables, scheme, recon):
     print(f"[gpu] fr dq, primitive reconstruction {recon}, Riemann solver {scheme}", flush=True)
@@ -234,7 +234,7 @@ def test_stage_residual_primitive_reconstruction(tables, scheme, recon):
         s.close()
 
 
-@pytest.mark.parametrize("scheme", [3, 4], ids=["ausm+", "hllc"])
+@pytest.mark.parametrize("scheme", [2, 3, 4], ids=["roe", "ausm+", "hllc"])
 def test_steps_primitive_reconstruction(tables, scheme):
     print(f"[gpu] fr 10 steps, MUSCL + Riemann solver {scheme}", flush=True)
     case = _case(tables, scheme=scheme)

@@ -422,6 +422,57 @@ def test_hllc_resolves_an_isolated_contact(T):
         assert np.max(np.abs(f - ref)) <= 1e-11 * np.max(np.abs(ref))
 
 
+def test_roe_flux_is_consistent(T):
+    """Roe (src/RiemannSolver.cuh:234-417): F(q, q) is the physical flux
+    whatever the entropy-fix width (no jump, no dissipation)."""
+    rng = np.random.default_rng(21)
+    for _ in range(10):
+        q = _prim(T, rng)
+        k = rng.normal(size=3)
+        f = OF.roe(T, q, q, k, k, 0.8, 0.8, rng.uniform(0, 300), rng.uniform(0, 300))
+        ref = _physical_flux(T, q, k, 0.8)
+        assert np.max(np.abs(f - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_roe_flux_upwinds_weak_supersonic_jumps_to_second_order(T, sign):
+    """With every characteristic of one sign, R |Lambda| L dQ = +-A dQ, and
+    A dQ = F(Q_r) - F(Q_l) + O(|dQ|^2): the Roe flux is the upwind physical
+    flux up to an error that, relative to the flux jump, vanishes with the
+    jump.  A sign error in the dissipation, a dropped species term or a
+    transposed wave strength leaves an error of the size of the jump."""
+    rng = np.random.default_rng(22 if sign > 0 else 23)
+    k = np.array([0.9, 0.3, -0.2])
+    ql = _prim(T, rng, u=sign * np.array([2600.0, 40.0, -30.0]), p=1e5, rho=0.6)
+    d = np.concatenate([[0.05, 90.0, -60.0, 45.0, 4e3], rng.normal(size=9)])
+    d[5:] -= d[5:].mean()   # mass fractions still sum to one
+    d[5:] *= 0.2
+    errs = []
+    for eps in (1e-3, 5e-4, 2.5e-4):
+        qr = ql + eps * d
+        f = OF.roe(T, ql, qr, k, k, 1.0, 1.0, 0.0, 0.0)
+        up = ql if sign > 0 else qr
+        ref = _physical_flux(T, up, k, 1.0)
+        jump = _physical_flux(T, qr, k, 1.0) - _physical_flux(T, ql, k, 1.0)
+        errs.append(np.max(np.abs(f - ref) / np.maximum(np.abs(jump), 1e-300 + np.max(np.abs(jump)) * 1e-6)))
+    assert all(e <= 10 * eps for e, eps in zip(errs, (1e-3, 5e-4, 2.5e-4))), errs
+
+
+def test_roe_entropy_fix_only_acts_below_the_width(T):
+    """Harten's fix replaces |lambda| by (lambda^2/delta + delta)/2 only
+    where |lambda| < delta: a width below every |lambda| changes nothing, a
+    wider one adds dissipation (a different flux)."""
+    rng = np.random.default_rng(24)
+    k = np.array([1.0, 0.0, 0.0])
+    ql = _prim(T, rng, u=[30.0, 0.0, 0.0], p=1e5)
+    qr = _prim(T, rng, u=[35.0, 5.0, 0.0], p=1.02e5)
+    f0 = OF.roe(T, ql, qr, k, k, 1.0, 1.0, 0.0, 0.0)
+    f1 = OF.roe(T, ql, qr, k, k, 1.0, 1.0, 1.0, 2.0)    # below |u| ~ 32
+    assert np.array_equal(f0, f1)
+    f2 = OF.roe(T, ql, qr, k, k, 1.0, 1.0, 80.0, 80.0)  # above |u|, below c
+    assert not np.allclose(f0, f2, rtol=1e-8, atol=0)
+
+
 @pytest.mark.parametrize("recon", [2, 3])
 def test_limited_reconstruction_is_exact_on_linear_data(recon):
     pv = np.outer(np.arange(4.0), np.linspace(-2, 3, 14)) + np.linspace(1, 5, 14)
