@@ -15,6 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libostc_oracle.so")
 TANGENT_LIB_PATH = os.path.join(HERE, "_build", "libostc_oracle_tangent.so")
+COUNT_LIB_PATH = os.path.join(HERE, "_build", "libostc_oracle_count.so")
 
 CV, BV, QN, DQ, MUL, MACH, JAC, METRIC, DT_LOCAL, X, Y, Z, BV_LAST = range(13)
 
@@ -23,7 +24,7 @@ class _Setup(C.Structure):
     _fields_ = [("n_var", C.c_int), ("inviscid_scheme", C.c_int), ("viscous_order", C.c_int),
                 ("fixed_time_step", C.c_int), ("dt", C.c_double), ("cfl", C.c_double),
                 ("prandtl", C.c_double), ("rho_ref", C.c_double), ("v_ref", C.c_double),
-                ("n_threads", C.c_int), ("reconstruction", C.c_int)]
+                ("n_threads", C.c_int), ("reconstruction", C.c_int), ("entropy_fix_factor", C.c_double)]
 
 
 class _Streams(C.Structure):
@@ -67,7 +68,7 @@ class _Fourier(C.Structure):
 
 
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(LIB_PATH) or not os.path.exists(TANGENT_LIB_PATH):
+    if force or not all(os.path.exists(p) for p in (LIB_PATH, TANGENT_LIB_PATH, COUNT_LIB_PATH)):
         subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB_PATH
 
@@ -75,13 +76,14 @@ def build(force: bool = False) -> str:
 _libs = {}
 
 
-def lib(tangent: bool = False):
+def lib(tangent=False):
     """The double oracle, or (tangent=True) the same source built with
-    {value, tangent} dual numbers as state values."""
+    {value, tangent} dual numbers as state values, or (tangent="count") the
+    operation-counting build (single-threaded; orc_op_counts)."""
     if tangent not in _libs:
-        path = TANGENT_LIB_PATH if tangent else LIB_PATH
+        path = COUNT_LIB_PATH if tangent == "count" else (TANGENT_LIB_PATH if tangent else LIB_PATH)
         if not os.path.exists(path):
-            build()
+            build(force=True)
         L = C.CDLL(path)
         vp, i, d = C.c_void_p, C.c_int, C.c_double
         dp = C.POINTER(C.c_double)
@@ -119,6 +121,8 @@ def lib(tangent: bool = False):
         L.orc_global_dt.argtypes = [vp]
         L.orc_update.argtypes = [vp, i, d, i]
         L.orc_residual.argtypes = [vp, dp]
+        if tangent == "count":
+            L.orc_op_counts.argtypes = [C.POINTER(C.c_ulonglong)]
         for name in ("orc_weno5z", "orc_weno7z", "orc_weno5js"):
             getattr(L, name).restype = d
             getattr(L, name).argtypes = [dp, dp, d]
@@ -184,7 +188,8 @@ class Oracle:
         self.L = L = lib(tangent)
         s = _Setup(case["n_var"], case["inviscid_scheme"], case["viscous_order"],
                    int(case.get("fixed_time_step", 0)), float(case.get("dt", 0.0)), case["cfl"],
-                   case["prandtl"], case["rho_ref"], case["v_ref"], n_threads, int(case.get("reconstruction", 2)))
+                   case["prandtl"], case["rho_ref"], case["v_ref"], n_threads, int(case.get("reconstruction", 2)),
+                   float(case.get("entropy_fix_factor", 0.125)))
         self.case = case
         self.ctx = L.orc_create(C.byref(s))
         self.ngg = L.orc_ngg(self.ctx)
@@ -250,7 +255,7 @@ class Oracle:
         p = self.L.orc_array(self.ctx, b, which, C.byref(n))
         shp = self.shape(b, which)
         per = shp[0] * shp[1] * shp[2]
-        w = 2 if self.tangent and which not in (JAC, METRIC, X, Y, Z) else 1
+        w = 2 if self.tangent is True and which not in (JAC, METRIC, X, Y, Z) else 1
         flat = np.ctypeslib.as_array(p, shape=(n.value,))
         if w == 1:
             return flat.reshape(n.value // per, *shp)
@@ -379,3 +384,17 @@ def lf_split(q, k3, jac):
     fp, fm = np.zeros(nv), np.zeros(nv)
     lib().orc_lf_split(_dp(q), nv, _dp(k3), float(jac), _dp(fp), _dp(fm))
     return fp, fm
+
+
+OP_NAMES = ("add", "mul", "div", "sqrt", "trans")
+
+
+def op_counts(o) -> dict:
+    """Operation counts of a count-build Oracle since the last reset."""
+    out = (C.c_ulonglong * 5)()
+    o.L.orc_op_counts(out)
+    return dict(zip(OP_NAMES, (int(v) for v in out)))
+
+
+def op_counts_reset(o):
+    o.L.orc_op_counts_reset()

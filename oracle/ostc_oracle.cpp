@@ -29,19 +29,32 @@
 #ifdef ORC_DUAL
 #include "dual.h"
 #endif
+#ifdef ORC_COUNT
+#include "count.h"
+#endif
 
 namespace {
 
 // State values are `real`: double, or a {value, tangent} dual number in the
-// tangent-linear build (-DORC_DUAL).  Geometry (grid, metric, Jacobian) and
-// the time step stay double in both.
+// tangent-linear build (-DORC_DUAL).  Geometry values (grid, metric,
+// Jacobian) are `geo`: double in both, and in the operation-counting build
+// (-DORC_COUNT) state and geometry are both counted scalars.  The time step
+// stays double.
 #ifdef ORC_DUAL
 using real = orc::Dual;
+using geo = double;
 using orc::tangent_of;
 using orc::value_of;
 constexpr int kWidth = 2;
+#elif defined(ORC_COUNT)
+using real = orc::Counted;
+using geo = orc::Counted;
+using orc::value_of;
+inline double tangent_of(const real&) { return 0.0; }
+constexpr int kWidth = 1;
 #else
 using real = double;
+using geo = double;
 inline double value_of(double a) { return a; }
 inline double tangent_of(double) { return 0.0; }
 constexpr int kWidth = 1;
@@ -75,7 +88,8 @@ struct Block {
   int mx, my, mz, g;
   long long sx, sy, sz;  // padded strides for g ghosts
   std::vector<real> cv, bv, qn, dq, mul, mach, dtl, bv_last;
-  std::vector<double> jac, metric, x, y, z;
+  std::vector<geo> jac, metric;
+  std::vector<double> x, y, z;
   std::vector<Face> faces;
   std::vector<InnerFace> inner;
   unsigned long long bc_passes = 0;  // boundary-condition passes so far (keys the inflow noise)
@@ -110,80 +124,90 @@ inline real sutherland(const real& t) {
 }
 
 // ---------------------------------------------------------------- WENO kernels
+// Smoothness indicators of the three 3-point candidates of a 5-point stencil
+// v[0..4] (src/WENO.cu:1587-1592; the same formulas serve the F- side).  The
+// reference writes each squared factor twice; it is computed once here, which
+// rounds identically.
+inline void beta5(const real* v, real b[3]) {
+  constexpr double c1312{13.0 / 12};
+  real s = v[2] + v[4] - 2 * v[3], t = 3 * v[2] - 4 * v[3] + v[4];
+  b[0] = c1312 * s * s + 0.25 * t * t;
+  s = v[1] + v[3] - 2 * v[2];
+  t = v[1] - v[3];
+  b[1] = c1312 * s * s + 0.25 * t * t;
+  s = v[0] + v[2] - 2 * v[1];
+  t = v[0] - 4 * v[1] + 3 * v[2];
+  b[2] = c1312 * s * s + 0.25 * t * t;
+}
+
+// WENO-Z5 weights alpha_k = d_k (1 + tau5 / (eps + beta_k)^2), tau5 = (beta_0 - beta_2)^2.
+inline void alpha5z(const real b[3], geo eps, double d0, double d1, double d2, real a[3]) {
+  const real db = b[0] - b[2];
+  const real tau = db * db;
+  const real e0 = eps + b[0], e1 = eps + b[1], e2 = eps + b[2];
+  a[0] = d0 + d0 * tau / (e0 * e0);
+  a[1] = d1 + d1 * tau / (e1 * e1);
+  a[2] = d2 + d2 * tau / (e2 * e2);
+}
+
 // WENO-Z5, src/WENO.cu:1582-1617
-real weno5z(const real* vp, const real* vm, double eps) {
+real weno5z(const real* vp, const real* vm, geo eps) {
   constexpr double one6th{1.0 / 6};
   real v0{one6th * (2 * vp[2] + 5 * vp[3] - vp[4])};
   real v1{one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3])};
   real v2{one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2])};
-  constexpr double c1312{13.0 / 12};
-  real b0 = c1312 * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
-              0.25 * (3 * vp[2] - 4 * vp[3] + vp[4]) * (3 * vp[2] - 4 * vp[3] + vp[4]);
-  real b1 = c1312 * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
-              0.25 * (vp[1] - vp[3]) * (vp[1] - vp[3]);
-  real b2 = c1312 * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
-              0.25 * (vp[0] - 4 * vp[1] + 3 * vp[2]) * (vp[0] - 4 * vp[1] + 3 * vp[2]);
-  real tau{(b0 - b2) * (b0 - b2)};
-  real a0{0.3 + 0.3 * tau / ((eps + b0) * (eps + b0))};
-  real a1{0.6 + 0.6 * tau / ((eps + b1) * (eps + b1))};
-  real a2{0.1 + 0.1 * tau / ((eps + b2) * (eps + b2))};
-  const real fp{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
+  real b[3], a[3];
+  beta5(vp, b);
+  alpha5z(b, eps, 0.3, 0.6, 0.1, a);
+  const real fp{(a[0] * v0 + a[1] * v1 + a[2] * v2) / (a[0] + a[1] + a[2])};
 
   v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
   v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
   v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
-  b0 = c1312 * (vm[2] + vm[4] - 2 * vm[3]) * (vm[2] + vm[4] - 2 * vm[3]) +
-       0.25 * (3 * vm[2] - 4 * vm[3] + vm[4]) * (3 * vm[2] - 4 * vm[3] + vm[4]);
-  b1 = c1312 * (vm[1] + vm[3] - 2 * vm[2]) * (vm[1] + vm[3] - 2 * vm[2]) +
-       0.25 * (vm[1] - vm[3]) * (vm[1] - vm[3]);
-  b2 = c1312 * (vm[0] + vm[2] - 2 * vm[1]) * (vm[0] + vm[2] - 2 * vm[1]) +
-       0.25 * (vm[0] - 4 * vm[1] + 3 * vm[2]) * (vm[0] - 4 * vm[1] + 3 * vm[2]);
-  tau = (b0 - b2) * (b0 - b2);
-  a0 = 0.1 + 0.1 * tau / ((eps + b0) * (eps + b0));
-  a1 = 0.6 + 0.6 * tau / ((eps + b1) * (eps + b1));
-  a2 = 0.3 + 0.3 * tau / ((eps + b2) * (eps + b2));
-  const real fm{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
+  beta5(vm, b);
+  alpha5z(b, eps, 0.1, 0.6, 0.3, a);
+  const real fm{(a[0] * v0 + a[1] * v1 + a[2] * v2) / (a[0] + a[1] + a[2])};
   return fp + fm;
 }
 
 // WENO5-JS (Jiang & Shu 1996, J. Comput. Phys. 126:202): the same candidates
 // and smoothness indicators as WENO-Z5 above, weights alpha_k = d_k/(eps+beta_k)^2.
 // Not present in the reference; selected by inviscid_scheme = 50.
-real weno5js(const real* vp, const real* vm, double eps) {
+real weno5js(const real* vp, const real* vm, geo eps) {
   constexpr double one6th{1.0 / 6};
   real v0{one6th * (2 * vp[2] + 5 * vp[3] - vp[4])};
   real v1{one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3])};
   real v2{one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2])};
-  constexpr double c1312{13.0 / 12};
-  real b0 = c1312 * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
-              0.25 * (3 * vp[2] - 4 * vp[3] + vp[4]) * (3 * vp[2] - 4 * vp[3] + vp[4]);
-  real b1 = c1312 * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
-              0.25 * (vp[1] - vp[3]) * (vp[1] - vp[3]);
-  real b2 = c1312 * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
-              0.25 * (vp[0] - 4 * vp[1] + 3 * vp[2]) * (vp[0] - 4 * vp[1] + 3 * vp[2]);
-  real a0{0.3 / ((eps + b0) * (eps + b0))};
-  real a1{0.6 / ((eps + b1) * (eps + b1))};
-  real a2{0.1 / ((eps + b2) * (eps + b2))};
+  real b[3];
+  beta5(vp, b);
+  real e0 = eps + b[0], e1 = eps + b[1], e2 = eps + b[2];
+  real a0{0.3 / (e0 * e0)};
+  real a1{0.6 / (e1 * e1)};
+  real a2{0.1 / (e2 * e2)};
   const real fp{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
 
   v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
   v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
   v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
-  b0 = c1312 * (vm[2] + vm[4] - 2 * vm[3]) * (vm[2] + vm[4] - 2 * vm[3]) +
-       0.25 * (3 * vm[2] - 4 * vm[3] + vm[4]) * (3 * vm[2] - 4 * vm[3] + vm[4]);
-  b1 = c1312 * (vm[1] + vm[3] - 2 * vm[2]) * (vm[1] + vm[3] - 2 * vm[2]) +
-       0.25 * (vm[1] - vm[3]) * (vm[1] - vm[3]);
-  b2 = c1312 * (vm[0] + vm[2] - 2 * vm[1]) * (vm[0] + vm[2] - 2 * vm[1]) +
-       0.25 * (vm[0] - 4 * vm[1] + 3 * vm[2]) * (vm[0] - 4 * vm[1] + 3 * vm[2]);
-  a0 = 0.1 / ((eps + b0) * (eps + b0));
-  a1 = 0.6 / ((eps + b1) * (eps + b1));
-  a2 = 0.3 / ((eps + b2) * (eps + b2));
+  beta5(vm, b);
+  e0 = eps + b[0];
+  e1 = eps + b[1];
+  e2 = eps + b[2];
+  a0 = 0.1 / (e0 * e0);
+  a1 = 0.6 / (e1 * e1);
+  a2 = 0.3 / (e2 * e2);
   const real fm{(a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2)};
   return fp + fm;
 }
 
+// One WENO-Z weight d (1 + tau / (eps + beta)^2), (eps + beta) formed once.
+inline real zweight(double d, const real& tau, geo eps, const real& beta) {
+  const real e = eps + beta;
+  return d + d * tau / (e * e);
+}
+
 // WENO-Z7, src/WENO.cu:1619-1693
-real weno7z(const real* vp, const real* vm, double eps) {
+real weno7z(const real* vp, const real* vm, geo eps) {
   constexpr double one6th{1.0 / 6};
   constexpr double d12{13.0 / 12.0}, d13{1043.0 / 960}, d14{1.0 / 12};
   real s1{one6th * (-2 * vp[0] + 9 * vp[1] - 18 * vp[2] + 11 * vp[3])};
@@ -201,12 +225,13 @@ real weno7z(const real* vp, const real* vm, double eps) {
   s2 = 2 * vp[3] - 5 * vp[4] + 4 * vp[5] - vp[6];
   s3 = -vp[3] + 3 * vp[4] - 3 * vp[5] + vp[6];
   real b3{s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3};
-  real tau{(b0 - b3) * (b0 - b3)};
+  const real db{b0 - b3};
+  real tau{db * db};
   constexpr double c0{1.0 / 35}, c1{12.0 / 35}, c2{18.0 / 35}, c3{4.0 / 35};
-  real a0{c0 + c0 * tau / ((eps + b0) * (eps + b0))};
-  real a1{c1 + c1 * tau / ((eps + b1) * (eps + b1))};
-  real a2{c2 + c2 * tau / ((eps + b2) * (eps + b2))};
-  real a3{c3 + c3 * tau / ((eps + b3) * (eps + b3))};
+  real a0{zweight(c0, tau, eps, b0)};
+  real a1{zweight(c1, tau, eps, b1)};
+  real a2{zweight(c2, tau, eps, b2)};
+  real a3{zweight(c3, tau, eps, b3)};
   constexpr double one12th{1.0 / 12};
   real v0{one12th * (-3 * vp[0] + 13 * vp[1] - 23 * vp[2] + 25 * vp[3])};
   real v1{one12th * (vp[1] - 5 * vp[2] + 13 * vp[3] + 3 * vp[4])};
@@ -229,11 +254,12 @@ real weno7z(const real* vp, const real* vm, double eps) {
   s2 = 2 * vm[3] - 5 * vm[2] + 4 * vm[1] - vm[0];
   s3 = -vm[3] + 3 * vm[2] - 3 * vm[1] + vm[0];
   b3 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
-  tau = (b0 - b3) * (b0 - b3);
-  a0 = c0 + c0 * tau / ((eps + b0) * (eps + b0));
-  a1 = c1 + c1 * tau / ((eps + b1) * (eps + b1));
-  a2 = c2 + c2 * tau / ((eps + b2) * (eps + b2));
-  a3 = c3 + c3 * tau / ((eps + b3) * (eps + b3));
+  const real dbm{b0 - b3};
+  tau = dbm * dbm;
+  a0 = zweight(c0, tau, eps, b0);
+  a1 = zweight(c1, tau, eps, b1);
+  a2 = zweight(c2, tau, eps, b2);
+  a3 = zweight(c3, tau, eps, b3);
   v0 = one12th * (-3 * vm[6] + 13 * vm[5] - 23 * vm[4] + 25 * vm[3]);
   v1 = one12th * (vm[5] - 5 * vm[4] + 13 * vm[3] + 3 * vm[2]);
   v2 = one12th * (-vm[4] + 7 * vm[3] + 7 * vm[2] - vm[1]);
@@ -244,24 +270,25 @@ real weno7z(const real* vp, const real* vm, double eps) {
 
 // Point-local Lax-Friedrichs splitting, src/WENO.cu:697-720.
 // q = [cv(0..nv-1), p, c]; k3 = metric row of the sweep direction.
-void lf_split(const real* q, int nv, const double* m, double jac, real* fp, real* fm) {
+void lf_split(const real* q, int nv, const geo* m, geo jac, real* fp, real* fm) {
   const real Uk{(q[1] * m[0] + q[2] * m[1] + q[3] * m[2]) / q[0]};
   const real pk{q[nv]};
-  const real cGradK = q[nv + 1] * std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  const real cGradK = q[nv + 1] * sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
   const real lambda0 = abs(Uk) + cGradK;
-  fp[0] = 0.5 * jac * (Uk * q[0] + lambda0 * q[0]);
-  fp[1] = 0.5 * jac * (Uk * q[1] + pk * m[0] + lambda0 * q[1]);
-  fp[2] = 0.5 * jac * (Uk * q[2] + pk * m[1] + lambda0 * q[2]);
-  fp[3] = 0.5 * jac * (Uk * q[3] + pk * m[2] + lambda0 * q[3]);
-  fp[4] = 0.5 * jac * (Uk * (q[4] + pk) + lambda0 * q[4]);
-  fm[0] = 0.5 * jac * (Uk * q[0] - lambda0 * q[0]);
-  fm[1] = 0.5 * jac * (Uk * q[1] + pk * m[0] - lambda0 * q[1]);
-  fm[2] = 0.5 * jac * (Uk * q[2] + pk * m[1] - lambda0 * q[2]);
-  fm[3] = 0.5 * jac * (Uk * q[3] + pk * m[2] - lambda0 * q[3]);
-  fm[4] = 0.5 * jac * (Uk * (q[4] + pk) - lambda0 * q[4]);
+  const geo hj = 0.5 * jac;  // the reference's 0.5 * jac of every line, formed once
+  fp[0] = hj * (Uk * q[0] + lambda0 * q[0]);
+  fp[1] = hj * (Uk * q[1] + pk * m[0] + lambda0 * q[1]);
+  fp[2] = hj * (Uk * q[2] + pk * m[1] + lambda0 * q[2]);
+  fp[3] = hj * (Uk * q[3] + pk * m[2] + lambda0 * q[3]);
+  fp[4] = hj * (Uk * (q[4] + pk) + lambda0 * q[4]);
+  fm[0] = hj * (Uk * q[0] - lambda0 * q[0]);
+  fm[1] = hj * (Uk * q[1] + pk * m[0] - lambda0 * q[1]);
+  fm[2] = hj * (Uk * q[2] + pk * m[1] - lambda0 * q[2]);
+  fm[3] = hj * (Uk * q[3] + pk * m[2] - lambda0 * q[3]);
+  fm[4] = hj * (Uk * (q[4] + pk) - lambda0 * q[4]);
   for (int l = 5; l < nv; ++l) {
-    fp[l] = 0.5 * jac * (Uk * q[l] + lambda0 * q[l]);
-    fm[l] = 0.5 * jac * (Uk * q[l] - lambda0 * q[l]);
+    fp[l] = hj * (Uk * q[l] + lambda0 * q[l]);
+    fm[l] = hj * (Uk * q[l] - lambda0 * q[l]);
   }
 }
 
@@ -317,11 +344,11 @@ void local_dt(const Ctx& c, Block& b) {
     for (int j = 0; j < b.my; ++j)
       for (int i = 0; i < b.mx; ++i) {
         const long long p = b.id(i, j, k);
-        const double* M = &b.metric[0];
+        const geo* M = &b.metric[0];
         auto m = [&](int r, int cc) { return M[p + (3 * (r - 1) + (cc - 1)) * ms]; };
-        const double gxi = std::sqrt(m(1, 1) * m(1, 1) + m(1, 2) * m(1, 2) + m(1, 3) * m(1, 3));
-        const double geta = std::sqrt(m(2, 1) * m(2, 1) + m(2, 2) * m(2, 2) + m(2, 3) * m(2, 3));
-        const double gzeta = std::sqrt(m(3, 1) * m(3, 1) + m(3, 2) * m(3, 2) + m(3, 3) * m(3, 3));
+        const geo gxi = sqrt(m(1, 1) * m(1, 1) + m(1, 2) * m(1, 2) + m(1, 3) * m(1, 3));
+        const geo geta = sqrt(m(2, 1) * m(2, 1) + m(2, 2) * m(2, 2) + m(2, 3) * m(2, 3));
+        const geo gzeta = sqrt(m(3, 1) * m(3, 1) + m(3, 2) * m(3, 2) + m(3, 3) * m(3, 3));
         const real u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
         const real U = u * m(1, 1) + v * m(1, 2) + w * m(1, 3);
         const real V = u * m(2, 1) + v * m(2, 2) + w * m(2, 3);
@@ -349,7 +376,7 @@ struct CharAir {
   real L[5][5], R[5][5];
 };
 
-void char_air(const real* cvl, const real* cvr, const double* m1, const double* m2, double jac1, double jac2,
+void char_air(const real* cvl, const real* cvr, const geo* m1, const geo* m2, geo jac1, geo jac2,
               CharAir& M) {
   const real rlc{sqrt(cvl[0]) / (sqrt(cvl[0]) + sqrt(cvr[0]))};
   const real rrc{sqrt(cvr[0]) / (sqrt(cvl[0]) + sqrt(cvr[0]))};
@@ -363,10 +390,10 @@ void char_air(const real* cvl, const real* cvr, const double* m1, const double* 
   const real hm{rlc * hl + rrc * hr};
   const real cm2{gm1 * (hm - ekm)};
   const real cm{sqrt(cm2)};
-  double kx{(jac1 * m1[0] + jac2 * m2[0]) / (jac1 + jac2)};
-  double ky{(jac1 * m1[1] + jac2 * m2[1]) / (jac1 + jac2)};
-  double kz{(jac1 * m1[2] + jac2 * m2[2]) / (jac1 + jac2)};
-  const double gradK{std::sqrt(kx * kx + ky * ky + kz * kz)};
+  geo kx{(jac1 * m1[0] + jac2 * m2[0]) / (jac1 + jac2)};
+  geo ky{(jac1 * m1[1] + jac2 * m2[1]) / (jac1 + jac2)};
+  geo kz{(jac1 * m1[2] + jac2 * m2[2]) / (jac1 + jac2)};
+  const geo gradK{sqrt(kx * kx + ky * ky + kz * kz)};
   kx /= gradK;
   ky /= gradK;
   kz /= gradK;
@@ -465,11 +492,11 @@ void reconstruct_air(const real* pv, int is, int recon, real* pl, real* pr) {
 }
 
 // riemannSolver_ausmPlus<Air>, src/RiemannSolver.cuh:9-96
-void ausm_air(const real* pl, const real* pr, const double* ml, const double* mr, double jl, double jr, real* fci) {
-  const double k1 = 0.5 * (jl * ml[0] + jr * mr[0]);
-  const double k2 = 0.5 * (jl * ml[1] + jr * mr[1]);
-  const double k3 = 0.5 * (jl * ml[2] + jr * mr[2]);
-  const double grad_k_div_jac = std::sqrt(k1 * k1 + k2 * k2 + k3 * k3);
+void ausm_air(const real* pl, const real* pr, const geo* ml, const geo* mr, geo jl, geo jr, real* fci) {
+  const geo k1 = 0.5 * (jl * ml[0] + jr * mr[0]);
+  const geo k2 = 0.5 * (jl * ml[1] + jr * mr[1]);
+  const geo k3 = 0.5 * (jl * ml[2] + jr * mr[2]);
+  const geo grad_k_div_jac = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
   const real ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) / grad_k_div_jac;
   const real ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) / grad_k_div_jac;
   const real p_l = pl[4], p_r = pr[4], rho_l = pl[0], rho_r = pr[0];
@@ -510,7 +537,7 @@ void ausm_air(const real* pl, const real* pr, const double* ml, const double* mr
 }
 
 // riemannSolver_hllc<Air>, src/RiemannSolver.cuh:98-230
-void hllc_air(const real* pl, const real* pr, const double* ml, const double* mr, double jl, double jr, real* fci) {
+void hllc_air(const real* pl, const real* pr, const geo* ml, const geo* mr, geo jl, geo jr, real* fci) {
   const real rl_c{sqrt(pl[0]) / (sqrt(pl[0]) + sqrt(pr[0]))}, rr_c{sqrt(pr[0]) / (sqrt(pl[0]) + sqrt(pr[0]))};
   const real u_tilde{pl[1] * rl_c + pr[1] * rr_c};
   const real v_tilde{pl[2] * rl_c + pr[2] * rr_c};
@@ -521,12 +548,12 @@ void hllc_air(const real* pl, const real* pr, const double* ml, const double* mr
   const real h_tilde{hl * rl_c + hr * rr_c};
   const real ek_tilde{0.5 * (u_tilde * u_tilde + v_tilde * v_tilde + w_tilde * w_tilde)};
   const real c_tilde = sqrt((gamma - 1) * (h_tilde - ek_tilde));
-  const double kx = 0.5 * (ml[0] + mr[0]);
-  const double ky = 0.5 * (ml[1] + mr[1]);
-  const double kz = 0.5 * (ml[2] + mr[2]);
-  const double gradK = std::sqrt(kx * kx + ky * ky + kz * kz);
+  const geo kx = 0.5 * (ml[0] + mr[0]);
+  const geo ky = 0.5 * (ml[1] + mr[1]);
+  const geo kz = 0.5 * (ml[2] + mr[2]);
+  const geo gradK = sqrt(kx * kx + ky * ky + kz * kz);
   const real U_tilde_bar{(kx * u_tilde + ky * v_tilde + kz * w_tilde) / gradK};
-  const double jac_ave{0.5 * (jl + jr)};
+  const geo jac_ave{0.5 * (jl + jr)};
   const real Ul{kx * pl[1] + ky * pl[2] + kz * pl[3]};
   const real cl{sqrt(kGammaAir * pl[4] / pl[0])};
   const real sl{std::min(Ul / gradK - cl, U_tilde_bar - c_tilde)};
@@ -562,17 +589,17 @@ void hllc_air(const real* pl, const real* pr, const double* ml, const double* mr
 }
 
 // riemannSolver_laxFriedrich<Air>, src/RiemannSolver.cuh:426-461
-void lf_air(const real* pl, const real* pr, const double* ml, const double* mr, double jl, double jr, real* fci) {
-  const double kx = 0.5 * (ml[0] + mr[0]);
-  const double ky = 0.5 * (ml[1] + mr[1]);
-  const double kz = 0.5 * (ml[2] + mr[2]);
-  const double gradK = std::sqrt(kx * kx + ky * ky + kz * kz);
+void lf_air(const real* pl, const real* pr, const geo* ml, const geo* mr, geo jl, geo jr, real* fci) {
+  const geo kx = 0.5 * (ml[0] + mr[0]);
+  const geo ky = 0.5 * (ml[1] + mr[1]);
+  const geo kz = 0.5 * (ml[2] + mr[2]);
+  const geo gradK = sqrt(kx * kx + ky * ky + kz * kz);
   const real Ukl{pl[1] * kx + pl[2] * ky + pl[3] * kz};
   const real Ukr{pr[1] * kx + pr[2] * ky + pr[3] * kz};
   const real cl{sqrt(kGammaAir * pl[4] / pl[0])};
   const real cr{sqrt(kGammaAir * pr[4] / pr[0])};
   const real spectral_radius{std::max(abs(Ukl) + cl * gradK, abs(Ukr) + cr * gradK)};
-  const double half_jac_ave{0.5 * 0.5 * (jl + jr)};
+  const geo half_jac_ave{0.5 * 0.5 * (jl + jr)};
   const real rhoUl{pl[0] * Ukl};
   const real rhoUr{pr[0] * Ukr};
   fci[0] = (rhoUl + rhoUr - spectral_radius * (pr[0] - pl[0])) * half_jac_ave;
@@ -585,24 +612,24 @@ void lf_air(const real* pl, const real* pr, const double* ml, const double* mr, 
   fci[4] = ((pl[5] + pl[4]) * Ukl + (pr[5] + pr[4]) * Ukr - spectral_radius * (pr[5] - pl[5])) * half_jac_ave;
 }
 
-// compute_entropy_fix_delta<Air>, src/InviscidScheme.cu:421-453 (3D),
-// entropy_fix_factor 0.125 (the default, src/Parameter.cpp:471-507).
-real entropy_delta(const real* q, const double* m) {
+// compute_entropy_fix_delta<Air>, src/InviscidScheme.cu:421-453 (3D), with
+// the entropy_fix_factor of the setup (src/Parameter.cpp:474, default 0.125).
+real entropy_delta(const real* q, const geo* m, double factor) {
   const real U = abs(q[1] * m[0] + q[2] * m[1] + q[3] * m[2]);
   const real V = abs(q[1] * m[3] + q[2] * m[4] + q[3] * m[5]);
   const real W = abs(q[1] * m[6] + q[2] * m[7] + q[3] * m[8]);
-  const double kx = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
-  const double ky = std::sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
-  const double kz = std::sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
+  const geo kx = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  const geo ky = sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
+  const geo kz = sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
   const real c = sqrt(kGammaAir * q[4] / q[0]);
-  return 0.125 * (U + V + W + c * (kx + ky + kz) / 3.0);
+  return factor * (U + V + W + c * (kx + ky + kz) / 3.0);
 }
 
 // riemannSolver_Roe<Air>, src/RiemannSolver.cuh:279-417, with
 // compute_half_sum_left_right_flux :234-277.
-void roe_air(const real* pl, const real* pr, const double* ml, const double* mr, double jl, double jr, real del_l,
+void roe_air(const real* pl, const real* pr, const geo* ml, const geo* mr, geo jl, geo jr, real del_l,
              real del_r, real* fci) {
-  double JacKx = jl * ml[0], JacKy = jl * ml[1], JacKz = jl * ml[2];
+  geo JacKx = jl * ml[0], JacKy = jl * ml[1], JacKz = jl * ml[2];
   real Uk = pl[1] * JacKx + pl[2] * JacKy + pl[3] * JacKz;
   real coeff = Uk * pl[0];
   fci[0] = 0.5 * coeff;
@@ -632,10 +659,10 @@ void roe_air(const real* pl, const real* pr, const double* ml, const double* mr,
   const real h = (dl * hl + dr * hr) * inv_denominator;
   const double gamma = kGammaAir;
   const real c = sqrt((gamma - 1) * (h - ek));
-  double kx = 0.5 * (ml[0] + mr[0]);
-  double ky = 0.5 * (ml[1] + mr[1]);
-  double kz = 0.5 * (ml[2] + mr[2]);
-  const double gradK = std::sqrt(kx * kx + ky * ky + kz * kz);
+  geo kx = 0.5 * (ml[0] + mr[0]);
+  geo ky = 0.5 * (ml[1] + mr[1]);
+  geo kz = 0.5 * (ml[2] + mr[2]);
+  const geo gradK = sqrt(kx * kx + ky * ky + kz * kz);
   real Ukk = kx * u + ky * v + kz * w;
   real characteristic[3] = {Ukk - gradK * c, Ukk, Ukk + gradK * c};
   const real delta_ave = 0.5 * (del_l + del_r);
@@ -647,7 +674,7 @@ void roe_air(const real* pl, const real* pr, const double* ml, const double* mr,
   ky /= gradK;
   kz /= gradK;
   Ukk /= gradK;
-  const double jac_ave = 0.5 * (jl + jr);
+  const geo jac_ave = 0.5 * (jl + jr);
   real dq[5];
   dq[0] = jac_ave * (pr[0] - pl[0]);
   for (int l = 1; l < 5; ++l) dq[l] = jac_ave * (pr[0] * pr[l] - pl[0] * pl[l]);
@@ -677,7 +704,7 @@ void roe_air(const real* pl, const real* pr, const double* ml, const double* mr,
                    c * ((kz * v - ky * w) * bb[1] + (kx * w - kz * u) * bb[2] + (ky * u - kx * v) * bb[3]) + c3);
 }
 
-void riemann_air(int scheme, const real* pl, const real* pr, const double* ml, const double* mr, double jl, double jr,
+void riemann_air(int scheme, const real* pl, const real* pr, const geo* ml, const geo* mr, geo jl, geo jr,
                  real* fci) {
   if (scheme == 1) lf_air(pl, pr, ml, mr, jl, jr, fci);
   else if (scheme == 3) ausm_air(pl, pr, ml, mr, jl, jr, fci);
@@ -697,7 +724,7 @@ void inviscid_pv(const Ctx& c, Block& b) {
     {
       const int np = m + 2 * g;
       std::vector<real> PV(np * 5), fc((m + 1) * 5), D(np);
-      std::vector<double> K(np * 3), J(np);
+      std::vector<geo> K(np * 3), J(np);
 #pragma omp for collapse(2) schedule(static)
       for (int bb = 0; bb < nb; ++bb)
         for (int aa = 0; aa < na; ++aa) {
@@ -712,9 +739,9 @@ void inviscid_pv(const Ctx& c, Block& b) {
             for (int cc = 0; cc < 3; ++cc) K[ps * 3 + cc] = b.metric[p + (3 * dir + cc) * s];
             J[ps] = b.jac[p];
             if (c.s.inviscid_scheme == 2) {
-              double mm[9];
+              geo mm[9];
               for (int cc = 0; cc < 9; ++cc) mm[cc] = b.metric[p + cc * s];
-              D[ps] = entropy_delta(&PV[ps * 5], mm);
+              D[ps] = entropy_delta(&PV[ps * 5], mm, c.s.entropy_fix_factor);
             }
           }
           for (int t = -1; t < m; ++t) {
@@ -735,6 +762,13 @@ void inviscid_pv(const Ctx& c, Block& b) {
         }
     }
   }
+}
+
+// |m1 J1 + m2 J2|^2 of the two points' Jacobian-weighted metric rows, the
+// factor of the WENO epsilon (src/WENO.cu:1469-1476); each sum formed once.
+inline geo face_metric_sq(const geo* m1, const geo* m2, geo jac1, geo jac2) {
+  const geo ax = m1[0] * jac1 + m2[0] * jac2, ay = m1[1] * jac1 + m2[1] * jac2, az = m1[2] * jac1 + m2[2] * jac2;
+  return ax * ax + ay * ay + az * az;
 }
 
 // compute_convective_term_weno_{x,y,z} + compute_weno_flux_cp,
@@ -758,7 +792,7 @@ void inviscid(const Ctx& c, Block& b) {
     {
       const int np = m + 2 * g;
       std::vector<real> Fp(np * nv), Fm(np * nv), fc((m + 1) * nv);
-      std::vector<double> kj(np * 3);
+      std::vector<geo> kj(np * 3);
       std::vector<real> q(nv + 2);
 #pragma omp for collapse(2) schedule(static)
       for (int bb = 0; bb < nb; ++bb)
@@ -773,9 +807,9 @@ void inviscid(const Ctx& c, Block& b) {
             for (int l = 0; l < nv; ++l) q[l] = b.cv[p + l * s];
             q[nv] = b.bv[p + 4 * s];
             q[nv + 1] = sqrt(kGammaAir * kRAir * b.bv[p + 5 * s]);
-            double mrow[3];
+            geo mrow[3];
             for (int cc = 0; cc < 3; ++cc) mrow[cc] = b.metric[p + (3 * dir + cc) * s];
-            const double jac = b.jac[p];
+            const geo jac = b.jac[p];
             const int ps = t + g;
             lf_split(q.data(), nv, mrow, jac, &Fp[ps * nv], &Fm[ps * nv]);
             for (int cc = 0; cc < 3; ++cc) kj[ps * 3 + cc] = mrow[cc];
@@ -787,9 +821,9 @@ void inviscid(const Ctx& c, Block& b) {
             const long long p1 = b.id(idx[0], idx[1], idx[2]);
             idx[dir] = t + 1;
             const long long p2 = b.id(idx[0], idx[1], idx[2]);
-            const double jac1{b.jac[p1]}, jac2{b.jac[p2]};
-            const double* m1 = &kj[is * 3];
-            const double* m2 = &kj[(is + 1) * 3];
+            const geo jac1{b.jac[p1]}, jac2{b.jac[p2]};
+            const geo* m1 = &kj[is * 3];
+            const geo* m2 = &kj[(is + 1) * 3];
             if (scheme == 52 || scheme == 72) {
               // compute_weno_flux_ch<Air>, "method 1", src/WENO.cu:1147-1453;
               // 72 projects 7 points and applies WENO-Z7 to the five fields
@@ -806,10 +840,7 @@ void inviscid(const Ctx& c, Block& b) {
               CharAir M;
               char_air(cvl, cvr, m1, m2, jac1, jac2, M);
               constexpr double eps{1e-40};
-              const double eps_scaled = eps * c.weno_eps_scale * 0.25 *
-                                        ((m1[0] * jac1 + m2[0] * jac2) * (m1[0] * jac1 + m2[0] * jac2) +
-                                         (m1[1] * jac1 + m2[1] * jac2) * (m1[1] * jac1 + m2[1] * jac2) +
-                                         (m1[2] * jac1 + m2[2] * jac2) * (m1[2] * jac1 + m2[2] * jac2));
+              const geo eps_scaled = eps * c.weno_eps_scale * 0.25 * face_metric_sq(m1, m2, jac1, jac2);
               real fChar[5];
               for (int l = 0; l < 5; ++l) {
                 if (scheme == 72) {
@@ -838,16 +869,13 @@ void inviscid(const Ctx& c, Block& b) {
               continue;
             }
             constexpr double eps{1e-20};
-            const double eps_ref = eps * c.weno_eps_scale * 0.25 *
-                                   ((m1[0] * jac1 + m2[0] * jac2) * (m1[0] * jac1 + m2[0] * jac2) +
-                                    (m1[1] * jac1 + m2[1] * jac2) * (m1[1] * jac1 + m2[1] * jac2) +
-                                    (m1[2] * jac1 + m2[2] * jac2) * (m1[2] * jac1 + m2[2] * jac2));
-            double eps_scaled[3];
+            const geo eps_ref = eps * c.weno_eps_scale * 0.25 * face_metric_sq(m1, m2, jac1, jac2);
+            geo eps_scaled[3];
             eps_scaled[0] = eps_ref;
             eps_scaled[1] = eps_ref * c.s.v_ref * c.s.v_ref;
             eps_scaled[2] = eps_scaled[1] * c.s.v_ref * c.s.v_ref;
             for (int l = 0; l < nv; ++l) {
-              double eh{eps_scaled[0]};
+              geo eh{eps_scaled[0]};
               if (l == 1 || l == 2 || l == 3) eh = eps_scaled[1];
               else if (l == 4) eh = eps_scaled[2];
               real f;
@@ -890,15 +918,15 @@ void viscous_face(const Ctx& c, const Block& b, int dir, int i, int j, int k, re
   o[dir] = 1;
   const long long p = b.id(i, j, k), p1 = b.id(i + o[0], j + o[1], k + o[2]);
   auto M = [&](long long pt, int r, int cc) { return b.metric[pt + (3 * (r - 1) + (cc - 1)) * s]; };
-  const double xi_x = 0.5 * (M(p, 1, 1) + M(p1, 1, 1));
-  const double xi_y = 0.5 * (M(p, 1, 2) + M(p1, 1, 2));
-  const double xi_z = 0.5 * (M(p, 1, 3) + M(p1, 1, 3));
-  const double eta_x = 0.5 * (M(p, 2, 1) + M(p1, 2, 1));
-  const double eta_y = 0.5 * (M(p, 2, 2) + M(p1, 2, 2));
-  const double eta_z = 0.5 * (M(p, 2, 3) + M(p1, 2, 3));
-  const double zeta_x = 0.5 * (M(p, 3, 1) + M(p1, 3, 1));
-  const double zeta_y = 0.5 * (M(p, 3, 2) + M(p1, 3, 2));
-  const double zeta_z = 0.5 * (M(p, 3, 3) + M(p1, 3, 3));
+  const geo xi_x = 0.5 * (M(p, 1, 1) + M(p1, 1, 1));
+  const geo xi_y = 0.5 * (M(p, 1, 2) + M(p1, 1, 2));
+  const geo xi_z = 0.5 * (M(p, 1, 3) + M(p1, 1, 3));
+  const geo eta_x = 0.5 * (M(p, 2, 1) + M(p1, 2, 1));
+  const geo eta_y = 0.5 * (M(p, 2, 2) + M(p1, 2, 2));
+  const geo eta_z = 0.5 * (M(p, 2, 3) + M(p1, 2, 3));
+  const geo zeta_x = 0.5 * (M(p, 3, 1) + M(p1, 3, 1));
+  const geo zeta_y = 0.5 * (M(p, 3, 2) + M(p1, 3, 2));
+  const geo zeta_z = 0.5 * (M(p, 3, 3) + M(p1, 3, 3));
   auto pv = [&](int ii, int jj, int kk, int l) { return b.bv[b.id(ii, jj, kk) + l * s]; };
   // derivative along the face normal: 2-point; tangential: 4-point average
   real d[4][3];  // [u,v,w,T][xi,eta,zeta]
@@ -938,9 +966,9 @@ void viscous_face(const Ctx& c, const Block& b, int dir, int i, int j, int k, re
   const real tau_xz = viscosity * (u_z + w_x);
   const real tau_yz = viscosity * (v_z + w_y);
   const int r = dir + 1;
-  const double kx = 0.5 * (M(p, r, 1) * b.jac[p] + M(p1, r, 1) * b.jac[p1]);
-  const double ky = 0.5 * (M(p, r, 2) * b.jac[p] + M(p1, r, 2) * b.jac[p1]);
-  const double kz = 0.5 * (M(p, r, 3) * b.jac[p] + M(p1, r, 3) * b.jac[p1]);
+  const geo kx = 0.5 * (M(p, r, 1) * b.jac[p] + M(p1, r, 1) * b.jac[p1]);
+  const geo ky = 0.5 * (M(p, r, 2) * b.jac[p] + M(p1, r, 2) * b.jac[p1]);
+  const geo kz = 0.5 * (M(p, r, 3) * b.jac[p] + M(p1, r, 3) * b.jac[p1]);
   fv[0] = 0;
   fv[1] = kx * tau_xx + ky * tau_xy + kz * tau_xz;
   fv[2] = kx * tau_xy + ky * tau_yy + kz * tau_yz;
@@ -998,7 +1026,7 @@ void update(const Ctx& c, Block& b, double dt, int rk) {
     for (int j = 0; j < b.my; ++j)
       for (int i = 0; i < b.mx; ++i) {
         const long long p = b.id(i, j, k), p0 = b.id0(i, j, k);
-        const double dt_div_jac = dt / b.jac[p];
+        const geo dt_div_jac = dt / b.jac[p];
         for (int l = 0; l < nv; ++l)
           b.cv[p + l * s] = kRkA[rk] * b.qn[p + l * s] + kRkB[rk] * b.cv[p + l * s] +
                             kRkC[rk] * dt_div_jac * b.dq[p0 + l * n0];
@@ -1063,9 +1091,9 @@ void apply_farfield(Block& b, const Face& f) {
     for (int j = lo[1]; j <= hi[1]; ++j)
       for (int i = lo[0]; i <= hi[0]; ++i) {
         const long long p = b.id(i, j, k);
-        double nx = b.metric[p + (3 * f.axis) * s], ny = b.metric[p + (3 * f.axis + 1) * s],
+        geo nx = b.metric[p + (3 * f.axis) * s], ny = b.metric[p + (3 * f.axis + 1) * s],
                nz = b.metric[p + (3 * f.axis + 2) * s];
-        const double grad_n_inv = f.side / std::sqrt(nx * nx + ny * ny + nz * nz);
+        const geo grad_n_inv = f.side / sqrt(nx * nx + ny * ny + nz * nz);
         nx *= grad_n_inv;
         ny *= grad_n_inv;
         nz *= grad_n_inv;
@@ -1084,8 +1112,8 @@ void apply_farfield(Block& b, const Face& f) {
           face_too = false;
         } else {
           const real riemann_pos{u_face + 2 * a_b / (gamma_air - 1)};
-          const double u_inf{ff.u * nx + ff.v * ny + ff.w * nz};
-          const double riemann_neg{u_inf - 2 * ff.acoustic_speed / (ff.specific_heat_ratio - 1)};
+          const geo u_inf{ff.u * nx + ff.v * ny + ff.w * nz};
+          const geo riemann_neg{u_inf - 2 * ff.acoustic_speed / (ff.specific_heat_ratio - 1)};
           const real Un{0.5 * (riemann_pos + riemann_neg)};
           const real c_b{0.25 * (gamma_air - 1) * (riemann_pos - riemann_neg)};
           real s_b;
@@ -1144,7 +1172,7 @@ void apply_inflow(Block& b, const Face& f) {
         const double density = S[0], w = S[3], p = S[4], T = S[5];
         double v = S[2];
         if (in.fluctuation_type == 1) {
-          const double vel = std::sqrt(u * u + v * v + w * w);
+          const double vel = sqrt(u * u + v * v + w * w);
           const uint64_t point = f.axis == 0 ? (uint64_t)(k + g) * (b.my + 2 * g) + (j + g)
                                  : f.axis == 1 ? (uint64_t)(k + g) * (b.mx + 2 * g) + (i + g)
                                                : (uint64_t)(j + g) * (b.mx + 2 * g) + (i + g);
@@ -1188,14 +1216,14 @@ void sponge_source(Block& b) {
       for (int i = 0; i < b.mx; ++i) {
         const long long p = b.id(i, j, k), t = ((long long)k * b.my + j) * b.mx + i, q = b.idg(i, j, k);
         const double pos[3] = {b.x[q], b.y[q], b.z[q]};
-        double sigma = 0;
+        geo sigma = 0;
         for (int side = 0; side < 6; ++side) {
           if (!sp.enabled[side]) continue;
-          const double x = pos[side / 2], x1 = sp.start[side];
+          const geo x = pos[side / 2], x1 = sp.start[side];
           if (!(side % 2 == 0 ? x < x1 : x > x1)) continue;
-          double xi = (x - x1) / (sp.end[side] - x1);
+          geo xi = (x - x1) / (sp.end[side] - x1);
           if (xi > 1) xi = 1;
-          const double xi3 = xi * xi * xi;
+          const geo xi3 = xi * xi * xi;
           sigma += sp.sigma[side] * (6 * xi3 * xi * xi - 15 * xi3 * xi + 10 * xi3);
         }
         if (sp.jacobian_scaled) sigma = sigma * b.jac[p];
@@ -1416,7 +1444,7 @@ int orc_farfield_state_air(const orc_farfield_spec* in, orc_farfield* out) {
   if (temperature < 0) temperature = pressure * mw / (density * kRu);
   const double viscosity = value_of(sutherland(temperature));
   const double gamma = kGammaAir;
-  const double acoustic_speed = std::sqrt(gamma * kRu / mw * temperature);
+  const double acoustic_speed = sqrt(gamma * kRu / mw * temperature);
   if (mach < 0) mach = velocity / acoustic_speed;
   else velocity = mach * acoustic_speed;
   u *= velocity;
@@ -1467,7 +1495,8 @@ double* orc_array(void* ctx, int block, int which, long long* n_out) {
   auto* c = static_cast<Ctx*>(ctx);
   Block& b = c->blocks[block];
   std::vector<real>* v = nullptr;
-  std::vector<double>* g = nullptr;
+  std::vector<geo>* g = nullptr;
+  std::vector<double>* xyz = nullptr;
   switch (which) {
     case ORC_CV: v = &b.cv; break;
     case ORC_BV: v = &b.bv; break;
@@ -1478,15 +1507,20 @@ double* orc_array(void* ctx, int block, int which, long long* n_out) {
     case ORC_JAC: g = &b.jac; break;
     case ORC_METRIC: g = &b.metric; break;
     case ORC_DT_LOCAL: v = &b.dtl; break;
-    case ORC_X: g = &b.x; break;
-    case ORC_Y: g = &b.y; break;
-    case ORC_Z: g = &b.z; break;
+    case ORC_X: xyz = &b.x; break;
+    case ORC_Y: xyz = &b.y; break;
+    case ORC_Z: xyz = &b.z; break;
     case ORC_BV_LAST: v = &b.bv_last; break;
     default: return nullptr;
   }
-  if (g) {
+  if (xyz) {
+    if (n_out) *n_out = (long long)xyz->size();
+    return xyz->data();
+  }
+  if (g) {  // geometry: double, or a counted scalar of the same layout
+    static_assert(sizeof(geo) == sizeof(double), "geometry layout");
     if (n_out) *n_out = (long long)g->size();
-    return g->data();
+    return reinterpret_cast<double*>(g->data());
   }
   // state arrays: kWidth doubles per value ({value, tangent} in the tangent build)
   if (n_out) *n_out = (long long)v->size() * kWidth;
@@ -1524,18 +1558,18 @@ int orc_compute_metric(void* ctx, int block) {
         auto X = [&](int ii, int jj, int kk) { return b.x[b.idg(ii, jj, kk)]; };
         auto Y = [&](int ii, int jj, int kk) { return b.y[b.idg(ii, jj, kk)]; };
         auto Z = [&](int ii, int jj, int kk) { return b.z[b.idg(ii, jj, kk)]; };
-        const double dxd1 = (X(i + 1, j, k) - X(i - 1, j, k)) * 0.5, dyd1 = (Y(i + 1, j, k) - Y(i - 1, j, k)) * 0.5,
+        const geo dxd1 = (X(i + 1, j, k) - X(i - 1, j, k)) * 0.5, dyd1 = (Y(i + 1, j, k) - Y(i - 1, j, k)) * 0.5,
                      dzd1 = (Z(i + 1, j, k) - Z(i - 1, j, k)) * 0.5;
-        const double dxd2 = (X(i, j + 1, k) - X(i, j - 1, k)) * 0.5, dyd2 = (Y(i, j + 1, k) - Y(i, j - 1, k)) * 0.5,
+        const geo dxd2 = (X(i, j + 1, k) - X(i, j - 1, k)) * 0.5, dyd2 = (Y(i, j + 1, k) - Y(i, j - 1, k)) * 0.5,
                      dzd2 = (Z(i, j + 1, k) - Z(i, j - 1, k)) * 0.5;
-        const double dxd3 = (X(i, j, k + 1) - X(i, j, k - 1)) * 0.5, dyd3 = (Y(i, j, k + 1) - Y(i, j, k - 1)) * 0.5,
+        const geo dxd3 = (X(i, j, k + 1) - X(i, j, k - 1)) * 0.5, dyd3 = (Y(i, j, k + 1) - Y(i, j, k - 1)) * 0.5,
                      dzd3 = (Z(i, j, k + 1) - Z(i, j, k - 1)) * 0.5;
-        const double jac = dxd1 * dyd2 * dzd3 + dxd2 * dyd3 * dzd1 + dxd3 * dyd1 * dzd2 - dxd1 * dyd3 * dzd2 -
+        const geo jac = dxd1 * dyd2 * dzd3 + dxd2 * dyd3 * dzd1 + dxd3 * dyd1 * dzd2 - dxd1 * dyd3 * dzd2 -
                            dxd2 * dyd1 * dzd3 - dxd3 * dyd2 * dzd1;
         if (jac <= 0) return -2;
         const long long p = b.id(i, j, k);
         b.jac[p] = jac;
-        double* M = b.metric.data();
+        geo* M = b.metric.data();
         M[p + 0 * s] = (dyd2 * dzd3 - dyd3 * dzd2) / jac;
         M[p + 1 * s] = (dxd3 * dzd2 - dxd2 * dzd3) / jac;
         M[p + 2 * s] = (dxd2 * dyd3 - dxd3 * dyd2) / jac;
@@ -1556,8 +1590,8 @@ double orc_sutherland(double t) { return value_of(sutherland(t)); }
 // Mach number given (:104-107 for upper_faster, :108-110 otherwise).
 void orc_mixing_layer_streams(double ma_c, double ma_upper, double t_upper, double t_lower, double p_upper,
                               double p_lower, int upper_faster, orc_ml_streams* out) {
-  const double c_upper = std::sqrt(kGammaAir * kRu / kMwAir * t_upper);
-  const double c_lower = std::sqrt(kGammaAir * kRu / kMwAir * t_lower);
+  const double c_upper = sqrt(kGammaAir * kRu / kMwAir * t_upper);
+  const double c_lower = sqrt(kGammaAir * kRu / kMwAir * t_lower);
   const double rho_upper = p_upper * kMwAir / (kRu * t_upper);
   const double rho_lower = p_lower * kMwAir / (kRu * t_lower);
   double ma_lower;
@@ -1818,6 +1852,7 @@ int orc_residual(void* ctx, double res[4]) {
   return 0;
 }
 
+// point-function entry points of the tests (double in, double out)
 double orc_weno5z(const double* vp, const double* vm, double eps) {
   real a[5], b[5];
   std::copy(vp, vp + 5, a);
@@ -1847,7 +1882,8 @@ int orc_riemann(int scheme, const double* ql, const double* qr, const double* kl
     pv[15 + l] = qr[l];
   }
   reconstruct_air(pv, 1, 1, pl, pr);
-  riemann_air(scheme, pl, pr, kl, kr, jl, jr, f);
+  const geo ml[3] = {kl[0], kl[1], kl[2]}, mr[3] = {kr[0], kr[1], kr[2]};
+  riemann_air(scheme, pl, pr, ml, mr, jl, jr, f);
   for (int l = 0; l < 5; ++l) fc[l] = value_of(f[l]);
   return 0;
 }
@@ -1860,7 +1896,8 @@ int orc_char_matrices(const double* ql, const double* qr, const double* kl, cons
     b[l] = qr[l];
   }
   CharAir M;
-  char_air(a, b, kl, kr, jl, jr, M);
+  const geo ml[3] = {kl[0], kl[1], kl[2]}, mr[3] = {kr[0], kr[1], kr[2]};
+  char_air(a, b, ml, mr, jl, jr, M);
   for (int r = 0; r < 5; ++r)
     for (int q = 0; q < 5; ++q) {
       L[r * 5 + q] = value_of(M.L[r][q]);
@@ -1876,19 +1913,34 @@ int orc_roe(const double* ql, const double* qr, const double* kl, const double* 
     a[l] = ql[l];
     b[l] = qr[l];
   }
-  roe_air(a, b, kl, kr, jl, jr, dl, dr, out);
+  const geo ml[3] = {kl[0], kl[1], kl[2]}, mr[3] = {kr[0], kr[1], kr[2]};
+  roe_air(a, b, ml, mr, jl, jr, dl, dr, out);
   for (int l = 0; l < 5; ++l) f[l] = value_of(out[l]);
   return 0;
 }
 
 void orc_lf_split(const double* q, int n_var, const double* k3, double jac, double* fp, double* fm) {
   std::vector<real> qq(q, q + n_var + 2), p(n_var), m(n_var);
-  lf_split(qq.data(), n_var, k3, jac, p.data(), m.data());
+  const geo k[3] = {k3[0], k3[1], k3[2]};
+  lf_split(qq.data(), n_var, k, jac, p.data(), m.data());
   for (int l = 0; l < n_var; ++l) {
     fp[l] = value_of(p[l]);
     fm[l] = value_of(m[l]);
   }
 }
+
+#ifdef ORC_COUNT
+// Operation counts since the last reset: additions and subtractions,
+// multiplications, divisions, square roots, transcendental calls.
+void orc_op_counts(unsigned long long out[5]) {
+  out[0] = orc::g_ops.add;
+  out[1] = orc::g_ops.mul;
+  out[2] = orc::g_ops.div;
+  out[3] = orc::g_ops.sqrt;
+  out[4] = orc::g_ops.trans;
+}
+void orc_op_counts_reset(void) { orc::g_ops = orc::OpCounts{}; }
+#endif
 
 int orc_scalar_width(void) { return kWidth; }
 

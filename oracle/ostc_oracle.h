@@ -38,6 +38,7 @@ typedef struct {
   double v_ref;            /* v_inf of the reference state */
   int    n_threads;        /* OpenMP threads, <=0: library default */
   int    reconstruction;   /* schemes 1/3/4: 1 first order, 2 MUSCL, 3 NND2 (minmod) */
+  double entropy_fix_factor;  /* Roe flux (scheme 2); the reference's default 0.125 */
 } orc_setup;
 
 /* Mixing-layer stream states, as produced by get_mixing_layer_info for air
