@@ -69,7 +69,7 @@ typedef struct {
   int n_var;            /* 5 */
   int inviscid_scheme;  /* 51 WENO-Z5 cp, 71 WENO-Z7 cp, 50 WENO5-JS cp (no reference counterpart);
                            OSTC_MODE_PLAIN only: 52 / 72 WENO-Z5 / WENO-Z7 characteristic (src/WENO.cu:1147-1453),
-                           1 LF, 2 Roe (entropy fix 0.125), 3 AUSM+, 4 HLLC on reconstructed primitives
+                           1 LF, 2 Roe (entropy fix below), 3 AUSM+, 4 HLLC on reconstructed primitives
                            (src/InviscidScheme.cu:12-215, src/RiemannSolver.cuh:279-417) */
   int viscous_order;    /* 0 inviscid, 2 second-order central */
   int fixed_time_step;  /* 1: use dt below, 0: CFL-based global dt */
@@ -80,6 +80,8 @@ typedef struct {
   double v_ref;         /* v_inf of the reference state */
   int reconstruction;   /* schemes 1/3/4: 1 first order, 2 MUSCL, 3 NND2, minmod limiter
                            (src/Reconstruction.cu:5-46, src/Limiter.cuh:7-41) */
+  double entropy_fix_factor;  /* entropy_fix_factor of the Roe flux (scheme 2), >= 0; the reference's
+                                 default is 0.125 (src/Parameter.cpp:474, src/DParameter.cu:16) */
 } ostc_setup;
 
 typedef struct {
@@ -347,7 +349,7 @@ typedef struct ostc_fr_ctx_s* ostc_fr_ctx;
 
 typedef struct {
   int inviscid_scheme;       /* 51 / 71 WENO-Z5 / WENO-Z7 component-wise, 52 / 72 characteristic,
-                                2 Roe (entropy fix 0.125), 3 AUSM+, 4 HLLC on reconstructed primitives
+                                2 Roe (entropy fix below), 3 AUSM+, 4 HLLC on reconstructed primitives
                                 (inviscid_type 0) */
   int viscous_order;         /* 0 or 2 */
   int reaction;              /* 1: finite-rate chemistry source */
@@ -359,6 +361,13 @@ typedef struct {
   double dt, cfl;
   double prandtl;            /* prandtl_number (viscous spectral radius of the time step) */
   double rho_ref, v_ref;     /* reference state: weno_eps_scale = (rho_ref v_ref)^2 */
+  double entropy_fix_factor; /* Roe flux (scheme 2), >= 0; reference default 0.125 (src/Parameter.cpp:474) */
+  int ch_stencil;            /* scheme 52, points of F+/F- that the five acoustic/entropy fields project:
+                                0: i-2..i+2 / i-1..i+3 for every field (the stencil of the species fields,
+                                of the air specialisation and of WENO-Z7); 1: the mixture code of the
+                                reference, i-3..i+1 / i-2..i+2 for the five conservative components
+                                (src/WENO.cu:880-881) -- at the first face of a block, whose point i-3 lies
+                                beyond the ghost layers, the outermost ghost point stands in */
 } ostc_fr_setup;
 
 #define OSTC_FR_FIELD_CV 0

@@ -47,7 +47,7 @@ class _Setup(C.Structure):
     _fields_ = [("n_var", C.c_int), ("inviscid_scheme", C.c_int), ("viscous_order", C.c_int),
                 ("fixed_time_step", C.c_int), ("dt", C.c_double), ("cfl", C.c_double),
                 ("prandtl", C.c_double), ("rho_ref", C.c_double), ("v_ref", C.c_double),
-                ("reconstruction", C.c_int)]
+                ("reconstruction", C.c_int), ("entropy_fix_factor", C.c_double)]
 
 
 class Farfield(C.Structure):
@@ -260,7 +260,8 @@ class Solver:
         self.case = case
         s = _Setup(case["n_var"], case["inviscid_scheme"], case["viscous_order"],
                    int(case.get("fixed_time_step", 0)), float(case.get("dt", 0.0)), case["cfl"],
-                   case["prandtl"], case["rho_ref"], case["v_ref"], int(case.get("reconstruction", 2)))
+                   case["prandtl"], case["rho_ref"], case["v_ref"], int(case.get("reconstruction", 2)),
+                   float(case.get("entropy_fix_factor", 0.125)))
         ctx = C.c_void_p()
         r = L.ostc_create(C.byref(s), stream, C.byref(ctx))
         if r != 0:

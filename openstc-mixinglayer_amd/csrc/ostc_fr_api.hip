@@ -122,6 +122,8 @@ FrArgs args(ostc_fr_ctx c, int b) {
   a.prandtl = c->s.prandtl;
   a.weno_eps_scale = c->weno_eps_scale;
   a.v_ref = c->s.v_ref;
+  a.entropy_fix = c->s.entropy_fix_factor;
+  a.ch_ref = c->s.ch_stencil;
   a.dt = c->d_dt;
   return a;
 }
@@ -279,6 +281,7 @@ int ostc_fr_create(const ostc_fr_setup* s, const ostc_mech_tables* mech, void* s
   if (!supported_ns(mech->n_spec)) return OSTC_ERR_UNSUPPORTED;
   if (s->cfl <= 0 && !s->fixed_time_step) return OSTC_ERR_ARG;
   if (s->fixed_time_step && !(s->dt > 0)) return OSTC_ERR_ARG;
+  if (!(s->entropy_fix_factor >= 0) || (s->ch_stencil != 0 && s->ch_stencil != 1)) return OSTC_ERR_ARG;
   auto* c = new ostc_fr_ctx_s();
   c->s = *s;
   c->tab = *mech;

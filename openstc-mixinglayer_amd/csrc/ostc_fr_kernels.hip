@@ -547,9 +547,10 @@ __device__ __forceinline__ void pp_limit(double* fc, const double* f1, const dou
 
 // Point-local Lax-Friedrichs split fluxes F+ and F- (compute_flux,
 // src/WENO.cu:697-720) of the TW columns x P points of a sweep tile, cells
-// t0-G .. t0+P-G-1 along DIR, into LDS planes [NV][P * TW]; points outside
-// the padded block hold zeros.
-template <int DIR, int NS, int G, int TW, int P>
+// t0-G-LEFT .. t0+P-G-LEFT-1 along DIR, into LDS planes [NV][P * TW]; points
+// past the end of the padded block hold zeros, and a point before its start
+// (t < -G, only with LEFT > 0) holds the fluxes of the outermost ghost point.
+template <int DIR, int NS, int G, int TW, int P, int LEFT = 0>
 __device__ __forceinline__ void split_fluxes(const FrArgs& A, double* Fp, double* Fm, int col0, int third, int t0) {
   constexpr int NV = 5 + NS, NP = P * TW;
   const FrBlockDev& B = A.B;
@@ -567,9 +568,9 @@ __device__ __forceinline__ void split_fluxes(const FrArgs& A, double* Fp, double
       col = e % TW;
       sp = e / TW;
     }
-    const int t = t0 - G + sp;
+    const int t = t0 - G - LEFT + sp;
     int idx[3];
-    tile_idx<DIR>(col0, t, col, third, idx);
+    tile_idx<DIR>(col0, t < -G ? -G : t, col, third, idx);
     const int slot = col * P + sp;
     if (idx[across] < ext[across] && t < m + G) {
       const long long p = pid(B, idx[0], idx[1], idx[2]);
@@ -733,8 +734,12 @@ template <> struct TileCh<2> { static constexpr int TW = 8, F = 16; };
 template <int DIR, int NS, int SCHEME>
 __global__ __launch_bounds__(kT) void k_inviscid_ch(FrArgs A) {
   constexpr int G = SCHEME == 72 ? 4 : 3, W = 2 * G - 1;
+  // scheme 52 keeps one more point on the left for the reference's stencil
+  // of the acoustic/entropy fields (A.ch_ref: F+ at i-3..i+1, F- at i-2..i+2
+  // for the five conservative components, src/WENO.cu:880-881)
+  constexpr int LEFT = SCHEME == 52 ? 1 : 0;
   constexpr int NV = 5 + NS, SH = NS / 2;  // species fields < SH belong to half 0
-  constexpr int TW = TileCh<DIR>::TW, F = TileCh<DIR>::F, TS = F - 1, P = TS + 2 * G, NP = P * TW;
+  constexpr int TW = TileCh<DIR>::TW, F = TileCh<DIR>::F, TS = F - 1, P = TS + 2 * G + LEFT, NP = P * TW;
   constexpr int NF = F * TW;  // faces of the tile (128)
   extern __shared__ double lds[];
   double* Fp = lds;                // [NV][NP]
@@ -750,8 +755,9 @@ __global__ __launch_bounds__(kT) void k_inviscid_ch(FrArgs A) {
   const long long s = B.sz;
   const int tid = threadIdx.x;
 
-  split_fluxes<DIR, NS, G, TW, P>(A, Fp, Fm, col0, third, t0);
+  split_fluxes<DIR, NS, G, TW, P, LEFT>(A, Fp, Fm, col0, third, t0);
   __syncthreads();
+  const int sh = SCHEME == 52 ? A.ch_ref : 0;  // shift of the conservative components' stencil
 
   const int half = tid / NF, q = tid % NF;  // wave-uniform half
   const int f = DIR == 0 ? q % F : q / TW, col = DIR == 0 ? q / F : q % TW;
@@ -763,7 +769,7 @@ __global__ __launch_bounds__(kT) void k_inviscid_ch(FrArgs A) {
   CharFace<NS> C;
   long long pl = 0, pr = 0;
   double jac1 = 0, jac2 = 0, f1[NS];
-  const int sl = col * P + (f + G - 1);  // LDS slot of the left cell
+  const int sl = col * P + (f + G - 1 + LEFT);  // LDS slot of the left cell
   if (active) {
     pl = pid(B, idx[0], idx[1], idx[2]);
     int idr[3] = {idx[0], idx[1], idx[2]};
@@ -814,8 +820,8 @@ __global__ __launch_bounds__(kT) void k_inviscid_ch(FrArgs A) {
         double ap = 0, am = 0;
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
-          ap += L[n] * Fp[n * NP + sl - W / 2 + r];
-          am += L[n] * Fm[n * NP + sl - W / 2 + 1 + r];
+          ap += L[n] * Fp[n * NP + sl - W / 2 + r - sh];
+          am += L[n] * Fm[n * NP + sl - W / 2 + 1 + r - sh];
         }
 #pragma unroll
         for (int n = 0; n < NS; ++n) {
@@ -999,7 +1005,7 @@ __global__ __launch_bounds__(kT) void k_inviscid_pv(FrArgs A) {
       for (int q = 0; q < 2; ++q) {
 #pragma unroll
         for (int c = 0; c < 9; ++c) mm[c] = B.met[pts[q] + c * s];
-        delta[q] = entropy_delta_fr(B.bv[pts[q] + s], B.bv[pts[q] + 2 * s], B.bv[pts[q] + 3 * s], B.cc[pts[q]], mm);
+        delta[q] = entropy_delta_fr(B.bv[pts[q] + s], B.bv[pts[q] + 2 * s], B.bv[pts[q] + 3 * s], B.cc[pts[q]], mm, A.entropy_fix);
       }
       roe_fr<NS>(A.T, pl, pr, ml, mr, B.jac[p0], B.jac[p1], delta[0], delta[1], fc);
     } else if (A.scheme == 3)
@@ -1216,7 +1222,7 @@ void launch_rhs_t(const FrArgs& A, hipStream_t st, long long* n, KernelTimer* tm
     rec(KT_INV_X + D, false);
     if (A.scheme == 52 || A.scheme == 72) {
       constexpr int TWc = TileCh<D>::TW, Fc = TileCh<D>::F, TSc = Fc - 1;
-      const int Pc = TSc + 2 * (A.scheme == 72 ? 4 : 3);
+      const int Pc = TSc + 2 * (A.scheme == 72 ? 4 : 3) + (A.scheme == 52 ? 1 : 0);
       dim3 gridc((ext[D] + TSc - 1) / TSc, (ext[across] + TWc - 1) / TWc, ext[third]);
       const size_t ldsc = ((size_t)2 * (5 + NS) * Pc * TWc + (size_t)(5 + NS) * Fc * TWc) * sizeof(double);
       if (A.scheme == 52)

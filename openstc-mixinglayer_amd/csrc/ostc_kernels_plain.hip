@@ -165,7 +165,7 @@ __global__ __launch_bounds__(64) void k_pv_pencil(DevBlock B, Params P, int slot
         double mm[9], qq[5];
         for (int cc = 0; cc < 9; ++cc) mm[cc] = B.met[p + cc * s];
         for (int l = 0; l < 5; ++l) qq[l] = sPv[l][tid + 1 + q];
-        dlr[q] = entropy_delta_air(qq, mm);
+        dlr[q] = entropy_delta_air(qq, mm, P.entropy_fix);
       }
       roe_air(pl, pr, ml, mr, sJ[tid + 1], sJ[tid + 2], dlr[0], dlr[1], f);
     } else {

@@ -60,7 +60,8 @@ class FrSetup(C.Structure):
     _fields_ = [("inviscid_scheme", C.c_int), ("viscous_order", C.c_int), ("reaction", C.c_int),
                 ("positive_preserving", C.c_int), ("grad_p_diffusion", C.c_int), ("fixed_time_step", C.c_int),
                 ("reconstruction", C.c_int), ("dt", C.c_double), ("cfl", C.c_double), ("prandtl", C.c_double),
-                ("rho_ref", C.c_double), ("v_ref", C.c_double)]
+                ("rho_ref", C.c_double), ("v_ref", C.c_double), ("entropy_fix_factor", C.c_double),
+                ("ch_stencil", C.c_int)]
 
 
 class _BlockDesc(C.Structure):
@@ -171,7 +172,8 @@ class FrSolver:
                     int(case.get("positive_preserving", 0)), int(case.get("grad_p_diffusion", 0)),
                     int(case.get("fixed_time_step", 0)), int(case.get("reconstruction", 2)),
                     float(case.get("dt", 0.0)), case["cfl"], case["prandtl"],
-                    case["rho_ref"], case["v_ref"])
+                    case["rho_ref"], case["v_ref"], float(case.get("entropy_fix_factor", 0.125)),
+                    int(case.get("ch_stencil", 0)))
         ctx = C.c_void_p()
         r = L.ostc_fr_create(C.byref(s), C.byref(self.T), stream, C.byref(ctx))
         if r != 0:

@@ -28,7 +28,8 @@ class _Setup(C.Structure):
     _fields_ = [("inviscid_scheme", C.c_int), ("viscous_order", C.c_int), ("reaction", C.c_int),
                 ("positive_preserving", C.c_int), ("grad_p_diffusion", C.c_int), ("fixed_time_step", C.c_int),
                 ("reconstruction", C.c_int), ("dt", C.c_double), ("cfl", C.c_double), ("prandtl", C.c_double),
-                ("rho_ref", C.c_double), ("v_ref", C.c_double), ("n_threads", C.c_int)]
+                ("rho_ref", C.c_double), ("v_ref", C.c_double), ("n_threads", C.c_int),
+                ("entropy_fix_factor", C.c_double), ("ch_stencil", C.c_int)]
 
 
 _lib = None
@@ -211,7 +212,8 @@ class FrOracle:
                    int(case.get("positive_preserving", 0)), int(case.get("grad_p_diffusion", 0)),
                    int(case.get("fixed_time_step", 0)), int(case.get("reconstruction", 2)),
                    float(case.get("dt", 0.0)), case["cfl"],
-                   case["prandtl"], case["rho_ref"], case["v_ref"], n_threads)
+                   case["prandtl"], case["rho_ref"], case["v_ref"], n_threads,
+                   float(case.get("entropy_fix_factor", 0.125)), int(case.get("ch_stencil", 0)))
         self.ctx = L.orf_create(C.byref(s), C.byref(self.T))
         if not self.ctx:
             raise ValueError("unsupported setup")

@@ -15,12 +15,14 @@
 // telescoping conservation of the whole mixture RHS, and an independent
 // numpy restatement of the chemistry source and transport.
 //
-// One deliberate difference from the reference: the characteristic WENO-Z5
-// (scheme 52) of the five acoustic/entropy fields uses the stencil
-// i-2..i+2 / i-1..i+3 of the species fields and of the air specialisation;
-// the mixture version in the reference reads i-3..i+1 / i-2..i+2 for those
-// five fields (src/WENO.cu:861-873), which for the first face of each pencil
-// is outside its shared-memory arrays.
+// The characteristic WENO-Z5 (scheme 52) of the five acoustic/entropy fields
+// has two stencils (setup key ch_stencil).  0: i-2..i+2 / i-1..i+3, the
+// stencil of the species fields, of the air specialisation and of WENO-Z7.
+// 1: the mixture code of the reference, which reads i-3..i+1 / i-2..i+2 for
+// the five conservative components of those fields (src/WENO.cu:880-881); at
+// the first face of every pencil the reference's point i-3 lies outside its
+// shared-memory arrays, and at the first face of a block it lies beyond the
+// ghost layers: there the outermost ghost point stands in for it.
 #include "ostc_oracle_fr.h"
 #include "ostc_oracle_faces.h"
 #include "ostc_oracle_inflow.h"
@@ -977,16 +979,17 @@ void hllc(const Tab& T, int nv, const double* pl, const double* pr, const double
 }
 
 // compute_entropy_fix_delta (mixture, 3D), src/InviscidScheme.cu:420-453:
-// the factor 0.125 times (|U| + |V| + |W| + c (|grad xi| + |grad eta| +
-// |grad zeta|) / 3) at one point, m the 9 metric entries.
-double entropy_delta(double u, double v, double w, double c, const double* m) {
+// entropy_fix_factor (src/Parameter.cpp:474, default 0.125) times (|U| + |V| +
+// |W| + c (|grad xi| + |grad eta| + |grad zeta|) / 3) at one point, m the 9
+// metric entries.
+double entropy_delta(double u, double v, double w, double c, const double* m, double factor) {
   const double U = fabs(u * m[0] + v * m[1] + w * m[2]);
   const double V = fabs(u * m[3] + v * m[4] + w * m[5]);
   const double W = fabs(u * m[6] + v * m[7] + w * m[8]);
   const double kx = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
   const double ky = sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
   const double kz = sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
-  return 0.125 * (U + V + W + c * (kx + ky + kz) / 3.0);
+  return factor * (U + V + W + c * (kx + ky + kz) / 3.0);
 }
 
 // riemannSolver_Roe (mixture), src/RiemannSolver.cuh:234-417:
@@ -1114,7 +1117,7 @@ void inviscid_pv(const Ctx& c, Block& b) {
             if (c.s.inviscid_scheme == 2) {
               double mm[9];
               for (int cc = 0; cc < 9; ++cc) mm[cc] = b.metric[p + cc * s];
-              D[ps] = entropy_delta(b.bv[p + s], b.bv[p + 2 * s], b.bv[p + 3 * s], b.cc[p], mm);
+              D[ps] = entropy_delta(b.bv[p + s], b.bv[p + 2 * s], b.bv[p + 3 * s], b.cc[p], mm, c.s.entropy_fix_factor);
             }
           }
           for (int t = -1; t < m; ++t) {
@@ -1231,14 +1234,21 @@ void inviscid(const Ctx& c, Block& b, double dt) {
                   fChar[5 + l] = weno7z(vPlus, vMinus, eps_scaled);
                 }
               } else {
+              // the five conservative components of the acoustic/entropy fields
+              // read F+ at i-2..i+2 and F- at i-1..i+3, or with ch_stencil 1 the
+              // reference's i-3..i+1 and i-2..i+2 (src/WENO.cu:880-881); the
+              // point i-3 of the block's first face lies beyond the ghost
+              // layers, and the outermost ghost point stands in for it
+              const int sh = c.s.ch_stencil ? 1 : 0;
               for (int l = 0; l < 5; ++l) {
                 double L[5];
                 const double cas = char_row(F, l, L);
                 double vPlus[5] = {}, vMinus[5] = {};
                 for (int r = 0; r < 5; ++r) {
+                  const int ip = std::max(is - 2 - sh + r, 0), im = is - 1 - sh + r;
                   for (int n = 0; n < 5; ++n) {
-                    vPlus[r] += L[n] * Fp[(is - 2 + r) * nv + n];
-                    vMinus[r] += L[n] * Fm[(is - 1 + r) * nv + n];
+                    vPlus[r] += L[n] * Fp[ip * nv + n];
+                    vMinus[r] += L[n] * Fm[im * nv + n];
                   }
                   for (int n = 0; n < ns; ++n) {
                     vPlus[r] += cas * F.alpha_l[n] * Fp[(is - 2 + r) * nv + 5 + n];

@@ -37,6 +37,10 @@ typedef struct {
   double prandtl;              /* prandtl_number (viscous spectral radius of the time step) */
   double rho_ref, v_ref;       /* weno_eps_scale = (rho_ref v_ref)^2 */
   int    n_threads;
+  double entropy_fix_factor;   /* Roe flux (scheme 2); the reference's default 0.125 */
+  int    ch_stencil;           /* scheme 52: 0 the i-2..i+2 / i-1..i+3 stencil for every field, 1 the
+                                  reference's i-3..i+1 / i-2..i+2 for the five conservative components of
+                                  the acoustic/entropy fields (src/WENO.cu:880-881) */
 } orf_setup;
 
 /* arrays */

@@ -76,11 +76,37 @@ def test_steps(scheme, recon, kind):
     s.close()
 
 
-@pytest.mark.parametrize("scheme", [52, 72, 3])
+@pytest.mark.parametrize("factor", [0.0, 0.6])
+def test_roe_entropy_fix_factor(factor):
+    """entropy_fix_factor of the setup (src/Parameter.cpp:474) reaches the Roe
+    kernels: dq at a non-default factor matches the oracle at that factor and
+    differs from dq at the default 0.125 on the rough case."""
+    print(f"[gpu] air Roe dq, entropy_fix_factor {factor}", flush=True)
+    out = {}
+    for f in (factor, 0.125):
+        case = _case("rough", 2, 2)
+        case["entropy_fix_factor"] = f
+        o = oracle_for(case)
+        s = pkg.Solver(case)
+        s.setup_mixing_layer()
+        s.set(0, pkg.FIELD_CV, o.array(0, O.CV))
+        s.set(0, pkg.FIELD_BV, o.array(0, O.BV))
+        s.set(0, pkg.FIELD_MUL, o.array(0, O.MUL))
+        o.rhs(0)
+        s.compute_rhs(0)
+        out[f] = s.get(0, pkg.FIELD_DQ)
+        e = rel_err(out[f], o.array(0, O.DQ))
+        print(f"  factor {f}: dq rel err {e}", flush=True)
+        assert np.all(e <= DQ_TOL), e
+        s.close()
+    assert np.max(np.abs(out[factor] - out[0.125])) > 0
+
+
+@pytest.mark.parametrize("scheme", [52, 72, 1, 2, 3, 4])
 def test_fused_paths_refuse_these_schemes(scheme):
     """The batched kernel implements schemes 50/51/71 only; the others are
     refused with an error instead of running another scheme."""
-    case = cases.single_block_case(16, scheme=scheme)
+    case = cases.single_block_case(16, scheme=scheme, reconstruction=2)
     s = pkg.Solver(case)
     s.setup_mixing_layer()
     s.initialize()

@@ -300,3 +300,48 @@ def test_residual_matches_oracle(tables):
     np.testing.assert_allclose(got, want, rtol=1e-12)
     assert np.all(want > 0)
     s.close()
+
+
+@pytest.mark.parametrize("pp", [0, 1])
+def test_stage_residual_reference_stencil(tables, pp):
+    """Scheme 52 with ch_stencil 1: the reference's stencil of the five
+    acoustic/entropy fields (src/WENO.cu:880-881) on the GPU and in the oracle."""
+    print(f"[gpu] fr dq scheme 52, reference stencil, pp {pp}", flush=True)
+    case = _case(tables, scheme=52, positive_preserving=pp)
+    case["ch_stencil"] = 1
+    o, s = _pair(case, tables)
+    dt = 2e-9
+    dq_o = o.rhs(0, dt).copy()
+    s.compute_rhs(0, dt)
+    err = rel_err(s.get(0, MX.FIELD_DQ), dq_o)
+    print("  per-variable:", " ".join(f"{e:.1e}" for e in err), flush=True)
+    assert np.all(err[:5] <= 1e-12)
+    assert np.all(err[5:] <= 1e-10)
+    s.close()
+
+
+def test_steps_reference_stencil(tables):
+    print("[gpu] fr 10 graph steps, scheme 52 with the reference stencil", flush=True)
+    case = _case(tables, scheme=52, positive_preserving=1)
+    case["ch_stencil"] = 1
+    o, s = _pair(case, tables)
+    dt_o = o.step(10)
+    dt_g = s.rk3(10, MX.MODE_GRAPH)
+    assert abs(dt_g - dt_o) <= 1e-12 * dt_o
+    _compare_state(o, s, 1, 1e-9, "reference stencil, 10 steps")
+    s.close()
+
+
+def test_roe_entropy_fix_factor(tables):
+    """entropy_fix_factor reaches the mixture Roe kernel (MUSCL, factor 0.6)."""
+    print("[gpu] fr Roe dq, entropy_fix_factor 0.6", flush=True)
+    case = _case(tables, scheme=2)
+    case["entropy_fix_factor"] = 0.6
+    o, s = _pair(case, tables)
+    dq_o = o.rhs(0, 0.0).copy()
+    s.compute_rhs(0, 0.0)
+    err = rel_err(s.get(0, MX.FIELD_DQ), dq_o)
+    print("  per-variable:", " ".join(f"{e:.1e}" for e in err), flush=True)
+    assert np.all(err[:5] <= 1e-12)
+    assert np.all(err[5:] <= 1e-10)
+    s.close()
