@@ -1,7 +1,8 @@
 """Benchmark: FP64 RHS + SSP-RK3 Mcell-updates/s on one MI355X.
 
 Workload: the batched multi-block air mixing layer -- 24 blocks of uneven
-sizes (4 x 3 x 2 tensor-product cuts drawn from seed 42), about 12 M cells,
+sizes (4 x 3 x 2 tensor-product cuts of 16..192 x 16..128 x 16..64 cells),
+about 12.5 M cells,
 WENO-Z5 component-wise + Lax-Friedrichs splitting, second-order viscous
 fluxes, SSP-RK3 at CFL 0.5, in-GPU inner-face exchange.  The BATCHED path
 (fused RHS+update kernel over the block work-list, graph-captured steps) is
@@ -37,13 +38,34 @@ FP64_PEAK_TFLOPS = 78.6
 BYTES_PER_CELL_STAGE = 8 * (5 + 5 + 10) + 8 * (5 + 6 + 2)
 
 
+def cpu_threads():
+    """OpenMP threads of the CPU baseline: OMP_NUM_THREADS when the host sets
+    it (the share of host cores that goes with one GPU), else every core."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
+def cpu_model():
+    """The host CPU's model name and logical core count (lscpu)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cores": os.cpu_count()}
+
+
 def cpu_baseline(case, steps: int):
     """The CPU oracle (host C++, OpenMP over cells) on a bounded sample of the
     same workload: `steps` RK3 steps of the full 24-block case."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     o = O.Oracle(case, n_threads=threads)
     o.setup_mixing_layer()
     o.initialize()
@@ -52,6 +74,7 @@ def cpu_baseline(case, steps: int):
     dt = time.perf_counter() - t0
     cells = case["n_cells"]
     return {"value": cells * steps / dt / 1e6, "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
             "sample": f"{steps} RK3 steps of the full {len(case['blocks'])}-block case ({cells} cells), "
                       f"{dt:.1f} s on {threads} OpenMP threads"}
 
@@ -155,7 +178,7 @@ def reacting_cpu_baseline(tables, n, steps, spatial=False):
     import mech
     import oracle_fr as OF
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     to = mech.load(*pkg.mixture.H2_AIR_FILES)
     case = pkg.cases.reacting_single_block(n, n // 2, n // 2, tables, scheme=52, spatial=spatial)
     o = OF.FrOracle(case, to, n_threads=threads)
@@ -166,6 +189,7 @@ def reacting_cpu_baseline(tables, n, steps, spatial=False):
     dt = time.perf_counter() - t0
     cells = case["n_cells"]
     return {"value": cells * steps / dt / 1e6, "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
             "sample": f"{steps} RK3 steps of a {n}x{n // 2}x{n // 2} block ({cells} cells) of the same reacting "
                       f"layer, {dt:.1f} s on {threads} OpenMP threads"}
 
@@ -186,6 +210,54 @@ def measured_traffic(tile, split: int, waves: int):
                 fields = dict(kv.split("=") for kv in line.split()[1:])
                 return int(fields["bytes"])
     return None
+
+
+def flop_counts(path=os.path.join(ROOT, "profiles", "flops_per_cell_air.txt")):
+    """FP64 operations per interior cell counted by the oracle's counting
+    build (tests/golden/make_flop_counts.py): {name: flops} of each line."""
+    out = {}
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#") or not line.strip():
+                continue
+            name, *fields = line.split()
+            out[name] = float(dict(kv.split("=") for kv in fields)["flops"])
+    return out
+
+
+def fused_roofline(cells, avg_kernel_ms, rate_mcells, tile_index, tile_split, tile_waves):
+    """Roofline of the fused RHS+update kernel (one RK stage over all blocks)
+    and of the whole step.  The kernel is measured against both roofs: HBM
+    with the contract-minimal 264 B per cell and stage, FP64 with the
+    oracle-counted flops per cell and stage; `bound` is the roof that binds
+    (the lower attainable rate), `achieved`/`peak`/`frac` are on that roof.
+    step: rate / min(8e12 / B_step, P_FP64 / F_step) in cell-updates/s."""
+    f = flop_counts()
+    avg_s = avg_kernel_ms / 1e3
+    bytes_per_launch = BYTES_PER_CELL_STAGE * cells
+    flops_per_launch = f["kernel_stage_per_cell"] * cells
+    gbs = bytes_per_launch / avg_s / 1e9
+    tfs = flops_per_launch / avg_s / 1e12
+    # attainable launches/s on each roof; the smaller one binds
+    hbm_bound = HBM_PEAK_GBS * 1e9 / bytes_per_launch < FP64_PEAK_TFLOPS * 1e12 / flops_per_launch
+    b_step = 3 * BYTES_PER_CELL_STAGE + 8 * 5
+    step_roof = min(HBM_PEAK_GBS * 1e9 / b_step, FP64_PEAK_TFLOPS * 1e12 / f["step_per_cell"]) / 1e6
+    traffic = measured_traffic(pkg.tile_shapes()[tile_index], tile_split, tile_waves)
+    hbm = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+    fp64 = {"achieved": tfs, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tfs / FP64_PEAK_TFLOPS}
+    main = hbm if hbm_bound else fp64
+    return {"bound": "hbm" if hbm_bound else "fp64", **main, "traffic": traffic,
+            "traffic_source": "profiles/fused_rhs_update_batched24_pmc.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                              "copy-calibrated, bytes per launch)" if traffic else None,
+            "kernel": "k_fused (RHS+update, one RK stage over all blocks)",
+            "hbm": hbm, "fp64": fp64,
+            "algorithmic_bytes_per_launch": bytes_per_launch, "algorithmic_flops_per_launch": flops_per_launch,
+            "flops_source": "profiles/flops_per_cell_air.txt (oracle on a counting scalar; fp64 vector peak)",
+            "avg_launch_ms": avg_kernel_ms,
+            "step": {"bytes_per_cell": b_step, "flops_per_cell": f["step_per_cell"],
+                     "roof_mcell_updates_per_s": step_roof, "frac": rate_mcells / step_roof,
+                     "hbm_frac": rate_mcells * 1e6 * b_step / (HBM_PEAK_GBS * 1e9),
+                     "fp64_frac": rate_mcells * 1e6 * f["step_per_cell"] / (FP64_PEAK_TFLOPS * 1e12)}}
 
 
 def copy_probe(mib: int):
@@ -240,7 +312,7 @@ def main():
         print(json.dumps({"reacting": {"single_block_512x256x128": r}}))
         return
 
-    case = pkg.cases.batched_case(seed=42)
+    case = pkg.cases.batched_case()
     cells = case["n_cells"]
     print(f"[bench] {len(case['blocks'])} blocks, {cells} interior cells", file=sys.stderr, flush=True)
     s = pkg.Solver(case)
@@ -291,16 +363,7 @@ def main():
     print(f"[bench] plain: {wall_p * 1e3 / a.plain_steps:.3f} ms/step, "
           f"{launches_p // max(a.plain_steps, 1)} launches/step", file=sys.stderr, flush=True)
 
-    avg_kernel_s = avg_kernel_ms / 1e3
-    bytes_per_launch = BYTES_PER_CELL_STAGE * cells
-    achieved = bytes_per_launch / avg_kernel_s / 1e9
-    traffic = measured_traffic(pkg.tile_shapes()[tile_index], tile_split, tile_waves)
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "traffic_source": "profiles/fused_rhs_update_batched24_pmc.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                                  "copy-calibrated, bytes per launch)" if traffic else None,
-                "kernel": "k_fused (RHS+update, one RK stage over all blocks)",
-                "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_kernel_s * 1e3}
+    roofline = fused_roofline(cells, avg_kernel_ms, value, tile_index, tile_split, tile_waves)
     s.close()
 
     adjoint = None
@@ -330,7 +393,8 @@ def main():
         "metric": METRIC, "value": value, "unit": "Mcell-updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "batched multi-block air temporal mixing layer, 24 uneven blocks (seed 42), "
+        "config": {"workload": f"batched multi-block air temporal mixing layer, 24 uneven blocks (cuts of 16..192 x "
+                               f"16..128 x 16..64 cells, seed {pkg.cases.batched_seed()}), "
                                "WENO-Z5 cp + LF, 2nd-order viscous, SSP-RK3 CFL 0.5",
                    "cells": cells, "blocks": len(case["blocks"]), "mode": "BATCHED",
                    "tile": list(pkg.tile_shapes()[tile_index]), "workgroups_per_cu": tile_waves,

@@ -86,6 +86,36 @@ typedef struct {
 
 typedef struct {
   int mx, my, mz;       /* interior node counts (mz > 1: three-dimensional) */
+  /* Optional caller-owned device arrays of the block, the reference's DZone
+   * arrays (Field::setup_device_memory, src/Field.cu:469-647) in the layout
+   * above for the context's ghost depth.  NULL (the zero-initialised rest of
+   * the struct): the library allocates, zeroes and owns the array.  What the
+   * caller passes stays the caller's: the library never frees it, and it must
+   * stay valid until the context is destroyed.
+   *   cv   n_var planes: the block's conservative state (the state the
+   *        batched step reads and leaves its result in)
+   *   bv   6 planes: rho, u, v, w, p, T
+   *   sv   n_spec planes of mass fractions (mixture contexts; ignored for air)
+   *   qn   n_var planes: the state at the start of the step (per-block steps);
+   *        batched steps use the array as intermediate stage storage
+   *   dq   n_var planes, no ghosts
+   *   mul  viscosity (air contexts: the state the batched step leaves)
+   *   jac  Jacobian */
+  double* cv;
+  double* bv;
+  double* sv;
+  double* qn;
+  double* dq;
+  double* mul;
+  double* jac;
+  /* Optional device metric, read by finalize into the library's own SoA
+   * planes: component c (0..8, c = 3(r-1)+(c'-1)) of padded point p at
+   * metric[p*metric_elem_stride + c*metric_comp_stride + metric_offset]
+   * (doubles), so SoA planes (1, sz, 0) and the reference's 80-byte
+   * gxl::Matrix AoS (10, 1, 1; src/gxl_lib/Matrix.hpp:53-72) are read where
+   * they lie.  NULL: give the geometry with ostc_set_geometry. */
+  const double* metric;
+  long long metric_elem_stride, metric_comp_stride, metric_offset;
 } ostc_block_desc;
 
 /* Device view of one block, the DZone equivalent (src/Field.h:12-90). */

@@ -100,7 +100,26 @@ def farfield_state_air(**kw) -> Farfield:
 
 
 class _BlockDesc(C.Structure):
-    _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int)]
+    """ostc_block_desc (include/openstc/abi.h): extents and the optional
+    caller-owned device arrays (0: the library allocates)."""
+    _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int),
+                ("cv", C.c_void_p), ("bv", C.c_void_p), ("sv", C.c_void_p), ("qn", C.c_void_p),
+                ("dq", C.c_void_p), ("mul", C.c_void_p), ("jac", C.c_void_p), ("metric", C.c_void_p),
+                ("metric_elem_stride", C.c_longlong), ("metric_comp_stride", C.c_longlong),
+                ("metric_offset", C.c_longlong)]
+
+
+def block_desc(blk: dict, arrays: dict | None = None) -> _BlockDesc:
+    """The descriptor of a case block; ``arrays`` maps cv, bv, sv, qn, dq,
+    mul, jac and metric to device addresses the caller owns, and
+    metric_strides to (element, component, offset) of the metric."""
+    d = _BlockDesc(blk["mx"], blk["my"], blk["mz"])
+    for k, v in (arrays or {}).items():
+        if k == "metric_strides":
+            d.metric_elem_stride, d.metric_comp_stride, d.metric_offset = v
+        else:
+            setattr(d, k, v)
+    return d
 
 
 class Zone(C.Structure):
@@ -255,7 +274,9 @@ def padded_shape(blk: dict, g: int):
 class Solver:
     """A set of blocks on one GPU with the reference's layout and operators."""
 
-    def __init__(self, case: dict, stream: int | None = None):
+    def __init__(self, case: dict, stream: int | None = None, device_arrays=None):
+        """device_arrays: per block, a dict of caller-owned device arrays
+        (block_desc) that the library adopts instead of allocating its own."""
         L = lib()
         self.case = case
         s = _Setup(case["n_var"], case["inviscid_scheme"], case["viscous_order"],
@@ -270,8 +291,9 @@ class Solver:
         g = C.c_int()
         L.ostc_ngg(self.ctx, C.byref(g))
         self.ngg = g.value
-        for b in case["blocks"]:
-            self._chk(L.ostc_add_block(self.ctx, C.byref(_BlockDesc(b["mx"], b["my"], b["mz"])), None))
+        for i, b in enumerate(case["blocks"]):
+            arrays = device_arrays[i] if device_arrays else None
+            self._chk(L.ostc_add_block(self.ctx, C.byref(block_desc(b, arrays)), None))
         for f in case["boundaries"]:
             if f["type"] == 5:  # inflow: its state and the y of the block's nodes
                 blk = case["blocks"][f["block"]]

@@ -81,10 +81,15 @@ def write_plt(path: str, names, zones, solution_time: float = 0.0):
 
 
 class _Reader:
-    def __init__(self, data: bytes):
-        self.d, self.p = data, 0
+    def __init__(self, data: bytes, path: str = ""):
+        self.d, self.p, self.path = data, 0, path
+
+    def need(self, n: int):
+        if self.p + n > len(self.d):
+            raise ValueError(f"{self.path}: truncated at byte {len(self.d)} (reading {n} bytes at {self.p})")
 
     def take(self, fmt: str):
+        self.need(struct.calcsize(fmt))
         v = struct.unpack_from(fmt, self.d, self.p)
         self.p += struct.calcsize(fmt)
         return v if len(v) > 1 else v[0]
@@ -102,7 +107,7 @@ def read_plt(path: str):
     """{'names', 'time', 'zones': [{'shape': (K, J, I), name: array, ...}]}
     of a file in the layout above (ordered zones, double data)."""
     with open(path, "rb") as f:
-        r = _Reader(f.read())
+        r = _Reader(f.read(), path)
     if r.d[:8] != MAGIC:
         raise ValueError(f"{path}: not a #!TDV112 file")
     r.p = 8
@@ -133,6 +138,7 @@ def read_plt(path: str):
     for shape in dims:
         if r.take("<f") != ZONE_MARKER:
             raise ValueError(f"{path}: bad data-section zone marker")
+        r.need(16 * n_var)
         fmts = r.take(f"<{n_var}i") if n_var > 1 else (r.take("<i"),)
         if any(v != 2 for v in fmts):
             raise ValueError(f"{path}: only double data is read")
@@ -141,6 +147,7 @@ def read_plt(path: str):
         n = shape[0] * shape[1] * shape[2]
         zone = {"shape": shape}
         for nm in names:
+            r.need(8 * n)
             zone[nm] = np.frombuffer(r.d, dtype="<f8", count=n, offset=r.p).reshape(shape).copy()
             r.p += 8 * n
         zones.append(zone)

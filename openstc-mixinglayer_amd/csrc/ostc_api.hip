@@ -5,6 +5,8 @@
 #include "ostc_faces.hpp"
 #include "ostc_sponge.hpp"
 #include "ostc_kernels.hpp"
+#include "ostc_nanscan.hpp"
+#include "ostc_repack.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -58,6 +60,7 @@ struct HBlock {
   std::vector<FaceMap> inner;
   std::vector<double> sp_sigma;  // sponge sigma of every interior node (empty: none)
   int sp_jac = 1;
+  ostc_block_desc desc{};  // the caller's descriptor: optional device arrays it owns
   DevBlock d{};
 };
 
@@ -132,6 +135,7 @@ struct ostc_ctx_s {
   ostc_theta adj_theta{};
   hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr, ev_b0 = nullptr, ev_b1 = nullptr;
 };
+
 
 namespace {
 
@@ -458,6 +462,9 @@ bool fused_scheme(const ostc_ctx c) { return c->P.scheme == 51 || c->P.scheme ==
 
 int need_fused(ostc_ctx c) {
   if (c->sponge) return fail(c, OSTC_ERR_UNSUPPORTED, "the sponge layer runs with OSTC_MODE_PLAIN");
+  for (auto& B : c->blocks)  // the fused kernel indexes a block's planes with 32-bit offsets
+    if (9 * B.d.sz >= (1LL << 31))
+      return fail(c, OSTC_ERR_UNSUPPORTED, "block too large for the batched kernel (9 padded planes >= 2^31 doubles)");
   if (fused_scheme(c)) return OSTC_OK;
   return fail(c, OSTC_ERR_UNSUPPORTED,
               "inviscid_scheme 52/72/1/2/3/4 run with OSTC_MODE_PLAIN and the stage operators only");
@@ -668,6 +675,7 @@ int ostc_create(const ostc_setup* s, void* hip_stream, ostc_ctx* out) {
     return OSTC_ERR_UNSUPPORTED;
   if (s->viscous_order != 0 && s->viscous_order != 2) return OSTC_ERR_UNSUPPORTED;
   if (!(s->cfl > 0) || !(s->prandtl > 0)) return OSTC_ERR_ARG;
+  if (!(s->entropy_fix_factor >= 0)) return OSTC_ERR_ARG;
   auto* c = new ostc_ctx_s;
   c->s = *s;
   c->ngg = (sc == 71 || sc == 72) ? 4 : 3;  // src/Parameter.cpp:227-238
@@ -679,6 +687,7 @@ int ostc_create(const ostc_setup* s, void* hip_stream, ostc_ctx* out) {
   c->P.prandtl = s->prandtl;
   c->P.weno_eps_scale = s->rho_ref * s->v_ref * s->rho_ref * s->v_ref;  // src/DParameter.cu:26-29
   c->P.v_ref = s->v_ref;
+  c->P.entropy_fix = s->entropy_fix_factor;
   if (hip_stream) {
     c->st = static_cast<hipStream_t>(hip_stream);
   } else {
@@ -716,10 +725,13 @@ int ostc_add_block(ostc_ctx c, const ostc_block_desc* d, int* id) {
   if (c->finalized) return fail(c, OSTC_ERR_STATE, "blocks are fixed after ostc_finalize");
   if (d->mx < c->ngg + 2 || d->my < c->ngg + 2 || d->mz < c->ngg + 2)
     return fail(c, OSTC_ERR_UNSUPPORTED, "each extent must be at least ngg+2 (three-dimensional blocks only)");
+  if (d->metric && (d->metric_elem_stride < 1 || d->metric_comp_stride < 1 || d->metric_offset < 0))
+    return fail(c, OSTC_ERR_ARG, "device metric: strides must be positive and the offset non-negative");
   HBlock B;
   B.m[0] = d->mx;
   B.m[1] = d->my;
   B.m[2] = d->mz;
+  B.desc = *d;
   c->blocks.push_back(B);
   if (id) *id = (int)c->blocks.size() - 1;
   return OSTC_OK;
@@ -911,17 +923,23 @@ int ostc_finalize(ostc_ctx c) {
     d.sy = d.my + 2 * G;
     d.sz = (long long)d.sx * d.sy * (d.mz + 2 * G);
     d.n0 = (long long)d.mx * d.my * d.mz;
+    // the caller's arrays where it gave them (slot 0 is the state, qn is
+    // slot 1), the library's own otherwise
+    const ostc_block_desc& u = B.desc;
+    auto take = [&](double* mine, double** p, size_t n) { return mine ? (*p = mine, 0) : dalloc(c, p, n); };
     int r = 0;
     for (int q = 0; q < kRing && !r; ++q) {
-      r = dalloc(c, &d.cv[q], 5 * d.sz);
-      if (!r) r = dalloc(c, &d.bv[q], 6 * d.sz);
-      if (!r) r = dalloc(c, &d.mul[q], d.sz);
+      r = take(q == 0 ? u.cv : (q == 1 ? u.qn : nullptr), &d.cv[q], 5 * d.sz);
+      if (!r) r = take(q == 0 ? u.bv : nullptr, &d.bv[q], 6 * d.sz);
+      if (!r) r = take(q == 0 ? u.mul : nullptr, &d.mul[q], d.sz);
     }
     double *jac = nullptr, *met = nullptr;
     if (!r) r = dalloc(c, &d.mach, d.sz);
-    if (!r) r = dalloc(c, &jac, d.sz);
+    if (!r) r = take(u.jac, &jac, d.sz);
     if (!r) r = dalloc(c, &met, 9 * d.sz);
-    if (!r) r = dalloc(c, &d.dq, 5 * d.n0);
+    if (!r && u.metric) repack_metric(u.metric, u.metric_elem_stride, u.metric_comp_stride, u.metric_offset, met, d.sz,
+                                      c->st);
+    if (!r) r = take(u.dq, &d.dq, 5 * d.n0);
     if (!r) r = dalloc(c, &d.dtl, d.n0);
     if (!r) r = dalloc(c, &d.bv_last, 4 * d.n0);
     if (r) return r;
@@ -974,6 +992,7 @@ int ostc_finalize(ostc_ctx c) {
     B.d.sp_mean = mean;
     B.d.sp_jac = B.sp_jac;
   }
+  HIPCHK(c, hipGetLastError());  // the metric copies of caller-owned geometry
   int r = dalloc(c, &c->d_scratch, 1 + 4 * (size_t)c->n_first);
   if (!r) r = dalloc(c, &c->d_dt, 1);
   if (!r) r = build_ops(c);
@@ -1290,7 +1309,17 @@ int ostc_residual(ostc_ctx c, double res[4]) {
     n_total += B.d.n0;
   }
   for (int l = 0; l < 4; ++l) res[l] = std::sqrt(acc[l] / (double)n_total);
-  if (std::isnan(res[0] + res[1] + res[2] + res[3])) return fail(c, OSTC_ERR_NUMERIC, "NaN in the residual");
+  if (std::isnan(res[0] + res[1] + res[2] + res[3])) {
+    // where: the first node of the first block whose primitives hold a NaN
+    std::string at;
+    auto* word = reinterpret_cast<unsigned long long*>(c->d_scratch);
+    for (size_t b = 0; b < c->blocks.size() && at.empty(); ++b) {
+      const DevBlock& d = c->blocks[b].d;
+      at = first_nan_text(d.bv[0], d.sz, d.mx, d.my, d.mz, d.g, word, c->st);
+      if (!at.empty()) at = ": block " + std::to_string(b) + ", " + at;
+    }
+    return fail(c, OSTC_ERR_NUMERIC, "NaN in the residual" + at);
+  }
   return OSTC_OK;
 }
 

@@ -15,6 +15,8 @@
 #include "ostc_faces.hpp"
 #include "ostc_fr.hpp"
 #include "ostc_inflow.hpp"
+#include "ostc_nanscan.hpp"
+#include "ostc_repack.hpp"
 #include "ostc_sponge.hpp"
 
 using namespace ostc::fr;
@@ -37,6 +39,7 @@ struct HBlock {
   std::vector<FaceMap> inner;
   std::vector<double> sp_sigma;  // sponge sigma of every interior node (empty: none)
   bool sp_jac = true;            // the sponge term carries the Jacobian
+  ostc_block_desc desc{};        // the caller's descriptor: optional device arrays it owns
   int e[3];  // mx, my, mz
   const int* ext() const { return e; }
 };
@@ -324,10 +327,13 @@ int ostc_fr_add_block(ostc_fr_ctx c, const ostc_block_desc* d, int* id) {
   if (c->finalized) return fail(c, OSTC_ERR_STATE, "blocks are fixed after finalize");
   if (d->mz < 2) return fail(c, OSTC_ERR_UNSUPPORTED, "two-dimensional blocks are not supported");
   if (d->mx < 2 || d->my < 2) return fail(c, OSTC_ERR_ARG, "block too small");
+  if (d->metric && (d->metric_elem_stride < 1 || d->metric_comp_stride < 1 || d->metric_offset < 0))
+    return fail(c, OSTC_ERR_ARG, "device metric: strides must be positive and the offset non-negative");
   HBlock h;
   h.mx = h.e[0] = d->mx;
   h.my = h.e[1] = d->my;
   h.mz = h.e[2] = d->mz;
+  h.desc = *d;
   c->blocks.push_back(h);
   if (id) *id = (int)c->blocks.size() - 1;
   return OSTC_OK;
@@ -597,15 +603,21 @@ int ostc_fr_finalize(ostc_fr_ctx c) {
     d.sz = (long long)d.sx * d.sy * (h.mz + 2 * g);
     d.n0 = (long long)h.mx * h.my * h.mz;
     const size_t sz = (size_t)d.sz, n0 = (size_t)d.n0;
+    // the caller's arrays where it gave them, the library's own otherwise
+    const ostc_block_desc& u = h.desc;
+    auto take = [&](double* mine, double** p, size_t n) { return mine ? (*p = mine, 0) : dalloc(c, p, n); };
     int r = OSTC_OK;
-    if ((r = dalloc(c, &d.cv, sz * nv)) || (r = dalloc(c, &d.qn, sz * nv)) || (r = dalloc(c, &d.bv, sz * 6)) ||
-        (r = dalloc(c, &d.sv, sz * ns)) || (r = dalloc(c, &d.dq, n0 * nv)) || (r = dalloc(c, &d.mul, sz)) ||
+    if ((r = take(u.cv, &d.cv, sz * nv)) || (r = take(u.qn, &d.qn, sz * nv)) || (r = take(u.bv, &d.bv, sz * 6)) ||
+        (r = take(u.sv, &d.sv, sz * ns)) || (r = take(u.dq, &d.dq, n0 * nv)) || (r = take(u.mul, &d.mul, sz)) ||
         (r = dalloc(c, &d.lam, sz)) || (r = dalloc(c, &d.cp, sz)) || (r = dalloc(c, &d.gam, sz)) ||
         (r = dalloc(c, &d.cc, sz)) || (r = dalloc(c, &d.mach, sz)) || (r = dalloc(c, &d.rhod, sz * ns)) ||
-        (r = dalloc(c, &d.jac, sz)) || (r = dalloc(c, &d.met, sz * 9)) || (r = dalloc(c, &d.dtl, n0)) ||
+        (r = take(u.jac, &d.jac, sz)) || (r = dalloc(c, &d.met, sz * 9)) || (r = dalloc(c, &d.dtl, n0)) ||
         (r = dalloc(c, &d.bvl, n0 * 4)))
       return r;
+    if (u.metric)
+      ostc::repack_metric(u.metric, u.metric_elem_stride, u.metric_comp_stride, u.metric_offset, d.met, d.sz, c->st);
   }
+  FR_HIP(c, hipGetLastError());  // the metric copies of caller-owned geometry
   if (int r = dalloc(c, &c->d_dt, 1)) return r;
   if (int r = dalloc(c, &c->d_res, 4 * kResidualBlocks)) return r;
   if (!c->farfields.empty()) {
@@ -827,7 +839,16 @@ int ostc_fr_residual(ostc_fr_ctx c, double res[4]) {
     n_total += B.d.n0;
   }
   for (int l = 0; l < 4; ++l) res[l] = std::sqrt(acc[l] / (double)n_total);
-  if (std::isnan(res[0] + res[1] + res[2] + res[3])) return fail(c, OSTC_ERR_NUMERIC, "NaN in the residual");
+  if (std::isnan(res[0] + res[1] + res[2] + res[3])) {
+    std::string at;  // the first node of the first block whose primitives hold a NaN
+    auto* word = reinterpret_cast<unsigned long long*>(c->d_res);
+    for (size_t b = 0; b < c->blocks.size() && at.empty(); ++b) {
+      const FrBlockDev& d = c->blocks[b].d;
+      at = ostc::first_nan_text(d.bv, d.sz, d.mx, d.my, d.mz, d.g, word, c->st);
+      if (!at.empty()) at = ": block " + std::to_string(b) + ", " + at;
+    }
+    return fail(c, OSTC_ERR_NUMERIC, "NaN in the residual" + at);
+  }
   return OSTC_OK;
 }
 

@@ -6,9 +6,13 @@
 // slot `out`; ghost layers are filled afterwards by the op launches.
 //
 // Work decomposition: a tile is TX x TY x TZ cells of one block, one thread
-// per cell.  Each direction phase computes split fluxes F+/F- for the tile's
-// (T_d + 2R) x (columns) points, then one WENO flux per face, then each cell
-// subtracts its face difference.  Accumulation order follows the reference:
+// per cell in the cell phases.  Each inviscid direction phase computes split
+// fluxes F+/F- for the tile's (T_d + 2R) x (columns) points, then the WENO
+// flux of every (variable, face) item, then each cell subtracts its face
+// difference.  The viscous phase computes the faces of all three directions
+// as one work list.  Items rather than faces or cells are dealt to the
+// threads, so no wave runs a tail round alone while the others wait at the
+// next barrier.  Accumulation order follows the reference:
 // dq = ((0 - dFx) - dFy) - dFz, then + dGx + dGy + dGz (src/RK.cuh:113-139).
 #include "ostc_kernels.hpp"
 
@@ -20,7 +24,6 @@ struct Dir {
   static constexpr int TD = D == 0 ? TX : (D == 1 ? TY : TZ);
   static constexpr int PC = NT / TD;  // columns
   static constexpr int NF = (TD + 1) * PC;
-  static constexpr int FPT = (NF + NT - 1) / NT;
   // column -> the two tile coordinates other than D
   __device__ static void col_coords(int col, int& a, int& b) {
     if (D == 0) { a = col % TY; b = col / TY; }   // (tj, tk)
@@ -33,19 +36,28 @@ struct Dir {
   // index of (column, position s) in a per-column array of length LEN
   template <int LEN>
   __device__ static int at(int col, int s) { return D == 0 ? col * LEN + s : s * PC + col; }
+  // face f of the tile -> (column, position of the face in the column)
+  __device__ static void face(int f, int& col, int& sf) {
+    if (D == 0) { col = f / (TD + 1); sf = f % (TD + 1); } else { sf = f / PC; col = f % PC; }
+  }
 };
 
+// LDS words of one workgroup.  Inviscid phase of direction d: F+ and F- of
+// the 5 variables at the (T_d + 2R) x columns points, plus the WENO epsilon
+// of every face.  Viscous phase: u, v, w, T, mu on the tile box (radius 1)
+// and the 4 viscous fluxes of the faces of all three directions.
 template <int TX, int TY, int TZ, int SCHEME>
 struct FusedLds {
   static constexpr int NT = TX * TY * TZ;
   static constexpr int R = Stencil<SCHEME>::R;
   static constexpr int pts(int td) { return (td + 2 * R) * (NT / td); }
+  static constexpr int nf(int td) { return (td + 1) * (NT / td); }
   static constexpr int mx3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
-  static constexpr int NPTS = mx3(pts(TX), pts(TY), pts(TZ));
+  static constexpr int IWORDS = mx3(10 * pts(TX) + nf(TX), 10 * pts(TY) + nf(TY), 10 * pts(TZ) + nf(TZ));
   static constexpr int NB = (TX + 2) * (TY + 2) * (TZ + 2);
-  static constexpr int NFMAX = mx3((TX + 1) * (NT / TX), (TY + 1) * (NT / TY), (TZ + 1) * (NT / TZ));
-  static constexpr int VWORDS = 5 * NB + 10 * NT + 4 * NFMAX;  // viscous phase
-  static constexpr int WORDS = (10 * NPTS > VWORDS) ? 10 * NPTS : VWORDS;
+  static constexpr int NFV = nf(TX) + nf(TY) + nf(TZ);
+  static constexpr int VWORDS = 5 * NB + 4 * NFV;
+  static constexpr int WORDS = IWORDS > VWORDS ? IWORDS : VWORDS;
   static constexpr int BYTES = WORDS * 8;
   static constexpr int VBYTES = VWORDS * 8;
 };
@@ -56,20 +68,49 @@ struct TileCtx {
   bool valid;          // cell inside the block
 };
 
+// 32-bit offset of node (i, j, k) in a padded plane; the host refuses blocks
+// whose 9 metric planes would not fit 2^31 doubles.
+__device__ __forceinline__ int poff(const DevBlock& b, int i, int j, int k) {
+  return ((k + b.g) * b.sy + (j + b.g)) * b.sx + (i + b.g);
+}
+
+// The block coordinates of face f of direction D (its lower node L along D,
+// face L+1/2) and whether the face lies in the block (faces past the end of a
+// column, or in columns outside the block, are never read).
+template <int TX, int TY, int TZ, int D>
+__device__ __forceinline__ bool face_node(const DevBlock& B, const TileCtx& t, int f, int ijk[3]) {
+  using Dr = Dir<TX, TY, TZ, D>;
+  int col, sf, a, b;
+  Dr::face(f, col, sf);
+  Dr::col_coords(col, a, b);
+  if (D == 0) { ijk[1] = t.j0 + a; ijk[2] = t.k0 + b; }
+  if (D == 1) { ijk[0] = t.i0 + a; ijk[2] = t.k0 + b; }
+  if (D == 2) { ijk[0] = t.i0 + a; ijk[1] = t.j0 + b; }
+  const int origin = D == 0 ? t.i0 : (D == 1 ? t.j0 : t.k0);
+  const int m = D == 0 ? B.mx : (D == 1 ? B.my : B.mz);
+  ijk[D] = origin + sf - 1;
+  return ijk[D] <= m - 1 && ijk[0] < B.mx + (D == 0 ? 1 : 0) && ijk[1] < B.my + (D == 1 ? 1 : 0) &&
+         ijk[2] < B.mz + (D == 2 ? 1 : 0);
+}
+
+// Inviscid fluxes along direction D, subtracted from dq.  The face work is
+// spread over (variable, face) items, 5 NF of them for NT threads, so every
+// wave carries the same share whatever NF mod NT is.
 template <int TX, int TY, int TZ, int SCHEME, int D>
 __device__ __forceinline__ void inviscid_phase(double* smem, const DevBlock& B, const Params& P, int slot,
                                                const TileCtx& t, double dq[5]) {
   using Dr = Dir<TX, TY, TZ, D>;
   constexpr int NT = Dr::NT, R = Stencil<SCHEME>::R, W = Stencil<SCHEME>::W;
-  constexpr int TD = Dr::TD, PC = Dr::PC, LP = TD + 2 * R, NPTS = LP * PC, NF = Dr::NF, FPT = Dr::FPT;
+  constexpr int TD = Dr::TD, PC = Dr::PC, LP = TD + 2 * R, NPTS = LP * PC, NF = Dr::NF;
+  constexpr int NI = 5 * NF, IPT = (NI + NT - 1) / NT;
   const int tid = threadIdx.x;
-  const int m = D == 0 ? B.mx : (D == 1 ? B.my : B.mz);
   const int origin = D == 0 ? t.i0 : (D == 1 ? t.j0 : t.k0);
   const double* cv = B.cv[slot];
   const double* bv = B.bv[slot];
-  const long long s = B.sz;
+  const int s = (int)B.sz;
   double* sFp = smem;
   double* sFm = smem + 5 * NPTS;
+  double* sEps = smem + 10 * NPTS;
 
   // split fluxes at the points of every column
   for (int p = tid; p < NPTS; p += NT) {
@@ -81,12 +122,11 @@ __device__ __forceinline__ void inviscid_phase(double* smem, const DevBlock& B, 
     if (D == 0) { ijk[1] = t.j0 + a; ijk[2] = t.k0 + b; }
     if (D == 1) { ijk[0] = t.i0 + a; ijk[2] = t.k0 + b; }
     if (D == 2) { ijk[0] = t.i0 + a; ijk[1] = t.j0 + b; }
-    const int pos = origin + sp - R;
-    ijk[D] = pos;
+    ijk[D] = origin + sp - R;
     const bool ok = ijk[0] < B.mx + (D == 0 ? R : 0) && ijk[1] < B.my + (D == 1 ? R : 0) &&
                     ijk[2] < B.mz + (D == 2 ? R : 0);
     if (ok) {
-      const long long q = pidx(B, ijk[0], ijk[1], ijk[2]);
+      const int q = poff(B, ijk[0], ijk[1], ijk[2]);
       double c5[5];
 #pragma unroll
       for (int l = 0; l < 5; ++l) c5[l] = cv[q + l * s];
@@ -98,57 +138,62 @@ __device__ __forceinline__ void inviscid_phase(double* smem, const DevBlock& B, 
       for (int l = 0; l < 5; ++l) { sFp[l * NPTS + p] = fp[l]; sFm[l * NPTS + p] = fm[l]; }
     }
   }
-  __syncthreads();
-
-  double fres[FPT][5];
-#pragma unroll
-  for (int u = 0; u < FPT; ++u) {
-    const int f = tid + u * NT;
-    if (f < NF) {
-      int col, sf;
-      if (D == 0) { col = f / (TD + 1); sf = f % (TD + 1); } else { sf = f / PC; col = f % PC; }
-      int a, b;
-      Dr::col_coords(col, a, b);
-      int ijk[3];
-      if (D == 0) { ijk[1] = t.j0 + a; ijk[2] = t.k0 + b; }
-      if (D == 1) { ijk[0] = t.i0 + a; ijk[2] = t.k0 + b; }
-      if (D == 2) { ijk[0] = t.i0 + a; ijk[1] = t.j0 + b; }
-      const int L = origin + sf - 1;  // face L+1/2
-      ijk[D] = L;
-      const bool ok = L <= m - 1 && ijk[0] < B.mx + (D == 0 ? 1 : 0) && ijk[1] < B.my + (D == 1 ? 1 : 0) &&
-                      ijk[2] < B.mz + (D == 2 ? 1 : 0);
-      if (ok) {
-        const long long p1 = pidx(B, ijk[0], ijk[1], ijk[2]);
-        const long long p2 = p1 + (D == 0 ? 1 : (D == 1 ? (long long)B.sx : (long long)B.sx * B.sy));
-        const double j1 = B.jac[p1], j2 = B.jac[p2];
-        double eps3[3];
-        weno_eps(P, B.met[p1 + 3 * D * s] * j1, B.met[p1 + (3 * D + 1) * s] * j1, B.met[p1 + (3 * D + 2) * s] * j1,
-                 B.met[p2 + 3 * D * s] * j2, B.met[p2 + (3 * D + 1) * s] * j2, B.met[p2 + (3 * D + 2) * s] * j2,
-                 eps3);
-        const int sL = sf - 1 + R;
-#pragma unroll
-        for (int l = 0; l < 5; ++l) {
-          double vp[W], vm[W];
-#pragma unroll
-          for (int r = 0; r < W; ++r) {
-            vp[r] = sFp[l * NPTS + Dr::template at<LP>(col, sL - W / 2 + r)];
-            vm[r] = sFm[l * NPTS + Dr::template at<LP>(col, sL - W / 2 + 1 + r)];
-          }
-          const double e = l == 0 ? eps3[0] : (l == 4 ? eps3[2] : eps3[1]);
-          fres[u][l] = weno_face<SCHEME>(vp, vm, e);
-        }
-      }
+  // the epsilon of every face (src/WENO.cu:1469-1482), shared by its 5 items
+  for (int f = tid; f < NF; f += NT) {
+    int ijk[3];
+    if (face_node<TX, TY, TZ, D>(B, t, f, ijk)) {
+      const int p1 = poff(B, ijk[0], ijk[1], ijk[2]);
+      const int p2 = p1 + (D == 0 ? 1 : (D == 1 ? B.sx : B.sx * B.sy));
+      const double j1 = B.jac[p1], j2 = B.jac[p2];
+      double eps3[3];
+      weno_eps(P, B.met[p1 + 3 * D * s] * j1, B.met[p1 + (3 * D + 1) * s] * j1, B.met[p1 + (3 * D + 2) * s] * j1,
+               B.met[p2 + 3 * D * s] * j2, B.met[p2 + (3 * D + 1) * s] * j2, B.met[p2 + (3 * D + 2) * s] * j2,
+               eps3);
+      sEps[f] = eps3[0];
     }
   }
   __syncthreads();
-  double* sFc = smem;  // aliases the split fluxes, which are no longer read
+
+  // one item per iteration (the loop is not unrolled, so the WENO evaluations
+  // of different items do not overlap in registers); each result is kept in
+  // its own register through a chain of selects
+  double fres[IPT];
 #pragma unroll
-  for (int u = 0; u < FPT; ++u) {
-    const int f = tid + u * NT;
-    if (f < NF) {
+  for (int u = 0; u < IPT; ++u) fres[u] = 0.0;
+#pragma unroll 1
+  for (int u = 0; u < IPT; ++u) {
+    const int it = tid + u * NT;
+    double val = 0.0;
+    if (it < NI) {
+      const int l = it / NF, f = it - l * NF;
+      int ijk[3];
+      if (face_node<TX, TY, TZ, D>(B, t, f, ijk)) {
+        int col, sf;
+        Dr::face(f, col, sf);
+        const int sL = sf - 1 + R;
+        // eps_scaled of src/WENO.cu:1478-1481 for variable l
+        const double e0 = sEps[f];
+        const double e = l == 0 ? e0 : (l == 4 ? e0 * P.v_ref * P.v_ref * P.v_ref * P.v_ref : e0 * P.v_ref * P.v_ref);
+        const double* fpl = sFp + l * NPTS;
+        const double* fml = sFm + l * NPTS;
+        double vp[W], vm[W];
 #pragma unroll
-      for (int l = 0; l < 5; ++l) sFc[l * NF + f] = fres[u][l];
+        for (int r = 0; r < W; ++r) {
+          vp[r] = fpl[Dr::template at<LP>(col, sL - W / 2 + r)];
+          vm[r] = fml[Dr::template at<LP>(col, sL - W / 2 + 1 + r)];
+        }
+        val = weno_face<SCHEME>(vp, vm, e);
+      }
     }
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) fres[q] = q == u ? val : fres[q];
+  }
+  __syncthreads();
+  double* sFc = smem;  // [5][NF], aliases the split fluxes, which are no longer read
+#pragma unroll
+  for (int u = 0; u < IPT; ++u) {
+    const int it = tid + u * NT;
+    if (it < NI) sFc[it] = fres[u];
   }
   __syncthreads();
   if (t.valid) {
@@ -160,124 +205,84 @@ __device__ __forceinline__ void inviscid_phase(double* smem, const DevBlock& B, 
   __syncthreads();
 }
 
+// Viscous flux of face f of direction D (src/ViscousScheme.cuh:23-909),
+// written to sFv[l * NF + f]: u, v, w, T and mu from the staged tile box,
+// the metric and Jacobian of the two face cells from global memory.
 template <int TX, int TY, int TZ, int D>
-__device__ __forceinline__ void viscous_dir(double* smem, const DevBlock& B, const Params& P, const TileCtx& t,
-                                            double dq[5]) {
+__device__ __forceinline__ void viscous_face(const double* sV, double* sFv, const DevBlock& B, const Params& P,
+                                             const TileCtx& t, int f) {
   using Dr = Dir<TX, TY, TZ, D>;
-  constexpr int NT = Dr::NT, TD = Dr::TD, PC = Dr::PC, NF = Dr::NF, FPT = Dr::FPT;
-  constexpr int BX = TX + 2, BY = TY + 2, NB = (TX + 2) * (TY + 2) * (TZ + 2);
-  const int tid = threadIdx.x;
-  const int m = D == 0 ? B.mx : (D == 1 ? B.my : B.mz);
-  const int origin = D == 0 ? t.i0 : (D == 1 ? t.j0 : t.k0);
-  const long long s = B.sz;
-  const double* sV = smem;          // [5][NB]: u, v, w, T, mu on the tile box
-  const double* sM = smem + 5 * NB; // [10][NT]: metric (9) and Jacobian of the tile cells
-  double* sFv = smem + 5 * NB + 10 * NT;  // [4][NF]
+  constexpr int NF = Dr::NF, BX = TX + 2, BY = TY + 2, NB = (TX + 2) * (TY + 2) * (TZ + 2);
+  int ijk[3];
+  if (!face_node<TX, TY, TZ, D>(B, t, f, ijk)) return;
+  const int s = (int)B.sz;
+  auto box = [&](int i, int j, int k) { return ((k - t.k0 + 1) * BY + (j - t.j0 + 1)) * BX + (i - t.i0 + 1); };
   auto pv = [&](int i, int j, int k, int l) {
     const int q = l == 5 ? 3 : l - 1;
-    return sV[q * NB + ((k - t.k0 + 1) * BY + (j - t.j0 + 1)) * BX + (i - t.i0 + 1)];
+    return sV[q * NB + box(i, j, k)];
   };
-  // one face at a time, each flux written straight to LDS (sFv does not
-  // alias the staged box, so no barrier is needed before the writes)
-#pragma unroll 1
-  for (int u = 0; u < FPT; ++u) {
-    const int f = tid + u * NT;
-    if (f < NF) {
-      int col, sf;
-      if (D == 0) { col = f / (TD + 1); sf = f % (TD + 1); } else { sf = f / PC; col = f % PC; }
-      int a, b;
-      Dr::col_coords(col, a, b);
-      int ijk[3];
-      if (D == 0) { ijk[1] = t.j0 + a; ijk[2] = t.k0 + b; }
-      if (D == 1) { ijk[0] = t.i0 + a; ijk[2] = t.k0 + b; }
-      if (D == 2) { ijk[0] = t.i0 + a; ijk[1] = t.j0 + b; }
-      const int L = origin + sf - 1;
-      ijk[D] = L;
-      const bool ok = L <= m - 1 && ijk[0] < B.mx + (D == 0 ? 1 : 0) && ijk[1] < B.my + (D == 1 ? 1 : 0) &&
-                      ijk[2] < B.mz + (D == 2 ? 1 : 0);
-      if (ok) {
-        const int i = ijk[0], j = ijk[1], k = ijk[2];
-        double d[4][3];
-        const int var[4] = {1, 2, 3, 5};
+  const int i = ijk[0], j = ijk[1], k = ijk[2];
+  const int iB = i + (D == 0), jB = j + (D == 1), kB = k + (D == 2);
+  const int pA = poff(B, i, j, k);
+  const int pB = pA + (D == 0 ? 1 : (D == 1 ? B.sx : B.sx * B.sy));
+  double fm[9], kr[3];
+  viscous_face_metric<D>(B.met, s, pA, pB, B.jac[pA], B.jac[pB], fm, kr);
+  // one variable at a time: its differences along xi, eta, zeta, then its
+  // physical gradient, so only the gradients stay live
+  double g[4][3];
+  const int var[4] = {1, 2, 3, 5};
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int l = var[q];
-          if (D == 0) {
-            d[q][0] = pv(i + 1, j, k, l) - pv(i, j, k, l);
-            d[q][1] = 0.25 * (pv(i, j + 1, k, l) - pv(i, j - 1, k, l) + pv(i + 1, j + 1, k, l) - pv(i + 1, j - 1, k, l));
-            d[q][2] = 0.25 * (pv(i, j, k + 1, l) - pv(i, j, k - 1, l) + pv(i + 1, j, k + 1, l) - pv(i + 1, j, k - 1, l));
-          } else if (D == 1) {
-            d[q][0] = 0.25 * (pv(i + 1, j, k, l) - pv(i - 1, j, k, l) + pv(i + 1, j + 1, k, l) - pv(i - 1, j + 1, k, l));
-            d[q][1] = pv(i, j + 1, k, l) - pv(i, j, k, l);
-            d[q][2] = 0.25 * (pv(i, j, k + 1, l) - pv(i, j, k - 1, l) + pv(i, j + 1, k + 1, l) - pv(i, j + 1, k - 1, l));
-          } else {
-            d[q][0] = 0.25 * (pv(i + 1, j, k, l) - pv(i - 1, j, k, l) + pv(i + 1, j, k + 1, l) - pv(i - 1, j, k + 1, l));
-            d[q][1] = 0.25 * (pv(i, j + 1, k, l) - pv(i, j - 1, k, l) + pv(i, j + 1, k + 1, l) - pv(i, j - 1, k + 1, l));
-            d[q][2] = pv(i, j, k + 1, l) - pv(i, j, k, l);
-          }
-        }
-        const long long pA = pidx(B, i, j, k);
-        const long long pB = pA + (D == 0 ? 1 : (D == 1 ? (long long)B.sx : (long long)B.sx * B.sy));
-        const int iB = i + (D == 0), jB = j + (D == 1), kB = k + (D == 2);
-        // metric and Jacobian of the two face cells: from the tile's LDS copy,
-        // or from global memory for the cell beyond either end of a column
-        double mA[9], mB[9], jacA, jacB;
-        const int cA = ((k - t.k0) * TY + (j - t.j0)) * TX + (i - t.i0);
-        const int cB = ((kB - t.k0) * TY + (jB - t.j0)) * TX + (iB - t.i0);
-        const bool inA = (D == 0 ? i >= t.i0 : (D == 1 ? j >= t.j0 : k >= t.k0));
-        const bool inB = (D == 0 ? iB < t.i0 + TX : (D == 1 ? jB < t.j0 + TY : kB < t.k0 + TZ));
-        if (inA) {
-#pragma unroll
-          for (int c = 0; c < 9; ++c) mA[c] = sM[c * NT + cA];
-          jacA = sM[9 * NT + cA];
-        } else {
-#pragma unroll
-          for (int c = 0; c < 9; ++c) mA[c] = B.met[pA + c * s];
-          jacA = B.jac[pA];
-        }
-        if (inB) {
-#pragma unroll
-          for (int c = 0; c < 9; ++c) mB[c] = sM[c * NT + cB];
-          jacB = sM[9 * NT + cB];
-        } else {
-#pragma unroll
-          for (int c = 0; c < 9; ++c) mB[c] = B.met[pB + c * s];
-          jacB = B.jac[pB];
-        }
-        const double um = 0.5 * (pv(i, j, k, 1) + pv(iB, jB, kB, 1));
-        const double vm = 0.5 * (pv(i, j, k, 2) + pv(iB, jB, kB, 2));
-        const double wm = 0.5 * (pv(i, j, k, 3) + pv(iB, jB, kB, 3));
-        const double muA = sV[4 * NB + ((k - t.k0 + 1) * BY + (j - t.j0 + 1)) * BX + (i - t.i0 + 1)];
-        const double muB = sV[4 * NB + ((kB - t.k0 + 1) * BY + (jB - t.j0 + 1)) * BX + (iB - t.i0 + 1)];
-        double fv[4];
-        viscous_face_flux<D>(d, mA, mB, jacA, jacB, muA, muB, um, vm, wm, P.prandtl, fv);
-#pragma unroll
-        for (int l = 0; l < 4; ++l) sFv[l * NF + f] = fv[l];
-      }
+  for (int q = 0; q < 4; ++q) {
+    const int l = var[q];
+    double d[3];
+    if (D == 0) {
+      d[0] = pv(i + 1, j, k, l) - pv(i, j, k, l);
+      d[1] = 0.25 * (pv(i, j + 1, k, l) - pv(i, j - 1, k, l) + pv(i + 1, j + 1, k, l) - pv(i + 1, j - 1, k, l));
+      d[2] = 0.25 * (pv(i, j, k + 1, l) - pv(i, j, k - 1, l) + pv(i + 1, j, k + 1, l) - pv(i + 1, j, k - 1, l));
+    } else if (D == 1) {
+      d[0] = 0.25 * (pv(i + 1, j, k, l) - pv(i - 1, j, k, l) + pv(i + 1, j + 1, k, l) - pv(i - 1, j + 1, k, l));
+      d[1] = pv(i, j + 1, k, l) - pv(i, j, k, l);
+      d[2] = 0.25 * (pv(i, j, k + 1, l) - pv(i, j, k - 1, l) + pv(i, j + 1, k + 1, l) - pv(i, j + 1, k - 1, l));
+    } else {
+      d[0] = 0.25 * (pv(i + 1, j, k, l) - pv(i - 1, j, k, l) + pv(i + 1, j, k + 1, l) - pv(i - 1, j, k + 1, l));
+      d[1] = 0.25 * (pv(i, j + 1, k, l) - pv(i, j - 1, k, l) + pv(i, j + 1, k + 1, l) - pv(i, j - 1, k + 1, l));
+      d[2] = pv(i, j, k + 1, l) - pv(i, j, k, l);
     }
+    viscous_grad(d, fm, g[q]);
+    // keep the scheduler from hoisting the next variable's LDS reads above
+    // this one's arithmetic (they would all be live at once)
+    __builtin_amdgcn_sched_barrier(0);
   }
-  __syncthreads();
-  if (t.valid) {
-    const int col = Dr::my_col(t.ti, t.tj, t.tk), pos = Dr::my_pos(t.ti, t.tj, t.tk);
-    const int fL = Dr::template at<TD + 1>(col, pos), fR = Dr::template at<TD + 1>(col, pos + 1);
+  const double um = 0.5 * (pv(i, j, k, 1) + pv(iB, jB, kB, 1));
+  const double vm = 0.5 * (pv(i, j, k, 2) + pv(iB, jB, kB, 2));
+  const double wm = 0.5 * (pv(i, j, k, 3) + pv(iB, jB, kB, 3));
+  const double muA = sV[4 * NB + box(i, j, k)];
+  const double muB = sV[4 * NB + box(iB, jB, kB)];
+  double fv[4];
+  viscous_flux_from_grad(g, kr, muA, muB, um, vm, wm, P.prandtl, fv);
 #pragma unroll
-    for (int l = 0; l < 4; ++l) dq[l + 1] += sFv[l * NF + fR] - sFv[l * NF + fL];
-  }
-  __syncthreads();
+  for (int l = 0; l < 4; ++l) sFv[l * NF + f] = fv[l];
 }
 
+// Viscous fluxes of the three directions, added to dq in the reference's
+// order (x, then y, then z).  The faces of all three directions form one
+// work list, so the threads share them evenly.
 template <int TX, int TY, int TZ>
 __device__ __forceinline__ void viscous_phase(double* smem, const DevBlock& B, const Params& P, int slot,
                                               const TileCtx& t, double dq[5]) {
   constexpr int NT = TX * TY * TZ, BX = TX + 2, BY = TY + 2, NB = (TX + 2) * (TY + 2) * (TZ + 2);
+  using D0 = Dir<TX, TY, TZ, 0>;
+  using D1 = Dir<TX, TY, TZ, 1>;
+  using D2 = Dir<TX, TY, TZ, 2>;
+  constexpr int NF0 = D0::NF, NF1 = D1::NF, NF2 = D2::NF, NFV = NF0 + NF1 + NF2;
   const double* bv = B.bv[slot];
   const double* mu = B.mul[slot];
-  const long long s = B.sz;
+  const int s = (int)B.sz;
   for (int p = threadIdx.x; p < NB; p += NT) {
     const int a = p % BX, b = (p / BX) % BY, c = p / (BX * BY);
     const int i = t.i0 - 1 + a, j = t.j0 - 1 + b, k = t.k0 - 1 + c;
     if (i <= B.mx && j <= B.my && k <= B.mz) {
-      const long long q = pidx(B, i, j, k);
+      const int q = poff(B, i, j, k);
       smem[p] = bv[q + s];
       smem[NB + p] = bv[q + 2 * s];
       smem[2 * NB + p] = bv[q + 3 * s];
@@ -285,22 +290,38 @@ __device__ __forceinline__ void viscous_phase(double* smem, const DevBlock& B, c
       smem[4 * NB + p] = mu[q];
     }
   }
-  {
-    // this thread's cell: metric and Jacobian, read once per stage
-    const int i = t.i0 + t.ti, j = t.j0 + t.tj, k = t.k0 + t.tk;
-    double* sM = smem + 5 * NB;
-    const int c = threadIdx.x;
-    if (i <= B.mx && j <= B.my && k <= B.mz) {
-      const long long q = pidx(B, i, j, k);
+  __syncthreads();
+  double* sF0 = smem + 5 * NB;
+  double* sF1 = sF0 + 4 * NF0;
+  double* sF2 = sF1 + 4 * NF1;
+#pragma unroll 1
+  for (int it = threadIdx.x; it < NFV; it += NT) {
+    if (it < NF0) viscous_face<TX, TY, TZ, 0>(smem, sF0, B, P, t, it);
+    else if (it < NF0 + NF1) viscous_face<TX, TY, TZ, 1>(smem, sF1, B, P, t, it - NF0);
+    else viscous_face<TX, TY, TZ, 2>(smem, sF2, B, P, t, it - NF0 - NF1);
+  }
+  __syncthreads();
+  if (t.valid) {
+    {
+      const int col = D0::my_col(t.ti, t.tj, t.tk), pos = D0::my_pos(t.ti, t.tj, t.tk);
+      const int fL = D0::template at<D0::TD + 1>(col, pos), fR = D0::template at<D0::TD + 1>(col, pos + 1);
 #pragma unroll
-      for (int m = 0; m < 9; ++m) sM[m * NT + c] = B.met[q + m * s];
-      sM[9 * NT + c] = B.jac[q];
+      for (int l = 0; l < 4; ++l) dq[l + 1] += sF0[l * NF0 + fR] - sF0[l * NF0 + fL];
+    }
+    {
+      const int col = D1::my_col(t.ti, t.tj, t.tk), pos = D1::my_pos(t.ti, t.tj, t.tk);
+      const int fL = D1::template at<D1::TD + 1>(col, pos), fR = D1::template at<D1::TD + 1>(col, pos + 1);
+#pragma unroll
+      for (int l = 0; l < 4; ++l) dq[l + 1] += sF1[l * NF1 + fR] - sF1[l * NF1 + fL];
+    }
+    {
+      const int col = D2::my_col(t.ti, t.tj, t.tk), pos = D2::my_pos(t.ti, t.tj, t.tk);
+      const int fL = D2::template at<D2::TD + 1>(col, pos), fR = D2::template at<D2::TD + 1>(col, pos + 1);
+#pragma unroll
+      for (int l = 0; l < 4; ++l) dq[l + 1] += sF2[l * NF2 + fR] - sF2[l * NF2 + fL];
     }
   }
   __syncthreads();
-  viscous_dir<TX, TY, TZ, 0>(smem, B, P, t, dq);
-  viscous_dir<TX, TY, TZ, 1>(smem, B, P, t, dq);
-  viscous_dir<TX, TY, TZ, 2>(smem, B, P, t, dq);
 }
 
 // MODE 0: inviscid + viscous + RK update; MODE 1: write dq (inviscid, and
@@ -364,7 +385,7 @@ __global__ __launch_bounds__(TX * TY * TZ) __attribute__((amdgpu_waves_per_eu(WP
     }
     if (VISC) viscous_phase<TX, TY, TZ>(smem, B, A.P, A.in, t, dq);
     if (!t.valid) continue;
-    const long long p = pidx(B, i, j, k), s = B.sz;
+    const int p = poff(B, i, j, k), s = (int)B.sz;
     if (MODE == 1) {
       const long long p0 = pidx0(B, i, j, k);
 #pragma unroll
@@ -455,7 +476,9 @@ __global__ __launch_bounds__(TX * TY * TZ) void k_dt_tiles(FusedArgs A, unsigned
 }
 
 // ---------------------------------------------------------- shape table
-#define OSTC_SHAPES(X) X(16, 4, 4, 1) X(8, 8, 4, 1) X(8, 4, 4, 1) X(32, 4, 2, 1) X(16, 8, 2, 1) X(8, 8, 8, 1)
+// (TX, TY, TZ, waves per SIMD the register allocation must allow).  Shapes
+// whose LDS admits three workgroups per CU are compiled for 3 waves per SIMD.
+#define OSTC_SHAPES(X) X(8, 8, 4, 3) X(16, 4, 4, 2) X(8, 4, 4, 3) X(4, 8, 8, 3) X(16, 8, 2, 2) X(8, 8, 8, 2)
 
 static const TileShape kShapes[] = {
 #define X(a, b, c, w) {a, b, c, w},

@@ -65,7 +65,12 @@ class FrSetup(C.Structure):
 
 
 class _BlockDesc(C.Structure):
-    _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int)]
+    """ostc_block_desc (include/openstc/abi.h), as in the package's mirror."""
+    _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int),
+                ("cv", C.c_void_p), ("bv", C.c_void_p), ("sv", C.c_void_p), ("qn", C.c_void_p),
+                ("dq", C.c_void_p), ("mul", C.c_void_p), ("jac", C.c_void_p), ("metric", C.c_void_p),
+                ("metric_elem_stride", C.c_longlong), ("metric_comp_stride", C.c_longlong),
+                ("metric_offset", C.c_longlong)]
 
 
 _I, _D, _P, _LL = C.c_int, C.c_double, C.c_void_p, C.c_longlong
@@ -161,7 +166,11 @@ class FrSolver:
     """Blocks of a finite-rate mixture on one GPU (the reference's layout and
     operators, MixtureModel::FR)."""
 
-    def __init__(self, case: dict, tables: MechTables | None = None, stream: int | None = None):
+    def __init__(self, case: dict, tables: MechTables | None = None, stream: int | None = None,
+                 device_arrays=None):
+        """device_arrays: per block, a dict of caller-owned device arrays (cv,
+        bv, sv, qn, dq, mul, jac, metric + metric_strides; see the package's
+        block_desc) that the library adopts instead of allocating its own."""
         pkg = _pkg()
         self.L = L = pkg.lib()
         self.case = case
@@ -182,8 +191,14 @@ class FrSolver:
         g = C.c_int()
         self._chk(L.ostc_fr_ngg(self.ctx, C.byref(g)))
         self.ngg = g.value
-        for b in case["blocks"]:
-            self._chk(L.ostc_fr_add_block(self.ctx, C.byref(_BlockDesc(b["mx"], b["my"], b["mz"])), None))
+        for i, b in enumerate(case["blocks"]):
+            d = _BlockDesc(b["mx"], b["my"], b["mz"])
+            for k, v in ((device_arrays[i] if device_arrays else None) or {}).items():
+                if k == "metric_strides":
+                    d.metric_elem_stride, d.metric_comp_stride, d.metric_offset = v
+                else:
+                    setattr(d, k, v)
+            self._chk(L.ostc_fr_add_block(self.ctx, C.byref(d), None))
         ints = _pkg().ints
         for f in case["boundaries"]:
             if f["type"] == 5:  # inflow: its state and the y of the block's nodes

@@ -8,6 +8,7 @@ import struct
 import tempfile
 
 import numpy as np
+import pytest
 
 from _util import pkg
 
@@ -67,3 +68,24 @@ def test_residual_scale_round_trip():
         path = os.path.join(d, "residual_scale.txt")
         ck.write_residual_scale(path, [1.5, 2e-3, 101325.0, 300.0])
         assert ck.read_residual_scale(path) == [1.5, 2e-3, 101325.0, 300.0]
+
+
+@pytest.mark.parametrize("cut", ["header", "zone_header", "data", "last_byte"])
+def test_truncated_file_gives_a_reader_error(cut):
+    """A flowfield.plt cut short anywhere -- in the header, in a zone's data
+    header, inside the data, or by its last byte -- is refused with a clean
+    ValueError naming the truncation, not a struct or numpy error."""
+    rng = np.random.default_rng(9)
+    names = ck.BASE_NAMES
+    zones = _zones(rng, [(3, 4, 5), (2, 3, 4)], len(names))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "flowfield.plt")
+        ck.write_plt(path, names, zones)
+        data = open(path, "rb").read()
+        header_end = data.index(struct.pack("<f", ck.EOH_MARKER)) + 4
+        at = {"header": 40, "zone_header": header_end + 10, "data": header_end + 16 + 20 * len(names) + 100,
+              "last_byte": len(data) - 1}[cut]
+        with open(path, "wb") as f:
+            f.write(data[:at])
+        with pytest.raises(ValueError, match="truncated"):
+            ck.read_plt(path)
