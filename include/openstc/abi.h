@@ -343,6 +343,14 @@ typedef struct {
   int ni, nk;
 } ostc_theta;
 int ostc_momentum_thickness(ostc_ctx ctx, int block, const ostc_theta* t, double* theta);
+/* No counterpart in the reference.  The same momentum thickness, bit for bit,
+ * for callers that cannot allocate device memory (code under stream capture,
+ * tight optimisation loops): it calls no hipMalloc/hipFree and keeps its row
+ * sums in a file-scope device array of 4 * 16384 + 1 doubles that each GPU
+ * holds until the process ends.  Blocks of more than 16384 rows (my) return
+ * OSTC_ERR_ARG; the context's device must be current (else OSTC_ERR_STATE).
+ * Calls that overlap on one GPU share the array and race. */
+int ostc_momentum_thickness_noalloc(ostc_ctx ctx, int block, const ostc_theta* t, double* theta);
 
 /* Forward half of the differentiated run: n_steps batched SSP-RK3 steps from
  * the current (initialised) state with the fixed dt, keeping the state every

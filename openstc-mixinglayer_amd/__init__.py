@@ -213,6 +213,7 @@ SIGNATURES = {
     "ostc_rhs_vjp": (_I, [_P, _I, _P, _P]),
     "ostc_perturb_fourier": (_I, [_P, _I, C.POINTER(Fourier)]),
     "ostc_momentum_thickness": (_I, [_P, _I, C.POINTER(Theta), _DP]),
+    "ostc_momentum_thickness_noalloc": (_I, [_P, _I, C.POINTER(Theta), _DP]),
     "ostc_adjoint_forward": (_I, [_P, _I, _I, C.POINTER(Theta), _DP]),
     "ostc_adjoint_backward": (_I, [_P, C.POINTER(Fourier), _DP]),
     "ostc_adjoint_timing": (_I, [_P, _DP, _DP]),
@@ -475,6 +476,13 @@ class Solver:
     def momentum_thickness(self, b: int, theta: dict) -> float:
         v = C.c_double()
         self._chk(lib().ostc_momentum_thickness(self.ctx, b, C.byref(theta_struct(theta)), C.byref(v)))
+        return v.value
+
+    def momentum_thickness_noalloc(self, b: int, theta: dict) -> float:
+        """momentum_thickness without device allocation (the library's
+        static device scratch; blocks of at most 16384 rows)."""
+        v = C.c_double()
+        self._chk(lib().ostc_momentum_thickness_noalloc(self.ctx, b, C.byref(theta_struct(theta)), C.byref(v)))
         return v.value
 
     def adjoint_forward(self, n_steps: int, checkpoint_every: int, theta: dict) -> float:

@@ -87,6 +87,7 @@ struct ostc_ctx_s {
   hipStream_t st = nullptr;
   bool own_stream = false;
   int cus = 256;
+  int dev = 0;  // the device current at ostc_create
   std::vector<HBlock> blocks;
   std::vector<ostc_farfield> farfields;  // states of the farfield faces, in registration order
   double* d_ff = nullptr;
@@ -697,9 +698,8 @@ int ostc_create(const ostc_setup* s, void* hip_stream, ostc_ctx* out) {
     }
     c->own_stream = true;
   }
-  int dev = 0;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev);
+  hipGetDevice(&c->dev);
+  hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->dev);
   *out = c;
   return OSTC_OK;
 }
@@ -1537,6 +1537,33 @@ int ostc_momentum_thickness(ostc_ctx c, int b, const ostc_theta* t, double* thet
   if (!theta) return fail(c, OSTC_ERR_ARG, "null theta");
   double* obj = c->d_obj;
   theta_eval(B, to_theta(*t), B.cv[0], obj, obj + 2 * B.my, obj + 4 * B.my, c->st);
+  HIPCHK(c, hipMemcpyAsync(theta, obj + 4 * B.my, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_momentum_thickness_noalloc(ostc_ctx c, int b, const ostc_theta* t, double* theta) {
+  int r = check_block(c, b);
+  if (r) return r;
+  const DevBlock& B = c->blocks[b].d;
+  r = check_theta(c, B, t);
+  if (r) return r;
+  if (!theta) return fail(c, OSTC_ERR_ARG, "null theta");
+  if (B.my > kThetaMaxRows)
+    return fail(c, OSTC_ERR_ARG, "block " + std::to_string(b) + " has " + std::to_string(B.my) +
+                                     " rows; the allocation-free momentum thickness covers at most " +
+                                     std::to_string(kThetaMaxRows));
+  int dev = -1;
+  HIPCHK(c, hipGetDevice(&dev));
+  if (dev != c->dev)
+    return fail(c, OSTC_ERR_STATE, "device " + std::to_string(dev) + " is current, the context runs on device " +
+                                       std::to_string(c->dev));
+  double* obj = nullptr;
+  const int e = theta_scratch_ptr(dev, &obj);
+  if (e != (int)hipSuccess)
+    return fail(c, OSTC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString((hipError_t)e));
+  theta_eval(B, to_theta(*t), B.cv[0], obj, obj + 2 * B.my, obj + 4 * B.my, c->st);
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(theta, obj + 4 * B.my, sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   return OSTC_OK;

@@ -82,5 +82,10 @@ void fourier_grad(const DevBlock& B, const FourierModes& F, const double* lam, c
 void theta_eval(const DevBlock& B, const ThetaParams& T, const double* cv, double* rows, double* coef, double* theta,
                 hipStream_t st);
 void theta_seed(const DevBlock& B, const ThetaParams& T, const double* coef, double* lam, hipStream_t st);
+// The momentum thickness with its row sums, coefficients and result in a
+// file-scope device array (4 kThetaMaxRows + 1 doubles per GPU) instead of
+// library-allocated memory; `scratch_ptr` comes from theta_scratch_ptr.
+constexpr int kThetaMaxRows = 16384;
+int theta_scratch_ptr(int device, double** scratch_ptr);
 
 }  // namespace ostc

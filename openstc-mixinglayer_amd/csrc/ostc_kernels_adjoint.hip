@@ -9,6 +9,7 @@
 //
 // Every cotangent is accumulated in a fixed order (gathers, no atomics), so a
 // reverse sweep is bitwise reproducible run to run.
+#include <mutex>
 #include <type_traits>
 
 #include "ostc_kernels.hpp"
@@ -726,6 +727,39 @@ void theta_eval(const DevBlock& B, const ThetaParams& T, const double* cv, doubl
 
 void theta_seed(const DevBlock& B, const ThetaParams& T, const double* coef, double* lam, hipStream_t st) {
   k_theta_seed<<<blocks_for(B.sz), kT, 0, st>>>(geo_of(B), T.ni, T.nk, coef, lam);
+}
+
+}  // namespace ostc
+
+// ------------------------------------------------ allocation-free objective
+// Row sums (2 my), theta coefficients (2 my) and theta of the momentum
+// thickness for blocks of up to kThetaMaxRows rows, as one file-scope device
+// array: every GPU the library runs on holds its own copy until the process
+// ends.  k_theta_rows writes every row it covers and k_theta_final every
+// coefficient and theta before anything reads them, so nothing needs
+// clearing; calls that overlap on one GPU share the array and race.
+
+namespace ostc {
+
+static __device__ __align__(256) double scratch[4 * kThetaMaxRows + 1];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current: the
+// array's host-side name is not a device address, so kernels and copies
+// take this pointer.  Looked up once per device and kept.
+int theta_scratch_ptr(int device, double** scratch_ptr) {
+  if (device < 0 || device >= kMaxDevices || !scratch_ptr) return (int)hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *scratch_ptr = g_scratch_ptr[device];
+  return (int)hipSuccess;
 }
 
 }  // namespace ostc
