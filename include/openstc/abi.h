@@ -427,6 +427,11 @@ typedef struct {
 #define OSTC_FR_MODE_DIRECT 0   /* kernels launched one by one */
 #define OSTC_FR_MODE_GRAPH 1    /* one step captured in a hipGraph, replayed */
 #define OSTC_FR_MODE_TIMED 2    /* direct, with HIP events around every kernel (ostc_fr_kernel_ms) */
+/* One launch per kernel kind over a work-list of every block's tiles (source, sweeps, update,
+ * properties, local dt), the step captured in a hipGraph and replayed; boundary conditions and
+ * inner faces stay per face.  Same results as the per-block modes. */
+#define OSTC_FR_MODE_BATCHED 3
+#define OSTC_FR_MODE_BATCHED_TIMED 4  /* the batched step launched directly, with HIP events per kernel kind */
 
 /* kernel kinds of ostc_fr_kernel_ms */
 #define OSTC_FR_K_SOURCE 0

@@ -462,7 +462,6 @@ int upload_blocks(ostc_ctx c) {
 bool fused_scheme(const ostc_ctx c) { return c->P.scheme == 51 || c->P.scheme == 50 || c->P.scheme == 71; }
 
 int need_fused(ostc_ctx c) {
-  if (c->sponge) return fail(c, OSTC_ERR_UNSUPPORTED, "the sponge layer runs with OSTC_MODE_PLAIN");
   for (auto& B : c->blocks)  // the fused kernel indexes a block's planes with 32-bit offsets
     if (9 * B.d.sz >= (1LL << 31))
       return fail(c, OSTC_ERR_UNSUPPORTED, "block too large for the batched kernel (9 padded planes >= 2^31 doubles)");
@@ -611,6 +610,11 @@ int batched_stage(ostc_ctx c, int rk, long long* n, int ev_base) {
 }
 
 int batched_record(ostc_ctx c, long long* n, int ev_base) {
+  if (c->sponge) {  // update_sponge_layer_value, then update_sponge_iter (src/RK.cuh:104-109), on slot 0
+    for (auto& B : c->blocks)
+      if (B.d.sp_sigma) sponge_mean(B.d, c->d_sp_iter, c->st, n);
+    count_pass(c->d_sp_iter, 0, 1, c->st, n);
+  }
   if (!c->s.fixed_time_step) {
     FusedArgs a = fused_args(c, 0, 1, 0);
     const TileShape t = fused_shape(c->P.scheme, c->tile_index);

@@ -77,6 +77,11 @@ struct ostc_fr_ctx_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   long long graph_launches = 0;
+  // batched step: the per-block kernel arguments and its own graph
+  std::vector<FrArgs> h_args;
+  hipGraph_t bgraph = nullptr;
+  hipGraphExec_t bgexec = nullptr;
+  long long bgraph_launches = 0;
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   double last_ms = 0;
   long long last_launches = 0;
@@ -218,23 +223,61 @@ void step(ostc_fr_ctx c, long long* n, KernelTimer* tm) {
   }
 }
 
+// The same step with one launch per kernel kind over every block
+// (source, sweeps, update, properties, local dt); boundary conditions and
+// inner faces stay per face.  Each block's stage only reads and writes its
+// own arrays until the boundary conditions, so running all residuals, then
+// all updates, then all boundary conditions gives the per-block order's
+// result.
+void step_batched(ostc_fr_ctx c, long long* n, KernelTimer* tm) {
+  const int nb = (int)c->blocks.size();
+  const FrArgs* h = c->h_args.data();
+  if (c->s.fixed_time_step) launch_dt_set(c->d_dt, c->s.dt, c->st, n);
+  for (int b = 0; b < nb; ++b) {  // qn <- cv
+    const FrBlockDev& d = c->blocks[b].d;
+    (void)hipMemcpyAsync(d.qn, d.cv, (size_t)d.sz * c->nv * sizeof(double), hipMemcpyDeviceToDevice, c->st);
+    ++*n;
+  }
+  if (c->sponge) {
+    for (int b = 0; b < nb; ++b)
+      if (c->blocks[b].d.sp_sigma) launch_sponge_mean(h[b], c->ns, c->d_sp_iter, c->st, n);
+    launch_count_pass(c->d_sp_iter, c->st, n);
+  }
+  for (int rk = 0; rk < 3; ++rk) {
+    if (!c->s.fixed_time_step && rk == 0) {
+      launch_dt_set(c->d_dt, 1e6, c->st, n);
+      launch_local_dt_batched(h, nb, reinterpret_cast<unsigned long long*>(c->d_dt), c->st, n);
+    }
+    launch_rhs_batched(h, nb, c->ns, c->st, n, tm);
+    launch_update_batched(h, nb, c->ns, kRkA[rk], kRkB[rk], kRkC[rk], c->st, n, tm);
+    for (int b = 0; b < nb; ++b) apply_bc(c, b, n);
+    exchange(c, n);
+    launch_props_batched(h, nb, c->ns, c->st, n, tm);
+  }
+}
+
 void invalidate_graph(ostc_fr_ctx c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->bgexec) (void)hipGraphExecDestroy(c->bgexec);
+  if (c->bgraph) (void)hipGraphDestroy(c->bgraph);
   c->gexec = nullptr;
   c->graph = nullptr;
+  c->bgexec = nullptr;
+  c->bgraph = nullptr;
 }
 
-int build_graph(ostc_fr_ctx c) {
-  if (c->gexec) return OSTC_OK;
+int build_graph(ostc_fr_ctx c, bool batched) {
+  if (batched ? c->bgexec : c->gexec) return OSTC_OK;
   long long n = 0;
   FR_HIP(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-  step(c, &n, nullptr);
+  if (batched) step_batched(c, &n, nullptr);
+  else step(c, &n, nullptr);
   hipGraph_t g = nullptr;
   FR_HIP(c, hipStreamEndCapture(c->st, &g));
-  c->graph = g;
-  FR_HIP(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
-  c->graph_launches = n;
+  (batched ? c->bgraph : c->graph) = g;
+  FR_HIP(c, hipGraphInstantiate(batched ? &c->bgexec : &c->gexec, g, nullptr, nullptr, 0));
+  (batched ? c->bgraph_launches : c->graph_launches) = n;
   return OSTC_OK;
 }
 
@@ -654,6 +697,8 @@ int ostc_fr_finalize(ostc_fr_ctx c) {
                              hipMemcpyHostToDevice, c->st));
     FR_HIP(c, hipStreamSynchronize(c->st));
   }
+  c->h_args.clear();
+  for (int b = 0; b < (int)c->blocks.size(); ++b) c->h_args.push_back(args(c, b));
   long long n = 0;
   launch_dt_set(c->d_dt, c->s.fixed_time_step ? c->s.dt : 1e6, c->st, &n);
   FR_HIP(c, hipEventCreate(&c->ev_a));
@@ -722,11 +767,14 @@ int ostc_fr_initialize(ostc_fr_ctx c) {
 int ostc_fr_step(ostc_fr_ctx c, int n_steps, int mode, double* last_dt) {
   if (int r = check_ready(c)) return r;
   if (n_steps < 0) return fail(c, OSTC_ERR_ARG, "negative step count");
-  if (mode != OSTC_FR_MODE_DIRECT && mode != OSTC_FR_MODE_GRAPH && mode != OSTC_FR_MODE_TIMED)
+  if (mode != OSTC_FR_MODE_DIRECT && mode != OSTC_FR_MODE_GRAPH && mode != OSTC_FR_MODE_TIMED &&
+      mode != OSTC_FR_MODE_BATCHED && mode != OSTC_FR_MODE_BATCHED_TIMED)
     return fail(c, OSTC_ERR_ARG, "unknown mode");
   long long n = 0;
   KernelTimer* tm = nullptr;
-  if (mode == OSTC_FR_MODE_TIMED) {
+  const bool batched = mode == OSTC_FR_MODE_BATCHED || mode == OSTC_FR_MODE_BATCHED_TIMED;
+  const bool graph = mode == OSTC_FR_MODE_GRAPH || mode == OSTC_FR_MODE_BATCHED;
+  if (mode == OSTC_FR_MODE_TIMED || mode == OSTC_FR_MODE_BATCHED_TIMED) {
     if (!c->timer) {
       c->timer = new KernelTimer();
       for (int i = 0; i < KernelTimer::kMax; ++i) FR_HIP(c, hipEventCreate(&c->timer->ev[i]));
@@ -735,14 +783,16 @@ int ostc_fr_step(ostc_fr_ctx c, int n_steps, int mode, double* last_dt) {
     tm->n = 0;
     tm->ok = true;
   }
-  if (mode == OSTC_FR_MODE_GRAPH) {
-    if (int r = build_graph(c)) return r;
+  if (graph) {
+    if (int r = build_graph(c, batched)) return r;
   }
   FR_HIP(c, hipEventRecord(c->ev_a, c->st));
   for (int i = 0; i < n_steps; ++i) {
-    if (mode == OSTC_FR_MODE_GRAPH) {
-      FR_HIP(c, hipGraphLaunch(c->gexec, c->st));
-      n += c->graph_launches;
+    if (graph) {
+      FR_HIP(c, hipGraphLaunch(batched ? c->bgexec : c->gexec, c->st));
+      n += batched ? c->bgraph_launches : c->graph_launches;
+    } else if (batched) {
+      step_batched(c, &n, tm);
     } else {
       step(c, &n, tm);
     }

@@ -146,6 +146,17 @@ __device__ __forceinline__ void cv_point(const FrArgs& A, long long p) {
       total_energy<NS>(A.T, rho, B.bv[p + s], B.bv[p + 2 * s], B.bv[p + 3 * s], B.bv[p + 4 * s], B.bv[p + 5 * s], Y);
 }
 
+// The block and block-local workgroup of a batched launch (FrBatch): a
+// linear search of the uniform prefix table, at most kFrBatchMax entries.
+struct WgPos { int b, x, y, z; };
+__device__ __forceinline__ WgPos wg_pos(const FrBatch& W) {
+  const int w = blockIdx.x;
+  int b = 0;
+  while (b + 1 < W.nb && W.first[b + 1] <= w) ++b;
+  const int local = w - W.first[b], gx = W.gx[b], gy = W.gy[b];
+  return WgPos{b, local % gx, (local / gx) % gy, local / (gx * gy)};
+}
+
 // ------------------------------------------------------------ point kernels
 template <int NS>
 __global__ __launch_bounds__(kT) void k_cv_from_bv(FrArgs A) {
@@ -156,9 +167,9 @@ __global__ __launch_bounds__(kT) void k_cv_from_bv(FrArgs A) {
 // update_physical_properties<FR>, src/FieldOperation.cuh:107-140, over the
 // padded domain
 template <int NS>
-__global__ __launch_bounds__(kT) void k_props(FrArgs A) {
+__device__ __forceinline__ void props_body(const FrArgs& A, const int wx, const int wy, const int wz) {
   const FrBlockDev& B = A.B;
-  const long long p = (long long)blockIdx.x * kT + threadIdx.x;
+  const long long p = (long long)wx * kT + threadIdx.x;
   if (p >= B.sz) return;
   const long long s = B.sz;
   const double t = B.bv[p + 5 * s];
@@ -187,13 +198,24 @@ __global__ __launch_bounds__(kT) void k_props(FrArgs A) {
   for (int l = 0; l < NS; ++l) B.rhod[p + l * s] = rd[l];
   B.mach[p] = V / c;
 }
+template <int NS>
+__global__ __launch_bounds__(kT) void k_props(FrArgs A) {
+  props_body<NS>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+template <int NS>
+__global__ __launch_bounds__(kT) void k_props_batched(FrBatch W) {
+  const WgPos w = wg_pos(W);
+  FrArgs A = W.common;
+  A.B = W.blocks[w.b];
+  props_body<NS>(A, w.x, w.y, w.z);
+}
 
 // dq = 0, then the chemical source J omega on the species rows
 // (cudaMemset + compute_source<FR>, src/RK.cuh:114-118, src/SourceTerm.cuh:11-35)
 template <int NS>
-__global__ __launch_bounds__(kT) void k_source(FrArgs A) {
+__device__ __forceinline__ void source_body(const FrArgs& A, const int wx, const int wy, const int wz) {
   const FrBlockDev& B = A.B;
-  const long long q = (long long)blockIdx.x * kT + threadIdx.x;
+  const long long q = (long long)wx * kT + threadIdx.x;
   if (q >= B.n0) return;
   const int i = (int)(q % B.mx), j = (int)((q / B.mx) % B.my), k = (int)(q / ((long long)B.mx * B.my));
   const long long p = pid(B, i, j, k);
@@ -213,15 +235,26 @@ __global__ __launch_bounds__(kT) void k_source(FrArgs A) {
     for (int l = 0; l < NS; ++l) B.dq[q + (5 + l) * n0] = 0;
   }
 }
+template <int NS>
+__global__ __launch_bounds__(kT) void k_source(FrArgs A) {
+  source_body<NS>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+template <int NS>
+__global__ __launch_bounds__(kT) void k_source_batched(FrBatch W) {
+  const WgPos w = wg_pos(W);
+  FrArgs A = W.common;
+  A.B = W.blocks[w.b];
+  source_body<NS>(A, w.x, w.y, w.z);
+}
 
 __global__ void k_dt_set(double* dt, double v) { *dt = v; }
 
 // local_time_step<FR, Laminar>, src/TimeAdvanceFunc.cuh:28-83, and the MIN of
 // global_time_step: dt_local per cell, a 64-lane shuffle MIN per wave and one
 // 64-bit atomicMin on the bit pattern of a positive double.
-__global__ __launch_bounds__(kT) void k_local_dt(FrArgs A, unsigned long long* dt_bits) {
+__device__ __forceinline__ void local_dt_body(const FrArgs& A, unsigned long long* dt_bits, const int wx, const int wy, const int wz) {
   const FrBlockDev& B = A.B;
-  const long long q = (long long)blockIdx.x * kT + threadIdx.x;
+  const long long q = (long long)wx * kT + threadIdx.x;
   double dtl = 1e6;
   if (q < B.n0) {
     const int i = (int)(q % B.mx), j = (int)((q / B.mx) % B.my), k = (int)(q / ((long long)B.mx * B.my));
@@ -249,12 +282,21 @@ __global__ __launch_bounds__(kT) void k_local_dt(FrArgs A, unsigned long long* d
   for (int o = 32; o > 0; o >>= 1) dtl = fmin(dtl, __shfl_xor(dtl, o, 64));
   if ((threadIdx.x & 63) == 0) atomicMin(dt_bits, (unsigned long long)__double_as_longlong(dtl));
 }
+__global__ __launch_bounds__(kT) void k_local_dt(FrArgs A, unsigned long long* dt_bits) {
+  local_dt_body(A, dt_bits, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+__global__ __launch_bounds__(kT) void k_local_dt_batched(FrBatch W, unsigned long long* dt_bits) {
+  const WgPos w = wg_pos(W);
+  FrArgs A = W.common;
+  A.B = W.blocks[w.b];
+  local_dt_body(A, dt_bits, w.x, w.y, w.z);
+}
 
 // update_cv_and_bv_rk<FR>, src/RK.cuh:230-284 + compute_temperature_and_pressure
 template <int NS>
-__global__ __launch_bounds__(kT) void k_update(FrArgs A, double ra, double rb, double rc) {
+__device__ __forceinline__ void update_body(const FrArgs& A, double ra, double rb, double rc, const int wx, const int wy, const int wz) {
   const FrBlockDev& B = A.B;
-  const long long q = (long long)blockIdx.x * kT + threadIdx.x;
+  const long long q = (long long)wx * kT + threadIdx.x;
   if (q >= B.n0) return;
   const int i = (int)(q % B.mx), j = (int)((q / B.mx) % B.my), k = (int)(q / ((long long)B.mx * B.my));
   const long long p = pid(B, i, j, k), s = B.sz, n0 = B.n0;
@@ -286,6 +328,17 @@ __global__ __launch_bounds__(kT) void k_update(FrArgs A, double ra, double rb, d
   const double t = temperature<NS>(A.T, e, Y, gas_const, B.bv[p + 5 * s]);
   B.bv[p + 5 * s] = t;
   B.bv[p + 4 * s] = cv[0] * t * gas_const;
+}
+template <int NS>
+__global__ __launch_bounds__(kT) void k_update(FrArgs A, double ra, double rb, double rc) {
+  update_body<NS>(A, ra, rb, rc, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+template <int NS>
+__global__ __launch_bounds__(kT) void k_update_batched(FrBatch W, double ra, double rb, double rc) {
+  const WgPos w = wg_pos(W);
+  FrArgs A = W.common;
+  A.B = W.blocks[w.b];
+  update_body<NS>(A, ra, rb, rc, w.x, w.y, w.z);
 }
 
 // apply_outflow / apply_periodic with scalars, src/BoundCond.cuh:205-233,
@@ -614,7 +667,7 @@ __device__ __forceinline__ void split_fluxes(const FrArgs& A, double* Fp, double
 // flux, src/WENO.cu:1455-1536; the characteristic schemes run in
 // k_inviscid_ch.
 template <int DIR, int NS, int SCHEME>
-__global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
+__device__ __forceinline__ void inviscid_body(const FrArgs& A, const int wx, const int wy, const int wz) {
   // WENO-Z5 (51, 52): 3 ghost layers, 5-point candidates; WENO-Z7 (71, 72): 4 and 7
   constexpr int G = (SCHEME == 71 || SCHEME == 72) ? 4 : 3, W = 2 * G - 1;
   constexpr int NV = 5 + NS;
@@ -626,9 +679,9 @@ __global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
   const int ext[3] = {B.mx, B.my, B.mz};
   const int m = ext[DIR];
   const int across = DIR == 0 ? 1 : 0;
-  const int col0 = blockIdx.y * TW;
-  const int third = blockIdx.z;
-  const int t0 = blockIdx.x * TS;  // first cell of the tile along DIR
+  const int col0 = wy * TW;
+  const int third = wz;
+  const int t0 = wx * TS;  // first cell of the tile along DIR
   const long long s = B.sz;
   const int tid = threadIdx.x;
 
@@ -714,6 +767,17 @@ __global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
       B.dq[q0 + l * n0] -= fcs[l * (F * TW) + col * F + f] - fcs[l * (F * TW) + col * F + f - 1];
   }
 }
+template <int DIR, int NS, int SCHEME>
+__global__ __launch_bounds__(kT) void k_inviscid(FrArgs A) {
+  inviscid_body<DIR, NS, SCHEME>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+template <int DIR, int NS, int SCHEME>
+__global__ __launch_bounds__(kT) void k_inviscid_batched(FrBatch W) {
+  const WgPos w = wg_pos(W);
+  FrArgs A = W.common;
+  A.B = W.blocks[w.b];
+  inviscid_body<DIR, NS, SCHEME>(A, w.x, w.y, w.z);
+}
 
 // Tiles of the characteristic sweep: 128 faces per 256-thread workgroup,
 // two threads per face.
@@ -732,7 +796,7 @@ template <> struct TileCh<2> { static constexpr int TW = 8, F = 16; };
 // same as one thread computing every field.  Both halves form the face's Roe
 // state themselves.
 template <int DIR, int NS, int SCHEME>
-__global__ __launch_bounds__(kT) void k_inviscid_ch(FrArgs A) {
+__device__ __forceinline__ void inviscid_ch_body(const FrArgs& A, const int wx, const int wy, const int wz) {
   constexpr int G = SCHEME == 72 ? 4 : 3, W = 2 * G - 1;
   // scheme 52 keeps one more point on the left for the reference's stencil
   // of the acoustic/entropy fields (A.ch_ref: F+ at i-3..i+1, F- at i-2..i+2
@@ -749,9 +813,9 @@ __global__ __launch_bounds__(kT) void k_inviscid_ch(FrArgs A) {
   const int ext[3] = {B.mx, B.my, B.mz};
   const int m = ext[DIR];
   const int across = DIR == 0 ? 1 : 0;
-  const int col0 = blockIdx.y * TW;
-  const int third = blockIdx.z;
-  const int t0 = blockIdx.x * TS;
+  const int col0 = wy * TW;
+  const int third = wz;
+  const int t0 = wx * TS;
   const long long s = B.sz;
   const int tid = threadIdx.x;
 
@@ -926,13 +990,24 @@ __global__ __launch_bounds__(kT) void k_inviscid_ch(FrArgs A) {
       if (l >= v_lo && l < v_hi) B.dq[q0 + l * n0] -= fcs[l * NF + col * F + f] - fcs[l * NF + col * F + f - 1];
   }
 }
+template <int DIR, int NS, int SCHEME>
+__global__ __launch_bounds__(kT) void k_inviscid_ch(FrArgs A) {
+  inviscid_ch_body<DIR, NS, SCHEME>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+template <int DIR, int NS, int SCHEME>
+__global__ __launch_bounds__(kT) void k_inviscid_ch_batched(FrBatch W) {
+  const WgPos w = wg_pos(W);
+  FrArgs A = W.common;
+  A.B = W.blocks[w.b];
+  inviscid_ch_body<DIR, NS, SCHEME>(A, w.x, w.y, w.z);
+}
 
 // One inviscid sweep along DIR with reconstructed primitives and an
 // approximate Riemann solver: compute_convective_term_pv_1D<FR>,
 // src/InviscidScheme.cu:50-165 (AUSM+ for scheme 3, HLLC for 4).  The tile
 // holds rho, u, v, w, p, Y_k of TS + 4 points per column in LDS.
 template <int DIR, int NS>
-__global__ __launch_bounds__(kT) void k_inviscid_pv(FrArgs A) {
+__device__ __forceinline__ void inviscid_pv_body(const FrArgs& A, const int wx, const int wy, const int wz) {
   constexpr int NV = 5 + NS;
   constexpr int TW = Tile<DIR>::TW, F = Tile<DIR>::F, TS = F - 1, P = TS + 4, NP = P * TW;
   extern __shared__ double lds[];
@@ -940,9 +1015,9 @@ __global__ __launch_bounds__(kT) void k_inviscid_pv(FrArgs A) {
   const int ext[3] = {B.mx, B.my, B.mz};
   const int m = ext[DIR];
   const int across = DIR == 0 ? 1 : 0;
-  const int col0 = blockIdx.y * TW;
-  const int third = blockIdx.z;
-  const int t0 = blockIdx.x * TS;
+  const int col0 = wy * TW;
+  const int third = wz;
+  const int t0 = wx * TS;
   const long long s = B.sz;
   const int tid = threadIdx.x;
   for (int e = tid; e < NP; e += kT) {  // points t0-2 .. t0+TS+1
@@ -1028,12 +1103,23 @@ __global__ __launch_bounds__(kT) void k_inviscid_pv(FrArgs A) {
       B.dq[q0 + l * n0] -= fcs[l * (F * TW) + col * F + f] - fcs[l * (F * TW) + col * F + f - 1];
   }
 }
+template <int DIR, int NS>
+__global__ __launch_bounds__(kT) void k_inviscid_pv(FrArgs A) {
+  inviscid_pv_body<DIR, NS>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+template <int DIR, int NS>
+__global__ __launch_bounds__(kT) void k_inviscid_pv_batched(FrBatch W) {
+  const WgPos w = wg_pos(W);
+  FrArgs A = W.common;
+  A.B = W.blocks[w.b];
+  inviscid_pv_body<DIR, NS>(A, w.x, w.y, w.z);
+}
 
 // One viscous sweep along DIR: compute_{fv,gv,hv}_2nd_order<FR, Laminar>,
 // src/ViscousScheme.cuh:23-318, :320-616, :618-909; one thread per face,
 // face fluxes through LDS, dq += fv(right) - fv(left).
 template <int DIR, int NS>
-__global__ __launch_bounds__(kT) void k_viscous(FrArgs A) {
+__device__ __forceinline__ void viscous_body(const FrArgs& A, const int wx, const int wy, const int wz) {
   constexpr int NV = 5 + NS;
   constexpr int TW = Tile<DIR>::TW, F = Tile<DIR>::F, TS = F - 1;
   extern __shared__ double lds[];
@@ -1041,9 +1127,9 @@ __global__ __launch_bounds__(kT) void k_viscous(FrArgs A) {
   const int ext[3] = {B.mx, B.my, B.mz};
   const int m = ext[DIR];
   const int across = DIR == 0 ? 1 : 0;
-  const int col0 = blockIdx.y * TW;
-  const int third = blockIdx.z;
-  const int t0 = blockIdx.x * TS;
+  const int col0 = wy * TW;
+  const int third = wz;
+  const int t0 = wx * TS;
   const long long s = B.sz;
   const int tid = threadIdx.x;
   // face f of column col; along x consecutive lanes take consecutive faces,
@@ -1180,6 +1266,17 @@ __global__ __launch_bounds__(kT) void k_viscous(FrArgs A) {
       B.dq[q0 + l * n0] += lds[l * (F * TW) + col * F + f] - lds[l * (F * TW) + col * F + f - 1];
   }
 }
+template <int DIR, int NS>
+__global__ __launch_bounds__(kT) void k_viscous(FrArgs A) {
+  viscous_body<DIR, NS>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+template <int DIR, int NS>
+__global__ __launch_bounds__(kT) void k_viscous_batched(FrBatch W) {
+  const WgPos w = wg_pos(W);
+  FrArgs A = W.common;
+  A.B = W.blocks[w.b];
+  viscous_body<DIR, NS>(A, w.x, w.y, w.z);
+}
 
 unsigned nblk(long long n) { return (unsigned)((n + kT - 1) / kT); }
 
@@ -1197,66 +1294,136 @@ __global__ __launch_bounds__(kT) void k_sponge_source(FrBlockDev B) {
   for (int l = 0; l < 5 + NS; ++l) B.dq[t + l * B.n0] -= sigma * (B.cv[p + l * B.sz] - B.sp_mean[t + l * B.n0]);
 }
 
-template <int NS>
-void launch_rhs_t(const FrArgs& A, hipStream_t st, long long* n, KernelTimer* tm) {
-  const FrBlockDev& B = A.B;
+// Grids and LDS bytes of the sweeps: a component-wise or viscous tile is
+// TW columns x (F - 1) cells of one pencil (Tile<D>), a characteristic tile
+// TileCh<D>; the third index of the grid runs over the remaining axis.
+template <int D, class T>
+dim3 sweep_grid(const FrBlockDev& B) {
+  constexpr int TW = T::TW, TS = T::F - 1;
   const int ext[3] = {B.mx, B.my, B.mz};
+  const int across = D == 0 ? 1 : 0, third = D == 2 ? 1 : 2;
+  return dim3((ext[D] + TS - 1) / TS, (ext[across] + TW - 1) / TW, ext[third]);
+}
+dim3 point_grid(long long n) { return dim3(nblk(n)); }
+
+template <int D, int NS>
+size_t inviscid_lds(int scheme) {
+  if (scheme == 52 || scheme == 72) {
+    constexpr int TWc = TileCh<D>::TW, Fc = TileCh<D>::F, TSc = Fc - 1;
+    const int Pc = TSc + 2 * (scheme == 72 ? 4 : 3) + (scheme == 52 ? 1 : 0);
+    return ((size_t)2 * (5 + NS) * Pc * TWc + (size_t)(5 + NS) * Fc * TWc) * sizeof(double);
+  }
+  constexpr int TW = Tile<D>::TW, F = Tile<D>::F, TS = F - 1;
+  if (scheme == 51 || scheme == 71) return (size_t)2 * (5 + NS) * (TS + 2 * (scheme == 71 ? 4 : 3)) * TW * sizeof(double);
+  return (size_t)(5 + NS) * std::max(TS + 4, F) * TW * sizeof(double);  // primitive reconstruction (3, 4)
+}
+
+// The FrBatch of one kernel kind over the nb (<= kFrBatchMax) blocks of h.
+template <class GridOf>
+FrBatch make_batch(const FrArgs* h, int nb, GridOf grid_of, unsigned* total) {
+  FrBatch W{};
+  W.common = h[0];
+  W.nb = nb;
+  long long acc = 0;
+  for (int b = 0; b < nb; ++b) {
+    const dim3 g = grid_of(h[b].B);
+    W.blocks[b] = h[b].B;
+    W.first[b] = (int)acc;
+    W.gx[b] = (int)g.x;
+    W.gy[b] = (int)g.y;
+    acc += (long long)g.x * g.y * g.z;
+  }
+  W.first[nb] = (int)acc;
+  *total = (unsigned)acc;
+  return W;
+}
+
+// The stage residual of the nb blocks of h: one launch per kernel kind and
+// block (batched false, the reference's loop) or per kernel kind and
+// kFrBatchMax blocks (batched true).
+template <int NS>
+void launch_rhs_t(const FrArgs* h, int nb, bool batched, hipStream_t st, long long* n, KernelTimer* tm) {
+  const FrArgs& A = h[0];  // scheme switches are the same for every block
+  const int step = batched ? kFrBatchMax : 1;
   auto rec = [&](int which, bool end) {
     if (tm) tm->mark(which, end, st);
   };
-  rec(KT_SOURCE, false);
-  hipLaunchKernelGGL(k_source<NS>, dim3(nblk(B.n0)), dim3(kT), 0, st, A);
-  ++*n;
-  if (B.sp_sigma) {
-    hipLaunchKernelGGL(k_sponge_source<NS>, dim3(nblk(B.n0)), dim3(kT), 0, st, B);
+  unsigned total = 0;
+  for (int c0 = 0; c0 < nb; c0 += step) {
+    const int nc = std::min(step, nb - c0);
+    const FrArgs* hc = h + c0;
+    rec(KT_SOURCE, false);
+    if (batched) {
+      const FrBatch W = make_batch(hc, nc, [](const FrBlockDev& B) { return point_grid(B.n0); }, &total);
+      hipLaunchKernelGGL(k_source_batched<NS>, dim3(total), dim3(kT), 0, st, W);
+    } else {
+      hipLaunchKernelGGL(k_source<NS>, point_grid(hc[0].B.n0), dim3(kT), 0, st, hc[0]);
+    }
     ++*n;
-  }
-  rec(KT_SOURCE, true);
-  auto inv = [&](auto dirc) {
-    constexpr int D = decltype(dirc)::value;
-    constexpr int TW = Tile<D>::TW, F = Tile<D>::F, TS = F - 1;
-    const int P = TS + 2 * (A.scheme == 71 || A.scheme == 72 ? 4 : 3);
-    const int across = D == 0 ? 1 : 0, third = D == 2 ? 1 : 2;
-    dim3 grid((ext[D] + TS - 1) / TS, (ext[across] + TW - 1) / TW, ext[third]);
-    const size_t lds = (size_t)2 * (5 + NS) * P * TW * sizeof(double);
-    rec(KT_INV_X + D, false);
-    if (A.scheme == 52 || A.scheme == 72) {
-      constexpr int TWc = TileCh<D>::TW, Fc = TileCh<D>::F, TSc = Fc - 1;
-      const int Pc = TSc + 2 * (A.scheme == 72 ? 4 : 3) + (A.scheme == 52 ? 1 : 0);
-      dim3 gridc((ext[D] + TSc - 1) / TSc, (ext[across] + TWc - 1) / TWc, ext[third]);
-      const size_t ldsc = ((size_t)2 * (5 + NS) * Pc * TWc + (size_t)(5 + NS) * Fc * TWc) * sizeof(double);
-      if (A.scheme == 52)
-        hipLaunchKernelGGL((k_inviscid_ch<D, NS, 52>), gridc, dim3(kT), ldsc, st, A);
-      else
-        hipLaunchKernelGGL((k_inviscid_ch<D, NS, 72>), gridc, dim3(kT), ldsc, st, A);
-    } else if (A.scheme == 51)
-      hipLaunchKernelGGL((k_inviscid<D, NS, 51>), grid, dim3(kT), lds, st, A);
-    else if (A.scheme == 71)
-      hipLaunchKernelGGL((k_inviscid<D, NS, 71>), grid, dim3(kT), lds, st, A);
-    else  // primitive reconstruction + Riemann solver (3, 4): TS + 4 points, or F faces of fluxes
-      hipLaunchKernelGGL((k_inviscid_pv<D, NS>), grid, dim3(kT),
-                         (size_t)(5 + NS) * std::max(TS + 4, F) * TW * sizeof(double), st, A);
-    rec(KT_INV_X + D, true);
-    ++*n;
-  };
-  inv(std::integral_constant<int, 0>{});
-  inv(std::integral_constant<int, 1>{});
-  inv(std::integral_constant<int, 2>{});
-  if (A.viscous) {
-    auto vis = [&](auto dirc) {
+    for (int b = 0; b < nc; ++b)
+      if (hc[b].B.sp_sigma) {
+        hipLaunchKernelGGL(k_sponge_source<NS>, point_grid(hc[b].B.n0), dim3(kT), 0, st, hc[b].B);
+        ++*n;
+      }
+    rec(KT_SOURCE, true);
+    auto inv = [&](auto dirc) {
       constexpr int D = decltype(dirc)::value;
-      constexpr int TW = Tile<D>::TW, F = Tile<D>::F, TS = F - 1;
-      const int across = D == 0 ? 1 : 0, third = D == 2 ? 1 : 2;
-      dim3 grid((ext[D] + TS - 1) / TS, (ext[across] + TW - 1) / TW, ext[third]);
-      const size_t lds = (size_t)(5 + NS) * F * TW * sizeof(double);
-      rec(KT_VIS_X + D, false);
-      hipLaunchKernelGGL((k_viscous<D, NS>), grid, dim3(kT), lds, st, A);
-      rec(KT_VIS_X + D, true);
+      const size_t lds = inviscid_lds<D, NS>(A.scheme);
+      rec(KT_INV_X + D, false);
+      const bool ch = A.scheme == 52 || A.scheme == 72;
+      FrBatch W{};
+      if (batched) {
+        W = ch ? make_batch(hc, nc, sweep_grid<D, TileCh<D>>, &total)
+               : make_batch(hc, nc, sweep_grid<D, Tile<D>>, &total);
+      }
+      const FrArgs& Ab = hc[0];
+      const dim3 grid = batched ? dim3(total) : (ch ? sweep_grid<D, TileCh<D>>(Ab.B) : sweep_grid<D, Tile<D>>(Ab.B));
+      switch (A.scheme) {
+        case 52:
+          if (batched) hipLaunchKernelGGL((k_inviscid_ch_batched<D, NS, 52>), grid, dim3(kT), lds, st, W);
+          else hipLaunchKernelGGL((k_inviscid_ch<D, NS, 52>), grid, dim3(kT), lds, st, Ab);
+          break;
+        case 72:
+          if (batched) hipLaunchKernelGGL((k_inviscid_ch_batched<D, NS, 72>), grid, dim3(kT), lds, st, W);
+          else hipLaunchKernelGGL((k_inviscid_ch<D, NS, 72>), grid, dim3(kT), lds, st, Ab);
+          break;
+        case 51:
+          if (batched) hipLaunchKernelGGL((k_inviscid_batched<D, NS, 51>), grid, dim3(kT), lds, st, W);
+          else hipLaunchKernelGGL((k_inviscid<D, NS, 51>), grid, dim3(kT), lds, st, Ab);
+          break;
+        case 71:
+          if (batched) hipLaunchKernelGGL((k_inviscid_batched<D, NS, 71>), grid, dim3(kT), lds, st, W);
+          else hipLaunchKernelGGL((k_inviscid<D, NS, 71>), grid, dim3(kT), lds, st, Ab);
+          break;
+        default:  // primitive reconstruction + Riemann solver (2, 3, 4)
+          if (batched) hipLaunchKernelGGL((k_inviscid_pv_batched<D, NS>), grid, dim3(kT), lds, st, W);
+          else hipLaunchKernelGGL((k_inviscid_pv<D, NS>), grid, dim3(kT), lds, st, Ab);
+      }
+      rec(KT_INV_X + D, true);
       ++*n;
     };
-    vis(std::integral_constant<int, 0>{});
-    vis(std::integral_constant<int, 1>{});
-    vis(std::integral_constant<int, 2>{});
+    inv(std::integral_constant<int, 0>{});
+    inv(std::integral_constant<int, 1>{});
+    inv(std::integral_constant<int, 2>{});
+    if (A.viscous) {
+      auto vis = [&](auto dirc) {
+        constexpr int D = decltype(dirc)::value;
+        const size_t lds = (size_t)(5 + NS) * Tile<D>::F * Tile<D>::TW * sizeof(double);
+        rec(KT_VIS_X + D, false);
+        if (batched) {
+          const FrBatch W = make_batch(hc, nc, sweep_grid<D, Tile<D>>, &total);
+          hipLaunchKernelGGL((k_viscous_batched<D, NS>), dim3(total), dim3(kT), lds, st, W);
+        } else {
+          const dim3 grid = sweep_grid<D, Tile<D>>(hc[0].B);
+          hipLaunchKernelGGL((k_viscous<D, NS>), grid, dim3(kT), lds, st, hc[0]);
+        }
+        rec(KT_VIS_X + D, true);
+        ++*n;
+      };
+      vis(std::integral_constant<int, 0>{});
+      vis(std::integral_constant<int, 1>{});
+      vis(std::integral_constant<int, 2>{});
+    }
   }
 }
 
@@ -1275,7 +1442,46 @@ bool supported_ns(int ns) { return ns == 9; }
   }
 
 void launch_rhs(const FrArgs& A, int ns, hipStream_t st, long long* n, KernelTimer* tm) {
-  OSTC_FR_DISPATCH(ns, launch_rhs_t<NS>(A, st, n, tm))
+  OSTC_FR_DISPATCH(ns, launch_rhs_t<NS>(&A, 1, false, st, n, tm))
+}
+
+void launch_rhs_batched(const FrArgs* h, int nb, int ns, hipStream_t st, long long* n, KernelTimer* tm) {
+  OSTC_FR_DISPATCH(ns, launch_rhs_t<NS>(h, nb, true, st, n, tm))
+}
+
+void launch_update_batched(const FrArgs* h, int nb, int ns, double ra, double rb, double rc, hipStream_t st,
+                           long long* n, KernelTimer* tm) {
+  if (tm) tm->mark(KT_UPDATE, false, st);
+  for (int c0 = 0; c0 < nb; c0 += kFrBatchMax) {
+    unsigned total = 0;
+    const FrBatch W = make_batch(h + c0, std::min(kFrBatchMax, nb - c0),
+                                 [](const FrBlockDev& B) { return point_grid(B.n0); }, &total);
+    OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_update_batched<NS>, dim3(total), dim3(kT), 0, st, W, ra, rb, rc))
+    ++*n;
+  }
+  if (tm) tm->mark(KT_UPDATE, true, st);
+}
+
+void launch_props_batched(const FrArgs* h, int nb, int ns, hipStream_t st, long long* n, KernelTimer* tm) {
+  if (tm) tm->mark(KT_PROPS, false, st);
+  for (int c0 = 0; c0 < nb; c0 += kFrBatchMax) {
+    unsigned total = 0;
+    const FrBatch W = make_batch(h + c0, std::min(kFrBatchMax, nb - c0),
+                                 [](const FrBlockDev& B) { return point_grid(B.sz); }, &total);
+    OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_props_batched<NS>, dim3(total), dim3(kT), 0, st, W))
+    ++*n;
+  }
+  if (tm) tm->mark(KT_PROPS, true, st);
+}
+
+void launch_local_dt_batched(const FrArgs* h, int nb, unsigned long long* dt_bits, hipStream_t st, long long* n) {
+  for (int c0 = 0; c0 < nb; c0 += kFrBatchMax) {
+    unsigned total = 0;
+    const FrBatch W = make_batch(h + c0, std::min(kFrBatchMax, nb - c0),
+                                 [](const FrBlockDev& B) { return point_grid(B.n0); }, &total);
+    hipLaunchKernelGGL(k_local_dt_batched, dim3(total), dim3(kT), 0, st, W, dt_bits);
+    ++*n;
+  }
 }
 
 void launch_props(const FrArgs& A, int ns, hipStream_t st, long long* n, KernelTimer* tm) {

@@ -1,7 +1,8 @@
 // ostc_kernels_fused.hip -- one persistent kernel per RK stage over the tiles
 // of every block: inviscid WENO fluxes along x, y, z and second-order viscous
 // fluxes staged through LDS, the residual dq kept in registers, and the
-// SSP-RK3 update, primitives and Sutherland viscosity written out.  Reads
+// SSP-RK3 update, primitives and Sutherland viscosity written out (with the
+// sponge layer's relaxation term first where a block has one).  Reads
 // ring slot `in` (state incl. ghosts) and qn = slot 0, writes the interior of
 // slot `out`; ghost layers are filled afterwards by the op launches.
 //
@@ -68,9 +69,46 @@ struct TileCtx {
   bool valid;          // cell inside the block
 };
 
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+
+// What one stage reads and writes of a block, with the arrays as pointers
+// into global memory.  The block table is itself read from memory, so the
+// pointers it holds are generic to the compiler, which would then access
+// every array with flat instructions: those wait on the LDS counter as well
+// and need 64-bit addresses per lane.
+struct StageBlock {
+  int mx, my, mz, g, sx, sy, sz;  // sz fits 32 bits: the host refuses 9 sz >= 2^31
+  int ntx, nty, tile_base, sp_jac;
+  long long n0;
+  gptr<const double> cvi, bvi, muli, qn, jac, met, sp_sigma, sp_mean;
+  gptr<double> cvo, bvo, mulo, mach, dq;
+};
+
+__device__ __forceinline__ StageBlock stage_block(const DevBlock& B, int in, int out) {
+  StageBlock S;
+  S.mx = B.mx; S.my = B.my; S.mz = B.mz; S.g = B.g; S.sx = B.sx; S.sy = B.sy; S.sz = (int)B.sz;
+  S.ntx = B.ntx; S.nty = B.nty; S.tile_base = B.tile_base; S.sp_jac = B.sp_jac;
+  S.n0 = B.n0;
+  S.cvi = (gptr<const double>)B.cv[in];
+  S.bvi = (gptr<const double>)B.bv[in];
+  S.muli = (gptr<const double>)B.mul[in];
+  S.qn = (gptr<const double>)B.cv[0];
+  S.jac = (gptr<const double>)B.jac;
+  S.met = (gptr<const double>)B.met;
+  S.sp_sigma = (gptr<const double>)B.sp_sigma;
+  S.sp_mean = (gptr<const double>)B.sp_mean;
+  S.cvo = (gptr<double>)B.cv[out];
+  S.bvo = (gptr<double>)B.bv[out];
+  S.mulo = (gptr<double>)B.mul[out];
+  S.mach = (gptr<double>)B.mach;
+  S.dq = (gptr<double>)B.dq;
+  return S;
+}
+
 // 32-bit offset of node (i, j, k) in a padded plane; the host refuses blocks
 // whose 9 metric planes would not fit 2^31 doubles.
-__device__ __forceinline__ int poff(const DevBlock& b, int i, int j, int k) {
+__device__ __forceinline__ int poff(const StageBlock& b, int i, int j, int k) {
   return ((k + b.g) * b.sy + (j + b.g)) * b.sx + (i + b.g);
 }
 
@@ -78,7 +116,7 @@ __device__ __forceinline__ int poff(const DevBlock& b, int i, int j, int k) {
 // face L+1/2) and whether the face lies in the block (faces past the end of a
 // column, or in columns outside the block, are never read).
 template <int TX, int TY, int TZ, int D>
-__device__ __forceinline__ bool face_node(const DevBlock& B, const TileCtx& t, int f, int ijk[3]) {
+__device__ __forceinline__ bool face_node(const StageBlock& B, const TileCtx& t, int f, int ijk[3]) {
   using Dr = Dir<TX, TY, TZ, D>;
   int col, sf, a, b;
   Dr::face(f, col, sf);
@@ -97,7 +135,7 @@ __device__ __forceinline__ bool face_node(const DevBlock& B, const TileCtx& t, i
 // spread over (variable, face) items, 5 NF of them for NT threads, so every
 // wave carries the same share whatever NF mod NT is.
 template <int TX, int TY, int TZ, int SCHEME, int D>
-__device__ __forceinline__ void inviscid_phase(double* smem, const DevBlock& B, const Params& P, int slot,
+__device__ __forceinline__ void inviscid_phase(double* smem, const StageBlock& B, const Params& P,
                                                const TileCtx& t, double dq[5]) {
   using Dr = Dir<TX, TY, TZ, D>;
   constexpr int NT = Dr::NT, R = Stencil<SCHEME>::R, W = Stencil<SCHEME>::W;
@@ -105,9 +143,9 @@ __device__ __forceinline__ void inviscid_phase(double* smem, const DevBlock& B, 
   constexpr int NI = 5 * NF, IPT = (NI + NT - 1) / NT;
   const int tid = threadIdx.x;
   const int origin = D == 0 ? t.i0 : (D == 1 ? t.j0 : t.k0);
-  const double* cv = B.cv[slot];
-  const double* bv = B.bv[slot];
-  const int s = (int)B.sz;
+  const gptr<const double> cv = B.cvi;
+  const gptr<const double> bv = B.bvi;
+  const int s = B.sz;
   double* sFp = smem;
   double* sFm = smem + 5 * NPTS;
   double* sEps = smem + 10 * NPTS;
@@ -209,13 +247,13 @@ __device__ __forceinline__ void inviscid_phase(double* smem, const DevBlock& B, 
 // written to sFv[l * NF + f]: u, v, w, T and mu from the staged tile box,
 // the metric and Jacobian of the two face cells from global memory.
 template <int TX, int TY, int TZ, int D>
-__device__ __forceinline__ void viscous_face(const double* sV, double* sFv, const DevBlock& B, const Params& P,
+__device__ __forceinline__ void viscous_face(const double* sV, double* sFv, const StageBlock& B, const Params& P,
                                              const TileCtx& t, int f) {
   using Dr = Dir<TX, TY, TZ, D>;
   constexpr int NF = Dr::NF, BX = TX + 2, BY = TY + 2, NB = (TX + 2) * (TY + 2) * (TZ + 2);
   int ijk[3];
   if (!face_node<TX, TY, TZ, D>(B, t, f, ijk)) return;
-  const int s = (int)B.sz;
+  const int s = B.sz;
   auto box = [&](int i, int j, int k) { return ((k - t.k0 + 1) * BY + (j - t.j0 + 1)) * BX + (i - t.i0 + 1); };
   auto pv = [&](int i, int j, int k, int l) {
     const int q = l == 5 ? 3 : l - 1;
@@ -268,16 +306,16 @@ __device__ __forceinline__ void viscous_face(const double* sV, double* sFv, cons
 // order (x, then y, then z).  The faces of all three directions form one
 // work list, so the threads share them evenly.
 template <int TX, int TY, int TZ>
-__device__ __forceinline__ void viscous_phase(double* smem, const DevBlock& B, const Params& P, int slot,
+__device__ __forceinline__ void viscous_phase(double* smem, const StageBlock& B, const Params& P,
                                               const TileCtx& t, double dq[5]) {
   constexpr int NT = TX * TY * TZ, BX = TX + 2, BY = TY + 2, NB = (TX + 2) * (TY + 2) * (TZ + 2);
   using D0 = Dir<TX, TY, TZ, 0>;
   using D1 = Dir<TX, TY, TZ, 1>;
   using D2 = Dir<TX, TY, TZ, 2>;
   constexpr int NF0 = D0::NF, NF1 = D1::NF, NF2 = D2::NF, NFV = NF0 + NF1 + NF2;
-  const double* bv = B.bv[slot];
-  const double* mu = B.mul[slot];
-  const int s = (int)B.sz;
+  const gptr<const double> bv = B.bvi;
+  const gptr<const double> mu = B.muli;
+  const int s = B.sz;
   for (int p = threadIdx.x; p < NB; p += NT) {
     const int a = p % BX, b = (p / BX) % BY, c = p / (BX * BY);
     const int i = t.i0 - 1 + a, j = t.j0 - 1 + b, k = t.k0 - 1 + c;
@@ -363,62 +401,64 @@ __global__ __launch_bounds__(TX * TY * TZ) __attribute__((amdgpu_waves_per_eu(WP
   for (int tile = first; tile < last; tile += step) {
     const int b = A.tile_block[tile];
     if (A.target_block >= 0 && b != A.target_block) continue;
-    // a reference into global memory: the ring slots are picked by runtime
-    // indices, which would push a private copy of the descriptor to scratch
-    const DevBlock& B = A.blocks[b];
+    // the descriptor is read from memory once per tile into the stage's view
+    const StageBlock B = stage_block(A.blocks[b], A.in, A.out);
     const int local = tile - B.tile_base;
     t.i0 = (local % B.ntx) * TX;
     t.j0 = ((local / B.ntx) % B.nty) * TY;
     t.k0 = (local / (B.ntx * B.nty)) * TZ;
     const int i = t.i0 + t.ti, j = t.j0 + t.tj, k = t.k0 + t.tk;
     t.valid = i < B.mx && j < B.my && k < B.mz;
+    const long long q0 = ((long long)k * B.my + j) * B.mx + i;  // interior index (dq, sponge planes)
     double dq[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (MODE != 2 && B.sp_sigma && t.valid) {
+      // sponge_layer_source (src/SpongeLayer.cu:68-160), the first term of the
+      // stage's dq as in compute_source: dq -= sigma (cv - mean)
+      const int p = poff(B, i, j, k), s = B.sz;
+      const double sigma = B.sp_jac ? B.sp_sigma[q0] * B.jac[p] : B.sp_sigma[q0];
+#pragma unroll
+      for (int l = 0; l < 5; ++l) dq[l] -= sigma * (B.cvi[p + l * s] - B.sp_mean[q0 + l * B.n0]);
+    }
     if (MODE != 2) {
-      inviscid_phase<TX, TY, TZ, SCHEME, 0>(smem, B, A.P, A.in, t, dq);
-      inviscid_phase<TX, TY, TZ, SCHEME, 1>(smem, B, A.P, A.in, t, dq);
-      inviscid_phase<TX, TY, TZ, SCHEME, 2>(smem, B, A.P, A.in, t, dq);
+      inviscid_phase<TX, TY, TZ, SCHEME, 0>(smem, B, A.P, t, dq);
+      inviscid_phase<TX, TY, TZ, SCHEME, 1>(smem, B, A.P, t, dq);
+      inviscid_phase<TX, TY, TZ, SCHEME, 2>(smem, B, A.P, t, dq);
     } else if (t.valid) {
       // split stage: the inviscid part of dq was written by the MODE 1 launch
-      const long long p0 = pidx0(B, i, j, k);
 #pragma unroll
-      for (int l = 0; l < 5; ++l) dq[l] = B.dq[p0 + l * B.n0];
+      for (int l = 0; l < 5; ++l) dq[l] = B.dq[q0 + l * B.n0];
     }
-    if (VISC) viscous_phase<TX, TY, TZ>(smem, B, A.P, A.in, t, dq);
+    if (VISC) viscous_phase<TX, TY, TZ>(smem, B, A.P, t, dq);
     if (!t.valid) continue;
-    const int p = poff(B, i, j, k), s = (int)B.sz;
+    const int p = poff(B, i, j, k), s = B.sz;
     if (MODE == 1) {
-      const long long p0 = pidx0(B, i, j, k);
 #pragma unroll
-      for (int l = 0; l < 5; ++l) B.dq[p0 + l * B.n0] = dq[l];
+      for (int l = 0; l < 5; ++l) B.dq[q0 + l * B.n0] = dq[l];
       continue;
     }
     // update_cv_and_bv_rk<Air> (src/RK.cuh:230-284) and the interior part of
     // update_physical_properties<Air> (src/FieldOperation.cuh:107-140)
     const double dt = *A.dt;
     const double dt_div_jac = dt / B.jac[p];
-    const double* qn = B.cv[0];
-    const double* cvi = B.cv[A.in];
-    double* cvo = B.cv[A.out];
     double c[5];
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
-      c[l] = A.ra * qn[p + l * s] + A.rb * cvi[p + l * s] + A.rc * dt_div_jac * dq[l];
-      cvo[p + l * s] = c[l];
+      c[l] = A.ra * B.qn[p + l * s] + A.rb * B.cvi[p + l * s] + A.rc * dt_div_jac * dq[l];
+      B.cvo[p + l * s] = c[l];
     }
-    double* bvo = B.bv[A.out];
     const double density_inv = 1.0 / c[0];
     const double u = c[1] * density_inv, v = c[2] * density_inv, w = c[3] * density_inv;
     const double V2 = u * u + v * v + w * w;
     const double pr = (kGamma - 1) * (c[4] - 0.5 * c[0] * V2);
     const double T = pr * kMwAir * density_inv / kRu;
-    bvo[p] = c[0];
-    bvo[p + s] = u;
-    bvo[p + 2 * s] = v;
-    bvo[p + 3 * s] = w;
-    bvo[p + 4 * s] = pr;
-    bvo[p + 5 * s] = T;
+    B.bvo[p] = c[0];
+    B.bvo[p + s] = u;
+    B.bvo[p + 2 * s] = v;
+    B.bvo[p + 3 * s] = w;
+    B.bvo[p + 4 * s] = pr;
+    B.bvo[p + 5 * s] = T;
     constexpr double c_temp{kGamma * kRu / kMwAir};
-    B.mul[A.out][p] = sutherland(T);
+    B.mulo[p] = sutherland(T);
     B.mach[p] = sqrt(V2) / sqrt(c_temp * T);
   }
 }
@@ -427,13 +467,13 @@ __global__ __launch_bounds__(TX * TY * TZ) __attribute__((amdgpu_waves_per_eu(WP
 // src/TimeAdvanceFunc.cu:39-89) over every block's interior, reduced with
 // 64-lane shuffles and one 64-bit atomicMin per workgroup on the bit pattern
 // of the (positive) time step.
-__device__ double local_dt_point_f(const DevBlock& B, const Params& P, long long p) {
-  const long long s = B.sz;
-  const double* m = B.met;
+__device__ double local_dt_point_f(const StageBlock& B, const Params& P, int p) {
+  const int s = B.sz;
+  const gptr<const double> m = B.met;
   const double gxi = sqrt(m[p] * m[p] + m[p + s] * m[p + s] + m[p + 2 * s] * m[p + 2 * s]);
   const double geta = sqrt(m[p + 3 * s] * m[p + 3 * s] + m[p + 4 * s] * m[p + 4 * s] + m[p + 5 * s] * m[p + 5 * s]);
   const double gzeta = sqrt(m[p + 6 * s] * m[p + 6 * s] + m[p + 7 * s] * m[p + 7 * s] + m[p + 8 * s] * m[p + 8 * s]);
-  const double* bv = B.bv[0];
+  const gptr<const double> bv = B.bvi;
   const double u = bv[p + s], v = bv[p + 2 * s], w = bv[p + 3 * s];
   const double U = u * m[p] + v * m[p + s] + w * m[p + 2 * s];
   const double V = u * m[p + 3 * s] + v * m[p + 4 * s] + w * m[p + 5 * s];
@@ -441,7 +481,7 @@ __device__ double local_dt_point_f(const DevBlock& B, const Params& P, long long
   const double a = sqrt(kGamma * kRu * bv[p + 5 * s] / kMwAir);
   const double sr_inv = (fabs(U) + a * gxi) + (fabs(V) + a * geta) + (fabs(Wc) + a * gzeta);
   const double coeff_1 = (kGamma > 4.0 / 3.0 ? kGamma : 4.0 / 3.0) / bv[p];
-  const double coeff_2 = B.mul[0][p] / P.prandtl;
+  const double coeff_2 = B.muli[p] / P.prandtl;
   double sr_visc = gxi * gxi + geta * geta;
   sr_visc += gzeta * gzeta;
   sr_visc *= coeff_1 * coeff_2;
@@ -458,11 +498,11 @@ __global__ __launch_bounds__(TX * TY * TZ) void k_dt_tiles(FusedArgs A, unsigned
   const int ti = tid % TX, tj = (tid / TX) % TY, tk = tid / (TX * TY);
   double v = 1e6;
   for (int tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
-    const DevBlock& B = A.blocks[A.tile_block[tile]];
+    const StageBlock B = stage_block(A.blocks[A.tile_block[tile]], 0, 0);  // the step's state, slot 0
     const int local = tile - B.tile_base;
     const int i = (local % B.ntx) * TX + ti, j = ((local / B.ntx) % B.nty) * TY + tj,
               k = (local / (B.ntx * B.nty)) * TZ + tk;
-    if (i < B.mx && j < B.my && k < B.mz) v = fmin(v, local_dt_point_f(B, A.P, pidx(B, i, j, k)));
+    if (i < B.mx && j < B.my && k < B.mz) v = fmin(v, local_dt_point_f(B, A.P, poff(B, i, j, k)));
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));

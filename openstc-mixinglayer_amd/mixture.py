@@ -26,6 +26,8 @@ H2_AIR_FILES = tuple(os.path.join(DATA, f) for f in ("H2_mech.inp", "therm.dat",
 (FIELD_CV, FIELD_BV, FIELD_SV, FIELD_QN, FIELD_DQ, FIELD_MUL, FIELD_LAMBDA, FIELD_CP, FIELD_GAMMA, FIELD_C,
  FIELD_MACH, FIELD_RHOD, FIELD_JAC, FIELD_METRIC, FIELD_DT_LOCAL) = range(15)
 MODE_DIRECT, MODE_GRAPH, MODE_TIMED = 0, 1, 2
+# one launch per kernel kind over every block, graph-replayed / direct with per-kind HIP events
+MODE_BATCHED, MODE_BATCHED_TIMED = 3, 4
 KERNELS = ("source", "inviscid_x", "inviscid_y", "inviscid_z", "viscous_x", "viscous_y", "viscous_z", "update",
            "props")
 
@@ -343,7 +345,7 @@ class FrSolver:
         return ms.value, n.value
 
     def kernel_ms(self):
-        """{kernel kind: (summed ms, launches)} of the last MODE_TIMED call."""
+        """{kernel kind: (summed ms, launches)} of the last MODE_TIMED or MODE_BATCHED_TIMED call."""
         ms = np.zeros(len(KERNELS))
         cnt = (C.c_longlong * len(KERNELS))()
         self._chk(self.L.ostc_fr_kernel_ms(self.ctx, _dp(ms), cnt))

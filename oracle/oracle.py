@@ -76,6 +76,15 @@ def build(force: bool = False) -> str:
 _libs = {}
 
 
+def default_threads() -> int:
+    """OpenMP threads of an oracle created without a count: OMP_NUM_THREADS
+    when set, else every core.  The count is set on every creation because
+    omp_set_num_threads is process-wide: an oracle made with one thread
+    would otherwise leave every later one single-threaded."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
 def lib(tangent=False):
     """The double oracle, or (tangent=True) the same source built with
     {value, tangent} dual numbers as state values, or (tangent="count") the
@@ -186,6 +195,8 @@ class Oracle:
     def __init__(self, case: dict, n_threads: int = 0, tangent: bool = False):
         self.tangent = tangent
         self.L = L = lib(tangent)
+        n_threads = n_threads or default_threads()
+        self.n_threads = n_threads
         s = _Setup(case["n_var"], case["inviscid_scheme"], case["viscous_order"],
                    int(case.get("fixed_time_step", 0)), float(case.get("dt", 0.0)), case["cfl"],
                    case["prandtl"], case["rho_ref"], case["v_ref"], n_threads, int(case.get("reconstruction", 2)),

@@ -206,6 +206,8 @@ class FrOracle:
 
     def __init__(self, case, T=None, n_threads=0):
         self.case = case
+        n_threads = n_threads or OA.default_threads()
+        self.n_threads = n_threads
         self.T = T if T is not None else tables(*case["mech_files"], reaction=case["reaction"])
         self.L = L = lib()
         s = _Setup(case["inviscid_scheme"], case["viscous_order"], case["reaction"],
@@ -297,7 +299,7 @@ class FrOracle:
         case = self.case
         st = case["streams"]
         # an air oracle with the same ghost depth provides grid and profile
-        ao = OA.Oracle(dict(case, n_var=5, inviscid_scheme=71 if self.ngg == 4 else 51), n_threads=1)
+        ao = OA.Oracle(dict(case, n_var=5, inviscid_scheme=71 if self.ngg == 4 else 51), n_threads=self.n_threads)
         streams = OA._Streams((C.c_double * 6)(*st["upper"]), (C.c_double * 6)(*st["lower"]))
         g = self.ngg
         for bi, b in enumerate(case["blocks"]):

@@ -8,9 +8,8 @@
   momentum thickness after n SSP-RK3 steps) against the tangent oracle run
   once per amplitude, component by component;
 * the same gradient against central finite differences of the GPU forward
-  run with the steps 1e-6 and 1e-4, for the amplitudes that move theta at
-  first order (the others are dominated by the |U| kink of the
-  Lax-Friedrichs splitting, which finite differences straddle);
+  run with the step 1e-6, all 32 amplitudes, norm-wise within 1e-6 of the
+  largest gradient component;
 * checkpoint spacing changes the recomputation, never the result (bitwise).
 """
 import numpy as np
@@ -126,27 +125,29 @@ def test_checkpoint_spacing_does_not_change_the_gradient():
         np.testing.assert_array_equal(g, ref[1])
 
 
-@pytest.mark.parametrize("h,tol", [(1e-6, 1e-5), (1e-4, 1e-6)])
-def test_gradient_matches_central_differences(h, tol):
-    case = cases.fourier_case(12, n_steps=4)
-    print(f"[gpu] adjoint gradient vs central differences, h={h}", flush=True)
+@pytest.mark.parametrize("n,steps", [(12, 4), (24, 6)])
+def test_gradient_matches_central_differences(n, steps):
+    """All 32 amplitudes against FP64 central differences of the GPU forward
+    run at h = 1e-6: max_m |fd_m - g_m| <= 1e-6 max_m |g_m|."""
+    case = cases.fourier_case(n, n_steps=steps, checkpoint_every=2)
+    print(f"[gpu] adjoint vs central differences, all amplitudes, {n}^3 x {steps} steps", flush=True)
     _, g = _gpu_gradient(case, 2)
     a = np.array(case["modes"]["amp"])
+    h = 1e-6
     s = pkg.Solver(case)
 
     def theta(amp):
         s.setup_mixing_layer()
         s.perturb_fourier(0, case["modes"], amp)
         s.initialize()
-        s.rk3(case["n_steps"], pkg.MODE_BATCHED, want_dt=False)
+        s.rk3(steps, pkg.MODE_BATCHED, want_dt=False)
         return s.momentum_thickness(0, case["theta"])
 
-    for m in (0, 1):
-        e = np.eye(len(a))[m]
-        fd = (theta(a + h * e) - theta(a - h * e)) / (2 * h)
-        print(f"  mode {m}: adjoint {g[m]:.12e} fd {fd:.12e} rel {abs(fd - g[m]) / abs(g[m]):.2e}", flush=True)
-        assert abs(fd - g[m]) <= tol * abs(g[m])
+    fd = np.array([(theta(a + h * e) - theta(a - h * e)) / (2 * h) for e in np.eye(len(a))])
     s.close()
+    worst = np.max(np.abs(fd - g)) / np.max(np.abs(g))
+    print(f"  |fd - g|_inf / |g|_inf = {worst:.3e} (mode {int(np.argmax(np.abs(fd - g)))})", flush=True)
+    assert worst <= 1e-6
 
 
 def test_reverse_sweep_rejects_what_it_does_not_cover():

@@ -36,7 +36,9 @@ def _c2_oracle():
         case = cases.batched_case()
         print(f"  oracle: {len(case['blocks'])} blocks, {case['n_cells']} cells, {C2_STEPS} steps", flush=True)
         o = oracle_for(case)
-        dt = o.step(C2_STEPS)
+        for n in range(C2_STEPS):
+            dt = o.step(1)
+            print(f"  oracle step {n + 1} done", flush=True)
         _C2.update(case=case, dt=dt, bv=[interior(o.array(b, O.BV), o.ngg).copy() for b in range(len(case["blocks"]))])
         del o
     return _C2
@@ -112,7 +114,8 @@ def test_reacting_512x256x128_stencil_difference(tables):
         s.close()
     scale = np.max(np.abs(dq[0]), axis=(1, 2, 3))
     diff = np.abs(dq[1] - dq[0])
-    rel = np.max(diff, axis=(1, 2, 3)) / scale
+    live = scale > 0   # species absent from both streams have dq = 0 everywhere
+    rel = np.where(live, np.max(diff, axis=(1, 2, 3)) / np.where(live, scale, 1.0), 0.0)
     moved = np.mean(diff > 1e-12 * scale[:, None, None, None], axis=(1, 2, 3))
     print("  max relative difference per variable: " + " ".join(f"{e:.2e}" for e in rel), flush=True)
     print("  share of nodes moved: " + " ".join(f"{m:.3f}" for m in moved), flush=True)
