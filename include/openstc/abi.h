@@ -236,6 +236,31 @@ int ostc_initialize(ostc_ctx ctx);
 /* n_steps SSP-RK3 steps (src/RK.cuh:85-226).  last_dt may be NULL. */
 int ostc_step(ostc_ctx ctx, int n_steps, int mode, double* last_dt);
 
+/* Dual time stepping with the DPLUR inner solver (temporal_scheme 2,
+ * implicit_method 1: dual_time_stepping, src/DualTimeStepping.cuh:33-236,
+ * DPLUR, src/DPLUR.cuh:14-327), air, on the per-block kernels.  Keys of
+ * setup.txt: the physical time step dt, inner_iteration,
+ * max_inner_iteration, iteration_adjust_step (every this many steps the inner
+ * iterations are monitored: the count grows by 5 while the last iteration
+ * still changes the state by more than 1e-3 of the first iteration's change,
+ * and drops to the iteration where the change falls below; 0: never),
+ * DPLUR_inner_step, and the step counter of the run ("step").  The pseudo time
+ * step is the local CFL step (cfl of ostc_setup). */
+typedef struct {
+  double dt;
+  int inner_iteration;
+  int max_inner_iteration;
+  int iteration_adjust_step;
+  int dplur_inner_step;
+  int step;
+} ostc_dual_time;
+/* Arms dual time stepping; q^(n-1) becomes the current state. */
+int ostc_set_dual_time(ostc_ctx ctx, const ostc_dual_time* d);
+/* n_steps physical steps; *inner_iteration (may be NULL) receives the inner
+ * iteration count the next step will take.  OSTC_ERR_NUMERIC on a NaN in a
+ * monitored inner-iteration residual. */
+int ostc_dual_time_step(ostc_ctx ctx, int n_steps, int* inner_iteration);
+
 /* Stage-level operators, shaped like the reference's per-block launches. */
 int ostc_stage_rhs(ostc_ctx ctx, int block);                 /* dq = 0; inviscid; viscous (src/RK.cuh:113-139) */
 int ostc_compute_dt(ostc_ctx ctx, double* dt_out);            /* local_time_step + global_time_step */

@@ -158,6 +158,18 @@ def fourier_struct(modes: dict, amp=None) -> Fourier:
     return f
 
 
+class DualTime(C.Structure):
+    """ostc_dual_time: dual time stepping + DPLUR keys of setup.txt."""
+    _fields_ = [("dt", C.c_double), ("inner_iteration", C.c_int), ("max_inner_iteration", C.c_int),
+                ("iteration_adjust_step", C.c_int), ("dplur_inner_step", C.c_int), ("step", C.c_int)]
+
+
+def dual_time_struct(d: dict) -> DualTime:
+    """The reference's defaults (src/Parameter.cpp:449-468) for keys not given."""
+    return DualTime(d["dt"], d.get("inner_iteration", 20), d.get("max_inner_iteration", 30),
+                    d.get("iteration_adjust_step", 20), d.get("dplur_inner_step", 3), d.get("step", 0))
+
+
 def theta_struct(t: dict) -> Theta:
     return Theta(t["u1"], t["u2"], t["rho0"], t["du"], t["dy"], t["ni"], t["nk"])
 
@@ -213,6 +225,8 @@ SIGNATURES = {
     "ostc_rhs_vjp": (_I, [_P, _I, _P, _P]),
     "ostc_perturb_fourier": (_I, [_P, _I, C.POINTER(Fourier)]),
     "ostc_momentum_thickness": (_I, [_P, _I, C.POINTER(Theta), _DP]),
+    "ostc_set_dual_time": (_I, [_P, C.POINTER(DualTime)]),
+    "ostc_dual_time_step": (_I, [_P, _I, C.POINTER(C.c_int)]),
     "ostc_momentum_thickness_noalloc": (_I, [_P, _I, C.POINTER(Theta), _DP]),
     "ostc_adjoint_forward": (_I, [_P, _I, _I, C.POINTER(Theta), _DP]),
     "ostc_adjoint_backward": (_I, [_P, C.POINTER(Fourier), _DP]),
@@ -400,6 +414,16 @@ class Solver:
         dt = C.c_double(0.0)
         self._chk(lib().ostc_step(self.ctx, n_steps, mode, C.byref(dt) if want_dt else None))
         return dt.value
+
+    def set_dual_time(self, d: dict):
+        """Arms dual time stepping + DPLUR (temporal_scheme 2); q^(n-1) = the current state."""
+        self._chk(lib().ostc_set_dual_time(self.ctx, C.byref(dual_time_struct(d))))
+
+    def dual_time_step(self, n_steps: int = 1) -> int:
+        """n physical dual time steps; returns the inner iterations of the next step."""
+        it = C.c_int()
+        self._chk(lib().ostc_dual_time_step(self.ctx, n_steps, C.byref(it)))
+        return it.value
 
     def compute_rhs(self, b: int):
         self._chk(lib().ostc_stage_rhs(self.ctx, b))

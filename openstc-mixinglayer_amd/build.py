@@ -22,7 +22,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libostc_hip.so")
 SOURCES = ["ostc_api.hip", "ostc_kernels_plain.hip", "ostc_kernels_fused.hip", "ostc_kernels_adjoint.hip",
-           "ostc_mech.cpp", "ostc_fr_kernels.hip", "ostc_fr_api.hip"]
+           "ostc_kernels_dual.hip", "ostc_mech.cpp", "ostc_fr_kernels.hip", "ostc_fr_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 

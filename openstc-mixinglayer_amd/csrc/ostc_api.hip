@@ -135,6 +135,11 @@ struct ostc_ctx_s {
   bool adj_ready = false;
   ostc_theta adj_theta{};
   hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr, ev_b0 = nullptr, ev_b1 = nullptr;
+  // dual time stepping + DPLUR (per-block path), armed by ostc_set_dual_time
+  ostc_dual_time dual{};
+  std::vector<DualBlock> dual_blocks;
+  bool dual_ready = false;
+  double dual_res_scale[4] = {1, 1, 1, 1};
 };
 
 
@@ -586,6 +591,89 @@ int plain_step(ostc_ctx c, double* dt_io, long long* n) {
   return OSTC_OK;
 }
 
+// ------------------------------------------------ dual time stepping
+// inner_converged (src/DualTimeStepping.cu:33-134, one rank): RMS change of
+// (rho, |V|, p, T) over the inner iteration, scaled by its value at
+// iteration 1; at the last iteration a change above 1e-3 adds 5 iterations (up
+// to max_inner_iteration); below 1e-3 the count drops to this iteration and
+// the loop ends (*stop = true).
+int dual_converged(ostc_ctx c, int iter, bool* stop) {
+  *stop = false;
+  constexpr int nblk = 256;
+  double acc[4] = {0, 0, 0, 0};
+  long long n_total = 0;
+  std::vector<double> h(4 * nblk);
+  for (size_t b = 0; b < c->blocks.size(); ++b) {
+    const DevBlock& B = c->blocks[b].d;
+    dual_last_square(B, c->dual_blocks[b], c->d_scratch + 1, nblk, c->st);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_scratch + 1, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int q = 0; q < nblk; ++q)
+      for (int l = 0; l < 4; ++l) acc[l] += h[4 * q + l];
+    n_total += B.n0;
+  }
+  double res[4];
+  for (int l = 0; l < 4; ++l) res[l] = std::sqrt(acc[l] / (double)n_total);
+  if (iter == 1)
+    for (int l = 0; l < 4; ++l) c->dual_res_scale[l] = res[l] < 1e-20 ? 1e-20 : res[l];
+  for (int l = 0; l < 4; ++l) res[l] /= c->dual_res_scale[l];
+  double err_max = res[0];
+  for (int l = 1; l < 4; ++l)
+    if (std::abs(res[l]) > err_max) err_max = res[l];
+  if (std::isnan(err_max))
+    return fail(c, OSTC_ERR_NUMERIC, "NaN in the inner-iteration residual of dual time step " +
+                                         std::to_string(c->dual.step) + ", iteration " + std::to_string(iter));
+  int& inner = c->dual.inner_iteration;
+  if (iter == inner && err_max > 1e-3 && inner < c->dual.max_inner_iteration)
+    inner = std::min(inner + 5, c->dual.max_inner_iteration);
+  if (err_max < 1e-3) {
+    if (inner - iter > 0) inner = iter;
+    *stop = true;
+  }
+  return OSTC_OK;
+}
+
+// One physical step of dual_time_stepping (src/DualTimeStepping.cuh:33-236)
+// with dual_time_stepping_implicit_treat and DPLUR (:239-249,
+// src/DPLUR.cuh:271-327), blocks in order as the reference loops over them.
+int dual_step(ostc_ctx c, long long* n) {
+  const double dt = c->dual.dt, diag_factor = 1.5 / dt;
+  const int step = ++c->dual.step;
+  for (size_t b = 0; b < c->blocks.size(); ++b) {
+    const DevBlock& B = c->blocks[b].d;
+    const DualBlock& D = c->dual_blocks[b];
+    dual_qn_star(B, D, dt, c->st, n);
+    HIPCHK(c, hipMemcpyAsync(D.qn1, B.cv[0], 5 * (size_t)B.sz * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  }
+  const bool monitor = c->dual.iteration_adjust_step > 0 && step % c->dual.iteration_adjust_step == 0;
+  for (int iter = 1; iter <= c->dual.inner_iteration; ++iter) {
+    for (size_t b = 0; b < c->blocks.size(); ++b) {
+      const DevBlock& B = c->blocks[b].d;
+      const DualBlock& D = c->dual_blocks[b];
+      if (monitor) dual_last_store(B, D, c->st);
+      HIPCHK(c, hipMemsetAsync(B.dq, 0, 5 * B.n0 * sizeof(double), c->st));
+      plain_rhs(B, c->P, c->st, n);
+      dual_pre(B, c->P, D, dt, diag_factor, c->st, n);
+      for (int it = 0; it < c->dual.dplur_inner_step; ++it) {
+        dual_dplur(B, D, diag_factor, c->st, n);
+        HIPCHK(c, hipMemcpyAsync(B.dq, D.dqk, 5 * B.n0 * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+      }
+      dual_update(B, c->st, n);
+      plain_bc(c, (int)b, n);
+    }
+    plain_exchange(c, n);
+    for (auto& B : c->blocks) padded_pass(B.d, 0, OP_PROPS, c->st, n);
+    if (monitor) {
+      bool stop = false;
+      const int r = dual_converged(c, iter, &stop);
+      if (r) return r;
+      if (stop) break;
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
 // ------------------------------------------------------- batched step
 // One batched step.  ev_base >= 0: record event pairs evpool[ev_base + 2 rk],
 // evpool[ev_base + 2 rk + 1] around the fused kernel of stage rk.
@@ -674,7 +762,8 @@ int ostc_create(const ostc_setup* s, void* hip_stream, ostc_ctx* out) {
   *out = nullptr;
   if (s->n_var != 5) return OSTC_ERR_UNSUPPORTED;
   const int sc = s->inviscid_scheme;
-  if (sc != 51 && sc != 50 && sc != 71 && sc != 52 && sc != 72 && sc != 1 && sc != 2 && sc != 3 && sc != 4)
+  if (sc != 51 && sc != 50 && sc != 71 && sc != 52 && sc != 72 && sc != 1 && sc != 2 && sc != 3 && sc != 4 &&
+      sc != 6)
     return OSTC_ERR_UNSUPPORTED;
   if ((sc == 1 || sc == 2 || sc == 3 || sc == 4) && (s->reconstruction < 1 || s->reconstruction > 3))
     return OSTC_ERR_UNSUPPORTED;
@@ -1528,6 +1617,53 @@ int ostc_perturb_fourier(ostc_ctx c, int b, const ostc_fourier* f) {
   const DevBlock& B = c->blocks[b].d;
   fourier_add(B, to_modes(*f), B.bv[0], c->st);
   HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_set_dual_time(ostc_ctx c, const ostc_dual_time* d) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (!d || !(d->dt > 0) || d->inner_iteration < 1 || d->max_inner_iteration < 1 || d->dplur_inner_step < 0 ||
+      d->iteration_adjust_step < 0)
+    return fail(c, OSTC_ERR_ARG, "bad dual time stepping settings");
+  if (c->P.n_var != 5) return fail(c, OSTC_ERR_UNSUPPORTED, "dual time stepping covers air");
+  if (c->dual_blocks.empty()) {
+    for (auto& B : c->blocks) {
+      const size_t sz = (size_t)B.d.sz, n0 = (size_t)B.d.n0;
+      DualBlock D{};
+      if ((r = dalloc(c, &D.qn1, 5 * sz)) || (r = dalloc(c, &D.qn_star, 5 * n0)) || (r = dalloc(c, &D.sr, 3 * n0)) ||
+          (r = dalloc(c, &D.dq0, 5 * n0)) || (r = dalloc(c, &D.dqk, 5 * n0)) || (r = dalloc(c, &D.last, 4 * n0)))
+        return r;
+      c->dual_blocks.push_back(D);
+    }
+  }
+  c->dual = *d;
+  // q^(n-1) starts as the current state (src/DualTimeStepping.cuh:66-71)
+  for (size_t b = 0; b < c->blocks.size(); ++b)
+    HIPCHK(c, hipMemcpyAsync(c->dual_blocks[b].qn1, c->blocks[b].d.cv[0], 5 * (size_t)c->blocks[b].d.sz * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->dual_ready = true;
+  return OSTC_OK;
+}
+
+int ostc_dual_time_step(ostc_ctx c, int n_steps, int* inner_iteration) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (!c->dual_ready) return fail(c, OSTC_ERR_STATE, "ostc_set_dual_time has not been called");
+  if (n_steps < 0) return fail(c, OSTC_ERR_ARG, "negative step count");
+  invalidate_graph(c);  // the batched graph's state assumptions do not survive other steps
+  long long n = 0;
+  HIPCHK(c, hipEventRecord(c->ev_a, c->st));
+  for (int i = 0; i < n_steps; ++i)
+    if ((r = dual_step(c, &n))) return r;
+  HIPCHK(c, hipEventRecord(c->ev_b, c->st));
+  HIPCHK(c, hipEventSynchronize(c->ev_b));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+  c->last_ms = ms;
+  c->last_launches = n;
+  if (inner_iteration) *inner_iteration = c->dual.inner_iteration;
   return OSTC_OK;
 }
 

@@ -563,6 +563,16 @@ __device__ __forceinline__ void tile_idx(int col0, int t, int col, int third, in
   }
 }
 
+// LDS slot of face f of column col among a tile's face fluxes: along x the
+// lanes of a wave take consecutive faces of a column, along y and z
+// consecutive columns, so consecutive lanes always hit consecutive slots
+// (a column-major slot along y/z put the lanes F doubles apart: bank
+// conflicts on every face-flux store and read).
+template <int DIR, int TW, int F>
+__device__ __forceinline__ int face_slot(int col, int f) {
+  return DIR == 0 ? col * F + f : f * TW + col;
+}
+
 // Positivity-preserving limiter on the species part of one face,
 // src/WENO.cu:1538-1580.
 template <int NS>
@@ -751,10 +761,10 @@ __device__ __forceinline__ void inviscid_body(const FrArgs& A, const int wx, con
     }
   }
   __syncthreads();  // every face has read its stencil; the LDS now holds face fluxes
-  double* fcs = lds;  // [NV][F * TW], face slot col * F + f
+  double* fcs = lds;  // [NV][F * TW], face slot face_slot(col, f)
   if (active) {
 #pragma unroll
-    for (int l = 0; l < NV; ++l) fcs[l * (F * TW) + col * F + f] = fc[l];
+    for (int l = 0; l < NV; ++l) fcs[l * (F * TW) + face_slot<DIR, TW, F>(col, f)] = fc[l];
   }
   __syncthreads();
 
@@ -764,7 +774,8 @@ __device__ __forceinline__ void inviscid_body(const FrArgs& A, const int wx, con
     const long long n0 = B.n0;
 #pragma unroll
     for (int l = 0; l < NV; ++l)
-      B.dq[q0 + l * n0] -= fcs[l * (F * TW) + col * F + f] - fcs[l * (F * TW) + col * F + f - 1];
+      B.dq[q0 + l * n0] -= fcs[l * (F * TW) + face_slot<DIR, TW, F>(col, f)] -
+                           fcs[l * (F * TW) + face_slot<DIR, TW, F>(col, f - 1)];
   }
 }
 template <int DIR, int NS, int SCHEME>
@@ -974,12 +985,12 @@ __device__ __forceinline__ void inviscid_ch_body(const FrArgs& A, const int wx, 
     }
   }
   __syncthreads();  // every face has read F+/F- and fch; the LDS now holds face fluxes
-  double* fcs = lds;  // [NV][NF], face slot col * F + f
+  double* fcs = lds;  // [NV][NF], face slot face_slot(col, f)
   const int v_lo = half == 0 ? 0 : 5, v_hi = half == 0 ? 5 : NV;
   if (active) {
 #pragma unroll
     for (int l = 0; l < NV; ++l)
-      if (l >= v_lo && l < v_hi) fcs[l * NF + col * F + f] = fc[l];
+      if (l >= v_lo && l < v_hi) fcs[l * NF + face_slot<DIR, TW, F>(col, f)] = fc[l];
   }
   __syncthreads();
   if (f >= 1 && active) {
@@ -987,7 +998,8 @@ __device__ __forceinline__ void inviscid_ch_body(const FrArgs& A, const int wx, 
     const long long n0 = B.n0;
 #pragma unroll
     for (int l = 0; l < NV; ++l)
-      if (l >= v_lo && l < v_hi) B.dq[q0 + l * n0] -= fcs[l * NF + col * F + f] - fcs[l * NF + col * F + f - 1];
+      if (l >= v_lo && l < v_hi)
+        B.dq[q0 + l * n0] -= fcs[l * NF + face_slot<DIR, TW, F>(col, f)] - fcs[l * NF + face_slot<DIR, TW, F>(col, f - 1)];
   }
 }
 template <int DIR, int NS, int SCHEME>
@@ -1092,7 +1104,7 @@ __device__ __forceinline__ void inviscid_pv_body(const FrArgs& A, const int wx, 
   double* fcs = lds;
   if (active) {
 #pragma unroll
-    for (int l = 0; l < NV; ++l) fcs[l * (F * TW) + col * F + f] = fc[l];
+    for (int l = 0; l < NV; ++l) fcs[l * (F * TW) + face_slot<DIR, TW, F>(col, f)] = fc[l];
   }
   __syncthreads();
   if (f >= 1 && active) {
@@ -1100,7 +1112,8 @@ __device__ __forceinline__ void inviscid_pv_body(const FrArgs& A, const int wx, 
     const long long n0 = B.n0;
 #pragma unroll
     for (int l = 0; l < NV; ++l)
-      B.dq[q0 + l * n0] -= fcs[l * (F * TW) + col * F + f] - fcs[l * (F * TW) + col * F + f - 1];
+      B.dq[q0 + l * n0] -= fcs[l * (F * TW) + face_slot<DIR, TW, F>(col, f)] -
+                           fcs[l * (F * TW) + face_slot<DIR, TW, F>(col, f - 1)];
   }
 }
 template <int DIR, int NS>
@@ -1255,7 +1268,7 @@ __device__ __forceinline__ void viscous_body(const FrArgs& A, const int wx, cons
       fv[4] += h[l] * flux;
     }
 #pragma unroll
-    for (int l = 0; l < NV; ++l) lds[l * (F * TW) + col * F + f] = fv[l];
+    for (int l = 0; l < NV; ++l) lds[l * (F * TW) + face_slot<DIR, TW, F>(col, f)] = fv[l];
   }
   __syncthreads();
   if (f >= 1 && active) {
@@ -1263,7 +1276,8 @@ __device__ __forceinline__ void viscous_body(const FrArgs& A, const int wx, cons
     const long long n0 = B.n0;
 #pragma unroll
     for (int l = 0; l < NV; ++l)
-      B.dq[q0 + l * n0] += lds[l * (F * TW) + col * F + f] - lds[l * (F * TW) + col * F + f - 1];
+      B.dq[q0 + l * n0] += lds[l * (F * TW) + face_slot<DIR, TW, F>(col, f)] -
+                           lds[l * (F * TW) + face_slot<DIR, TW, F>(col, f - 1)];
   }
 }
 template <int DIR, int NS>

@@ -18,6 +18,23 @@ void ops_launch(const DevBlock* d_blocks, int n_blocks, const DevOp* d_ops, int 
 void store_last(const DevBlock& B, hipStream_t st);
 void residual_partial(const DevBlock& B, double* out4, int nblk, hipStream_t st);
 
+// dual time stepping + DPLUR on the per-block path, ostc_kernels_dual.hip
+struct DualBlock {
+  double* qn1;      // q^(n-1), 5 padded planes
+  double* qn_star;  // J/(2 dt) (4 q^n - q^(n-1)), 5 interior planes
+  double* sr;       // inviscid spectral radii along xi, eta, zeta, 3 interior planes
+  double* dq0;      // dt dq / diag, 5 interior planes
+  double* dqk;      // the DPLUR iterate, 5 interior planes
+  double* last;     // rho, |V|, p, T of the last inner iteration, then their squared change
+};
+void dual_qn_star(const DevBlock& B, const DualBlock& D, double dt, hipStream_t st, long long* n);
+void dual_pre(const DevBlock& B, const Params& P, const DualBlock& D, double dt, double diag_factor, hipStream_t st,
+              long long* n);
+void dual_dplur(const DevBlock& B, const DualBlock& D, double diag_factor, hipStream_t st, long long* n);
+void dual_update(const DevBlock& B, hipStream_t st, long long* n);
+void dual_last_store(const DevBlock& B, const DualBlock& D, hipStream_t st);
+void dual_last_square(const DevBlock& B, const DualBlock& D, double* out4, int nblk, hipStream_t st);
+
 // fused multi-block path, ostc_kernels_fused.hip
 struct FusedArgs {
   const DevBlock* blocks;

@@ -114,6 +114,81 @@ __global__ __launch_bounds__(64) void k_weno_pencil(DevBlock B, Params P, int sl
   }
 }
 
+// compute_convective_term_ep_1D<Air> (src/InviscidScheme.cu:550-768), the
+// sixth-order energy-preserving scheme (inviscid_scheme 6; Pirozzoli,
+// J. Comput. Phys. 230 (2011) 2997-3014): rho, u, v, w, p, H = (E + p)/rho,
+// the metric row, J and UJ = (k . u) J of 69 pencil points in LDS; the
+// two-point fluxes T(i, l), l = 1..3, of 66 points; 64 face fluxes
+// 2 (3/4 T(i,1) - 3/20 (T(i,2) + T(i-1,2)) + 1/60 (T(i,3) + T(i-1,3) +
+// T(i-2,3))); 63 cells updated.
+template <int DIR>
+__global__ __launch_bounds__(64) void k_ep_pencil(DevBlock B, int slot) {
+  constexpr int R = 3, NP = 63 + 2 * R, NT = 66;
+  __shared__ double sP[11][NP], sT[3][5][NT], sFc[5][64];
+  const int m = DIR == 0 ? B.mx : (DIR == 1 ? B.my : B.mz);
+  const int c0 = 63 * blockIdx.x;
+  const int tid = threadIdx.x;
+  int base[3];
+  if (DIR == 0) { base[1] = blockIdx.y; base[2] = blockIdx.z; }
+  if (DIR == 1) { base[0] = blockIdx.y; base[2] = blockIdx.z; }
+  if (DIR == 2) { base[0] = blockIdx.y; base[1] = blockIdx.z; }
+  const double* cv = B.cv[slot];
+  const double* bv = B.bv[slot];
+  const long long s = B.sz;
+  for (int pt = tid; pt < NP; pt += 64) {
+    const int pos = c0 - R + pt;
+    if (pos > m - 1 + R) continue;
+    int ijk[3] = {base[0], base[1], base[2]};
+    ijk[DIR] = pos;
+    const long long p = pidx(B, ijk[0], ijk[1], ijk[2]);
+#pragma unroll
+    for (int l = 0; l < 5; ++l) sP[l][pt] = bv[p + l * s];
+    sP[5][pt] = (cv[p + 4 * s] + bv[p + 4 * s]) / bv[p];
+    const double kx = B.met[p + (3 * DIR) * s], ky = B.met[p + (3 * DIR + 1) * s], kz = B.met[p + (3 * DIR + 2) * s];
+    sP[6][pt] = kx;
+    sP[7][pt] = ky;
+    sP[8][pt] = kz;
+    const double jac = B.jac[p];
+    sP[9][pt] = jac;
+    double uk = kx * bv[p + s] + ky * bv[p + 2 * s] + kz * bv[p + 3 * s];
+    uk *= jac;
+    sP[10][pt] = uk;
+  }
+  __syncthreads();
+  for (int t = tid; t < NT; t += 64) {
+    const int pos = c0 - R + t;
+#pragma unroll
+    for (int l = 1; l <= 3; ++l) {
+      if (pos + l > m - 1 + R) continue;
+      const int a = t, b = t + l;
+      const double weight = 0.125 * (sP[10][a] + sP[10][b]) * (sP[0][a] + sP[0][b]);
+      const double pWeight = 0.25 * (sP[4][a] + sP[4][b]);
+      sT[l - 1][0][t] = 2 * weight;
+      sT[l - 1][1][t] = weight * (sP[1][a] + sP[1][b]) + pWeight * (sP[6][a] * sP[9][a] + sP[6][b] * sP[9][b]);
+      sT[l - 1][2][t] = weight * (sP[2][a] + sP[2][b]) + pWeight * (sP[7][a] * sP[9][a] + sP[7][b] * sP[9][b]);
+      sT[l - 1][3][t] = weight * (sP[3][a] + sP[3][b]) + pWeight * (sP[8][a] * sP[9][a] + sP[8][b] * sP[9][b]);
+      sT[l - 1][4][t] = weight * (sP[5][a] + sP[5][b]);
+    }
+  }
+  __syncthreads();
+  const int L = c0 - 1 + tid;  // face L+1/2
+  if (L <= m - 1) {
+    constexpr double central_1[3] = {0.75, -3.0 / 20, 1.0 / 60};
+    const int t = tid + R - 1;  // point L
+#pragma unroll
+    for (int l = 0; l < 5; ++l)
+      sFc[l][tid] = 2 * (central_1[0] * sT[0][l][t] + central_1[1] * (sT[1][l][t] + sT[1][l][t - 1]) +
+                         central_1[2] * (sT[2][l][t] + sT[2][l][t - 1] + sT[2][l][t - 2]));
+  }
+  __syncthreads();
+  if (tid > 0 && L <= m - 1) {
+    int ijk[3] = {base[0], base[1], base[2]};
+    ijk[DIR] = L;
+    const long long p0 = pidx0(B, ijk[0], ijk[1], ijk[2]);
+    for (int l = 0; l < 5; ++l) B.dq[p0 + l * B.n0] -= sFc[l][tid] - sFc[l][tid - 1];
+  }
+}
+
 // compute_convective_term_pv_1D<Air> (src/InviscidScheme.cu:50-165): the
 // primitive variables of 67 pencil points staged in LDS, MUSCL / NND2 /
 // first-order reconstruction (src/Reconstruction.cu:5-46) and an LF, AUSM+ or
@@ -475,6 +550,12 @@ void plain_rhs(const DevBlock& B, const Params& P, hipStream_t st, long long* n)
   else if (P.scheme == 50) launch_weno<50>(B, P, st, n);
   else if (P.scheme == 52) launch_weno<52>(B, P, st, n);
   else if (P.scheme == 72) launch_weno<72>(B, P, st, n);
+  else if (P.scheme == 6) {
+    hipLaunchKernelGGL(k_ep_pencil<0>, pencil_grid(B, 0), dim3(64), 0, st, B, 0);
+    hipLaunchKernelGGL(k_ep_pencil<1>, pencil_grid(B, 1), dim3(64), 0, st, B, 0);
+    hipLaunchKernelGGL(k_ep_pencil<2>, pencil_grid(B, 2), dim3(64), 0, st, B, 0);
+    *n += 3;
+  }
   else if (P.scheme == 1 || P.scheme == 2 || P.scheme == 3 || P.scheme == 4) {
     hipLaunchKernelGGL(k_pv_pencil<0>, pencil_grid(B, 0), dim3(64), 0, st, B, P, 0);
     hipLaunchKernelGGL(k_pv_pencil<1>, pencil_grid(B, 1), dim3(64), 0, st, B, P, 0);

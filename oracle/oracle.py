@@ -60,6 +60,17 @@ def farfield_state_air(**kw) -> Farfield:
     return out
 
 
+class DualTime(C.Structure):
+    """orc_dual_time: the dual time stepping keys of setup.txt."""
+    _fields_ = [("dt", C.c_double), ("inner_iteration", C.c_int), ("max_inner_iteration", C.c_int),
+                ("iteration_adjust_step", C.c_int), ("dplur_inner_step", C.c_int), ("step", C.c_int)]
+
+
+def dual_time_struct(d: dict) -> DualTime:
+    return DualTime(d["dt"], d.get("inner_iteration", 20), d.get("max_inner_iteration", 30),
+                    d.get("iteration_adjust_step", 20), d.get("dplur_inner_step", 3), d.get("step", 0))
+
+
 class _Fourier(C.Structure):
     _fields_ = [("n_modes", C.c_int), ("comp", C.c_void_p), ("kx", C.c_void_p), ("kz", C.c_void_p),
                 ("trig", C.c_void_p), ("amp", C.c_void_p), ("amp_dot", C.c_void_p),
@@ -121,6 +132,9 @@ def lib(tangent=False):
         L.orc_mixing_layer_ic.argtypes = [vp, i, C.POINTER(_Streams), d, i, i, i, i, i, i,
                                           C.c_uint64, d]
         L.orc_step.argtypes = [vp, i, dp]
+        L.orc_dual_time_init.argtypes = [vp, C.POINTER(DualTime)]
+        L.orc_dual_time_step.argtypes = [vp, i, C.POINTER(C.c_int)]
+        L.orc_jacobian_times_dq.argtypes = [dp, dp, d, dp, dp]
         for name in ("orc_cv_from_bv", "orc_copy_qn", "orc_zero_dq", "orc_inviscid", "orc_viscous",
                      "orc_local_dt", "orc_apply_bc", "orc_properties"):
             getattr(L, name).argtypes = [vp, i]
@@ -316,6 +330,19 @@ class Oracle:
         self.L.orc_step(self.ctx, n, C.byref(dt))
         return dt.value
 
+    def dual_time_init(self, d: dict):
+        """Dual time stepping + DPLUR settings; q^(n-1) = the current cv."""
+        if self.L.orc_dual_time_init(self.ctx, C.byref(dual_time_struct(d))) != 0:
+            raise ValueError("bad dual time settings")
+
+    def dual_time_step(self, n: int = 1) -> int:
+        """n physical steps; returns the inner iterations of the next step."""
+        it = C.c_int()
+        r = self.L.orc_dual_time_step(self.ctx, n, C.byref(it))
+        if r < 0:
+            raise FloatingPointError("NaN in the inner-iteration residual" if r == -1 else "not initialised")
+        return it.value
+
     # -- stage-level operators ----------------------------------------
     def rhs(self, b: int):
         """dq = 0; source (the sponge layer); inviscid; viscous (src/RK.cuh:113-139)."""
@@ -338,6 +365,16 @@ class Oracle:
         r = np.zeros(4)
         self.L.orc_residual(self.ctx, _dp(r))
         return r
+
+
+def jacobian_times_dq(k3, pv, lam, dq):
+    """compute_jacobian_times_dq<Air> at one point."""
+    k3 = np.ascontiguousarray(k3, dtype=np.float64)
+    pv = np.ascontiguousarray(pv, dtype=np.float64)
+    dq = np.ascontiguousarray(dq, dtype=np.float64)
+    out = np.zeros(5)
+    lib().orc_jacobian_times_dq(_dp(k3), _dp(pv), lam, _dp(dq), _dp(out))
+    return out
 
 
 def weno5z(vp, vm, eps):

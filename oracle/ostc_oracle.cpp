@@ -88,6 +88,11 @@ struct Block {
   int mx, my, mz, g;
   long long sx, sy, sz;  // padded strides for g ghosts
   std::vector<real> cv, bv, qn, dq, mul, mach, dtl, bv_last;
+  // dual time stepping + DPLUR (allocated by orc_dual_time_init): the state
+  // of the previous physical step (padded, n_var planes), qn_star, the
+  // inviscid spectral radii (3 planes), dq0 and dqk (interior), and the
+  // primitives of the last inner iteration (4 planes)
+  std::vector<real> qn1, qn_star, sr, dq0, dqk, last_iter;
   std::vector<geo> jac, metric;
   std::vector<double> x, y, z;
   std::vector<Face> faces;
@@ -114,6 +119,9 @@ struct Ctx {
   double weno_eps_scale;
   double dt_cur = 1e6;
   unsigned long long sponge_iter = 0;  // steps averaged into the sponge means
+  orc_dual_time dual{};                // dual time stepping settings and counters
+  bool dual_ready = false;
+  double dual_res_scale[4] = {1, 1, 1, 1};
 };
 
 inline double sqr(double a) { return a * a; }
@@ -771,6 +779,78 @@ inline geo face_metric_sq(const geo* m1, const geo* m2, geo jac1, geo jac2) {
   return ax * ax + ay * ay + az * az;
 }
 
+// compute_convective_term_ep<Air> / _ep_1D, src/InviscidScheme.cu:550-768:
+// the sixth-order energy-preserving central scheme (Pirozzoli, J. Comput.
+// Phys. 230 (2011) 2997-3014).  Two-point fluxes between points i and i+l,
+// l = 1..3, with the Jacobian-weighted contravariant velocity UJ and the
+// total enthalpy H = (E + p)/rho:
+//   w = (UJ_i + UJ_i+l)(rho_i + rho_i+l)/8,  pw = (p_i + p_i+l)/4,
+//   T = (2w, w(u_i + u_i+l) + pw(k_x J|_i + k_x J|_i+l), ..., w(H_i + H_i+l));
+// the face flux at i+1/2 is 2 (3/4 T(i,1) - 3/20 (T(i,2) + T(i-1,2))
+// + 1/60 (T(i,3) + T(i-1,3) + T(i-2,3))), and dq(i) -= fc(i+1/2) - fc(i-1/2).
+void inviscid_ep(const Ctx& c, Block& b) {
+  const int nv = c.s.n_var, g = b.g;
+  const long long s = b.sz;
+  const int ext[3] = {b.mx, b.my, b.mz};
+  constexpr double central_1[3] = {0.75, -3.0 / 20, 1.0 / 60};
+  for (int dir = 0; dir < 3; ++dir) {
+    const int m = ext[dir];
+    const int oa = (dir == 0) ? 1 : 0, ob = (dir == 2) ? 1 : 2;
+    const int na = ext[oa], nb = ext[ob];
+#pragma omp parallel
+    {
+      const int np = m + 2 * g;
+      // per pencil point: rho, u, v, w, p, H, k_x, k_y, k_z, J, UJ
+      std::vector<real> pt(np * 11);
+      std::vector<real> tilde(np * 3 * 5), fc((m + 1) * 5);
+#pragma omp for collapse(2) schedule(static)
+      for (int bb = 0; bb < nb; ++bb)
+        for (int aa = 0; aa < na; ++aa) {
+          int idx[3];
+          idx[oa] = aa;
+          idx[ob] = bb;
+          for (int t = -g; t < m + g; ++t) {
+            idx[dir] = t;
+            const long long p = b.id(idx[0], idx[1], idx[2]);
+            real* P = &pt[(t + g) * 11];
+            for (int l = 0; l < 5; ++l) P[l] = b.bv[p + l * s];
+            P[5] = (b.cv[p + 4 * s] + b.bv[p + 4 * s]) / b.bv[p];
+            for (int cc = 0; cc < 3; ++cc) P[6 + cc] = b.metric[p + (3 * dir + cc) * s];
+            P[9] = b.jac[p];
+            real uk = P[6] * P[1] + P[7] * P[2] + P[8] * P[3];
+            uk *= P[9];
+            P[10] = uk;
+          }
+          // two-point fluxes T(i, l), i = -g .. m+g-1-l
+          for (int t = -g; t < m + g; ++t)
+            for (int l = 1; l <= 3; ++l) {
+              if (t + l > m + g - 1) continue;
+              const real* A = &pt[(t + g) * 11];
+              const real* Bq = &pt[(t + l + g) * 11];
+              const real weight = 0.125 * (A[10] + Bq[10]) * (A[0] + Bq[0]);
+              const real pWeight = 0.25 * (A[4] + Bq[4]);
+              real* T = &tilde[((t + g) * 3 + (l - 1)) * 5];
+              T[0] = 2 * weight;
+              T[1] = weight * (A[1] + Bq[1]) + pWeight * (A[6] * A[9] + Bq[6] * Bq[9]);
+              T[2] = weight * (A[2] + Bq[2]) + pWeight * (A[7] * A[9] + Bq[7] * Bq[9]);
+              T[3] = weight * (A[3] + Bq[3]) + pWeight * (A[8] * A[9] + Bq[8] * Bq[9]);
+              T[4] = weight * (A[5] + Bq[5]);
+            }
+          auto Tq = [&](int t, int l, int v) { return tilde[((t + g) * 3 + (l - 1)) * 5 + v]; };
+          for (int t = -1; t < m; ++t)
+            for (int v = 0; v < 5; ++v)
+              fc[(t + 1) * 5 + v] = 2 * (central_1[0] * Tq(t, 1, v) + central_1[1] * (Tq(t, 2, v) + Tq(t - 1, 2, v)) +
+                                         central_1[2] * (Tq(t, 3, v) + Tq(t - 1, 3, v) + Tq(t - 2, 3, v)));
+          for (int t = 0; t < m; ++t) {
+            idx[dir] = t;
+            const long long q0 = b.id0(idx[0], idx[1], idx[2]);
+            for (int v = 0; v < nv; ++v) b.dq[q0 + v * b.n0()] -= fc[(t + 1) * 5 + v] - fc[t * 5 + v];
+          }
+        }
+    }
+  }
+}
+
 // compute_convective_term_weno_{x,y,z} + compute_weno_flux_cp,
 // src/WENO.cu:8-461 (pencil kernels), :1455-1536 (cp flux), :463-491 (launcher).
 // dq(i) -= fc(i+1/2) - fc(i-1/2), directions x, y, z in that order.
@@ -778,6 +858,10 @@ void inviscid(const Ctx& c, Block& b) {
   if (c.s.inviscid_scheme == 1 || c.s.inviscid_scheme == 2 || c.s.inviscid_scheme == 3 ||
       c.s.inviscid_scheme == 4) {
     inviscid_pv(c, b);
+    return;
+  }
+  if (c.s.inviscid_scheme == 6) {
+    inviscid_ep(c, b);
     return;
   }
   const int nv = c.s.n_var, g = b.g;
@@ -1324,6 +1408,231 @@ void alloc_block(Block& b, int nv) {
   b.z.assign(gs, 0.0);
 }
 
+// ------------------------------------------------ dual time stepping + DPLUR
+// local_time_step (src/TimeAdvanceFunc.cuh:28-83) keeping the inviscid
+// spectral radii for the implicit operator: sr[d] = |U_d| + c |grad xi_d|.
+void local_dt_sr(const Ctx& c, Block& b) {
+  local_dt(c, b);
+  const long long s = b.sz, n0 = b.n0();
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = b.id(i, j, k), q = b.id0(i, j, k);
+        const geo* M = &b.metric[0];
+        auto m = [&](int r, int cc) { return M[p + (3 * (r - 1) + (cc - 1)) * s]; };
+        const geo gxi = sqrt(m(1, 1) * m(1, 1) + m(1, 2) * m(1, 2) + m(1, 3) * m(1, 3));
+        const geo geta = sqrt(m(2, 1) * m(2, 1) + m(2, 2) * m(2, 2) + m(2, 3) * m(2, 3));
+        const geo gzeta = sqrt(m(3, 1) * m(3, 1) + m(3, 2) * m(3, 2) + m(3, 3) * m(3, 3));
+        const real u = b.bv[p + s], v = b.bv[p + 2 * s], w = b.bv[p + 3 * s];
+        const real U = u * m(1, 1) + v * m(1, 2) + w * m(1, 3);
+        const real V = u * m(2, 1) + v * m(2, 2) + w * m(2, 3);
+        const real W = u * m(3, 1) + v * m(3, 2) + w * m(3, 3);
+        const real a = sqrt(kGammaAir * kRu * b.bv[p + 5 * s] / kMwAir);
+        b.sr[q] = abs(U) + a * gxi;
+        b.sr[q + n0] = abs(V) + a * geta;
+        b.sr[q + 2 * n0] = abs(W) + a * gzeta;
+      }
+}
+
+// compute_qn_star, src/DualTimeStepping.cu:3-15: qn* = J/(2 dt) (4 q^n - q^(n-1))
+void dual_qn_star(const Ctx& c, Block& b, double dt) {
+  const int nv = c.s.n_var;
+  const long long s = b.sz, n0 = b.n0();
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = b.id(i, j, k), q = b.id0(i, j, k);
+        const real factor = 0.5 * b.jac[p] / dt;
+        for (int l = 0; l < nv; ++l) b.qn_star[q + l * n0] = factor * (4 * b.cv[p + l * s] - b.qn1[p + l * s]);
+      }
+}
+
+// compute_modified_rhs, src/DualTimeStepping.cu:17-31: dq += qn* - 3J/(2 dt) q
+void dual_modified_rhs(const Ctx& c, Block& b, double dt) {
+  const int nv = c.s.n_var;
+  const long long s = b.sz, n0 = b.n0();
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = b.id(i, j, k), q = b.id0(i, j, k);
+        const real factor = 1.5 * b.jac[p] / dt;
+        for (int l = 0; l < nv; ++l) b.dq[q + l * n0] += b.qn_star[q + l * n0] - factor * b.cv[p + l * s];
+      }
+}
+
+// The DPLUR diagonal, 1 + diag_factor dt + dt (sr_xi + sr_eta + sr_zeta).  The
+// reference adds 2 dt times the viscous spectral radii, an array it
+// allocates zeroed and never writes (src/Field.cu:553-566), so that term is 0.
+inline real dplur_diag(const Block& b, long long q, double diag_factor) {
+  const long long n0 = b.n0();
+  const real dtl = b.dtl[q];
+  const real visc = 2 * dtl * (0.0 + 0.0 + 0.0);
+  return 1 + diag_factor * dtl + dtl * (b.sr[q] + b.sr[q + n0] + b.sr[q + 2 * n0]) + visc;
+}
+
+// compute_DQ_0<Air>, src/DPLUR.cuh:14-45: dq = dt dq / diag, then dq0 = dq
+void dplur_dq0(const Ctx& c, Block& b, double diag_factor) {
+  const int nv = c.s.n_var;
+  const long long n0 = b.n0();
+  for (long long q = 0; q < n0; ++q) {
+    const real dtl = b.dtl[q];
+    for (int l = 0; l < nv; ++l) b.dq[q + l * n0] *= dtl;
+    const real diag = dplur_diag(b, q, diag_factor);
+    for (int l = 0; l < nv; ++l) b.dq[q + l * n0] /= diag;
+  }
+  b.dq0 = b.dq;
+}
+
+// compute_jacobian_times_dq<Air>, src/DPLUR.cuh:98-160: (A_d + lambda) dq at
+// one point, A_d the inviscid flux Jacobian along direction d (metric row
+// d+1, not Jacobian-weighted), lambda = +-(spectral radius).
+void jacobian_times_dq(const geo* k3, const real* pv, real lambda, const real* dq, real* out) {
+  const real u = pv[1], v = pv[2], w = pv[3];
+  const real U = k3[0] * u + k3[1] * v + k3[2] * w;
+  const real lmd1 = U + lambda;
+  const real e = 0.5 * (u * u + v * v + w * w);
+  const real gamma = kGammaAir;
+  const real b3 = 0, b4 = 0;
+  const real h = gamma / (gamma - 1) * pv[4] / pv[0] + e;
+  const real b1 = k3[0] * dq[1] + k3[1] * dq[2] + k3[2] * dq[3] - U * dq[0];
+  const real b2 = (gamma - 1) * (e * dq[0] - u * dq[1] - v * dq[2] - w * dq[3] + dq[4]);
+  out[0] = b1 + lmd1 * dq[0];
+  out[1] = u * b1 + k3[0] * b2 + lmd1 * dq[1] + k3[0] * (b3 - b4);
+  out[2] = v * b1 + k3[1] * b2 + lmd1 * dq[2] + k3[1] * (b3 - b4);
+  out[3] = w * b1 + k3[2] * b2 + lmd1 * dq[3] + k3[2] * (b3 - b4);
+  out[4] = h * b1 + U * b2 + lmd1 * dq[4] + U * (b3 - b4);
+}
+
+// DPLUR_inner_iteration<Air>, src/DPLUR.cuh:162-266: dqk = dq0 + dt/diag *
+// sum over the six neighbours inside the block of +-1/2 (A +- lambda) dq.
+void dplur_sweep(const Ctx& c, Block& b, double diag_factor) {
+  const int nv = c.s.n_var;
+  const long long s = b.sz, n0 = b.n0();
+  const int ext[3] = {b.mx, b.my, b.mz};
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const int ijk[3] = {i, j, k};
+        real total[5] = {0, 0, 0, 0, 0};
+        auto neighbour = [&](int d, int side) {
+          int n[3] = {i, j, k};
+          n[d] += side;
+          const long long pn = b.id(n[0], n[1], n[2]), qn = b.id0(n[0], n[1], n[2]);
+          geo k3[3];
+          for (int cc = 0; cc < 3; ++cc) k3[cc] = b.metric[pn + (3 * d + cc) * s];
+          real pv[6], dqn[5], out[5];
+          for (int l = 0; l < 6; ++l) pv[l] = b.bv[pn + l * s];
+          for (int l = 0; l < nv; ++l) dqn[l] = b.dq[qn + l * n0];
+          const real visc = 0.0;  // the zero viscous spectral radius, as above
+          const real lambda = side < 0 ? b.sr[qn + d * n0] + 2 * visc : -b.sr[qn + d * n0] - 2 * visc;
+          jacobian_times_dq(k3, pv, lambda, dqn, out);
+          for (int l = 0; l < nv; ++l) {
+            if (side < 0) total[l] += 0.5 * out[l];
+            else total[l] -= 0.5 * out[l];
+          }
+        };
+        for (int d = 0; d < 3; ++d)
+          if (ijk[d] > 0) neighbour(d, -1);
+        for (int d = 0; d < 3; ++d)
+          if (ijk[d] != ext[d] - 1) neighbour(d, +1);
+        const long long q = b.id0(i, j, k);
+        const real dtl = b.dtl[q];
+        const real diag = dplur_diag(b, q, diag_factor);
+        for (int l = 0; l < nv; ++l) b.dqk[q + l * n0] = b.dq0[q + l * n0] + dtl * total[l] / diag;
+      }
+}
+
+// update_cv_and_bv<Air>, src/FieldOperation.cuh:188-239: q += dt_local / J dq
+void dual_update(const Ctx& c, Block& b) {
+  const int nv = c.s.n_var;
+  const long long s = b.sz, n0 = b.n0();
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = b.id(i, j, k), q = b.id0(i, j, k);
+        const real dt_div_jac = b.dtl[q] / b.jac[p];
+        for (int l = 0; l < nv; ++l) b.cv[p + l * s] += dt_div_jac * b.dq[q + l * n0];
+        b.bv[p] = b.cv[p];
+        const real density_inv = 1.0 / b.cv[p];
+        b.bv[p + s] = b.cv[p + s] * density_inv;
+        b.bv[p + 2 * s] = b.cv[p + 2 * s] * density_inv;
+        b.bv[p + 3 * s] = b.cv[p + 3 * s] * density_inv;
+        const real V2 = b.bv[p + s] * b.bv[p + s] + b.bv[p + 2 * s] * b.bv[p + 2 * s] +
+                        b.bv[p + 3 * s] * b.bv[p + 3 * s];
+        b.bv[p + 4 * s] = (kGammaAir - 1) * (b.cv[p + 4 * s] - 0.5 * b.bv[p] * V2);
+        b.bv[p + 5 * s] = b.bv[p + 4 * s] * kMwAir * density_inv / kRu;
+      }
+}
+
+// store_last_iter and compute_square_of_dbv_wrt_last_inner_iter
+// (src/DualTimeStepping.cu:137-171), one block: last = (rho, |V|, p, T), then
+// last = (new - last)^2, summed into acc.
+void last_iter_store(Block& b) {
+  const long long s = b.sz, n0 = b.n0();
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = b.id(i, j, k), q = b.id0(i, j, k);
+        b.last_iter[q] = b.bv[p];
+        b.last_iter[q + n0] = sqrt(b.bv[p + s] * b.bv[p + s] + b.bv[p + 2 * s] * b.bv[p + 2 * s] +
+                                   b.bv[p + 3 * s] * b.bv[p + 3 * s]);
+        b.last_iter[q + 2 * n0] = b.bv[p + 4 * s];
+        b.last_iter[q + 3 * n0] = b.bv[p + 5 * s];
+      }
+}
+
+void last_iter_square(Block& b, real acc[4]) {
+  const long long s = b.sz, n0 = b.n0();
+  for (int k = 0; k < b.mz; ++k)
+    for (int j = 0; j < b.my; ++j)
+      for (int i = 0; i < b.mx; ++i) {
+        const long long p = b.id(i, j, k), q = b.id0(i, j, k);
+        const real vel = sqrt(b.bv[p + s] * b.bv[p + s] + b.bv[p + 2 * s] * b.bv[p + 2 * s] +
+                              b.bv[p + 3 * s] * b.bv[p + 3 * s]);
+        const real cur[4] = {b.bv[p], vel, b.bv[p + 4 * s], b.bv[p + 5 * s]};
+        for (int l = 0; l < 4; ++l) {
+          const real d = cur[l] - b.last_iter[q + l * n0];
+          b.last_iter[q + l * n0] = d * d;
+          acc[l] += b.last_iter[q + l * n0];
+        }
+      }
+}
+
+// inner_converged, src/DualTimeStepping.cu:33-134 (one rank): the RMS change
+// of (rho, |V|, p, T) over the inner iteration, scaled by its value at
+// iteration 1; at the last iteration a residual above 1e-3 adds 5 iterations
+// (up to max_inner_iteration); below 1e-3 the step's iteration count drops to
+// this iteration and the inner loop ends.  Returns 1 to end the loop, -1 on NaN.
+int inner_converged(Ctx& c, int iter, int* inner_iter) {
+  real acc[4] = {0, 0, 0, 0};
+  long long n_total = 0;
+  for (auto& b : c.blocks) {
+    last_iter_square(b, acc);
+    n_total += b.n0();
+  }
+  double res[4];
+  for (int l = 0; l < 4; ++l) res[l] = std::sqrt(value_of(acc[l]) / (double)n_total);
+  if (iter == 1)
+    for (int l = 0; l < 4; ++l) c.dual_res_scale[l] = res[l] < 1e-20 ? 1e-20 : res[l];
+  for (int l = 0; l < 4; ++l) res[l] /= c.dual_res_scale[l];
+  double err_max = res[0];
+  for (int l = 1; l < 4; ++l)
+    if (std::abs(res[l]) > err_max) err_max = res[l];
+  if (std::isnan(err_max)) return -1;
+  if (iter == *inner_iter) {
+    const int max_iter = c.dual.max_inner_iteration;
+    if (err_max > 1e-3 && *inner_iter < max_iter) *inner_iter = std::min(*inner_iter + 5, max_iter);
+  }
+  if (err_max < 1e-3) {
+    if (*inner_iter - iter > 0) *inner_iter = iter;
+    return 1;
+  }
+  return 0;
+}
+
 uint64_t splitmix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;
   uint64_t z = x;
@@ -1806,6 +2115,88 @@ int orc_step(void* ctx, int n_steps, double* last_dt) {
   c->dt_cur = dt;
   if (last_dt) *last_dt = dt;
   return 0;
+}
+
+// dual_time_stepping's set-up (src/DualTimeStepping.cuh:33-76): the
+// settings, and q^(n-1) = the current cv.
+int orc_dual_time_init(void* ctx, const orc_dual_time* d) {
+  auto* c = static_cast<Ctx*>(ctx);
+  if (!d || !(d->dt > 0) || d->inner_iteration < 1 || d->dplur_inner_step < 0) return -1;
+  c->dual = *d;
+  const int nv = c->s.n_var;
+  for (auto& b : c->blocks) {
+    const long long n0 = b.n0();
+    b.qn1 = b.cv;
+    b.qn_star.assign(n0 * nv, 0.0);
+    b.sr.assign(n0 * 3, 0.0);
+    b.dq0.assign(n0 * nv, 0.0);
+    b.dqk.assign(n0 * nv, 0.0);
+    b.last_iter.assign(n0 * 4, 0.0);
+  }
+  c->dual_ready = true;
+  return 0;
+}
+
+// Physical steps of dual_time_stepping with the DPLUR inner solver
+// (src/DualTimeStepping.cuh:33-236, dual_time_stepping_implicit_treat,
+// src/DPLUR.cuh:271-327), air, laminar.  Returns the number of inner
+// iterations the next step will take; -1 on a NaN residual.
+int orc_dual_time_step(void* ctx, int n_steps, int* inner_iteration) {
+  auto* c = static_cast<Ctx*>(ctx);
+  if (!c->dual_ready) return -2;
+  const double dt = c->dual.dt;
+  const double diag_factor = 1.5 / dt;
+  for (int st = 0; st < n_steps; ++st) {
+    const int step = ++c->dual.step;
+    for (auto& b : c->blocks) {
+      dual_qn_star(*c, b, dt);
+      b.qn1 = b.cv;
+    }
+    // (the reference's dual time loop does not update the sponge layer's
+    // running mean; compute_source reads it as it stands)
+    const bool monitor = c->dual.iteration_adjust_step > 0 && step % c->dual.iteration_adjust_step == 0;
+    for (int iter = 1; iter <= c->dual.inner_iteration; ++iter) {
+      for (auto& b : c->blocks) {
+        if (monitor) last_iter_store(b);
+        std::fill(b.dq.begin(), b.dq.end(), 0.0);
+        if (b.sponge) sponge_source(b);
+        inviscid(*c, b);
+        viscous(*c, b);
+        local_dt_sr(*c, b);
+        dual_modified_rhs(*c, b, dt);
+        dplur_dq0(*c, b, diag_factor);
+        for (int it = 0; it < c->dual.dplur_inner_step; ++it) {
+          dplur_sweep(*c, b, diag_factor);
+          b.dq = b.dqk;
+        }
+        const long long n0 = b.n0();
+        for (int l = 0; l < c->s.n_var; ++l)
+          for (long long q = 0; q < n0; ++q) b.dq[q + l * n0] /= b.dtl[q];
+        dual_update(*c, b);
+        apply_bc(b);
+      }
+      communicate(*c);
+      for (auto& b : c->blocks) properties(b);
+      if (monitor) {
+        const int r = inner_converged(*c, iter, &c->dual.inner_iteration);
+        if (r < 0) return -1;
+        if (r > 0) break;
+      }
+    }
+  }
+  if (inner_iteration) *inner_iteration = c->dual.inner_iteration;
+  return c->dual.inner_iteration;
+}
+
+// compute_jacobian_times_dq<Air> at one point (test entry point): pv =
+// (rho, u, v, w, p, T), k3 the metric row, lambda the signed spectral term.
+void orc_jacobian_times_dq(const double* k3, const double* pv, double lambda, const double* dq, double* out) {
+  geo k[3] = {k3[0], k3[1], k3[2]};
+  real p[6], d[5], o[5];
+  for (int l = 0; l < 6; ++l) p[l] = pv[l];
+  for (int l = 0; l < 5; ++l) d[l] = dq[l];
+  jacobian_times_dq(k, p, lambda, d, o);
+  for (int l = 0; l < 5; ++l) out[l] = value_of(o[l]);
 }
 
 // store_last_step, src/TimeAdvanceFunc.cu:6-19

@@ -160,6 +160,19 @@ int     orc_scalar_width(void);
 int     orc_initialize(void* ctx);
 int     orc_step(void* ctx, int n_steps, double* last_dt);
 
+/* Dual time stepping with the DPLUR inner solver (temporal_scheme 2,
+ * implicit_method 1; src/DualTimeStepping.cuh:33-236, src/DPLUR.cuh:14-327):
+ * the setup.txt keys dt, inner_iteration, max_inner_iteration,
+ * iteration_adjust_step, DPLUR_inner_step and the step counter of the run. */
+typedef struct {
+  double dt;
+  int inner_iteration, max_inner_iteration, iteration_adjust_step, dplur_inner_step;
+  int step;
+} orc_dual_time;
+int     orc_dual_time_init(void* ctx, const orc_dual_time* d);
+int     orc_dual_time_step(void* ctx, int n_steps, int* inner_iteration);
+void    orc_jacobian_times_dq(const double* k3, const double* pv, double lambda, const double* dq, double* out);
+
 /* stage-level operators */
 int     orc_cv_from_bv(void* ctx, int block);
 int     orc_copy_qn(void* ctx, int block);
