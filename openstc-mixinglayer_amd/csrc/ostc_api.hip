@@ -472,7 +472,7 @@ int need_fused(ostc_ctx c) {
       return fail(c, OSTC_ERR_UNSUPPORTED, "block too large for the batched kernel (9 padded planes >= 2^31 doubles)");
   if (fused_scheme(c)) return OSTC_OK;
   return fail(c, OSTC_ERR_UNSUPPORTED,
-              "inviscid_scheme 52/72/1/2/3/4 run with OSTC_MODE_PLAIN and the stage operators only");
+              "inviscid_scheme 52/72/1/2/3/4/6 run with OSTC_MODE_PLAIN and the stage operators only");
 }
 
 int set_tiles(ostc_ctx c, int index, int n_wg) {

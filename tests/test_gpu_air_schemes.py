@@ -1,8 +1,8 @@
 """HIP per-block kernels of the air schemes beyond component-wise WENO vs the
 CPU oracle (needs a GPU): characteristic WENO-Z5 / WENO-Z7 (inviscid_scheme
-52 / 72) and
-the primitive reconstruction + Riemann path (1 LF, 3 AUSM+, 4 HLLC with
-first-order, MUSCL or NND2 reconstruction).
+52 / 72), the primitive reconstruction + Riemann path (1 LF, 3 AUSM+, 4 HLLC
+with first-order, MUSCL or NND2 reconstruction) and the sixth-order
+energy-preserving split form (6; reconstruction unused).
 
 Tolerances (per variable, max|gpu - oracle| / max|oracle|):
   dq (inviscid + viscous)            <= 1e-12
@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 DQ_TOL = 1e-12
 BV_TOL = 1e-9
 
-SCHEMES = [(52, 2), (72, 2), (1, 1), (1, 2), (2, 1), (2, 2), (3, 2), (3, 3), (4, 1), (4, 2), (4, 3)]
+SCHEMES = [(52, 2), (72, 2), (1, 1), (1, 2), (2, 1), (2, 2), (3, 2), (3, 3), (4, 1), (4, 2), (4, 3), (6, 2)]
 IDS = [f"s{s}-r{r}" for s, r in SCHEMES]
 
 
@@ -102,7 +102,7 @@ def test_roe_entropy_fix_factor(factor):
     assert np.max(np.abs(out[factor] - out[0.125])) > 0
 
 
-@pytest.mark.parametrize("scheme", [52, 72, 1, 2, 3, 4])
+@pytest.mark.parametrize("scheme", [52, 72, 1, 2, 3, 4, 6])
 def test_fused_paths_refuse_these_schemes(scheme):
     """The batched kernel implements schemes 50/51/71 only; the others are
     refused with an error instead of running another scheme."""
