@@ -47,6 +47,13 @@ struct Dir {
 // the 5 variables at the (T_d + 2R) x columns points, plus the WENO epsilon
 // of every face.  Viscous phase: u, v, w, T, mu on the tile box (radius 1)
 // and the 4 viscous fluxes of the faces of all three directions.
+//
+// Two layouts of the inviscid phase.  Compact: the face fluxes overwrite the
+// split fluxes after a barrier, so each thread keeps its items' results in
+// registers until then.  Separate (SEP): the face fluxes and the three
+// per-class epsilons of every face have their own words, so each item writes
+// its result straight to LDS; used when it leaves the workgroups per CU
+// unchanged.
 template <int TX, int TY, int TZ, int SCHEME>
 struct FusedLds {
   static constexpr int NT = TX * TY * TZ;
@@ -54,11 +61,16 @@ struct FusedLds {
   static constexpr int pts(int td) { return (td + 2 * R) * (NT / td); }
   static constexpr int nf(int td) { return (td + 1) * (NT / td); }
   static constexpr int mx3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
+  static constexpr int mx2(int a, int b) { return a > b ? a : b; }
   static constexpr int IWORDS = mx3(10 * pts(TX) + nf(TX), 10 * pts(TY) + nf(TY), 10 * pts(TZ) + nf(TZ));
+  static constexpr int IWORDS_SEP =
+      mx3(10 * pts(TX) + 8 * nf(TX), 10 * pts(TY) + 8 * nf(TY), 10 * pts(TZ) + 8 * nf(TZ));
   static constexpr int NB = (TX + 2) * (TY + 2) * (TZ + 2);
   static constexpr int NFV = nf(TX) + nf(TY) + nf(TZ);
   static constexpr int VWORDS = 5 * NB + 4 * NFV;
-  static constexpr int WORDS = IWORDS > VWORDS ? IWORDS : VWORDS;
+  static constexpr int CU_BYTES = 160 * 1024;
+  static constexpr bool SEP = CU_BYTES / (mx2(IWORDS_SEP, VWORDS) * 8) >= CU_BYTES / (mx2(IWORDS, VWORDS) * 8);
+  static constexpr int WORDS = mx2(SEP ? IWORDS_SEP : IWORDS, VWORDS);
   static constexpr int BYTES = WORDS * 8;
   static constexpr int VBYTES = VWORDS * 8;
 };
@@ -141,6 +153,7 @@ __device__ __forceinline__ void inviscid_phase(double* smem, const StageBlock& B
   constexpr int NT = Dr::NT, R = Stencil<SCHEME>::R, W = Stencil<SCHEME>::W;
   constexpr int TD = Dr::TD, PC = Dr::PC, LP = TD + 2 * R, NPTS = LP * PC, NF = Dr::NF;
   constexpr int NI = 5 * NF, IPT = (NI + NT - 1) / NT;
+  constexpr bool SEP = FusedLds<TX, TY, TZ, SCHEME>::SEP;
   const int tid = threadIdx.x;
   const int origin = D == 0 ? t.i0 : (D == 1 ? t.j0 : t.k0);
   const gptr<const double> cv = B.cvi;
@@ -148,7 +161,7 @@ __device__ __forceinline__ void inviscid_phase(double* smem, const StageBlock& B
   const int s = B.sz;
   double* sFp = smem;
   double* sFm = smem + 5 * NPTS;
-  double* sEps = smem + 10 * NPTS;
+  double* sEps = smem + 10 * NPTS;  // SEP: [3][NF], the epsilon of rho, of the momenta and of the energy
 
   // split fluxes at the points of every column
   for (int p = tid; p < NPTS; p += NT) {
@@ -188,9 +201,48 @@ __device__ __forceinline__ void inviscid_phase(double* smem, const StageBlock& B
                B.met[p2 + 3 * D * s] * j2, B.met[p2 + (3 * D + 1) * s] * j2, B.met[p2 + (3 * D + 2) * s] * j2,
                eps3);
       sEps[f] = eps3[0];
+      if (SEP) {
+        sEps[NF + f] = eps3[1];
+        sEps[2 * NF + f] = eps3[2];
+      }
     }
   }
   __syncthreads();
+
+  if constexpr (SEP) {
+    double* sFc = smem + 10 * NPTS + 3 * NF;  // [5][NF]
+#pragma unroll 1
+    for (int it = tid; it < NI; it += NT) {
+      const int l = it / NF, f = it - l * NF;
+      double val = 0.0;
+      int ijk[3];
+      if (face_node<TX, TY, TZ, D>(B, t, f, ijk)) {
+        int col, sf;
+        Dr::face(f, col, sf);
+        const int sL = sf - 1 + R;
+        const double e = sEps[(l == 0 ? 0 : (l == 4 ? 2 : 1)) * NF + f];
+        const double* fpl = sFp + l * NPTS;
+        const double* fml = sFm + l * NPTS;
+        double vp[W], vm[W];
+#pragma unroll
+        for (int r = 0; r < W; ++r) {
+          vp[r] = fpl[Dr::template at<LP>(col, sL - W / 2 + r)];
+          vm[r] = fml[Dr::template at<LP>(col, sL - W / 2 + 1 + r)];
+        }
+        val = weno_face<SCHEME>(vp, vm, e);
+      }
+      sFc[it] = val;
+    }
+    __syncthreads();
+    if (t.valid) {
+      const int col = Dr::my_col(t.ti, t.tj, t.tk), pos = Dr::my_pos(t.ti, t.tj, t.tk);
+      const int fL = Dr::template at<TD + 1>(col, pos), fR = Dr::template at<TD + 1>(col, pos + 1);
+#pragma unroll
+      for (int l = 0; l < 5; ++l) dq[l] -= sFc[l * NF + fR] - sFc[l * NF + fL];
+    }
+    __syncthreads();
+    return;
+  }
 
   // one item per iteration (the loop is not unrolled, so the WENO evaluations
   // of different items do not overlap in registers); each result is kept in
