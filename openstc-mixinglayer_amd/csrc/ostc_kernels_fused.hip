@@ -52,9 +52,12 @@ struct Dir {
 // split fluxes after a barrier, so each thread keeps its items' results in
 // registers until then.  Separate (SEP): the face fluxes and the three
 // per-class epsilons of every face have their own words, so each item writes
-// its result straight to LDS; used when it leaves the workgroups per CU
-// unchanged.
-template <int TX, int TY, int TZ, int SCHEME>
+// its result straight to LDS.  And two of the viscous phase: one work list
+// for the faces of all three directions, or (VSPLIT) the x and y faces, then
+// the z faces, with flux words for the larger pass only.  Each choice takes
+// the smaller layout only when the larger one would hold fewer workgroups on
+// a CU than the register budget WPE (waves per SIMD) allows.
+template <int TX, int TY, int TZ, int SCHEME, int WPE>
 struct FusedLds {
   static constexpr int NT = TX * TY * TZ;
   static constexpr int R = Stencil<SCHEME>::R;
@@ -62,14 +65,20 @@ struct FusedLds {
   static constexpr int nf(int td) { return (td + 1) * (NT / td); }
   static constexpr int mx3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
   static constexpr int mx2(int a, int b) { return a > b ? a : b; }
+  static constexpr int mn2(int a, int b) { return a < b ? a : b; }
+  static constexpr int CU_BYTES = 160 * 1024;
+  static constexpr int WGS = mx2(1, WPE * 4 * 64 / NT);  // workgroups per CU the registers allow
+  static constexpr int wgs(int words) { return mn2(WGS, CU_BYTES / (words * 8)); }
   static constexpr int IWORDS = mx3(10 * pts(TX) + nf(TX), 10 * pts(TY) + nf(TY), 10 * pts(TZ) + nf(TZ));
   static constexpr int IWORDS_SEP =
       mx3(10 * pts(TX) + 8 * nf(TX), 10 * pts(TY) + 8 * nf(TY), 10 * pts(TZ) + 8 * nf(TZ));
   static constexpr int NB = (TX + 2) * (TY + 2) * (TZ + 2);
   static constexpr int NFV = nf(TX) + nf(TY) + nf(TZ);
-  static constexpr int VWORDS = 5 * NB + 4 * NFV;
-  static constexpr int CU_BYTES = 160 * 1024;
-  static constexpr bool SEP = CU_BYTES / (mx2(IWORDS_SEP, VWORDS) * 8) >= CU_BYTES / (mx2(IWORDS, VWORDS) * 8);
+  static constexpr int VWORDS_ONE = 5 * NB + 4 * NFV;
+  static constexpr int VWORDS_TWO = 5 * NB + 4 * mx2(nf(TX) + nf(TY), nf(TZ));
+  static constexpr bool VSPLIT = wgs(mx2(IWORDS, VWORDS_TWO)) > wgs(mx2(IWORDS, VWORDS_ONE));
+  static constexpr int VWORDS = VSPLIT ? VWORDS_TWO : VWORDS_ONE;
+  static constexpr bool SEP = wgs(mx2(IWORDS_SEP, VWORDS)) >= wgs(mx2(IWORDS, VWORDS));
   static constexpr int WORDS = mx2(SEP ? IWORDS_SEP : IWORDS, VWORDS);
   static constexpr int BYTES = WORDS * 8;
   static constexpr int VBYTES = VWORDS * 8;
@@ -146,14 +155,14 @@ __device__ __forceinline__ bool face_node(const StageBlock& B, const TileCtx& t,
 // Inviscid fluxes along direction D, subtracted from dq.  The face work is
 // spread over (variable, face) items, 5 NF of them for NT threads, so every
 // wave carries the same share whatever NF mod NT is.
-template <int TX, int TY, int TZ, int SCHEME, int D>
+template <int TX, int TY, int TZ, int SCHEME, int WPE, int D>
 __device__ __forceinline__ void inviscid_phase(double* smem, const StageBlock& B, const Params& P,
                                                const TileCtx& t, double dq[5]) {
   using Dr = Dir<TX, TY, TZ, D>;
   constexpr int NT = Dr::NT, R = Stencil<SCHEME>::R, W = Stencil<SCHEME>::W;
   constexpr int TD = Dr::TD, PC = Dr::PC, LP = TD + 2 * R, NPTS = LP * PC, NF = Dr::NF;
   constexpr int NI = 5 * NF, IPT = (NI + NT - 1) / NT;
-  constexpr bool SEP = FusedLds<TX, TY, TZ, SCHEME>::SEP;
+  constexpr bool SEP = FusedLds<TX, TY, TZ, SCHEME, WPE>::SEP;
   const int tid = threadIdx.x;
   const int origin = D == 0 ? t.i0 : (D == 1 ? t.j0 : t.k0);
   const gptr<const double> cv = B.cvi;
@@ -356,8 +365,9 @@ __device__ __forceinline__ void viscous_face(const double* sV, double* sFv, cons
 
 // Viscous fluxes of the three directions, added to dq in the reference's
 // order (x, then y, then z).  The faces of all three directions form one
-// work list, so the threads share them evenly.
-template <int TX, int TY, int TZ>
+// work list, so the threads share them evenly (VSPLIT: two lists, the x and y
+// faces, then the z faces, reusing the flux words).
+template <int TX, int TY, int TZ, bool VSPLIT>
 __device__ __forceinline__ void viscous_phase(double* smem, const StageBlock& B, const Params& P,
                                               const TileCtx& t, double dq[5]) {
   constexpr int NT = TX * TY * TZ, BX = TX + 2, BY = TY + 2, NB = (TX + 2) * (TY + 2) * (TZ + 2);
@@ -383,7 +393,41 @@ __device__ __forceinline__ void viscous_phase(double* smem, const StageBlock& B,
   __syncthreads();
   double* sF0 = smem + 5 * NB;
   double* sF1 = sF0 + 4 * NF0;
-  double* sF2 = sF1 + 4 * NF1;
+  double* sF2 = VSPLIT ? sF0 : sF1 + 4 * NF1;
+  if constexpr (VSPLIT) {
+#pragma unroll 1
+    for (int it = threadIdx.x; it < NF0 + NF1; it += NT) {
+      if (it < NF0) viscous_face<TX, TY, TZ, 0>(smem, sF0, B, P, t, it);
+      else viscous_face<TX, TY, TZ, 1>(smem, sF1, B, P, t, it - NF0);
+    }
+    __syncthreads();
+    if (t.valid) {
+      {
+        const int col = D0::my_col(t.ti, t.tj, t.tk), pos = D0::my_pos(t.ti, t.tj, t.tk);
+        const int fL = D0::template at<D0::TD + 1>(col, pos), fR = D0::template at<D0::TD + 1>(col, pos + 1);
+#pragma unroll
+        for (int l = 0; l < 4; ++l) dq[l + 1] += sF0[l * NF0 + fR] - sF0[l * NF0 + fL];
+      }
+      {
+        const int col = D1::my_col(t.ti, t.tj, t.tk), pos = D1::my_pos(t.ti, t.tj, t.tk);
+        const int fL = D1::template at<D1::TD + 1>(col, pos), fR = D1::template at<D1::TD + 1>(col, pos + 1);
+#pragma unroll
+        for (int l = 0; l < 4; ++l) dq[l + 1] += sF1[l * NF1 + fR] - sF1[l * NF1 + fL];
+      }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int it = threadIdx.x; it < NF2; it += NT) viscous_face<TX, TY, TZ, 2>(smem, sF2, B, P, t, it);
+    __syncthreads();
+    if (t.valid) {
+      const int col = D2::my_col(t.ti, t.tj, t.tk), pos = D2::my_pos(t.ti, t.tj, t.tk);
+      const int fL = D2::template at<D2::TD + 1>(col, pos), fR = D2::template at<D2::TD + 1>(col, pos + 1);
+#pragma unroll
+      for (int l = 0; l < 4; ++l) dq[l + 1] += sF2[l * NF2 + fR] - sF2[l * NF2 + fL];
+    }
+    __syncthreads();
+    return;
+  }
 #pragma unroll 1
   for (int it = threadIdx.x; it < NFV; it += NT) {
     if (it < NF0) viscous_face<TX, TY, TZ, 0>(smem, sF0, B, P, t, it);
@@ -472,15 +516,15 @@ __global__ __launch_bounds__(TX * TY * TZ) __attribute__((amdgpu_waves_per_eu(WP
       for (int l = 0; l < 5; ++l) dq[l] -= sigma * (B.cvi[p + l * s] - B.sp_mean[q0 + l * B.n0]);
     }
     if (MODE != 2) {
-      inviscid_phase<TX, TY, TZ, SCHEME, 0>(smem, B, A.P, t, dq);
-      inviscid_phase<TX, TY, TZ, SCHEME, 1>(smem, B, A.P, t, dq);
-      inviscid_phase<TX, TY, TZ, SCHEME, 2>(smem, B, A.P, t, dq);
+      inviscid_phase<TX, TY, TZ, SCHEME, WPE, 0>(smem, B, A.P, t, dq);
+      inviscid_phase<TX, TY, TZ, SCHEME, WPE, 1>(smem, B, A.P, t, dq);
+      inviscid_phase<TX, TY, TZ, SCHEME, WPE, 2>(smem, B, A.P, t, dq);
     } else if (t.valid) {
       // split stage: the inviscid part of dq was written by the MODE 1 launch
 #pragma unroll
       for (int l = 0; l < 5; ++l) dq[l] = B.dq[q0 + l * B.n0];
     }
-    if (VISC) viscous_phase<TX, TY, TZ>(smem, B, A.P, t, dq);
+    if (VISC) viscous_phase<TX, TY, TZ, FusedLds<TX, TY, TZ, SCHEME, WPE>::VSPLIT>(smem, B, A.P, t, dq);
     if (!t.valid) continue;
     const int p = poff(B, i, j, k), s = B.sz;
     if (MODE == 1) {
@@ -584,8 +628,8 @@ TileShape fused_shape(int, int index) { return kShapes[index]; }
 
 template <int TX, int TY, int TZ, int WPE, int SCHEME>
 static int launch_shape(int mode, int visc, const FusedArgs& a, int n_wg, hipStream_t st) {
-  constexpr int bytes = FusedLds<TX, TY, TZ, SCHEME>::BYTES;
-  constexpr int vbytes = FusedLds<TX, TY, TZ, SCHEME>::VBYTES;
+  constexpr int bytes = FusedLds<TX, TY, TZ, SCHEME, WPE>::BYTES;
+  constexpr int vbytes = FusedLds<TX, TY, TZ, SCHEME, WPE>::VBYTES;
   if (bytes > 160 * 1024) return -1;
   static bool attr_set = false;  // dynamic LDS above 64 KiB has to be enabled per kernel
   if (!attr_set) {
@@ -639,8 +683,8 @@ int fused_lds_bytes(int scheme, int index) {
   int i = 0;
 #define X(a_, b_, c_, w_)                                                        \
   if (index == i) {                                                          \
-    if (scheme == 71) return FusedLds<a_, b_, c_, 71>::BYTES;                \
-    return FusedLds<a_, b_, c_, 51>::BYTES;                                  \
+    if (scheme == 71) return FusedLds<a_, b_, c_, 71, w_>::BYTES;            \
+    return FusedLds<a_, b_, c_, 51, w_>::BYTES;                              \
   }                                                                          \
   ++i;
   OSTC_SHAPES(X)
