@@ -882,7 +882,7 @@ __device__ __forceinline__ void inviscid_ch_body(const FrArgs& A, const int wx, 
       }
     }
     constexpr double eps{1e-40};
-    const double eps_scaled = eps * A.weno_eps_scale * 0.25 * (C.kxJ * C.kxJ + C.kyJ * C.kyJ + C.kzJ * C.kzJ);
+    const double eps_scaled = eps * A.weno_eps_scale * 0.25 * (C.nj[0] * C.nj[0] + C.nj[1] * C.nj[1] + C.nj[2] * C.nj[2]);
     // acoustic and entropy fields of this half
     const int l_lo = half == 0 ? 0 : 3, l_hi = half == 0 ? 3 : 5;
 #pragma unroll 1
@@ -900,8 +900,8 @@ __device__ __forceinline__ void inviscid_ch_body(const FrArgs& A, const int wx, 
         }
 #pragma unroll
         for (int n = 0; n < NS; ++n) {
-          ap += cas * C.alpha_l[n] * Fp[(5 + n) * NP + sl - W / 2 + r];
-          am += cas * C.alpha_l[n] * Fm[(5 + n) * NP + sl - W / 2 + 1 + r];
+          ap += cas * C.coupling[n] * Fp[(5 + n) * NP + sl - W / 2 + r];
+          am += cas * C.coupling[n] * Fm[(5 + n) * NP + sl - W / 2 + 1 + r];
         }
 #pragma unroll
         for (int qq = 0; qq < W; ++qq) {
@@ -919,11 +919,11 @@ __device__ __forceinline__ void inviscid_ch_body(const FrArgs& A, const int wx, 
     for (int l = 0; l < NS; ++l) {
       if ((l < SH) != (half == 0)) continue;
       double vPlus[W], vMinus[W];
-      const double svm = C.svm[l];
+      const double y = C.y[l];
 #pragma unroll
       for (int r = 0; r < W; ++r) {
-        vPlus[r] = -svm * Fp[sl - W / 2 + r] + Fp[(5 + l) * NP + sl - W / 2 + r];
-        vMinus[r] = -svm * Fm[sl - W / 2 + 1 + r] + Fm[(5 + l) * NP + sl - W / 2 + 1 + r];
+        vPlus[r] = -y * Fp[sl - W / 2 + r] + Fp[(5 + l) * NP + sl - W / 2 + r];
+        vMinus[r] = -y * Fm[sl - W / 2 + 1 + r] + Fm[(5 + l) * NP + sl - W / 2 + 1 + r];
       }
       double v;
       if constexpr (W == 7) v = weno7z(vPlus, vMinus, eps_scaled);
@@ -933,46 +933,35 @@ __device__ __forceinline__ void inviscid_ch_body(const FrArgs& A, const int wx, 
   }
   __syncthreads();
   if (active) {
-    // back projection (src/WENO.cu:1121-1142), fields accumulated in order
-    const double um = C.um, vm = C.vm, wm = C.wm, cm = C.cm, gm1 = C.gm1, kx = C.kx, ky = C.ky, kz = C.kz;
-    const double hm = C.hm, Ukb = C.Uk_bar;
-    const double c0 = fch[0 * NF + q], c1 = fch[1 * NF + q], c2 = fch[2 * NF + q], c3 = fch[3 * NF + q],
-                 c4 = fch[4 * NF + q];
-    double f0 = c0;  // 1 * fChar[0] is exact
-    f0 = f0 + kx * c1;
-    f0 = f0 + ky * c2;
-    f0 = f0 + kz * c3;
-    f0 = f0 + c4;
+    // back projection (src/WENO.cu:1121-1142): flux of variable r =
+    // sum over the waves w = 0..4, in order, of R[r][w] times the wave's
+    // characteristic flux (the first term initialises the sum)
+    const double cw[5] = {fch[0 * NF + q], fch[1 * NF + q], fch[2 * NF + q], fch[3 * NF + q], fch[4 * NF + q]};
+    double f0 = cw[0];  // R[0][0] = 1: the product is exact
+#pragma unroll
+    for (int w = 1; w < 5; ++w) f0 = f0 + (w == 4 ? cw[4] : pick3(C.n, w - 1) * cw[w]);
     if (half == 0) {
-      double f1a = (um - kx * cm) * c0, f2 = (vm - ky * cm) * c0, f3 = (wm - kz * cm) * c0,
-             f4 = (hm - Ukb * cm) * c0;
-      f1a = f1a + kx * um * c1;
-      f2 = f2 + (kx * vm + kz * cm) * c1;
-      f3 = f3 + (kx * wm - ky * cm) * c1;
-      f4 = f4 + (kx * (hm - cm * cm / gm1) + (kz * vm - ky * wm) * cm) * c1;
-      f1a = f1a + (ky * um - kz * cm) * c2;
-      f2 = f2 + ky * vm * c2;
-      f3 = f3 + (ky * wm + kx * cm) * c2;
-      f4 = f4 + (ky * (hm - cm * cm / gm1) + (kx * wm - kz * um) * cm) * c2;
-      f1a = f1a + (kz * um + ky * cm) * c3;
-      f2 = f2 + (kz * vm - kx * cm) * c3;
-      f3 = f3 + kz * wm * c3;
-      f4 = f4 + (kz * (hm - cm * cm / gm1) + (ky * um - kx * vm) * cm) * c3;
-      f1a = f1a + (um + kx * cm) * c4;
-      f2 = f2 + (vm + ky * cm) * c4;
-      f3 = f3 + (wm + kz * cm) * c4;
-      f4 = f4 + (hm + Ukb * cm) * c4;
+      double col[5], f[5];
+      char_column<NS>(C, 0, col);
+#pragma unroll
+      for (int r = 1; r < 5; ++r) f[r] = col[r] * cw[0];
+#pragma unroll
+      for (int w = 1; w < 5; ++w) {
+        char_column<NS>(C, w, col);
+#pragma unroll
+        for (int r = 1; r < 5; ++r) f[r] = f[r] + col[r] * cw[w];
+      }
       double add{0};
 #pragma unroll
-      for (int l = 0; l < NS; ++l) add += C.alpha_l[l] * fch[(5 + l) * NF + q];
+      for (int l = 0; l < NS; ++l) add += C.coupling[l] * fch[(5 + l) * NF + q];
       fc[0] = f0;
-      fc[1] = f1a;
-      fc[2] = f2;
-      fc[3] = f3;
-      fc[4] = f4 - add * cm * cm / gm1;
+      fc[1] = f[1];
+      fc[2] = f[2];
+      fc[3] = f[3];
+      fc[4] = f[4] - add * C.c * C.c / C.gm1;
     } else {
 #pragma unroll
-      for (int l = 0; l < NS; ++l) fc[5 + l] = C.svm[l] * f0 + fch[(5 + l) * NF + q];
+      for (int l = 0; l < NS; ++l) fc[5 + l] = C.y[l] * f0 + fch[(5 + l) * NF + q];
       if (A.pp) {
         double ql[NS], qr[NS];
 #pragma unroll
