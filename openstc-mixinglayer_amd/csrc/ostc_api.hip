@@ -462,9 +462,12 @@ int upload_blocks(ostc_ctx c) {
 }
 
 // The fused batched kernel and the reverse mode implement the component-wise
-// WENO schemes 50/51/71; the characteristic WENO (52, 72) and the primitive
-// reconstruction + Riemann schemes (1/3/4) run on the per-block kernels.
-bool fused_scheme(const ostc_ctx c) { return c->P.scheme == 51 || c->P.scheme == 50 || c->P.scheme == 71; }
+// WENO schemes 50/51/71 and the characteristic WENO-Z 52/72; the primitive
+// reconstruction + Riemann schemes (1/2/3/4) and EP6 run on the per-block kernels.
+bool fused_scheme(const ostc_ctx c) {
+  const int sc = c->P.scheme;
+  return sc == 51 || sc == 50 || sc == 71 || sc == 52 || sc == 72;
+}
 
 int need_fused(ostc_ctx c) {
   for (auto& B : c->blocks)  // the fused kernel indexes a block's planes with 32-bit offsets
@@ -472,7 +475,7 @@ int need_fused(ostc_ctx c) {
       return fail(c, OSTC_ERR_UNSUPPORTED, "block too large for the batched kernel (9 padded planes >= 2^31 doubles)");
   if (fused_scheme(c)) return OSTC_OK;
   return fail(c, OSTC_ERR_UNSUPPORTED,
-              "inviscid_scheme 52/72/1/2/3/4/6 run with OSTC_MODE_PLAIN and the stage operators only");
+              "inviscid_scheme 1/2/3/4/6 run with OSTC_MODE_PLAIN and the stage operators only");
 }
 
 int set_tiles(ostc_ctx c, int index, int n_wg) {

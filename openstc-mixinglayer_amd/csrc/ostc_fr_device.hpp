@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "openstc/mech.h"
+#include "ostc_select.hpp"
 
 namespace ostc {
 namespace fr {

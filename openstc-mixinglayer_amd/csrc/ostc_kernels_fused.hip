@@ -79,7 +79,13 @@ struct FusedLds {
   static constexpr bool VSPLIT = wgs(mx2(IWORDS, VWORDS_TWO)) > wgs(mx2(IWORDS, VWORDS_ONE));
   static constexpr int VWORDS = VSPLIT ? VWORDS_TWO : VWORDS_ONE;
   static constexpr bool SEP = wgs(mx2(IWORDS_SEP, VWORDS)) >= wgs(mx2(IWORDS, VWORDS));
-  static constexpr int WORDS = mx2(SEP ? IWORDS_SEP : IWORDS, VWORDS);
+  // characteristic schemes (52, 72): split fluxes, the face states (AirFace:
+  // 9 words, and the WENO epsilon) and the 5 characteristic face fluxes
+  static constexpr bool CH = SCHEME == 52 || SCHEME == 72;
+  static constexpr int CH_STATE = 10;
+  static constexpr int IWORDS_CH = mx3(10 * pts(TX) + (CH_STATE + 5) * nf(TX), 10 * pts(TY) + (CH_STATE + 5) * nf(TY),
+                                       10 * pts(TZ) + (CH_STATE + 5) * nf(TZ));
+  static constexpr int WORDS = mx2(CH ? IWORDS_CH : (SEP ? IWORDS_SEP : IWORDS), VWORDS);
   static constexpr int BYTES = WORDS * 8;
   static constexpr int VBYTES = VWORDS * 8;
 };
@@ -152,6 +158,133 @@ __device__ __forceinline__ bool face_node(const StageBlock& B, const TileCtx& t,
          ijk[2] < B.mz + (D == 2 ? 1 : 0);
 }
 
+// Characteristic WENO along direction D (schemes 52 / 72,
+// compute_weno_flux_ch<Air> "method 1", src/WENO.cu:1147-1453), after the
+// split fluxes are staged: (1) the Roe-averaged state of every face
+// (AirFace) and its epsilon into LDS; (2) (field, face) items: the left
+// eigenvector row of the field projects the W split fluxes of each side, and
+// WENO-Z gives the characteristic flux; (3) each cell forms the flux of its
+// two faces from the right eigenvectors and subtracts their difference.  The
+// arithmetic is that of the per-block kernel (k_weno_pencil), element by
+// element.
+template <int TX, int TY, int TZ, int SCHEME, int D>
+__device__ __forceinline__ void characteristic_faces(double* smem, const StageBlock& B, const Params& P,
+                                                     const TileCtx& t, double dq[5]) {
+  using Dr = Dir<TX, TY, TZ, D>;
+  constexpr int NT = Dr::NT, R = Stencil<SCHEME>::R, W = Stencil<SCHEME>::W;
+  constexpr int TD = Dr::TD, LP = TD + 2 * R, NPTS = LP * Dr::PC, NF = Dr::NF, NI = 5 * NF;
+  constexpr int ST = 9;  // words of a face state: vel[3], enthalpy, c_sq, c, n[3]
+  static_assert(FusedLds<TX, TY, TZ, SCHEME, 1>::CH_STATE == ST + 1, "LDS words per face: state and epsilon");
+  const int tid = threadIdx.x;
+  const int s = B.sz;
+  const double* sFp = smem;
+  const double* sFm = smem + 5 * NPTS;
+  double* sSt = smem + 10 * NPTS;        // [ST][NF] face states
+  double* sEps = sSt + ST * NF;          // [NF] WENO epsilon of each face
+  double* sCh = sEps + NF;               // [5][NF] characteristic face fluxes
+  for (int f = tid; f < NF; f += NT) {
+    int ijk[3];
+    if (!face_node<TX, TY, TZ, D>(B, t, f, ijk)) continue;
+    const int p1 = poff(B, ijk[0], ijk[1], ijk[2]);
+    const int p2 = p1 + (D == 0 ? 1 : (D == 1 ? B.sx : B.sx * B.sy));
+    double cvl[6], cvr[6];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      cvl[l] = B.cvi[p1 + l * s];
+      cvr[l] = B.cvi[p2 + l * s];
+    }
+    cvl[5] = B.bvi[p1 + 4 * s];
+    cvr[5] = B.bvi[p2 + 4 * s];
+    const double m1[3] = {B.met[p1 + 3 * D * s], B.met[p1 + (3 * D + 1) * s], B.met[p1 + (3 * D + 2) * s]};
+    const double m2[3] = {B.met[p2 + 3 * D * s], B.met[p2 + (3 * D + 1) * s], B.met[p2 + (3 * D + 2) * s]};
+    const double j1 = B.jac[p1], j2 = B.jac[p2];
+    AirFace F;
+    air_face_state(cvl, cvr, m1, m2, j1, j2, F);
+    sSt[0 * NF + f] = F.vel[0];
+    sSt[1 * NF + f] = F.vel[1];
+    sSt[2 * NF + f] = F.vel[2];
+    sSt[3 * NF + f] = F.enthalpy;
+    sSt[4 * NF + f] = F.c_sq;
+    sSt[5 * NF + f] = F.c;
+    sSt[6 * NF + f] = F.n[0];
+    sSt[7 * NF + f] = F.n[1];
+    sSt[8 * NF + f] = F.n[2];
+    constexpr double eps{1e-40};
+    const double ax = m1[0] * j1 + m2[0] * j2, ay = m1[1] * j1 + m2[1] * j2, az = m1[2] * j1 + m2[2] * j2;
+    sEps[f] = eps * P.weno_eps_scale * 0.25 * (ax * ax + ay * ay + az * az);
+  }
+  __syncthreads();
+  auto load_face = [&](int f, AirFace& F) {
+    F.vel[0] = sSt[0 * NF + f];
+    F.vel[1] = sSt[1 * NF + f];
+    F.vel[2] = sSt[2 * NF + f];
+    F.enthalpy = sSt[3 * NF + f];
+    F.c_sq = sSt[4 * NF + f];
+    F.c = sSt[5 * NF + f];
+    F.n[0] = sSt[6 * NF + f];
+    F.n[1] = sSt[7 * NF + f];
+    F.n[2] = sSt[8 * NF + f];
+  };
+#pragma unroll 1
+  for (int it = tid; it < NI; it += NT) {
+    const int l = it / NF, f = it - l * NF;
+    double val = 0.0;
+    int ijk[3];
+    if (face_node<TX, TY, TZ, D>(B, t, f, ijk)) {
+      int col, sf;
+      Dr::face(f, col, sf);
+      const int sL = sf - 1 + R;
+      AirFace F;
+      load_face(f, F);
+      double Lr[5];
+      air_left_row(F, l, Lr);
+      double vp[W], vm[W];
+#pragma unroll
+      for (int r = 0; r < W; ++r) {
+        const int ip = Dr::template at<LP>(col, sL - W / 2 + r), im = Dr::template at<LP>(col, sL - W / 2 + 1 + r);
+        vp[r] = Lr[0] * sFp[ip];
+        vm[r] = Lr[0] * sFm[im];
+#pragma unroll
+        for (int n = 1; n < 5; ++n) {
+          vp[r] += Lr[n] * sFp[n * NPTS + ip];
+          vm[r] += Lr[n] * sFm[n * NPTS + im];
+        }
+      }
+      if constexpr (SCHEME == 72) val = weno7z(vp, vm, sEps[f]);
+      else val = weno5z(vp, vm, sEps[f]);
+    }
+    sCh[it] = val;
+  }
+  __syncthreads();
+  if (t.valid) {
+    const int col = Dr::my_col(t.ti, t.tj, t.tk), pos = Dr::my_pos(t.ti, t.tj, t.tk);
+    const int fs[2] = {Dr::template at<TD + 1>(col, pos), Dr::template at<TD + 1>(col, pos + 1)};
+    double fc[2][5];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int f = fs[side];
+      AirFace F;
+      load_face(f, F);
+      double fch[5];
+#pragma unroll
+      for (int w = 0; w < 5; ++w) fch[w] = sCh[w * NF + f];
+      double Rc[5];
+      air_right_column(F, 0, Rc);
+#pragma unroll
+      for (int r = 0; r < 5; ++r) fc[side][r] = Rc[r] * fch[0];
+#pragma unroll
+      for (int w = 1; w < 5; ++w) {
+        air_right_column(F, w, Rc);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) fc[side][r] += Rc[r] * fch[w];
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < 5; ++l) dq[l] -= fc[1][l] - fc[0][l];
+  }
+  __syncthreads();
+}
+
 // Inviscid fluxes along direction D, subtracted from dq.  The face work is
 // spread over (variable, face) items, 5 NF of them for NT threads, so every
 // wave carries the same share whatever NF mod NT is.
@@ -197,6 +330,10 @@ __device__ __forceinline__ void inviscid_phase(double* smem, const StageBlock& B
 #pragma unroll
       for (int l = 0; l < 5; ++l) { sFp[l * NPTS + p] = fp[l]; sFm[l * NPTS + p] = fm[l]; }
     }
+  }
+  if constexpr (Stencil<SCHEME>::CH) {
+    characteristic_faces<TX, TY, TZ, SCHEME, D>(smem, B, P, t, dq);
+    return;
   }
   // the epsilon of every face (src/WENO.cu:1469-1482), shared by its 5 items
   for (int f = tid; f < NF; f += NT) {
@@ -665,6 +802,8 @@ static int launch_scheme(int mode, const FusedArgs& a, int n_wg, hipStream_t st)
     case 51: return launch_shape<TX, TY, TZ, WPE, 51>(mode, a.P.visc, a, n_wg, st);
     case 50: return launch_shape<TX, TY, TZ, WPE, 50>(mode, a.P.visc, a, n_wg, st);
     case 71: return launch_shape<TX, TY, TZ, WPE, 71>(mode, a.P.visc, a, n_wg, st);
+    case 52: return launch_shape<TX, TY, TZ, WPE, 52>(mode, a.P.visc, a, n_wg, st);
+    case 72: return launch_shape<TX, TY, TZ, WPE, 72>(mode, a.P.visc, a, n_wg, st);
     default: return -1;
   }
 }
@@ -684,6 +823,8 @@ int fused_lds_bytes(int scheme, int index) {
 #define X(a_, b_, c_, w_)                                                        \
   if (index == i) {                                                          \
     if (scheme == 71) return FusedLds<a_, b_, c_, 71, w_>::BYTES;            \
+    if (scheme == 52) return FusedLds<a_, b_, c_, 52, w_>::BYTES;            \
+    if (scheme == 72) return FusedLds<a_, b_, c_, 72, w_>::BYTES;            \
     return FusedLds<a_, b_, c_, 51, w_>::BYTES;                              \
   }                                                                          \
   ++i;

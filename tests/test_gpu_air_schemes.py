@@ -1,6 +1,6 @@
-"""HIP per-block kernels of the air schemes beyond component-wise WENO vs the
-CPU oracle (needs a GPU): characteristic WENO-Z5 / WENO-Z7 (inviscid_scheme
-52 / 72), the primitive reconstruction + Riemann path (1 LF, 3 AUSM+, 4 HLLC
+"""HIP kernels of the air schemes beyond component-wise WENO vs the CPU
+oracle (needs a GPU): characteristic WENO-Z5 / WENO-Z7 (inviscid_scheme
+52 / 72; per-block and fused batched kernels), the primitive reconstruction + Riemann path (1 LF, 3 AUSM+, 4 HLLC
 with first-order, MUSCL or NND2 reconstruction) and the sixth-order
 energy-preserving split form (6; reconstruction unused).
 
@@ -102,10 +102,57 @@ def test_roe_entropy_fix_factor(factor):
     assert np.max(np.abs(out[factor] - out[0.125])) > 0
 
 
-@pytest.mark.parametrize("scheme", [52, 72, 1, 2, 3, 4, 6])
+@pytest.mark.parametrize("scheme", [52, 72])
+@pytest.mark.parametrize("kind", ["single", "multi", "rough"])
+@pytest.mark.parametrize("index", range(6))
+def test_batched_characteristic_dq(scheme, kind, index):
+    """The fused batched kernel's characteristic WENO-Z5 / Z7 (52 / 72) against
+    the oracle, every tile shape: dq within the per-block path's 1e-12."""
+    case = _case(kind, scheme, 2)
+    print(f"[gpu] batched air scheme {scheme} dq, {kind}, tile entry {index}", flush=True)
+    o = oracle_for(case)
+    s = pkg.Solver(case)
+    s.setup_mixing_layer()
+    for b in range(len(case["blocks"])):
+        s.set(b, pkg.FIELD_CV, o.array(b, O.CV))
+        s.set(b, pkg.FIELD_BV, o.array(b, O.BV))
+        s.set(b, pkg.FIELD_MUL, o.array(b, O.MUL))
+    s.set_tile(index, 0)
+    for b in range(len(case["blocks"])):
+        o.rhs(b)
+        s.compute_rhs_batched(b)
+        e = rel_err(s.get(b, pkg.FIELD_DQ), o.array(b, O.DQ))
+        assert np.all(e <= DQ_TOL), (index, b, e)
+    s.close()
+
+
+@pytest.mark.parametrize("scheme", [52, 72])
+@pytest.mark.parametrize("kind", ["single", "multi"])
+def test_batched_characteristic_steps(scheme, kind):
+    """10 batched RK3 steps (graph-replayed) with 52 / 72 against the oracle's
+    sequential steps, and against the per-block path."""
+    n = 10
+    case = _case(kind, scheme, 2)
+    print(f"[gpu] batched air scheme {scheme}, {n} RK3 steps, {kind}", flush=True)
+    o = oracle_for(case)
+    s = pkg.Solver(case)
+    s.setup_mixing_layer()
+    s.initialize()
+    dt_o = o.step(n)
+    dt_g = s.rk3(n, pkg.MODE_BATCHED)
+    assert abs(dt_g - dt_o) <= 1e-12 * dt_o, (dt_g, dt_o)
+    g = s.ngg
+    for b in range(len(case["blocks"])):
+        e = rel_err(interior(s.get(b, pkg.FIELD_BV), g), interior(o.array(b, O.BV), g))
+        print(f"  block {b} bv rel err {e}", flush=True)
+        assert np.all(e <= BV_TOL), (b, e)
+    s.close()
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4, 6])
 def test_fused_paths_refuse_these_schemes(scheme):
-    """The batched kernel implements schemes 50/51/71 only; the others are
-    refused with an error instead of running another scheme."""
+    """The batched kernel implements schemes 50/51/71 and 52/72; the others
+    are refused with an error instead of running another scheme."""
     case = cases.single_block_case(16, scheme=scheme, reconstruction=2)
     s = pkg.Solver(case)
     s.setup_mixing_layer()
