@@ -133,6 +133,37 @@ def adjoint_leg(n: int, steps: int, every: int, h=1e-6):
                         f"momentum thickness"}
 
 
+def characteristic_leg(steps: int):
+    """The 24-block air layer with characteristic WENO-Z5 (inviscid_scheme 52):
+    autotuned batched steps (graph replay) against the per-block path, same
+    state and step count."""
+    case = pkg.cases.batched_case(scheme=52)
+    cells = case["n_cells"]
+    s = pkg.Solver(case)
+    s.setup_mixing_layer()
+    s.initialize()
+    idx, best = s.autotune(2)
+    s.rk3(2, pkg.MODE_BATCHED)
+    s.sync()
+    t0 = time.perf_counter()
+    s.rk3(steps, pkg.MODE_BATCHED, want_dt=False)
+    s.sync()
+    ms_b = (time.perf_counter() - t0) * 1e3 / steps
+    s.rk3(1, pkg.MODE_PLAIN)
+    s.sync()
+    t0 = time.perf_counter()
+    s.rk3(steps, pkg.MODE_PLAIN, want_dt=False)
+    s.sync()
+    ms_p = (time.perf_counter() - t0) * 1e3 / steps
+    s.close()
+    print(f"[bench] scheme 52, 24 blocks: batched {ms_b:.2f} ms/step (tile {pkg.tile_shapes()[idx]}, "
+          f"fused {best:.3f} ms/stage), per-block {ms_p:.2f} ms/step", file=sys.stderr, flush=True)
+    return {"workload": "24-block air mixing layer, characteristic WENO-Z5 (inviscid_scheme 52) + LF, viscous",
+            "cells": cells, "steps": steps, "value": cells / ms_b / 1e3, "unit": "Mcell-updates/s",
+            "ms_per_step": ms_b, "plain_ms_per_step": ms_p, "speedup_vs_plain": ms_p / ms_b,
+            "tile": list(pkg.tile_shapes()[idx]), "fused_ms_per_stage": best}
+
+
 def reacting_leg(x_cells, y_cells, z_cells, steps: int, warmup: int, label: str, cpu_sample=None, spatial=False):
     """Reacting H2/air mixing layer (9 species, 19 reactions, characteristic
     WENO-Z5 + LF-type splitting, mixture-averaged transport, finite-rate
@@ -308,6 +339,7 @@ def main():
     ap.add_argument("--tile", type=int, default=-1, metavar="INDEX",
                     help="use this entry of the tile table (one workgroup per tile) instead of the autotuner")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-characteristic", action="store_true", help="skip the scheme-52 batched/per-block leg")
     ap.add_argument("--adjoint-n", type=int, default=128)
     ap.add_argument("--adjoint-steps", type=int, default=50)
     ap.add_argument("--adjoint-every", type=int, default=10)
@@ -387,6 +419,8 @@ def main():
     roofline = fused_roofline(cells, avg_kernel_ms, value, tile_index, tile_split, tile_waves)
     s.close()
 
+    characteristic = None if a.no_characteristic else characteristic_leg(5)
+
     adjoint = None
     if not a.no_adjoint:
         adjoint = adjoint_leg(a.adjoint_n, a.adjoint_steps, a.adjoint_every)
@@ -422,7 +456,8 @@ def main():
                    "workgroups_per_cu": tile_waves,
                    "split_stage": bool(tile_split)},
         "value_graph": value_graph, "plain_value": plain_value, "speedup_vs_plain": value / plain_value,
-        "roofline": roofline, "cpu_baseline": cpu, "adjoint": adjoint, "reacting": reacting,
+        "roofline": roofline, "cpu_baseline": cpu, "characteristic": characteristic, "adjoint": adjoint,
+        "reacting": reacting,
     }
     print(json.dumps(out))
 

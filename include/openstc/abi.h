@@ -49,6 +49,10 @@ extern "C" {
 #define OSTC_MODE_BATCHED_TIMED 2  /* the batched kernels launched directly, with HIP events
                                       around every fused launch (ostc_fused_kernel_ms) */
 
+/* properties of the ghost layers after a batched stage (ostc_set_ghost_props) */
+#define OSTC_GHOST_PROPS_IN_OPS 0  /* each ghost op recomputes the nodes it writes (default) */
+#define OSTC_GHOST_PROPS_PASS 1    /* one more launch over every ghost layer and shared node layer */
+
 /* fields for ostc_get_field / ostc_set_field */
 #define OSTC_FIELD_CV 0
 #define OSTC_FIELD_BV 1
@@ -235,6 +239,13 @@ int ostc_initialize(ostc_ctx ctx);
 
 /* n_steps SSP-RK3 steps (src/RK.cuh:85-226).  last_dt may be NULL. */
 int ostc_step(ostc_ctx ctx, int n_steps, int mode, double* last_dt);
+
+/* Where a batched stage recomputes the viscosity and Mach number of the
+ * ghost layers (update_physical_properties over the padded domain,
+ * src/RK.cuh:216-223): OSTC_GHOST_PROPS_IN_OPS or OSTC_GHOST_PROPS_PASS.
+ * Both give bit-identical states; the first saves a pass over the ghost
+ * layers.  No counterpart in the reference (a launch-shape choice). */
+int ostc_set_ghost_props(ostc_ctx ctx, int mode);
 
 /* Dual time stepping with the DPLUR inner solver (temporal_scheme 2,
  * implicit_method 1: dual_time_stepping, src/DualTimeStepping.cuh:33-236,

@@ -34,6 +34,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libostc_hip.so")
 cases = importlib.import_module(__name__ + ".cases")
 
 MODE_PLAIN, MODE_BATCHED, MODE_BATCHED_TIMED = 0, 1, 2
+GHOST_PROPS_IN_OPS, GHOST_PROPS_PASS = 0, 1
 (FIELD_CV, FIELD_BV, FIELD_MUL, FIELD_MACH, FIELD_DQ, FIELD_DT_LOCAL, FIELD_JAC, FIELD_METRIC, FIELD_QN,
  FIELD_DQ_BAR, FIELD_CV_BAR) = range(11)
 MAX_MODES = 64
@@ -212,6 +213,7 @@ SIGNATURES = {
     "ostc_tile_count": (_I, [C.POINTER(_I)]),
     "ostc_tile_shape": (_I, [_I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "ostc_set_tile": (_I, [_P, _I, _I, _I]),
+    "ostc_set_ghost_props": (_I, [_P, _I]),
     "ostc_tile_config": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "ostc_autotune": (_I, [_P, _I, C.POINTER(_I), _DP]),
     "ostc_last_step_timing": (_I, [_P, _DP, C.POINTER(_LL)]),
@@ -462,6 +464,12 @@ class Solver:
     # -- tuning and timing -------------------------------------------------
     def set_tile(self, index: int, waves_per_cu: int = 0, split: int = 0):
         self._chk(lib().ostc_set_tile(self.ctx, index, waves_per_cu, split))
+
+    def set_ghost_props(self, mode: int):
+        """GHOST_PROPS_IN_OPS (default): the ghost ops of a batched stage
+        recompute the properties of the nodes they write; GHOST_PROPS_PASS: one
+        more launch over the ghost layers."""
+        self._chk(lib().ostc_set_ghost_props(self.ctx, mode))
 
     def tile_config(self):
         i, w, s = C.c_int(), C.c_int(), C.c_int()

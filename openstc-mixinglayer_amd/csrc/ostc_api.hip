@@ -110,6 +110,9 @@ struct ostc_ctx_s {
   double* d_scratch = nullptr;
   int n_first = 1024;
   double* d_dt = nullptr;
+  // properties of the ghost layers after a batched stage: computed by the
+  // ghost ops themselves (OSTC_GHOST_PROPS_IN_OPS) or by one more pass
+  int ghost_props = OSTC_GHOST_PROPS_IN_OPS;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool graph_valid = false;
@@ -526,8 +529,9 @@ FusedArgs fused_args(ostc_ctx c, int in, int out, int rk) {
   return a;
 }
 
-void launch_group(ostc_ctx c, const Group& g, int slot, int src, long long* n) {
-  if (g.hi > g.lo) ops_launch(c->d_blocks, (int)c->blocks.size(), c->d_ops, g.lo, g.hi, g.total, slot, src, c->st, n);
+void launch_group(ostc_ctx c, const Group& g, int slot, int src, long long* n, bool props = false) {
+  if (g.hi > g.lo)
+    ops_launch(c->d_blocks, (int)c->blocks.size(), c->d_ops, g.lo, g.hi, g.total, slot, src, c->st, n, props);
 }
 
 void launch_op(ostc_ctx c, int idx, int slot, long long* n) {
@@ -692,11 +696,12 @@ int batched_stage(ostc_ctx c, int rk, long long* n, int ev_base) {
     return fail(c, OSTC_ERR_UNSUPPORTED, "fused kernel: scheme/tile not available");
   *n += c->split ? 2 : 1;
   if (events) HIPCHK(c, hipEventRecord(c->evpool[ev_base + 2 * rk + 1], c->st));
-  launch_group(c, c->rim, out, in, n);
-  for (auto& g : c->bc_rounds) launch_group(c, g, out, out, n);
+  const bool fold = c->ghost_props == OSTC_GHOST_PROPS_IN_OPS;
+  launch_group(c, c->rim, out, in, n, fold);
+  for (auto& g : c->bc_rounds) launch_group(c, g, out, out, n, fold);
   if (c->inflow_noise) count_pass(c->d_pass, 0, (int)c->blocks.size(), c->st, n);
-  for (auto& g : c->comm_levels) launch_group(c, g, out, out, n);
-  launch_group(c, c->props, out, out, n);
+  for (auto& g : c->comm_levels) launch_group(c, g, out, out, n, fold);
+  if (!fold) launch_group(c, c->props, out, out, n);
   return OSTC_OK;
 }
 
@@ -1266,6 +1271,15 @@ int ostc_step(ostc_ctx c, int n_steps, int mode, double* last_dt) {
     HIPCHK(c, hipStreamSynchronize(c->st));
     *last_dt = dt;
   }
+  return OSTC_OK;
+}
+
+int ostc_set_ghost_props(ostc_ctx c, int mode) {
+  if (!c) return OSTC_ERR_ARG;
+  if (mode != OSTC_GHOST_PROPS_IN_OPS && mode != OSTC_GHOST_PROPS_PASS)
+    return fail(c, OSTC_ERR_ARG, "unknown ghost properties mode");
+  if (mode != c->ghost_props) invalidate_graph(c);
+  c->ghost_props = mode;
   return OSTC_OK;
 }
 

@@ -14,7 +14,7 @@ void padded_pass(const DevBlock& B, int slot, int what, hipStream_t st, long lon
 void count_pass(unsigned long long* pass, int lo, int hi, hipStream_t st, long long* n);
 void sponge_mean(const DevBlock& B, const unsigned long long* iter, hipStream_t st, long long* n);
 void ops_launch(const DevBlock* d_blocks, int n_blocks, const DevOp* d_ops, int op_lo, int op_hi, long long total, int slot,
-                int src, hipStream_t st, long long* n);
+                int src, hipStream_t st, long long* n, bool props = false);
 void store_last(const DevBlock& B, hipStream_t st);
 void residual_partial(const DevBlock& B, double* out4, int nblk, hipStream_t st);
 

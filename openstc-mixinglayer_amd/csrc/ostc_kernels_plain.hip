@@ -426,7 +426,7 @@ constexpr int kOpsInLds = 256;
 constexpr int kBlocksInLds = 64;
 
 __global__ __launch_bounds__(256) void k_ops(const DevBlock* blocks, int n_blocks, const DevOp* ops, int op_lo,
-                                             int op_hi, long long total, int slot, int src) {
+                                             int op_hi, long long total, int slot, int src, int props) {
   __shared__ long long s_begin[kOpsInLds];
   __shared__ DevBlock s_blocks[kBlocksInLds];
   const int n_ops = op_hi - op_lo;
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(256) void k_ops(const DevBlock* blocks, int n_block
     if (b <= gid) lo = mid; else hi = mid - 1;
   }
   const DevOp op = ops[op_lo + lo];
-  op_point(blk_lds ? s_blocks : blocks, op, gid - (op.begin - base), slot, src);
+  op_point(blk_lds ? s_blocks : blocks, op, gid - (op.begin - base), slot, src, props != 0);
 }
 
 // store_last_step and compute_square_of_dbv + reduction_of_dv_squared<4>
@@ -599,10 +599,11 @@ void padded_pass(const DevBlock& B, int slot, int what, hipStream_t st, long lon
 }
 
 void ops_launch(const DevBlock* d_blocks, int n_blocks, const DevOp* d_ops, int op_lo, int op_hi, long long total, int slot,
-                int src, hipStream_t st, long long* n) {
+                int src, hipStream_t st, long long* n, bool props) {
   if (total <= 0) return;
   const unsigned nb = (unsigned)((total + 255) / 256);
-  hipLaunchKernelGGL(k_ops, dim3(nb), dim3(256), 0, st, d_blocks, n_blocks, d_ops, op_lo, op_hi, total, slot, src);
+  hipLaunchKernelGGL(k_ops, dim3(nb), dim3(256), 0, st, d_blocks, n_blocks, d_ops, op_lo, op_hi, total, slot, src,
+                     props ? 1 : 0);
   *n += 1;
 }
 

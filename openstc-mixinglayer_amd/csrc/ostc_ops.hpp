@@ -42,9 +42,27 @@ struct DevOp {
   const unsigned long long* pass;  // OP_BC_INFLOW: boundary-condition passes per block so far
 };
 
+// update_physical_properties<Air> at one point (src/FieldOperation.cuh:107-140):
+// Sutherland viscosity of ring slot `slot` and the Mach number.
+__device__ __forceinline__ void props_point(const DevBlock& Z, long long p, int slot) {
+  const double* zbv = Z.bv[slot];
+  const long long zs = Z.sz;
+  const double t = zbv[p + 5 * zs];
+  const double u = zbv[p + zs], v = zbv[p + 2 * zs], w = zbv[p + 3 * zs];
+  const double V = sqrt(u * u + v * v + w * w);
+  constexpr double c_temp{kGamma * kRu / kMwAir};
+  Z.mul[slot][p] = sutherland(t);
+  Z.mach[p] = V / sqrt(c_temp * t);
+}
+
 // One op point.  Every op reads and writes the ring slot `slot` of cv/bv/mul.
+// With `props`, each op also recomputes the properties (props_point) of every
+// node whose primitives it writes, and the rim copy carries the viscosity of
+// slot `src` along: then no separate properties pass over the ghost layers is
+// needed (a node no op writes keeps the primitives and properties every slot
+// was initialised with).
 __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op, long long local, int slot,
-                                         int src) {
+                                         int src, bool props = false) {
   const int i = op.lo[0] + (int)(local % op.n[0]);
   const int j = op.lo[1] + (int)((local / op.n[0]) % op.n[1]);
   const int k = op.lo[2] + (int)(local / ((long long)op.n[0] * op.n[1]));
@@ -84,6 +102,7 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
                                                          : pidx(B, i, j, k);
         for (int l = 0; l < 6; ++l) bv[pg + l * s] = bv[ps + l * s];
         cv_point(B, pg);
+        if (props) props_point(B, pg, slot);
       }
       break;
     }
@@ -94,6 +113,7 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
         const long long q = pidx(B, i + g * di, j + g * dj, k + g * dk);
         for (int l = 0; l < 6; ++l) bv[q + l * s] = st[l];
         cv_point(B, q);
+        if (props) props_point(B, q, slot);
       }
       break;
     }
@@ -150,12 +170,14 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
       if (face_too) {
         for (int l = 0; l < 6; ++l) bv[p + l * s] = st[l];
         cv_point(B, p);
+        if (props) props_point(B, p, slot);
       }
       for (int g = 1; g <= B.g; ++g) {
         const long long pg = pidx(B, i + g * di, j + g * dj, k + g * dk);
 #pragma unroll
         for (int l = 0; l < 6; ++l) bv[pg + l * s] = st[l];
         cv_point(B, pg);
+        if (props) props_point(B, pg, slot);
       }
       break;
     }
@@ -179,20 +201,16 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
           tbv[pt + l * ts] = ave;
         }
         if (op.flags & 1) cv_point(T, pt);
+        if (props) props_point(T, pt, slot);
       } else {
         for (int l = 0; l < 6; ++l) bv[p + l * s] = tbv[pt + l * ts];
       }
       cv_point(B, p);
+      if (props) props_point(B, p, slot);
       break;
     }
     case OP_PROPS: {
-      const long long p = pidx(B, i, j, k);
-      const double t = bv[p + 5 * s];
-      const double u = bv[p + s], v = bv[p + 2 * s], w = bv[p + 3 * s];
-      const double V = sqrt(u * u + v * v + w * w);
-      constexpr double c_temp{kGamma * kRu / kMwAir};
-      B.mul[slot][p] = sutherland(t);
-      B.mach[p] = V / sqrt(c_temp * t);
+      props_point(B, pidx(B, i, j, k), slot);
       break;
     }
     case OP_CV_FROM_BV: {
@@ -203,6 +221,7 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
       const long long p = pidx(B, i, j, k);
       const double* sbv = B.bv[src];
       for (int l = 0; l < 6; ++l) bv[p + l * s] = sbv[p + l * s];
+      if (props) B.mul[slot][p] = B.mul[src][p];
       break;
     }
     default:
