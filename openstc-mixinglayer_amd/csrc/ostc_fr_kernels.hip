@@ -1012,6 +1012,7 @@ __device__ __forceinline__ void inviscid_pv_body(const FrArgs& A, const int wx, 
   constexpr int NV = 5 + NS;
   constexpr int TW = Tile<DIR>::TW, F = Tile<DIR>::F, TS = F - 1, P = TS + 4, NP = P * TW;
   extern __shared__ double lds[];
+  double* s_delta = lds + NV * (P > F ? P : F) * TW;  // [NP] entropy-fix deltas (Roe)
   const FrBlockDev& B = A.B;
   const int ext[3] = {B.mx, B.my, B.mz};
   const int m = ext[DIR];
@@ -1040,6 +1041,12 @@ __device__ __forceinline__ void inviscid_pv_body(const FrArgs& A, const int wx, 
       for (int l = 0; l < 5; ++l) lds[l * NP + slot] = B.bv[p + l * s];
 #pragma unroll
       for (int l = 0; l < NS; ++l) lds[(5 + l) * NP + slot] = B.sv[p + l * s];
+      if (A.scheme == 2) {  // the point's entropy-fix delta from its full metric, once for both of its faces
+        double mm[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) mm[c] = B.met[p + c * s];
+        s_delta[slot] = entropy_delta_fr(B.bv[p + s], B.bv[p + 2 * s], B.bv[p + 3 * s], B.cc[p], mm, A.entropy_fix);
+      }
     } else {
 #pragma unroll
       for (int l = 0; l < NV; ++l) lds[l * NP + slot] = 1.0;
@@ -1075,15 +1082,7 @@ __device__ __forceinline__ void inviscid_pv_body(const FrArgs& A, const int wx, 
       mr[c] = B.met[p1 + (3 * DIR + c) * s];
     }
     if (A.scheme == 2) {
-      double mm[9], delta[2];
-      const long long pts[2] = {p0, p1};
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-#pragma unroll
-        for (int c = 0; c < 9; ++c) mm[c] = B.met[pts[q] + c * s];
-        delta[q] = entropy_delta_fr(B.bv[pts[q] + s], B.bv[pts[q] + 2 * s], B.bv[pts[q] + 3 * s], B.cc[pts[q]], mm, A.entropy_fix);
-      }
-      roe_fr<NS>(A.T, pl, pr, ml, mr, B.jac[p0], B.jac[p1], delta[0], delta[1], fc);
+      roe_fr<NS>(A.T, pl, pr, ml, mr, B.jac[p0], B.jac[p1], s_delta[sl], s_delta[sl + 1], fc);
     } else if (A.scheme == 3)
       ausm_plus<NS>(pl, pr, ml, mr, B.jac[p0], B.jac[p1], fc);
     else
@@ -1318,7 +1317,8 @@ size_t inviscid_lds(int scheme) {
   }
   constexpr int TW = Tile<D>::TW, F = Tile<D>::F, TS = F - 1;
   if (scheme == 51 || scheme == 71) return (size_t)2 * (5 + NS) * (TS + 2 * (scheme == 71 ? 4 : 3)) * TW * sizeof(double);
-  return (size_t)(5 + NS) * std::max(TS + 4, F) * TW * sizeof(double);  // primitive reconstruction (3, 4)
+  // primitive reconstruction (2, 3, 4): the fields, then one entropy-fix delta per point
+  return ((size_t)(5 + NS) * std::max(TS + 4, F) * TW + (size_t)(TS + 4) * TW) * sizeof(double);
 }
 
 // The FrBatch of one kernel kind over the nb (<= kFrBatchMax) blocks of h.

@@ -196,7 +196,7 @@ __global__ __launch_bounds__(64) void k_ep_pencil(DevBlock B, int slot) {
 template <int DIR>
 __global__ __launch_bounds__(64) void k_pv_pencil(DevBlock B, Params P, int slot) {
   constexpr int NP = 67;  // points c0-2 .. c0+64
-  __shared__ double sPv[5][NP], sK[3][NP], sJ[NP], sFc[5][64];
+  __shared__ double sPv[5][NP], sK[3][NP], sJ[NP], sFc[5][64], sDelta[NP];
   const int m = DIR == 0 ? B.mx : (DIR == 1 ? B.my : B.mz);
   const int c0 = 63 * blockIdx.x;
   const int tid = threadIdx.x;
@@ -215,6 +215,12 @@ __global__ __launch_bounds__(64) void k_pv_pencil(DevBlock B, Params P, int slot
     for (int l = 0; l < 5; ++l) sPv[l][pt] = bv[p + l * s];
     for (int c = 0; c < 3; ++c) sK[c][pt] = B.met[p + (3 * DIR + c) * s];
     sJ[pt] = B.jac[p];
+    if (P.scheme == 2) {  // the point's entropy-fix delta from its full metric, once for both of its faces
+      double mm[9], qq[5];
+      for (int cc = 0; cc < 9; ++cc) mm[cc] = B.met[p + cc * s];
+      for (int l = 0; l < 5; ++l) qq[l] = bv[p + l * s];
+      sDelta[pt] = entropy_delta_air(qq, mm, P.entropy_fix);
+    }
   }
   __syncthreads();
   const int L = c0 - 1 + tid;  // face L+1/2 between LDS points tid+1 and tid+2
@@ -231,18 +237,7 @@ __global__ __launch_bounds__(64) void k_pv_pencil(DevBlock B, Params P, int slot
     const double mr[3] = {sK[0][tid + 2], sK[1][tid + 2], sK[2][tid + 2]};
     double f[5];
     if (P.scheme == 2) {
-      // entropy-fix deltas of the face's two points from their full metric
-      double dlr[2];
-      for (int q = 0; q < 2; ++q) {
-        int ijk[3] = {base[0], base[1], base[2]};
-        ijk[DIR] = L + q;
-        const long long p = pidx(B, ijk[0], ijk[1], ijk[2]);
-        double mm[9], qq[5];
-        for (int cc = 0; cc < 9; ++cc) mm[cc] = B.met[p + cc * s];
-        for (int l = 0; l < 5; ++l) qq[l] = sPv[l][tid + 1 + q];
-        dlr[q] = entropy_delta_air(qq, mm, P.entropy_fix);
-      }
-      roe_air(pl, pr, ml, mr, sJ[tid + 1], sJ[tid + 2], dlr[0], dlr[1], f);
+      roe_air(pl, pr, ml, mr, sJ[tid + 1], sJ[tid + 2], sDelta[tid + 1], sDelta[tid + 2], f);
     } else {
       riemann_air(P.scheme, pl, pr, ml, mr, sJ[tid + 1], sJ[tid + 2], f);
     }
