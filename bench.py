@@ -164,6 +164,67 @@ def characteristic_leg(steps: int):
             "tile": list(pkg.tile_shapes()[idx]), "fused_ms_per_stage": best}
 
 
+def parallel_leg(reps: int = 10):
+    """Pack and unpack of the faces one GPU shares with its neighbours when the
+    1024x512x256 reacting case is cut into 8 x-slabs (one 128x512x256 block
+    per GPU, faces to the ranks on both sides): 6 primitives + 9 mass
+    fractions x (ngg + 1) layers per face node, the reference's message
+    (parallel_communication).  Algorithmic HBM bytes per face: pack reads and
+    writes the message once; unpack reads the message and the face nodes'
+    own 15 values, writes 15 primitives and 14 conservative variables per
+    node.  The transfer between GPUs is not measured."""
+    MX = pkg.mixture
+    tables = MX.load_mechanism(*MX.H2_AIR_FILES)
+    nx, ny, nz = 128, 512, 256
+    case = pkg.cases.reacting_case([nx - 1], [ny - 1], [nz - 1], tables)
+    case["boundaries"] = [b for b in case["boundaries"] if b.get("axis") != 0]
+    case["parallel_faces"] = [dict(block=0, range=[1, 1, 1, ny, -1, -nz], rank=0),
+                              dict(block=0, range=[nx, nx, 1, ny, -1, -nz], rank=2)]
+    s = MX.FrSolver(case, tables)
+    s.setup_mixing_layer()
+    s.initialize()
+    info = s.parallel_faces()
+    nv, layers = 6 + s.ns, s.ngg + 1
+    L, ctx = s.L, s.ctx
+
+    def pack():
+        if L.ostc_fr_pack_parallel(ctx):
+            raise RuntimeError(L.ostc_fr_last_error(ctx))
+
+    def pack_unpack():
+        pack()
+        if L.ostc_fr_unpack_parallel(ctx):
+            raise RuntimeError(L.ostc_fr_last_error(ctx))
+
+    out = {}
+    for name, fn in (("pack", pack), ("pair", pack_unpack)):
+        fn()
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        s.sync()
+        out[name + "_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+    out["unpack_ms"] = out["pair_ms"] - out["pack_ms"]  # an unpack needs the pack before it
+    total = sum(f["count"] for f in info) * 8
+    nodes = sum(f["points"] for f in info) * layers
+    pack_bytes = 2 * total
+    unpack_bytes = total + nv * 8 * (nodes // layers) + total + (nv - 1) * 8 * nodes
+    s.close()
+    res = {"block": [nx, ny, nz], "faces": len(info), "bytes_per_face": info[0]["count"] * 8,
+           "message_bytes": total, "pack_ms": out["pack_ms"], "unpack_ms": out["unpack_ms"],
+           "pack_hbm_bytes": pack_bytes, "unpack_hbm_bytes": unpack_bytes,
+           "pack_gbs": pack_bytes / out["pack_ms"] / 1e6, "unpack_gbs": unpack_bytes / out["unpack_ms"] / 1e6,
+           "pack_frac_hbm": pack_bytes / out["pack_ms"] / 1e6 / HBM_PEAK_GBS,
+           "unpack_frac_hbm": unpack_bytes / out["unpack_ms"] / 1e6 / HBM_PEAK_GBS,
+           "note": "one GPU's share of the 8-way x-slab cut of the 1024x512x256 reacting case; per call both faces "
+                   "(one launch each); algorithmic bytes; the transfer between GPUs (xGMI) is not measured"}
+    print(f"[bench] parallel faces of a 128x512x256 reacting slab: {len(info)} faces x "
+          f"{info[0]['count'] * 8 / 1e6:.1f} MB, pack {out['pack_ms']:.3f} ms ({res['pack_gbs']:.0f} GB/s), "
+          f"unpack {out['unpack_ms']:.3f} ms ({res['unpack_gbs']:.0f} GB/s)", file=sys.stderr, flush=True)
+    return res
+
+
 def reacting_leg(x_cells, y_cells, z_cells, steps: int, warmup: int, label: str, cpu_sample=None, spatial=False):
     """Reacting H2/air mixing layer (9 species, 19 reactions, characteristic
     WENO-Z5 + LF-type splitting, mixture-averaged transport, finite-rate
@@ -348,6 +409,8 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 134M-cell reacting case")
     ap.add_argument("--reacting-only", action="store_true",
                     help="run only the 512x256x128 reacting case and print its result (profiling)")
+    ap.add_argument("--parallel-only", action="store_true",
+                    help="only the pack/unpack leg of the parallel faces (one JSON line)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain FP64 copy of MIB MiB five times (calibrates HBM byte counters)")
     a = ap.parse_args()
@@ -356,6 +419,9 @@ def main():
               "single GPU and does not shard it.", file=sys.stderr)
         sys.exit(2)
 
+    if a.parallel_only:
+        print(json.dumps(parallel_leg(reps=20)), flush=True)
+        return
     if a.copy_probe:
         copy_probe(a.copy_probe)
         return
@@ -420,6 +486,7 @@ def main():
     s.close()
 
     characteristic = None if a.no_characteristic else characteristic_leg(5)
+    parallel = None if (a.no_reacting or a.no_large) else parallel_leg()
 
     adjoint = None
     if not a.no_adjoint:
@@ -457,7 +524,7 @@ def main():
                    "split_stage": bool(tile_split)},
         "value_graph": value_graph, "plain_value": plain_value, "speedup_vs_plain": value / plain_value,
         "roofline": roofline, "cpu_baseline": cpu, "characteristic": characteristic, "adjoint": adjoint,
-        "reacting": reacting,
+        "reacting": reacting, "parallel_faces": parallel,
     }
     print(json.dumps(out))
 

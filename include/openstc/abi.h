@@ -240,6 +240,48 @@ int ostc_initialize(ostc_ctx ctx);
 /* n_steps SSP-RK3 steps (src/RK.cuh:85-226).  last_dt may be NULL. */
 int ostc_step(ostc_ctx ctx, int n_steps, int mode, double* last_dt);
 
+/* Faces shared with a block of another rank (one GPU per rank), for a
+ * driver that runs the stage operators (ostc_stage_rhs ... ostc_exchange,
+ * then the parallel exchange, then ostc_update_properties) and moves the
+ * bytes itself (RCCL send/recv, a peer copy, MPI).  parallel_communication,
+ * src/DataCommunication.cuh:108-199, with the record format of
+ * Mesh::read_parallel_interface / ParallelFace::register_boundary
+ * (src/Mesh.cpp:846-876, :177-231).  ostc_step refuses a context with
+ * parallel faces. */
+typedef struct {
+  int block, rank, flag_send, flag_recv;
+  long long offset;  /* first double of the face in the send and receive buffers */
+  long long count;   /* doubles: 6 primitives x (ngg + 1) layers x points */
+  long long points;  /* face nodes */
+} ostc_parallel_info;
+/* One parallel N.txt record, before ostc_finalize: range = iMin iMax jMin
+ * jMax kMin kMax as in the file (1-based; the normal axis has equal ends; the
+ * tangential axis listed second in the buffer is given negated). */
+int ostc_add_parallel_face(ostc_ctx ctx, int block, const int range[6], int rank, int flag_send, int flag_recv);
+int ostc_parallel_face_count(ostc_ctx ctx, int* n);
+int ostc_parallel_face_info(ostc_ctx ctx, int face, ostc_parallel_info* out);
+/* The device send and receive buffers (total doubles each), owned by the context. */
+int ostc_parallel_buffers(ostc_ctx ctx, double** send, double** recv, long long* total);
+/* setup_data_to_be_sent (src/DataCommunication.cu:3-40) of every parallel face
+ * into the send buffer: per face point the face node, then the ngg nodes
+ * inside it, 6 primitives each. */
+int ostc_pack_parallel(ostc_ctx ctx);
+/* assign_data_received (src/DataCommunication.cuh:201-240) of every parallel
+ * face, in face order, from the receive buffer (the partner's send buffer
+ * slice of that face): the face node takes the mean of its own and the
+ * partner's primitives, the ngg ghost nodes beyond it the partner's interior
+ * ones; cv from them.  As in the reference nothing may change the fields
+ * between ostc_pack_parallel and this call: the own values of a face come from
+ * the send buffer, except for a face that shares nodes with an earlier face
+ * of its block, which reads the field (and so sees the earlier face's
+ * means).  Returns OSTC_ERR_STATE unless ostc_pack_parallel ran since the last
+ * unpack or ostc_set_field. */
+int ostc_unpack_parallel(ostc_ctx ctx);
+/* Host copies for drivers without device transport: which = 0 copies the send
+ * buffer to host, which = 1 the host array into the receive buffer (total
+ * doubles; synchronous). */
+int ostc_parallel_copy(ostc_ctx ctx, int which, double* host);
+
 /* Where a batched stage recomputes the viscosity and Mach number of the
  * ghost layers (update_physical_properties over the padded domain,
  * src/RK.cuh:216-223): OSTC_GHOST_PROPS_IN_OPS or OSTC_GHOST_PROPS_PASS.
@@ -558,6 +600,24 @@ int ostc_fr_stage_rhs(ostc_fr_ctx ctx, int block, double dt);
 int ostc_fr_compute_dt(ostc_fr_ctx ctx, double* dt_out);   /* local_time_step + global MIN */
 int ostc_fr_stage_update(ostc_fr_ctx ctx, int block, int rk, double dt);
 int ostc_fr_update_properties(ostc_fr_ctx ctx, int block);
+/* qn = cv (src/RK.cuh:99-102), apply_boundary_conditions of one block
+ * (src/BoundCond.cuh:1488-1678) and data_communication over the inner faces
+ * (src/DataCommunication.cuh:29-58): the rest of a stage for a driver that
+ * runs the pieces itself, e.g. around the parallel exchange below. */
+int ostc_fr_copy_qn(ostc_fr_ctx ctx, int block);
+int ostc_fr_apply_bc(ostc_fr_ctx ctx, int block);
+int ostc_fr_exchange(ostc_fr_ctx ctx);
+/* Faces shared with another rank, as ostc_add_parallel_face ... ostc_parallel_copy
+ * for the air context; each face point carries 6 primitives and ns mass
+ * fractions per layer.  ostc_fr_step refuses a context with parallel faces. */
+int ostc_fr_add_parallel_face(ostc_fr_ctx ctx, int block, const int range[6], int rank, int flag_send,
+                              int flag_recv);
+int ostc_fr_parallel_face_count(ostc_fr_ctx ctx, int* n);
+int ostc_fr_parallel_face_info(ostc_fr_ctx ctx, int face, ostc_parallel_info* out);
+int ostc_fr_parallel_buffers(ostc_fr_ctx ctx, double** send, double** recv, long long* total);
+int ostc_fr_pack_parallel(ostc_fr_ctx ctx);
+int ostc_fr_unpack_parallel(ostc_fr_ctx ctx);
+int ostc_fr_parallel_copy(ostc_fr_ctx ctx, int which, double* host);
 /* timing of the last ostc_fr_step: device ms and launches; per kernel kind
  * (OSTC_FR_K_*) the summed ms and launch counts of the last TIMED call */
 int ostc_fr_last_step_timing(ostc_fr_ctx ctx, double* ms, long long* launches);

@@ -130,6 +130,12 @@ class Zone(C.Structure):
                 ("jac", C.c_void_p), ("metric", C.c_void_p), ("dq", C.c_void_p), ("dt_local", C.c_void_p)]
 
 
+class ParallelInfo(C.Structure):
+    """ostc_parallel_info: one face shared with another rank."""
+    _fields_ = [("block", C.c_int), ("rank", C.c_int), ("flag_send", C.c_int), ("flag_recv", C.c_int),
+                ("offset", C.c_longlong), ("count", C.c_longlong), ("points", C.c_longlong)]
+
+
 class Fourier(C.Structure):
     """ostc_fourier: Fourier-mode velocity perturbation (differentiation workload)."""
     _fields_ = [("n_modes", C.c_int), ("comp", C.c_int * MAX_MODES), ("kx", C.c_int * MAX_MODES),
@@ -193,6 +199,13 @@ SIGNATURES = {
     "ostc_add_inflow": (_I, [_P, _I, C.POINTER(_I), C.POINTER(Inflow), _DP]),
     "ostc_set_sponge": (_I, [_P, _I, _P, _DP, _DP, _DP]),
     "ostc_finalize": (_I, [_P]),
+    "ostc_add_parallel_face": (_I, [_P, _I, C.POINTER(_I), _I, _I, _I]),
+    "ostc_parallel_face_count": (_I, [_P, C.POINTER(_I)]),
+    "ostc_parallel_face_info": (_I, [_P, _I, C.POINTER(ParallelInfo)]),
+    "ostc_parallel_buffers": (_I, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_longlong)]),
+    "ostc_pack_parallel": (_I, [_P]),
+    "ostc_unpack_parallel": (_I, [_P]),
+    "ostc_parallel_copy": (_I, [_P, _I, _DP]),
     "ostc_ngg": (_I, [_P, C.POINTER(_I)]),
     "ostc_zone_view": (_I, [_P, _I, C.POINTER(Zone)]),
     "ostc_set_geometry": (_I, [_P, _I, _DP, _DP, _LL, _LL, _LL]),
@@ -337,6 +350,9 @@ class Solver:
             for b in range(len(case["blocks"])):
                 xyz = [np.ascontiguousarray(a) for a in cases.node_coordinates(case, b, self.ngg)]
                 self._chk(L.ostc_set_sponge(self.ctx, b, C.cast(C.byref(sp), C.c_void_p), *(_dp(a) for a in xyz)))
+        for f in case.get("parallel_faces", []):  # parallel N.txt records
+            self._chk(L.ostc_add_parallel_face(self.ctx, f["block"], ints(f["range"]), f["rank"],
+                                               f.get("flag_send", 0), f.get("flag_recv", 0)))
         self._chk(L.ostc_finalize(self.ctx))
 
     def _chk(self, r):
@@ -460,6 +476,40 @@ class Solver:
         r = np.zeros(4)
         self._chk(lib().ostc_residual(self.ctx, _dp(r)))
         return r
+
+    # -- faces shared with other ranks ------------------------------------
+    def parallel_faces(self):
+        """[{block, rank, flag_send, flag_recv, offset, count, points}] of the parallel faces."""
+        L = lib()
+        n = C.c_int()
+        self._chk(L.ostc_parallel_face_count(self.ctx, C.byref(n)))
+        out = []
+        for q in range(n.value):
+            info = ParallelInfo()
+            self._chk(L.ostc_parallel_face_info(self.ctx, q, C.byref(info)))
+            out.append({k: getattr(info, k) for k, _ in ParallelInfo._fields_})
+        return out
+
+    def _parallel_total(self):
+        t = C.c_longlong()
+        self._chk(lib().ostc_parallel_buffers(self.ctx, None, None, C.byref(t)))
+        return t.value
+
+    def pack_parallel(self) -> np.ndarray:
+        """Packs every parallel face and returns the send buffer (host copy)."""
+        self._chk(lib().ostc_pack_parallel(self.ctx))
+        out = np.zeros(self._parallel_total())
+        self._chk(lib().ostc_parallel_copy(self.ctx, 0, _dp(out)))
+        return out
+
+    def unpack_parallel(self, recv: np.ndarray):
+        """Copies `recv` (the partners' slices, face by face) into the receive
+        buffer and unpacks every parallel face."""
+        recv = np.ascontiguousarray(recv, dtype=np.float64)
+        if recv.size != self._parallel_total():
+            raise ValueError(f"receive buffer of {recv.size} doubles, expected {self._parallel_total()}")
+        self._chk(lib().ostc_parallel_copy(self.ctx, 1, _dp(recv)))
+        self._chk(lib().ostc_unpack_parallel(self.ctx))
 
     # -- tuning and timing -------------------------------------------------
     def set_tile(self, index: int, waves_per_cu: int = 0, split: int = 0):

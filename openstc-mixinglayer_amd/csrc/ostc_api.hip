@@ -113,6 +113,11 @@ struct ostc_ctx_s {
   // properties of the ghost layers after a batched stage: computed by the
   // ghost ops themselves (OSTC_GHOST_PROPS_IN_OPS) or by one more pass
   int ghost_props = OSTC_GHOST_PROPS_IN_OPS;
+  // faces shared with other ranks (parallel N.txt) and the send / receive buffers
+  std::vector<ParFace> par_faces;
+  double *d_par_send = nullptr, *d_par_recv = nullptr;
+  long long par_total = 0;
+  bool par_packed = false;  // a pack has run since the last unpack or host write of a field
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool graph_valid = false;
@@ -1009,6 +1014,22 @@ int ostc_add_inner_face_range(ostc_ctx c, int block, const int range[6], const i
   return OSTC_OK;
 }
 
+int ostc_add_parallel_face(ostc_ctx c, int block, const int range[6], int rank, int flag_send, int flag_recv) {
+  if (!c) return OSTC_ERR_ARG;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_finalize");
+  if (block < 0 || block >= (int)c->blocks.size() || !range) return fail(c, OSTC_ERR_ARG, "bad parallel face");
+  if (c->ngg + 1 > ostc::kParMaxLayers) return fail(c, OSTC_ERR_UNSUPPORTED, "parallel faces: ngg above 4");
+  ParFace f{};
+  std::string err;
+  if (!parallel_face_decode(range, c->blocks[block].m, &f, &err)) return fail(c, OSTC_ERR_ARG, err);
+  f.block = block;
+  f.rank = rank;
+  f.flag_send = flag_send;
+  f.flag_recv = flag_recv;
+  c->par_faces.push_back(f);
+  return OSTC_OK;
+}
+
 int ostc_finalize(ostc_ctx c) {
   if (!c) return OSTC_ERR_ARG;
   if (c->finalized) return fail(c, OSTC_ERR_STATE, "already finalized");
@@ -1096,6 +1117,16 @@ int ostc_finalize(ostc_ctx c) {
   HIPCHK(c, hipGetLastError());  // the metric copies of caller-owned geometry
   int r = dalloc(c, &c->d_scratch, 1 + 4 * (size_t)c->n_first);
   if (!r) r = dalloc(c, &c->d_dt, 1);
+  if (!r && !c->par_faces.empty()) {  // n_trans (6) x (ngg + 1) layers per face point
+    c->par_total = 0;
+    for (ParFace& f : c->par_faces) {
+      f.offset = c->par_total;
+      c->par_total += 6ll * (G + 1) * f.points;
+    }
+    par_assign_own_source(c->par_faces.data(), (int)c->par_faces.size());
+    r = dalloc(c, &c->d_par_send, (size_t)c->par_total);
+    if (!r) r = dalloc(c, &c->d_par_recv, (size_t)c->par_total);
+  }
   if (!r) r = build_ops(c);
   if (r) return r;
   HIPCHK(c, hipEventCreate(&c->ev_a));
@@ -1179,6 +1210,7 @@ int ostc_set_field(ostc_ctx c, int b, int f, const double* host) {
   int r = check_block(c, b);
   if (r) return r;
   if (!host) return OSTC_ERR_ARG;
+  c->par_packed = false;
   const bool ring = f == OSTC_FIELD_CV || f == OSTC_FIELD_BV || f == OSTC_FIELD_MUL;
   for (int slot = 0; slot < (ring ? kRing : 1); ++slot) {
     double* p;
@@ -1228,6 +1260,10 @@ int ostc_initialize(ostc_ctx c) {
 int ostc_step(ostc_ctx c, int n_steps, int mode, double* last_dt) {
   int r = check_ready(c);
   if (r) return r;
+  if (!c->par_faces.empty())
+    return fail(c, OSTC_ERR_STATE,
+                "parallel faces: run the stage operators, with ostc_pack_parallel / the transfer / "
+                "ostc_unpack_parallel after ostc_exchange");
   if (n_steps < 0) return fail(c, OSTC_ERR_ARG, "negative step count");
   long long n = 0;
   HIPCHK(c, hipEventRecord(c->ev_a, c->st));
@@ -1271,6 +1307,68 @@ int ostc_step(ostc_ctx c, int n_steps, int mode, double* last_dt) {
     HIPCHK(c, hipStreamSynchronize(c->st));
     *last_dt = dt;
   }
+  return OSTC_OK;
+}
+
+int ostc_parallel_face_count(ostc_ctx c, int* n) {
+  if (!c || !n) return OSTC_ERR_ARG;
+  *n = (int)c->par_faces.size();
+  return OSTC_OK;
+}
+
+int ostc_parallel_face_info(ostc_ctx c, int face, ostc_parallel_info* out) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (!out || face < 0 || face >= (int)c->par_faces.size()) return fail(c, OSTC_ERR_ARG, "bad parallel face index");
+  const ParFace& f = c->par_faces[face];
+  out->block = f.block;
+  out->rank = f.rank;
+  out->flag_send = f.flag_send;
+  out->flag_recv = f.flag_recv;
+  out->offset = f.offset;
+  out->count = 6ll * (c->ngg + 1) * f.points;
+  out->points = f.points;
+  return OSTC_OK;
+}
+
+int ostc_parallel_buffers(ostc_ctx c, double** send, double** recv, long long* total) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (send) *send = c->d_par_send;
+  if (recv) *recv = c->d_par_recv;
+  if (total) *total = c->par_total;
+  return OSTC_OK;
+}
+
+int ostc_pack_parallel(ostc_ctx c) {
+  int r = check_ready(c);
+  if (r) return r;
+  for (const ParFace& f : c->par_faces) par_pack(c->blocks[f.block].d, f, c->d_par_send, c->st);
+  HIPCHK(c, hipGetLastError());
+  c->par_packed = true;
+  return OSTC_OK;
+}
+
+int ostc_unpack_parallel(ostc_ctx c) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (!c->par_packed && !c->par_faces.empty())
+    return fail(c, OSTC_ERR_STATE, "ostc_unpack_parallel: call ostc_pack_parallel first (the face nodes take the "
+                                   "mean of what was sent and what was received)");
+  c->par_packed = false;
+  for (const ParFace& f : c->par_faces) par_unpack(c->blocks[f.block].d, f, c->d_par_recv, c->d_par_send, c->st);
+  HIPCHK(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_parallel_copy(ostc_ctx c, int which, double* host) {
+  int r = check_ready(c);
+  if (r) return r;
+  if (!host || (which != 0 && which != 1)) return fail(c, OSTC_ERR_ARG, "bad parallel buffer copy");
+  const size_t bytes = (size_t)c->par_total * sizeof(double);
+  if (which == 0) HIPCHK(c, hipMemcpyAsync(host, c->d_par_send, bytes, hipMemcpyDeviceToHost, c->st));
+  else HIPCHK(c, hipMemcpyAsync(c->d_par_recv, host, bytes, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
   return OSTC_OK;
 }
 

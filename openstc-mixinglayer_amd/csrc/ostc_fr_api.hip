@@ -22,6 +22,8 @@
 using namespace ostc::fr;
 using ostc::FaceMap;
 using ostc::FaceRange;
+using ostc::ParFace;
+using ostc::parallel_face_decode;
 
 namespace {
 
@@ -88,6 +90,11 @@ struct ostc_fr_ctx_s {
   KernelTimer* timer = nullptr;
   double kms[KT_COUNT] = {};
   long long kcount[KT_COUNT] = {};
+  // faces shared with other ranks (parallel N.txt) and the send / receive buffers
+  std::vector<ParFace> par_faces;
+  double *d_par_send = nullptr, *d_par_recv = nullptr;
+  long long par_total = 0;
+  bool par_packed = false;  // a pack has run since the last unpack or host write of a field
 };
 
 namespace {
@@ -616,6 +623,22 @@ int ostc_fr_add_inner_face_range(ostc_fr_ctx c, int b, const int range[6], const
   return OSTC_OK;
 }
 
+int ostc_fr_add_parallel_face(ostc_fr_ctx c, int b, const int range[6], int rank, int flag_send, int flag_recv) {
+  if (!c) return OSTC_ERR_ARG;
+  if (c->finalized) return fail(c, OSTC_ERR_STATE, "faces are fixed after ostc_fr_finalize");
+  if (b < 0 || b >= (int)c->blocks.size() || !range) return fail(c, OSTC_ERR_ARG, "bad parallel face");
+  if (c->ngg + 1 > ostc::kParMaxLayers) return fail(c, OSTC_ERR_UNSUPPORTED, "parallel faces: ngg above 4");
+  ParFace f{};
+  std::string err;
+  if (!parallel_face_decode(range, c->blocks[b].ext(), &f, &err)) return fail(c, OSTC_ERR_ARG, err);
+  f.block = b;
+  f.rank = rank;
+  f.flag_send = flag_send;
+  f.flag_recv = flag_recv;
+  c->par_faces.push_back(f);
+  return OSTC_OK;
+}
+
 int ostc_fr_finalize(ostc_fr_ctx c) {
   if (!c) return OSTC_ERR_ARG;
   if (c->finalized) return OSTC_OK;
@@ -697,6 +720,16 @@ int ostc_fr_finalize(ostc_fr_ctx c) {
                              hipMemcpyHostToDevice, c->st));
     FR_HIP(c, hipStreamSynchronize(c->st));
   }
+  if (!c->par_faces.empty()) {  // (6 + ns) transferred values x (ngg + 1) layers per face point
+    c->par_total = 0;
+    for (ParFace& f : c->par_faces) {
+      f.offset = c->par_total;
+      c->par_total += (long long)(6 + ns) * (g + 1) * f.points;
+    }
+    ostc::par_assign_own_source(c->par_faces.data(), (int)c->par_faces.size());
+    if (int r = dalloc(c, &c->d_par_send, (size_t)c->par_total)) return r;
+    if (int r = dalloc(c, &c->d_par_recv, (size_t)c->par_total)) return r;
+  }
   c->h_args.clear();
   for (int b = 0; b < (int)c->blocks.size(); ++b) c->h_args.push_back(args(c, b));
   long long n = 0;
@@ -728,6 +761,7 @@ int ostc_fr_set_field(ostc_fr_ctx c, int b, int f, const double* host) {
   if (int r = check_ready(c)) return r;
   if (int r = check_block(c, b)) return r;
   if (!host) return fail(c, OSTC_ERR_ARG, "null host pointer");
+  c->par_packed = false;
   double* p = nullptr;
   size_t n = 0;
   if (int r = field_ptr(c, b, f, &p, &n)) return r;
@@ -766,6 +800,10 @@ int ostc_fr_initialize(ostc_fr_ctx c) {
 
 int ostc_fr_step(ostc_fr_ctx c, int n_steps, int mode, double* last_dt) {
   if (int r = check_ready(c)) return r;
+  if (!c->par_faces.empty())
+    return fail(c, OSTC_ERR_STATE,
+                "parallel faces: run the stage operators, with ostc_fr_pack_parallel / the transfer / "
+                "ostc_fr_unpack_parallel after ostc_fr_exchange");
   if (n_steps < 0) return fail(c, OSTC_ERR_ARG, "negative step count");
   if (mode != OSTC_FR_MODE_DIRECT && mode != OSTC_FR_MODE_GRAPH && mode != OSTC_FR_MODE_TIMED &&
       mode != OSTC_FR_MODE_BATCHED && mode != OSTC_FR_MODE_BATCHED_TIMED)
@@ -856,6 +894,34 @@ int ostc_fr_stage_update(ostc_fr_ctx c, int b, int rk, double dt) {
   return OSTC_OK;
 }
 
+int ostc_fr_copy_qn(ostc_fr_ctx c, int b) {
+  if (int r = check_ready(c)) return r;
+  if (int r = check_block(c, b)) return r;
+  const FrBlockDev& d = c->blocks[b].d;
+  FR_HIP(c, hipMemcpyAsync(d.qn, d.cv, (size_t)d.sz * c->nv * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_fr_apply_bc(ostc_fr_ctx c, int b) {
+  if (int r = check_ready(c)) return r;
+  if (int r = check_block(c, b)) return r;
+  long long n = 0;
+  apply_bc(c, b, &n);
+  FR_HIP(c, hipGetLastError());
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
+int ostc_fr_exchange(ostc_fr_ctx c) {
+  if (int r = check_ready(c)) return r;
+  long long n = 0;
+  exchange(c, &n);
+  FR_HIP(c, hipGetLastError());
+  FR_HIP(c, hipStreamSynchronize(c->st));
+  return OSTC_OK;
+}
+
 int ostc_fr_update_properties(ostc_fr_ctx c, int b) {
   if (int r = check_ready(c)) return r;
   if (int r = check_block(c, b)) return r;
@@ -916,6 +982,64 @@ int ostc_fr_kernel_ms(ostc_fr_ctx c, double ms[OSTC_FR_K_COUNT], long long count
     if (ms) ms[k] = c->kms[k];
     if (counts) counts[k] = c->kcount[k];
   }
+  return OSTC_OK;
+}
+
+int ostc_fr_parallel_face_count(ostc_fr_ctx c, int* n) {
+  if (!c || !n) return OSTC_ERR_ARG;
+  *n = (int)c->par_faces.size();
+  return OSTC_OK;
+}
+
+int ostc_fr_parallel_face_info(ostc_fr_ctx c, int face, ostc_parallel_info* out) {
+  if (int r = check_ready(c)) return r;
+  if (!out || face < 0 || face >= (int)c->par_faces.size()) return fail(c, OSTC_ERR_ARG, "bad parallel face index");
+  const ParFace& f = c->par_faces[face];
+  out->block = f.block;
+  out->rank = f.rank;
+  out->flag_send = f.flag_send;
+  out->flag_recv = f.flag_recv;
+  out->offset = f.offset;
+  out->count = (long long)(6 + c->ns) * (c->ngg + 1) * f.points;
+  out->points = f.points;
+  return OSTC_OK;
+}
+
+int ostc_fr_parallel_buffers(ostc_fr_ctx c, double** send, double** recv, long long* total) {
+  if (int r = check_ready(c)) return r;
+  if (send) *send = c->d_par_send;
+  if (recv) *recv = c->d_par_recv;
+  if (total) *total = c->par_total;
+  return OSTC_OK;
+}
+
+int ostc_fr_pack_parallel(ostc_fr_ctx c) {
+  if (int r = check_ready(c)) return r;
+  for (const ParFace& f : c->par_faces) launch_par_pack(c->h_args[f.block], c->ns, f, c->d_par_send, c->st);
+  FR_HIP(c, hipGetLastError());
+  c->par_packed = true;
+  return OSTC_OK;
+}
+
+int ostc_fr_unpack_parallel(ostc_fr_ctx c) {
+  if (int r = check_ready(c)) return r;
+  if (!c->par_packed && !c->par_faces.empty())
+    return fail(c, OSTC_ERR_STATE, "ostc_fr_unpack_parallel: call ostc_fr_pack_parallel first (the face nodes take "
+                                   "the mean of what was sent and what was received)");
+  c->par_packed = false;
+  for (const ParFace& f : c->par_faces)
+    launch_par_unpack(c->h_args[f.block], c->ns, f, c->d_par_recv, c->d_par_send, c->st);
+  FR_HIP(c, hipGetLastError());
+  return OSTC_OK;
+}
+
+int ostc_fr_parallel_copy(ostc_fr_ctx c, int which, double* host) {
+  if (int r = check_ready(c)) return r;
+  if (!host || (which != 0 && which != 1)) return fail(c, OSTC_ERR_ARG, "bad parallel buffer copy");
+  const size_t bytes = (size_t)c->par_total * sizeof(double);
+  if (which == 0) FR_HIP(c, hipMemcpyAsync(host, c->d_par_send, bytes, hipMemcpyDeviceToHost, c->st));
+  else FR_HIP(c, hipMemcpyAsync(c->d_par_recv, host, bytes, hipMemcpyHostToDevice, c->st));
+  FR_HIP(c, hipStreamSynchronize(c->st));
   return OSTC_OK;
 }
 

@@ -164,6 +164,51 @@ __global__ __launch_bounds__(kT) void k_cv_from_bv(FrArgs A) {
   if (p < A.B.sz) cv_point<NS>(A, p);
 }
 
+// Parallel faces (faces shared with another rank), mixture: per face point
+// the 6 primitives and ns mass fractions of the face node and of the ngg
+// nodes inside it (setup_data_to_be_sent, src/DataCommunication.cu:3-40) ...
+template <int NS>
+__global__ __launch_bounds__(kParThreads) void k_par_pack(FrArgs A, ParFace f, double* buf) {
+  const FrBlockDev& B = A.B;
+  const long long s = B.sz;
+  par_pack_chunk<6 + NS>(f, B.sx, B.sy, B.g, buf,
+                         [&](int v, long long p) { return v < 6 ? B.bv[p + v * s] : B.sv[p + (v - 6) * s]; });
+}
+
+// ... and from the partner's buffer: the face node takes the mean of its own
+// and the received values, the ngg ghost nodes beyond it the received inner
+// ones, each with cv from its primitives (assign_data_received,
+// src/DataCommunication.cuh:201-240; cv as compute_cv_from_bv_1_point).
+template <int NS>
+__global__ __launch_bounds__(kParThreads) void k_par_unpack(FrArgs A, ParFace f, const double* recv,
+                                                            const double* sent) {
+  const FrBlockDev& B = A.B;
+  const long long s = B.sz;
+  par_unpack_chunk<6 + NS>(
+      f, B.sx, B.sy, B.g, recv, sent,
+      [&](int v, long long p) { return v < 6 ? B.bv[p + v * s] : B.sv[p + (v - 6) * s]; },
+      [&](int v, long long p, double x) {
+        if (v < 6)
+          B.bv[p + v * s] = x;
+        else
+          B.sv[p + (v - 6) * s] = x;
+      },
+      [&](long long p, const double* q) {
+        const double rho = q[0];
+        B.cv[p] = rho;
+        B.cv[p + s] = rho * q[1];
+        B.cv[p + 2 * s] = rho * q[2];
+        B.cv[p + 3 * s] = rho * q[3];
+        double Y[NS];
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          Y[l] = q[6 + l];
+          B.cv[p + (5 + l) * s] = rho * Y[l];
+        }
+        B.cv[p + 4 * s] = total_energy<NS>(A.T, rho, q[1], q[2], q[3], q[4], q[5], Y);
+      });
+}
+
 // update_physical_properties<FR>, src/FieldOperation.cuh:107-140, over the
 // padded domain
 template <int NS>
@@ -1492,6 +1537,18 @@ void launch_props(const FrArgs& A, int ns, hipStream_t st, long long* n, KernelT
   OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_props<NS>, dim3(nblk(A.B.sz)), dim3(kT), 0, st, A))
   if (tm) tm->mark(KT_PROPS, true, st);
   ++*n;
+}
+
+void launch_par_pack(const FrArgs& A, int ns, const ParFace& f, double* buf, hipStream_t st) {
+  if (f.points <= 0) return;
+  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_par_pack<NS>, dim3(par_chunks(f)), dim3(kParThreads), 0, st, A, f, buf))
+}
+
+void launch_par_unpack(const FrArgs& A, int ns, const ParFace& f, const double* recv, const double* sent,
+                       hipStream_t st) {
+  if (f.points <= 0) return;
+  OSTC_FR_DISPATCH(ns, hipLaunchKernelGGL(k_par_unpack<NS>, dim3(par_chunks(f)), dim3(kParThreads), 0, st, A, f, recv,
+                                          sent))
 }
 
 void launch_cv_from_bv(const FrArgs& A, int ns, hipStream_t st, long long* n) {

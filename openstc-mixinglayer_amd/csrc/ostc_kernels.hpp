@@ -2,6 +2,7 @@
 #pragma once
 #include <algorithm>
 #include "ostc_ops.hpp"
+#include "ostc_parallel.hpp"
 
 namespace ostc {
 
@@ -15,6 +16,9 @@ void count_pass(unsigned long long* pass, int lo, int hi, hipStream_t st, long l
 void sponge_mean(const DevBlock& B, const unsigned long long* iter, hipStream_t st, long long* n);
 void ops_launch(const DevBlock* d_blocks, int n_blocks, const DevOp* d_ops, int op_lo, int op_hi, long long total, int slot,
                 int src, hipStream_t st, long long* n, bool props = false);
+// parallel faces (ring slot 0): pack into / unpack from buf + f.offset
+void par_pack(const DevBlock& B, const ParFace& f, double* buf, hipStream_t st);
+void par_unpack(const DevBlock& B, const ParFace& f, const double* recv, const double* sent, hipStream_t st);
 void store_last(const DevBlock& B, hipStream_t st);
 void residual_partial(const DevBlock& B, double* out4, int nblk, hipStream_t st);
 

@@ -602,6 +602,47 @@ void ops_launch(const DevBlock* d_blocks, int n_blocks, const DevOp* d_ops, int 
   *n += 1;
 }
 
+// Parallel faces (faces shared with another rank), air: the 6 primitives of
+// the face node and the ngg nodes inside it, per face point, into `buf`
+// (setup_data_to_be_sent, src/DataCommunication.cu:3-40) ...
+__global__ __launch_bounds__(kParThreads) void k_par_pack(DevBlock B, ParFace f, double* buf) {
+  const double* bv = B.bv[0];
+  const long long s = B.sz;
+  par_pack_chunk<6>(f, B.sx, B.sy, B.g, buf, [&](int v, long long p) { return bv[p + v * s]; });
+}
+
+// ... and from the partner's buffer: the face node takes the mean of its own
+// and the received primitives, the ngg ghost nodes beyond it the received
+// interior ones, each with cv from its primitives (assign_data_received,
+// src/DataCommunication.cuh:201-240; cv as compute_cv_from_bv_1_point<Air>).
+__global__ __launch_bounds__(kParThreads) void k_par_unpack(DevBlock B, ParFace f, const double* recv,
+                                                            const double* sent) {
+  double* bv = B.bv[0];
+  double* cv = B.cv[0];
+  const long long s = B.sz;
+  par_unpack_chunk<6>(
+      f, B.sx, B.sy, B.g, recv, sent, [&](int v, long long p) { return bv[p + v * s]; },
+      [&](int v, long long p, double x) { bv[p + v * s] = x; },
+      [&](long long p, const double* q) {
+        const double rho = q[0];
+        cv[p] = rho;
+        cv[p + s] = rho * q[1];
+        cv[p + 2 * s] = rho * q[2];
+        cv[p + 3 * s] = rho * q[3];
+        cv[p + 4 * s] = total_energy(rho, q[1], q[2], q[3], q[4]);
+      });
+}
+
+void par_pack(const DevBlock& B, const ParFace& f, double* buf, hipStream_t st) {
+  if (f.points <= 0) return;
+  hipLaunchKernelGGL(k_par_pack, dim3(par_chunks(f)), dim3(kParThreads), 0, st, B, f, buf);
+}
+
+void par_unpack(const DevBlock& B, const ParFace& f, const double* recv, const double* sent, hipStream_t st) {
+  if (f.points <= 0) return;
+  hipLaunchKernelGGL(k_par_unpack, dim3(par_chunks(f)), dim3(kParThreads), 0, st, B, f, recv, sent);
+}
+
 // One more boundary-condition pass for blocks lo..hi-1 (the counter that
 // keys the inflow's white noise).  A kernel rather than a memory operation, so
 // that it records into a captured graph like the ops it follows.

@@ -42,6 +42,19 @@ struct DevOp {
   const unsigned long long* pass;  // OP_BC_INFLOW: boundary-condition passes per block so far
 };
 
+// compute_cv_from_bv_1_point<Air> (src/FieldOperation.cuh:40-76) in ring slot `slot`.
+__device__ __forceinline__ void air_cv_point(const DevBlock& Z, long long p, int slot) {
+  const double* zbv = Z.bv[slot];
+  double* zcv = Z.cv[slot];
+  const long long zs = Z.sz;
+  const double rho = zbv[p];
+  zcv[p] = rho;
+  zcv[p + zs] = rho * zbv[p + zs];
+  zcv[p + 2 * zs] = rho * zbv[p + 2 * zs];
+  zcv[p + 3 * zs] = rho * zbv[p + 3 * zs];
+  zcv[p + 4 * zs] = total_energy(rho, zbv[p + zs], zbv[p + 2 * zs], zbv[p + 3 * zs], zbv[p + 4 * zs]);
+}
+
 // update_physical_properties<Air> at one point (src/FieldOperation.cuh:107-140):
 // Sutherland viscosity of ring slot `slot` and the Mach number.
 __device__ __forceinline__ void props_point(const DevBlock& Z, long long p, int slot) {
@@ -74,17 +87,7 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
   double* bv = B.bv[slot];
   double* cv = B.cv[slot];
   const long long s = B.sz;
-  auto cv_point = [&](const DevBlock& Z, long long p) {
-    double* zbv = Z.bv[slot];
-    double* zcv = Z.cv[slot];
-    const long long zs = Z.sz;
-    const double rho = zbv[p];
-    zcv[p] = rho;
-    zcv[p + zs] = rho * zbv[p + zs];
-    zcv[p + 2 * zs] = rho * zbv[p + 2 * zs];
-    zcv[p + 3 * zs] = rho * zbv[p + 3 * zs];
-    zcv[p + 4 * zs] = total_energy(rho, zbv[p + zs], zbv[p + 2 * zs], zbv[p + 3 * zs], zbv[p + 4 * zs]);
-  };
+  auto cv_point = [&](const DevBlock& Z, long long p) { air_cv_point(Z, p, slot); };
   switch (op.type) {
     case OP_BC_OUTFLOW:
     case OP_BC_PERIODIC: {

@@ -132,15 +132,31 @@ def write_inner(path: str, n_block: int, records):
                 f.write("".join(f"{v:7d}" for v in (*tgt, target + 1)) + "\n")
 
 
-def write_parallel(path: str, n_block: int):
+def write_parallel(path: str, n_block: int, records=()):
+    """records: (block, range[6], rank, send_flag, recv_flag), as read_parallel returns them."""
+    records = list(records)
     with open(path, "w") as f:
-        f.write(f"{n_block:9d}{0:9d}\n")
-        f.write("".join(f"{0:9d}" for _ in range(n_block)) + "\n")
+        f.write(f"{n_block:9d}{len(records):9d}\n")
+        f.write("".join(f"{sum(1 for r in records if r[0] == b):9d}" for b in range(n_block)) + "\n")
         f.write("iMin  iMax  jMin  jMax  kMin  kMax   s_r_id  send_flag  recv_flag\n")
+        for blk in range(n_block):
+            for _, rng, rank, fs, fr in (r for r in records if r[0] == blk):
+                f.write("".join(f"{v:7d}" for v in (*rng, rank, fs, fr)) + "\n")
+
+
+def parallel_faces(directory: str, n_block: int, rank: int = 0):
+    """The faces shared with other ranks from ``parallel N.txt``, as the
+    ``parallel_faces`` records that ``Solver`` and ``mixture.FrSolver``
+    register (ostc_add_parallel_face): block, range as in the file, rank, tags."""
+    path = os.path.join(directory, file_name("parallel", rank))
+    if not os.path.exists(path):
+        return []
+    return [dict(block=blk, range=list(rng), rank=r, flag_send=fs, flag_recv=fr)
+            for blk, rng, r, fs, fr in read_parallel(path, n_block)]
 
 
 def connectivity(directory: str, n_block: int, rank: int = 0, labels: dict | None = None,
-                 farfields: dict | None = None, inflows: dict | None = None):
+                 farfields: dict | None = None, inflows: dict | None = None, allow_parallel: bool = False):
     """(boundaries, inner_faces) of a case dict from the three files of one
     rank.  ``labels`` maps a boundary label to its type (default: 4 farfield,
     5 inflow, 6 outflow, 8 periodic, the labels of
@@ -153,8 +169,9 @@ def connectivity(directory: str, n_block: int, rank: int = 0, labels: dict | Non
     bnd = read_boundary(os.path.join(directory, file_name("boundary", rank)), n_block)
     inn = read_inner(os.path.join(directory, file_name("inner", rank)), n_block)
     par_path = os.path.join(directory, file_name("parallel", rank))
-    if os.path.exists(par_path) and read_parallel(par_path, n_block):
-        raise ValueError(f"{par_path} lists faces to other ranks: one context holds all blocks on one GPU")
+    if os.path.exists(par_path) and read_parallel(par_path, n_block) and not allow_parallel:
+        raise ValueError(f"{par_path} lists faces to other ranks: read them with parallel_faces() "
+                         "(allow_parallel=True) for a context that exchanges with them")
     boundaries = []
     for blk, rng, label in bnd:
         if label not in labels:

@@ -109,6 +109,16 @@ SIGNATURES = {
     "ostc_fr_compute_dt": (_I, [_P, _DP]),
     "ostc_fr_stage_update": (_I, [_P, _I, _I, _D]),
     "ostc_fr_update_properties": (_I, [_P, _I]),
+    "ostc_fr_copy_qn": (_I, [_P, _I]),
+    "ostc_fr_apply_bc": (_I, [_P, _I]),
+    "ostc_fr_exchange": (_I, [_P]),
+    "ostc_fr_add_parallel_face": (_I, [_P, _I, C.POINTER(_I), _I, _I, _I]),
+    "ostc_fr_parallel_face_count": (_I, [_P, C.POINTER(_I)]),
+    "ostc_fr_parallel_face_info": (_I, [_P, _I, C.POINTER(_pkg().ParallelInfo)]),
+    "ostc_fr_parallel_buffers": (_I, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(_LL)]),
+    "ostc_fr_pack_parallel": (_I, [_P]),
+    "ostc_fr_unpack_parallel": (_I, [_P]),
+    "ostc_fr_parallel_copy": (_I, [_P, _I, _DP]),
     "ostc_fr_last_step_timing": (_I, [_P, _DP, C.POINTER(_LL)]),
     "ostc_fr_kernel_ms": (_I, [_P, _DP, C.POINTER(_LL)]),
     "ostc_fr_sync": (_I, [_P]),
@@ -230,7 +240,42 @@ class FrSolver:
                 xyz = [np.ascontiguousarray(a) for a in pkg.cases.node_coordinates(case, b, self.ngg)]
                 self._chk(L.ostc_fr_set_sponge(self.ctx, b, C.cast(C.byref(sp), C.c_void_p),
                                                *(a.ctypes.data_as(C.POINTER(C.c_double)) for a in xyz)))
+        for f in case.get("parallel_faces", []):  # parallel N.txt records
+            self._chk(L.ostc_fr_add_parallel_face(self.ctx, f["block"], ints(f["range"]), f["rank"],
+                                                  f.get("flag_send", 0), f.get("flag_recv", 0)))
         self._chk(L.ostc_fr_finalize(self.ctx))
+
+    # -- faces shared with other ranks ------------------------------------
+    def parallel_faces(self):
+        """[{block, rank, flag_send, flag_recv, offset, count, points}] of the parallel faces."""
+        n = C.c_int()
+        self._chk(self.L.ostc_fr_parallel_face_count(self.ctx, C.byref(n)))
+        out = []
+        for q in range(n.value):
+            info = _pkg().ParallelInfo()
+            self._chk(self.L.ostc_fr_parallel_face_info(self.ctx, q, C.byref(info)))
+            out.append({k: getattr(info, k) for k, _ in _pkg().ParallelInfo._fields_})
+        return out
+
+    def _parallel_total(self):
+        t = _LL()
+        self._chk(self.L.ostc_fr_parallel_buffers(self.ctx, None, None, C.byref(t)))
+        return t.value
+
+    def pack_parallel(self) -> np.ndarray:
+        """Packs every parallel face and returns the send buffer (host copy)."""
+        self._chk(self.L.ostc_fr_pack_parallel(self.ctx))
+        out = np.zeros(self._parallel_total())
+        self._chk(self.L.ostc_fr_parallel_copy(self.ctx, 0, out.ctypes.data_as(_DP)))
+        return out
+
+    def unpack_parallel(self, recv: np.ndarray):
+        """Copies `recv` into the receive buffer and unpacks every parallel face."""
+        recv = np.ascontiguousarray(recv, dtype=np.float64)
+        if recv.size != self._parallel_total():
+            raise ValueError(f"receive buffer of {recv.size} doubles, expected {self._parallel_total()}")
+        self._chk(self.L.ostc_fr_parallel_copy(self.ctx, 1, recv.ctypes.data_as(_DP)))
+        self._chk(self.L.ostc_fr_unpack_parallel(self.ctx))
 
     def _chk(self, r):
         if r != 0:
