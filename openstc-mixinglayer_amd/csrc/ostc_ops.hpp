@@ -68,6 +68,40 @@ __device__ __forceinline__ void props_point(const DevBlock& Z, long long p, int 
   Z.mach[p] = V / sqrt(c_temp * t);
 }
 
+// Writes the primitives q[6] of node p in ring slot `slot` and, from the
+// same registers, its cv (compute_cv_from_bv_1_point<Air>) and, with
+// `props`, its viscosity and Mach number: the values air_cv_point and
+// props_point would compute after reading q back.  An op loads everything it
+// reads before its first store, so a point costs one memory round trip
+// rather than one per store the next load might alias.
+__device__ __forceinline__ void put_node(const DevBlock& Z, long long p, int slot, const double q[6], bool with_cv,
+                                         bool props) {
+  double* zbv = Z.bv[slot];
+  const long long zs = Z.sz;
+#pragma unroll
+  for (int l = 0; l < 6; ++l) zbv[p + l * zs] = q[l];
+  if (with_cv) {
+    double* zcv = Z.cv[slot];
+    const double rho = q[0];
+    zcv[p] = rho;
+    zcv[p + zs] = rho * q[1];
+    zcv[p + 2 * zs] = rho * q[2];
+    zcv[p + 3 * zs] = rho * q[3];
+    zcv[p + 4 * zs] = total_energy(rho, q[1], q[2], q[3], q[4]);
+  }
+  if (props) {
+    const double t = q[5];
+    const double u = q[1], v = q[2], w = q[3];
+    const double V = sqrt(u * u + v * v + w * w);
+    constexpr double c_temp{kGamma * kRu / kMwAir};
+    Z.mul[slot][p] = sutherland(t);
+    Z.mach[p] = V / sqrt(c_temp * t);
+  }
+}
+
+// Largest ngg (WENO-Z7); the ghost loops below unroll to it.
+constexpr int kMaxGhost = 4;
+
 // One op point.  Every op reads and writes the ring slot `slot` of cv/bv/mul.
 // With `props`, each op also recomputes the properties (props_point) of every
 // node whose primitives it writes, and the rim copy carries the viscosity of
@@ -99,25 +133,26 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
         sj = ax == 1 ? src : j;
         sk = ax == 2 ? src : k;
       }
-      for (int g = 1; g <= B.g; ++g) {
-        const long long pg = pidx(B, i + g * di, j + g * dj, k + g * dk);
+      double q[kMaxGhost][6];
+#pragma unroll
+      for (int g = 1; g <= kMaxGhost; ++g) {
+        if (g > B.g) break;
         const long long ps = (op.type == OP_BC_PERIODIC) ? pidx(B, si + g * di, sj + g * dj, sk + g * dk)
                                                          : pidx(B, i, j, k);
-        for (int l = 0; l < 6; ++l) bv[pg + l * s] = bv[ps + l * s];
-        cv_point(B, pg);
-        if (props) props_point(B, pg, slot);
+#pragma unroll
+        for (int l = 0; l < 6; ++l) q[g - 1][l] = bv[ps + l * s];
+      }
+#pragma unroll
+      for (int g = 1; g <= kMaxGhost; ++g) {
+        if (g > B.g) break;
+        put_node(B, pidx(B, i + g * di, j + g * dj, k + g * dk), slot, q[g - 1], true, props);
       }
       break;
     }
     case OP_BC_INFLOW: {
       double st[6];
       inflow_state(op.ff, op.aux[local], op.pass[op.block], inflow_point(ax, i, j, k, B.mx, B.my, B.g), st);
-      for (int g = 0; g <= B.g; ++g) {
-        const long long q = pidx(B, i + g * di, j + g * dj, k + g * dk);
-        for (int l = 0; l < 6; ++l) bv[q + l * s] = st[l];
-        cv_point(B, q);
-        if (props) props_point(B, q, slot);
-      }
+      for (int g = 0; g <= B.g; ++g) put_node(B, pidx(B, i + g * di, j + g * dj, k + g * dk), slot, st, true, props);
       break;
     }
     case OP_BC_FARFIELD: {
@@ -170,18 +205,8 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
         st[4] = st[0] * c_b * c_b / kGamma;
         st[5] = st[4] * kMwAir / (st[0] * kRu);
       }
-      if (face_too) {
-        for (int l = 0; l < 6; ++l) bv[p + l * s] = st[l];
-        cv_point(B, p);
-        if (props) props_point(B, p, slot);
-      }
-      for (int g = 1; g <= B.g; ++g) {
-        const long long pg = pidx(B, i + g * di, j + g * dj, k + g * dk);
-#pragma unroll
-        for (int l = 0; l < 6; ++l) bv[pg + l * s] = st[l];
-        cv_point(B, pg);
-        if (props) props_point(B, pg, slot);
-      }
+      if (face_too) put_node(B, p, slot, st, true, props);
+      for (int g = 1; g <= B.g; ++g) put_node(B, pidx(B, i + g * di, j + g * dj, k + g * dk), slot, st, true, props);
       break;
     }
     case OP_COMM: {
@@ -195,21 +220,20 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
       const long long p = pidx(B, i, j, k);
       const long long pt = pidx(T, op.toff[0] + src_index(op.tcode[0]), op.toff[1] + src_index(op.tcode[1]),
                                 op.toff[2] + src_index(op.tcode[2]));
-      double* tbv = T.bv[slot];
+      const double* tbv = T.bv[slot];
       const long long ts = T.sz;
+      double q[6];
+#pragma unroll
+      for (int l = 0; l < 6; ++l) q[l] = tbv[pt + l * ts];
       if (idx_a == op.shared) {
-        for (int l = 0; l < 6; ++l) {
-          const double ave = 0.5 * (tbv[pt + l * ts] + bv[p + l * s]);
-          bv[p + l * s] = ave;
-          tbv[pt + l * ts] = ave;
-        }
-        if (op.flags & 1) cv_point(T, pt);
-        if (props) props_point(T, pt, slot);
-      } else {
-        for (int l = 0; l < 6; ++l) bv[p + l * s] = tbv[pt + l * ts];
+        double own[6];
+#pragma unroll
+        for (int l = 0; l < 6; ++l) own[l] = bv[p + l * s];
+#pragma unroll
+        for (int l = 0; l < 6; ++l) q[l] = 0.5 * (q[l] + own[l]);
+        put_node(T, pt, slot, q, (op.flags & 1) != 0, props);
       }
-      cv_point(B, p);
-      if (props) props_point(B, p, slot);
+      put_node(B, p, slot, q, true, props);
       break;
     }
     case OP_PROPS: {
@@ -223,8 +247,13 @@ __device__ __forceinline__ void op_point(const DevBlock* blocks, const DevOp& op
     case OP_COPY_BV: {
       const long long p = pidx(B, i, j, k);
       const double* sbv = B.bv[src];
-      for (int l = 0; l < 6; ++l) bv[p + l * s] = sbv[p + l * s];
-      if (props) B.mul[slot][p] = B.mul[src][p];
+      double q[6];
+#pragma unroll
+      for (int l = 0; l < 6; ++l) q[l] = sbv[p + l * s];
+      const double mu = props ? B.mul[src][p] : 0.0;
+#pragma unroll
+      for (int l = 0; l < 6; ++l) bv[p + l * s] = q[l];
+      if (props) B.mul[slot][p] = mu;
       break;
     }
     default:
