@@ -1313,12 +1313,15 @@ __device__ __forceinline__ void viscous_body(const FrArgs& A, const int wx, cons
                            lds[l * (F * TW) + face_slot<DIR, TW, F>(col, f - 1)];
   }
 }
+// At most 256 VGPRs, two waves per SIMD: the y and z sweeps need 258-260
+// unconstrained, which leaves one wave per SIMD; capped they spill 2-4
+// registers and take 12.3 instead of 17.0 ms per step on 512x256x128.
 template <int DIR, int NS>
-__global__ __launch_bounds__(kT) void k_viscous(FrArgs A) {
+__global__ __launch_bounds__(kT, 2) void k_viscous(FrArgs A) {
   viscous_body<DIR, NS>(A, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 template <int DIR, int NS>
-__global__ __launch_bounds__(kT) void k_viscous_batched(FrBatch W) {
+__global__ __launch_bounds__(kT, 2) void k_viscous_batched(FrBatch W) {
   const WgPos w = wg_pos(W);
   FrArgs A = W.common;
   A.B = W.blocks[w.b];
