@@ -133,6 +133,8 @@ struct ostc_ctx_s {
   std::vector<double*> dq_bar, cv_bar;   // per block, allocated on first use
   double* d_fs = nullptr;                // viscous face cotangents
   long long fs_stride = 0;
+  double* d_gs = nullptr;                // partial gathers of the viscous cotangents
+  long long gs_stride = 0;
   double* d_lam = nullptr;               // cv cotangent of the reverse sweep (padded)
   double* d_lam_qn = nullptr;            // cotangent of the step's initial state, through qn
   double* d_w = nullptr;                 // dq cotangent of one stage
@@ -1653,16 +1655,22 @@ int check_theta(ostc_ctx c, const DevBlock& B, const ostc_theta* t) {
 // Buffers shared by every VJP of the context (face cotangents sized for the
 // largest block) and the objective's row sums.
 int adj_common(ostc_ctx c) {
-  long long need = 0;
+  long long need = 0, gneed = 0;
   int my = 0;
   for (auto& B : c->blocks) {
     need = std::max(need, vjp_face_stride(B.d.mx, B.d.my, B.d.mz));
+    gneed = std::max(gneed, vjp_gather_stride(B.d.mx, B.d.my, B.d.mz));
     my = std::max(my, B.d.my);
   }
   if (need > c->fs_stride) {
     int r = dalloc(c, &c->d_fs, 48 * (size_t)need);
     if (r) return r;
     c->fs_stride = need;
+  }
+  if (gneed > c->gs_stride) {
+    int r = dalloc(c, &c->d_gs, 8 * (size_t)gneed);
+    if (r) return r;
+    c->gs_stride = gneed;
   }
   if (!c->d_obj) {
     int r = dalloc(c, &c->d_obj, 4 * (size_t)my + 1);
@@ -1677,6 +1685,7 @@ VjpArgs vjp_args(ostc_ctx c, const DevBlock& B, const double* cv, const double* 
   a.mx = B.mx; a.my = B.my; a.mz = B.mz; a.g = B.g; a.sx = B.sx; a.sy = B.sy; a.sz = B.sz; a.n0 = B.n0;
   a.cv = cv; a.bv = bv; a.mul = mul; a.jac = B.jac; a.met = B.met;
   a.w = w; a.lam = lam; a.fs = c->d_fs; a.fs_stride = c->fs_stride; a.P = c->P;
+  a.gs = c->d_gs; a.gs_stride = c->gs_stride;
   return a;
 }
 

@@ -76,9 +76,12 @@ struct VjpArgs {
   double* lam;           // cv cotangent (5 planes, padded), accumulated
   double* fs;            // viscous face cotangents: 3 directions x 16 planes
   long long fs_stride;   // plane stride of fs, >= vjp_face_stride()
+  double* gs;            // partial gathers of the viscous cotangents: 8 planes over [-1, m]^3
+  long long gs_stride;   // plane stride of gs, >= vjp_gather_stride()
   Params P;
 };
 long long vjp_face_stride(int mx, int my, int mz);
+long long vjp_gather_stride(int mx, int my, int mz);
 void rhs_vjp(const VjpArgs& a, hipStream_t st, long long* n);
 void adj_stage_pre(const DevBlock& B, double* lam, double* lam_qn, double* w, double ra, double rb, double cdt,
                    hipStream_t st);

@@ -429,7 +429,14 @@ __global__ __launch_bounds__(kT) void k_visc_face_vjp(VjpArgs A) {
 }
 
 // Gathers the viscous face cotangents onto the points [-1, m]^3 and chains
-// them through Sutherland's law and the primitive variables to cv.
+// them through Sutherland's law and the primitive variables to cv.  One pass
+// per face direction D: pass 0 starts the point's sums (vb: velocity and
+// temperature, mb: the face-mean terms), passes 1 and 2 continue them from
+// A.gs in the same order, and pass 2 chains them to lam.  The sums are those of
+// one pass over all three directions, term for term; split, a pass holds a
+// third of the 144 face loads in flight (all of them take 302 VGPRs, one
+// wave per SIMD).
+template <int D>
 __global__ __launch_bounds__(kT) void k_visc_gather(VjpArgs A) {
   const Geo G = geo_of(A);
   const int ex = G.mx + 2, ey = G.my + 2, ez = G.mz + 2;
@@ -437,32 +444,40 @@ __global__ __launch_bounds__(kT) void k_visc_gather(VjpArgs A) {
   if (id >= (long long)ex * ey * ez) return;
   const int i = (int)(id % ex) - 1, j = (int)((id / ex) % ey) - 1, k = (int)(id / ((long long)ex * ey)) - 1;
   double vb[4] = {0, 0, 0, 0}, mb[4] = {0, 0, 0, 0};
-  auto face = [&](auto dirc, int plane, int a, int b, int c) -> double {
-    constexpr int D = decltype(dirc)::value;
+  if (D > 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      vb[q] = A.gs[q * A.gs_stride + id];
+      mb[q] = A.gs[(4 + q) * A.gs_stride + id];
+    }
+  }
+  auto face = [&](int plane, int a, int b, int c) -> double {
     if (!face_exists<D>(G, a, b, c)) return 0.0;
     return A.fs[((long long)D * 16 + plane) * A.fs_stride + face_id<D>(G, a, b, c)];
   };
-  auto one_dir = [&](auto dirc) {
-    constexpr int D = decltype(dirc)::value;
-    const int o[3] = {D == 0, D == 1, D == 2};
+  const int o[3] = {D == 0, D == 1, D == 2};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) vb[q] += face(dirc, 3 * q + D, i - o[0], j - o[1], k - o[2]) - face(dirc, 3 * q + D, i, j, k);
+  for (int q = 0; q < 4; ++q) vb[q] += face(3 * q + D, i - o[0], j - o[1], k - o[2]) - face(3 * q + D, i, j, k);
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      if (r == D) continue;
-      const int e[3] = {r == 0, r == 1, r == 2};
+  for (int r = 0; r < 3; ++r) {
+    if (r == D) continue;
+    const int e[3] = {r == 0, r == 1, r == 2};
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        vb[q] += 0.25 * (face(dirc, 3 * q + r, i - e[0], j - e[1], k - e[2]) - face(dirc, 3 * q + r, i + e[0], j + e[1], k + e[2]) +
-                         face(dirc, 3 * q + r, i - e[0] - o[0], j - e[1] - o[1], k - e[2] - o[2]) -
-                         face(dirc, 3 * q + r, i + e[0] - o[0], j + e[1] - o[1], k + e[2] - o[2]));
+    for (int q = 0; q < 4; ++q)
+      vb[q] += 0.25 * (face(3 * q + r, i - e[0], j - e[1], k - e[2]) - face(3 * q + r, i + e[0], j + e[1], k + e[2]) +
+                       face(3 * q + r, i - e[0] - o[0], j - e[1] - o[1], k - e[2] - o[2]) -
+                       face(3 * q + r, i + e[0] - o[0], j + e[1] - o[1], k + e[2] - o[2]));
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) mb[r] += 0.5 * (face(12 + r, i, j, k) + face(12 + r, i - o[0], j - o[1], k - o[2]));
+  if (D < 2) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      A.gs[q * A.gs_stride + id] = vb[q];
+      A.gs[(4 + q) * A.gs_stride + id] = mb[q];
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) mb[r] += 0.5 * (face(dirc, 12 + r, i, j, k) + face(dirc, 12 + r, i - o[0], j - o[1], k - o[2]));
-  };
-  one_dir(std::integral_constant<int, 0>{});
-  one_dir(std::integral_constant<int, 1>{});
-  one_dir(std::integral_constant<int, 2>{});
+    return;
+  }
   // points no face reaches (the corners of the shell) carry no cotangent
   if (vb[0] == 0 && vb[1] == 0 && vb[2] == 0 && vb[3] == 0 && mb[0] == 0 && mb[1] == 0 && mb[2] == 0 && mb[3] == 0)
     return;
@@ -684,13 +699,18 @@ void rhs_vjp(const VjpArgs& a, hipStream_t st, long long* n) {
   k_visc_face_vjp<0><<<blocks_for(f0), kT, 0, st>>>(a);
   k_visc_face_vjp<1><<<blocks_for(f1), kT, 0, st>>>(a);
   k_visc_face_vjp<2><<<blocks_for(f2), kT, 0, st>>>(a);
-  k_visc_gather<<<blocks_for((long long)(a.mx + 2) * (a.my + 2) * (a.mz + 2)), kT, 0, st>>>(a);
-  *n += 4;
+  const long long gp = vjp_gather_stride(a.mx, a.my, a.mz);
+  k_visc_gather<0><<<blocks_for(gp), kT, 0, st>>>(a);
+  k_visc_gather<1><<<blocks_for(gp), kT, 0, st>>>(a);
+  k_visc_gather<2><<<blocks_for(gp), kT, 0, st>>>(a);
+  *n += 6;
 }
 
 long long vjp_face_stride(int mx, int my, int mz) {
   return std::max({(long long)(mx + 1) * my * mz, (long long)mx * (my + 1) * mz, (long long)mx * my * (mz + 1)});
 }
+
+long long vjp_gather_stride(int mx, int my, int mz) { return (long long)(mx + 2) * (my + 2) * (mz + 2); }
 
 void adj_stage_pre(const DevBlock& B, double* lam, double* lam_qn, double* w, double ra, double rb, double cdt,
                    hipStream_t st) {
