@@ -110,6 +110,37 @@ def test_two_ranks_exchange(axis, neg, rev_other, rev_neg):
     sb.close()
 
 
+def test_large_face_exchange():
+    """A 130 x 70 face (9100 points: 285 workgroups of 32 points, the last one
+    partial), y varying fastest and reversed: both send buffers and both
+    unpacked blocks bit-identical to the oracle."""
+    ny, nz, ca, cb = 130, 70, 6, 7
+    print(f"[gpu] parallel faces, large face {ny} x {nz}", flush=True)
+    rec_a = [ca, ca, -ny, -1, 1, nz]
+    rec_b = [1, 1, -ny, -1, 1, nz]
+    cut = lambda x: cases.mixing_layer_case([x - 1], [ny - 1], [nz - 1])
+    solvers, faces, states = [], [], []
+    for nx, rec, rank in ((ca, rec_a, 1), (cb, rec_b, 0)):
+        c = cut(nx)
+        c["boundaries"] = [b for b in c["boundaries"] if b.get("axis") != 0]
+        c["parallel_faces"] = [dict(block=0, range=rec, rank=rank)]
+        s = pkg.Solver(c)
+        bv = np.random.default_rng(40 + nx).uniform(0.5, 2.0, size=s.get(0, pkg.FIELD_BV).shape)
+        s.set(0, pkg.FIELD_BV, bv)
+        solvers.append(s)
+        faces.append(P.decode(rec, (nx, ny, nz)))
+        states.append(bv)
+    g = solvers[0].ngg
+    sends = [s.pack_parallel() for s in solvers]
+    for s, f, bv, send in zip(solvers, faces, states, sends):
+        np.testing.assert_array_equal(send, P.pack(bv, f, g))
+    solvers[0].unpack_parallel(sends[1])
+    solvers[1].unpack_parallel(sends[0])
+    for s, f, bv, recv in zip(solvers, faces, states, sends[::-1]):
+        np.testing.assert_array_equal(s.get(0, pkg.FIELD_BV), P.unpack(bv.copy(), f, g, recv))
+        s.close()
+
+
 # Two parallel faces of one block that share the edge line i = mx-1, j = my-1:
 # the second face's unpack averages with the first face's means there, as the
 # reference's in-order assign_data_received does.
