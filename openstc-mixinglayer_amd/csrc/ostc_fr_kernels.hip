@@ -4,7 +4,7 @@
 // Per RK stage and block: k_source (dq = J omega, or 0), three inviscid
 // sweeps, three viscous sweeps, k_update (SSP-RK3 + primitives + T-Newton),
 // then ghost ops and k_props over the padded domain.  dt lives on the device
-// (k_local_dt reduces it with 64-lane shuffles and one atomicMin per wave), so
+// (k_local_dt reduces it with 64-lane shuffles and one atomicMin per workgroup), so
 // a whole step has no host synchronisation and can be captured in a graph.
 //
 // Inviscid sweeps stage the split fluxes of a tile in LDS: a tile is TW
@@ -295,8 +295,10 @@ __global__ __launch_bounds__(kT) void k_source_batched(FrBatch W) {
 __global__ void k_dt_set(double* dt, double v) { *dt = v; }
 
 // local_time_step<FR, Laminar>, src/TimeAdvanceFunc.cuh:28-83, and the MIN of
-// global_time_step: dt_local per cell, a 64-lane shuffle MIN per wave and one
-// 64-bit atomicMin on the bit pattern of a positive double.
+// global_time_step: dt_local per cell, a 64-lane shuffle MIN per wave, the
+// waves' MIN through LDS and one 64-bit atomicMin per workgroup on the bit
+// pattern of a positive double (one per wave put ~260k atomics on one
+// address for 16.5 M cells, most of the kernel's time).
 __device__ __forceinline__ void local_dt_body(const FrArgs& A, unsigned long long* dt_bits, const int wx, const int wy, const int wz) {
   const FrBlockDev& B = A.B;
   const long long q = (long long)wx * kT + threadIdx.x;
@@ -325,7 +327,15 @@ __device__ __forceinline__ void local_dt_body(const FrArgs& A, unsigned long lon
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) dtl = fmin(dtl, __shfl_xor(dtl, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMin(dt_bits, (unsigned long long)__double_as_longlong(dtl));
+  __shared__ double s_min[kT / 64];
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = dtl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = s_min[0];
+#pragma unroll
+    for (int w = 1; w < kT / 64; ++w) m = fmin(m, s_min[w]);
+    atomicMin(dt_bits, (unsigned long long)__double_as_longlong(m));
+  }
 }
 __global__ __launch_bounds__(kT) void k_local_dt(FrArgs A, unsigned long long* dt_bits) {
   local_dt_body(A, dt_bits, blockIdx.x, blockIdx.y, blockIdx.z);
