@@ -720,8 +720,7 @@ int batched_record(ostc_ctx c, long long* n, int ev_base) {
   }
   if (!c->s.fixed_time_step) {
     FusedArgs a = fused_args(c, 0, 1, 0);
-    const TileShape t = fused_shape(c->P.scheme, c->tile_index);
-    dt_batched(a, t.tx, t.ty, t.tz, std::min(c->n_tiles, c->cus * 4), c->d_dt, c->st, n);
+    dt_rows(a, (int)c->blocks.size(), c->cus * 4, c->d_dt, c->st, n);
   }
   for (int rk = 0; rk < 3; ++rk) {
     const int r = batched_stage(c, rk, n, ev_base);

@@ -61,7 +61,7 @@ TileShape fused_shape(int scheme, int index);
 // used, or -1.
 int fused_launch(int index, int mode, const FusedArgs& a, int n_wg, hipStream_t st);
 int fused_lds_bytes(int scheme, int index);
-void dt_batched(const FusedArgs& a, int tx, int ty, int tz, int n_wg, double* dt_bits, hipStream_t st, long long* n);
+void dt_rows(const FusedArgs& a, int n_blocks, int n_wg, double* dt_bits, hipStream_t st, long long* n);
 
 // reverse mode, ostc_kernels_adjoint.hip
 struct VjpArgs {

@@ -723,26 +723,29 @@ __device__ double local_dt_point_f(const StageBlock& B, const Params& P, int p) 
 
 __global__ void k_dt_init(unsigned long long* bits) { *bits = (unsigned long long)__double_as_longlong(1e6); }
 
-template <int TX, int TY, int TZ>
-__global__ __launch_bounds__(TX * TY * TZ) void k_dt_tiles(FusedArgs A, unsigned long long* bits) {
-  constexpr int NT = TX * TY * TZ;
-  __shared__ double part[NT / 64];
+// Every block's interior walked row by row (consecutive lanes, consecutive i:
+// whole cache lines of the metric and primitive planes; tiles would read
+// 16-double pieces of rows that straddle lines), grid-stride over the cells of
+// one block after another.
+__global__ __launch_bounds__(256) void k_dt_rows(FusedArgs A, int n_blocks, unsigned long long* bits) {
+  __shared__ double part[4];
   const int tid = threadIdx.x;
-  const int ti = tid % TX, tj = (tid / TX) % TY, tk = tid / (TX * TY);
   double v = 1e6;
-  for (int tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
-    const StageBlock B = stage_block(A.blocks[A.tile_block[tile]], 0, 0);  // the step's state, slot 0
-    const int local = tile - B.tile_base;
-    const int i = (local % B.ntx) * TX + ti, j = ((local / B.ntx) % B.nty) * TY + tj,
-              k = (local / (B.ntx * B.nty)) * TZ + tk;
-    if (i < B.mx && j < B.my && k < B.mz) v = fmin(v, local_dt_point_f(B, A.P, poff(B, i, j, k)));
+  const unsigned stride = gridDim.x * 256u;
+  for (int b = 0; b < n_blocks; ++b) {
+    const StageBlock B = stage_block(A.blocks[b], 0, 0);  // the step's state, slot 0
+    const unsigned row = (unsigned)B.mx, plane = row * (unsigned)B.my, n = plane * (unsigned)B.mz;
+    for (unsigned q = blockIdx.x * 256u + tid; q < n; q += stride) {
+      const unsigned k = q / plane, r = q - k * plane, j = r / row, i = r - j * row;
+      v = fmin(v, local_dt_point_f(B, A.P, poff(B, (int)i, (int)j, (int)k)));
+    }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
   if ((tid & 63) == 0) part[tid >> 6] = v;
   __syncthreads();
   if (tid == 0) {
-    for (int w = 1; w < NT / 64; ++w) v = fmin(v, part[w]);
+    for (int w = 1; w < 4; ++w) v = fmin(v, part[w]);
     v = fmin(v, part[0]);
     atomicMin(bits, (unsigned long long)__double_as_longlong(v));
   }
@@ -833,21 +836,10 @@ int fused_lds_bytes(int scheme, int index) {
   return -1;
 }
 
-void dt_batched(const FusedArgs& a, int tx, int ty, int tz, int n_wg, double* dt_bits, hipStream_t st,
-                long long* n) {
+void dt_rows(const FusedArgs& a, int n_blocks, int n_wg, double* dt_bits, hipStream_t st, long long* n) {
   auto* bits = reinterpret_cast<unsigned long long*>(dt_bits);
   hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, bits);
-  int i = 0;
-  bool done = false;
-#define X(a_, b_, c_, w_)                                                                              \
-  if (!done && tx == a_ && ty == b_ && tz == c_) {                                                 \
-    hipLaunchKernelGGL((k_dt_tiles<a_, b_, c_>), dim3(n_wg), dim3(a_ * b_ * c_), 0, st, a, bits); \
-    done = true;                                                                                   \
-  }                                                                                                \
-  ++i;
-  OSTC_SHAPES(X)
-#undef X
-  (void)i;
+  hipLaunchKernelGGL(k_dt_rows, dim3(n_wg), dim3(256), 0, st, a, n_blocks, bits);
   *n += 2;
 }
 
